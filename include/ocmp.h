@@ -1,0 +1,263 @@
+/*
+ * ocmp.h -- C ABI of the MI355X (gfx950) assemble+solve backend for OpenCMP models.
+ *
+ * The reference has no FFI of its own: OpenCMP calls NGSolve through pybind11
+ * (BilinearForm.Assemble, LinearForm.Assemble, Preconditioner, solvers.CG/GMRes,
+ * GridFunction.Set). Each entry point below names the reference call site it
+ * replaces. Paths are relative to the OpenCMP source tree.
+ *
+ * Conventions
+ *  - Every function returns int: OCMP_OK (0) or a negative error code;
+ *    ocmp_last_error() returns a thread-local message for the last failure.
+ *  - Host arrays passed in are borrowed for the duration of the call.
+ *  - Pointers documented as "device" are HIP device pointers owned by the caller
+ *    (e.g. the storage of a torch tensor on cuda:N); the library never frees them.
+ *  - Each context owns one HIP stream; all work is enqueued on it. Calls are
+ *    synchronous with respect to the host only where documented (solve, copies
+ *    to host). A handle is not thread-safe; distinct contexts are independent.
+ */
+#ifndef OCMP_H
+#define OCMP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  OCMP_OK = 0,
+  OCMP_ERR_ARG = -1,         /* invalid argument (shape, index, NULL)      */
+  OCMP_ERR_HIP = -2,         /* HIP runtime failure                         */
+  OCMP_ERR_STATE = -3,       /* call sequence error (e.g. not assembled)    */
+  OCMP_ERR_UNSUPPORTED = -4, /* valid request outside the implemented set   */
+  OCMP_ERR_INTERNAL = -5
+};
+
+typedef struct ocmp_ctx ocmp_ctx;
+typedef struct ocmp_mesh ocmp_mesh;
+typedef struct ocmp_space ocmp_space;
+typedef struct ocmp_matrix ocmp_matrix;
+typedef struct ocmp_precond ocmp_precond;
+
+/* ------------------------------------------------------------------ context */
+
+/* Replaces the ngcore.TaskManager() scope of opencmp/run.py:57-61.
+ * hip_stream: an existing hipStream_t to run on, or NULL to create one
+ * (default priority). */
+int ocmp_ctx_create(int device, void *hip_stream, ocmp_ctx **out);
+int ocmp_ctx_destroy(ocmp_ctx *ctx);
+int ocmp_ctx_sync(ocmp_ctx *ctx);
+void *ocmp_ctx_stream(ocmp_ctx *ctx);
+/* Per-kernel-class HIP event timing (no reference counterpart; the reference
+ * has no timers, opencmp/run.py:58). */
+int ocmp_ctx_timing(ocmp_ctx *ctx, int enable);
+/* Writes a JSON object {"class": [launches, total_ms, algorithmic_bytes], ...} and resets. */
+int ocmp_ctx_timing_report(ocmp_ctx *ctx, char *buf, int64_t cap);
+const char *ocmp_last_error(void);
+int ocmp_version(void);
+
+/* --------------------------------------------------------------------- mesh */
+
+/* Replaces ngs.Mesh(...) from load_mesh (opencmp/helpers/io.py:69-110).
+ * coords: host (nv, dim) f64; elems: host (nel, dim+1) i32 vertex ids;
+ * bnd_facets: host (nbnd, dim) i32; bnd_markers: host (nbnd) i32 in [0, 64).
+ * dim is 2 (triangles) or 3 (tetrahedra). */
+int ocmp_mesh_create(ocmp_ctx *ctx, int dim, int64_t nv, const double *coords, int64_t nel,
+                     const int32_t *elems, int64_t nbnd, const int32_t *bnd_facets,
+                     const int32_t *bnd_markers, ocmp_mesh **out);
+/* out[8] = {nv, nedge, nface, nel, nfacet, n_interior_facets, nbnd, dim} */
+int ocmp_mesh_info(const ocmp_mesh *mesh, int64_t *out);
+int ocmp_mesh_destroy(ocmp_mesh *mesh);
+
+/* -------------------------------------------------------------------- space */
+
+enum { OCMP_H1 = 0, OCMP_L2 = 1 };
+
+typedef struct {
+  int32_t kind;            /* OCMP_H1 or OCMP_L2                                  */
+  int32_t order;           /* H1: 1..3; L2: 0..3 (triangles)                       */
+  int32_t ncomp;           /* 1, or dim for VectorH1                               */
+  int32_t reserved;
+  uint64_t dirichlet_mask; /* bit m set: boundary marker m is Dirichlet (H1 only)  */
+} ocmp_fe_desc;
+
+/* Replaces Model._construct_fes: ngs.FESpace([H1|VectorH1|L2 ...], dgjumps=...)
+ * (opencmp/models/poisson.py:60-64, opencmp/models/ins.py:95-128,
+ * opencmp/models/multi_component_ins.py:79-102). Components are numbered in
+ * "blocks": a VectorH1 component expands into ncomp consecutive scalar blocks. */
+int ocmp_space_create(ocmp_mesh *mesh, int ncomp, const ocmp_fe_desc *comps, int dgjumps,
+                      ocmp_space **out);
+/* out[4] = {ndof, nloc (compound element dofs), nblocks, nfree} */
+int ocmp_space_info(const ocmp_space *s, int64_t *out);
+/* out[nblocks*4] = {dof offset, ndof, element-local offset, nloc} per block */
+int ocmp_space_blocks(const ocmp_space *s, int64_t *out);
+/* Replaces fes.FreeDofs() (opencmp/models/base_model.py:906): host u8[ndof]. */
+int ocmp_space_freedofs(const ocmp_space *s, uint8_t *mask);
+/* host i32[nel*nloc]: the compound element dof table (for parity checks). */
+int ocmp_space_element_dofs(const ocmp_space *s, int32_t *out);
+int ocmp_space_destroy(ocmp_space *s);
+
+/* ------------------------------------------------------------------- matrix */
+
+/* Builds the NGSolve-style CSR pattern (full element cliques, plus facet
+ * neighbours when dgjumps) and the deterministic assembly schedule: element
+ * colouring plus element->CSR slot map (H1) or owner-computes block rows (L2).
+ * Replaces the pattern half of BilinearForm(fes) + Assemble. */
+int ocmp_matrix_create(ocmp_space *s, ocmp_matrix **out);
+/* out[4] = {n, nnz, ncolours, is_dg} */
+int ocmp_matrix_info(const ocmp_matrix *A, int64_t *out);
+int ocmp_matrix_pattern(const ocmp_matrix *A, int32_t *row_ptr, int32_t *col);  /* host */
+int ocmp_matrix_values(const ocmp_matrix *A, double *vals);                    /* host, syncs */
+int ocmp_matrix_destroy(ocmp_matrix *A);
+
+/* --------------------------------------------------------------- integrators */
+
+enum {
+  OCMP_BFI_GRADGRAD = 0, /* coef * w * grad u . grad v                         */
+  OCMP_BFI_MASS = 1,     /* coef * w * u v                                     */
+  OCMP_BFI_CONV = 2,     /* coef * w * u (wind . grad v)                       */
+  OCMP_BFI_DTRIAL = 3,   /* coef * w * d_dir u * v                             */
+  OCMP_BFI_DTEST = 4,    /* coef * w * u * d_dir v                             */
+  OCMP_BFI_DG_FACET = 16 /* interior + boundary facet terms of an L2 block, see below */
+};
+
+enum { OCMP_WEIGHT_NONE = 0, OCMP_WEIGHT_FIELD = 1, OCMP_WEIGHT_ONE_MINUS_FIELD = 2 };
+
+/* A grid function on one scalar H1 block of a space (device dof vector of that space). */
+typedef struct {
+  const ocmp_space *space;
+  int32_t block;
+  int32_t reserved;
+  const double *vec; /* device, length = ndof of `space` */
+} ocmp_field;
+
+/* One bilinear term. Volume terms: trial/test are scalar block indices. The
+ * weight w is 1, a field phi, or 1-phi (phi-weighting of the diffuse-interface
+ * models, opencmp/models/stokes_dim.py:42-87). elem_coef (device, per element)
+ * multiplies coef when non-NULL (per-element diffusivity). The wind is the
+ * constant `wind` unless wind_field >= 0, in which case fields
+ * wind_field .. wind_field+dim-1 are its components (Oseen wind).
+ *
+ * OCMP_BFI_DG_FACET (L2 block, trial == test), the skeleton terms of
+ * opencmp/models/multi_component_ins.py:185-197 and :287-301:
+ *  interior: eps*(alpha/h [c][r] - {grad c}.n [r] - {grad r}.n [c])      if sipg
+ *            + [r] (c max(w.n,0) + c.Other() min(w.n,0))                  if upwind
+ *  boundary, marker in dirichlet_mask: eps*(alpha/h c r - r grad c.n - c grad r.n)  if sipg
+ *  every boundary facet: r c max(w.n,0)                                 if upwind
+ * with alpha = ipc*p^2 (opencmp/models/base_model.py:150) and h the element size
+ * |det J|^(1/dim) (facet: mean of both sides). */
+typedef struct {
+  int32_t kind;
+  int32_t trial, test;
+  int32_t dir;
+  double coef;
+  const double *elem_coef;
+  double wind[3];
+  int32_t wind_field;
+  int32_t weight_mode;
+  int32_t weight_field;
+  int32_t sipg;
+  int32_t upwind;
+  int32_t reserved;
+  double eps;
+  double alpha;
+  uint64_t dirichlet_mask;
+} ocmp_term;
+
+/* Replaces BilinearForm.Assemble() (opencmp/solvers/base_solver.py:368-377,
+ * opencmp/models/ins.py:339-342). Writes (accumulate=0) or adds (1) the terms
+ * into A's device CSR values. Deterministic: no floating-point atomics. */
+int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, int nfields,
+                           const ocmp_field *fields, int accumulate);
+
+enum { OCMP_LFI_SOURCE = 0, OCMP_LFI_DG_DIRICHLET = 1 };
+enum {
+  OCMP_FN_CONST = 0,     /* c[0]                                               */
+  OCMP_FN_SINPROD = 1,   /* c[0] * prod_d sin(pi x_d)                           */
+  OCMP_FN_POLY2 = 2,     /* 2D: c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2;
+                            3D: c0 + c1 x + c2 y + c3 z + c4 x^2 + c5 y^2 + c6 z^2 + c7 xy + c8 xz + c9 yz */
+  OCMP_FN_FIELD = 3      /* the field `field` times c[0]                        */
+};
+
+/* One linear-form term: coef * w * f * v (SOURCE), or the DG Dirichlet data
+ * terms eps*g*(alpha/h v - grad v.n) - v g min(w.n,0) on boundary facets in
+ * dirichlet_mask (opencmp/models/multi_component_ins.py:371-386). */
+typedef struct {
+  int32_t kind;
+  int32_t test;
+  int32_t fn;
+  int32_t field;
+  double c[10];
+  double coef;
+  int32_t weight_mode;
+  int32_t weight_field;
+  double eps;
+  double alpha;
+  double wind[3];
+  uint64_t dirichlet_mask;
+} ocmp_lterm;
+
+/* Replaces LinearForm.Assemble(). b: device f64[ndof]. */
+int ocmp_assemble_linear(ocmp_space *s, double *b, int nterms, const ocmp_lterm *terms, int nfields,
+                         const ocmp_field *fields, int accumulate);
+
+/* Replaces gfu.Set(g, definedon=mesh.Boundaries(...)) of
+ * Model.apply_dirichlet_bcs_to (opencmp/models/base_model.py:321-341) for one
+ * scalar H1 block: vertex dofs on the given markers take g's vertex values,
+ * edge dofs the 1D L2 projection of g minus its vertex interpolant; 3D face
+ * dofs (order 3) are set to zero. x: device f64[ndof]. */
+int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double *c,
+                       uint64_t marker_mask);
+
+/* y = A x; x, y device f64[n]. */
+int ocmp_matvec(ocmp_matrix *A, const double *x, double *y);
+
+/* ----------------------------------------------------- preconditioner, solve */
+
+enum { OCMP_PRE_NONE = 0, OCMP_PRE_JACOBI = 1, OCMP_PRE_BLOCK_JACOBI = 2 };
+enum { OCMP_FP64 = 0, OCMP_MIXED = 1 };
+
+/* Replaces ngs.Preconditioner(a, 'local') (opencmp/models/base_model.py:365-383):
+ * point Jacobi on free dofs (H1), or element-block Jacobi (L2). With
+ * precision OCMP_MIXED the library also keeps the FP32 copies used by the
+ * inner solve of iterative refinement. */
+int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **out);
+/* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
+ * recompute from A's current values. */
+int ocmp_precond_update(ocmp_precond *P);
+int ocmp_precond_destroy(ocmp_precond *P);
+
+enum { OCMP_SOLVER_CG = 0, OCMP_SOLVER_GMRES = 1, OCMP_SOLVER_RICHARDSON = 2 };
+
+typedef struct {
+  int32_t solver;      /* OCMP_SOLVER_*                                          */
+  int32_t precision;   /* OCMP_FP64 or OCMP_MIXED (FP32 inner, FP64 refinement)  */
+  int32_t restart;     /* GMRES restart length m                                 */
+  int32_t maxit;       /* total inner iterations                                 */
+  double tol;          /* ||b - A x||_2 / ||b||_2 on free dofs, computed in FP64 */
+  double inner_tol;    /* mixed: relative reduction asked of each inner solve    */
+  int32_t check_every; /* CG: residual polled by the host every k iterations     */
+  int32_t max_outer;   /* mixed: refinement steps                                */
+} ocmp_solve_opts;
+
+typedef struct {
+  int32_t iters;        /* Krylov iterations (all inner solves)           */
+  int32_t outer;        /* refinement steps (1 for FP64)                  */
+  int32_t converged;
+  int32_t reserved;
+  double rel_residual;  /* final ||b-Ax||/||b||, FP64, free dofs          */
+  double initial_rel_residual;
+} ocmp_solve_stats;
+
+/* Replaces Model.linear_solve (opencmp/models/base_model.py:886-947): solves
+ * A x = b on the free dofs starting from x (which holds the Dirichlet lift),
+ * leaving Dirichlet entries of x unchanged. b, x: device f64[n]. Returns OCMP_OK
+ * also when not converged; check stats->converged. */
+int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x,
+               const ocmp_solve_opts *opts, ocmp_solve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCMP_H */

@@ -1,0 +1,863 @@
+// C ABI entry points, context, mesh topology, spaces, CSR pattern and the
+// deterministic assembly schedule (host side of the library).
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <numeric>
+
+#include "internal.h"
+
+namespace ocmp {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+template <class F>
+static int guard(F &&f) {
+  try {
+    f();
+    return OCMP_OK;
+  } catch (const Error &e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return OCMP_ERR_INTERNAL;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return OCMP_ERR_INTERNAL;
+  }
+}
+
+// ------------------------------------------------------------------ context timing
+
+int Ctx::cls(const std::string &name) {
+  auto it = class_ids.find(name);
+  if (it != class_ids.end()) return it->second;
+  int id = (int)class_names.size();
+  class_names.push_back(name);
+  class_ids[name] = id;
+  return id;
+}
+
+hipEvent_t Ctx::get_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  OCMP_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::tic(int c, hipEvent_t *a) {
+  *a = get_event();
+  OCMP_HIP(hipEventRecord(*a, stream));
+  (void)c;
+}
+
+void Ctx::toc(int c, hipEvent_t a, double bytes) {
+  hipEvent_t b = get_event();
+  OCMP_HIP(hipEventRecord(b, stream));
+  pending.push_back({c, a, b, bytes});
+  if (pending.size() > 4096) flush_timing();
+}
+
+void Ctx::flush_timing() {
+  if (pending.empty()) return;
+  OCMP_HIP(hipEventSynchronize(pending.back().b));
+  for (auto &r : pending) {
+    float ms = 0.f;
+    OCMP_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto &slot = acc[r.cls];
+    slot.first += 1;
+    slot.second += ms;
+    acc_bytes[r.cls] += r.bytes;
+    event_pool.push_back(r.a);
+    event_pool.push_back(r.b);
+  }
+  pending.clear();
+}
+
+// ------------------------------------------------------------------ topology
+
+static const int TRI_EDGES[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+static const int TET_EDGES[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+static const int TET_FACES[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+static const int TRI_FACET_EDGE[3] = {2, 1, 0};  // facet k (opposite vertex k) -> local edge
+
+static void unique_keys(const std::vector<uint64_t> &keys, std::vector<uint64_t> &uniq, std::vector<int32_t> &inv) {
+  std::vector<int64_t> idx(keys.size());
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return keys[a] < keys[b]; });
+  inv.assign(keys.size(), -1);
+  uniq.clear();
+  for (size_t t = 0; t < idx.size(); ++t) {
+    uint64_t k = keys[idx[t]];
+    if (uniq.empty() || uniq.back() != k) uniq.push_back(k);
+    inv[idx[t]] = (int32_t)(uniq.size() - 1);
+  }
+}
+
+static void build_topology(Mesh &m, int64_t nbnd, const int32_t *bnd, const int32_t *bmark) {
+  const int d = m.dim, nvl = d + 1;
+  const uint64_t nv = (uint64_t)m.nv;
+  // edges
+  const int nle = d == 2 ? 3 : 6;
+  std::vector<uint64_t> keys((size_t)m.nel * nle), uniq;
+  std::vector<int32_t> inv;
+  for (int64_t e = 0; e < m.nel; ++e)
+    for (int k = 0; k < nle; ++k) {
+      int a = d == 2 ? TRI_EDGES[k][0] : TET_EDGES[k][0];
+      int b = d == 2 ? TRI_EDGES[k][1] : TET_EDGES[k][1];
+      keys[e * nle + k] = (uint64_t)m.elems[e * nvl + a] * nv + (uint64_t)m.elems[e * nvl + b];
+    }
+  unique_keys(keys, uniq, inv);
+  m.nedge = (int64_t)uniq.size();
+  m.edges.resize(2 * m.nedge);
+  for (int64_t i = 0; i < m.nedge; ++i) {
+    m.edges[2 * i] = (int32_t)(uniq[i] / nv);
+    m.edges[2 * i + 1] = (int32_t)(uniq[i] % nv);
+  }
+  m.elem_edges = inv;
+  std::vector<uint64_t> facet_keys;
+  if (d == 3) {
+    keys.assign((size_t)m.nel * 4, 0);
+    for (int64_t e = 0; e < m.nel; ++e)
+      for (int k = 0; k < 4; ++k) {
+        const int32_t *v = &m.elems[e * 4];
+        keys[e * 4 + k] = ((uint64_t)v[TET_FACES[k][0]] * nv + (uint64_t)v[TET_FACES[k][1]]) * nv +
+                          (uint64_t)v[TET_FACES[k][2]];
+      }
+    unique_keys(keys, uniq, inv);
+    m.nface = (int64_t)uniq.size();
+    m.faces.resize(3 * m.nface);
+    for (int64_t i = 0; i < m.nface; ++i) {
+      m.faces[3 * i] = (int32_t)(uniq[i] / (nv * nv));
+      m.faces[3 * i + 1] = (int32_t)((uniq[i] / nv) % nv);
+      m.faces[3 * i + 2] = (int32_t)(uniq[i] % nv);
+    }
+    m.elem_faces = inv;
+    m.elem_facets = inv;
+    m.nfacet = m.nface;
+    m.facet_verts = m.faces;
+    facet_keys = uniq;
+  } else {
+    m.elem_facets.resize((size_t)m.nel * 3);
+    for (int64_t e = 0; e < m.nel; ++e)
+      for (int k = 0; k < 3; ++k) m.elem_facets[e * 3 + k] = m.elem_edges[e * 3 + TRI_FACET_EDGE[k]];
+    m.nfacet = m.nedge;
+    m.facet_verts = m.edges;
+    facet_keys.resize(m.nedge);
+    for (int64_t i = 0; i < m.nedge; ++i)
+      facet_keys[i] = (uint64_t)m.edges[2 * i] * nv + (uint64_t)m.edges[2 * i + 1];
+  }
+  const int64_t nf = m.nfacet;
+  m.f_e0.assign(nf, -1);
+  m.f_k0.assign(nf, -1);
+  m.f_e1.assign(nf, -1);
+  m.f_k1.assign(nf, -1);
+  m.f_marker.assign(nf, -1);
+  for (int64_t e = 0; e < m.nel; ++e)
+    for (int k = 0; k < nvl; ++k) {
+      int32_t f = m.elem_facets[e * nvl + k];
+      if (m.f_e0[f] < 0) {
+        m.f_e0[f] = (int32_t)e;
+        m.f_k0[f] = k;
+      } else if (m.f_e1[f] < 0) {
+        m.f_e1[f] = (int32_t)e;
+        m.f_k1[f] = k;
+      } else {
+        throw Error(OCMP_ERR_ARG, "non-manifold mesh: facet shared by more than two elements");
+      }
+    }
+  m.ninterior = 0;
+  for (int64_t f = 0; f < nf; ++f) m.ninterior += m.f_e1[f] >= 0;
+  m.nbnd = nbnd;
+  for (int64_t i = 0; i < nbnd; ++i) {
+    int32_t v[3];
+    for (int a = 0; a < d; ++a) v[a] = bnd[i * d + a];
+    std::sort(v, v + d);
+    for (int a = 0; a < d; ++a) OCMP_REQUIRE(v[a] >= 0 && v[a] < m.nv, "boundary facet vertex out of range");
+    uint64_t key = d == 2 ? (uint64_t)v[0] * nv + (uint64_t)v[1]
+                          : ((uint64_t)v[0] * nv + (uint64_t)v[1]) * nv + (uint64_t)v[2];
+    auto it = std::lower_bound(facet_keys.begin(), facet_keys.end(), key);
+    OCMP_REQUIRE(it != facet_keys.end() && *it == key, "boundary facet is not a facet of the mesh");
+    int64_t f = it - facet_keys.begin();
+    OCMP_REQUIRE(m.f_e1[f] < 0, "boundary facet is interior");
+    OCMP_REQUIRE(bmark[i] >= 0 && bmark[i] < 64, "boundary marker must be in [0,64)");
+    m.f_marker[f] = bmark[i];
+  }
+  m.elem_nb.assign((size_t)m.nel * nvl, 0);
+  m.elem_nbk.assign((size_t)m.nel * nvl, -1);
+  for (int64_t e = 0; e < m.nel; ++e)
+    for (int k = 0; k < nvl; ++k) {
+      int32_t f = m.elem_facets[e * nvl + k];
+      int32_t other = m.f_e0[f] == e ? m.f_e1[f] : m.f_e0[f];
+      int32_t ok = m.f_e0[f] == e ? m.f_k1[f] : m.f_k0[f];
+      if (other >= 0) {
+        m.elem_nb[e * nvl + k] = other;
+        m.elem_nbk[e * nvl + k] = (int8_t)ok;
+      } else {
+        int mk = m.f_marker[f] >= 0 ? m.f_marker[f] : 64;
+        m.elem_nb[e * nvl + k] = -1 - mk;
+      }
+    }
+}
+
+// ------------------------------------------------------------------ spaces
+
+static int64_t find_edge(const Mesh &m, int32_t a, int32_t b) {
+  if (a > b) std::swap(a, b);
+  // edges are sorted by (a, b)
+  int64_t lo = 0, hi = m.nedge;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) / 2;
+    int32_t ma = m.edges[2 * mid], mb = m.edges[2 * mid + 1];
+    if (ma < a || (ma == a && mb < b))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo >= m.nedge || m.edges[2 * lo] != a || m.edges[2 * lo + 1] != b)
+    throw Error(OCMP_ERR_INTERNAL, "edge lookup failed");
+  return lo;
+}
+
+static void build_space(Space &s, int ncomp, const ocmp_fe_desc *comps) {
+  Mesh &m = *s.mesh;
+  const int d = m.dim, nvl = d + 1;
+  int loc = 0;
+  int64_t off = 0;
+  for (int c = 0; c < ncomp; ++c) {
+    const ocmp_fe_desc &fd = comps[c];
+    OCMP_REQUIRE(fd.ncomp == 1 || fd.ncomp == d, "ncomp must be 1 or dim");
+    OCMP_REQUIRE(fd.kind == OCMP_H1 || fd.dirichlet_mask == 0, "Dirichlet dofs exist only for H1");
+    for (int k = 0; k < fd.ncomp; ++k) {
+      Block b;
+      b.kind = fd.kind;
+      b.order = fd.order;
+      b.dim = d;
+      b.nloc = nloc1(fd.kind, d, fd.order);
+      if (fd.kind == OCMP_H1) {
+        b.n1 = m.nv + (int64_t)(fd.order - 1) * m.nedge;
+        if (fd.order >= 3) b.n1 += d == 2 ? m.nel : m.nface;
+      } else {
+        b.n1 = m.nel * (int64_t)b.nloc;
+      }
+      b.dof_off = off;
+      b.loc_off = loc;
+      b.dirichlet_mask = fd.dirichlet_mask;
+      b.comp = c;
+      off += b.n1;
+      loc += b.nloc;
+      s.blocks.push_back(b);
+    }
+  }
+  OCMP_REQUIRE(off < (int64_t)INT32_MAX, "too many dofs for 32-bit indices");
+  s.ndof = off;
+  s.nloc = loc;
+  s.edofs.assign((size_t)m.nel * loc, 0);
+  const int nle = d == 2 ? 3 : 6;
+  for (const Block &b : s.blocks) {
+    for (int64_t e = 0; e < m.nel; ++e) {
+      int32_t *ed = &s.edofs[e * loc + b.loc_off];
+      if (b.kind == OCMP_L2) {
+        for (int i = 0; i < b.nloc; ++i) ed[i] = (int32_t)(b.dof_off + e * b.nloc + i);
+        continue;
+      }
+      int c = 0;
+      for (int i = 0; i < nvl; ++i) ed[c++] = (int32_t)(b.dof_off + m.elems[e * nvl + i]);
+      for (int k = 0; k < nle; ++k)
+        for (int q = 0; q < b.order - 1; ++q)
+          ed[c++] = (int32_t)(b.dof_off + m.nv + (int64_t)(b.order - 1) * m.elem_edges[e * nle + k] + q);
+      if (b.order >= 3) {
+        int64_t base = b.dof_off + m.nv + 2 * m.nedge;
+        if (d == 2)
+          ed[c++] = (int32_t)(base + e);
+        else
+          for (int f = 0; f < 4; ++f) ed[c++] = (int32_t)(base + m.elem_faces[e * 4 + f]);
+      }
+    }
+  }
+  // free dofs
+  s.free.assign(s.ndof, 1);
+  for (const Block &b : s.blocks) {
+    if (b.kind != OCMP_H1 || !b.dirichlet_mask) continue;
+    for (int64_t f = 0; f < m.nfacet; ++f) {
+      int mk = m.f_marker[f];
+      if (m.f_e1[f] >= 0 || mk < 0 || !((b.dirichlet_mask >> mk) & 1)) continue;
+      const int32_t *fv = &m.facet_verts[f * d];
+      for (int a = 0; a < d; ++a) s.free[b.dof_off + fv[a]] = 0;
+      if (b.order >= 2) {
+        if (d == 2) {
+          for (int q = 0; q < b.order - 1; ++q) s.free[b.dof_off + m.nv + (b.order - 1) * f + q] = 0;
+        } else {
+          const int pr[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+          for (auto &p : pr) {
+            int64_t ei = find_edge(m, fv[p[0]], fv[p[1]]);
+            for (int q = 0; q < b.order - 1; ++q) s.free[b.dof_off + m.nv + (b.order - 1) * ei + q] = 0;
+          }
+        }
+      }
+      if (b.order >= 3 && d == 3) s.free[b.dof_off + m.nv + 2 * m.nedge + f] = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ pattern + schedule
+
+void build_pattern(Matrix *A) {
+  Space &s = *A->space;
+  Mesh &m = *s.mesh;
+  const int d = m.dim, nvl = d + 1;
+  bool all_l2 = true, all_h1 = true;
+  for (auto &b : s.blocks) {
+    all_l2 &= b.kind == OCMP_L2;
+    all_h1 &= b.kind == OCMP_H1;
+  }
+  OCMP_REQUIRE(all_l2 || all_h1, "compound spaces mixing H1 and L2 blocks are not supported");
+  A->n = s.ndof;
+  if (all_l2) {
+    OCMP_REQUIRE(s.blocks.size() == 1, "DG path supports a single L2 block");
+    const int nb = s.blocks[0].nloc;
+    A->dg = true;
+    A->dg_nb = nb;
+    std::vector<int8_t> pos((size_t)m.nel * (nvl + 1), -1);
+    std::vector<int32_t> bcol((size_t)m.nel * 4, -1);
+    A->row_ptr.assign(A->n + 1, 0);
+    for (int64_t e = 0; e < m.nel; ++e) {
+      int32_t blist[4];
+      int nbk = 0;
+      blist[nbk++] = (int32_t)e;
+      if (s.dgjumps)
+        for (int k = 0; k < nvl; ++k)
+          if (m.elem_nb[e * nvl + k] >= 0) blist[nbk++] = m.elem_nb[e * nvl + k];
+      std::sort(blist, blist + nbk);
+      for (int t = 0; t < nbk; ++t) {
+        bcol[e * 4 + t] = blist[t];
+        if (blist[t] == e) pos[e * (nvl + 1)] = (int8_t)t;
+        for (int k = 0; k < nvl; ++k)
+          if (s.dgjumps && m.elem_nb[e * nvl + k] == blist[t]) pos[e * (nvl + 1) + 1 + k] = (int8_t)t;
+      }
+      for (int i = 0; i < nb; ++i) A->row_ptr[e * nb + i + 1] = nbk * nb;
+    }
+    for (int64_t r = 0; r < A->n; ++r) A->row_ptr[r + 1] += A->row_ptr[r];
+    A->nnz = A->row_ptr[A->n];
+    A->col.resize(A->nnz);
+    for (int64_t e = 0; e < m.nel; ++e)
+      for (int i = 0; i < nb; ++i) {
+        int64_t p = A->row_ptr[e * nb + i];
+        for (int t = 0; t < 4 && bcol[e * 4 + t] >= 0; ++t)
+          for (int j = 0; j < nb; ++j) A->col[p++] = bcol[e * 4 + t] * nb + j;
+      }
+    A->d_dg_pos.upload(pos, A->ctx->stream);
+    A->d_dg_bcol.upload(bcol, A->ctx->stream);
+  } else {
+    A->dg = false;
+    const int nl = s.nloc;
+    // dof -> elements incidence
+    std::vector<int64_t> inc_ptr(s.ndof + 1, 0);
+    for (int64_t t = 0; t < m.nel * (int64_t)nl; ++t) inc_ptr[s.edofs[t] + 1]++;
+    for (int64_t r = 0; r < s.ndof; ++r) inc_ptr[r + 1] += inc_ptr[r];
+    std::vector<int32_t> inc(inc_ptr[s.ndof]);
+    {
+      std::vector<int64_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+      for (int64_t e = 0; e < m.nel; ++e)
+        for (int i = 0; i < nl; ++i) inc[fill[s.edofs[e * nl + i]]++] = (int32_t)e;
+    }
+    A->row_ptr.assign(s.ndof + 1, 0);
+    auto row_cols = [&](int64_t r, std::vector<int32_t> &buf) {
+      buf.clear();
+      for (int64_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+        const int32_t *ed = &s.edofs[(int64_t)inc[t] * nl];
+        buf.insert(buf.end(), ed, ed + nl);
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+    };
+#pragma omp parallel
+    {
+      std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 4096)
+      for (int64_t r = 0; r < s.ndof; ++r) {
+        row_cols(r, buf);
+        A->row_ptr[r + 1] = (int32_t)buf.size();
+      }
+    }
+    std::vector<int64_t> rp(s.ndof + 1, 0);
+    for (int64_t r = 0; r < s.ndof; ++r) rp[r + 1] = rp[r] + A->row_ptr[r + 1];
+    OCMP_REQUIRE(rp[s.ndof] < (int64_t)INT32_MAX, "nnz exceeds 32-bit CSR offsets");
+    for (int64_t r = 0; r <= s.ndof; ++r) A->row_ptr[r] = (int32_t)rp[r];
+    A->nnz = rp[s.ndof];
+    A->col.resize(A->nnz);
+#pragma omp parallel
+    {
+      std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 4096)
+      for (int64_t r = 0; r < s.ndof; ++r) {
+        row_cols(r, buf);
+        std::copy(buf.begin(), buf.end(), A->col.begin() + A->row_ptr[r]);
+      }
+    }
+  }
+  hipStream_t st = A->ctx->stream;
+  A->d_row_ptr.upload(A->row_ptr, st);
+  A->d_col.upload(A->col, st);
+  A->d_val.alloc(A->nnz);
+  OCMP_HIP(hipMemsetAsync(A->d_val.p, 0, A->nnz * sizeof(double), st));
+  // sub-wave group width for the scalar CSR SpMV: smallest power of two >= average row length / 2
+  double avg = A->n ? double(A->nnz) / double(A->n) : 1.0;
+  int g = 4;
+  while (g < 64 && g < avg * 0.75) g *= 2;
+  A->spmv_group = g;
+}
+
+void build_colouring(Space &s) {
+  Mesh &m = *s.mesh;
+  const int d = m.dim, nvl = d + 1;
+  // greedy colouring: elements sharing a vertex get different colours
+  std::vector<int64_t> vptr(m.nv + 1, 0);
+  for (int64_t t = 0; t < m.nel * nvl; ++t) vptr[m.elems[t] + 1]++;
+  for (int64_t v = 0; v < m.nv; ++v) vptr[v + 1] += vptr[v];
+  std::vector<int32_t> vel(vptr[m.nv]);
+  {
+    std::vector<int64_t> fill(vptr.begin(), vptr.end() - 1);
+    for (int64_t e = 0; e < m.nel; ++e)
+      for (int k = 0; k < nvl; ++k) vel[fill[m.elems[e * nvl + k]]++] = (int32_t)e;
+  }
+  std::vector<int16_t> colour(m.nel, -1);
+  int ncol = 0;
+  for (int64_t e = 0; e < m.nel; ++e) {
+    uint64_t used[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nvl; ++k) {
+      int32_t v = m.elems[e * nvl + k];
+      for (int64_t t = vptr[v]; t < vptr[v + 1]; ++t) {
+        int c = colour[vel[t]];
+        if (c >= 0) used[c >> 6] |= 1ull << (c & 63);
+      }
+    }
+    int c = 0;
+    while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1)) ++c;
+    OCMP_REQUIRE(c < 256, "colouring needs more than 256 colours");
+    colour[e] = (int16_t)c;
+    ncol = std::max(ncol, c + 1);
+  }
+  s.ncolours = ncol;
+  s.colour_off.assign(ncol + 1, 0);
+  for (int64_t e = 0; e < m.nel; ++e) s.colour_off[colour[e] + 1]++;
+  for (int c = 0; c < ncol; ++c) s.colour_off[c + 1] += s.colour_off[c];
+  std::vector<int32_t> order(m.nel);
+  {
+    std::vector<int64_t> fill(s.colour_off.begin(), s.colour_off.end() - 1);
+    for (int64_t e = 0; e < m.nel; ++e) order[fill[colour[e]]++] = (int32_t)e;
+  }
+  s.d_colour_elems.upload(order, m.ctx->stream);
+}
+
+void build_h1_schedule(Matrix *A) {
+  Space &s = *A->space;
+  Mesh &m = *s.mesh;
+  const int nl = s.nloc;
+  // element -> CSR slot map
+  std::vector<int32_t> slot((size_t)m.nel * nl * nl);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < m.nel; ++e) {
+    const int32_t *ed = &s.edofs[e * nl];
+    for (int i = 0; i < nl; ++i) {
+      int32_t r = ed[i];
+      const int32_t *c0 = A->col.data() + A->row_ptr[r];
+      const int32_t *c1 = A->col.data() + A->row_ptr[r + 1];
+      for (int j = 0; j < nl; ++j) {
+        const int32_t *p = std::lower_bound(c0, c1, ed[j]);
+        slot[((size_t)e * nl + i) * nl + j] = (int32_t)(p - A->col.data());
+      }
+    }
+  }
+  A->d_slot.upload(slot, A->ctx->stream);
+  OCMP_HIP(hipStreamSynchronize(A->ctx->stream));
+}
+
+}  // namespace ocmp
+
+using namespace ocmp;
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *ocmp_last_error(void) { return g_err.c_str(); }
+int ocmp_version(void) { return 1; }
+
+int ocmp_ctx_create(int device, void *hip_stream, ocmp_ctx **out) {
+  return guard([&] {
+    OCMP_REQUIRE(out, "out is NULL");
+    int n = 0;
+    OCMP_HIP(hipGetDeviceCount(&n));
+    OCMP_REQUIRE(device >= 0 && device < n, "device index out of range");
+    OCMP_HIP(hipSetDevice(device));
+    auto *c = new ocmp_ctx();
+    c->c.device = device;
+    if (hip_stream) {
+      c->c.stream = (hipStream_t)hip_stream;
+    } else {
+      OCMP_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+      c->c.own_stream = true;
+    }
+    c->c.scal.alloc(256);
+    c->c.red.alloc(1 << 20);
+    OCMP_HIP(hipHostMalloc((void **)&c->c.host_scal, 256 * sizeof(double), hipHostMallocDefault));
+    *out = c;
+  });
+}
+
+int ocmp_ctx_destroy(ocmp_ctx *ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto &r : ctx->c.pending) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ctx->c.event_pool) (void)hipEventDestroy(e);
+    if (ctx->c.host_scal) (void)hipHostFree(ctx->c.host_scal);
+    if (ctx->c.own_stream) (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+  });
+}
+
+int ocmp_ctx_sync(ocmp_ctx *ctx) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx, "ctx is NULL");
+    OCMP_HIP(hipStreamSynchronize(ctx->c.stream));
+  });
+}
+
+void *ocmp_ctx_stream(ocmp_ctx *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
+
+int ocmp_ctx_timing(ocmp_ctx *ctx, int enable) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx, "ctx is NULL");
+    ctx->c.timing = enable != 0;
+  });
+}
+
+int ocmp_ctx_timing_report(ocmp_ctx *ctx, char *buf, int64_t cap) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && buf && cap > 2, "bad arguments");
+    ctx->c.flush_timing();
+    std::string s = "{";
+    bool first = true;
+    for (auto &kv : ctx->c.acc) {
+      char tmp[256];
+      snprintf(tmp, sizeof tmp, "%s\"%s\": [%ld, %.6f, %.6e]", first ? "" : ", ",
+               ctx->c.class_names[kv.first].c_str(), kv.second.first, kv.second.second, ctx->c.acc_bytes[kv.first]);
+      s += tmp;
+      first = false;
+    }
+    s += "}";
+    OCMP_REQUIRE((int64_t)s.size() + 1 <= cap, "timing report buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    ctx->c.acc.clear();
+    ctx->c.acc_bytes.clear();
+  });
+}
+
+int ocmp_mesh_create(ocmp_ctx *ctx, int dim, int64_t nv, const double *coords, int64_t nel, const int32_t *elems,
+                     int64_t nbnd, const int32_t *bnd_facets, const int32_t *bnd_markers, ocmp_mesh **out) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && out && coords && elems, "NULL argument");
+    OCMP_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+    OCMP_REQUIRE(nv > 0 && nel > 0 && nbnd >= 0, "empty mesh");
+    OCMP_REQUIRE(nbnd == 0 || (bnd_facets && bnd_markers), "NULL boundary arrays");
+    OCMP_REQUIRE(nv < INT32_MAX && nel < INT32_MAX, "mesh too large for 32-bit ids");
+    OCMP_HIP(hipSetDevice(ctx->c.device));
+    auto *mm = new ocmp_mesh();
+    Mesh &m = mm->m;
+    try {
+      m.ctx = &ctx->c;
+      m.dim = dim;
+      m.nv = nv;
+      m.nel = nel;
+      m.coords.assign(coords, coords + nv * dim);
+      m.elems.assign(elems, elems + nel * (dim + 1));
+      for (int64_t e = 0; e < nel; ++e) {
+        int32_t *v = &m.elems[e * (dim + 1)];
+        for (int a = 0; a <= dim; ++a) OCMP_REQUIRE(v[a] >= 0 && v[a] < nv, "element vertex out of range");
+        std::sort(v, v + dim + 1);
+        for (int a = 0; a < dim; ++a) OCMP_REQUIRE(v[a] != v[a + 1], "degenerate element (repeated vertex)");
+      }
+      build_topology(m, nbnd, bnd_facets, bnd_markers);
+      hipStream_t st = ctx->c.stream;
+      m.d_coords.upload(m.coords, st);
+      m.d_elems.upload(m.elems, st);
+      m.d_elem_nb.upload(m.elem_nb, st);
+      m.d_elem_nbk.upload(m.elem_nbk, st);
+      OCMP_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete mm;
+      throw;
+    }
+    *out = mm;
+  });
+}
+
+int ocmp_mesh_info(const ocmp_mesh *mesh, int64_t *out) {
+  return guard([&] {
+    OCMP_REQUIRE(mesh && out, "NULL argument");
+    const Mesh &m = mesh->m;
+    int64_t v[8] = {m.nv, m.nedge, m.nface, m.nel, m.nfacet, m.ninterior, m.nbnd, m.dim};
+    std::memcpy(out, v, sizeof v);
+  });
+}
+
+int ocmp_mesh_destroy(ocmp_mesh *mesh) {
+  return guard([&] { delete mesh; });
+}
+
+int ocmp_space_create(ocmp_mesh *mesh, int ncomp, const ocmp_fe_desc *comps, int dgjumps, ocmp_space **out) {
+  return guard([&] {
+    OCMP_REQUIRE(mesh && comps && out && ncomp > 0, "bad arguments");
+    auto *sp = new ocmp_space();
+    try {
+      sp->s.mesh = &mesh->m;
+      sp->s.dgjumps = dgjumps != 0;
+      build_space(sp->s, ncomp, comps);
+      hipStream_t st = mesh->m.ctx->stream;
+      sp->s.d_edofs.upload(sp->s.edofs, st);
+      sp->s.d_free.upload(sp->s.free, st);
+      sp->s.d_edges.upload(mesh->m.edges, st);
+      bool all_h1 = true;
+      for (auto &b : sp->s.blocks) all_h1 &= b.kind == OCMP_H1;
+      if (all_h1) build_colouring(sp->s);
+      OCMP_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      delete sp;
+      throw;
+    }
+    *out = sp;
+  });
+}
+
+int ocmp_space_info(const ocmp_space *s, int64_t *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && out, "NULL argument");
+    int64_t nfree = 0;
+    for (auto f : s->s.free) nfree += f;
+    out[0] = s->s.ndof;
+    out[1] = s->s.nloc;
+    out[2] = (int64_t)s->s.blocks.size();
+    out[3] = nfree;
+  });
+}
+
+int ocmp_space_blocks(const ocmp_space *s, int64_t *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && out, "NULL argument");
+    for (size_t i = 0; i < s->s.blocks.size(); ++i) {
+      const Block &b = s->s.blocks[i];
+      out[4 * i + 0] = b.dof_off;
+      out[4 * i + 1] = b.n1;
+      out[4 * i + 2] = b.loc_off;
+      out[4 * i + 3] = b.nloc;
+    }
+  });
+}
+
+int ocmp_space_freedofs(const ocmp_space *s, uint8_t *mask) {
+  return guard([&] {
+    OCMP_REQUIRE(s && mask, "NULL argument");
+    std::memcpy(mask, s->s.free.data(), s->s.free.size());
+  });
+}
+
+int ocmp_space_element_dofs(const ocmp_space *s, int32_t *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && out, "NULL argument");
+    std::memcpy(out, s->s.edofs.data(), s->s.edofs.size() * sizeof(int32_t));
+  });
+}
+
+int ocmp_space_destroy(ocmp_space *s) {
+  return guard([&] { delete s; });
+}
+
+int ocmp_matrix_create(ocmp_space *s, ocmp_matrix **out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && out, "NULL argument");
+    auto *A = new ocmp_matrix();
+    try {
+      A->A.space = &s->s;
+      A->A.ctx = s->s.mesh->ctx;
+      OCMP_HIP(hipSetDevice(A->A.ctx->device));
+      build_pattern(&A->A);
+      if (!A->A.dg) build_h1_schedule(&A->A);
+      OCMP_HIP(hipStreamSynchronize(A->A.ctx->stream));
+    } catch (...) {
+      delete A;
+      throw;
+    }
+    *out = A;
+  });
+}
+
+int ocmp_matrix_info(const ocmp_matrix *A, int64_t *out) {
+  return guard([&] {
+    OCMP_REQUIRE(A && out, "NULL argument");
+    out[0] = A->A.n;
+    out[1] = A->A.nnz;
+    out[2] = A->A.space->ncolours;
+    out[3] = A->A.dg ? 1 : 0;
+  });
+}
+
+int ocmp_matrix_pattern(const ocmp_matrix *A, int32_t *row_ptr, int32_t *col) {
+  return guard([&] {
+    OCMP_REQUIRE(A && row_ptr && col, "NULL argument");
+    std::memcpy(row_ptr, A->A.row_ptr.data(), A->A.row_ptr.size() * sizeof(int32_t));
+    std::memcpy(col, A->A.col.data(), A->A.col.size() * sizeof(int32_t));
+  });
+}
+
+int ocmp_matrix_values(const ocmp_matrix *A, double *vals) {
+  return guard([&] {
+    OCMP_REQUIRE(A && vals, "NULL argument");
+    OCMP_HIP(hipMemcpyAsync(vals, A->A.d_val.p, A->A.nnz * sizeof(double), hipMemcpyDeviceToHost,
+                            A->A.ctx->stream));
+    OCMP_HIP(hipStreamSynchronize(A->A.ctx->stream));
+  });
+}
+
+int ocmp_matrix_destroy(ocmp_matrix *A) {
+  return guard([&] {
+    if (A) (void)hipStreamSynchronize(A->A.ctx->stream);
+    delete A;
+  });
+}
+
+int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, int nfields,
+                           const ocmp_field *fields, int accumulate) {
+  return guard([&] {
+    OCMP_REQUIRE(A && (nterms == 0 || terms) && (nfields == 0 || fields), "NULL argument");
+    std::vector<ocmp_term> t(terms, terms + nterms);
+    std::vector<ocmp_field> f(fields, fields + nfields);
+    const Space &s = *A->A.space;
+    for (auto &x : t) {
+      OCMP_REQUIRE(x.trial >= 0 && x.trial < (int)s.blocks.size() && x.test >= 0 && x.test < (int)s.blocks.size(),
+                   "term block index out of range");
+      OCMP_REQUIRE(x.kind == OCMP_BFI_DG_FACET || (x.kind >= 0 && x.kind <= OCMP_BFI_DTEST), "unknown term kind");
+      OCMP_REQUIRE(x.weight_mode == OCMP_WEIGHT_NONE || (x.weight_field >= 0 && x.weight_field < nfields),
+                   "weight field index out of range");
+      OCMP_REQUIRE(x.wind_field < 0 || x.wind_field + s.mesh->dim <= nfields, "wind field index out of range");
+      OCMP_REQUIRE(x.kind != OCMP_BFI_DTRIAL && x.kind != OCMP_BFI_DTEST ||
+                       (x.dir >= 0 && x.dir < s.mesh->dim),
+                   "derivative direction out of range");
+    }
+    for (auto &x : f) {
+      OCMP_REQUIRE(x.space && x.vec, "field without space or vector");
+      OCMP_REQUIRE(x.space->s.mesh == s.mesh, "field lives on another mesh");
+      OCMP_REQUIRE(x.block >= 0 && x.block < (int)x.space->s.blocks.size(), "field block out of range");
+      OCMP_REQUIRE(x.space->s.blocks[x.block].kind == OCMP_H1, "fields must be H1 blocks");
+    }
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    if (A->A.dg)
+      launch_assemble_dg(&A->A, t, accumulate != 0);
+    else
+      launch_assemble_h1(&A->A, t, f, accumulate != 0);
+    A->A.assembled = true;
+  });
+}
+
+int ocmp_assemble_linear(ocmp_space *s, double *b, int nterms, const ocmp_lterm *terms, int nfields,
+                         const ocmp_field *fields, int accumulate) {
+  return guard([&] {
+    OCMP_REQUIRE(s && b && (nterms == 0 || terms) && (nfields == 0 || fields), "NULL argument");
+    std::vector<ocmp_lterm> t(terms, terms + nterms);
+    std::vector<ocmp_field> f(fields, fields + nfields);
+    for (auto &x : t) {
+      OCMP_REQUIRE(x.test >= 0 && x.test < (int)s->s.blocks.size(), "term block index out of range");
+      OCMP_REQUIRE(x.fn != OCMP_FN_FIELD || (x.field >= 0 && x.field < nfields), "field index out of range");
+      OCMP_REQUIRE(x.weight_mode == OCMP_WEIGHT_NONE || (x.weight_field >= 0 && x.weight_field < nfields),
+                   "weight field index out of range");
+      OCMP_REQUIRE(x.kind != OCMP_LFI_DG_DIRICHLET || s->s.blocks[x.test].kind == OCMP_L2,
+                   "DG Dirichlet data needs an L2 block");
+    }
+    for (auto &x : f) {
+      OCMP_REQUIRE(x.space && x.vec, "field without space or vector");
+      OCMP_REQUIRE(x.space->s.mesh == s->s.mesh, "field lives on another mesh");
+      OCMP_REQUIRE(x.block >= 0 && x.block < (int)x.space->s.blocks.size(), "field block out of range");
+      OCMP_REQUIRE(x.space->s.blocks[x.block].kind == OCMP_H1, "fields must be H1 blocks");
+    }
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    launch_assemble_linear(&s->s, b, t, f, accumulate != 0);
+  });
+}
+
+int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double *c, uint64_t marker_mask) {
+  return guard([&] {
+    OCMP_REQUIRE(s && x && c, "NULL argument");
+    OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
+    OCMP_REQUIRE(s->s.blocks[block].kind == OCMP_H1, "Dirichlet values apply to H1 blocks");
+    OCMP_REQUIRE(fn == OCMP_FN_CONST || fn == OCMP_FN_SINPROD || fn == OCMP_FN_POLY2, "unsupported function");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    launch_dirichlet_set(&s->s, block, x, fn, c, marker_mask);
+  });
+}
+
+int ocmp_matvec(ocmp_matrix *A, const double *x, double *y) {
+  return guard([&] {
+    OCMP_REQUIRE(A && x && y, "NULL argument");
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    spmv_f64(&A->A, x, y);
+  });
+}
+
+int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **out) {
+  return guard([&] {
+    OCMP_REQUIRE(A && out, "NULL argument");
+    OCMP_REQUIRE(kind == OCMP_PRE_NONE || kind == OCMP_PRE_JACOBI || kind == OCMP_PRE_BLOCK_JACOBI,
+                 "unknown preconditioner");
+    OCMP_REQUIRE(precision == OCMP_FP64 || precision == OCMP_MIXED, "unknown precision");
+    OCMP_REQUIRE(kind != OCMP_PRE_BLOCK_JACOBI || A->A.dg, "block Jacobi needs an L2 (DG) matrix");
+    auto *P = new ocmp_precond();
+    P->P.A = &A->A;
+    P->P.kind = kind;
+    P->P.precision = precision;
+    *out = P;
+  });
+}
+
+int ocmp_precond_update(ocmp_precond *P) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.A->assembled, "matrix not assembled");
+    OCMP_HIP(hipSetDevice(P->P.A->ctx->device));
+    precond_update(&P->P);
+  });
+}
+
+int ocmp_precond_destroy(ocmp_precond *P) {
+  return guard([&] { delete P; });
+}
+
+int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x, const ocmp_solve_opts *opts,
+               ocmp_solve_stats *stats) {
+  return guard([&] {
+    OCMP_REQUIRE(A && b && x && opts && stats, "NULL argument");
+    OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
+    OCMP_REQUIRE(!P || P->P.A == &A->A, "preconditioner built for another matrix");
+    OCMP_REQUIRE(opts->tol > 0 && opts->maxit > 0, "tol and maxit must be positive");
+    OCMP_REQUIRE(opts->solver != OCMP_SOLVER_GMRES || (opts->restart >= 1 && opts->restart <= 200),
+                 "GMRES restart must be in [1, 200]");
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    if (P && !P->P.ready) precond_update(&P->P);
+    std::memset(stats, 0, sizeof(*stats));
+    solve(&A->A, P ? &P->P : nullptr, b, x, *opts, *stats);
+  });
+}
+
+}  // extern "C"

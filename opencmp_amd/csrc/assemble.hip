@@ -1,0 +1,1040 @@
+// Assembly kernels: element matrices of the OpenCMP weak forms on affine
+// simplices, scattered deterministically into CSR.
+//
+//  * H1 (continuous) spaces: one thread per (element, local test row).
+//    Constant-coefficient terms contract precomputed reference tensors with
+//    the element metric (K = |det J| sum_k g_k(e) T_k); terms with field
+//    coefficients (diffuse-interface weight phi, Oseen wind) integrate by
+//    quadrature. Elements are launched colour by colour (no two elements of a
+//    colour share a vertex), so the read-modify-write into CSR through the
+//    element->slot map is race-free and bitwise reproducible.
+//  * L2 (DG) spaces: owner-computes. The thread of (element e, row i) forms the
+//    whole CSR row: the volume block plus, for every facet of e, its own side of
+//    the facet integral (diagonal block) and the coupling to the neighbour's
+//    dofs. Each value is written exactly once; no colouring, no atomics.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace ocmp {
+
+constexpr int MAXT = 8;   // terms per launch
+constexpr int MAXF = 8;   // fields per launch
+constexpr int MAXB = 8;   // scalar blocks per space
+constexpr int MAXL = 40;  // compound local dofs
+constexpr int MAXNB = 10; // DG block size (P3)
+
+struct KTerm {
+  int kind, trial, test, dir;
+  double coef;
+  const double *elem_coef;
+  double wind[3];
+  int wind_field, weight_mode, weight_field, quad;
+  int tab;  // contraction: offset of the tensor set (ntens tensors of nl_test x nl_trial)
+  int ntens;
+  int sipg, upwind;
+  double eps, alpha;
+  unsigned long long dmask;
+};
+
+struct KField {
+  const double *vec;
+  const int32_t *edofs;  // element dof table of the field's space
+  int ld;                // its nloc
+  int loc_off;           // block offset within the element dof list
+  int nl;                // block local dofs
+  int tab_phi;           // offset of phi table (nq x nl) at the volume rule
+};
+
+struct KBlocks {
+  int n;
+  int loc_off[MAXB];
+  int nloc[MAXB];
+  int tab_phi[MAXB];   // volume rule: phi (nq x nl)
+  int tab_grad[MAXB];  // volume rule: reference gradients (nq x nl x dim)
+};
+
+struct KArgs {
+  int nterms, nfields, nq, nloc;
+  KTerm t[MAXT];
+  KField f[MAXF];
+  KBlocks b;
+  int tab_w;   // quadrature weights of the volume rule
+  int tab_lam; // barycentric points (nq x (dim+1))
+  const double *tab;
+};
+
+// ------------------------------------------------------------------ geometry
+
+template <int D>
+struct Geo {
+  double Jinv[D][D];  // Jinv[a][c] = d xhat_a / d x_c
+  double adet;
+  double x0[D];
+  double J[D][D];     // J[c][a] = (v_{a+1} - v_0)_c
+};
+
+template <int D>
+__device__ __forceinline__ void geometry(const double *__restrict__ coords, const int32_t *__restrict__ v, Geo<D> &g) {
+  double X[D + 1][D];
+#pragma unroll
+  for (int i = 0; i <= D; ++i)
+#pragma unroll
+    for (int c = 0; c < D; ++c) X[i][c] = coords[(int64_t)v[i] * D + c];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    g.x0[c] = X[0][c];
+#pragma unroll
+    for (int a = 0; a < D; ++a) g.J[c][a] = X[a + 1][c] - X[0][c];
+  }
+  if constexpr (D == 2) {
+    double det = g.J[0][0] * g.J[1][1] - g.J[0][1] * g.J[1][0];
+    double id = 1.0 / det;
+    g.Jinv[0][0] = g.J[1][1] * id;
+    g.Jinv[0][1] = -g.J[0][1] * id;
+    g.Jinv[1][0] = -g.J[1][0] * id;
+    g.Jinv[1][1] = g.J[0][0] * id;
+    g.adet = fabs(det);
+  } else {
+    const double(*J)[3] = g.J;
+    double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    double id = 1.0 / det;
+    g.Jinv[0][0] = c00 * id;
+    g.Jinv[1][0] = c01 * id;
+    g.Jinv[2][0] = c02 * id;
+    g.Jinv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    g.Jinv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    g.Jinv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    g.Jinv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    g.Jinv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    g.Jinv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    g.adet = fabs(det);
+  }
+}
+
+__device__ __forceinline__ double eval_fn(int fn, const double *c, const double *x, int dim) {
+  if (fn == OCMP_FN_CONST) return c[0];
+  if (fn == OCMP_FN_SINPROD) {
+    double s = c[0];
+    for (int d = 0; d < dim; ++d) s *= sin(M_PI * x[d]);
+    return s;
+  }
+  // POLY2
+  if (dim == 2) {
+    double X = x[0], Y = x[1];
+    return c[0] + c[1] * X + c[2] * Y + c[3] * X * X + c[4] * X * Y + c[5] * Y * Y;
+  }
+  double X = x[0], Y = x[1], Z = x[2];
+  return c[0] + c[1] * X + c[2] * Y + c[3] * Z + c[4] * X * X + c[5] * Y * Y + c[6] * Z * Z + c[7] * X * Y +
+         c[8] * X * Z + c[9] * Y * Z;
+}
+
+// value of a field (H1 block) at volume quadrature point q of element e
+__device__ __forceinline__ double field_at(const KField &f, const double *__restrict__ tab, int64_t e, int q) {
+  const double *phi = tab + f.tab_phi + q * f.nl;
+  const int32_t *ed = f.edofs + e * f.ld + f.loc_off;
+  double s = 0.0;
+  for (int l = 0; l < f.nl; ++l) s += phi[l] * f.vec[ed[l]];
+  return s;
+}
+
+// ------------------------------------------------------------------ H1 row kernel
+
+template <int D>
+__device__ void h1_row(const KArgs &A, const double *__restrict__ coords, const int32_t *__restrict__ elems, int64_t e,
+                       int i, double *row) {
+  Geo<D> g;
+  geometry<D>(coords, elems + e * (D + 1), g);
+  // which block is row i in
+  int bi = 0;
+  while (bi + 1 < A.b.n && i >= A.b.loc_off[bi + 1]) ++bi;
+  const int il = i - A.b.loc_off[bi];
+  const int nlt = A.b.nloc[bi];
+  for (int j = 0; j < A.nloc; ++j) row[j] = 0.0;
+  for (int t = 0; t < A.nterms; ++t) {
+    const KTerm &T = A.t[t];
+    if (T.test != bi) continue;
+    const int bj = T.trial;
+    const int nlj = A.b.nloc[bj];
+    double *r = row + A.b.loc_off[bj];
+    double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
+    if (!T.quad) {
+      // reference-tensor contraction
+      double fac[9];
+      int nf = 0;
+      if (T.kind == OCMP_BFI_GRADGRAD) {
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int b = 0; b < D; ++b) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < D; ++c) s += g.Jinv[a][c] * g.Jinv[b][c];
+            fac[nf++] = s * g.adet;
+          }
+      } else if (T.kind == OCMP_BFI_MASS) {
+        fac[nf++] = g.adet;
+      } else if (T.kind == OCMP_BFI_CONV) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          double s = 0.0;
+#pragma unroll
+          for (int c = 0; c < D; ++c) s += g.Jinv[a][c] * T.wind[c];
+          fac[nf++] = s * g.adet;
+        }
+      } else {  // DTRIAL / DTEST
+#pragma unroll
+        for (int a = 0; a < D; ++a) fac[nf++] = g.Jinv[a][T.dir] * g.adet;
+      }
+      const double *ten = A.tab + T.tab;
+      const int tsz = nlt * nlj;
+      for (int j = 0; j < nlj; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < nf; ++k) s += fac[k] * ten[k * tsz + il * nlj + j];
+        r[j] += ce * s;
+      }
+    } else {
+      // quadrature with field coefficients
+      const double *w = A.tab + A.tab_w;
+      const double *phit = A.tab + A.b.tab_phi[bi];
+      const double *grt = A.tab + A.b.tab_grad[bi];
+      const double *phij = A.tab + A.b.tab_phi[bj];
+      const double *grj = A.tab + A.b.tab_grad[bj];
+      for (int q = 0; q < A.nq; ++q) {
+        double c = ce * w[q] * g.adet;
+        if (T.weight_mode != OCMP_WEIGHT_NONE) {
+          double ph = field_at(A.f[T.weight_field], A.tab, e, q);
+          c *= T.weight_mode == OCMP_WEIGHT_FIELD ? ph : (1.0 - ph);
+        }
+        // physical gradient of the test function
+        double gi[D];
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc) {
+          double s = 0.0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) s += g.Jinv[a][cc] * grt[(q * nlt + il) * D + a];
+          gi[cc] = s;
+        }
+        const double pi = phit[q * nlt + il];
+        if (T.kind == OCMP_BFI_GRADGRAD) {
+          for (int j = 0; j < nlj; ++j) {
+            double s = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < D; ++cc) {
+              double gj = 0.0;
+#pragma unroll
+              for (int a = 0; a < D; ++a) gj += g.Jinv[a][cc] * grj[(q * nlj + j) * D + a];
+              s += gi[cc] * gj;
+            }
+            r[j] += c * s;
+          }
+        } else if (T.kind == OCMP_BFI_MASS) {
+          for (int j = 0; j < nlj; ++j) r[j] += c * pi * phij[q * nlj + j];
+        } else if (T.kind == OCMP_BFI_CONV) {
+          double wv[D];
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) wv[cc] = T.wind_field >= 0 ? field_at(A.f[T.wind_field + cc], A.tab, e, q)
+                                                                    : T.wind[cc];
+          double wg = 0.0;
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) wg += wv[cc] * gi[cc];
+          for (int j = 0; j < nlj; ++j) r[j] += c * wg * phij[q * nlj + j];
+        } else if (T.kind == OCMP_BFI_DTEST) {
+          for (int j = 0; j < nlj; ++j) r[j] += c * gi[T.dir] * phij[q * nlj + j];
+        } else {  // DTRIAL
+          for (int j = 0; j < nlj; ++j) {
+            double gj = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) gj += g.Jinv[a][T.dir] * grj[(q * nlj + j) * D + a];
+            r[j] += c * pi * gj;
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_h1_assemble(KArgs A, const double *__restrict__ coords,
+                                                     const int32_t *__restrict__ elems,
+                                                     const int32_t *__restrict__ colour_elems, int64_t cbeg,
+                                                     int64_t ccount, const int32_t *__restrict__ slot,
+                                                     double *__restrict__ val) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ccount * A.nloc) return;
+  const int64_t e = colour_elems[cbeg + t / A.nloc];
+  const int i = (int)(t % A.nloc);
+  double row[MAXL];
+  h1_row<D>(A, coords, elems, e, i, row);
+  const int32_t *sl = slot + (e * A.nloc + i) * A.nloc;
+  for (int j = 0; j < A.nloc; ++j) val[sl[j]] += row[j];
+}
+
+__global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ DG owner-computes kernel (triangles)
+
+struct KDG {
+  int nb;          // dofs per element
+  int nq;          // facet quadrature points
+  int tab_fphi;    // facet phi: [k][q][l]
+  int tab_fgrad;   // facet reference gradients: [k][q][l][2]
+  int tab_fw;      // facet weights (sum 1)
+  int has_facet;
+  double eps, alpha;
+  double wind[2];
+  unsigned long long dmask;
+  int sipg, upwind;
+};
+
+__global__ void __launch_bounds__(256) k_dg_assemble(KArgs A, KDG F, int64_t nel, const double *__restrict__ coords,
+                                                     const int32_t *__restrict__ elems,
+                                                     const int32_t *__restrict__ elem_nb,
+                                                     const int8_t *__restrict__ elem_nbk,
+                                                     const int8_t *__restrict__ dg_pos,
+                                                     const int32_t *__restrict__ row_ptr, double *__restrict__ val,
+                                                     int accumulate) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = F.nb;
+  if (t >= nel * nb) return;
+  const int64_t e = t / nb;
+  const int i = (int)(t % nb);
+  double diag[MAXNB];
+  h1_row<2>(A, coords, elems, e, i, diag);  // volume terms (contraction tables of the L2 block)
+  double off[3][MAXNB];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < nb; ++j) off[k][j] = 0.0;
+  if (F.has_facet) {
+    Geo<2> g;
+    geometry<2>(coords, elems + e * 3, g);
+    const double h_e = sqrt(g.adet);
+    const double *fphi = A.tab + F.tab_fphi;
+    const double *fgr = A.tab + F.tab_fgrad;
+    const double *fw = A.tab + F.tab_fw;
+    for (int k = 0; k < 3; ++k) {
+      // outward normal: -grad lambda_k / |grad lambda_k|
+      double gl[2];
+      if (k == 0) {
+        gl[0] = -(g.Jinv[0][0] + g.Jinv[1][0]);
+        gl[1] = -(g.Jinv[0][1] + g.Jinv[1][1]);
+      } else {
+        gl[0] = g.Jinv[k - 1][0];
+        gl[1] = g.Jinv[k - 1][1];
+      }
+      const double gn = sqrt(gl[0] * gl[0] + gl[1] * gl[1]);
+      const double n0 = -gl[0] / gn, n1 = -gl[1] / gn;
+      // facet length = 2 |T| |grad lambda_k| ... |F| = |det J| * |grad lambda_k| (2D)
+      const double flen = g.adet * gn;
+      const double wn = F.wind[0] * n0 + F.wind[1] * n1;
+      const int32_t nbv = elem_nb[e * 3 + k];
+      if (nbv >= 0) {
+        const int kk = elem_nbk[e * 3 + k];
+        Geo<2> g2;
+        geometry<2>(coords, elems + (int64_t)nbv * 3, g2);
+        const double hF = 0.5 * (h_e + sqrt(g2.adet));
+        const double al = F.alpha / hF;
+        for (int q = 0; q < F.nq; ++q) {
+          const double ww = fw[q] * flen;
+          const double *ph = fphi + (k * F.nq + q) * nb;
+          const double *gr = fgr + (k * F.nq + q) * nb * 2;
+          const double *ph2 = fphi + (kk * F.nq + q) * nb;
+          const double *gr2 = fgr + (kk * F.nq + q) * nb * 2;
+          const double pi = ph[i];
+          const double dni = (g.Jinv[0][0] * gr[2 * i] + g.Jinv[1][0] * gr[2 * i + 1]) * n0 +
+                             (g.Jinv[0][1] * gr[2 * i] + g.Jinv[1][1] * gr[2 * i + 1]) * n1;
+          const double up = wn > 0.0 ? wn : 0.0, dn = wn < 0.0 ? wn : 0.0;
+          for (int j = 0; j < nb; ++j) {
+            const double pj = ph[j];
+            const double dnj = (g.Jinv[0][0] * gr[2 * j] + g.Jinv[1][0] * gr[2 * j + 1]) * n0 +
+                               (g.Jinv[0][1] * gr[2 * j] + g.Jinv[1][1] * gr[2 * j + 1]) * n1;
+            const double qj = ph2[j];
+            const double dqj = (g2.Jinv[0][0] * gr2[2 * j] + g2.Jinv[1][0] * gr2[2 * j + 1]) * n0 +
+                               (g2.Jinv[0][1] * gr2[2 * j] + g2.Jinv[1][1] * gr2[2 * j + 1]) * n1;
+            double vd = 0.0, vo = 0.0;
+            if (F.sipg) {
+              vd += F.eps * (al * pi * pj - 0.5 * dnj * pi - 0.5 * dni * pj);
+              vo += F.eps * (-al * pi * qj - 0.5 * dqj * pi + 0.5 * dni * qj);
+            }
+            if (F.upwind) {
+              vd += up * pi * pj;
+              vo += dn * pi * qj;
+            }
+            diag[j] += ww * vd;
+            off[k][j] += ww * vo;
+          }
+        }
+      } else {
+        const int mk = -1 - nbv;
+        const bool dir = mk < 64 && ((F.dmask >> mk) & 1ull);
+        const double al = F.alpha / h_e;
+        const double up = wn > 0.0 ? wn : 0.0;
+        for (int q = 0; q < F.nq; ++q) {
+          const double ww = fw[q] * flen;
+          const double *ph = fphi + (k * F.nq + q) * nb;
+          const double *gr = fgr + (k * F.nq + q) * nb * 2;
+          const double pi = ph[i];
+          const double dni = (g.Jinv[0][0] * gr[2 * i] + g.Jinv[1][0] * gr[2 * i + 1]) * n0 +
+                             (g.Jinv[0][1] * gr[2 * i] + g.Jinv[1][1] * gr[2 * i + 1]) * n1;
+          for (int j = 0; j < nb; ++j) {
+            const double pj = ph[j];
+            double v = 0.0;
+            if (F.sipg && dir) {
+              const double dnj = (g.Jinv[0][0] * gr[2 * j] + g.Jinv[1][0] * gr[2 * j + 1]) * n0 +
+                                 (g.Jinv[0][1] * gr[2 * j] + g.Jinv[1][1] * gr[2 * j + 1]) * n1;
+              v += F.eps * (al * pi * pj - pi * dnj - pj * dni);
+            }
+            if (F.upwind) v += up * pi * pj;
+            diag[j] += ww * v;
+          }
+        }
+      }
+    }
+  }
+  // write the owned CSR row
+  const int64_t r = e * nb + i;
+  double *rv = val + row_ptr[r];
+  const int8_t *pos = dg_pos + e * 4;
+  if (accumulate) {
+    for (int j = 0; j < nb; ++j) rv[pos[0] * nb + j] += diag[j];
+    for (int k = 0; k < 3; ++k)
+      if (pos[1 + k] >= 0)
+        for (int j = 0; j < nb; ++j) rv[pos[1 + k] * nb + j] += off[k][j];
+  } else {
+    for (int j = 0; j < nb; ++j) rv[pos[0] * nb + j] = diag[j];
+    for (int k = 0; k < 3; ++k)
+      if (pos[1 + k] >= 0)
+        for (int j = 0; j < nb; ++j) rv[pos[1 + k] * nb + j] = off[k][j];
+  }
+}
+
+// ------------------------------------------------------------------ linear forms
+
+struct KLTerm {
+  int kind, test, fn, field;
+  double c[10];
+  double coef;
+  int weight_mode, weight_field;
+  double eps, alpha;
+  double wind[3];
+  unsigned long long dmask;
+};
+
+struct KLArgs {
+  int nterms, nfields, nq, nloc;
+  KLTerm t[MAXT];
+  KField f[MAXF];
+  KBlocks b;
+  int tab_w, tab_lam;
+  // DG facet data (L2 spaces)
+  int fnq, tab_fphi, tab_fgrad, tab_fw, tab_ft;
+  const double *tab;
+};
+
+template <int D>
+__device__ void lin_local(const KLArgs &A, const double *__restrict__ coords, const int32_t *__restrict__ elems,
+                          const int32_t *__restrict__ elem_nb, int64_t e, double *loc) {
+  Geo<D> g;
+  geometry<D>(coords, elems + e * (D + 1), g);
+  for (int j = 0; j < A.nloc; ++j) loc[j] = 0.0;
+  const double *w = A.tab + A.tab_w;
+  const double *lam = A.tab + A.tab_lam;
+  for (int t = 0; t < A.nterms; ++t) {
+    const KLTerm &T = A.t[t];
+    const int bi = T.test;
+    const int nl = A.b.nloc[bi];
+    double *r = loc + A.b.loc_off[bi];
+    if (T.kind == OCMP_LFI_SOURCE) {
+      const double *phi = A.tab + A.b.tab_phi[bi];
+      for (int q = 0; q < A.nq; ++q) {
+        double x[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          double s = g.x0[c];
+#pragma unroll
+          for (int a = 0; a < D; ++a) s += g.J[c][a] * lam[q * (D + 1) + a + 1];
+          x[c] = s;
+        }
+        double fv = T.fn == OCMP_FN_FIELD ? T.c[0] * field_at(A.f[T.field], A.tab, e, q) : eval_fn(T.fn, T.c, x, D);
+        double c = T.coef * fv * w[q] * g.adet;
+        if (T.weight_mode != OCMP_WEIGHT_NONE) {
+          double ph = field_at(A.f[T.weight_field], A.tab, e, q);
+          c *= T.weight_mode == OCMP_WEIGHT_FIELD ? ph : (1.0 - ph);
+        }
+        for (int j = 0; j < nl; ++j) r[j] += c * phi[q * nl + j];
+      }
+    } else if constexpr (D == 2) {
+      // DG Dirichlet data on boundary facets of this element
+      const double *fphi = A.tab + A.tab_fphi;
+      const double *fgr = A.tab + A.tab_fgrad;
+      const double *fw = A.tab + A.tab_fw;
+      const double *ft = A.tab + A.tab_ft;
+      const int32_t *v = elems + e * 3;
+      for (int k = 0; k < 3; ++k) {
+        const int32_t nbv = elem_nb[e * 3 + k];
+        if (nbv >= 0) continue;
+        const int mk = -1 - nbv;
+        if (mk >= 64 || !((T.dmask >> mk) & 1ull)) continue;
+        double gl[2];
+        if (k == 0) {
+          gl[0] = -(g.Jinv[0][0] + g.Jinv[1][0]);
+          gl[1] = -(g.Jinv[0][1] + g.Jinv[1][1]);
+        } else {
+          gl[0] = g.Jinv[k - 1][0];
+          gl[1] = g.Jinv[k - 1][1];
+        }
+        const double gn = sqrt(gl[0] * gl[0] + gl[1] * gl[1]);
+        const double n0 = -gl[0] / gn, n1 = -gl[1] / gn;
+        const double flen = g.adet * gn;
+        const double al = T.alpha / sqrt(g.adet);
+        const double wn = T.wind[0] * n0 + T.wind[1] * n1;
+        const double dn = wn < 0.0 ? wn : 0.0;
+        // facet k runs from local vertex a to b (sorted) = lower to higher global id
+        const int a = k == 0 ? 1 : 0, b = k == 2 ? 1 : 2;
+        for (int q = 0; q < A.fnq; ++q) {
+          const double tq = ft[q];
+          double x[2];
+          for (int c = 0; c < 2; ++c)
+            x[c] = (1.0 - tq) * coords[(int64_t)v[a] * 2 + c] + tq * coords[(int64_t)v[b] * 2 + c];
+          const double gv = eval_fn(T.fn, T.c, x, 2) * T.coef;
+          const double ww = fw[q] * flen;
+          const double *ph = fphi + (k * A.fnq + q) * nl;
+          const double *gr = fgr + (k * A.fnq + q) * nl * 2;
+          for (int j = 0; j < nl; ++j) {
+            const double dnj = (g.Jinv[0][0] * gr[2 * j] + g.Jinv[1][0] * gr[2 * j + 1]) * n0 +
+                               (g.Jinv[0][1] * gr[2 * j] + g.Jinv[1][1] * gr[2 * j + 1]) * n1;
+            r[j] += ww * gv * (T.eps * (al * ph[j] - dnj) - dn * ph[j]);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_linear(KLArgs A, const double *__restrict__ coords,
+                                                const int32_t *__restrict__ elems,
+                                                const int32_t *__restrict__ elem_nb,
+                                                const int32_t *__restrict__ elem_list, int64_t ebeg, int64_t ecount,
+                                                const int32_t *__restrict__ edofs, double *__restrict__ b,
+                                                int accumulate_owned) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ecount) return;
+  const int64_t e = elem_list ? elem_list[ebeg + t] : ebeg + t;
+  double loc[MAXL];
+  lin_local<D>(A, coords, elems, elem_nb, e, loc);
+  const int32_t *ed = edofs + e * A.nloc;
+  if (elem_list || accumulate_owned) {
+    for (int j = 0; j < A.nloc; ++j) b[ed[j]] += loc[j];
+  } else {
+    for (int j = 0; j < A.nloc; ++j) b[ed[j]] = loc[j];
+  }
+}
+
+// ------------------------------------------------------------------ Dirichlet lift
+
+__global__ void k_dir_verts(const int32_t *__restrict__ verts, int64_t nv, const double *__restrict__ coords, int dim,
+                            int fn, const double *cc, int64_t off, double *__restrict__ x) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nv) return;
+  double c[10];
+  for (int i = 0; i < 10; ++i) c[i] = cc[i];
+  const int32_t v = verts[t];
+  x[off + v] = eval_fn(fn, c, coords + (int64_t)v * dim, dim);
+}
+
+// 1D L2 projection onto the edge bubbles of (g - linear interpolant) along edge a->b (a < b)
+__global__ void k_dir_edges(const int32_t *__restrict__ eids, int64_t ne, const int32_t *__restrict__ edges,
+                            const double *__restrict__ coords, int dim, int fn, const double *cc, int order,
+                            int64_t off, int64_t nvtx, double *__restrict__ x) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ne) return;
+  double c[10];
+  for (int i = 0; i < 10; ++i) c[i] = cc[i];
+  const int32_t ei = eids[t];
+  const int32_t a = edges[2 * ei], b = edges[2 * ei + 1];
+  double xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    xa[d] = coords[(int64_t)a * dim + d];
+    xb[d] = coords[(int64_t)b * dim + d];
+  }
+  const double ga = eval_fn(fn, c, xa, dim), gb = eval_fn(fn, c, xb, dim);
+  // 6-point Gauss-Legendre on [0,1]
+  const double gt[6] = {0.033765242898423986, 0.16939530676686776, 0.38069040695840156,
+                        0.61930959304159844, 0.83060469323313224, 0.96623475710157603};
+  const double gw[6] = {0.085662246189585172, 0.18038078652406930, 0.23395696728634552,
+                        0.23395696728634552, 0.18038078652406930, 0.085662246189585172};
+  double r2 = 0.0, r3 = 0.0, m2 = 0.0, m3 = 0.0;
+  for (int q = 0; q < 6; ++q) {
+    const double s = gt[q];
+    double xq[3];
+    for (int d = 0; d < dim; ++d) xq[d] = (1.0 - s) * xa[d] + s * xb[d];
+    const double res = eval_fn(fn, c, xq, dim) - ((1.0 - s) * ga + s * gb);
+    const double b2 = (1.0 - s) * s;             // lambda_a lambda_b
+    const double b3 = (1.0 - s) * s * (2.0 * s - 1.0);  // lambda_a lambda_b (lambda_b - lambda_a)
+    r2 += gw[q] * res * b2;
+    r3 += gw[q] * res * b3;
+    m2 += gw[q] * b2 * b2;
+    m3 += gw[q] * b3 * b3;
+  }
+  // b2 and b3 are orthogonal on [0,1]
+  const int64_t base = off + nvtx + (int64_t)(order - 1) * ei;
+  x[base] = r2 / m2;
+  if (order >= 3) x[base + 1] = r3 / m3;
+}
+
+__global__ void k_dir_faces(const int32_t *__restrict__ fids, int64_t nf, int64_t base, double *__restrict__ x) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nf) return;
+  x[base + fids[t]] = 0.0;
+}
+
+// ================================================================== host side
+
+static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// Tables for a set of blocks: volume rule, per-block phi/grad, contraction tensors.
+struct TableBuilder {
+  std::vector<double> v;
+  int add(const std::vector<double> &x) {
+    int o = (int)v.size();
+    v.insert(v.end(), x.begin(), x.end());
+    return o;
+  }
+};
+
+static void fill_block_tables(TableBuilder &tb, const Space &s, const Rule &rule, KBlocks &kb) {
+  const int d = s.mesh->dim;
+  kb.n = (int)s.blocks.size();
+  OCMP_REQUIRE(kb.n <= MAXB, "too many scalar blocks");
+  for (int bi = 0; bi < kb.n; ++bi) {
+    const Block &b = s.blocks[bi];
+    kb.loc_off[bi] = b.loc_off;
+    kb.nloc[bi] = b.nloc;
+    std::vector<double> phi((size_t)rule.nq * b.nloc), gr((size_t)rule.nq * b.nloc * d);
+    for (int q = 0; q < rule.nq; ++q)
+      eval_basis(b.kind, d, b.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b.nloc], &gr[(size_t)q * b.nloc * d]);
+    kb.tab_phi[bi] = tb.add(phi);
+    kb.tab_grad[bi] = tb.add(gr);
+  }
+}
+
+// reference tensors of a constant-coefficient term
+static int add_tensors(TableBuilder &tb, const Space &s, const ocmp_term &t, int &ntens) {
+  const int d = s.mesh->dim;
+  const Block &bt = s.blocks[t.test], &bj = s.blocks[t.trial];
+  Rule rule = simplex_rule(d, bt.order + bj.order + (bt.kind == OCMP_L2 || bj.kind == OCMP_L2 ? 2 : 0));
+  const int nt = bt.nloc, nj = bj.nloc;
+  std::vector<double> pt(nt), gt(nt * d), pj(nj), gj(nj * d);
+  int nk = 0;
+  switch (t.kind) {
+    case OCMP_BFI_GRADGRAD: nk = d * d; break;
+    case OCMP_BFI_MASS: nk = 1; break;
+    default: nk = d; break;
+  }
+  std::vector<double> T((size_t)nk * nt * nj, 0.0);
+  for (int q = 0; q < rule.nq; ++q) {
+    eval_basis(bt.kind, d, bt.order, &rule.lam[q * (d + 1)], pt.data(), gt.data());
+    eval_basis(bj.kind, d, bj.order, &rule.lam[q * (d + 1)], pj.data(), gj.data());
+    const double w = rule.w[q];
+    for (int i = 0; i < nt; ++i)
+      for (int j = 0; j < nj; ++j) {
+        double *o = &T[(size_t)i * nj + j];
+        const size_t tsz = (size_t)nt * nj;
+        if (t.kind == OCMP_BFI_GRADGRAD) {
+          for (int a = 0; a < d; ++a)
+            for (int b = 0; b < d; ++b) o[(a * d + b) * tsz] += w * gt[i * d + a] * gj[j * d + b];
+        } else if (t.kind == OCMP_BFI_MASS) {
+          o[0] += w * pt[i] * pj[j];
+        } else if (t.kind == OCMP_BFI_CONV || t.kind == OCMP_BFI_DTEST) {
+          for (int a = 0; a < d; ++a) o[a * tsz] += w * gt[i * d + a] * pj[j];
+        } else {  // DTRIAL
+          for (int a = 0; a < d; ++a) o[a * tsz] += w * pt[i] * gj[j * d + a];
+        }
+      }
+  }
+  ntens = nk;
+  return tb.add(T);
+}
+
+static int max_order(const Space &s) {
+  int p = 0;
+  for (auto &b : s.blocks) p = std::max(p, b.order);
+  return p;
+}
+
+static DBuf<double> *cached_table(Space &s, const std::string &key, const TableBuilder &tb, hipStream_t st) {
+  auto it = s.tables.find(key);
+  if (it != s.tables.end()) return it->second;
+  auto *buf = new DBuf<double>();
+  buf->upload(tb.v, st);
+  OCMP_HIP(hipStreamSynchronize(st));  // host vector goes out of scope
+  s.tables[key] = buf;
+  return buf;
+}
+
+static std::string term_key(const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields) {
+  std::string k = "B";
+  for (auto &t : terms) {
+    bool quad = t.weight_mode != OCMP_WEIGHT_NONE || t.wind_field >= 0;
+    k += "|" + std::to_string(t.kind) + "," + std::to_string(t.trial) + "," + std::to_string(t.test) + "," +
+         std::to_string(quad);
+  }
+  for (auto &f : fields) k += "|f" + std::to_string((uintptr_t)f.space) + "," + std::to_string(f.block);
+  return k;
+}
+
+static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
+                        KArgs &A, TableBuilder &tb, hipStream_t st) {
+  const int d = s.mesh->dim;
+  OCMP_REQUIRE((int)terms.size() <= MAXT, "too many terms in one assembly call (max 8)");
+  OCMP_REQUIRE((int)fields.size() <= MAXF, "too many fields (max 8)");
+  OCMP_REQUIRE(s.nloc <= MAXL, "compound element has too many dofs");
+  std::memset(&A, 0, sizeof A);
+  A.nterms = (int)terms.size();
+  A.nfields = (int)fields.size();
+  A.nloc = s.nloc;
+  int pf = 0;
+  for (auto &f : fields) pf = std::max(pf, f.space->s.blocks[f.block].order);
+  Rule rule = simplex_rule(d, 2 * max_order(s) + pf + 1);
+  A.nq = rule.nq;
+  fill_block_tables(tb, s, rule, A.b);
+  A.tab_w = tb.add(rule.w);
+  A.tab_lam = tb.add(rule.lam);
+  for (size_t i = 0; i < fields.size(); ++i) {
+    const Space &fs = fields[i].space->s;
+    const Block &fb = fs.blocks[fields[i].block];
+    KField &kf = A.f[i];
+    kf.vec = fields[i].vec;
+    kf.edofs = fs.d_edofs.p;
+    kf.ld = fs.nloc;
+    kf.loc_off = fb.loc_off;
+    kf.nl = fb.nloc;
+    std::vector<double> phi((size_t)rule.nq * fb.nloc), gr((size_t)rule.nq * fb.nloc * d);
+    for (int q = 0; q < rule.nq; ++q)
+      eval_basis(fb.kind, d, fb.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * fb.nloc],
+                 &gr[(size_t)q * fb.nloc * d]);
+    kf.tab_phi = tb.add(phi);
+  }
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const ocmp_term &t = terms[i];
+    KTerm &k = A.t[i];
+    k.kind = t.kind;
+    k.trial = t.trial;
+    k.test = t.test;
+    k.dir = t.dir;
+    k.coef = t.coef;
+    k.elem_coef = t.elem_coef;
+    for (int c = 0; c < 3; ++c) k.wind[c] = t.wind[c];
+    k.wind_field = t.wind_field;
+    k.weight_mode = t.weight_mode;
+    k.weight_field = t.weight_field;
+    k.quad = (t.weight_mode != OCMP_WEIGHT_NONE || t.wind_field >= 0) ? 1 : 0;
+    k.sipg = t.sipg;
+    k.upwind = t.upwind;
+    k.eps = t.eps;
+    k.alpha = t.alpha;
+    k.dmask = t.dirichlet_mask;
+    if (t.kind == OCMP_BFI_DG_FACET) {
+      k.test = -1;  // never matches a row block in the volume pass
+      continue;
+    }
+    if (!k.quad) k.tab = add_tensors(tb, s, t, k.ntens);
+  }
+}
+
+void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
+                        bool accumulate) {
+  Space &s = *M->space;
+  Ctx *ctx = M->ctx;
+  hipStream_t st = ctx->stream;
+  for (auto &t : terms) OCMP_REQUIRE(t.kind != OCMP_BFI_DG_FACET, "facet terms need an L2 (DG) space");
+  KArgs A;
+  TableBuilder tb;
+  build_kargs(s, terms, fields, A, tb, st);
+  DBuf<double> *tab = cached_table(s, term_key(terms, fields), tb, st);
+  A.tab = tab->p;
+  // algorithmic: every CSR value written once + element table + coordinates
+  KTimer tm(ctx, ctx->cls("assemble_h1"),
+            8.0 * M->nnz + 4.0 * s.mesh->nel * (s.mesh->dim + 1) + 8.0 * s.mesh->nv * s.mesh->dim);
+  if (!accumulate) k_fill<<<nblk(M->nnz), 256, 0, st>>>(M->d_val.p, M->nnz, 0.0);
+  const Mesh &m = *s.mesh;
+  for (int c = 0; c < s.ncolours; ++c) {
+    int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;
+    if (!cn) continue;
+    if (m.dim == 2)
+      k_h1_assemble<2><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb, cn,
+                                                           M->d_slot.p, M->d_val.p);
+    else
+      k_h1_assemble<3><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb, cn,
+                                                           M->d_slot.p, M->d_val.p);
+  }
+  OCMP_HIP(hipGetLastError());
+}
+
+void launch_assemble_dg(Matrix *M, const std::vector<ocmp_term> &terms, bool accumulate) {
+  Space &s = *M->space;
+  Ctx *ctx = M->ctx;
+  hipStream_t st = ctx->stream;
+  const Mesh &m = *s.mesh;
+  OCMP_REQUIRE(m.dim == 2, "DG assembly is implemented on triangles");
+  const Block &b = s.blocks[0];
+  OCMP_REQUIRE(b.nloc <= MAXNB, "DG block too large");
+  std::vector<ocmp_term> vol;
+  const ocmp_term *fac = nullptr;
+  for (auto &t : terms) {
+    OCMP_REQUIRE(t.weight_mode == OCMP_WEIGHT_NONE && t.wind_field < 0,
+                 "DG terms take constant coefficients (per-element factors allowed)");
+    if (t.kind == OCMP_BFI_DG_FACET) {
+      OCMP_REQUIRE(!fac, "one facet term per assembly call");
+      OCMP_REQUIRE(t.elem_coef == nullptr, "facet term takes scalar coefficients");
+      fac = &t;
+    } else {
+      vol.push_back(t);
+    }
+  }
+  KArgs A;
+  TableBuilder tb;
+  std::vector<ocmp_field> nofields;
+  build_kargs(s, vol, nofields, A, tb, st);
+  KDG F;
+  std::memset(&F, 0, sizeof F);
+  F.nb = b.nloc;
+  std::vector<double> ft, fw;
+  line_rule(2 * b.order + 2, ft, fw);
+  F.nq = (int)ft.size();
+  {
+    std::vector<double> phi((size_t)3 * F.nq * b.nloc), gr((size_t)3 * F.nq * b.nloc * 2);
+    const int fv[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < F.nq; ++q) {
+        double lam[3] = {0, 0, 0};
+        lam[fv[k][0]] = 1.0 - ft[q];
+        lam[fv[k][1]] = ft[q];
+        eval_basis(b.kind, 2, b.order, lam, &phi[((size_t)k * F.nq + q) * b.nloc],
+                   &gr[((size_t)k * F.nq + q) * b.nloc * 2]);
+      }
+    F.tab_fphi = tb.add(phi);
+    F.tab_fgrad = tb.add(gr);
+    F.tab_fw = tb.add(fw);
+  }
+  if (fac) {
+    F.has_facet = 1;
+    F.eps = fac->eps * fac->coef;
+    F.alpha = fac->alpha;
+    F.wind[0] = fac->wind[0];
+    F.wind[1] = fac->wind[1];
+    F.dmask = fac->dirichlet_mask;
+    F.sipg = fac->sipg;
+    F.upwind = fac->upwind;
+    if (fac->coef != 1.0 && fac->upwind) {
+      // scale the upwind flux with coef as well
+      F.wind[0] *= fac->coef;
+      F.wind[1] *= fac->coef;
+    }
+  }
+  std::string key = "DG" + term_key(terms, nofields);
+  DBuf<double> *tab = cached_table(s, key, tb, st);
+  A.tab = tab->p;
+  KTimer tm(ctx, ctx->cls("assemble_dg"),
+            8.0 * M->nnz + 4.0 * m.nel * 3 * 3 + 4.0 * m.nel * 4 + 8.0 * m.nv * 2 + 4.0 * (M->n + 1));
+  k_dg_assemble<<<nblk(m.nel * b.nloc), 256, 0, st>>>(A, F, m.nel, m.d_coords.p, m.d_elems.p, m.d_elem_nb.p,
+                                                      m.d_elem_nbk.p, M->d_dg_pos.p, M->d_row_ptr.p, M->d_val.p,
+                                                      accumulate ? 1 : 0);
+  OCMP_HIP(hipGetLastError());
+}
+
+void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lterm> &terms,
+                            const std::vector<ocmp_field> &fields, bool accumulate) {
+  Space &s = *sp;
+  Ctx *ctx = s.mesh->ctx;
+  hipStream_t st = ctx->stream;
+  const Mesh &m = *s.mesh;
+  const int d = m.dim;
+  OCMP_REQUIRE((int)terms.size() <= MAXT, "too many terms (max 8)");
+  OCMP_REQUIRE((int)fields.size() <= MAXF, "too many fields (max 8)");
+  OCMP_REQUIRE(s.nloc <= MAXL, "compound element has too many dofs");
+  bool all_l2 = true;
+  for (auto &b : s.blocks) all_l2 &= b.kind == OCMP_L2;
+  KLArgs A;
+  std::memset(&A, 0, sizeof A);
+  TableBuilder tb;
+  A.nterms = (int)terms.size();
+  A.nfields = (int)fields.size();
+  A.nloc = s.nloc;
+  Rule rule = simplex_rule(d, 2 * max_order(s) + 4);
+  A.nq = rule.nq;
+  fill_block_tables(tb, s, rule, A.b);
+  A.tab_w = tb.add(rule.w);
+  A.tab_lam = tb.add(rule.lam);
+  for (size_t i = 0; i < fields.size(); ++i) {
+    const Space &fs = fields[i].space->s;
+    const Block &fb = fs.blocks[fields[i].block];
+    KField &kf = A.f[i];
+    kf.vec = fields[i].vec;
+    kf.edofs = fs.d_edofs.p;
+    kf.ld = fs.nloc;
+    kf.loc_off = fb.loc_off;
+    kf.nl = fb.nloc;
+    std::vector<double> phi((size_t)rule.nq * fb.nloc), gr((size_t)rule.nq * fb.nloc * d);
+    for (int q = 0; q < rule.nq; ++q)
+      eval_basis(fb.kind, d, fb.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * fb.nloc],
+                 &gr[(size_t)q * fb.nloc * d]);
+    kf.tab_phi = tb.add(phi);
+  }
+  bool has_dg = false;
+  for (size_t i = 0; i < terms.size(); ++i) {
+    const ocmp_lterm &t = terms[i];
+    KLTerm &k = A.t[i];
+    k.kind = t.kind;
+    k.test = t.test;
+    k.fn = t.fn;
+    k.field = t.field;
+    for (int c = 0; c < 10; ++c) k.c[c] = t.c[c];
+    k.coef = t.coef;
+    k.weight_mode = t.weight_mode;
+    k.weight_field = t.weight_field;
+    k.eps = t.eps;
+    k.alpha = t.alpha;
+    for (int c = 0; c < 3; ++c) k.wind[c] = t.wind[c];
+    k.dmask = t.dirichlet_mask;
+    OCMP_REQUIRE(t.fn >= OCMP_FN_CONST && t.fn <= OCMP_FN_FIELD, "unknown function id");
+    if (t.kind == OCMP_LFI_DG_DIRICHLET) {
+      OCMP_REQUIRE(d == 2, "DG Dirichlet data on triangles only");
+      OCMP_REQUIRE(t.fn != OCMP_FN_FIELD, "DG Dirichlet data must be a closed-form function");
+      has_dg = true;
+    }
+  }
+  if (has_dg) {
+    const Block &b = s.blocks[0];
+    std::vector<double> ft, fw;
+    line_rule(2 * b.order + 4, ft, fw);
+    A.fnq = (int)ft.size();
+    std::vector<double> phi((size_t)3 * A.fnq * b.nloc), gr((size_t)3 * A.fnq * b.nloc * 2);
+    const int fv[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < A.fnq; ++q) {
+        double lam[3] = {0, 0, 0};
+        lam[fv[k][0]] = 1.0 - ft[q];
+        lam[fv[k][1]] = ft[q];
+        eval_basis(b.kind, 2, b.order, lam, &phi[((size_t)k * A.fnq + q) * b.nloc],
+                   &gr[((size_t)k * A.fnq + q) * b.nloc * 2]);
+      }
+    A.tab_fphi = tb.add(phi);
+    A.tab_fgrad = tb.add(gr);
+    A.tab_fw = tb.add(fw);
+    A.tab_ft = tb.add(ft);
+  }
+  std::string key = "L";
+  for (auto &t : terms) key += "|" + std::to_string(t.kind) + "," + std::to_string(t.test) + "," +
+                               std::to_string(t.fn == OCMP_FN_FIELD ? t.field : -1);
+  for (auto &f : fields) key += "|f" + std::to_string((uintptr_t)f.space) + "," + std::to_string(f.block);
+  DBuf<double> *tab = cached_table(s, key, tb, st);
+  A.tab = tab->p;
+  KTimer tm(ctx, ctx->cls("assemble_linear"));
+  if (all_l2) {
+    // owner-computes: every dof belongs to exactly one element
+    if (d == 2)
+      k_linear<2><<<nblk(m.nel), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, m.d_elem_nb.p, nullptr, 0, m.nel,
+                                               s.d_edofs.p, bvec, accumulate ? 1 : 0);
+    else
+      k_linear<3><<<nblk(m.nel), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, m.d_elem_nb.p, nullptr, 0, m.nel,
+                                               s.d_edofs.p, bvec, accumulate ? 1 : 0);
+  } else {
+    OCMP_REQUIRE(s.ncolours > 0, "space has no colouring (mixed H1/L2 compound)");
+    if (!accumulate) k_fill<<<nblk(s.ndof), 256, 0, st>>>(bvec, s.ndof, 0.0);
+    for (int c = 0; c < s.ncolours; ++c) {
+      int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;
+      if (!cn) continue;
+      if (d == 2)
+        k_linear<2><<<nblk(cn), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, m.d_elem_nb.p, s.d_colour_elems.p, cb,
+                                              cn, s.d_edofs.p, bvec, 1);
+      else
+        k_linear<3><<<nblk(cn), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, m.d_elem_nb.p, s.d_colour_elems.p, cb,
+                                              cn, s.d_edofs.p, bvec, 1);
+    }
+  }
+  OCMP_HIP(hipGetLastError());
+}
+
+void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double *c, uint64_t mask) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const Block &b = s.blocks[block];
+  auto key = std::make_pair(block, mask);
+  Space::Lift *L;
+  auto it = s.lifts.find(key);
+  if (it == s.lifts.end()) {
+    L = new Space::Lift();
+    std::vector<uint8_t> vflag(m.nv, 0), eflag(m.nedge, 0);
+    std::vector<int32_t> fl;
+    const int d = m.dim;
+    // edge lookup by vertex pair (3D)
+    auto find_edge = [&](int32_t a, int32_t bb) -> int64_t {
+      if (a > bb) std::swap(a, bb);
+      int64_t lo = 0, hi = m.nedge;
+      while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (m.edges[2 * mid] < a || (m.edges[2 * mid] == a && m.edges[2 * mid + 1] < bb))
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      return lo;
+    };
+    for (int64_t f = 0; f < m.nfacet; ++f) {
+      int mk = m.f_marker[f];
+      if (m.f_e1[f] >= 0 || mk < 0 || !((mask >> mk) & 1)) continue;
+      const int32_t *fv = &m.facet_verts[f * d];
+      for (int a = 0; a < d; ++a) vflag[fv[a]] = 1;
+      if (d == 2)
+        eflag[f] = 1;
+      else {
+        eflag[find_edge(fv[0], fv[1])] = 1;
+        eflag[find_edge(fv[0], fv[2])] = 1;
+        eflag[find_edge(fv[1], fv[2])] = 1;
+        fl.push_back((int32_t)f);
+      }
+    }
+    std::vector<int32_t> vl, el;
+    for (int64_t v = 0; v < m.nv; ++v)
+      if (vflag[v]) vl.push_back((int32_t)v);
+    for (int64_t e = 0; e < m.nedge; ++e)
+      if (eflag[e]) el.push_back((int32_t)e);
+    L->verts.upload(vl, st);
+    L->edges.upload(el, st);
+    L->faces.upload(fl, st);
+    L->nverts = (int64_t)vl.size();
+    L->nedges = (int64_t)el.size();
+    L->nfaces = (int64_t)fl.size();
+    OCMP_HIP(hipStreamSynchronize(st));
+    s.lifts[key] = L;
+  } else {
+    L = it->second;
+  }
+  // function parameters live in the context's device scalar area (slots 200..209)
+  double *dc = ctx->scal.p + 200;
+  OCMP_HIP(hipMemcpyAsync(dc, c, 10 * sizeof(double), hipMemcpyHostToDevice, st));
+  KTimer tm(ctx, ctx->cls("dirichlet_set"));
+  if (L->nverts)
+    k_dir_verts<<<nblk(L->nverts), 256, 0, st>>>(L->verts.p, L->nverts, m.d_coords.p, m.dim, fn, dc, b.dof_off, x);
+  if (b.order >= 2 && L->nedges)
+    k_dir_edges<<<nblk(L->nedges), 256, 0, st>>>(L->edges.p, L->nedges, s.d_edges.p, m.d_coords.p, m.dim, fn, dc,
+                                                 b.order, b.dof_off, m.nv, x);
+  if (b.order >= 3 && m.dim == 3 && L->nfaces)
+    k_dir_faces<<<nblk(L->nfaces), 256, 0, st>>>(L->faces.p, L->nfaces, b.dof_off + m.nv + 2 * m.nedge, x);
+  OCMP_HIP(hipGetLastError());
+  // the parameter copy reads host memory that the caller may free on return
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace ocmp

@@ -1,0 +1,248 @@
+// Internal declarations of the MI355X assemble+solve library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ocmp.h"
+
+namespace ocmp {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string &msg);
+
+#define OCMP_HIP(call)                                                                       \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw ::ocmp::Error(OCMP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define OCMP_REQUIRE(cond, msg)                                         \
+  do {                                                                  \
+    if (!(cond)) throw ::ocmp::Error(OCMP_ERR_ARG, std::string(msg));   \
+  } while (0)
+
+// Device buffer owned by the library.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count) OCMP_HIP(hipMalloc((void **)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) OCMP_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+// ------------------------------------------------------------------ context
+
+struct TimingRec {
+  int cls;
+  hipEvent_t a, b;
+  double bytes;  // algorithmic bytes of the timed launch(es)
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  bool timing = false;
+  std::vector<std::string> class_names;
+  std::map<std::string, int> class_ids;
+  std::vector<TimingRec> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<int, std::pair<long, double>> acc;  // class -> (launches, ms)
+  std::map<int, double> acc_bytes;               // class -> algorithmic bytes
+  // reduction scratch (device) and pinned host mirror
+  DBuf<double> red;        // partial sums
+  DBuf<double> scal;       // device scalars
+  double *host_scal = nullptr;  // pinned
+  int cls(const std::string &name);
+  void tic(int c, hipEvent_t *a);
+  void toc(int c, hipEvent_t a, double bytes);
+  hipEvent_t get_event();
+  void flush_timing();
+};
+
+// RAII kernel-class timer: records events around launches when timing is on.
+// `bytes` is the algorithmic HBM traffic of the timed launch (what the
+// kernel must move at minimum), accumulated per class for roofline reports.
+struct KTimer {
+  Ctx *ctx;
+  int c;
+  double bytes;
+  hipEvent_t a = nullptr;
+  KTimer(Ctx *x, int cls, double alg_bytes = 0.0) : ctx(x), c(cls), bytes(alg_bytes) {
+    if (ctx->timing) ctx->tic(c, &a);
+  }
+  ~KTimer() {
+    if (ctx->timing && a) ctx->toc(c, a, bytes);
+  }
+};
+
+// ------------------------------------------------------------------ mesh
+
+struct Mesh {
+  Ctx *ctx;
+  int dim;
+  int64_t nv, nel, nedge = 0, nface = 0, nfacet = 0, ninterior = 0, nbnd = 0;
+  std::vector<double> coords;          // nv*dim
+  std::vector<int32_t> elems;          // nel*(dim+1), sorted per element
+  std::vector<int32_t> edges;          // nedge*2
+  std::vector<int32_t> elem_edges;     // nel*nle
+  std::vector<int32_t> faces;          // nface*3 (3D)
+  std::vector<int32_t> elem_faces;     // nel*4 (3D)
+  std::vector<int32_t> elem_facets;    // nel*(dim+1): facet id of local facet k (opposite vertex k)
+  std::vector<int32_t> facet_verts;    // nfacet*dim
+  std::vector<int32_t> f_e0, f_k0, f_e1, f_k1, f_marker;  // per facet
+  // per element-facet neighbour: nb[e*(dim+1)+k] = neighbour element or -(marker+1) on boundary
+  std::vector<int32_t> elem_nb;
+  std::vector<int8_t> elem_nbk;       // neighbour's local facet index
+  DBuf<double> d_coords;
+  DBuf<int32_t> d_elems;
+  DBuf<int32_t> d_elem_nb;
+  DBuf<int8_t> d_elem_nbk;
+};
+
+// ------------------------------------------------------------------ space
+
+struct Block {
+  int kind, order, dim;
+  int64_t dof_off;  // offset in compound numbering
+  int64_t n1;       // dofs in this block
+  int loc_off;      // offset in compound element dof list
+  int nloc;         // local dofs of this block
+  uint64_t dirichlet_mask;
+  int comp;         // index of the ocmp_fe_desc it came from
+};
+
+struct Space {
+  Mesh *mesh;
+  bool dgjumps = false;
+  std::vector<Block> blocks;
+  int64_t ndof = 0;
+  int nloc = 0;
+  std::vector<int32_t> edofs;   // nel*nloc (compound numbering)
+  std::vector<uint8_t> free;    // ndof
+  DBuf<int32_t> d_edofs;
+  DBuf<uint8_t> d_free;
+  // Dirichlet lift support per block: dirichlet vertex/edge/face lists with marker masks
+  // (computed lazily per (block, marker mask))
+  struct Lift {
+    DBuf<int32_t> verts;   // vertex ids
+    DBuf<int32_t> edges;   // edge ids
+    DBuf<int32_t> faces;   // face ids (3D)
+    int64_t nverts = 0, nedges = 0, nfaces = 0;
+  };
+  std::map<std::pair<int, uint64_t>, Lift *> lifts;
+  DBuf<int32_t> d_edges;   // mesh edges (for lifts)
+  // deterministic element schedule for H1 scatter: elements sharing a vertex get different colours
+  int ncolours = 0;
+  std::vector<int64_t> colour_off;   // ncolours+1
+  DBuf<int32_t> d_colour_elems;      // elements sorted by colour
+  // cached kernel tables keyed by a signature
+  std::map<std::string, DBuf<double> *> tables;
+  ~Space() {
+    for (auto &kv : tables) delete kv.second;
+    for (auto &kv : lifts) delete kv.second;
+  }
+};
+
+// ------------------------------------------------------------------ reference tables
+
+struct Rule {
+  int dim = 0, nq = 0;
+  std::vector<double> lam;  // nq*(dim+1)
+  std::vector<double> w;    // nq
+};
+Rule simplex_rule(int dim, int degree);
+void line_rule(int degree, std::vector<double> &t, std::vector<double> &w);
+int nloc1(int kind, int dim, int order);
+// values phi[nl] and reference gradients g[nl*dim] at one barycentric point
+void eval_basis(int kind, int dim, int order, const double *lam, double *phi, double *g);
+
+// ------------------------------------------------------------------ matrix
+
+struct Matrix {
+  Space *space;
+  Ctx *ctx;
+  int64_t n = 0, nnz = 0;
+  bool dg = false;
+  std::vector<int32_t> row_ptr, col;
+  DBuf<int32_t> d_row_ptr, d_col;
+  DBuf<double> d_val;
+  // H1 scatter schedule: element -> CSR slot map (colours live in the space)
+  DBuf<int32_t> d_slot;              // nel*nloc*nloc
+  // DG: per element-facet block position in the block row (0..3), diag block position
+  DBuf<int8_t> d_dg_pos;             // nel*(dim+2): pos of diag block, then of each facet neighbour (-1 none)
+  DBuf<int32_t> d_dg_bcol;           // nel*4 block columns (-1 padded), sorted
+  int dg_nb = 0;                     // local dofs per element (block size)
+  // SpMV launch parameters
+  int spmv_group = 16;
+  bool assembled = false;
+  // mixed precision copy
+  DBuf<float> d_val32;
+};
+
+struct Precond {
+  Matrix *A;
+  int kind, precision;
+  DBuf<double> dinv;     // Jacobi: n (0 on Dirichlet dofs)
+  DBuf<float> dinv32;
+  DBuf<double> binv;     // block Jacobi: nel*nb*nb
+  DBuf<float> binv32;
+  // right-preconditioned DG operator: Ahat = A * blockdiag^{-1}, off-diagonal blocks only
+  DBuf<double> ahat;     // nel*4*nb*nb (padded to 4 block slots: diag slot unused)
+  DBuf<float> ahat32;
+  bool ready = false;
+};
+
+// ------------------------------------------------------------------ kernels (launchers)
+
+void launch_assemble_h1(Matrix *A, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
+                        bool accumulate);
+void launch_assemble_dg(Matrix *A, const std::vector<ocmp_term> &terms, bool accumulate);
+void launch_assemble_linear(Space *s, double *b, const std::vector<ocmp_lterm> &terms,
+                            const std::vector<ocmp_field> &fields, bool accumulate);
+void launch_dirichlet_set(Space *s, int block, double *x, int fn, const double *c, uint64_t mask);
+void spmv_f64(Matrix *A, const double *x, double *y);
+void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
+void precond_update(Precond *P);
+
+// greedy colouring + slot map (host)
+void build_h1_schedule(Matrix *A);
+void build_colouring(Space &s);
+void build_pattern(Matrix *A);
+
+}  // namespace ocmp
+
+struct ocmp_ctx { ocmp::Ctx c; };
+struct ocmp_mesh { ocmp::Mesh m; };
+struct ocmp_space { ocmp::Space s; };
+struct ocmp_matrix { ocmp::Matrix A; };
+struct ocmp_precond { ocmp::Precond P; };

@@ -1,0 +1,404 @@
+"""Parity of the HIP assemble+solve path with the CPU oracle, through the C-ABI.
+
+Matrices: identical CSR pattern, values within relative Frobenius 1e-13.
+Solutions: iterative GPU solves vs the oracle's direct free-dof solve.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem
+from opencmp_amd import _lib as L
+from opencmp_amd.mesh import unit_square, unit_cube
+from opencmp_amd.io import load_mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'mesh_files')
+PI = np.pi
+TOL_FROB = 1e-13
+
+
+def say(*a):
+    print('[gpu-test]', *a, flush=True)
+    sys.stdout.flush()
+
+
+def core():
+    from opencmp_amd import core as C
+    return C
+
+
+def rel_frob(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def gpu_space(ctx, host_mesh, comps, dgjumps=False):
+    C = core()
+    gm = C.Mesh(host_mesh, ctx)
+    return gm, C.FESpace(gm, comps, dgjumps=dgjumps)
+
+
+def oracle_space(host_mesh, comps, dgjumps=False):
+    t = fem.Topology(host_mesh)
+    sp = []
+    for c in comps:
+        C = core()
+        if isinstance(c, C.H1):
+            mask = host_mesh.marker_mask(c.dirichlet) if c.dirichlet else 0
+            sp.append(fem.Space(t, 'H1', c.order, ncomp=c.ncomp, dirichlet_mask=mask))
+        else:
+            sp.append(fem.Space(t, 'L2', c.order))
+    return t, fem.Compound(sp, dgjumps=dgjumps)
+
+
+MESHES = {
+    'square5': lambda: unit_square(5),
+    'cube2': lambda: unit_cube(2),
+    'channel': lambda: load_mesh(os.path.join(GOLD, 'channel_3bcs.vol')),
+}
+
+
+@pytest.mark.parametrize('mname', list(MESHES))
+def test_topology_dofs_pattern(gpu_ctx, mname):
+    C = core()
+    hm = MESHES[mname]()
+    say('topology/dofs/pattern', mname)
+    cases = [[C.H1(1, 'left')], [C.H1(2, 'left|right')], [C.H1(3, 'left')],
+             [C.H1(2, 'left', ncomp=hm.dim), C.H1(1)]]
+    if hm.dim == 2:
+        cases.append([C.L2(2)])
+    for comps in cases:
+        dg = isinstance(comps[0], C.L2)
+        gm, gs = gpu_space(gpu_ctx, hm, comps, dgjumps=dg)
+        t, cs = oracle_space(hm, comps, dgjumps=dg)
+        assert (gm.nv, gm.nedge, gm.ne, gm.nfacet) == (t.nv, t.nedge, t.nel, t.nfacet)
+        assert gm.ninterior == int(np.sum(t.f_e1 >= 0))
+        assert gs.ndof == cs.ndof
+        assert (gs.element_dofs() == cs.edofs).all()
+        assert (gs.FreeDofs() == cs.freedofs).all()
+        a = C.BilinearForm(gs)
+        rp, col = a.pattern()
+        orp, ocol, _ = fem.build_pattern(cs)
+        assert (rp == orp).all() and (col == ocol).all()
+        if mname == 'channel':
+            assert (gm.nedge, gm.ne, gm.nfacet) == (1137, 726, 1137)
+
+
+def _h1_terms_2d():
+    # (gpu Vol, oracle dict)
+    w = (0.3, -0.7)
+    return [
+        (dict(kind='gradgrad', coef=1.3), dict(kind='grad_grad', coef=1.3)),
+        (dict(kind='mass', coef=0.7), dict(kind='mass', coef=0.7)),
+        (dict(kind='conv', coef=-1.0, wind=w + (0.0,)), dict(kind='conv', coef=-1.0, wind=np.array(w))),
+    ]
+
+
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_h1_matrix_values(gpu_ctx, dim, p):
+    C = core()
+    say('h1 values', dim, p)
+    hm = unit_square(6) if dim == 2 else unit_cube(3)
+    comps = [C.H1(p, 'left')]
+    gm, gs = gpu_space(gpu_ctx, hm, comps)
+    t, cs = oracle_space(hm, comps)
+    rng = np.random.default_rng(p)
+    kappa = np.exp(rng.standard_normal(t.nel))
+    w = (0.3, -0.7, 0.2)[:dim]
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0, coef=1.3, elem_coef=torch.tensor(kappa))
+    a += C.Vol('mass', 0, 0, coef=0.7)
+    a += C.Vol('conv', 0, 0, coef=-1.0, wind=w + (0.0,) * (3 - dim))
+    a.Assemble()
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.3 * kappa),
+                            dict(kind='mass', trial=0, test=0, coef=0.7),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=np.array(w))])
+    ref = A.matrix()
+    v = a.values()
+    assert rel_frob(v, ref.data) <= TOL_FROB, rel_frob(v, ref.data)
+    # bitwise repeatable
+    a.Assemble()
+    assert (a.values() == v).all()
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_taylor_hood_coupling_values(gpu_ctx, dim):
+    """Stokes-type compound: nu grad u:grad v - div(u) q - div(v) p - 1e-10 p q (opencmp/models/stokes.py:43-80)."""
+    C = core()
+    say('taylor-hood', dim)
+    hm = unit_square(4) if dim == 2 else unit_cube(2)
+    comps = [C.H1(2, 'left|bottom', ncomp=dim), C.H1(1)]
+    gm, gs = gpu_space(gpu_ctx, hm, comps)
+    t, cs = oracle_space(hm, comps)
+    a = C.BilinearForm(gs)
+    ub = gs.comp_blocks[0]
+    pb = gs.comp_blocks[1][0]
+    for k in range(dim):
+        a += C.Vol('gradgrad', ub[k], ub[k], coef=0.1)
+        a += C.Vol('dtrial', ub[k], pb, coef=-1.0, dir=k)
+        a += C.Vol('dtest', pb, ub[k], coef=-1.0, dir=k)
+    a += C.Vol('mass', pb, pb, coef=-1e-10)
+    a.Assemble()
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=0.1),
+                            dict(kind='dtrial', trial=0, test=1, coef=-1.0, dir=-1),
+                            dict(kind='dtest', trial=1, test=0, coef=-1.0, dir=-1),
+                            dict(kind='mass', trial=1, test=1, coef=-1e-10)])
+    ref = A.matrix()
+    assert rel_frob(a.values(), ref.data) <= TOL_FROB
+
+
+def test_field_weighted_and_wind_field_terms(gpu_ctx):
+    """phi-weighted stiffness/mass (StokesDIM, opencmp/models/stokes_dim.py:42-48) and an Oseen
+    wind given as a P2 vector grid function (opencmp/models/ins.py:191-195)."""
+    C = core()
+    say('field terms')
+    hm = unit_square(5)
+    gm, gs = gpu_space(gpu_ctx, hm, [C.H1(2, 'left')])
+    gmv, gv = None, C.FESpace(gm, [C.H1(2, ncomp=2)])
+    t, cs = oracle_space(hm, [C.H1(2, 'left')])
+    _, csv = oracle_space(hm, [C.H1(2, ncomp=2)])
+    rng = np.random.default_rng(3)
+    phi_vec = rng.random(gs.ndof)
+    wind_vec = rng.standard_normal(gv.ndof)
+    phi = C.GridFunction(gs, torch.tensor(phi_vec, device=gpu_ctx.torch_device))
+    wind = C.GridFunction(gv, torch.tensor(wind_vec, device=gpu_ctx.torch_device))
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0, coef=0.5, weight_gf=(phi, 0, L.WEIGHT_FIELD))
+    a += C.Vol('mass', 0, 0, coef=3.0, weight_gf=(phi, 0, L.WEIGHT_ONE_MINUS_FIELD))
+    a += C.Vol('conv', 0, 0, coef=-1.0, wind_gf=(wind, gv.comp_blocks[0]))
+    a.Assemble()
+    ophi = fem.GridFunctionEval(cs.spaces[0], phi_vec)
+    owind = fem.GridFunctionEval(csv.spaces[0], wind_vec)
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=('field', ophi, 1, 0.5)),
+                            dict(kind='mass', trial=0, test=0, coef=('field', ophi, 2, 3.0)),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=owind)])
+    ref = A.matrix()
+    assert rel_frob(a.values(), ref.data) <= TOL_FROB
+
+
+def _dg_forms(C, gs, hm, eps, w, alpha):
+    allm = hm.marker_mask(hm.bnd_names)
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0, coef=eps)
+    a += C.Vol('conv', 0, 0, coef=-1.0, wind=(w[0], w[1], 0.0))
+    a += C.DGFacet(0, eps=eps, alpha=alpha, wind=(w[0], w[1]), dirichlet=allm)
+    return a, allm
+
+
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_dg_matrix_values(gpu_ctx, p):
+    """DG SIPG + upwind transport form (opencmp/models/multi_component_ins.py:173-301)."""
+    C = core()
+    say('dg values', p)
+    hm = unit_square(6)
+    eps, w, alpha = 1e-2, (np.cos(0.3), np.sin(0.3)), 10.0 * p * p
+    gm, gs = gpu_space(gpu_ctx, hm, [C.L2(p)], dgjumps=True)
+    t, cs = oracle_space(hm, [C.L2(p)], dgjumps=True)
+    a, allm = _dg_forms(C, gs, hm, eps, w, alpha)
+    a.Assemble()
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=np.array(w))])
+    fem.assemble_dg_facets(A, 0, eps, alpha, np.array(w), allm, 0)
+    ref = A.matrix()
+    v = a.values()
+    assert rel_frob(v, ref.data) <= TOL_FROB, rel_frob(v, ref.data)
+    a.Assemble()
+    assert (a.values() == v).all()
+
+
+def test_linear_forms(gpu_ctx):
+    C = core()
+    say('linear forms')
+    # H1 P2 2D, sin source
+    hm = unit_square(6)
+    gm, gs = gpu_space(gpu_ctx, hm, [C.H1(2, 'left')])
+    t, cs = oracle_space(hm, [C.H1(2, 'left')])
+    f = C.LinearForm(gs)
+    f += C.Source(0, (L.FN_SINPROD, (2 * PI * PI,)))
+    f.Assemble()
+    ref = fem.assemble_source(cs, 0, lambda x, y: 2 * PI * PI * np.sin(PI * x) * np.sin(PI * y))
+    assert np.abs(f.vec.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    # H1 P3 3D, quadratic source
+    hm3 = unit_cube(2)
+    gm3, gs3 = gpu_space(gpu_ctx, hm3, [C.H1(3)])
+    t3, cs3 = oracle_space(hm3, [C.H1(3)])
+    cc = [1.0, 0.5, -0.2, 0.3, 0.1, 0.2, -0.4, 0.7, 0.05, -0.3]
+    f3 = C.LinearForm(gs3)
+    f3 += C.Source(0, (L.FN_POLY2, cc))
+    f3.Assemble()
+    pf = lambda x, y, z: (cc[0] + cc[1] * x + cc[2] * y + cc[3] * z + cc[4] * x * x + cc[5] * y * y +
+                          cc[6] * z * z + cc[7] * x * y + cc[8] * x * z + cc[9] * y * z)
+    ref3 = fem.assemble_source(cs3, 0, pf)
+    assert np.abs(f3.vec.cpu().numpy() - ref3).max() <= 1e-13 * np.abs(ref3).max()
+    # DG source + Dirichlet data
+    eps, w, alpha = 1e-2, (np.cos(0.3), np.sin(0.3)), 40.0
+    gmd, gsd = gpu_space(gpu_ctx, hm, [C.L2(2)], dgjumps=True)
+    td, csd = oracle_space(hm, [C.L2(2)], dgjumps=True)
+    allm = hm.marker_mask(hm.bnd_names)
+    g2 = [1.0, 0.0, -1.0, 1.0, 1.0, 0.0]   # 1 - y + x^2 + xy
+    fd = C.LinearForm(gsd)
+    fd += C.Source(0, (L.FN_CONST, (1.0,)))
+    fd += C.DGDirichletData(0, eps, alpha, w, allm, fn=(L.FN_POLY2, g2))
+    fd.Assemble()
+    u = lambda x, y: 1 - y + x * x + x * y
+    refd = fem.assemble_source(csd, 0, 1.0) + fem.assemble_dg_dirichlet_rhs(csd, 0, eps, alpha, np.array(w), u, allm)
+    assert np.abs(fd.vec.cpu().numpy() - refd).max() <= 1e-13 * np.abs(refd).max()
+
+
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 3), (3, 2)])
+def test_dirichlet_lift(gpu_ctx, dim, p):
+    C = core()
+    say('dirichlet lift', dim, p)
+    hm = unit_square(4) if dim == 2 else unit_cube(2)
+    names = '|'.join(hm.bnd_names)
+    gm, gs = gpu_space(gpu_ctx, hm, [C.H1(p, names)])
+    t, cs = oracle_space(hm, [C.H1(p, names)])
+    if dim == 2:
+        cc = [1.0, 1.0, -2.0, 1.0, 0.5, -1.0]
+        u = lambda x, y: 1 + x - 2 * y + x * x + 0.5 * x * y - y * y
+    else:
+        cc = [1.0, 0, 0, 0, 2.0, 0, -1.0, 1.0, 0, 0]
+        u = lambda x, y, z: 1 + 2 * x * x - z * z + x * y
+    gf = C.GridFunction(gs)
+    gf.Set((L.FN_POLY2, cc), names)
+    ref = fem.h1_lift(cs.spaces[0], u, hm.marker_mask(names))
+    got = gf.numpy()
+    assert np.abs(got - ref).max() <= 1e-12
+    # lift + solve reproduces the quadratic exactly
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0)
+    a.Assemble()
+    f = C.LinearForm(gs)
+    lap = (2 - 2) if dim == 2 else (4 - 2)
+    f += C.Source(0, (L.FN_CONST, (-lap,)))
+    f.Assemble()
+    pre = C.Preconditioner(a, 'local')
+    pre.Update()
+    r = C.solve(a, f.vec, gf.vec, pre, 'CG', tol=1e-13)
+    assert r.converged
+    assert fem.l2_error(cs.spaces[0], gf.numpy(), u) < 1e-10
+
+
+def _cfg1(C, gpu_ctx, n):
+    hm = unit_square(n)
+    names = '|'.join(hm.bnd_names)
+    gm, gs = gpu_space(gpu_ctx, hm, [C.H1(2, names)])
+    t, cs = oracle_space(hm, [C.H1(2, names)])
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0)
+    a.Assemble()
+    f = C.LinearForm(gs)
+    f += C.Source(0, (L.FN_SINPROD, (2 * PI * PI,)))
+    f.Assemble()
+    return hm, gs, t, cs, a, f
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_config1_poisson_cg(gpu_ctx, precision):
+    """Config 1: Poisson H1 P2 on the 32x32 split square, CG + Jacobi to 1e-10."""
+    C = core()
+    say('config1 cg', precision)
+    hm, gs, t, cs, a, f = _cfg1(C, gpu_ctx, 32)
+    assert gs.ndof == 4225 and a.nnz == 47617
+    pre = C.Preconditioner(a, 'local', precision=precision)
+    pre.Update()
+    x = C.GridFunction(gs)
+    r = C.solve(a, f.vec, x.vec, pre, 'CG', tol=1e-10, precision=precision)
+    say('  iters', r.iters, 'outer', r.outer, 'res', r.rel_residual)
+    assert r.converged and r.rel_residual <= 1e-10
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    K = A.matrix()
+    b = fem.assemble_source(cs, 0, lambda x, y: 2 * PI * PI * np.sin(PI * x) * np.sin(PI * y))
+    xd = fem.solve_free(K, b, np.zeros(cs.ndof), cs.freedofs)
+    xg = x.numpy()
+    # true residual of the GPU solution measured by the oracle
+    rr = (b - K @ xg)[cs.freedofs]
+    assert np.linalg.norm(rr) / np.linalg.norm(b[cs.freedofs]) <= 1.01e-10
+    assert np.abs(xg - xd).max() <= 1e-7 * np.abs(xd).max()
+    e_g = fem.l2_error(cs.spaces[0], xg, lambda x, y: np.sin(PI * x) * np.sin(PI * y))
+    e_o = fem.l2_error(cs.spaces[0], xd, lambda x, y: np.sin(PI * x) * np.sin(PI * y))
+    assert abs(e_g - e_o) <= 1e-6 * e_o
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_dg_convection_diffusion_gmres(gpu_ctx, precision):
+    """Config 2 form at 16x16: GMRES(50) + block Jacobi to 1e-10 vs the oracle's direct solve."""
+    C = core()
+    say('dg gmres', precision)
+    hm = unit_square(16)
+    eps, w, alpha = 1e-3, (np.cos(0.3), np.sin(0.3)), 40.0
+    gm, gs = gpu_space(gpu_ctx, hm, [C.L2(2)], dgjumps=True)
+    t, cs = oracle_space(hm, [C.L2(2)], dgjumps=True)
+    a, allm = _dg_forms(C, gs, hm, eps, w, alpha)
+    a.Assemble()
+    f = C.LinearForm(gs)
+    f += C.Source(0, (L.FN_CONST, (1.0,)))
+    f.Assemble()
+    pre = C.Preconditioner(a, 'local', precision=precision)
+    pre.Update()
+    x = C.GridFunction(gs)
+    r = C.solve(a, f.vec, x.vec, pre, 'GMRes', tol=1e-10, precision=precision, restart=50)
+    say('  iters', r.iters, 'outer', r.outer, 'res', r.rel_residual)
+    assert r.converged
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=np.array(w))])
+    fem.assemble_dg_facets(A, 0, eps, alpha, np.array(w), allm, 0)
+    K = A.matrix()
+    b = fem.assemble_source(cs, 0, 1.0)
+    xd = fem.solve_free(K, b, np.zeros(cs.ndof), cs.freedofs)
+    xg = x.numpy()
+    assert np.linalg.norm(b - K @ xg) / np.linalg.norm(b) <= 1.01e-10
+    assert np.abs(xg - xd).max() <= 1e-6 * np.abs(xd).max()
+
+
+def test_richardson_and_determinism(gpu_ctx):
+    C = core()
+    say('richardson + determinism')
+    hm, gs, t, cs, a, f = _cfg1(C, gpu_ctx, 8)
+    pre = C.Preconditioner(a, 'local')
+    pre.Update()
+    x1 = C.GridFunction(gs)
+    r1 = C.solve(a, f.vec, x1.vec, pre, 'CG', tol=1e-12)
+    x2 = C.GridFunction(gs)
+    r2 = C.solve(a, f.vec, x2.vec, pre, 'CG', tol=1e-12)
+    assert r1.iters == r2.iters and torch.equal(x1.vec, x2.vec)
+    x3 = C.GridFunction(gs)
+    r3 = C.solve(a, f.vec, x3.vec, pre, 'Richardson', tol=1e-3, maxit=20000)
+    assert r3.converged
+
+
+def test_3d_p3_kappa_mixed(gpu_ctx):
+    """Config 3 form at 4^3: H1 P3, kappa = exp(N(0,1)) per element, f = 1; mixed vs FP64 solution."""
+    C = core()
+    say('3d p3 kappa mixed')
+    hm = unit_cube(4)
+    names = '|'.join(hm.bnd_names)
+    gm, gs = gpu_space(gpu_ctx, hm, [C.H1(3, names)])
+    kappa = np.exp(np.random.default_rng(0).standard_normal(hm.nel))
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0, elem_coef=torch.tensor(kappa))
+    a.Assemble()
+    f = C.LinearForm(gs)
+    f += C.Source(0, (L.FN_CONST, (1.0,)))
+    f.Assemble()
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        pre = C.Preconditioner(a, 'local', precision=prec)
+        pre.Update()
+        x = C.GridFunction(gs)
+        r = C.solve(a, f.vec, x.vec, pre, 'CG', tol=1e-10, precision=prec)
+        say('  ', prec, r)
+        assert r.converged
+        res[prec] = x.numpy()
+    d = np.abs(res['fp64'] - res['mixed']).max() / np.abs(res['fp64']).max()
+    assert d < 1e-8

@@ -1,0 +1,283 @@
+"""Pins the CPU oracle (oracle/fem.py) against closed forms, invariants,
+brute force and the reference's own fixtures. CPU only."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import fem
+from opencmp_amd.mesh import unit_square, unit_cube, Mesh
+from opencmp_amd.io import load_mesh
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'mesh_files')
+PI = np.pi
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_quadrature_exact_on_monomials(dim):
+    for deg in range(0, 9):
+        lam, w = fem.simplex_rule(dim, deg)
+        assert abs(w.sum() - (0.5 if dim == 2 else 1.0 / 6.0)) < 1e-15
+        rng = np.random.default_rng(deg)
+        for _ in range(6):
+            e = rng.multinomial(deg, [1.0 / (dim + 1)] * (dim + 1))
+            v = np.sum(w * np.prod(lam ** e, axis=1))
+            assert abs(v - fem.simplex_monomial_integral(list(e))) < 1e-15
+
+
+def test_line_rule():
+    for deg in range(0, 12):
+        t, w = fem.line_rule(deg)
+        assert abs(np.sum(w * t ** deg) - 1.0 / (deg + 1)) < 1e-15
+
+
+@pytest.mark.parametrize('p', [0, 1, 2, 3])
+def test_dubiner_orthonormal_and_complete(p):
+    lam, w = fem.simplex_rule(2, 2 * p + 2)
+    phi, g = fem.l2_eval(2, p, lam)
+    M = np.einsum('q,qi,qj->ij', w, phi, phi)
+    assert np.abs(M - np.eye(M.shape[0])).max() < 1e-13
+    # every monomial x^a y^b, a+b<=p, is in the span
+    x, y = lam[:, 1], lam[:, 2]
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            f = x ** a * y ** b
+            c = np.einsum('q,qi,q->i', w, phi, f)
+            assert np.abs(phi @ c - f).max() < 1e-12
+    # gradients against central differences of the polynomial values
+    h = 1e-5
+    lp, lm = lam.copy(), lam.copy()
+    lp[:, 1] += h
+    lp[:, 0] -= h
+    lm[:, 1] -= h
+    lm[:, 0] += h
+    pp, _ = fem.l2_eval(2, p, lp)
+    pm, _ = fem.l2_eval(2, p, lm)
+    assert np.abs((pp - pm) / (2 * h) - g[:, :, 0]).max() < 1e-6
+
+
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_h1_basis_vertex_partition_and_gradients(dim, p):
+    lam, _ = fem.simplex_rule(dim, 4)
+    phi, dl = fem.h1_eval(dim, p, lam)
+    assert np.abs(phi[:, :dim + 1].sum(1) - 1.0).max() < 1e-15
+    # all non-vertex functions vanish at the vertices
+    V = np.eye(dim + 1)
+    pv, _ = fem.h1_eval(dim, p, V)
+    if pv.shape[1] > dim + 1:
+        assert np.abs(pv[:, dim + 1:]).max() == 0.0
+    assert np.allclose(pv[:, :dim + 1], np.eye(dim + 1))
+    # d/dlambda by finite differences
+    h = 1e-7
+    for k in range(dim + 1):
+        l2 = lam.copy()
+        l2[:, k] += h
+        p2, _ = fem.h1_eval(dim, p, l2)
+        assert np.abs((p2 - phi) / h - dl[:, :, k]).max() < 1e-5
+
+
+def test_p1_stiffness_closed_form():
+    # single random triangle: K = (1/4A)(b_i b_j + c_i c_j)
+    rng = np.random.default_rng(1)
+    X = rng.random((3, 2))
+    m = Mesh(X, np.array([[0, 1, 2]], np.int32), np.array([[0, 1], [1, 2], [0, 2]], np.int32),
+             np.zeros(3, np.int32), ['b'])
+    t = fem.Topology(m)
+    cs = fem.Compound([fem.Space(t, 'H1', 1)])
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    K = A.matrix().toarray()
+    Xs = X[np.sort([0, 1, 2])]
+    b = np.array([Xs[1, 1] - Xs[2, 1], Xs[2, 1] - Xs[0, 1], Xs[0, 1] - Xs[1, 1]])
+    c = np.array([Xs[2, 0] - Xs[1, 0], Xs[0, 0] - Xs[2, 0], Xs[1, 0] - Xs[0, 0]])
+    area = 0.5 * abs(b[0] * c[1] - b[1] * c[0])
+    Kc = (np.outer(b, b) + np.outer(c, c)) / (4 * area)
+    assert np.abs(K - Kc).max() < 1e-13 * np.abs(Kc).max()
+
+
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 2), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_stiffness_mass_invariants(dim, p):
+    m = unit_square(3) if dim == 2 else unit_cube(2)
+    t = fem.Topology(m)
+    s = fem.Space(t, 'H1', p)
+    cs = fem.Compound([s])
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    K = A.matrix()
+    one = np.zeros(cs.ndof)
+    one[:t.nv] = 1.0         # the constant function
+    assert np.abs(K @ one).max() < 1e-12
+    assert abs(K - K.T).max() < 1e-13
+    Mm = fem.Assembler(cs)
+    fem.assemble_volume(Mm, [dict(kind='mass', trial=0, test=0, coef=1.0)])
+    M = Mm.matrix()
+    assert abs(one @ (M @ one) - 1.0) < 1e-13    # |Omega| = 1
+    ev = np.linalg.eigvalsh(K.toarray())
+    assert ev.min() > -1e-10
+
+
+def test_pattern_bruteforce():
+    m = unit_square(2)
+    t = fem.Topology(m)
+    for dg in (False, True):
+        if dg:
+            cs = fem.Compound([fem.Space(t, 'L2', 1)], dgjumps=True)
+        else:
+            cs = fem.Compound([fem.Space(t, 'H1', 2), fem.Space(t, 'H1', 1)])
+        rp, col, _ = fem.build_pattern(cs)
+        D = np.zeros((cs.ndof, cs.ndof), bool)
+        for e in range(t.nel):
+            d = cs.edofs[e]
+            D[np.ix_(d, d)] = True
+        if dg:
+            for f in range(t.nfacet):
+                if t.f_e1[f] >= 0:
+                    a, b = cs.edofs[t.f_e0[f]], cs.edofs[t.f_e1[f]]
+                    D[np.ix_(a, b)] = True
+                    D[np.ix_(b, a)] = True
+        S = sp.csr_matrix((np.ones(len(col)), col, rp), shape=D.shape).toarray() > 0
+        assert (S == D).all()
+
+
+def test_config1_counts_and_rates():
+    """cfg1 sizes (4225 dofs, 47617 nnz) and manufactured-solution rates (P2: L2 3, H1 2)."""
+    errs = []
+    for n in (8, 16, 32):
+        m = unit_square(n)
+        t = fem.Topology(m)
+        s = fem.Space(t, 'H1', 2, dirichlet_mask=m.marker_mask(m.bnd_names))
+        cs = fem.Compound([s])
+        A = fem.Assembler(cs)
+        fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+        K = A.matrix()
+        b = fem.assemble_source(cs, 0, lambda x, y: 2 * PI * PI * np.sin(PI * x) * np.sin(PI * y))
+        x = fem.solve_free(K, b, np.zeros(cs.ndof), cs.freedofs)
+        e = fem.l2_error(s, x, lambda x, y: np.sin(PI * x) * np.sin(PI * y),
+                         grad_exact=[lambda x, y: PI * np.cos(PI * x) * np.sin(PI * y),
+                                     lambda x, y: PI * np.sin(PI * x) * np.cos(PI * y)])
+        errs.append(e)
+        if n == 32:
+            assert cs.ndof == 4225 and A.nnz == 47617
+    l2 = np.log2(errs[1][0] / errs[2][0])
+    h1 = np.log2(errs[1][1] / errs[2][1])
+    assert 2.85 < l2 < 3.15 and 1.9 < h1 < 2.1
+
+
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 3), (3, 2)])
+def test_polynomial_reproduction_with_lift(dim, p):
+    """u in P_p with Dirichlet data u on the whole boundary: Galerkin returns u exactly."""
+    if dim == 2:
+        m = unit_square(4)
+        if p == 2:
+            u = lambda x, y: 1 + x - 2 * y + x * x + 0.5 * x * y - y * y
+            f = lambda x, y: -(2 - 2) + 0 * x
+        else:
+            u = lambda x, y: x ** 3 - 2 * x * y * y + y + 1
+            f = lambda x, y: -(6 * x - 4 * x)
+    else:
+        m = unit_cube(2)
+        u = lambda x, y, z: 1 + x * y - z * z + 2 * x * x
+        f = lambda x, y, z: -(4 - 2) + 0 * x
+    t = fem.Topology(m)
+    mask = m.marker_mask(m.bnd_names)
+    s = fem.Space(t, 'H1', p, dirichlet_mask=mask)
+    cs = fem.Compound([s])
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    K = A.matrix()
+    b = fem.assemble_source(cs, 0, f)
+    x0 = fem.h1_lift(s, u, mask)
+    x = fem.solve_free(K, b, x0, cs.freedofs)
+    assert fem.l2_error(s, x, u) < 1e-12
+
+
+def test_dg_consistency_global_quadratic():
+    eps = 1e-2
+    w = np.array([np.cos(0.3), np.sin(0.3)])
+    u = lambda x, y: x * x + x * y - y + 1
+    f = lambda x, y: -2 * eps + w[0] * (2 * x + y) + w[1] * (x - 1)
+    m = unit_square(6)
+    t = fem.Topology(m)
+    s = fem.Space(t, 'L2', 2)
+    cs = fem.Compound([s], dgjumps=True)
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
+    allm = m.marker_mask(m.bnd_names)
+    fem.assemble_dg_facets(A, 0, eps, 40.0, w, allm, 0)
+    K = A.matrix()
+    b = fem.assemble_source(cs, 0, f) + fem.assemble_dg_dirichlet_rhs(cs, 0, eps, 40.0, w, u, allm)
+    x = fem.solve_free(K, b, np.zeros(cs.ndof), cs.freedofs)
+    assert fem.l2_error(s, x, u) < 1e-11
+    assert A.nnz == 36 * (t.nel + 2 * int(np.sum(t.f_e1 >= 0)))
+    # the SIPG part alone is symmetric
+    A2 = fem.Assembler(cs)
+    fem.assemble_dg_facets(A2, 0, eps, 40.0, w, allm, 0, upwind=False)
+    S = A2.matrix()
+    assert abs(S - S.T).max() < 1e-14
+
+
+def test_iterative_solvers_match_direct():
+    m = unit_square(8)
+    t = fem.Topology(m)
+    s = fem.Space(t, 'H1', 2, dirichlet_mask=m.marker_mask(m.bnd_names))
+    cs = fem.Compound([s])
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    K = A.matrix()
+    b = fem.assemble_source(cs, 0, 1.0)
+    xd = fem.solve_free(K, b, np.zeros(cs.ndof), cs.freedofs)
+    xc, it, res = fem.pcg_jacobi(K, b, np.zeros(cs.ndof), cs.freedofs, 1e-12)
+    assert res < 1e-12 and np.abs(xc - xd).max() < 1e-9 * np.abs(xd).max()
+    d = K.diagonal()
+    M = lambda v: np.where(cs.freedofs, v / d, 0.0)
+    xg, it, res = fem.gmres_right(K, b, np.zeros(cs.ndof), M, 1e-12, m=20, free=cs.freedofs)
+    assert res < 1e-12 and np.abs(xg - xd).max() < 1e-9 * np.abs(xd).max()
+
+
+def test_reference_mesh_topology_counts():
+    """Known answers of pytests/helpers/test_io.py:92-132."""
+    m = load_mesh(os.path.join(GOLD, 'channel_3bcs.vol'))
+    t = fem.Topology(m)
+    assert m.dim == 2 and set(m.bnd_names) <= {'wall', 'outlet', 'inlet'}
+    assert t.nedge == 1137 and t.nel == 726 and t.nfacet == 1137
+    m = load_mesh(os.path.join(GOLD, 'square.msh'))
+    t = fem.Topology(m)
+    assert m.dim == 2 and set(m.bnd_names) <= {'left', 'right', 'top', 'bottom'}
+    assert t.nedge == 259 and t.nel == 162 and t.nfacet == 259
+    m = load_mesh(os.path.join(GOLD, 'unit_square_coarse.vol'))
+    assert m.nel == 6 and m.nv == 8
+
+
+def test_reference_transient_poisson_value():
+    """pytests/full_system/poisson/test_poisson.py:60-78 (implicit Euler, CG): u_t = lap u on
+    unit_square_coarse.vol, H1 P3, u(0) = x(1-x), u = 0 on left/right, homogeneous Neumann
+    top/bottom, dt = 1e-3 to t = 1: the L2 norm of u (reference solution 0) is 9e-6 at rtol 3."""
+    m = load_mesh(os.path.join(GOLD, 'unit_square_coarse.vol'))
+    t = fem.Topology(m)
+    s = fem.Space(t, 'H1', 3, dirichlet_mask=m.marker_mask('left|right'))
+    cs = fem.Compound([s])
+    dt = 1e-3
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=dt),
+                            dict(kind='mass', trial=0, test=0, coef=1.0)])
+    Astep = A.matrix()
+    Mm = fem.Assembler(cs)
+    fem.assemble_volume(Mm, [dict(kind='mass', trial=0, test=0, coef=1.0)])
+    M = Mm.matrix()
+    # initial condition: L2 projection of x(1-x) (exact: it is in P3)
+    b0 = fem.assemble_source(cs, 0, lambda x, y: x * (1 - x))
+    u = fem.solve_free(M, b0, np.zeros(cs.ndof), np.ones(cs.ndof, bool))
+    assert fem.l2_error(s, u, lambda x, y: x * (1 - x)) < 1e-13
+    F = np.nonzero(cs.freedofs)[0]
+    import scipy.sparse.linalg as spla
+    lu = spla.splu(Astep[F][:, F].tocsc())
+    u[~cs.freedofs] = 0.0
+    for _ in range(1000):
+        rhs = M @ u
+        un = np.zeros_like(u)
+        un[F] = lu.solve(rhs[F])
+        u = un
+    err = fem.l2_error(s, u, 0.0)
+    assert np.isclose(err, 9e-6, rtol=3, atol=1e-12), err
