@@ -43,8 +43,8 @@ typedef struct ocmp_precond ocmp_precond;
 /* ------------------------------------------------------------------ context */
 
 /* Replaces the ngcore.TaskManager() scope of opencmp/run.py:57-61.
- * hip_stream: an existing hipStream_t to run on, or NULL to create one
- * (default priority). */
+ * hip_stream: the hipStream_t to run on; NULL selects the device's null
+ * stream (PyTorch's default stream). The caller keeps ownership. */
 int ocmp_ctx_create(int device, void *hip_stream, ocmp_ctx **out);
 int ocmp_ctx_destroy(ocmp_ctx *ctx);
 int ocmp_ctx_sync(ocmp_ctx *ctx);

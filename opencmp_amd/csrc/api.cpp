@@ -501,12 +501,10 @@ int ocmp_ctx_create(int device, void *hip_stream, ocmp_ctx **out) {
     OCMP_HIP(hipSetDevice(device));
     auto *c = new ocmp_ctx();
     c->c.device = device;
-    if (hip_stream) {
-      c->c.stream = (hipStream_t)hip_stream;
-    } else {
-      OCMP_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
-      c->c.own_stream = true;
-    }
+    // NULL selects the device's null stream, which is also PyTorch's default stream:
+    // work enqueued by the library and by torch is then ordered without extra events.
+    c->c.stream = (hipStream_t)hip_stream;
+    c->c.own_stream = false;
     c->c.scal.alloc(256);
     c->c.red.alloc(1 << 20);
     OCMP_HIP(hipHostMalloc((void **)&c->c.host_scal, 256 * sizeof(double), hipHostMallocDefault));

@@ -653,7 +653,7 @@ void precond_update(Precond *P) {
     w.nbr.alloc((size_t)3 * m.nel);
 #define OCMP_DG_CASE(NBV)                                                                                          \
   case NBV:                                                                                                        \
-    k_dg_block_inverse<NBV><<<nblk(m.nel), 128, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, A->d_val.p,       \
+    k_dg_block_inverse<NBV><<<nblk(m.nel, 128), 128, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, A->d_val.p,       \
                                                          P->binv.p, nullptr);                                      \
     k_dg_build_ahat<NBV><<<nblk(A->n), NTHR, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, m.d_elem_nb.p,       \
                                                       A->d_val.p, P->binv.p, mixed ? nullptr : P->ahat.p,          \

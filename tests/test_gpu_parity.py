@@ -67,8 +67,10 @@ def test_topology_dofs_pattern(gpu_ctx, mname):
     C = core()
     hm = MESHES[mname]()
     say('topology/dofs/pattern', mname)
-    cases = [[C.H1(1, 'left')], [C.H1(2, 'left|right')], [C.H1(3, 'left')],
-             [C.H1(2, 'left', ncomp=hm.dim), C.H1(1)]]
+    b0 = hm.bnd_names[0]
+    b01 = '|'.join(hm.bnd_names[:2])
+    cases = [[C.H1(1, b0)], [C.H1(2, b01)], [C.H1(3, b0)],
+             [C.H1(2, b0, ncomp=hm.dim), C.H1(1)]]
     if hm.dim == 2:
         cases.append([C.L2(2)])
     for comps in cases:
@@ -372,9 +374,22 @@ def test_richardson_and_determinism(gpu_ctx):
     x2 = C.GridFunction(gs)
     r2 = C.solve(a, f.vec, x2.vec, pre, 'CG', tol=1e-12)
     assert r1.iters == r2.iters and torch.equal(x1.vec, x2.vec)
-    x3 = C.GridFunction(gs)
-    r3 = C.solve(a, f.vec, x3.vec, pre, 'Richardson', tol=1e-3, maxit=20000)
-    assert r3.converged
+    # Jacobi-preconditioned Richardson (undamped, as NGSolve's PreconditionedRichardson default)
+    # converges for the P1 stiffness matrix of this mesh (5-point stencil); it need not for P2.
+    hm1 = unit_square(8)
+    names = '|'.join(hm1.bnd_names)
+    gm1, gs1 = gpu_space(gpu_ctx, hm1, [C.H1(1, names)])
+    a1 = C.BilinearForm(gs1)
+    a1 += C.Vol('gradgrad', 0, 0)
+    a1.Assemble()
+    f1 = C.LinearForm(gs1)
+    f1 += C.Source(0, (L.FN_CONST, (1.0,)))
+    f1.Assemble()
+    pre1 = C.Preconditioner(a1, 'local')
+    pre1.Update()
+    x3 = C.GridFunction(gs1)
+    r3 = C.solve(a1, f1.vec, x3.vec, pre1, 'Richardson', tol=1e-6, maxit=20000)
+    assert r3.converged, r3
 
 
 def test_3d_p3_kappa_mixed(gpu_ctx):
