@@ -210,8 +210,21 @@ int ocmp_assemble_linear(ocmp_space *s, double *b, int nterms, const ocmp_lterm 
 int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double *c,
                        uint64_t marker_mask);
 
-/* y = A x; x, y device f64[n]. */
+/* y = A x; x, y device f64[n]. Replaces a.mat * vec (opencmp/models/base_model.py:920). */
 int ocmp_matvec(ocmp_matrix *A, const double *x, double *y);
+
+/* y = a x + b y on the context stream; x, y device f64[n]. Composes time-scheme
+ * right-hand sides such as dt*f + M u_n (opencmp/solvers/time_integration_schemes.py:663-685). */
+int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
+
+/* out[0] = sqrt(int (u_h - g)^2), out[1] = sqrt(int |grad(u_h - g)|^2) for scalar
+ * block `block` of the device vector x. g is the grid function `ref` (device,
+ * same space) when ref != NULL, else the closed form (fn, c[10]). Replaces the
+ * L2 norms of opencmp/helpers/error.py:54-128 (norm, mean, _l_2) used by
+ * calc_error (:238-301) and by the Picard test of opencmp/models/ins.py:346-355.
+ * The H1-seminorm has no counterpart in the reference. Synchronises. */
+int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *ref, int fn, const double *c,
+                     double *out);
 
 /* ----------------------------------------------------- preconditioner, solve */
 
@@ -235,7 +248,8 @@ typedef struct {
   int32_t precision;   /* OCMP_FP64 or OCMP_MIXED (FP32 inner, FP64 refinement)  */
   int32_t restart;     /* GMRES restart length m                                 */
   int32_t maxit;       /* total inner iterations                                 */
-  double tol;          /* ||b - A x||_2 / ||b||_2 on free dofs, computed in FP64 */
+  double tol;          /* ||b - A x||_2 / ||b||_2 on free dofs, computed in FP64
+                          (relative to the initial residual when b is 0 on the free dofs) */
   double inner_tol;    /* mixed: relative reduction asked of each inner solve    */
   int32_t check_every; /* CG: residual polled by the host every k iterations     */
   int32_t max_outer;   /* mixed: refinement steps                                */

@@ -87,6 +87,8 @@ SIGNATURES = {
     'ocmp_assemble_linear': [P, P, C.c_int, C.POINTER(LTerm), C.c_int, C.POINTER(Field), C.c_int],
     'ocmp_dirichlet_set': [P, C.c_int, P, C.c_int, C.POINTER(C.c_double), C.c_uint64],
     'ocmp_matvec': [P, P, P],
+    'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
+    'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],

@@ -806,6 +806,25 @@ int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double
   });
 }
 
+int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *ref, int fn, const double *c,
+                     double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && x && c && out, "NULL argument");
+    OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
+    OCMP_REQUIRE(fn == OCMP_FN_CONST || fn == OCMP_FN_SINPROD || fn == OCMP_FN_POLY2, "unsupported function");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    error_norms(&s->s, block, x, ref, fn, c, out);
+  });
+}
+
+int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, double *y) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && x && y && n >= 0, "bad arguments");
+    OCMP_HIP(hipSetDevice(ctx->c.device));
+    if (n) axpby(&ctx->c, n, a, x, b, y);
+  });
+}
+
 int ocmp_matvec(ocmp_matrix *A, const double *x, double *y) {
   return guard([&] {
     OCMP_REQUIRE(A && x && y, "NULL argument");

@@ -596,6 +596,124 @@ __global__ void k_dir_faces(const int32_t *__restrict__ fids, int64_t nf, int64_
   x[base + fids[t]] = 0.0;
 }
 
+// ------------------------------------------------------------------ error norms
+
+__device__ __forceinline__ void eval_fn_grad(int fn, const double *c, const double *x, int dim, double *g) {
+  for (int d = 0; d < dim; ++d) g[d] = 0.0;
+  if (fn == OCMP_FN_SINPROD) {
+    for (int d = 0; d < dim; ++d) {
+      double s = c[0] * M_PI * cos(M_PI * x[d]);
+      for (int e = 0; e < dim; ++e)
+        if (e != d) s *= sin(M_PI * x[e]);
+      g[d] = s;
+    }
+  } else if (fn == OCMP_FN_POLY2) {
+    if (dim == 2) {
+      g[0] = c[1] + 2.0 * c[3] * x[0] + c[4] * x[1];
+      g[1] = c[2] + c[4] * x[0] + 2.0 * c[5] * x[1];
+    } else {
+      g[0] = c[1] + 2.0 * c[4] * x[0] + c[7] * x[1] + c[8] * x[2];
+      g[1] = c[2] + 2.0 * c[5] * x[1] + c[7] * x[0] + c[9] * x[2];
+      g[2] = c[3] + 2.0 * c[6] * x[2] + c[8] * x[0] + c[9] * x[1];
+    }
+  }
+}
+
+struct KNorm {
+  int nq, nl, ld, loc_off;
+  int tab_phi, tab_grad, tab_w, tab_lam;
+  int fn;
+  double c[10];
+  const double *tab;
+};
+
+// per-block partial sums of int (u_h - g)^2 and int |grad(u_h - g)|^2
+template <int D>
+__global__ void __launch_bounds__(256) k_error_norms(KNorm K, int64_t nel, const double *__restrict__ coords,
+                                                     const int32_t *__restrict__ elems,
+                                                     const int32_t *__restrict__ edofs, const double *__restrict__ x,
+                                                     const double *__restrict__ ref, double *__restrict__ part) {
+  __shared__ double sh[2][4];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e2 = 0.0, h2 = 0.0;
+  if (e < nel) {
+    Geo<D> g;
+    geometry<D>(coords, elems + e * (D + 1), g);
+    const int32_t *ed = edofs + e * K.ld + K.loc_off;
+    const double *phi = K.tab + K.tab_phi;
+    const double *gr = K.tab + K.tab_grad;
+    const double *w = K.tab + K.tab_w;
+    const double *lam = K.tab + K.tab_lam;
+    for (int q = 0; q < K.nq; ++q) {
+      double u = 0.0, gref[D];
+      for (int a = 0; a < D; ++a) gref[a] = 0.0;
+      for (int l = 0; l < K.nl; ++l) {
+        const double d = x[ed[l]] - (ref ? ref[ed[l]] : 0.0);
+        u += phi[q * K.nl + l] * d;
+        for (int a = 0; a < D; ++a) gref[a] += gr[(q * K.nl + l) * D + a] * d;
+      }
+      double gu[D];
+      for (int c = 0; c < D; ++c) {
+        double s = 0.0;
+        for (int a = 0; a < D; ++a) s += g.Jinv[a][c] * gref[a];
+        gu[c] = s;
+      }
+      if (!ref) {
+        double xq[D];
+        for (int c = 0; c < D; ++c) {
+          double s = g.x0[c];
+          for (int a = 0; a < D; ++a) s += g.J[c][a] * lam[q * (D + 1) + a + 1];
+          xq[c] = s;
+        }
+        u -= eval_fn(K.fn, K.c, xq, D);
+        double gg[3];
+        eval_fn_grad(K.fn, K.c, xq, D, gg);
+        for (int c = 0; c < D; ++c) gu[c] -= gg[c];
+      }
+      const double wq = w[q] * g.adet;
+      e2 += wq * u * u;
+      for (int c = 0; c < D; ++c) h2 += wq * gu[c] * gu[c];
+    }
+  }
+  // block reduction (fixed order)
+  for (int o = 32; o > 0; o >>= 1) {
+    e2 += __shfl_down(e2, o, 64);
+    h2 += __shfl_down(h2, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = e2;
+    sh[1][threadIdx.x >> 6] = h2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+__global__ void k_sum_pairs(const double *__restrict__ part, int64_t np, double *__restrict__ out) {
+  __shared__ double sh[2][256];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(sh[0][0]);
+    out[1] = sqrt(sh[1][0]);
+  }
+}
+
 // ================================================================== host side
 
 static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -1034,6 +1152,47 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
     k_dir_faces<<<nblk(L->nfaces), 256, 0, st>>>(L->faces.p, L->nfaces, b.dof_off + m.nv + 2 * m.nedge, x);
   OCMP_HIP(hipGetLastError());
   // the parameter copy reads host memory that the caller may free on return
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+void error_norms(Space *sp, int block, const double *x, const double *ref, int fn, const double *c, double *out) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const Block &b = s.blocks[block];
+  const int d = m.dim;
+  KNorm K;
+  std::memset(&K, 0, sizeof K);
+  TableBuilder tb;
+  Rule rule = simplex_rule(d, 2 * b.order + 6);
+  K.nq = rule.nq;
+  K.nl = b.nloc;
+  K.ld = s.nloc;
+  K.loc_off = b.loc_off;
+  std::vector<double> phi((size_t)rule.nq * b.nloc), gr((size_t)rule.nq * b.nloc * d);
+  for (int q = 0; q < rule.nq; ++q)
+    eval_basis(b.kind, d, b.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b.nloc], &gr[(size_t)q * b.nloc * d]);
+  K.tab_phi = tb.add(phi);
+  K.tab_grad = tb.add(gr);
+  K.tab_w = tb.add(rule.w);
+  K.tab_lam = tb.add(rule.lam);
+  K.fn = fn;
+  for (int i = 0; i < 10; ++i) K.c[i] = c[i];
+  DBuf<double> *tab = cached_table(s, "N" + std::to_string(block), tb, st);
+  K.tab = tab->p;
+  const unsigned nb = nblk(m.nel);
+  double *part = ctx->red.p;
+  OCMP_REQUIRE(ctx->red.n >= 2 * (size_t)nb + 2, "reduction scratch too small for this mesh");
+  KTimer tm(ctx, ctx->cls("error_norms"));
+  if (d == 2)
+    k_error_norms<2><<<nb, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, ref, part);
+  else
+    k_error_norms<3><<<nb, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, ref, part);
+  double *res = ctx->scal.p + 110;
+  k_sum_pairs<<<1, 256, 0, st>>>(part, nb, res);
+  OCMP_HIP(hipGetLastError());
+  OCMP_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   OCMP_HIP(hipStreamSynchronize(st));
 }
 

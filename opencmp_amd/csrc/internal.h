@@ -217,8 +217,10 @@ struct Precond {
   DBuf<double> binv;     // block Jacobi: nel*nb*nb
   DBuf<float> binv32;
   // right-preconditioned DG operator: Ahat = A * blockdiag^{-1}, off-diagonal blocks only
-  DBuf<double> ahat;     // nel*4*nb*nb (padded to 4 block slots: diag slot unused)
+  DBuf<double> ahat;     // n*3*nb, column-major ELL (facet-neighbour blocks)
   DBuf<float> ahat32;
+  // A itself in the same ELL layout with the diagonal block as slot 0 (FP64 outer residual)
+  DBuf<double> aell;     // n*4*nb
   bool ready = false;
 };
 
@@ -233,6 +235,8 @@ void launch_dirichlet_set(Space *s, int block, double *x, int fn, const double *
 void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
+void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
+void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 
 // greedy colouring + slot map (host)
 void build_h1_schedule(Matrix *A);

@@ -8,8 +8,10 @@
 // own contiguous row ranges; block ids are remapped so that each of the 8
 // XCDs (which receive workgroups round-robin) works on one contiguous slice
 // of the rows, keeping the x-gathers of neighbouring rows in one L2.
+// Streaming kernels move 16 bytes per lane per load (float4 / double2).
 #include <cmath>
 #include <cstring>
+#include <deque>
 
 #include "internal.h"
 
@@ -18,6 +20,7 @@ namespace ocmp {
 constexpr int NTHR = 256;
 constexpr int NXCD = 8;
 constexpr int MAX_RED_BLOCKS = 2048;
+constexpr int STREAM_BLOCKS = 1024;  // grid of the streaming reduction kernels (4 per CU)
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
 
@@ -49,9 +52,60 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
   return s;
 }
 
+// 16-byte vector access
+template <typename T>
+struct Vec;
+template <>
+struct Vec<float> {
+  using t = float4;
+  static constexpr int w = 4;
+};
+template <>
+struct Vec<double> {
+  using t = double2;
+  static constexpr int w = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ void vload(const T *__restrict__ p, T (&o)[Vec<T>::w]) {
+  const typename Vec<T>::t v = *reinterpret_cast<const typename Vec<T>::t *>(p);
+  if constexpr (Vec<T>::w == 4) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  } else {
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void vstore(T *__restrict__ p, const T (&o)[Vec<T>::w]) {
+  typename Vec<T>::t v;
+  if constexpr (Vec<T>::w == 4) {
+    v.x = o[0];
+    v.y = o[1];
+    v.z = o[2];
+    v.w = o[3];
+  } else {
+    v.x = o[0];
+    v.y = o[1];
+  }
+  *reinterpret_cast<typename Vec<T>::t *>(p) = v;
+}
+
 static inline int red_grid(int64_t work_items) {
   int64_t g = (work_items + NTHR - 1) / NTHR;
   g = std::min<int64_t>(g, MAX_RED_BLOCKS);
+  g = std::max<int64_t>(g, NXCD);
+  g = (g + NXCD - 1) / NXCD * NXCD;
+  return (int)g;
+}
+
+static inline int stream_grid(int64_t nvec) {
+  int64_t g = (nvec + NTHR - 1) / NTHR;
+  g = std::min<int64_t>(g, STREAM_BLOCKS);
   g = std::max<int64_t>(g, NXCD);
   g = (g + NXCD - 1) / NXCD * NXCD;
   return (int)g;
@@ -166,39 +220,95 @@ void spmv_f64(Matrix *A, const double *x, double *y) {
   OCMP_HIP(hipGetLastError());
 }
 
-// ------------------------------------------------------------------ DG right-preconditioned operator (ELL layout)
+// ------------------------------------------------------------------ DG operators (ELL layout)
 //
 // Ahat = A * blockdiag(A)^{-1}. Its diagonal blocks are the identity, so only
 // the three facet-neighbour blocks are stored, column-major "ELL":
 // ahat[(k*nb + j) * n + r] = Ahat(r, nbr_k*nb + j), r = e*nb + i, k = local facet.
 // Boundary facets store zero blocks and point at the element itself.
-// nbr[k*nel + e] is the neighbour element across facet k.
+// nbr[k*nel + e] is the neighbour element across facet k. A itself is kept in
+// the same layout with four slots (slot 0 = diagonal block) for the FP64
+// outer residual. Each lane handles VW consecutive rows so every matrix load
+// moves 16 bytes per lane.
 
 template <typename T, int NB>
 __global__ void __launch_bounds__(NTHR) k_dg_ahat_spmv(int64_t nel, const int32_t *__restrict__ nbr,
                                                        const T *__restrict__ ahat, const T *__restrict__ x,
                                                        const double *__restrict__ sx, T *__restrict__ y) {
+  constexpr int VW = Vec<T>::w;
   const int64_t n = nel * NB;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const int64_t e = r / NB;
+  const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VW;
+  if (r0 >= n) return;  // n % VW == 0 is checked by the host
   const T scale = sx ? (T)sx[0] : (T)1;
-  T s = x[r];
+  T s[VW];
+  vload(x + r0, s);
+  int64_t cb[VW][3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int64_t c0 = (int64_t)nbr[k * nel + e] * NB;
-    const T *a = ahat + (int64_t)(k * NB) * n + r;
+  for (int l = 0; l < VW; ++l) {
+    const int64_t e = (r0 + l) / NB;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) s += a[(int64_t)j * n] * x[c0 + j];
+    for (int k = 0; k < 3; ++k) cb[l][k] = (int64_t)nbr[k * nel + e] * NB;
   }
-  y[r] = s * scale;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      T a[VW];
+      vload(ahat + (int64_t)(k * NB + j) * n + r0, a);
+#pragma unroll
+      for (int l = 0; l < VW; ++l) s[l] += a[l] * x[cb[l][k] + j];
+    }
+#pragma unroll
+  for (int l = 0; l < VW; ++l) s[l] *= scale;
+  vstore(y + r0, s);
+}
+
+// r = b - A x with A in the 4-slot ELL layout; partial sums of r^2 (two rows per lane)
+template <int NB>
+__global__ void __launch_bounds__(NTHR) k_dg_ell_residual(int64_t nel, const int32_t *__restrict__ nbr,
+                                                          const double *__restrict__ aell,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ b, double *__restrict__ r,
+                                                          double *__restrict__ part) {
+  __shared__ double sh[NTHR / 64];
+  const int64_t n = nel * NB;
+  const int64_t nvec = n / 2;
+  const int lb = xcd_block();
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
+  double acc = 0.0;
+  for (int64_t v = i0 + threadIdx.x; v < i1; v += blockDim.x) {
+    const int64_t r0 = 2 * v;
+    double s[2];
+    vload(b + r0, s);
+    int64_t cb[2][4];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const int64_t e = (r0 + l) / NB;
+      cb[l][0] = e * NB;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) cb[l][k + 1] = (int64_t)nbr[k * nel + e] * NB;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        double a[2];
+        vload(aell + (int64_t)(k * NB + j) * n + r0, a);
+        s[0] -= a[0] * x[cb[0][k] + j];
+        s[1] -= a[1] * x[cb[1][k] + j];
+      }
+    vstore(r + r0, s);
+    acc += s[0] * s[0] + s[1] * s[1];
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[lb] = acc;
 }
 
 // invert the diagonal blocks (Gauss-Jordan, partial pivoting), one thread per element
 template <int NB>
 __global__ void k_dg_block_inverse(int64_t nel, const int32_t *__restrict__ rp, const int8_t *__restrict__ dg_pos,
-                                   const double *__restrict__ val, double *__restrict__ binv,
-                                   float *__restrict__ binv32) {
+                                   const double *__restrict__ val, double *__restrict__ binv) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   double a[NB][NB], v[NB][NB];
@@ -254,36 +364,39 @@ __global__ void k_dg_block_inverse(int64_t nel, const int32_t *__restrict__ rp, 
 #pragma unroll
   for (int i = 0; i < NB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      o[i * NB + j] = v[i][j];
-      if (binv32) binv32[e * NB * NB + i * NB + j] = (float)v[i][j];
-    }
+    for (int j = 0; j < NB; ++j) o[i * NB + j] = v[i][j];
 }
 
-// ahat(r, block k) = A(r, block nbr_k) * inv(D_nbr_k); one thread per row
+// ahat(r, block k) = A(r, block nbr_k) * inv(D_nbr_k); aell = A in 4-slot ELL; one thread per row
 template <int NB>
-__global__ void k_dg_build_ahat(int64_t nel, const int32_t *__restrict__ rp, const int8_t *__restrict__ dg_pos,
-                                const int32_t *__restrict__ elem_nb, const double *__restrict__ val,
-                                const double *__restrict__ binv, double *__restrict__ ahat,
-                                float *__restrict__ ahat32, int32_t *__restrict__ nbr) {
+__global__ void k_dg_build_ell(int64_t nel, const int32_t *__restrict__ rp, const int8_t *__restrict__ dg_pos,
+                               const int32_t *__restrict__ elem_nb, const double *__restrict__ val,
+                               const double *__restrict__ binv, double *__restrict__ ahat,
+                               float *__restrict__ ahat32, double *__restrict__ aell, int32_t *__restrict__ nbr) {
   const int64_t n = nel * NB;
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int64_t e = r / NB;
   const int i = (int)(r % NB);
+  {
+    const double *drow = val + rp[r] + dg_pos[e * 4] * NB;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) aell[(int64_t)j * n + r] = drow[j];
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int32_t nb = elem_nb[e * 3 + k];
     const int pos = dg_pos[e * 4 + 1 + k];
-    double out[NB];
+    double out[NB], raw[NB];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) out[j] = 0.0;
+    for (int j = 0; j < NB; ++j) out[j] = raw[j] = 0.0;
     if (nb >= 0 && pos >= 0) {
       const double *arow = val + rp[r] + pos * NB;
       const double *bi = binv + (int64_t)nb * NB * NB;
 #pragma unroll
       for (int l = 0; l < NB; ++l) {
         const double al = arow[l];
+        raw[l] = al;
 #pragma unroll
         for (int j = 0; j < NB; ++j) out[j] += al * bi[l * NB + j];
       }
@@ -292,6 +405,7 @@ __global__ void k_dg_build_ahat(int64_t nel, const int32_t *__restrict__ rp, con
     for (int j = 0; j < NB; ++j) {
       if (ahat) ahat[(int64_t)(k * NB + j) * n + r] = out[j];
       if (ahat32) ahat32[(int64_t)(k * NB + j) * n + r] = (float)out[j];
+      aell[(int64_t)((k + 1) * NB + j) * n + r] = raw[j];
     }
     if (i == 0) nbr[k * nel + e] = (nb >= 0 && pos >= 0) ? nb : (int32_t)e;
   }
@@ -340,7 +454,7 @@ __global__ void k_fill_t(T *__restrict__ p, int64_t n, T v) {
   if (i < n) p[i] = v;
 }
 
-// partial sums of a.b and (optionally) c.c
+// partial sums of a.b (masked)
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_dot(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
                                               const uint8_t *__restrict__ mask, double *__restrict__ part) {
@@ -425,106 +539,217 @@ __global__ void k_axpy64(int64_t n, double scale, const T *__restrict__ d, doubl
 // The Krylov basis is stored unnormalised: V[k] with device scale vs[k], so
 // v_k = vs[k] * V[k]. This saves the separate normalisation pass.
 
-constexpr int KC = 8;  // dot products per thread chunk
+constexpr int KC = 16;  // dot products per pass
 
-// partials of V[k].w for k in [0, nk) and of w.w (as index nk)
+// Last-block-done: every block publishes its partials (device-scope release),
+// then one atomic counter picks the block that finishes last; it acquires and
+// reduces all partials in a fixed order, so results stay deterministic. The
+// counter is reset by that block for the next launch (stream order).
+__device__ __forceinline__ bool last_block(unsigned *counter) {
+  __shared__ unsigned s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1) ? 1u : 0u;
+  __syncthreads();
+  if (s_last) __threadfence();
+  return s_last != 0u;
+}
+
+// dot[k] = V[k].w for k in [0, nk): partials per block for k in [k0, k0+KC), reduced by the last block
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ V, int64_t ldv,
-                                               const T *__restrict__ w, int nk, double *__restrict__ part) {
-  __shared__ double sh[NTHR / 64];
+                                               const T *__restrict__ w, int nk, double *__restrict__ part,
+                                               unsigned *__restrict__ counter, double *__restrict__ dot) {
+  constexpr int VW = Vec<T>::w;
+  __shared__ double sh[NTHR / 64][KC];
   const int lb = xcd_block();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
+  const int64_t nvec = n / VW;
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
   const int k0 = blockIdx.y * KC;
+  const int kend = min(KC, nk + 1 - k0);  // number of live accumulators (uniform)
   double acc[KC];
 #pragma unroll
   for (int c = 0; c < KC; ++c) acc[c] = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    const double wi = (double)w[i];
+    T wv[VW];
+    vload(w + i * VW, wv);
 #pragma unroll
     for (int c = 0; c < KC; ++c) {
-      const int k = k0 + c;
-      if (k < nk)
-        acc[c] += (double)V[(int64_t)k * ldv + i] * wi;
-      else if (k == nk)
-        acc[c] += wi * wi;
+      if (c < kend) {
+        const int k = k0 + c;
+        T vv[VW];
+        if (k < nk) {
+          vload(V + (int64_t)k * ldv + i * VW, vv);
+        } else {
+#pragma unroll
+          for (int l = 0; l < VW; ++l) vv[l] = wv[l];
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < VW; ++l) s += (double)vv[l] * (double)wv[l];
+        acc[c] += s;
+      }
+    }
+  }
+  if (lb == gridDim.x - 1) {  // tail rows (n % VW)
+    for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x) {
+      const double wi = (double)w[i];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        const int k = k0 + c;
+        if (c < kend) acc[c] += (k < nk ? (double)V[(int64_t)k * ldv + i] : wi) * wi;
+      }
     }
   }
 #pragma unroll
   for (int c = 0; c < KC; ++c) {
-    const int k = k0 + c;
-    const double s = block_sum(acc[c], sh);
-    if (threadIdx.x == 0 && k <= nk) part[(int64_t)k * gridDim.x + lb] = s;
+    const double v = wave_sum(acc[c]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][c] = v;
   }
+  __syncthreads();
+  if ((int)threadIdx.x < kend) {
+    double s = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < NTHR / 64; ++wv) s += sh[wv][threadIdx.x];
+    part[(int64_t)(k0 + threadIdx.x) * gridDim.x + lb] = s;
+  }
+  // the last block to finish sums the partials of every dot (fixed order) into dot[0..nk)
+  if (!last_block(counter)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int v = wave; v < nk; v += NTHR / 64) {
+    double s = 0.0;
+    for (int b = lane; b < (int)gridDim.x; b += 64) s += part[(int64_t)v * gridDim.x + b];
+    s = wave_sum(s);
+    if (lane == 0) dot[v] = s;
+  }
+  if (threadIdx.x == 0) *counter = 0u;
 }
 
-// w -= sum_k c_k V[k] with c_k = vs[k]^2 * dot[k]; partial of |w|^2
+// State of one GMRES cycle on the device: R column-major (ld = m+1), rotations, g, residual history.
+struct GState {
+  double *dot, *vs, *R, *csn, *g, *res;
+  int m;
+  double gscale;
+};
+
+// Givens step for column j, run by one block: R(:,j) = [vs[k]*dot[k]]_k, h_{j+1,j} = hn;
+// apply the previous rotations, form the new one, update g; vs[j+1] = 1/hn; res[j] = |g_{j+1}|*gscale.
+__device__ void givens_step(const GState &G, int j, double hn) {
+  __shared__ double col[256];
+  __shared__ double rot[512];
+  const int ld = G.m + 1;
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) col[k] = G.vs[k] * G.dot[k];
+  for (int k = threadIdx.x; k < 2 * j; k += blockDim.x) rot[k] = G.csn[k];
+  if (threadIdx.x == 0) col[j + 1] = hn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < j; ++k) {
+      const double c = rot[2 * k], s = rot[2 * k + 1];
+      const double a = col[k], b = col[k + 1];
+      col[k] = c * a + s * b;
+      col[k + 1] = -s * a + c * b;
+    }
+    const double a = col[j], b = col[j + 1];
+    const double den = hypot(a, b);
+    const double c = den > 0.0 ? a / den : 1.0, s = den > 0.0 ? b / den : 0.0;
+    G.csn[2 * j] = c;
+    G.csn[2 * j + 1] = s;
+    col[j] = den;
+    col[j + 1] = 0.0;
+    const double gj = G.g[j];
+    G.g[j] = c * gj;
+    G.g[j + 1] = -s * gj;
+    G.res[j] = fabs(G.g[j + 1]) * G.gscale;
+    G.vs[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
+  }
+  __syncthreads();
+  double *R = G.R + (int64_t)j * ld;
+  for (int k = threadIdx.x; k <= j + 1; k += blockDim.x) R[k] = col[k];
+}
+
+// w -= sum_k c_k V[k] with c_k = vs[k]^2 * dot[k] (classical Gram-Schmidt update); the last block
+// reduces |w|^2 and performs the Givens step of column j = nk-1
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_gs_update(int64_t n, const T *__restrict__ V, int64_t ldv,
                                                     T *__restrict__ w, int nk, const double *__restrict__ dot,
-                                                    const double *__restrict__ vs, double *__restrict__ part) {
+                                                    const double *__restrict__ vs, double *__restrict__ part,
+                                                    unsigned *__restrict__ counter, GState G) {
+  constexpr int VW = Vec<T>::w;
   __shared__ double sh[NTHR / 64];
   __shared__ double cks[256];
   for (int k = threadIdx.x; k < nk; k += blockDim.x) cks[k] = vs[k] * vs[k] * dot[k];
   __syncthreads();
   const int lb = xcd_block();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
+  const int64_t nvec = n / VW;
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
   double acc = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    double s = (double)w[i];
-    for (int k = 0; k < nk; ++k) s -= cks[k] * (double)V[(int64_t)k * ldv + i];
-    const T o = (T)s;
-    w[i] = o;
-    acc += (double)o * (double)o;
+    T wv[VW];
+    vload(w + i * VW, wv);
+    double s[VW];
+#pragma unroll
+    for (int l = 0; l < VW; ++l) s[l] = (double)wv[l];
+#pragma unroll 4
+    for (int k = 0; k < nk; ++k) {
+      T vv[VW];
+      vload(V + (int64_t)k * ldv + i * VW, vv);
+      const double c = cks[k];
+#pragma unroll
+      for (int l = 0; l < VW; ++l) s[l] -= c * (double)vv[l];
+    }
+#pragma unroll
+    for (int l = 0; l < VW; ++l) {
+      wv[l] = (T)s[l];
+      acc += (double)wv[l] * (double)wv[l];
+    }
+    vstore(w + i * VW, wv);
+  }
+  if (lb == gridDim.x - 1) {
+    for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x) {
+      double s = (double)w[i];
+      for (int k = 0; k < nk; ++k) s -= cks[k] * (double)V[(int64_t)k * ldv + i];
+      const T o = (T)s;
+      w[i] = o;
+      acc += (double)o * (double)o;
+    }
   }
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[lb] = acc;
-}
-
-// Givens step for column j (single thread).
-// dot[0..j] = V[k].w, wn2 = |w_after|^2 (from the update partials, reduced into dot[j+1]).
-// State: R (column-major, ldr = m+1), cs/sn, g; vs[j+1] = 1/h_{j+1,j}; res[it] = |g_{j+1}| * gscale.
-__global__ void k_givens(int j, int m, const double *__restrict__ dot, double *__restrict__ vs,
-                         double *__restrict__ R, double *__restrict__ csn, double *__restrict__ g,
-                         double *__restrict__ res, int it, double gscale) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int ld = m + 1;
-  double *col = R + (int64_t)j * ld;
-  for (int k = 0; k <= j; ++k) col[k] = vs[k] * dot[k];
-  const double hn = sqrt(dot[j + 1]);
-  col[j + 1] = hn;
-  vs[j + 1] = hn > 0.0 ? 1.0 / hn : 0.0;
-  for (int k = 0; k < j; ++k) {
-    const double c = csn[2 * k], s = csn[2 * k + 1];
-    const double a = col[k], b = col[k + 1];
-    col[k] = c * a + s * b;
-    col[k + 1] = -s * a + c * b;
+  if (!last_block(counter)) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += part[b];
+  s = block_sum(s, sh);
+  __shared__ double hn;
+  if (threadIdx.x == 0) {
+    hn = sqrt(s);
+    *counter = 0u;
   }
-  const double a = col[j], b = col[j + 1];
-  const double den = hypot(a, b);
-  const double c = den > 0.0 ? a / den : 1.0, s = den > 0.0 ? b / den : 0.0;
-  csn[2 * j] = c;
-  csn[2 * j + 1] = s;
-  col[j] = den;
-  col[j + 1] = 0.0;
-  const double gj = g[j];
-  g[j] = c * gj;
-  g[j + 1] = -s * gj;
-  res[it] = fabs(g[j + 1]) * gscale;
+  __syncthreads();
+  givens_step(G, nk - 1, hn);
 }
 
-// back substitution R y = g (k columns), then coef[l] = y_l * vs[l]
-__global__ void k_gmres_solve(int k, int m, const double *__restrict__ R, const double *__restrict__ g,
-                              const double *__restrict__ vs, double *__restrict__ coef) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// back substitution R y = g (k columns) as a column sweep: once y_i is known every
+// thread removes R(r,i) y_i from its g_r (column i of R is contiguous); then coef[l] = y_l * vs[l]
+__global__ void __launch_bounds__(256) k_gmres_solve(int k, int m, const double *__restrict__ R,
+                                                     const double *__restrict__ g, const double *__restrict__ vs,
+                                                     double *__restrict__ coef) {
+  __shared__ double gs[256];
+  __shared__ double yi;
   const int ld = m + 1;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) gs[i] = g[i];
+  __syncthreads();
   for (int i = k - 1; i >= 0; --i) {
-    double s = g[i];
-    for (int l = i + 1; l < k; ++l) s -= R[(int64_t)l * ld + i] * coef[l];
-    coef[i] = s / R[(int64_t)i * ld + i];
+    if (threadIdx.x == 0) {
+      yi = gs[i] / R[(int64_t)i * ld + i];
+      coef[i] = yi;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < i; r += blockDim.x) gs[r] -= R[(int64_t)i * ld + r] * yi;
+    __syncthreads();
   }
-  for (int i = 0; i < k; ++i) coef[i] *= vs[i];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) coef[i] *= vs[i];
 }
 
 // u = scale * sum_l coef[l] V[l]   (in FP64)
@@ -558,7 +783,19 @@ __global__ void k_diag_apply_add(int64_t n, const double *__restrict__ dinv, con
   if (i < n) x[i] += (dinv ? dinv[i] : 1.0) * u[i];
 }
 
+// y = a x + b y
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
 // ================================================================== host drivers
+
+void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y) {
+  KTimer tm(ctx, ctx->cls("axpby"), 24.0 * n);
+  k_axpby<<<nblk(n), NTHR, 0, ctx->stream>>>(n, a, x, b, y);
+  OCMP_HIP(hipGetLastError());
+}
 
 struct Work {
   // scratch owned by the matrix' solver state; grown on demand
@@ -569,7 +806,9 @@ struct Work {
   DBuf<double> small;// device scalars / Hessenberg etc.
   double *hbuf = nullptr;  // pinned host
   size_t hcap = 0;
-  DBuf<int32_t> nbr;       // DG neighbour table for the ELL operator
+  DBuf<int32_t> nbr;       // DG neighbour table for the ELL operators
+  DBuf<unsigned> counters; // last-block-done counters of the fused reductions (zeroed once)
+  std::vector<hipEvent_t> events;
 };
 
 static std::map<Matrix *, Work *> g_work;
@@ -591,18 +830,49 @@ static double *pinned(Work &w, size_t n) {
   return w.hbuf;
 }
 
+static hipEvent_t work_event(Work &w, size_t i) {
+  while (w.events.size() <= i) {
+    hipEvent_t e;
+    OCMP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    w.events.push_back(e);
+  }
+  return w.events[i];
+}
+
+static void ensure_part(Work &w, size_t n) {
+  w.part.alloc(std::max<size_t>(w.part.n, std::max<size_t>(n, (size_t)68 * MAX_RED_BLOCKS)));
+}
+
 // reduce nvals partial rows of `grid` entries into dst (device)
 static void reduce_to(const double *part, int grid, int nvals, double *dst, hipStream_t st, bool do_sqrt = false) {
   k_reduce<<<nvals, NTHR, 0, st>>>(part, grid, dst, do_sqrt ? 1 : 0);
 }
 
+static bool dg_ell_ready(Matrix *A, Precond *P) {
+  return A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && P->ready && P->aell.p && (A->n % 2 == 0);
+}
+
 // FP64 residual r = mask*(b - A x); returns ||r|| (syncs)
-static double residual64(Matrix *A, const double *b, const double *x, double *r, Work &w) {
+static double residual64(Matrix *A, Precond *P, const double *b, const double *x, double *r, Work &w) {
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
-  const int grid = spmv_grid(A);
-  w.part.alloc(std::max<size_t>(w.part.n, (size_t)4 * MAX_RED_BLOCKS + 64 * MAX_RED_BLOCKS));
-  {
+  ensure_part(w, 0);
+  int grid;
+  // the ELL residual loads b and stores r two rows at a time (16-byte aligned)
+  if (dg_ell_ready(A, P) && ((uintptr_t)b % 16 == 0) && ((uintptr_t)r % 16 == 0)) {
+    const int64_t nel = A->space->mesh->nel;
+    const int nb = A->dg_nb;
+    grid = stream_grid(A->n / 2);
+    KTimer tm(ctx, ctx->cls("dg_ell_residual_f64"), (double)A->n * 4 * nb * 8 + 12.0 * nel + 3.0 * A->n * 8);
+    switch (nb) {
+      case 1: k_dg_ell_residual<1><<<grid, NTHR, 0, st>>>(nel, w.nbr.p, P->aell.p, x, b, r, w.part.p); break;
+      case 3: k_dg_ell_residual<3><<<grid, NTHR, 0, st>>>(nel, w.nbr.p, P->aell.p, x, b, r, w.part.p); break;
+      case 6: k_dg_ell_residual<6><<<grid, NTHR, 0, st>>>(nel, w.nbr.p, P->aell.p, x, b, r, w.part.p); break;
+      case 10: k_dg_ell_residual<10><<<grid, NTHR, 0, st>>>(nel, w.nbr.p, P->aell.p, x, b, r, w.part.p); break;
+      default: throw Error(OCMP_ERR_UNSUPPORTED, "DG block size");
+    }
+  } else {
+    grid = spmv_grid(A);
     KTimer tm(ctx, ctx->cls("spmv_csr_f64_residual"), csr_bytes(A, 8, 8) + 8.0 * A->n);
     spmv_dispatch<double, double>(A, A->d_val.p, x, r, b, A->space->d_free.p, (const double *)nullptr, nullptr, 1,
                                   w.part.p, grid, st);
@@ -618,6 +888,7 @@ static double residual64(Matrix *A, const double *b, const double *x, double *r,
 static double norm_masked(Matrix *A, const double *v, Work &w) {
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
+  ensure_part(w, 0);
   const int grid = red_grid(A->n);
   k_dot<double><<<grid, NTHR, 0, st>>>(A->n, v, v, A->space->d_free.p, w.part.p);
   reduce_to(w.part.p, grid, 1, ctx->scal.p + 101, st);
@@ -646,18 +917,20 @@ void precond_update(Precond *P) {
   } else {
     const Mesh &m = *A->space->mesh;
     const int nb = A->dg_nb;
+    OCMP_REQUIRE(A->n % 4 == 0, "block Jacobi ELL operator needs a dof count divisible by 4");
     P->binv.alloc((size_t)m.nel * nb * nb);
     P->ahat.alloc(mixed ? 0 : (size_t)A->n * 3 * nb);
     if (mixed) P->ahat32.alloc((size_t)A->n * 3 * nb);
+    P->aell.alloc((size_t)A->n * 4 * nb);
     Work &w = work(A);
     w.nbr.alloc((size_t)3 * m.nel);
-#define OCMP_DG_CASE(NBV)                                                                                          \
-  case NBV:                                                                                                        \
-    k_dg_block_inverse<NBV><<<nblk(m.nel, 128), 128, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, A->d_val.p,       \
-                                                         P->binv.p, nullptr);                                      \
-    k_dg_build_ahat<NBV><<<nblk(A->n), NTHR, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, m.d_elem_nb.p,       \
-                                                      A->d_val.p, P->binv.p, mixed ? nullptr : P->ahat.p,          \
-                                                      mixed ? P->ahat32.p : nullptr, w.nbr.p);                     \
+#define OCMP_DG_CASE(NBV)                                                                                           \
+  case NBV:                                                                                                         \
+    k_dg_block_inverse<NBV><<<nblk(m.nel, 128), 128, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, A->d_val.p,    \
+                                                              P->binv.p);                                           \
+    k_dg_build_ell<NBV><<<nblk(A->n), NTHR, 0, st>>>(m.nel, A->d_row_ptr.p, A->d_dg_pos.p, m.d_elem_nb.p,          \
+                                                     A->d_val.p, P->binv.p, mixed ? nullptr : P->ahat.p,            \
+                                                     mixed ? P->ahat32.p : nullptr, P->aell.p, w.nbr.p);            \
     break;
     switch (nb) {
       OCMP_DG_CASE(1)
@@ -675,6 +948,8 @@ void precond_update(Precond *P) {
 
 // ------------------------------------------------------------------ CG
 
+// Batches of `check` iterations are enqueued ahead; the host checks the
+// residual history of batch b while batch b+1 runs, so the GPU never idles.
 template <typename VT, typename T>
 static int cg_inner(Matrix *A, const VT *val, const T *dinv, T *x, T *r, T *z, T *p, T *q, double tol_rr,
                     int maxit, int check_every, Work &w, int &iters_done, double *last_rr) {
@@ -684,16 +959,21 @@ static int cg_inner(Matrix *A, const VT *val, const T *dinv, T *x, T *r, T *z, T
   const int64_t n = A->n;
   const int sgrid = spmv_grid(A);
   const int vgrid = red_grid(n);
-  double *sc = ctx->scal.p;  // [0],[1]: rz ring, [2]: pq, [3]: rr
-  DBuf<double> &hist = w.small;
+  double *sc = ctx->scal.p;  // [0],[1]: rz ring, [2]: pq
   const int H = std::max(check_every, 1);
-  hist.alloc(std::max<size_t>(hist.n, 4096));
-  double *hh = pinned(w, H + 8);
+  w.small.alloc(std::max<size_t>(w.small.n, (size_t)2 * H + 64));
+  double *hist = w.small.p;  // two halves of H entries
+  double *hh = pinned(w, 2 * H + 8);
   const int c_spmv = ctx->cls(sizeof(VT) == 8 ? "spmv_csr_f64_cg" : "spmv_csr_f32_cg");
   const int c_upd = ctx->cls(sizeof(T) == 8 ? "cg_update_f64" : "cg_update_f32");
-  int it = 0;
-  while (it < maxit) {
+  struct Batch {
+    int it0, nb, half;
+  };
+  std::deque<Batch> inflight;
+  int it = 0, bidx = 0;
+  auto enqueue = [&]() {
     const int nb = std::min(H, maxit - it);
+    const int half = bidx & 1;
     for (int s = 0; s < nb; ++s, ++it) {
       {
         KTimer tm(ctx, c_spmv, csr_bytes(A, sizeof(VT), sizeof(T)));
@@ -705,17 +985,27 @@ static int cg_inner(Matrix *A, const VT *val, const T *dinv, T *x, T *r, T *z, T
         KTimer tm(ctx, c_upd, 8.0 * n * sizeof(T));
         k_cg_update<T><<<vgrid, NTHR, 0, st>>>(n, x, r, z, p, q, dinv, sc, it, w.part.p);
       }
-      // rz_new -> sc[(it+1)&1], rr -> hist[s]
       reduce_to(w.part.p, vgrid, 1, sc + ((it + 1) & 1), st);
-      reduce_to(w.part.p + vgrid, vgrid, 1, hist.p + s, st);
+      reduce_to(w.part.p + vgrid, vgrid, 1, hist + half * H + s, st);
       k_cg_p<T><<<nblk(n), NTHR, 0, st>>>(n, p, z, sc, it);
     }
-    OCMP_HIP(hipMemcpyAsync(hh, hist.p, nb * sizeof(double), hipMemcpyDeviceToHost, st));
-    OCMP_HIP(hipStreamSynchronize(st));
-    for (int s = 0; s < nb; ++s) {
-      *last_rr = hh[s];
-      if (!(hh[s] > tol_rr)) {  // also stops on NaN
-        iters_done = it - nb + s + 1;
+    OCMP_HIP(hipMemcpyAsync(hh + half * H, hist + half * H, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    OCMP_HIP(hipEventRecord(work_event(w, half), st));
+    inflight.push_back({it - nb, nb, half});
+    ++bidx;
+  };
+  while (true) {
+    while (inflight.size() < 2 && it < maxit) enqueue();
+    if (inflight.empty()) break;
+    Batch b = inflight.front();
+    inflight.pop_front();
+    OCMP_HIP(hipEventSynchronize(work_event(w, b.half)));
+    for (int s = 0; s < b.nb; ++s) {
+      *last_rr = hh[b.half * H + s];
+      if (!(*last_rr > tol_rr)) {  // also stops on NaN
+        // a batch still in flight only refines x further; let it finish so r and x stay consistent
+        OCMP_HIP(hipStreamSynchronize(st));
+        iters_done = it;
         return 1;
       }
     }
@@ -731,14 +1021,13 @@ static void solve_cg(Matrix *A, Precond *P, const double *b, double *x, const oc
   Work &w = work(A);
   const int64_t n = A->n;
   OCMP_REQUIRE(P && P->kind == OCMP_PRE_JACOBI, "CG needs a Jacobi preconditioner (as NGSolve's CG needs one)");
-  w.part.alloc(std::max<size_t>(w.part.n, (size_t)4 * MAX_RED_BLOCKS + 64 * MAX_RED_BLOCKS));
+  ensure_part(w, 0);
   w.r64.alloc(n);
   const double bn = norm_masked(A, b, w);
-  if (bn == 0.0) {
-    // b = 0 on free dofs: the solution is x_F = -A_FF^{-1} A_FD x_D; fall through with tol relative to ||A x0||
-  }
-  const double bref = bn > 0.0 ? bn : 1.0;
-  double rn = residual64(A, b, x, w.r64.p, w);
+  double rn = residual64(A, P, b, x, w.r64.p, w);
+  // ||b|| on free dofs is the reference; a problem driven only by Dirichlet data (b_F = 0)
+  // measures against the initial residual instead
+  const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
   st_.initial_rel_residual = rn / bref;
   st_.rel_residual = rn / bref;
   const int check = o.check_every > 0 ? o.check_every : 10;
@@ -749,38 +1038,32 @@ static void solve_cg(Matrix *A, Precond *P, const double *b, double *x, const oc
   }
   const bool mixed = o.precision == OCMP_MIXED;
   int total = 0;
+  const int vgrid = red_grid(n);
   if (!mixed) {
     w.vd.alloc((size_t)4 * n);
     double *z = w.vd.p, *p = z + n, *q = p + n;
     double *r = w.r64.p;
-    // z = dinv r, p = z, rz
-    k_jacobi_apply<double, double><<<nblk(n), NTHR, 0, st>>>(n, P->dinv.p, r, 1.0, z, p);
-    const int vgrid = red_grid(n);
-    k_dot<double><<<vgrid, NTHR, 0, st>>>(n, r, z, nullptr, w.part.p);
-    reduce_to(w.part.p, vgrid, 1, ctx->scal.p + 0, st);
     int outer = 0;
     while (true) {
       ++outer;
-      int done = 0;
-      double last = 0.0;
-      const int conv = cg_inner<double, double>(A, A->d_val.p, P->dinv.p, x, r, z, p, q,
-                                                o.tol * o.tol * bref * bref, o.maxit - total, check, w, done, &last);
-      total += done;
-      rn = residual64(A, b, x, w.r64.p, w);  // true residual (also restarts r)
-      st_.rel_residual = rn / bref;
-      if (rn <= o.tol * bref || total >= o.maxit || !std::isfinite(rn)) break;
-      (void)conv;
-      // recurrence drifted: restart from the true residual
+      // z = dinv r, p = z, rz = r.z   (r is the true residual)
       k_jacobi_apply<double, double><<<nblk(n), NTHR, 0, st>>>(n, P->dinv.p, r, 1.0, z, p);
       k_dot<double><<<vgrid, NTHR, 0, st>>>(n, r, z, nullptr, w.part.p);
       reduce_to(w.part.p, vgrid, 1, ctx->scal.p + 0, st);
+      int done = 0;
+      double last = 0.0;
+      cg_inner<double, double>(A, A->d_val.p, P->dinv.p, x, r, z, p, q, o.tol * o.tol * bref * bref,
+                               o.maxit - total, check, w, done, &last);
+      total += done;
+      rn = residual64(A, P, b, x, w.r64.p, w);  // true residual (also restarts r)
+      st_.rel_residual = rn / bref;
+      if (rn <= o.tol * bref || total >= o.maxit || !std::isfinite(rn) || done == 0) break;
     }
     st_.outer = outer;
   } else {
     OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
     w.vf.alloc((size_t)5 * n);
     float *d = w.vf.p, *r = d + n, *z = r + n, *p = z + n, *q = p + n;
-    const int vgrid = red_grid(n);
     const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-4;
     int outer = 0;
     while (true) {
@@ -800,7 +1083,7 @@ static void solve_cg(Matrix *A, Precond *P, const double *b, double *x, const oc
       total += done;
       k_axpy64<float><<<nblk(n), NTHR, 0, st>>>(n, rn, d, x);
       const double rn_old = rn;
-      rn = residual64(A, b, x, w.r64.p, w);
+      rn = residual64(A, P, b, x, w.r64.p, w);
       st_.rel_residual = rn / bref;
       if (rn <= o.tol * bref || total >= o.maxit || !std::isfinite(rn)) break;
       if (outer >= (o.max_outer > 0 ? o.max_outer : 50)) break;
@@ -824,42 +1107,53 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
   const int64_t n = A->n;
   const bool dg = A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI;
   const uint8_t *mask = A->dg ? nullptr : A->space->d_free.p;
+  constexpr int VW = Vec<T>::w;
+  // basis rows padded to a multiple of VW elements so every row is 16-byte aligned
+  const int64_t ldv = (n + VW - 1) / VW * VW;
   DBuf<T> *Vbuf;
   if constexpr (sizeof(T) == 8) {
-    w.vd.alloc(std::max<size_t>(w.vd.n, (size_t)(m + 1) * n));
+    w.vd.alloc(std::max<size_t>(w.vd.n, (size_t)(m + 1) * ldv));
     Vbuf = &w.vd;
   } else {
-    w.vf.alloc(std::max<size_t>(w.vf.n, (size_t)(m + 1) * n));
+    w.vf.alloc(std::max<size_t>(w.vf.n, (size_t)(m + 1) * ldv));
     Vbuf = &w.vf;
   }
   T *V = Vbuf->p;
-  const int64_t ldv = n;
-  // small device state: dot[m+2], vs[m+2], R[(m+1)*m], csn[2m], g[m+1], res[maxit], coef[m]
-  const size_t ns = (size_t)(m + 2) * 2 + (size_t)(m + 1) * m + 2 * m + (m + 1) + 4096 + m;
+  // small device state: dot[m+2], vs[m+2], R[(m+1)*m], csn[2m], g[m+1], res[m+8], coef[m]
+  const size_t ns = (size_t)(m + 2) * 2 + (size_t)(m + 1) * m + 2 * m + (m + 1) + (m + 8) + m;
   w.small.alloc(std::max(w.small.n, ns));
   double *dot = w.small.p, *vs = dot + (m + 2), *R = vs + (m + 2), *csn = R + (size_t)(m + 1) * m,
-         *g = csn + 2 * m, *res = g + (m + 1), *coef = res + 4096;
-  const int grid = red_grid(n);
-  const int nchunk = (m + 1 + KC) / KC;
-  w.part.alloc(std::max<size_t>(w.part.n, (size_t)(m + 2 + KC) * grid));
+         *g = csn + 2 * m, *res = g + (m + 1), *coef = res + (m + 8);
+  const int grid = stream_grid((n + VW - 1) / VW);
+  ensure_part(w, (size_t)(m + 2 + KC) * grid);
   // v0 = r0 (scale 1/beta, stored unnormalised)
   k_scale_copy<double, T><<<nblk(n), NTHR, 0, st>>>(n, r0, 1.0, V);
   k_gmres_init<<<1, 64, 0, st>>>(m, 1.0, g, vs, 1.0 / beta);
-  double *hres = pinned(w, check + 8);
+  double *hres = pinned(w, m + 8);
   const int c_op = ctx->cls(dg ? (sizeof(T) == 8 ? "dg_ahat_spmv_f64" : "dg_ahat_spmv_f32")
                                : (sizeof(T) == 8 ? "spmv_csr_f64_gmres" : "spmv_csr_f32_gmres"));
   const int c_dot = ctx->cls(sizeof(T) == 8 ? "gmres_mdot_f64" : "gmres_mdot_f32");
   const int c_upd = ctx->cls(sizeof(T) == 8 ? "gmres_update_f64" : "gmres_update_f32");
-  int j = 0, k_used = 0;
-  bool stop = false;
   const double gscale = beta;  // g is relative to beta
-  while (!stop) {
-    const int j_end = std::min(m, std::min(j + check, maxit));
+  const int limit = std::min(m, maxit);
+  GState G{dot, vs, R, csn, g, res, m, gscale};
+  if (!w.counters.p) {
+    w.counters.alloc(8);
+    OCMP_HIP(hipMemsetAsync(w.counters.p, 0, 8 * sizeof(unsigned), st));
+  }
+  struct Batch {
+    int j0, nb, ev;
+  };
+  std::deque<Batch> inflight;
+  int j = 0, bidx = 0, k_used = -1;
+  auto enqueue = [&]() {
+    const int j_end = std::min(limit, j + check);
     for (int jj = j; jj < j_end; ++jj) {
       T *vj = V + (int64_t)jj * ldv;
       T *wv = V + (int64_t)(jj + 1) * ldv;
       {
-        const double opb = dg ? (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * A->space->mesh->nel + 2.0 * n * sizeof(T)
+        const double opb = dg ? (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * A->space->mesh->nel +
+                                    2.0 * n * sizeof(T)
                               : csr_bytes(A, sizeof(T), sizeof(T));
         KTimer tm(ctx, c_op, opb);
         if (dg) {
@@ -868,13 +1162,13 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
             ah = P->ahat.p;
           else
             ah = P->ahat32.p;
-          const int nbk = A->dg_nb;
           const int64_t nel = A->space->mesh->nel;
-          switch (nbk) {
-            case 1: k_dg_ahat_spmv<T, 1><<<nblk(n), NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
-            case 3: k_dg_ahat_spmv<T, 3><<<nblk(n), NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
-            case 6: k_dg_ahat_spmv<T, 6><<<nblk(n), NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
-            case 10: k_dg_ahat_spmv<T, 10><<<nblk(n), NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
+          const unsigned gb = nblk((n + VW - 1) / VW);
+          switch (A->dg_nb) {
+            case 1: k_dg_ahat_spmv<T, 1><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
+            case 3: k_dg_ahat_spmv<T, 3><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
+            case 6: k_dg_ahat_spmv<T, 6><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
+            case 10: k_dg_ahat_spmv<T, 10><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, vj, vs + jj, wv); break;
             default: throw Error(OCMP_ERR_UNSUPPORTED, "DG block size");
           }
         } else {
@@ -893,37 +1187,38 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
         dim3 gd(grid, (jj + 2 + KC - 1) / KC);
-        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p);
+        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p, w.counters.p, dot);
       }
-      reduce_to(w.part.p, grid, jj + 1, dot, st);
       {
         KTimer tm(ctx, c_upd, (double)(jj + 3) * n * sizeof(T));
-        k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p);
+        k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p, w.counters.p + 1, G);
       }
-      reduce_to(w.part.p, grid, 1, dot + jj + 1, st);
-      k_givens<<<1, 64, 0, st>>>(jj, m, dot, vs, R, csn, g, res, jj, gscale);
     }
-    (void)nchunk;
     const int nb = j_end - j;
-    OCMP_HIP(hipMemcpyAsync(hres, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));
-    OCMP_HIP(hipStreamSynchronize(st));
-    int hit = -1;
-    for (int s = 0; s < nb; ++s)
-      if (!(hres[s] > stop_abs)) {
-        hit = s;
+    OCMP_HIP(hipMemcpyAsync(hres + j, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    const int ev = bidx & 1;
+    OCMP_HIP(hipEventRecord(work_event(w, ev), st));
+    inflight.push_back({j, nb, ev});
+    ++bidx;
+    j = j_end;
+  };
+  while (k_used < 0) {
+    while (inflight.size() < 2 && j < limit) enqueue();
+    if (inflight.empty()) {
+      k_used = j;
+      break;
+    }
+    Batch b = inflight.front();
+    inflight.pop_front();
+    OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
+    for (int s = 0; s < b.nb; ++s)
+      if (!(hres[b.j0 + s] > stop_abs)) {
+        k_used = b.j0 + s + 1;
         break;
       }
-    if (hit >= 0) {
-      k_used = j + hit + 1;
-      stop = true;
-    } else {
-      j = j_end;
-      k_used = j;
-      if (j >= m || j >= maxit) stop = true;
-    }
   }
-  // x += M^{-1} sum_l y_l v_l
-  k_gmres_solve<<<1, 64, 0, st>>>(k_used, m, R, g, vs, coef);
+  // x += M^{-1} sum_l y_l v_l   (columns beyond k_used computed by a batch still in flight are ignored)
+  k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, R, g, vs, coef);
   w.u64.alloc(n);
   k_combine<T><<<nblk(n), NTHR, 0, st>>>(n, V, ldv, k_used, coef, beta, w.u64.p);
   if (dg) {
@@ -946,7 +1241,7 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
                         ocmp_solve_stats &st_) {
   Work &w = work(A);
   const int64_t n = A->n;
-  w.part.alloc(std::max<size_t>(w.part.n, (size_t)4 * MAX_RED_BLOCKS + 64 * MAX_RED_BLOCKS));
+  ensure_part(w, 0);
   w.r64.alloc(n);
   const bool mixed = o.precision == OCMP_MIXED;
   if (mixed && P && P->kind == OCMP_PRE_JACOBI)
@@ -961,8 +1256,10 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     k_to_f32<<<nblk(A->nnz), NTHR, 0, A->ctx->stream>>>(A->nnz, A->d_val.p, A->d_val32.p);
   }
   const double bn = norm_masked(A, b, w);
-  const double bref = bn > 0.0 ? bn : 1.0;
-  double rn = residual64(A, b, x, w.r64.p, w);
+  double rn = residual64(A, P, b, x, w.r64.p, w);
+  // ||b|| on free dofs is the reference; a problem driven only by Dirichlet data (b_F = 0)
+  // measures against the initial residual instead
+  const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
   st_.initial_rel_residual = rn / bref;
   st_.rel_residual = rn / bref;
   const int check = o.check_every > 0 ? o.check_every : 5;
@@ -979,7 +1276,7 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
       used = gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
     total += used;
     const double rn_old = rn;
-    rn = residual64(A, b, x, w.r64.p, w);
+    rn = residual64(A, P, b, x, w.r64.p, w);
     st_.rel_residual = rn / bref;
     if (used == 0 || (mixed && o.max_outer > 0 && outer >= o.max_outer)) break;
     if (!(rn < rn_old) && used < m) break;  // no progress and the cycle thought it had converged
@@ -998,11 +1295,13 @@ static void solve_richardson(Matrix *A, Precond *P, const double *b, double *x, 
   const int64_t n = A->n;
   OCMP_REQUIRE(P && (P->kind == OCMP_PRE_JACOBI || P->kind == OCMP_PRE_BLOCK_JACOBI),
                "Richardson needs a preconditioner");
-  w.part.alloc(std::max<size_t>(w.part.n, (size_t)4 * MAX_RED_BLOCKS + 64 * MAX_RED_BLOCKS));
+  ensure_part(w, 0);
   w.r64.alloc(n);
   const double bn = norm_masked(A, b, w);
-  const double bref = bn > 0.0 ? bn : 1.0;
-  double rn = residual64(A, b, x, w.r64.p, w);
+  double rn = residual64(A, P, b, x, w.r64.p, w);
+  // ||b|| on free dofs is the reference; a problem driven only by Dirichlet data (b_F = 0)
+  // measures against the initial residual instead
+  const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
   st_.initial_rel_residual = rn / bref;
   int it = 0;
   while (rn > o.tol * bref && it < o.maxit && std::isfinite(rn)) {
@@ -1018,7 +1317,7 @@ static void solve_richardson(Matrix *A, Precond *P, const double *b, double *x, 
       }
     }
     ++it;
-    rn = residual64(A, b, x, w.r64.p, w);
+    rn = residual64(A, P, b, x, w.r64.p, w);
   }
   st_.iters = it;
   st_.outer = 1;

@@ -1,0 +1,374 @@
+"""Weak forms of the OpenCMP models on the GPU path.
+
+Each model mirrors the reference class's ``construct_bilinear_time_ODE``,
+``construct_bilinear_time_coefficient`` and ``construct_linear`` term by term,
+but returns explicit integrator terms (opencmp_amd.core) instead of NGSolve
+symbolic forms. Coefficients are constants, per-element arrays, closed-form
+functions ``(FN_*, params)`` or grid functions.
+
+Special functions follow opencmp/helpers/ngsolve_.py:48-70: ``alpha = nu / h``
+with ``nu = ipc * p^2`` (opencmp/models/base_model.py:150) and ``h`` the
+element size |det J|^(1/dim); ``dt`` multiplies every term as in the reference
+(its ``Parameter(1.0)`` for stationary solves).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import core as C
+from .mesh import Mesh as HostMesh
+
+Fn = Tuple[int, Sequence[float]]
+
+
+def const(v: float) -> Fn:
+    return (L.FN_CONST, (float(v),))
+
+
+ZERO = const(0.0)
+
+
+class Model:
+    """Common part of the GPU models (opencmp/models/base_model.py:39-987).
+
+    Parameters mirror the reference config: ``interp_ord``, ``DG``, ``ipc``
+    (interior penalty coefficient), boundary conditions as dicts
+    {marker_name: Fn}, and the [SOLVER] keys (``linear_solver``,
+    ``preconditioner``, ``linear_tolerance``, ``linear_max_iterations``) plus the
+    GPU-path keys ``precision`` (fp64|mixed) and ``restart``.
+    """
+
+    #: names of the model components in compound-space order
+    components: List[str] = ['u']
+    nonlinear = False
+
+    def __init__(self, mesh: HostMesh, interp_ord: int = 2, DG: bool = False, ipc: float = 10.0,
+                 dirichlet: Optional[Dict[str, Dict[str, Fn]]] = None, ctx: Optional[C.Context] = None,
+                 linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
+                 linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
+                 inner_tol: float = 0.0):
+        self.ctx = ctx or C.Context.default()
+        self.host_mesh = mesh
+        self.mesh = C.Mesh(mesh, self.ctx)
+        self.interp_ord = interp_ord
+        self.DG = DG
+        self.ipc = ipc
+        self.nu = ipc * interp_ord ** 2           # opencmp/models/base_model.py:150
+        self.dirichlet = dirichlet or {}
+        if linear_solver == 'default':
+            linear_solver = 'direct'
+        if linear_solver == 'direct':
+            # the GPU path has no sparse direct factorisation; the reference's default
+            # 'direct' maps to a preconditioned Krylov solve to a tolerance near machine precision
+            linear_solver = 'GMRes' if self._indefinite_or_nonsymmetric() else 'CG'
+            linear_tolerance = min(linear_tolerance, 1e-13)
+        self.linear_solver = linear_solver
+        self.preconditioner = 'local' if preconditioner in ('default', 'local') else preconditioner
+        if self.preconditioner in (None, 'None') and linear_solver == 'CG':
+            raise ValueError('Preconditioner cannot be None if using CG or MinRes solvers.')
+        self.linear_tolerance = linear_tolerance
+        self.linear_max_iterations = linear_max_iterations
+        self.precision = precision
+        self.restart = restart
+        self.inner_tol = inner_tol
+        self.fes = self._construct_fes()
+
+    # ---------------------------------------------------------------- hooks
+    def _indefinite_or_nonsymmetric(self) -> bool:
+        return False
+
+    def _construct_fes(self) -> C.FESpace:
+        raise NotImplementedError
+
+    def dirichlet_names(self, comp: str) -> str:
+        return '|'.join(self.dirichlet.get(comp, {}).keys())
+
+    def construct_bilinear_time_ODE(self, dt: float = 1.0) -> list:
+        return []
+
+    def construct_bilinear_time_coefficient(self, dt: float = 1.0) -> list:
+        return []
+
+    def construct_linear(self, dt: float = 1.0) -> list:
+        return []
+
+    def time_derivative_terms(self) -> list:
+        """U.V mass terms of every time-differentiated component (opencmp/models/base_model.py:418-497)."""
+        out = []
+        for comp in self.time_derivative_components():
+            for b in self.fes.comp_blocks[self.components.index(comp)]:
+                out.append(C.Vol('mass', b, b, coef=1.0, weight_gf=self._time_weight()))
+        return out
+
+    def time_derivative_components(self) -> List[str]:
+        return list(self.components)
+
+    def _time_weight(self):
+        return None
+
+    # ---------------------------------------------------------------- boundary conditions, solve
+    def apply_dirichlet_bcs_to(self, gfu: C.GridFunction) -> None:
+        """gfu.Set(g, definedon=Boundaries(...)) per component (opencmp/models/base_model.py:321-341)."""
+        for comp, bcs in self.dirichlet.items():
+            ci = self.components.index(comp)
+            if not isinstance(self.fes.comps[ci], C.H1):
+                continue
+            for marker, g in bcs.items():
+                gfu.Set(g, marker, component=ci)
+
+    def construct_preconditioners(self, a_assembled: C.BilinearForm) -> Optional[C.Preconditioner]:
+        """ngs.Preconditioner(a, 'local') (opencmp/models/base_model.py:365-383)."""
+        if self.preconditioner in (None, 'None'):
+            return None
+        return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
+
+    def linear_solve(self, a_assembled: C.BilinearForm, L_assembled: C.LinearForm,
+                     precond: Optional[C.Preconditioner], gfu: C.GridFunction) -> C.SolveResult:
+        """Model.linear_solve (opencmp/models/base_model.py:886-947): solve on the free dofs starting
+        from gfu (which carries the Dirichlet lift). Unlike the reference, returns the solver status."""
+        return C.solve(a_assembled, L_assembled.vec, gfu.vec, precond, self.linear_solver,
+                       tol=self.linear_tolerance, maxit=self.linear_max_iterations, precision=self.precision,
+                       restart=self.restart, inner_tol=self.inner_tol)
+
+    def solve_single_step(self, a: C.BilinearForm, Lf: C.LinearForm, pre, gfu: C.GridFunction,
+                          reassemble=None) -> C.SolveResult:
+        return self.linear_solve(a, Lf, pre, gfu)
+
+    # ---------------------------------------------------------------- error norms
+    def l2_error(self, gfu: C.GridFunction, comp: str, exact: Fn = ZERO, block: int = 0) -> Tuple[float, float]:
+        """(L2 error, H1-seminorm error) of one scalar block against a closed form (calc_error)."""
+        import ctypes
+        b = self.fes.comp_blocks[self.components.index(comp)][block]
+        c = (ctypes.c_double * 10)(*([float(v) for v in exact[1]] + [0.0] * (10 - len(exact[1]))))
+        out = (ctypes.c_double * 2)()
+        L.check(self.ctx.lib.ocmp_error_norms(self.fes.h, b, ctypes.c_void_p(gfu.vec.data_ptr()), None, exact[0],
+                                              c, out), 'error_norms')
+        return out[0], out[1]
+
+    def l2_diff(self, a: C.GridFunction, b: C.GridFunction, comp: str) -> float:
+        """sqrt(sum over the component's blocks of int (a - b)^2) -- norm('l2_norm', W, u) of the Picard loop."""
+        import ctypes
+        c = (ctypes.c_double * 10)()
+        out = (ctypes.c_double * 2)()
+        tot = 0.0
+        for blk in self.fes.comp_blocks[self.components.index(comp)]:
+            L.check(self.ctx.lib.ocmp_error_norms(self.fes.h, blk, ctypes.c_void_p(a.vec.data_ptr()),
+                                                  ctypes.c_void_p(b.vec.data_ptr()), L.FN_CONST, c, out),
+                    'error_norms')
+            tot += out[0] ** 2
+        return float(np.sqrt(tot))
+
+
+class Poisson(Model):
+    """opencmp/models/poisson.py:28-168: -div(dc grad u) = f, H1 (CG) or L2 with SIPG (DG)."""
+    components = ['u']
+
+    def __init__(self, mesh, dc: float = 1.0, f: Fn = ZERO, **kw):
+        self.dc = dc
+        self.f = f
+        super().__init__(mesh, **kw)
+
+    def _construct_fes(self):
+        if self.DG:
+            return C.FESpace(self.mesh, [C.L2(self.interp_ord)], dgjumps=True)
+        return C.FESpace(self.mesh, [C.H1(self.interp_ord, self.dirichlet_names('u'))])
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        a = [C.Vol('gradgrad', 0, 0, coef=dt * self.dc)]                      # poisson.py:78
+        if self.DG and self.dirichlet.get('u'):
+            # Dirichlet Nitsche terms (poisson.py:81-91) are part of the facet term below
+            pass
+        return a
+
+    def construct_bilinear_time_coefficient(self, dt=1.0):
+        if not self.DG:
+            return []                                                         # poisson.py:124
+        dmask = self.mesh.marker_mask(self.dirichlet_names('u')) if self.dirichlet.get('u') else 0
+        # poisson.py:106-121 (interior SIPG) and :81-91 (boundary Nitsche), no convection
+        return [C.DGFacet(0, eps=dt * self.dc, alpha=self.nu, dirichlet=dmask, upwind=False)]
+
+    def construct_linear(self, dt=1.0):
+        Lf = [C.Source(0, self.f, coef=dt)]                                   # poisson.py:133
+        if self.DG:
+            for marker, g in self.dirichlet.get('u', {}).items():             # poisson.py:136-146
+                Lf.append(C.DGDirichletData(0, eps=dt * self.dc, alpha=self.nu, wind=(0.0, 0.0),
+                                            dirichlet=self.mesh.marker_mask(marker), fn=g))
+        return Lf
+
+
+class DGTransport(Model):
+    """One species of MultiComponentINS with a fixed velocity, DG (opencmp/models/multi_component_ins.py):
+
+    D grad c . grad r - c w . grad r                             (bulk, :223-224, :258-260)
+    skeleton SIPG + upwind                                       (:185-197)
+    Dirichlet Nitsche + upwind, upwind on other boundaries       (:262-301)
+    L = f r + Dirichlet data                                     (:343-349, :371-386)
+    """
+    components = ['c']
+
+    def __init__(self, mesh, D: float = 1e-3, wind=(1.0, 0.0), f: Fn = const(1.0), **kw):
+        self.D = D
+        self.wind = tuple(float(v) for v in wind)
+        self.f = f
+        kw.setdefault('DG', True)
+        kw.setdefault('linear_solver', 'GMRes')
+        super().__init__(mesh, **kw)
+        if not self.DG:
+            raise NotImplementedError('the GPU transport model is implemented with DG (L2) spaces')
+
+    def _indefinite_or_nonsymmetric(self):
+        return True
+
+    def _construct_fes(self):
+        return C.FESpace(self.mesh, [C.L2(self.interp_ord)], dgjumps=True)
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        w = self.wind
+        return [C.Vol('gradgrad', 0, 0, coef=dt * self.D),
+                C.Vol('conv', 0, 0, coef=-dt, wind=(w[0], w[1], 0.0))]
+
+    def construct_bilinear_time_coefficient(self, dt=1.0):
+        dmask = self.mesh.marker_mask(self.dirichlet_names('c')) if self.dirichlet.get('c') else 0
+        return [C.DGFacet(0, eps=self.D, alpha=self.nu, wind=self.wind, dirichlet=dmask, coef=dt)]
+
+    def construct_linear(self, dt=1.0):
+        Lf = [C.Source(0, self.f, coef=dt)]
+        for marker, g in self.dirichlet.get('c', {}).items():
+            Lf.append(C.DGDirichletData(0, eps=self.D, alpha=self.nu, wind=self.wind,
+                                        dirichlet=self.mesh.marker_mask(marker), fn=g, coef=dt))
+        return Lf
+
+
+class Stokes(Model):
+    """opencmp/models/stokes.py:28-147 (CG Taylor-Hood): kv grad u : grad v - div(u) q - div(v) p - 1e-10 p q."""
+    components = ['u', 'p']
+
+    def __init__(self, mesh, kv: float = 1.0, f=None, **kw):
+        self.kv = kv
+        self.f = f
+        kw.setdefault('linear_solver', 'GMRes')
+        super().__init__(mesh, **kw)
+
+    def _indefinite_or_nonsymmetric(self):
+        return True
+
+    def _construct_fes(self):
+        d = self.host_mesh.dim
+        return C.FESpace(self.mesh, [C.H1(self.interp_ord, self.dirichlet_names('u'), ncomp=d),
+                                     C.H1(self.interp_ord - 1, self.dirichlet_names('p'))])
+
+    def time_derivative_components(self):
+        return ['u']
+
+    def _velocity_terms(self, dt):
+        ub = self.fes.comp_blocks[0]
+        return [C.Vol('gradgrad', b, b, coef=dt * self.kv) for b in ub]
+
+    def _pressure_terms(self, dt, weight=None):
+        ub = self.fes.comp_blocks[0]
+        pb = self.fes.comp_blocks[1][0]
+        t = []
+        for k, b in enumerate(ub):
+            t.append(C.Vol('dtrial', b, pb, coef=-dt, dir=k, weight_gf=weight))       # -div(u) q
+            t.append(C.Vol('dtest', pb, b, coef=-dt, dir=k, weight_gf=weight))        # -div(v) p
+        t.append(C.Vol('mass', pb, pb, coef=-1e-10 * dt, weight_gf=weight))          # -1e-10 p q
+        return t
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        return self._velocity_terms(dt)
+
+    def construct_bilinear_time_coefficient(self, dt=1.0):
+        return self._pressure_terms(dt)
+
+    def construct_linear(self, dt=1.0):
+        if self.f is None:
+            return []
+        ub = self.fes.comp_blocks[0]
+        return [C.Source(b, fk, coef=dt) for b, fk in zip(ub, self.f)]
+
+
+class INS(Stokes):
+    """opencmp/models/ins.py (CG, Oseen linearisation): adds -(u (x) w) : grad v with the wind w the
+    previous iterate (:191-195); solved by the Picard loop of solve_single_step (:323-359)."""
+    nonlinear = True
+
+    def __init__(self, mesh, kv: float = 1e-3, nonlinear_tolerance=(1e-8, 1e-8), nonlinear_max_iters: int = 10,
+                 **kw):
+        self.abs_tol, self.rel_tol = nonlinear_tolerance
+        self.nonlinear_max_iters = nonlinear_max_iters
+        super().__init__(mesh, kv=kv, **kw)
+        self.W = C.GridFunction(self.fes)    # linearisation point (opencmp/models/ins.py:109-116)
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        ub = self.fes.comp_blocks[0]
+        terms = self._velocity_terms(dt)
+        for b in ub:
+            # -(u (x) w) : grad v = -sum_i u_i (w . grad v_i)
+            terms.append(C.Vol('conv', b, b, coef=-dt, wind_gf=(self.W, ub)))
+        return terms
+
+    def solve_single_step(self, a, Lf, pre, gfu, reassemble=None):
+        """Picard iteration (opencmp/models/ins.py:323-359): re-assemble with the current wind,
+        solve, stop when ||W - u||_L2 < abs + rel * ||u||_L2."""
+        it = 1
+        res = None
+        while True:
+            self.apply_dirichlet_bcs_to(gfu)
+            if reassemble is not None:
+                reassemble()
+            res = self.linear_solve(a, Lf, pre, gfu)
+            err = self.l2_diff(self.W, gfu, 'u')
+            unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
+            it += 1
+            self.W.vec.copy_(gfu.vec)
+            if err < self.abs_tol + self.rel_tol * unorm or it > self.nonlinear_max_iters:
+                break
+        self.picard_iterations = it - 1
+        return res
+
+
+class StokesDIM(Stokes):
+    """opencmp/models/stokes_dim.py:28-153 with phi = 1 in the fluid, 0 in the solid:
+
+    kv grad u : grad v phi + alpha u.v (1 - phi)
+    (-div(u) q - div(v) p - 1e-10 p q) phi - p div(v) (1 - phi)
+    L = f.v phi
+    """
+
+    def __init__(self, mesh, phi: torch.Tensor, phi_order: int = 1, penalty: float = 1e4, **kw):
+        self.phi_order = phi_order
+        self.penalty = penalty
+        self._phi_vec = phi
+        super().__init__(mesh, **kw)
+        self.phi_space = C.FESpace(self.mesh, [C.H1(phi_order)])
+        self.phi = C.GridFunction(self.phi_space, phi.to(self.ctx.torch_device, torch.float64))
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        ub = self.fes.comp_blocks[0]
+        wphi = (self.phi, 0, L.WEIGHT_FIELD)
+        w1m = (self.phi, 0, L.WEIGHT_ONE_MINUS_FIELD)
+        t = [C.Vol('gradgrad', b, b, coef=dt * self.kv, weight_gf=wphi) for b in ub]
+        t += [C.Vol('mass', b, b, coef=dt * self.penalty, weight_gf=w1m) for b in ub]
+        return t
+
+    def construct_bilinear_time_coefficient(self, dt=1.0):
+        ub = self.fes.comp_blocks[0]
+        pb = self.fes.comp_blocks[1][0]
+        t = self._pressure_terms(dt, weight=(self.phi, 0, L.WEIGHT_FIELD))
+        for k, b in enumerate(ub):
+            t.append(C.Vol('dtest', pb, b, coef=-dt, dir=k, weight_gf=(self.phi, 0, L.WEIGHT_ONE_MINUS_FIELD)))
+        return t
+
+    def construct_linear(self, dt=1.0):
+        if self.f is None:
+            return []
+        ub = self.fes.comp_blocks[0]
+        return [C.Source(b, fk, coef=dt, weight_gf=(self.phi, 0, L.WEIGHT_FIELD)) for b, fk in zip(ub, self.f)]
+
+    def _time_weight(self):
+        return (self.phi, 0, L.WEIGHT_FIELD)                 # opencmp/solvers/time_integration_schemes.py:678-681

@@ -1,0 +1,124 @@
+"""Solver drivers of the GPU path: the caller side of the assemble+solve boundary.
+
+Mirrors opencmp/solvers/: ``Solver._assemble`` (base_solver.py:368-377),
+``StationarySolver`` (stationary.py:29-85) and the implicit-Euler step of
+``TransientMultiStepSolver`` (transient_multistep.py:32-169) built by
+``implicit_euler`` (time_integration_schemes.py:79-132, :663-685):
+
+    A_step = M + dt (A_ode + A_coef),   L_step = dt L + M u_n
+
+Differences from the reference, all on the performance side: the stationary
+path assembles once (the reference assembles at :812 and again at :620), and
+a transient linear model with time-independent coefficients keeps its
+assembled step matrix instead of re-assembling every step; the mass matrix M is
+assembled once and applied as a matrix-vector product for M u_n.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from typing import Optional
+
+import torch
+
+from . import _lib as L
+from . import core as C
+from .models import Model
+
+
+class StationarySolver:
+    def __init__(self, model: Model):
+        self.model = model
+        self.gfu = C.GridFunction(model.fes)
+        self.a = None
+        self.L = None
+        self.pre = None
+        self.result = None
+        self.timings = {}
+
+    def _create_linear_and_bilinear_forms(self):
+        m = self.model
+        self.a = C.BilinearForm(m.fes, m.construct_bilinear_time_coefficient(1.0) + m.construct_bilinear_time_ODE(1.0))
+        self.L = C.LinearForm(m.fes, m.construct_linear(1.0))
+
+    def _create_preconditioners(self):
+        self.pre = self.model.construct_preconditioners(self.a)
+
+    def _assemble(self):
+        """Solver._assemble: a.Assemble(), L.Assemble(), preconditioner.Update()."""
+        self.a.Assemble()
+        self.L.Assemble()
+        if self.pre is not None:
+            self.pre.Update()
+
+    def solve(self) -> C.GridFunction:
+        t0 = time.perf_counter()
+        self._create_linear_and_bilinear_forms()
+        self._create_preconditioners()
+        t1 = time.perf_counter()
+        self._assemble()
+        self.model.apply_dirichlet_bcs_to(self.gfu)
+        reassemble = self._assemble if self.model.nonlinear else None
+        self.result = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble)
+        torch.cuda.synchronize()
+        self.timings = {'setup_s': t1 - t0, 'assemble_solve_s': time.perf_counter() - t1}
+        return self.gfu
+
+
+class ImplicitEulerSolver:
+    """Fixed-step implicit Euler (time_integration_schemes.py:79-132)."""
+
+    def __init__(self, model: Model, dt: float, t_range=(0.0, 1.0), gfu0: Optional[C.GridFunction] = None,
+                 reassemble_each_step: Optional[bool] = None):
+        self.model = model
+        self.dt = dt
+        self.t0, self.t1 = t_range
+        self.gfu = C.GridFunction(model.fes)
+        self.gfu_prev = C.GridFunction(model.fes)
+        if gfu0 is not None:
+            self.gfu_prev.vec.copy_(gfu0.vec)
+        fes = model.fes
+        mass = model.time_derivative_terms()
+        self.a = C.BilinearForm(fes, mass + model.construct_bilinear_time_coefficient(dt) +
+                                model.construct_bilinear_time_ODE(dt))
+        self.M = C.BilinearForm(fes, mass)
+        self.L = C.LinearForm(fes, model.construct_linear(dt))
+        self.pre = model.construct_preconditioners(self.a)
+        self.tmp = model.ctx.zeros(fes.ndof)
+        self.reassemble_each_step = model.nonlinear if reassemble_each_step is None else reassemble_each_step
+        self.M.Assemble()
+        self._assemble_a()
+        self.step_results = []
+
+    def _assemble_a(self):
+        self.a.Assemble()
+        if self.pre is not None:
+            self.pre.Update()
+
+    def _rhs(self):
+        """L_step = dt L + M u_n (time_integration_schemes.py:663-685)."""
+        self.L.Assemble()
+        self.M.mult(self.gfu_prev.vec, self.tmp)
+        n = self.model.fes.ndof
+        L.check(self.model.ctx.lib.ocmp_axpby(self.model.ctx.h, n, 1.0, ctypes.c_void_p(self.tmp.data_ptr()), 1.0,
+                                              ctypes.c_void_p(self.L.vec.data_ptr())), 'axpby')
+
+    def step(self):
+        self._rhs()
+        # initial guess: previous solution (what the reference's gfu holds), then the Dirichlet lift
+        self.gfu.vec.copy_(self.gfu_prev.vec)
+        if self.reassemble_each_step and not self.model.nonlinear:
+            self._assemble_a()
+        self.model.apply_dirichlet_bcs_to(self.gfu)
+        reassemble = self._assemble_a if self.model.nonlinear else None
+        res = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble)
+        self.step_results.append(res)
+        self.gfu_prev.vec.copy_(self.gfu.vec)
+        return res
+
+    def solve(self, nsteps: Optional[int] = None) -> C.GridFunction:
+        if nsteps is None:
+            nsteps = int(round((self.t1 - self.t0) / self.dt))
+        for _ in range(nsteps):
+            self.step()
+        return self.gfu_prev

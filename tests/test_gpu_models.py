@@ -1,0 +1,151 @@
+"""Model/solver drivers of the GPU path against the oracle and the reference's known answers."""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem
+from opencmp_amd import _lib as L
+from opencmp_amd.mesh import unit_square, unit_cube
+from opencmp_amd.io import load_mesh
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'mesh_files')
+PI = np.pi
+
+
+def say(*a):
+    print('[gpu-test]', *a, flush=True)
+
+
+def test_poisson_cg_stationary_config1(gpu_ctx):
+    """Config 1 through StationarySolver: errors equal the oracle's for the same discretisation."""
+    from opencmp_amd.models import Poisson, const
+    from opencmp_amd.solvers import StationarySolver
+    say('poisson cg config1 via StationarySolver')
+    hm = unit_square(32)
+    allb = {b: const(0.0) for b in hm.bnd_names}
+    m = Poisson(hm, dc=1.0, f=(L.FN_SINPROD, (2 * PI * PI,)), interp_ord=2, dirichlet={'u': allb},
+                linear_solver='CG', linear_tolerance=1e-10, ctx=gpu_ctx)
+    s = StationarySolver(m)
+    gfu = s.solve()
+    assert s.result.converged
+    e_l2, e_h1 = m.l2_error(gfu, 'u', (L.FN_SINPROD, (1.0,)))
+    t = fem.Topology(hm)
+    sp = fem.Space(t, 'H1', 2, dirichlet_mask=hm.marker_mask(hm.bnd_names))
+    x = gfu.numpy()
+    o_l2, o_h1 = fem.l2_error(sp, x, lambda x, y: np.sin(PI * x) * np.sin(PI * y),
+                              grad_exact=[lambda x, y: PI * np.cos(PI * x) * np.sin(PI * y),
+                                          lambda x, y: PI * np.sin(PI * x) * np.cos(PI * y)])
+    say('  L2', e_l2, o_l2, 'H1', e_h1, o_h1)
+    assert abs(e_l2 - o_l2) <= 1e-10 * o_l2 and abs(e_h1 - o_h1) <= 1e-10 * o_h1
+    assert 8.0e-6 < e_l2 < 9.2e-6      # oracle value at N=32, P2: 8.6e-6
+
+
+def test_poisson_dg_reproduces_quadratic(gpu_ctx):
+    """DG SIPG Poisson (opencmp/models/poisson.py:81-158) is consistent: exact for a global quadratic."""
+    from opencmp_amd.models import Poisson
+    from opencmp_amd.solvers import StationarySolver
+    say('poisson dg quadratic')
+    hm = unit_square(6)
+    c = [1.0, 0.5, -1.0, 1.0, 0.5, 2.0]        # u = 1 + x/2 - y + x^2 + xy/2 + 2y^2, -lap u = -6
+    g = (L.FN_POLY2, c)
+    m = Poisson(hm, dc=1.0, f=(L.FN_CONST, (-6.0,)), interp_ord=2, DG=True,
+                dirichlet={'u': {b: g for b in hm.bnd_names}}, linear_solver='GMRes', linear_tolerance=1e-12,
+                ctx=gpu_ctx)
+    s = StationarySolver(m)
+    gfu = s.solve()
+    assert s.result.converged
+    e_l2, _ = m.l2_error(gfu, 'u', g)
+    assert e_l2 < 1e-9, e_l2
+
+
+def test_reference_transient_poisson_known_answer(gpu_ctx):
+    """pytests/full_system/poisson/test_poisson.py:60-78 on the GPU: unit_square_coarse.vol, H1 P3,
+    implicit Euler dt=1e-3 to t=1, u0 = x(1-x); printed L2 error 9e-6 at rtol 3. Also equal to the oracle."""
+    from opencmp_amd.models import Poisson, const
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    from opencmp_amd import core as C
+    say('transient poisson known answer')
+    hm = load_mesh(os.path.join(GOLD, 'unit_square_coarse.vol'))
+    m = Poisson(hm, dc=1.0, f=const(0.0), interp_ord=3, dirichlet={'u': {'left': const(0.0), 'right': const(0.0)}},
+                linear_solver='CG', linear_tolerance=1e-13, ctx=gpu_ctx)
+    # initial condition: L2 projection of x(1-x) (in P3, so exact)
+    Mf = C.BilinearForm(m.fes, [C.Vol('mass', 0, 0)])
+    Mf.Assemble()
+    b = C.LinearForm(m.fes, [C.Source(0, (L.FN_POLY2, [0, 1.0, 0, -1.0, 0, 0]))])
+    b.Assemble()
+    u0 = C.GridFunction(m.fes)
+    pre = C.Preconditioner(Mf, 'jacobi')
+    pre.Update()
+    r = C.solve(Mf, b.vec, u0.vec, pre, 'CG', tol=1e-14)
+    assert r.converged
+    s = ImplicitEulerSolver(m, dt=1e-3, t_range=(0.0, 1.0), gfu0=u0)
+    gfu = s.solve()
+    err, _ = m.l2_error(gfu, 'u', const(0.0))
+    say('  L2 norm at t=1:', err)
+    assert np.isclose(err, 9e-6, rtol=3, atol=1e-12)
+    # the oracle's value for the same discretisation (tests/test_oracle.py restates the loop)
+    t = fem.Topology(hm)
+    sp = fem.Space(t, 'H1', 3, dirichlet_mask=hm.marker_mask('left|right'))
+    assert fem.l2_error(sp, gfu.numpy(), 0.0) == pytest.approx(err, rel=1e-12)
+
+
+def test_stokes_taylor_hood_matches_oracle(gpu_ctx):
+    """Stationary Stokes, Taylor-Hood P2/P1, parabolic inflow (opencmp/models/stokes.py:43-129)."""
+    from opencmp_amd.models import Stokes, const
+    from opencmp_amd.solvers import StationarySolver
+    say('stokes taylor-hood')
+    hm = unit_square(6)
+    inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]    # u = (4y(1-y), 0) on left
+    wall = [const(0.0), const(0.0)]
+    m = Stokes(hm, kv=1.0, interp_ord=2,
+               dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
+               linear_solver='GMRes', preconditioner='None', linear_tolerance=1e-11, restart=200, ctx=gpu_ctx)
+    s = StationarySolver(m)
+    gfu = s.solve()
+    say('  ', s.result)
+    assert s.result.converged
+    # oracle: same forms, direct solve with the same lift
+    t = fem.Topology(hm)
+    dm = hm.marker_mask('left|top|bottom')
+    su = fem.Space(t, 'H1', 2, ncomp=2, dirichlet_mask=dm)
+    sp = fem.Space(t, 'H1', 1)
+    cs = fem.Compound([su, sp])
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=1.0),
+                            dict(kind='dtrial', trial=0, test=1, coef=-1.0, dir=-1),
+                            dict(kind='dtest', trial=1, test=0, coef=-1.0, dir=-1),
+                            dict(kind='mass', trial=1, test=1, coef=-1e-10)])
+    K = A.matrix()
+    assert np.linalg.norm(s.a.values() - K.data) <= 1e-13 * np.linalg.norm(K.data)
+    x0 = gfu.numpy().copy()
+    # the lift values the GPU set must be the oracle's lift of the same data
+    lift = np.zeros(cs.ndof)
+    lift[:su.n1] = fem.h1_lift(fem.Space(t, 'H1', 2, dirichlet_mask=hm.marker_mask('left')),
+                               lambda x, y: 4 * y * (1 - y), hm.marker_mask('left'))
+    bd = ~cs.freedofs
+    assert np.abs(x0[bd] - lift[bd]).max() < 1e-13
+    xd = fem.solve_free(K, np.zeros(cs.ndof), lift, cs.freedofs)
+    uu = slice(0, 2 * su.n1)
+    assert np.abs(x0[uu] - xd[uu]).max() < 1e-7 * np.abs(xd[uu]).max()
+
+
+def test_oseen_picard_converges(gpu_ctx):
+    """Stationary INS by Oseen/Picard (opencmp/models/ins.py:323-379) at low Reynolds number:
+    the Picard loop converges and the answer is close to the Stokes solution."""
+    from opencmp_amd.models import INS, const
+    from opencmp_amd.solvers import StationarySolver
+    say('oseen picard')
+    hm = unit_square(6)
+    inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
+    wall = [const(0.0), const(0.0)]
+    m = INS(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
+            linear_solver='GMRes', preconditioner='None', linear_tolerance=1e-11, restart=200,
+            nonlinear_tolerance=(1e-10, 1e-10), nonlinear_max_iters=20, ctx=gpu_ctx)
+    s = StationarySolver(m)
+    s.solve()
+    say('  picard iterations', m.picard_iterations, s.result)
+    assert s.result.converged and 2 <= m.picard_iterations < 20
