@@ -36,36 +36,66 @@ def log(*a):
     print('[bench]', *a, file=sys.stderr, flush=True)
 
 
-def build_problem(n, ctx):
+def build_problem(config, n, ctx):
+    """Returns (host mesh, gpu mesh, space, bilinear form, linear form, solver name, description)."""
+    import torch
     from opencmp_amd import core as C, _lib as L
-    from opencmp_amd.mesh import unit_square
-    hm = unit_square(n)
-    gm = C.Mesh(hm, ctx)
-    gs = C.FESpace(gm, [C.L2(2)], dgjumps=True)
-    eps, w, alpha = 1e-3, (np.cos(0.3), np.sin(0.3)), 10.0 * 2 * 2
-    allm = hm.marker_mask(hm.bnd_names)
-    a = C.BilinearForm(gs)
-    a += C.Vol('gradgrad', 0, 0, coef=eps)
-    a += C.Vol('conv', 0, 0, coef=-1.0, wind=(w[0], w[1], 0.0))
-    a += C.DGFacet(0, eps=eps, alpha=alpha, wind=(w[0], w[1]), dirichlet=allm)
-    f = C.LinearForm(gs)
-    f += C.Source(0, (L.FN_CONST, (1.0,)))
-    return hm, gm, gs, a, f
+    from opencmp_amd.mesh import unit_square, unit_cube
+    if config == 2:
+        hm = unit_square(n)
+        gm = C.Mesh(hm, ctx)
+        gs = C.FESpace(gm, [C.L2(2)], dgjumps=True)
+        eps, w, alpha = 1e-3, (np.cos(0.3), np.sin(0.3)), 10.0 * 2 * 2
+        allm = hm.marker_mask(hm.bnd_names)
+        a = C.BilinearForm(gs)
+        a += C.Vol('gradgrad', 0, 0, coef=eps)
+        a += C.Vol('conv', 0, 0, coef=-1.0, wind=(w[0], w[1], 0.0))
+        a += C.DGFacet(0, eps=eps, alpha=alpha, wind=(w[0], w[1]), dirichlet=allm)
+        f = C.LinearForm(gs)
+        f += C.Source(0, (L.FN_CONST, (1.0,)))
+        desc = (f'DG-P2 convection-diffusion (SIPG+upwind, eps 1e-3), {n}x{n} split unit square, '
+                f'{hm.nel} triangles, GMRES(50) + element-block Jacobi')
+        return hm, gm, gs, a, f, 'GMRes', desc
+    if config == 3:
+        hm = unit_cube(n)
+        gm = C.Mesh(hm, ctx)
+        gs = C.FESpace(gm, [C.H1(3, '|'.join(hm.bnd_names))])
+        # kappa = exp(g), g ~ N(0,1) i.i.d. per element in generator order, seed 0
+        kappa = np.exp(np.random.default_rng(0).standard_normal(hm.nel))
+        a = C.BilinearForm(gs)
+        a += C.Vol('gradgrad', 0, 0, elem_coef=torch.tensor(kappa))
+        f = C.LinearForm(gs)
+        f += C.Source(0, (L.FN_CONST, (1.0,)))
+        desc = (f'3D diffusion -div(kappa grad u) = 1, kappa = exp(N(0,1)) per element, {n}^3 cubes x 6 tets '
+                f'({hm.nel} tets), H1 P3, u = 0 on the boundary, Jacobi-PCG')
+        return hm, gm, gs, a, f, 'CG', desc
+    if config == 1:
+        hm = unit_square(n)
+        gm = C.Mesh(hm, ctx)
+        gs = C.FESpace(gm, [C.H1(2, '|'.join(hm.bnd_names))])
+        a = C.BilinearForm(gs)
+        a += C.Vol('gradgrad', 0, 0)
+        f = C.LinearForm(gs)
+        f += C.Source(0, (L.FN_SINPROD, (2 * np.pi ** 2,)))
+        desc = f'Poisson H1 P2, {n}x{n} split unit square ({hm.nel} triangles), u = sin(pi x) sin(pi y), Jacobi-PCG'
+        return hm, gm, gs, a, f, 'CG', desc
+    raise SystemExit(f'unknown config {config}')
 
 
-def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False):
+def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes'):
     import torch
     pre = C.Preconditioner(a, 'local', precision=precision)
     x = C.GridFunction(gs)
     res = None
+    inner = (1e-5 if solver == 'GMRes' else 1e-4) if precision == 'mixed' else 0.0
 
     def step():
         a.Assemble()
         f.Assemble()
         pre.Update()
         x.vec.zero_()
-        return C.solve(a, f.vec, x.vec, pre, 'GMRes', tol=tol, restart=restart, precision=precision,
-                       maxit=200000, inner_tol=1e-5 if precision == 'mixed' else 0.0)
+        return C.solve(a, f.vec, x.vec, pre, solver, tol=tol, restart=restart, precision=precision,
+                       maxit=500000, inner_tol=inner)
 
     for i in range(warmup):
         res = step()
@@ -89,42 +119,64 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
     return (t1 - t0) / steps, res, its, x.vec.clone(), rep
 
 
-def cpu_baseline(gpu_iters_fp64, n_full):
-    """Oracle (numpy/scipy FP64, 1 thread) timed on a bounded sample of the same workload.
+def cpu_baseline(config, gpu_iters_fp64, n_full):
+    """The oracle (numpy/scipy FP64, 1 thread) timed on a bounded sample of the same workload.
 
-    Sample: the 128x128 instance of the same discretisation; assembly is timed
-    in full and GMRES(50) + block Jacobi for 100 iterations; both scale linearly
-    in the number of elements/DOFs, so s/step at the full size is
-    t_asm * (n_full/128)^2 + t_iter * (n_full/128)^2 * (iterations of the FP64 GPU path)."""
+    Assembly (+ preconditioner setup) is timed in full on a smaller instance of the
+    same discretisation and a fixed number of Krylov iterations is timed on it;
+    both scale linearly in elements/DOFs, so s/step at the full size is
+    t_asm * scale + t_iter * scale * (iterations of the FP64 GPU path)."""
     from oracle import fem
-    from opencmp_amd.mesh import unit_square
-    ns = 128
-    eps, w = 1e-3, np.array([np.cos(0.3), np.sin(0.3)])
-    m = unit_square(ns)
-    t0 = time.perf_counter()
-    t = fem.Topology(m)
-    cs = fem.Compound([fem.Space(t, 'L2', 2)], dgjumps=True)
-    A_ = fem.Assembler(cs)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
-                             dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
-    fem.assemble_dg_facets(A_, 0, eps, 40.0, w, m.marker_mask(m.bnd_names), 0)
-    A = A_.matrix()
-    b = fem.assemble_source(cs, 0, 1.0)
-    Bi = fem.block_jacobi_inverse(A, 6)
-    t_asm = time.perf_counter() - t0
-    M = lambda v: np.einsum('eij,ej->ei', Bi, v.reshape(-1, 6)).ravel()
-    nit = 100
-    t0 = time.perf_counter()
-    fem.gmres_right(A, b, np.zeros(cs.ndof), M, 1e-30, m=50, maxit=nit)
-    t_it = (time.perf_counter() - t0) / nit
-    scale = (n_full / ns) ** 2
+    from opencmp_amd.mesh import unit_square, unit_cube
+    if config == 2:
+        ns, nit = 128, 100
+        eps, w = 1e-3, np.array([np.cos(0.3), np.sin(0.3)])
+        m = unit_square(ns)
+        t = fem.Topology(m)
+        cs = fem.Compound([fem.Space(t, 'L2', 2)], dgjumps=True)
+        A_ = fem.Assembler(cs)
+        t0 = time.perf_counter()
+        fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
+                                 dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
+        fem.assemble_dg_facets(A_, 0, eps, 40.0, w, m.marker_mask(m.bnd_names), 0)
+        A = A_.matrix()
+        b = fem.assemble_source(cs, 0, 1.0)
+        Bi = fem.block_jacobi_inverse(A, 6)
+        t_asm = time.perf_counter() - t0
+        M = lambda v: np.einsum('eij,ej->ei', Bi, v.reshape(-1, 6)).ravel()
+        t0 = time.perf_counter()
+        fem.gmres_right(A, b, np.zeros(cs.ndof), M, 1e-30, m=50, maxit=nit)
+        t_it = (time.perf_counter() - t0) / nit
+        scale = (n_full / ns) ** 2
+        what = f'{ns}x{ns} instance: assembly + block-Jacobi setup ({t_asm:.2f}s), {nit} GMRES(50) iterations'
+    else:
+        if config == 3:
+            ns, nit = 8, 200
+            m = unit_cube(ns)
+            p, coef = 3, np.exp(np.random.default_rng(0).standard_normal(m.nel))
+            f = 1.0
+        else:
+            ns, nit = n_full, 200
+            m = unit_square(ns)
+            p, coef = 2, 1.0
+            f = lambda x, y: 2 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y)
+        t = fem.Topology(m)
+        cs = fem.Compound([fem.Space(t, 'H1', p, dirichlet_mask=m.marker_mask(m.bnd_names))])
+        A_ = fem.Assembler(cs)
+        t0 = time.perf_counter()
+        fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=coef)])
+        A = A_.matrix()
+        b = fem.assemble_source(cs, 0, f)
+        t_asm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        fem.pcg_jacobi(A, b, np.zeros(cs.ndof), cs.freedofs, 1e-30, maxit=nit)
+        t_it = (time.perf_counter() - t0) / nit
+        scale = (n_full / ns) ** (m.dim)
+        what = f'{ns}^{m.dim} instance: assembly ({t_asm:.2f}s), {nit} Jacobi-PCG iterations'
     s_step = t_asm * scale + t_it * scale * gpu_iters_fp64
     return {'value': s_step, 'unit': 's/step', 'cores': 1, 'kind': 'port',
-            'sample': f'oracle/fem.py (numpy+scipy, FP64, 1 thread) on the {ns}x{ns} instance: full assembly '
-                      f'+ block-Jacobi setup ({t_asm:.2f}s) and {nit} GMRES(50) iterations ({t_it * 1e3:.2f} ms/iter), '
-                      f'scaled by DOFs ({scale:.0f}x) to {n_full}x{n_full} and by the FP64 GPU path\'s '
+            'sample': f'oracle/fem.py (numpy+scipy, FP64, 1 thread) on the {what} ({t_it * 1e3:.3f} ms/iter), '
+                      f'scaled by DOFs ({scale:.0f}x) to the full size and by the FP64 GPU path\'s '
                       f'{gpu_iters_fp64} iterations'}
 
 
@@ -133,7 +185,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--n', type=int, default=512)
+    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3))
+    ap.add_argument('--n', type=int, default=None, help='mesh cells per direction (defaults: 32 / 512 / 64)')
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=50)
     ap.add_argument('--no-cpu', action='store_true')
@@ -142,52 +195,52 @@ def main():
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
+    n = args.n or {1: 32, 2: 512, 3: 64}[args.config]
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
     t0 = time.perf_counter()
-    hm, gm, gs, a, f = build_problem(args.n, ctx)
+    hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
-    s64, r64, it64, x64, rep64 = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps,
-                                          args.warmup, timing=False)
-    smx, rmx, itmx, xmx, repmx = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps,
-                                          args.warmup, timing=False)
-    # instrumented step (HIP events around every launch) for the per-kernel roofline
-    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, 1, 0, timing=True)
-    _, _, _, _, rep_fp = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, timing=True)
+    kw = dict(solver=solver)
+    s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps, args.warmup, **kw)
+    smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps, args.warmup,
+                                      **kw)
+    # one more step of each path with HIP events around every launch: the per-kernel roofline
+    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, 1, 0, timing=True, **kw)
+    _, _, _, _, rep_fp = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, timing=True, **kw)
     err = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
-    kernels = {}
-    for k, (cnt, ms, byt) in rep.items():
-        kernels[k] = {'launches': cnt, 'ms': round(ms, 3), 'avg_us': round(1e3 * ms / max(cnt, 1), 2),
-                      'GBps': round(byt / (ms * 1e-3) / 1e9, 1) if byt > 0 and ms > 0 else None}
+
+    def table(r):
+        return {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2),
+                    'GBps': round(b_ / (m_ * 1e-3) / 1e9, 1) if b_ > 0 and m_ > 0 else None}
+                for k, (c, m_, b_) in r.items()}
     dom = max((k for k in rep if rep[k][2] > 0), key=lambda k: rep[k][1])
     cnt, ms, byt = rep[dom]
     achieved = byt / (ms * 1e-3) / 1e9
     roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
             'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
-            'alg_bytes_per_launch': byt / cnt, 'avg_launch_us': 1e3 * ms / cnt}
+            'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)}
     out = {
         'metric': METRIC, 'value': round(smx, 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(smx * 1e3, 3), 'higher_is_better': False, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f64+f32', 'data': 'synthetic',
-        'config': {'workload': f'DG-P2 convection-diffusion (SIPG+upwind), {args.n}x{args.n} split unit square, '
-                               f'{hm.nel} triangles, GMRES({args.restart}) + block Jacobi to rel residual {args.tol:g}',
-                   'ndof': gs.ndof, 'nnz': a.nnz, 'eps': 1e-3, 'tol': args.tol},
+        'config': {'workload': f'{desc} to rel residual {args.tol:g}', 'config_index': args.config,
+                   'ndof': gs.ndof, 'nnz': a.nnz, 'tol': args.tol},
         'plain_fp64': {'s_per_step': round(s64, 5), 'iters': it64[-1], 'rel_residual': r64.rel_residual},
         'mixed': {'s_per_step': round(smx, 5), 'iters': itmx[-1], 'outer': rmx.outer,
                   'rel_residual': rmx.rel_residual, 'rel_error_vs_fp64': err},
         'speedup_mixed_vs_fp64': round(s64 / smx, 3),
         'dofs_per_s': round(gs.ndof / smx, 1),
         'roofline': roof,
-        'kernels_mixed': kernels,
-        'kernels_fp64': {k: {'launches': c, 'ms': round(m_, 3),
-                             'GBps': round(b_ / (m_ * 1e-3) / 1e9, 1) if b_ > 0 and m_ > 0 else None}
-                         for k, (c, m_, b_) in rep_fp.items()},
+        'kernels_mixed': table(rep),
+        'kernels_fp64': table(rep_fp),
     }
     if not args.no_cpu:
         log('cpu baseline ...')
-        cb = cpu_baseline(it64[-1], args.n)
+        cb = cpu_baseline(args.config, it64[-1], n)
         cb['speedup_gpu_mixed'] = round(cb['value'] / smx, 1)
+        cb['speedup_gpu_fp64'] = round(cb['value'] / s64, 1)
         out['cpu_baseline'] = cb
     print(json.dumps(out), flush=True)
 
