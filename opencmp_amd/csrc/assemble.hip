@@ -987,7 +987,24 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
   A.nterms = (int)terms.size();
   A.nfields = (int)fields.size();
   A.nloc = s.nloc;
-  Rule rule = simplex_rule(d, 2 * max_order(s) + 4);
+  // quadrature degree: exact for polynomial data (constant: p, quadratic: p+2, grid function of
+  // order q: p+q, times a weight field), 2p+4 for the transcendental sin-product source
+  int qdeg = 0;
+  {
+    const int p = max_order(s);
+    for (auto &t : terms) {
+      int dg = 2 * p + 4;
+      if (t.kind == OCMP_LFI_SOURCE) {
+        if (t.fn == OCMP_FN_CONST) dg = p;
+        else if (t.fn == OCMP_FN_POLY2) dg = p + 2;
+        else if (t.fn == OCMP_FN_FIELD) dg = p + fields[t.field].space->s.blocks[fields[t.field].block].order;
+        if (t.weight_mode != OCMP_WEIGHT_NONE)
+          dg += fields[t.weight_field].space->s.blocks[fields[t.weight_field].block].order;
+      }
+      qdeg = std::max(qdeg, dg);
+    }
+  }
+  Rule rule = simplex_rule(d, qdeg);
   A.nq = rule.nq;
   fill_block_tables(tb, s, rule, A.b);
   A.tab_w = tb.add(rule.w);
@@ -1050,9 +1067,9 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
     A.tab_fw = tb.add(fw);
     A.tab_ft = tb.add(ft);
   }
-  std::string key = "L";
+  std::string key = "L" + std::to_string(qdeg);
   for (auto &t : terms) key += "|" + std::to_string(t.kind) + "," + std::to_string(t.test) + "," +
-                               std::to_string(t.fn == OCMP_FN_FIELD ? t.field : -1);
+                               std::to_string(t.fn) + "," + std::to_string(t.fn == OCMP_FN_FIELD ? t.field : -1);
   for (auto &f : fields) key += "|f" + std::to_string((uintptr_t)f.space) + "," + std::to_string(f.block);
   DBuf<double> *tab = cached_table(s, key, tb, st);
   A.tab = tab->p;

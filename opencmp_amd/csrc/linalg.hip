@@ -541,25 +541,13 @@ __global__ void k_axpy64(int64_t n, double scale, const T *__restrict__ d, doubl
 
 constexpr int KC = 16;  // dot products per pass
 
-// Last-block-done: every block publishes its partials (device-scope release),
-// then one atomic counter picks the block that finishes last; it acquires and
-// reduces all partials in a fixed order, so results stay deterministic. The
-// counter is reset by that block for the next launch (stream order).
-__device__ __forceinline__ bool last_block(unsigned *counter) {
-  __shared__ unsigned s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = (atomicAdd(counter, 1u) == gridDim.x * gridDim.y - 1) ? 1u : 0u;
-  __syncthreads();
-  if (s_last) __threadfence();
-  return s_last != 0u;
-}
-
-// dot[k] = V[k].w for k in [0, nk): partials per block for k in [k0, k0+KC), reduced by the last block
+// Partial sums of V[k].w for k in [k0, k0+KC), k <= nk (index nk: w.w), one row of `part` per k.
+// (A single-kernel "last block reduces" variant needs device-scope fences in every block; on
+// gfx950 each of those writes back the XCD's whole L2 and made this kernel 3x slower, so the
+// partials are reduced by a separate small launch.)
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ V, int64_t ldv,
-                                               const T *__restrict__ w, int nk, double *__restrict__ part,
-                                               unsigned *__restrict__ counter, double *__restrict__ dot) {
+                                               const T *__restrict__ w, int nk, double *__restrict__ part) {
   constexpr int VW = Vec<T>::w;
   __shared__ double sh[NTHR / 64][KC];
   const int lb = xcd_block();
@@ -614,16 +602,6 @@ __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ 
     for (int wv = 0; wv < NTHR / 64; ++wv) s += sh[wv][threadIdx.x];
     part[(int64_t)(k0 + threadIdx.x) * gridDim.x + lb] = s;
   }
-  // the last block to finish sums the partials of every dot (fixed order) into dot[0..nk)
-  if (!last_block(counter)) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int v = wave; v < nk; v += NTHR / 64) {
-    double s = 0.0;
-    for (int b = lane; b < (int)gridDim.x; b += 64) s += part[(int64_t)v * gridDim.x + b];
-    s = wave_sum(s);
-    if (lane == 0) dot[v] = s;
-  }
-  if (threadIdx.x == 0) *counter = 0u;
 }
 
 // State of one GMRES cycle on the device: R column-major (ld = m+1), rotations, g, residual history.
@@ -668,13 +646,11 @@ __device__ void givens_step(const GState &G, int j, double hn) {
   for (int k = threadIdx.x; k <= j + 1; k += blockDim.x) R[k] = col[k];
 }
 
-// w -= sum_k c_k V[k] with c_k = vs[k]^2 * dot[k] (classical Gram-Schmidt update); the last block
-// reduces |w|^2 and performs the Givens step of column j = nk-1
+// w -= sum_k c_k V[k] with c_k = vs[k]^2 * dot[k] (classical Gram-Schmidt update); partials of |w|^2
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_gs_update(int64_t n, const T *__restrict__ V, int64_t ldv,
                                                     T *__restrict__ w, int nk, const double *__restrict__ dot,
-                                                    const double *__restrict__ vs, double *__restrict__ part,
-                                                    unsigned *__restrict__ counter, GState G) {
+                                                    const double *__restrict__ vs, double *__restrict__ part) {
   constexpr int VW = Vec<T>::w;
   __shared__ double sh[NTHR / 64];
   __shared__ double cks[256];
@@ -717,17 +693,18 @@ __global__ void __launch_bounds__(NTHR) k_gs_update(int64_t n, const T *__restri
   }
   acc = block_sum(acc, sh);
   if (threadIdx.x == 0) part[lb] = acc;
-  if (!last_block(counter)) return;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) s += part[b];
-  s = block_sum(s, sh);
+}
+
+// h_{j+1,j} = sqrt(sum of the update partials), then the Givens step of column j (one block)
+__global__ void __launch_bounds__(256) k_norm_givens(const double *__restrict__ part, int nparts, int j, GState G) {
+  __shared__ double sh[4];
   __shared__ double hn;
-  if (threadIdx.x == 0) {
-    hn = sqrt(s);
-    *counter = 0u;
-  }
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += part[b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) hn = sqrt(s);
   __syncthreads();
-  givens_step(G, nk - 1, hn);
+  givens_step(G, j, hn);
 }
 
 // back substitution R y = g (k columns) as a column sweep: once y_i is known every
@@ -807,7 +784,6 @@ struct Work {
   double *hbuf = nullptr;  // pinned host
   size_t hcap = 0;
   DBuf<int32_t> nbr;       // DG neighbour table for the ELL operators
-  DBuf<unsigned> counters; // last-block-done counters of the fused reductions (zeroed once)
   std::vector<hipEvent_t> events;
 };
 
@@ -1137,10 +1113,6 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
   const double gscale = beta;  // g is relative to beta
   const int limit = std::min(m, maxit);
   GState G{dot, vs, R, csn, g, res, m, gscale};
-  if (!w.counters.p) {
-    w.counters.alloc(8);
-    OCMP_HIP(hipMemsetAsync(w.counters.p, 0, 8 * sizeof(unsigned), st));
-  }
   struct Batch {
     int j0, nb, ev;
   };
@@ -1187,12 +1159,14 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
         dim3 gd(grid, (jj + 2 + KC - 1) / KC);
-        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p, w.counters.p, dot);
+        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p);
       }
+      reduce_to(w.part.p, grid, jj + 1, dot, st);
       {
         KTimer tm(ctx, c_upd, (double)(jj + 3) * n * sizeof(T));
-        k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p, w.counters.p + 1, G);
+        k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p);
       }
+      k_norm_givens<<<1, 256, 0, st>>>(w.part.p, grid, jj, G);
     }
     const int nb = j_end - j;
     OCMP_HIP(hipMemcpyAsync(hres + j, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));

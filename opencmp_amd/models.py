@@ -326,7 +326,9 @@ class INS(Stokes):
             unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
             it += 1
             self.W.vec.copy_(gfu.vec)
-            if err < self.abs_tol + self.rel_tol * unorm or it > self.nonlinear_max_iters:
+            self.picard_converged = err < self.abs_tol + self.rel_tol * unorm
+            self.picard_error = err
+            if self.picard_converged or it > self.nonlinear_max_iters:
                 break
         self.picard_iterations = it - 1
         return res

@@ -147,5 +147,7 @@ def test_oseen_picard_converges(gpu_ctx):
             nonlinear_tolerance=(1e-10, 1e-10), nonlinear_max_iters=20, ctx=gpu_ctx)
     s = StationarySolver(m)
     s.solve()
-    say('  picard iterations', m.picard_iterations, s.result)
-    assert s.result.converged and 2 <= m.picard_iterations < 20
+    say('  picard iterations', m.picard_iterations, 'error', m.picard_error, s.result)
+    # inner solves are measured against their own initial residual (no body force), so the
+    # criterion that matters is the Picard one: ||W - u||_L2 < abs + rel ||u||_L2
+    assert m.picard_converged and 2 <= m.picard_iterations < 20
