@@ -82,12 +82,12 @@ def build_problem(config, n, ctx):
     raise SystemExit(f'unknown config {config}')
 
 
-def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes'):
+def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes', inner_tol=None):
     import torch
     pre = C.Preconditioner(a, 'local', precision=precision)
     x = C.GridFunction(gs)
     res = None
-    inner = (1e-5 if solver == 'GMRes' else 1e-4) if precision == 'mixed' else 0.0
+    inner = (inner_tol or (1e-5 if solver == 'GMRes' else 1e-6)) if precision == 'mixed' else 0.0
 
     def step():
         a.Assemble()
@@ -190,6 +190,7 @@ def main():
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=50)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--inner-tol', type=float, default=None, help='mixed path: relative reduction per inner solve')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
@@ -202,7 +203,7 @@ def main():
     t0 = time.perf_counter()
     hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
-    kw = dict(solver=solver)
+    kw = dict(solver=solver, inner_tol=args.inner_tol)
     s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps, args.warmup, **kw)
     smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps, args.warmup,
                                       **kw)

@@ -207,9 +207,14 @@ static double csr_bytes(const Matrix *A, int wv, int wx) {
   return (double)A->nnz * (wv + 4) + 4.0 * (A->n + 1) + 2.0 * A->n * wx;
 }
 
+// one row group per block over the whole matrix: the dependent loads (row_ptr -> col -> x) of
+// one group are latency-bound, so many resident waves, not a loop inside a wave, hide the latency
 static int spmv_grid(Matrix *A) {
   const int rpb = NTHR / A->spmv_group;
-  return red_grid((A->n + rpb - 1) / rpb * NTHR);
+  int64_t g = (A->n + rpb - 1) / rpb;
+  g = std::max<int64_t>(g, NXCD);
+  g = (g + NXCD - 1) / NXCD * NXCD;
+  return (int)g;
 }
 
 void spmv_f64(Matrix *A, const double *x, double *y) {
@@ -819,9 +824,29 @@ static void ensure_part(Work &w, size_t n) {
   w.part.alloc(std::max<size_t>(w.part.n, std::max<size_t>(n, (size_t)68 * MAX_RED_BLOCKS)));
 }
 
-// reduce nvals partial rows of `grid` entries into dst (device)
-static void reduce_to(const double *part, int grid, int nvals, double *dst, hipStream_t st, bool do_sqrt = false) {
-  k_reduce<<<nvals, NTHR, 0, st>>>(part, grid, dst, do_sqrt ? 1 : 0);
+// first stage of a long reduction: block s sums part[s*seg, (s+1)*seg) into out[s]
+__global__ void k_reduce_seg(const double *__restrict__ part, int64_t nparts, int64_t seg, double *__restrict__ out) {
+  __shared__ double sh[NTHR / 64];
+  const int64_t b0 = (int64_t)blockIdx.x * seg, b1 = min(nparts, b0 + seg);
+  double s = 0.0;
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) s += part[b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+constexpr int SEG_BLOCKS = 1024;
+
+// reduce nvals partial rows of `grid` entries into dst (device), fixed order. A single long row
+// (one partial per SpMV block) is reduced in two stages through part[grid, grid + SEG_BLOCKS).
+static void reduce_to(double *part, int64_t grid, int nvals, double *dst, hipStream_t st, bool do_sqrt = false) {
+  if (grid <= 8192 || nvals != 1) {
+    k_reduce<<<nvals, NTHR, 0, st>>>(part, (int)grid, dst, do_sqrt ? 1 : 0);
+    return;
+  }
+  const int64_t seg = (grid + SEG_BLOCKS - 1) / SEG_BLOCKS;
+  double *tmp = part + grid;
+  k_reduce_seg<<<SEG_BLOCKS, NTHR, 0, st>>>(part, grid, seg, tmp);
+  k_reduce<<<1, NTHR, 0, st>>>(tmp, SEG_BLOCKS, dst, do_sqrt ? 1 : 0);
 }
 
 static bool dg_ell_ready(Matrix *A, Precond *P) {
@@ -849,6 +874,7 @@ static double residual64(Matrix *A, Precond *P, const double *b, const double *x
     }
   } else {
     grid = spmv_grid(A);
+    ensure_part(w, (size_t)grid + SEG_BLOCKS + 64);
     KTimer tm(ctx, ctx->cls("spmv_csr_f64_residual"), csr_bytes(A, 8, 8) + 8.0 * A->n);
     spmv_dispatch<double, double>(A, A->d_val.p, x, r, b, A->space->d_free.p, (const double *)nullptr, nullptr, 1,
                                   w.part.p, grid, st);
@@ -934,6 +960,7 @@ static int cg_inner(Matrix *A, const VT *val, const T *dinv, T *x, T *r, T *z, T
   hipStream_t st = ctx->stream;
   const int64_t n = A->n;
   const int sgrid = spmv_grid(A);
+  ensure_part(w, (size_t)sgrid + SEG_BLOCKS + 64);
   const int vgrid = red_grid(n);
   double *sc = ctx->scal.p;  // [0],[1]: rz ring, [2]: pq
   const int H = std::max(check_every, 1);
