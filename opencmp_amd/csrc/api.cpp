@@ -409,12 +409,25 @@ void build_pattern(Matrix *A) {
   A->d_col.upload(A->col, st);
   A->d_val.alloc(A->nnz);
   OCMP_HIP(hipMemsetAsync(A->d_val.p, 0, A->nnz * sizeof(double), st));
-  double avg = A->n ? double(A->nnz) / double(A->n) : 1.0;
-  // lanes per row: about a quarter of the mean row length (each lane then handles ~4 entries), so a
-  // wave keeps several rows in flight; a full wave per row left the kernel latency-bound
-  int g = 4;
-  while (g < 64 && g * 4 < avg) g *= 2;
-  A->spmv_group = g;
+  build_row_blocks(A);
+}
+
+void build_row_blocks(Matrix *A) {
+  // greedy: extend a block while its nonzeros fit in SPMV_NNZB; a row longer than that is a block
+  // of its own (the kernel then loops over it in chunks)
+  A->rblk.clear();
+  A->rblk.push_back(0);
+  int64_t start_nz = 0;
+  for (int64_t r = 0; r < A->n; ++r) {
+    const int64_t end_nz = A->row_ptr[r + 1];
+    if (end_nz - start_nz > SPMV_NNZB && A->rblk.back() != r) {
+      A->rblk.push_back((int32_t)r);
+      start_nz = A->row_ptr[r];
+    }
+  }
+  A->rblk.push_back((int32_t)A->n);
+  A->nrblk = (int64_t)A->rblk.size() - 1;
+  A->d_rblk.upload(A->rblk, A->ctx->stream);
 }
 
 void build_colouring(Space &s) {

@@ -203,7 +203,10 @@ struct Matrix {
   DBuf<int32_t> d_dg_bcol;           // nel*4 block columns (-1 padded), sorted
   int dg_nb = 0;                     // local dofs per element (block size)
   // SpMV launch parameters
-  int spmv_group = 16;
+  // LDS-staged SpMV: row blocks of at most SPMV_NNZB nonzeros (block b = rows [rblk[b], rblk[b+1]))
+  std::vector<int32_t> rblk;
+  DBuf<int32_t> d_rblk;
+  int64_t nrblk = 0;
   bool assembled = false;
   // mixed precision copy
   DBuf<float> d_val32;
@@ -242,6 +245,9 @@ void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 void build_h1_schedule(Matrix *A);
 void build_colouring(Space &s);
 void build_pattern(Matrix *A);
+// rows grouped into blocks of at most SPMV_NNZB nonzeros for the LDS-staged SpMV
+constexpr int SPMV_NNZB = 2048;
+void build_row_blocks(Matrix *A);
 
 }  // namespace ocmp
 

@@ -123,56 +123,109 @@ __global__ void k_reduce(const double *__restrict__ part, int nparts, double *__
 
 // ------------------------------------------------------------------ CSR SpMV
 
+// LDS-staged CSR SpMV. Block b owns rows [rblk[b], rblk[b+1]) whose nonzeros form one
+// contiguous range of at most SPMV_NNZB entries: every thread loads 8 (col, val) pairs with
+// fully coalesced loads, gathers x, and stages the products in LDS; rows are then summed from
+// LDS by 4 lanes each. Two dependent memory levels (col -> x) instead of three
+// (row_ptr -> col -> x), and 8 independent loads in flight per thread.
+//
 // mode 0: y = A x
 // mode 1: y = mask*(b - A x), partial sum y^2
 // mode 2: y = mask*(A x), partial sum x.y                (CG: q = A p, p.q)
 // mode 3: y = mask*(A (cs*sx*x))   cs: column scaling (right Jacobi), sx: device scalar  (GMRES on H1)
-template <int G, typename VT, typename XT>
-__global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t n, const int32_t *__restrict__ rp,
-                                                   const int32_t *__restrict__ col, const VT *__restrict__ val,
-                                                   const XT *__restrict__ x, XT *__restrict__ y,
-                                                   const double *__restrict__ b, const uint8_t *__restrict__ mask,
-                                                   const XT *__restrict__ cs, const double *__restrict__ sx,
-                                                   int mode, double *__restrict__ part) {
+constexpr int SPMV_PER_THREAD = SPMV_NNZB / NTHR;
+
+template <typename VT, typename XT>
+__device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT *__restrict__ y,
+                                              const XT *__restrict__ x, const double *__restrict__ b,
+                                              const uint8_t *__restrict__ mask, XT scale, double &acc) {
+  if (mode == 0) {
+    y[r] = (XT)s;
+  } else if (mode == 1) {
+    const double v = (mask && !mask[r]) ? 0.0 : b[r] - s;
+    y[r] = (XT)v;
+    acc += v * v;
+  } else if (mode == 2) {
+    const XT v = (mask && !mask[r]) ? (XT)0 : (XT)s;
+    y[r] = v;
+    acc += (double)x[r] * (double)v;
+  } else {
+    y[r] = (mask && !mask[r]) ? (XT)0 : (XT)s * scale;
+  }
+}
+
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t *__restrict__ rblk,
+                                                   const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                   const VT *__restrict__ val, const XT *__restrict__ x,
+                                                   XT *__restrict__ y, const double *__restrict__ b,
+                                                   const uint8_t *__restrict__ mask, const XT *__restrict__ cs,
+                                                   const double *__restrict__ sx, int mode,
+                                                   double *__restrict__ part) {
+  __shared__ double prod[SPMV_NNZB];
   __shared__ double sh[NTHR / 64];
-  constexpr int RPB = NTHR / G;  // rows per block-iteration
-  const int lane = threadIdx.x % G;
-  const int64_t ngroups = (n + RPB - 1) / RPB;
   const int lb = xcd_block();
-  const int64_t chunk = (ngroups + gridDim.x - 1) / gridDim.x;
-  const int64_t g0 = lb * chunk, g1 = min(ngroups, g0 + chunk);
-  const XT scale = (mode == 3 && sx) ? (XT)sx[0] : (XT)1;
   double acc = 0.0;
-  for (int64_t gr = g0; gr < g1; ++gr) {
-    const int64_t r = gr * RPB + threadIdx.x / G;
-    XT s = 0;
-    if (r < n) {
-      const int32_t j0 = rp[r], j1 = rp[r + 1];
-      if (mode == 3 && cs) {
-        for (int32_t j = j0 + lane; j < j1; j += G) {
-          const int32_t c = col[j];
-          s += (XT)val[j] * (cs[c] * x[c]);
-        }
-      } else {
-        for (int32_t j = j0 + lane; j < j1; j += G) s += (XT)val[j] * x[col[j]];
+  const XT scale = (mode == 3 && sx) ? (XT)sx[0] : (XT)1;
+  if (lb < nrblk) {
+    const int32_t r0 = rblk[lb], r1 = rblk[lb + 1];
+    const int32_t j0 = rp[r0], j1 = rp[r1];
+    const int nz = j1 - j0;
+    const int nrows = r1 - r0;
+    if (nz <= SPMV_NNZB) {
+      // rows of the first pass: fetch their bounds early so the loads overlap the staging
+      const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
+      int a0 = 0, e0 = 0;
+      if (ri0 < nrows) {
+        a0 = rp[r0 + ri0] - j0;
+        e0 = rp[r0 + ri0 + 1] - j0;
       }
-    }
+      int32_t ci[SPMV_PER_THREAD];
+      VT vv[SPMV_PER_THREAD];
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-    if (r < n && lane == 0) {
-      if (mode == 0) {
-        y[r] = s;
-      } else if (mode == 1) {
-        const double v = (mask && !mask[r]) ? 0.0 : b[r] - (double)s;
-        y[r] = (XT)v;
-        acc += v * v;
-      } else if (mode == 2) {
-        const XT v = (mask && !mask[r]) ? (XT)0 : s;
-        y[r] = v;
-        acc += (double)x[r] * (double)v;
-      } else {
-        y[r] = (mask && !mask[r]) ? (XT)0 : s * scale;
+      for (int k = 0; k < SPMV_PER_THREAD; ++k) {
+        const int t = threadIdx.x + k * NTHR;
+        ci[k] = -1;
+        vv[k] = (VT)0;
+        if (t < nz) {
+          ci[k] = col[j0 + t];
+          vv[k] = val[j0 + t];
+        }
       }
+#pragma unroll
+      for (int k = 0; k < SPMV_PER_THREAD; ++k) {
+        if (ci[k] >= 0) {
+          XT xv = x[ci[k]];
+          if (mode == 3 && cs) xv *= cs[ci[k]];
+          prod[threadIdx.x + k * NTHR] = (double)((XT)vv[k] * xv);
+        }
+      }
+      __syncthreads();
+      for (int base = 0; base < nrows; base += NTHR / 4) {
+        const int ri = base + ri0;
+        int a = a0, e = e0;
+        if (base > 0 && ri < nrows) {
+          a = rp[r0 + ri] - j0;
+          e = rp[r0 + ri + 1] - j0;
+        }
+        double s = 0.0;
+        if (ri < nrows)
+          for (int t = a + q; t < e; t += 4) s += prod[t];
+        s += __shfl_xor(s, 1, 4);
+        s += __shfl_xor(s, 2, 4);
+        if (ri < nrows && q == 0) spmv_epilogue<VT, XT>(mode, r0 + ri, s, y, x, b, mask, scale, acc);
+      }
+    } else {
+      // a single row longer than the staging buffer
+      double s = 0.0;
+      for (int t = threadIdx.x; t < nz; t += NTHR) {
+        const int32_t c = col[j0 + t];
+        XT xv = x[c];
+        if (mode == 3 && cs) xv *= cs[c];
+        s += (double)((XT)val[j0 + t] * xv);
+      }
+      s = block_sum(s, sh);
+      if (threadIdx.x == 0) spmv_epilogue<VT, XT>(mode, r0, s, y, x, b, mask, scale, acc);
     }
   }
   if (part) {
@@ -181,40 +234,22 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t n, const int32_t *__r
   }
 }
 
+static int spmv_grid(Matrix *A) {
+  int64_t g = std::max<int64_t>(A->nrblk, NXCD);
+  g = (g + NXCD - 1) / NXCD * NXCD;
+  return (int)g;
+}
+
 template <typename VT, typename XT>
 static void spmv_dispatch(Matrix *A, const VT *val, const XT *x, XT *y, const double *b, const uint8_t *mask,
                           const XT *cs, const double *sx, int mode, double *part, int grid, hipStream_t st) {
-  const int64_t n = A->n;
-#define OCMP_SPMV_CASE(GG)                                                                                     \
-  case GG:                                                                                                     \
-    k_spmv_csr<GG, VT, XT><<<grid, NTHR, 0, st>>>(n, A->d_row_ptr.p, A->d_col.p, val, x, y, b, mask, cs, sx, \
-                                                  mode, part);                                                 \
-    break;
-  switch (A->spmv_group) {
-    OCMP_SPMV_CASE(4)
-    OCMP_SPMV_CASE(8)
-    OCMP_SPMV_CASE(16)
-    OCMP_SPMV_CASE(32)
-    OCMP_SPMV_CASE(64)
-    default:
-      throw Error(OCMP_ERR_INTERNAL, "bad SpMV group size");
-  }
-#undef OCMP_SPMV_CASE
+  k_spmv_csr<VT, XT><<<grid, NTHR, 0, st>>>(A->nrblk, A->d_rblk.p, A->d_row_ptr.p, A->d_col.p, val, x, y, b, mask,
+                                            cs, sx, mode, part);
 }
 
 // algorithmic bytes of one CSR SpMV: values + column indices + row pointers + x once + y once
 static double csr_bytes(const Matrix *A, int wv, int wx) {
   return (double)A->nnz * (wv + 4) + 4.0 * (A->n + 1) + 2.0 * A->n * wx;
-}
-
-// one row group per block over the whole matrix: the dependent loads (row_ptr -> col -> x) of
-// one group are latency-bound, so many resident waves, not a loop inside a wave, hide the latency
-static int spmv_grid(Matrix *A) {
-  const int rpb = NTHR / A->spmv_group;
-  int64_t g = (A->n + rpb - 1) / rpb;
-  g = std::max<int64_t>(g, NXCD);
-  g = (g + NXCD - 1) / NXCD * NXCD;
-  return (int)g;
 }
 
 void spmv_f64(Matrix *A, const double *x, double *y) {
