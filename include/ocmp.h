@@ -226,6 +226,12 @@ int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, do
 int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *ref, int fn, const double *c,
                      double *out);
 
+/* out[0] = ||b - A x||_2 and out[1] = ||b||_2 over the free dofs, FP64 (host
+ * doubles, syncs). The residual a solve starts from: NGSolve's solvers compute
+ * it inside (opencmp/models/base_model.py:886-947) without exposing it; the
+ * learned-guess fallback compares it between candidate initial guesses. */
+int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double *out);
+
 /* ----------------------------------------------------- preconditioner, solve */
 
 enum { OCMP_PRE_NONE = 0, OCMP_PRE_JACOBI = 1, OCMP_PRE_BLOCK_JACOBI = 2 };
@@ -270,6 +276,52 @@ typedef struct {
  * also when not converged; check stats->converged. */
 int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x,
                const ocmp_solve_opts *opts, ocmp_solve_stats *stats);
+
+/* ----------------------------------------------------- learned initial guess */
+
+/* Per-DOF MLP predicting the next time step of a transient solve from the last
+ * four states of each DOF. The reference has no counterpart: it starts every
+ * solve of a time step from the previous solution (the GridFunction carries it,
+ * opencmp/models/ins.py:323-359, opencmp/solvers/time_integration_schemes.py:26-80).
+ *
+ * Network: 4 -> 128 -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32 accumulation.
+ * Parameters (float32, this order): W1[128][4], b1[128], W2[128][128], b2[128],
+ * W3[128][128], b3[128], W4[128], b4[1].
+ * Features of DOF i in segment k (a segment = the DOFs of one scalar block):
+ *   x = ((u0-u1)/s_k, (u1-u2)/s_k, (u2-u3)/s_k, u0/U_k),  guess = u0 + s_k * f(x)
+ * with u0 the newest state. Training minimises 0.5/B sum (f(x) - t)^2 with
+ * t = (u_next - u0)/s_k by Adam (beta 0.9/0.999, eps 1e-8).
+ * X: device float32[n][4] (16-byte aligned), T/Y: device float32[n]. */
+#define OCMP_MLP_HIDDEN 128
+#define OCMP_MLP_NPARAM 33793
+#define OCMP_MLP_MAXSEG 8
+typedef struct ocmp_mlp ocmp_mlp;
+
+/* params: host float32[OCMP_MLP_NPARAM]; max_batch bounds the training batch. */
+int ocmp_mlp_create(ocmp_ctx *ctx, const float *params, int64_t max_batch, ocmp_mlp **out);
+int ocmp_mlp_destroy(ocmp_mlp *mlp);
+/* host copies; set also resets the Adam moments and step count */
+int ocmp_mlp_get_params(ocmp_mlp *mlp, float *params);
+int ocmp_mlp_set_params(ocmp_mlp *mlp, const float *params);
+/* feature scales per segment: seg_off[nseg+1] (host, seg_off[0] = 0), s/U[nseg] > 0 */
+int ocmp_mlp_set_scales(ocmp_mlp *mlp, int nseg, const int64_t *seg_off, const double *s, const double *U);
+int ocmp_mlp_get_scales(ocmp_mlp *mlp, int *nseg, int64_t *seg_off, double *s, double *U);
+/* s_k = rms of the increments, U_k = rms of the states over snapshots
+ * snap[j*ld + i] (device f64, j < nsnap); a segment that never moves gets 1 */
+int ocmp_mlp_fit_scales(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int nseg,
+                        const int64_t *seg_off);
+/* batch random samples (snapshot j in [3, nsnap-2], dof i < n) of (X, T), seeded */
+int ocmp_mlp_sample(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int64_t n, uint64_t seed,
+                    int64_t batch, float *X, float *T);
+/* Y = f(X) */
+int ocmp_mlp_forward(ocmp_mlp *mlp, int64_t n, const float *X, float *Y);
+/* gradient of the loss on (X, T) into grad (device float32[OCMP_MLP_NPARAM]); loss: host, may be NULL */
+int ocmp_mlp_grad(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float *grad, double *loss);
+/* one Adam step on (X, T); loss (host, may be NULL: then nothing synchronises) is the pre-step loss */
+int ocmp_mlp_train_step(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float lr, double *loss);
+/* out = u0 + s * f(features(u0, u1, u2, u3)) over n dofs (device f64); out may alias none of u0..u3 */
+int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1, const double *u2,
+                   const double *u3, double *out);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,7 @@ FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD = 0, 1, 2, 3
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI = 0, 1, 2
 FP64, MIXED = 0, 1
 SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON = 0, 1, 2
+MLP_HIDDEN, MLP_NPARAM, MLP_MAXSEG = 128, 33793, 8
 
 
 class FeDesc(C.Structure):
@@ -89,10 +90,23 @@ SIGNATURES = {
     'ocmp_matvec': [P, P, P],
     'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
+    'ocmp_mlp_create': [P, P, C.c_int64, C.POINTER(P)],
+    'ocmp_mlp_destroy': [P],
+    'ocmp_mlp_get_params': [P, P],
+    'ocmp_mlp_set_params': [P, P],
+    'ocmp_mlp_set_scales': [P, C.c_int, I64P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    'ocmp_mlp_get_scales': [P, C.POINTER(C.c_int), I64P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    'ocmp_mlp_fit_scales': [P, P, C.c_int, C.c_int64, C.c_int, I64P],
+    'ocmp_mlp_sample': [P, P, C.c_int, C.c_int64, C.c_int64, C.c_uint64, C.c_int64, P, P],
+    'ocmp_mlp_forward': [P, C.c_int64, P, P],
+    'ocmp_mlp_grad': [P, C.c_int64, P, P, P, C.POINTER(C.c_double)],
+    'ocmp_mlp_train_step': [P, C.c_int64, P, P, C.c_float, C.POINTER(C.c_double)],
+    'ocmp_mlp_guess': [P, C.c_int64, P, P, P, P, P],
 }
 RESTYPES = {'ocmp_last_error': C.c_char_p, 'ocmp_ctx_stream': C.c_void_p}
 

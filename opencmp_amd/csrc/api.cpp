@@ -847,6 +847,15 @@ int ocmp_matvec(ocmp_matrix *A, const double *x, double *y) {
   });
 }
 
+int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(A && b && x && out, "NULL argument");
+    OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    residual_norms(&A->A, b, x, out);
+  });
+}
+
 int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **out) {
   return guard([&] {
     OCMP_REQUIRE(A && out, "NULL argument");
@@ -888,6 +897,112 @@ int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x, cons
     if (P && !P->P.ready) precond_update(&P->P);
     std::memset(stats, 0, sizeof(*stats));
     solve(&A->A, P ? &P->P : nullptr, b, x, *opts, *stats);
+  });
+}
+
+// ------------------------------------------------------------------ learned initial guess
+
+#define MLP_ENTER(m)                                   \
+  OCMP_REQUIRE((m) && (m)->M, "NULL mlp handle");     \
+  OCMP_HIP(hipSetDevice((m)->ctx->device))
+
+int ocmp_mlp_create(ocmp_ctx *ctx, const float *params, int64_t max_batch, ocmp_mlp **out) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && params && out, "NULL argument");
+    OCMP_HIP(hipSetDevice(ctx->c.device));
+    auto *h = new ocmp_mlp();
+    h->ctx = &ctx->c;
+    try {
+      h->M = ocmp::mlp_create(&ctx->c, params, max_batch);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ocmp_mlp_destroy(ocmp_mlp *mlp) {
+  return guard([&] {
+    if (!mlp) return;
+    ocmp::mlp_destroy(mlp->M);
+    delete mlp;
+  });
+}
+
+int ocmp_mlp_get_params(ocmp_mlp *mlp, float *params) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_get_params(mlp->M, params);
+  });
+}
+
+int ocmp_mlp_set_params(ocmp_mlp *mlp, const float *params) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_set_params(mlp->M, params);
+  });
+}
+
+int ocmp_mlp_set_scales(ocmp_mlp *mlp, int nseg, const int64_t *seg_off, const double *s, const double *U) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    OCMP_REQUIRE(s && U, "NULL argument");
+    ocmp::mlp_set_scales(mlp->M, nseg, seg_off, s, U);
+  });
+}
+
+int ocmp_mlp_get_scales(ocmp_mlp *mlp, int *nseg, int64_t *seg_off, double *s, double *U) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    OCMP_REQUIRE(nseg && seg_off && s && U, "NULL argument");
+    ocmp::mlp_get_scales(mlp->M, nseg, seg_off, s, U);
+  });
+}
+
+int ocmp_mlp_fit_scales(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int nseg, const int64_t *seg_off) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_fit_scales(mlp->M, snap, nsnap, ld, nseg, seg_off);
+  });
+}
+
+int ocmp_mlp_sample(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int64_t n, uint64_t seed,
+                    int64_t batch, float *X, float *T) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_sample(mlp->M, snap, nsnap, ld, n, seed, batch, X, T);
+  });
+}
+
+int ocmp_mlp_forward(ocmp_mlp *mlp, int64_t n, const float *X, float *Y) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_forward(mlp->M, n, X, Y);
+  });
+}
+
+int ocmp_mlp_grad(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float *grad, double *loss) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    OCMP_REQUIRE(grad, "NULL gradient buffer");
+    ocmp::mlp_grad(mlp->M, n, X, T, grad, loss);
+  });
+}
+
+int ocmp_mlp_train_step(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float lr, double *loss) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    ocmp::mlp_train_step(mlp->M, n, X, T, lr, loss);
+  });
+}
+
+int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1, const double *u2, const double *u3,
+                   double *out) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    OCMP_REQUIRE(out != u0 && out != u1 && out != u2 && out != u3, "out must not alias the history");
+    ocmp::mlp_guess(mlp->M, n, u0, u1, u2, u3, out);
   });
 }
 

@@ -238,8 +238,26 @@ void launch_dirichlet_set(Space *s, int block, double *x, int fn, const double *
 void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
+void residual_norms(Matrix *A, const double *b, const double *x, double *out);
 void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
+
+// per-DOF MLP initial guess (mlp.hip)
+struct Mlp;
+Mlp *mlp_create(Ctx *ctx, const float *params, int64_t max_batch);
+void mlp_destroy(Mlp *M);
+void mlp_set_params(Mlp *M, const float *params);
+void mlp_get_params(Mlp *M, float *params);
+void mlp_set_scales(Mlp *M, int nseg, const int64_t *off, const double *s, const double *U);
+void mlp_get_scales(Mlp *M, int *nseg, int64_t *off, double *s, double *U);
+void mlp_fit_scales(Mlp *M, const double *snap, int nsnap, int64_t ld, int nseg, const int64_t *off);
+void mlp_sample(Mlp *M, const double *snap, int nsnap, int64_t ld, int64_t n, uint64_t seed, int64_t batch,
+                float *X, float *T);
+void mlp_forward(Mlp *M, int64_t n, const float *X, float *Y);
+void mlp_grad(Mlp *M, int64_t n, const float *X, const float *T, float *grad, double *loss);
+void mlp_train_step(Mlp *M, int64_t n, const float *X, const float *T, float lr, double *loss);
+void mlp_guess(Mlp *M, int64_t n, const double *u0, const double *u1, const double *u2, const double *u3,
+               double *out);
 
 // greedy colouring + slot map (host)
 void build_h1_schedule(Matrix *A);
@@ -256,3 +274,4 @@ struct ocmp_mesh { ocmp::Mesh m; };
 struct ocmp_space { ocmp::Space s; };
 struct ocmp_matrix { ocmp::Matrix A; };
 struct ocmp_precond { ocmp::Precond P; };
+struct ocmp_mlp { ocmp::Mlp *M; ocmp::Ctx *ctx; };

@@ -1361,6 +1361,14 @@ static void solve_richardson(Matrix *A, Precond *P, const double *b, double *x, 
   st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
 }
 
+// out[0] = ||b - A x||_2, out[1] = ||b||_2, both over the free dofs in FP64
+void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
+  Work &w = work(A);
+  w.r64.alloc(A->n);
+  out[0] = residual64(A, nullptr, b, x, w.r64.p, w);
+  out[1] = norm_masked(A, b, w);
+}
+
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st) {
   switch (o.solver) {
     case OCMP_SOLVER_CG: solve_cg(A, P, b, x, o, st); break;

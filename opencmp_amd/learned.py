@@ -1,0 +1,221 @@
+"""Learned initial guess for transient solves (ocmp_mlp_* in include/ocmp.h).
+
+A per-DOF MLP reads the last four states of each DOF and predicts the next
+one; the prediction becomes the initial guess of the next linear solve. The
+reference starts every solve from the previous solution held in the
+GridFunction (opencmp/models/ins.py:323-359,
+opencmp/solvers/time_integration_schemes.py:26-80); :func:`choose_guess`
+keeps that plain guess whenever the learned one does not lower the FP64
+residual, so the learned path never starts a solve from a worse point.
+
+Training runs on the GPU through the library's own forward, backward and Adam
+kernels on snapshots the plain solver produced.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .core import Context, _ptr
+
+NPARAM = L.MLP_NPARAM
+HID = L.MLP_HIDDEN
+SHAPES = [('W1', (HID, 4)), ('b1', (HID,)), ('W2', (HID, HID)), ('b2', (HID,)), ('W3', (HID, HID)),
+          ('b3', (HID,)), ('W4', (1, HID)), ('b4', (1,))]
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-normal weights, zero biases; the output layer is scaled down so the
+    untrained network predicts increments near 0."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in SHAPES:
+        if name.startswith('W'):
+            w = rng.standard_normal(shape) * math.sqrt(2.0 / shape[1])
+            if name == 'W4':
+                w *= 0.1
+            parts.append(w.ravel())
+        else:
+            parts.append(np.zeros(shape).ravel())
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == NPARAM
+    return p
+
+
+class MLPGuess:
+    """Device-resident network, its Adam state and the per-segment feature scales."""
+
+    def __init__(self, ctx: Optional[Context] = None, params: Optional[np.ndarray] = None, seed: int = 0,
+                 max_batch: int = 65536):
+        self.ctx = ctx or Context.default()
+        self.lib = self.ctx.lib
+        self.max_batch = int(max_batch)
+        p = init_params(seed) if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != NPARAM:
+            raise ValueError(f'expected {NPARAM} parameters, got {p.size}')
+        h = C.c_void_p()
+        L.check(self.lib.ocmp_mlp_create(self.ctx.h, p.ctypes.data_as(C.c_void_p), self.max_batch, C.byref(h)),
+                'mlp_create')
+        self.h = h
+        self.losses: List[float] = []
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None and h.value:
+            try:
+                self.lib.ocmp_mlp_destroy(h)
+            except Exception:
+                pass
+
+    # ------------------------------------------------------------ parameters and scales
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(NPARAM, dtype=np.float32)
+        L.check(self.lib.ocmp_mlp_get_params(self.h, out.ctypes.data_as(C.c_void_p)), 'mlp_get_params')
+        return out
+
+    @params.setter
+    def params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != NPARAM:
+            raise ValueError(f'expected {NPARAM} parameters, got {p.size}')
+        L.check(self.lib.ocmp_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p)), 'mlp_set_params')
+
+    def set_scales(self, seg_off: Sequence[int], s: Sequence[float], U: Sequence[float]) -> None:
+        k = len(seg_off) - 1
+        off = (C.c_int64 * (k + 1))(*[int(v) for v in seg_off])
+        L.check(self.lib.ocmp_mlp_set_scales(self.h, k, off, (C.c_double * k)(*s), (C.c_double * k)(*U)),
+                'mlp_set_scales')
+
+    def scales(self) -> Tuple[List[int], List[float], List[float]]:
+        k = C.c_int()
+        off = (C.c_int64 * (L.MLP_MAXSEG + 1))()
+        s = (C.c_double * L.MLP_MAXSEG)()
+        U = (C.c_double * L.MLP_MAXSEG)()
+        L.check(self.lib.ocmp_mlp_get_scales(self.h, C.byref(k), off, s, U), 'mlp_get_scales')
+        n = k.value
+        return list(off[:n + 1]), list(s[:n]), list(U[:n])
+
+    def fit_scales(self, snap: torch.Tensor, seg_off: Sequence[int]) -> None:
+        """snap: (nsnap, n) float64 on the device, oldest first."""
+        snap = self._snap(snap)
+        k = len(seg_off) - 1
+        off = (C.c_int64 * (k + 1))(*[int(v) for v in seg_off])
+        L.check(self.lib.ocmp_mlp_fit_scales(self.h, C.c_void_p(_ptr(snap)), snap.shape[0], snap.stride(0), k, off),
+                'mlp_fit_scales')
+
+    # ------------------------------------------------------------ kernels
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        X = self._x(X)
+        Y = torch.empty(X.shape[0], dtype=torch.float32, device=X.device)
+        L.check(self.lib.ocmp_mlp_forward(self.h, X.shape[0], C.c_void_p(_ptr(X)), C.c_void_p(_ptr(Y))),
+                'mlp_forward')
+        return Y
+
+    def grad(self, X: torch.Tensor, T: torch.Tensor) -> Tuple[torch.Tensor, float]:
+        X, T = self._x(X), self._t(T, X.shape[0])
+        g = torch.empty(NPARAM, dtype=torch.float32, device=X.device)
+        loss = C.c_double()
+        L.check(self.lib.ocmp_mlp_grad(self.h, X.shape[0], C.c_void_p(_ptr(X)), C.c_void_p(_ptr(T)),
+                                       C.c_void_p(_ptr(g)), C.byref(loss)), 'mlp_grad')
+        return g, loss.value
+
+    def train_step(self, X: torch.Tensor, T: torch.Tensor, lr: float, want_loss: bool = False) -> Optional[float]:
+        X, T = self._x(X), self._t(T, X.shape[0])
+        loss = C.c_double()
+        L.check(self.lib.ocmp_mlp_train_step(self.h, X.shape[0], C.c_void_p(_ptr(X)), C.c_void_p(_ptr(T)),
+                                             C.c_float(lr), C.byref(loss) if want_loss else None),
+                'mlp_train_step')
+        return loss.value if want_loss else None
+
+    def sample(self, snap: torch.Tensor, n: int, seed: int, batch: int,
+               X: Optional[torch.Tensor] = None, T: Optional[torch.Tensor] = None):
+        snap = self._snap(snap)
+        if X is None:
+            X = torch.empty((batch, 4), dtype=torch.float32, device=snap.device)
+        if T is None:
+            T = torch.empty(batch, dtype=torch.float32, device=snap.device)
+        L.check(self.lib.ocmp_mlp_sample(self.h, C.c_void_p(_ptr(snap)), snap.shape[0], snap.stride(0), int(n),
+                                         C.c_uint64(seed & (2 ** 64 - 1)), int(batch), C.c_void_p(_ptr(X)),
+                                         C.c_void_p(_ptr(T))), 'mlp_sample')
+        return X, T
+
+    def guess(self, hist: Sequence[torch.Tensor], out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """hist = [u^n, u^{n-1}, u^{n-2}, u^{n-3}] (device f64); returns u^n + s*f(features)."""
+        if len(hist) != 4:
+            raise ValueError('the guess needs the last four states')
+        n = hist[0].numel()
+        for h in hist:
+            if h.dtype != torch.float64 or not h.is_cuda or h.numel() != n or not h.is_contiguous():
+                raise ValueError('history vectors must be contiguous float64 device tensors of one length')
+        if out is None:
+            out = torch.empty_like(hist[0])
+        L.check(self.lib.ocmp_mlp_guess(self.h, n, *[C.c_void_p(_ptr(h)) for h in hist], C.c_void_p(_ptr(out))),
+                'mlp_guess')
+        return out
+
+    # ------------------------------------------------------------ training loop
+    def train(self, snap: torch.Tensor, seg_off: Sequence[int], steps: int = 2000, batch: int = 65536,
+              lr: float = 2e-3, lr_min: float = 1e-4, seed: int = 0, log_every: int = 100) -> List[float]:
+        """Fit scales on the snapshots, then `steps` Adam steps on fresh random batches
+        with a cosine learning-rate decay; returns the sampled loss history."""
+        snap = self._snap(snap)
+        if batch > self.max_batch:
+            raise ValueError(f'batch {batch} exceeds max_batch {self.max_batch}')
+        self.fit_scales(snap, seg_off)
+        n = int(seg_off[-1])
+        X = torch.empty((batch, 4), dtype=torch.float32, device=snap.device)
+        T = torch.empty(batch, dtype=torch.float32, device=snap.device)
+        out = []
+        for it in range(steps):
+            self.sample(snap, n, seed * 1000003 + it, batch, X, T)
+            a = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * it / max(1, steps)))
+            want = (it % log_every == 0) or it == steps - 1
+            loss = self.train_step(X, T, a, want_loss=want)
+            if want:
+                out.append(loss)
+        self.losses.extend(out)
+        return out
+
+    # ------------------------------------------------------------ helpers
+    @staticmethod
+    def _snap(snap: torch.Tensor) -> torch.Tensor:
+        if snap.dim() != 2 or snap.dtype != torch.float64 or not snap.is_cuda or snap.stride(1) != 1:
+            raise ValueError('snapshots must be a (nsnap, n) float64 device tensor with unit row stride')
+        return snap
+
+    @staticmethod
+    def _x(X: torch.Tensor) -> torch.Tensor:
+        if X.dim() != 2 or X.shape[1] != 4 or X.dtype != torch.float32 or not X.is_cuda:
+            raise ValueError('X must be an (n, 4) float32 device tensor')
+        return X.contiguous()
+
+    @staticmethod
+    def _t(T: torch.Tensor, n: int) -> torch.Tensor:
+        if T.numel() != n or T.dtype != torch.float32 or not T.is_cuda:
+            raise ValueError('T must be an (n,) float32 device tensor')
+        return T.contiguous()
+
+
+def residual_norms(a, b: torch.Tensor, x: torch.Tensor) -> Tuple[float, float]:
+    """(||b - A x||, ||b||) over the free dofs, FP64 (ocmp_residual_norms)."""
+    out = (C.c_double * 2)()
+    L.check(a.ctx.lib.ocmp_residual_norms(a.h, C.c_void_p(_ptr(b)), C.c_void_p(_ptr(x)), out), 'residual_norms')
+    return out[0], out[1]
+
+
+def choose_guess(a, b: torch.Tensor, plain: torch.Tensor, learned: torch.Tensor) -> Tuple[str, float, float]:
+    """Fallback rule: keep the learned guess only if its FP64 residual is lower
+    than the plain guess's. Copies the winner into `plain` (the solve's start
+    vector) and returns (choice, plain residual, learned residual)."""
+    rp, _ = residual_norms(a, b, plain)
+    rl, _ = residual_norms(a, b, learned)
+    if math.isfinite(rl) and rl < rp:
+        plain.copy_(learned)
+        return 'learned', rp, rl
+    return 'plain', rp, rl

@@ -1,0 +1,106 @@
+"""Pins the CPU restatement of the per-DOF MLP (oracle/mlp.py): central finite
+differences, an independent torch-autograd gradient, torch's bfloat16 rounding
+and torch.optim.Adam. CPU only."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp
+
+
+def small_problem(seed=0, B=96):
+    rng = np.random.default_rng(seed)
+    p = mlp.init_params(seed).astype(np.float64)
+    # non-zero biases and a larger output layer so every parameter has a gradient
+    p += 0.05 * rng.standard_normal(p.size)
+    X = rng.standard_normal((B, 4))
+    T = rng.standard_normal(B)
+    return p, X, T
+
+
+def test_parameter_layout():
+    assert mlp.NPARAM == 33793
+    d = mlp.unpack(np.arange(mlp.NPARAM, dtype=np.float64))
+    assert d['W1'][0, 0] == 0 and d['b1'][0] == 512 and d['W2'][0, 0] == 640
+    assert d['b4'][0] == mlp.NPARAM - 1
+    np.testing.assert_array_equal(mlp.pack(d), np.arange(mlp.NPARAM))
+
+
+def test_gradient_matches_central_differences():
+    p, X, T = small_problem(1)
+    _, g = mlp.loss_and_grad(p, X, T)
+    rng = np.random.default_rng(7)
+    offs = [0, 512, 640, 17024, 17152, 33536, 33664, 33792]
+    idx = sorted(set(list(rng.integers(0, mlp.NPARAM, 40)) + offs + [o + 3 for o in offs[:-1]]))
+    h = 1e-6
+    for j in idx:
+        e = np.zeros_like(p)
+        e[j] = h
+        fd = (mlp.loss_and_grad(p + e, X, T)[0] - mlp.loss_and_grad(p - e, X, T)[0]) / (2 * h)
+        assert abs(fd - g[j]) <= 1e-6 * max(1.0, abs(g[j])) + 1e-9, (j, fd, g[j])
+
+
+def test_gradient_matches_torch_autograd():
+    p, X, T = small_problem(2)
+    loss, g = mlp.loss_and_grad(p, X, T)
+    d = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in mlp.unpack(p).items()}
+    a = torch.tensor(X)
+    for L in (1, 2, 3):
+        a = torch.relu(a @ d[f'W{L}'].T + d[f'b{L}'])
+    y = (a @ d['W4'].T + d['b4'])[:, 0]
+    lt = 0.5 * ((y - torch.tensor(T)) ** 2).mean()
+    lt.backward()
+    gt = np.concatenate([d[k].grad.numpy().ravel() for k, _ in mlp.SHAPES])
+    assert abs(lt.item() - loss) < 1e-14 * max(1, loss)
+    np.testing.assert_allclose(g, gt, rtol=1e-10, atol=1e-14)
+
+
+def test_bf16_round_matches_torch():
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.standard_normal(5000) * 10 ** rng.uniform(-30, 30, 5000),
+                        np.array([0.0, -0.0, 1.0, 1.00390625, 1.01171875, 65504.0, 3e38])]).astype(np.float32)
+    # exact ties: 1 + 2^-8 (to even: 1) and 1 + 3*2^-8 (to even: 1 + 2^-6... upwards)
+    ref = torch.tensor(v).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(mlp.bf16_round(v), ref)
+
+
+def test_bf16_mode_is_close_to_exact():
+    p, X, T = small_problem(4)
+    _, g = mlp.loss_and_grad(p, X, T)
+    _, gb = mlp.loss_and_grad(p, X, T, bf16=True)
+    rel = np.linalg.norm(gb - g) / np.linalg.norm(g)
+    assert 1e-5 < rel < 5e-2   # the rounding is there, and small
+
+
+def test_adam_matches_torch():
+    rng = np.random.default_rng(5)
+    p0 = rng.standard_normal(50)
+    grads = [rng.standard_normal(50) for _ in range(5)]
+    t = torch.tensor(p0, requires_grad=True)
+    opt = torch.optim.Adam([t], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p, m, v = p0.copy(), np.zeros(50), np.zeros(50)
+    for k, g in enumerate(grads, 1):
+        opt.zero_grad()
+        t.grad = torch.tensor(g)
+        opt.step()
+        p, m, v = mlp.adam(p, g, m, v, k, lr=1e-2)
+    np.testing.assert_allclose(p, t.detach().numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_features_and_scales_closed_form():
+    # states linear in time: u_j = a + b j; increments are b, so s = rms(b) and the
+    # target (u_{j+1} - u_j)/s equals the first feature exactly
+    rng = np.random.default_rng(6)
+    n, K = 40, 9
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    snap = np.stack([a + b * j for j in range(K)])
+    s, U = mlp.scales(snap, [0, 25, 40])
+    assert abs(s[0] - np.sqrt(np.mean(b[:25] ** 2))) < 1e-14
+    assert abs(s[1] - np.sqrt(np.mean(b[25:] ** 2))) < 1e-14
+    assert abs(U[1] - np.sqrt(np.mean(snap[1:, 25:] ** 2))) < 1e-14
+    X = mlp.features(snap[5], snap[4], snap[3], snap[2], s[0], U[0])
+    np.testing.assert_allclose(X[:, 0], b / s[0], rtol=1e-12)
+    np.testing.assert_allclose(X[:, 0], X[:, 1], rtol=1e-10)
+    # a constant segment gets scale 1
+    s2, _ = mlp.scales(np.ones((4, 3)), [0, 3])
+    assert s2[0] == 1.0
