@@ -180,12 +180,110 @@ def cpu_baseline(config, gpu_iters_fp64, n_full):
                       f'{gpu_iters_fp64} iterations'}
 
 
+def bench_learned(args, ctx):
+    """Config 4: transient INS (Taylor-Hood P2/P1, implicit Euler, Oseen Picard) on the
+    channel [0,4]x[0,1] with 8 seeded holes. A plain FP64 run records `--record`
+    steps; the per-DOF MLP is trained on them with the library's own kernels;
+    then, from the same state, `--warmup` + `--steps` steps run on the plain FP64
+    path and on the mixed+learned path (FP32 inner GMRES in iterative
+    refinement + learned initial guess with the residual fallback)."""
+    import torch
+    from opencmp_amd import _lib as L
+    from opencmp_amd.learned import GuessPolicy, MLPGuess
+    from opencmp_amd.mesh import channel_with_holes
+    from opencmp_amd.models import INS, const
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    n = args.n or 228
+    dt = 1e-3
+    t0 = time.perf_counter()
+    hm = channel_with_holes(n)
+    inflow = [(L.FN_POLY2, [0, 0, 6.0, 0, 0, -6.0]), const(0.0)]
+    zero = [const(0.0), const(0.0)]
+
+    def model(precision):
+        return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
+                   linear_solver='GMRes', preconditioner='schur', linear_tolerance=args.tol, restart=args.restart,
+                   nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
+                   inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, ctx=ctx)
+
+    m0 = model('fp64')
+    ndof = m0.fes.ndof
+    log(f'config 4 setup {time.perf_counter() - t0:.1f}s: {hm.nel} triangles, ndof {ndof}')
+    pol = GuessPolicy(ctx, ndof, max_snapshots=args.record + 1)
+    s0 = ImplicitEulerSolver(m0, dt=dt, guess=pol)
+    t0 = time.perf_counter()
+    rec_its = []
+    for k in range(args.record):
+        s0.step()
+        rec_its.append((m0.linear_iterations, m0.picard_iterations))
+        if k % 10 == 0 or k == args.record - 1:
+            log(f'record step {k}: picard {m0.picard_iterations} lin iters {m0.linear_iterations} '
+                f'({time.perf_counter() - t0:.1f}s)')
+    torch.cuda.synchronize()
+    t_record = time.perf_counter() - t0
+    net = MLPGuess(ctx, seed=0, max_batch=65536)
+    t0 = time.perf_counter()
+    losses = net.train(pol.snapshots(), m0.fes.block_offsets(), steps=args.train_steps, batch=65536, lr=2e-3,
+                       seed=1, log_every=max(1, args.train_steps // 10))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f'training {args.train_steps} Adam steps x 65536 samples: {t_train:.2f}s, loss {losses[0]:.3e} -> '
+        f'{losses[-1]:.3e}')
+    state = s0.gfu_prev.vec.clone()
+    ring = [h.clone() for h in pol.history()]
+    wind = m0.W.vec.clone()
+    del s0, m0, pol
+
+    def branch(precision, use_net, timing=False):
+        m = model(precision)
+        p = GuessPolicy(ctx, ndof, net=net if use_net else None)
+        for h in reversed(ring[1:]):
+            p.push(h)
+        s = ImplicitEulerSolver(m, dt=dt, gfu0=type(m.W)(m.fes, state.clone()), guess=p)
+        m.W.vec.copy_(wind)
+        rows = []
+        for k in range(args.warmup):
+            s.step()
+        torch.cuda.synchronize()
+        if timing:
+            ctx.timing(True)
+            ctx.timing_report()
+        t1 = time.perf_counter()
+        for k in range(args.steps):
+            r = s.step()
+            rows.append((m.linear_iterations, m.picard_iterations, r.rel_residual))
+        torch.cuda.synchronize()
+        dt_ = (time.perf_counter() - t1) / args.steps
+        rep = None
+        if timing:
+            rep = ctx.timing_report()
+            ctx.timing(False)
+        log(f'{precision}{"+learned" if use_net else ""}: {dt_:.4f} s/step, (iters, picard, res) {rows}, '
+            f'choices {[c for c, _, _ in s.guess_log]}')
+        return dt_, rows, s.gfu_prev.vec.clone(), s.guess_log, rep
+
+    s64, rows64, u64, _, _ = branch('fp64', False)
+    sml, rowsml, uml, glog, _ = branch('mixed', True)
+    _, _, _, _, rep = branch('mixed', True, timing=True)
+    err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
+    return dict(desc=(f'transient INS Taylor-Hood P2/P1, implicit Euler dt 1e-3 + Oseen Picard (<=3, tol 1e-6), '
+                      f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), GMRES({args.restart}) + '
+                      f'Schur-diagonal preconditioner'),
+                ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep,
+                learned=dict(record_steps=args.record, record_s=round(t_record, 2), train_steps=args.train_steps,
+                             train_batch=65536, train_s=round(t_train, 3), loss_first=losses[0],
+                             loss_last=losses[-1], choices=[c for c, _, _ in glog],
+                             residual_plain_vs_learned=[(rp, rl) for _, rp, rl in glog]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3))
+    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4))
+    ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
+    ap.add_argument('--train-steps', type=int, default=400, help='config 4: Adam steps')
     ap.add_argument('--n', type=int, default=None, help='mesh cells per direction (defaults: 32 / 512 / 64)')
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=50)
@@ -196,10 +294,36 @@ def main():
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    n = args.n or {1: 32, 2: 512, 3: 64}[args.config]
+    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228}[args.config]
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
+    if args.config == 4:
+        r = bench_learned(args, ctx)
+        rep = r['rep']
+        dom = max((k for k in rep if rep[k][2] > 0), key=lambda k: rep[k][1])
+        cnt, ms, byt = rep[dom]
+        achieved = byt / (ms * 1e-3) / 1e9
+        out = {
+            'metric': METRIC, 'value': round(r['sml'], 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
+            'warmup': args.warmup, 'ms_per_step': round(r['sml'] * 1e3, 3), 'higher_is_better': False,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32+bf16', 'data': 'synthetic',
+            'config': {'workload': f"{r['desc']} to rel residual {args.tol:g}", 'config_index': 4,
+                       'ndof': r['ndof'], 'tol': args.tol},
+            'plain_fp64': {'s_per_step': round(r['s64'], 5), 'iters_picard_res': r['rows64']},
+            'mixed_learned': {'s_per_step': round(r['sml'], 5), 'iters_picard_res': r['rowsml'],
+                              'rel_error_vs_fp64': r['err']},
+            'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
+            'dofs_per_s': round(r['ndof'] / r['sml'], 1),
+            'learned': r['learned'],
+            'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
+                         'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
+                         'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)},
+            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
+                                      for k, (c, m_, b_) in rep.items()},
+        }
+        print(json.dumps(out), flush=True)
+        return
     t0 = time.perf_counter()
     hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')

@@ -242,6 +242,14 @@ enum { OCMP_FP64 = 0, OCMP_MIXED = 1 };
  * precision OCMP_MIXED the library also keeps the FP32 copies used by the
  * inner solve of iterative refinement. */
 int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **out);
+/* Diagonal preconditioner for the saddle-point systems of the Stokes/INS models
+ * [[A, B^T], [B, -C]] (velocity blocks A, pressure blocks C = 1e-10 p q,
+ * opencmp/models/ins.py:178-270): 1/a_ii on the velocity rows and
+ * -1/(C_ii + sum_j B_ij^2/A_jj) on the rows of the scalar blocks set in
+ * schur_blocks (bit k = block k), the diagonal of the Schur complement
+ * -(C + B diag(A)^{-1} B^T). The reference solves these systems with a direct
+ * factorisation (its 'direct' default); this replaces it on the Krylov path. */
+int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_blocks, ocmp_precond **out);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);

@@ -92,6 +92,7 @@ SIGNATURES = {
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
+    'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],

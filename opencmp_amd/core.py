@@ -167,6 +167,10 @@ class FESpace:
         L.check(self.ctx.lib.ocmp_space_element_dofs(self.h, out.ctypes.data), 'space_element_dofs')
         return out
 
+    def block_offsets(self) -> List[int]:
+        """[off_0, off_1, ..., ndof]: the dof ranges of the scalar blocks."""
+        return [b[0] for b in self.blocks] + [self.ndof]
+
     def block_slice(self, blk: int) -> slice:
         off, n, _, _ = self.blocks[blk]
         return slice(off, off + n)
@@ -444,15 +448,27 @@ class Preconditioner:
 
     KINDS = {'local': None, 'jacobi': L.PRE_JACOBI, 'block_jacobi': L.PRE_BLOCK_JACOBI, 'none': L.PRE_NONE}
 
-    def __init__(self, a: BilinearForm, kind: str = 'local', precision: str = 'fp64'):
+    def __init__(self, a: BilinearForm, kind: str = 'local', precision: str = 'fp64',
+                 schur_blocks: Sequence[int] = ()):
+        """kind 'schur': point Jacobi on all rows except those of `schur_blocks` (the
+        pressure blocks of a saddle-point system), which get the diagonal of the
+        Schur complement (ocmp_precond_create_schur)."""
         self.a = a
-        k = self.KINDS[kind]
-        if k is None:
-            k = L.PRE_BLOCK_JACOBI if a.is_dg else L.PRE_JACOBI
-        self.kind = k
         self.precision = L.MIXED if precision == 'mixed' else L.FP64
         h = C.c_void_p()
-        L.check(a.ctx.lib.ocmp_precond_create(a.h, k, self.precision, C.byref(h)), 'precond_create')
+        if kind == 'schur':
+            mask = 0
+            for b in schur_blocks:
+                mask |= 1 << int(b)
+            self.kind = L.PRE_JACOBI
+            L.check(a.ctx.lib.ocmp_precond_create_schur(a.h, self.precision, C.c_uint64(mask), C.byref(h)),
+                    'precond_create_schur')
+        else:
+            k = self.KINDS[kind]
+            if k is None:
+                k = L.PRE_BLOCK_JACOBI if a.is_dg else L.PRE_JACOBI
+            self.kind = k
+            L.check(a.ctx.lib.ocmp_precond_create(a.h, k, self.precision, C.byref(h)), 'precond_create')
         self.h = h
 
     def Update(self):

@@ -871,6 +871,23 @@ int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **
   });
 }
 
+int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_blocks, ocmp_precond **out) {
+  return guard([&] {
+    OCMP_REQUIRE(A && out, "NULL argument");
+    OCMP_REQUIRE(precision == OCMP_FP64 || precision == OCMP_MIXED, "unknown precision");
+    const auto &bl = A->A.space->blocks;
+    OCMP_REQUIRE(schur_blocks != 0, "empty Schur block mask");
+    OCMP_REQUIRE(bl.size() >= 64 || (schur_blocks >> bl.size()) == 0, "Schur block mask names a missing block");
+    OCMP_REQUIRE(!A->A.dg, "the Schur-diagonal preconditioner is for H1 (Taylor-Hood) systems");
+    auto *P = new ocmp_precond();
+    P->P.A = &A->A;
+    P->P.kind = OCMP_PRE_JACOBI;
+    P->P.precision = precision;
+    P->P.schur_blocks = schur_blocks;
+    *out = P;
+  });
+}
+
 int ocmp_precond_update(ocmp_precond *P) {
   return guard([&] {
     OCMP_REQUIRE(P, "NULL argument");

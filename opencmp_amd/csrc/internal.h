@@ -215,6 +215,10 @@ struct Matrix {
 struct Precond {
   Matrix *A;
   int kind, precision;
+  // saddle-point systems: rows of these scalar blocks get the diagonal Schur complement
+  // approximation -1/(C_ii + sum_j B_ij^2 / A_jj) instead of 1/a_ii (kind stays Jacobi)
+  uint64_t schur_blocks = 0;
+  DBuf<uint8_t> schur_flag;  // n: 1 on rows of the Schur blocks
   DBuf<double> dinv;     // Jacobi: n (0 on Dirichlet dofs)
   DBuf<float> dinv32;
   DBuf<double> binv;     // block Jacobi: nel*nb*nb

@@ -481,6 +481,29 @@ __global__ void k_jacobi_diag(int64_t n, const int32_t *__restrict__ rp, const i
   if (dinv32) dinv32[r] = (float)v;
 }
 
+// Schur rows r (flag[r] = 1): dinv[r] = -1 / (-a_rr + sum_{j not Schur, free} a_rj^2 / a_jj),
+// the diagonal of -(C + B diag(A)^{-1} B^T) for the system [[A, B^T], [B, -C]]. Reads
+// dinv only on non-Schur columns, which k_jacobi_diag has filled.
+__global__ void k_schur_diag(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                             const double *__restrict__ val, const uint8_t *__restrict__ mask,
+                             const uint8_t *__restrict__ flag, double *__restrict__ dinv,
+                             float *__restrict__ dinv32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !flag[r]) return;
+  double s = 0.0;
+  for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+    const int32_t c = col[j];
+    const double a = val[j];
+    if (c == r)
+      s -= a;
+    else if (!flag[c] && mask[c])
+      s += a * a * dinv[c];
+  }
+  const double v = !mask[r] || !(s > 0.0) ? 0.0 : -1.0 / s;
+  dinv[r] = v;
+  if (dinv32) dinv32[r] = (float)v;
+}
+
 __global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -947,6 +970,19 @@ void precond_update(Precond *P) {
     if (P->kind == OCMP_PRE_JACOBI)
       k_jacobi_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_row_ptr.p, A->d_col.p, A->d_val.p, A->space->d_free.p,
                                                  P->dinv.p, mixed ? P->dinv32.p : nullptr);
+    if (P->kind == OCMP_PRE_JACOBI && P->schur_blocks) {
+      if (!P->schur_flag.p) {
+        std::vector<uint8_t> f(A->n, 0);
+        const auto &bl = A->space->blocks;
+        for (size_t k = 0; k < bl.size(); ++k)
+          if (P->schur_blocks >> k & 1)
+            for (int64_t i = 0; i < bl[k].n1; ++i) f[bl[k].dof_off + i] = 1;
+        P->schur_flag.upload(f, st);
+        OCMP_HIP(hipStreamSynchronize(st));
+      }
+      k_schur_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_row_ptr.p, A->d_col.p, A->d_val.p, A->space->d_free.p,
+                                                P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+    }
     if (mixed) {
       A->d_val32.alloc(A->nnz);
       k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);

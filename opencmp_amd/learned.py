@@ -219,3 +219,40 @@ def choose_guess(a, b: torch.Tensor, plain: torch.Tensor, learned: torch.Tensor)
         plain.copy_(learned)
         return 'learned', rp, rl
     return 'plain', rp, rl
+
+
+class GuessPolicy:
+    """The last four accepted states of a transient run (device ring), optional
+    snapshot recording for training, and the trained network that turns the
+    history into the next step's initial guess."""
+
+    def __init__(self, ctx: Context, n: int, net: Optional[MLPGuess] = None, max_snapshots: int = 0):
+        self.ctx = ctx
+        self.n = int(n)
+        self.net = net
+        self.ring = [ctx.zeros(self.n) for _ in range(4)]
+        self.count = 0
+        self.snap = (torch.empty((max_snapshots, self.n), dtype=torch.float64, device=ctx.torch_device)
+                     if max_snapshots else None)
+        self.nsnap = 0
+        self.out = ctx.empty(self.n)
+
+    def push(self, u: torch.Tensor) -> None:
+        self.ring[self.count % 4].copy_(u)
+        self.count += 1
+        if self.snap is not None and self.nsnap < self.snap.shape[0]:
+            self.snap[self.nsnap].copy_(u)
+            self.nsnap += 1
+
+    def history(self) -> List[torch.Tensor]:
+        """[u^n, u^{n-1}, u^{n-2}, u^{n-3}]"""
+        return [self.ring[(self.count - 1 - k) % 4] for k in range(4)]
+
+    def ready(self) -> bool:
+        return self.net is not None and self.count >= 4
+
+    def propose(self) -> torch.Tensor:
+        return self.net.guess(self.history(), out=self.out)
+
+    def snapshots(self) -> torch.Tensor:
+        return self.snap[:self.nsnap]

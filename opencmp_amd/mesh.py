@@ -155,3 +155,140 @@ def unit_cube(n: int) -> Mesh:
         marks.append(np.full(f.shape[0], mk, dtype=np.int32))
     bnd = np.concatenate(faces).astype(np.int32)
     return Mesh(coords, elems, bnd, np.concatenate(marks), names)
+
+
+def _hole_centres(n_holes, r_range, h, seed, length, height, x_margin=0.25, max_tries=100000):
+    """Radii r ~ U[r_range] and centres rejection-sampled from default_rng(seed):
+    clear of the walls by 2h and of each other by a gap of at least 2h."""
+    rng = np.random.default_rng(seed)
+    holes = []
+    tries = 0
+    while len(holes) < n_holes:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError('could not place the holes; lower n_holes or the radii')
+        r = rng.uniform(*r_range)
+        cx = rng.uniform(x_margin + r, length - x_margin - r)
+        cy = rng.uniform(r + 2 * h, height - r - 2 * h)
+        if all(np.hypot(cx - x, cy - y) >= r + q + 2 * h for x, y, q in holes):
+            holes.append((cx, cy, r))
+    return holes
+
+
+def channel_with_holes(n: int = 228, n_holes: int = 8, r_range=(0.05, 0.12), seed: int = 0,
+                       length: float = 4.0, height: float = 1.0) -> Mesh:
+    """Channel [0,length] x [0,height] with circular holes, body-fitted by snapping.
+
+    A structured grid of (length*n) x (height*n) squares split along their
+    (0,0)->(1,1) diagonals; triangles whose centroid lies in a hole are removed,
+    and every vertex on a hole's staircase boundary is moved radially onto the
+    circle. Boundary names: ``inlet`` (x=0), ``outlet`` (x=length), ``wall``
+    (y=0 and y=height) and ``cyl`` (the holes), the names of the channel meshes
+    of the reference's INS examples. n=228 gives about 4e5 triangles.
+    """
+    h = 1.0 / n
+    nx, ny = int(round(length * n)), int(round(height * n))
+    holes = _hole_centres(n_holes, r_range, h, seed, length, height)
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing='xy')
+    coords = np.stack([i.ravel() * (length / nx), j.ravel() * (height / ny)], axis=1).astype(np.float64)
+    coords[:, 0][np.isclose(coords[:, 0], length)] = length
+    coords[:, 1][np.isclose(coords[:, 1], height)] = height
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing='xy')
+    ci = ci.ravel(); cj = cj.ravel()
+    p1 = cj * (nx + 1) + ci
+    t1 = np.stack([p1, p1 + 1, p1 + nx + 2], 1)
+    t2 = np.stack([p1, p1 + nx + 2, p1 + nx + 1], 1)
+    elems = np.empty((2 * nx * ny, 3), dtype=np.int64)
+    elems[0::2] = t1
+    elems[1::2] = t2
+    cen = coords[elems].mean(1)
+    keep = np.ones(len(elems), dtype=bool)
+    for cx, cy, r in holes:
+        keep &= np.hypot(cen[:, 0] - cx, cen[:, 1] - cy) >= r
+    elems = elems[keep]
+    # boundary edges of what is left, and the hole ones among them
+    e = np.concatenate([elems[:, [0, 1]], elems[:, [1, 2]], elems[:, [2, 0]]])
+    es = np.sort(e, 1)
+    key = es[:, 0] * (nx + 1) * (ny + 1) + es[:, 1]
+    uk, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    bmask = cnt[inv] == 1
+    bedges = e[bmask]                      # oriented as in their element (counter-clockwise)
+    mid = coords[bedges].mean(1)
+    outer = (np.isclose(mid[:, 0], 0.0) | np.isclose(mid[:, 0], length) | np.isclose(mid[:, 1], 0.0) |
+             np.isclose(mid[:, 1], height))
+    # Fit the holes: hole boundary vertices move radially onto their circle and
+    # the vertices near a hole are relaxed towards the mean of their neighbours.
+    # Every move is accepted only if no triangle's area drops below a quarter of
+    # the grid's, so no triangle inverts.
+    hv = np.unique(bedges[~outer])
+    deg = np.bincount(bedges[~outer].ravel(), minlength=len(coords))
+    if np.any(deg[hv] != 2):
+        raise RuntimeError('a hole boundary touches itself at a vertex; change n or the seed')
+    owner = np.zeros(len(coords), dtype=np.int64)
+    best = np.full(len(coords), np.inf)
+    for k, (cx, cy, r) in enumerate(holes):
+        dd = np.abs(np.hypot(coords[:, 0] - cx, coords[:, 1] - cy) - r)
+        upd = dd < best
+        best[upd] = dd[upd]
+        owner[upd] = k
+    hc = np.array([[cx, cy] for cx, cy, _ in holes])
+    hr = np.array([r for _, _, r in holes])
+    near = best < 6 * h
+    near[np.unique(bedges[outer])] = False
+    used = np.zeros(len(coords), dtype=bool)
+    used[elems.ravel()] = True
+    inner = np.nonzero(near & used)[0]
+    inner = np.setdiff1d(inner, hv)
+    ue = np.unique(np.sort(e, 1), axis=0)
+    ue = ue[near[ue[:, 0]] | near[ue[:, 1]]]
+    loc = np.nonzero(np.isin(elems, np.nonzero(near)[0]).any(1))[0]
+    min_area = 0.25 * 0.5 * (length / nx) * (height / ny)
+
+    def tri_area(c, el):
+        P = c[el]
+        return 0.5 * ((P[:, 1, 0] - P[:, 0, 0]) * (P[:, 2, 1] - P[:, 0, 1]) -
+                      (P[:, 2, 0] - P[:, 0, 0]) * (P[:, 1, 1] - P[:, 0, 1]))
+
+    def safe_move(c, verts, target):
+        new = c.copy()
+        new[verts] = target
+        moved = np.zeros(len(c), dtype=bool)
+        moved[verts] = True
+        el = elems[loc]
+        while True:
+            bad = tri_area(new, el) < min_area
+            if not bad.any():
+                return new
+            rv = np.unique(el[bad])
+            rv = rv[moved[rv]]
+            new[rv] = c[rv]
+            moved[rv] = False
+
+    for _ in range(40):
+        d = coords[hv] - hc[owner[hv]]
+        coords = safe_move(coords, hv, hc[owner[hv]] + d * (hr[owner[hv]] / np.hypot(d[:, 0], d[:, 1]))[:, None])
+        acc = np.zeros_like(coords)
+        cnt2 = np.zeros(len(coords))
+        np.add.at(acc, ue[:, 0], coords[ue[:, 1]])
+        np.add.at(acc, ue[:, 1], coords[ue[:, 0]])
+        np.add.at(cnt2, ue[:, 0], 1.0)
+        np.add.at(cnt2, ue[:, 1], 1.0)
+        coords = safe_move(coords, inner, acc[inner] / cnt2[inner][:, None])
+    d = coords[hv] - hc[owner[hv]]
+    fit_err = np.abs(np.hypot(d[:, 0], d[:, 1]) - hr[owner[hv]]).max() if len(hv) else 0.0
+    # renumber the vertices still in use
+    used = np.unique(elems)
+    remap = -np.ones(len(coords), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    coords = coords[used]
+    elems = remap[elems].astype(np.int32)
+    bedges = remap[bedges].astype(np.int32)
+    names = ['inlet', 'outlet', 'wall', 'cyl']
+    markers = np.full(len(bedges), 3, dtype=np.int32)
+    markers[outer & np.isclose(mid[:, 0], 0.0)] = 0
+    markers[outer & np.isclose(mid[:, 0], length)] = 1
+    markers[outer & (np.isclose(mid[:, 1], 0.0) | np.isclose(mid[:, 1], height))] = 2
+    m = Mesh(coords, elems, bedges, markers, names)
+    m.holes = holes
+    m.hole_fit_error = float(fit_err)
+    return m

@@ -125,7 +125,14 @@ class Model:
         """ngs.Preconditioner(a, 'local') (opencmp/models/base_model.py:365-383)."""
         if self.preconditioner in (None, 'None'):
             return None
+        if self.preconditioner == 'schur':
+            return C.Preconditioner(a_assembled, 'schur', precision=self.precision,
+                                    schur_blocks=self.schur_blocks())
         return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
+
+    def schur_blocks(self) -> List[int]:
+        """Scalar blocks of the constraint (pressure) components of a saddle-point model."""
+        return []
 
     def linear_solve(self, a_assembled: C.BilinearForm, L_assembled: C.LinearForm,
                      precond: Optional[C.Preconditioner], gfu: C.GridFunction) -> C.SolveResult:
@@ -257,6 +264,9 @@ class Stokes(Model):
     def _indefinite_or_nonsymmetric(self):
         return True
 
+    def schur_blocks(self):
+        return list(self.fes.comp_blocks[1])
+
     def _construct_fes(self):
         d = self.host_mesh.dim
         return C.FESpace(self.mesh, [C.H1(self.interp_ord, self.dirichlet_names('u'), ncomp=d),
@@ -317,11 +327,13 @@ class INS(Stokes):
         solve, stop when ||W - u||_L2 < abs + rel * ||u||_L2."""
         it = 1
         res = None
+        self.linear_iterations = 0
         while True:
             self.apply_dirichlet_bcs_to(gfu)
             if reassemble is not None:
                 reassemble()
             res = self.linear_solve(a, Lf, pre, gfu)
+            self.linear_iterations += res.iters
             err = self.l2_diff(self.W, gfu, 'u')
             unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
             it += 1

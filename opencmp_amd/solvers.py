@@ -69,8 +69,13 @@ class ImplicitEulerSolver:
     """Fixed-step implicit Euler (time_integration_schemes.py:79-132)."""
 
     def __init__(self, model: Model, dt: float, t_range=(0.0, 1.0), gfu0: Optional[C.GridFunction] = None,
-                 reassemble_each_step: Optional[bool] = None):
+                 reassemble_each_step: Optional[bool] = None, guess=None):
+        """guess: an optional :class:`opencmp_amd.learned.GuessPolicy` that records the
+        solution history (and snapshots for training) and, once it holds a trained
+        network, proposes the initial guess of each step (kept only if it lowers the
+        FP64 residual)."""
         self.model = model
+        self.guess = guess
         self.dt = dt
         self.t0, self.t1 = t_range
         self.gfu = C.GridFunction(model.fes)
@@ -89,6 +94,9 @@ class ImplicitEulerSolver:
         self.M.Assemble()
         self._assemble_a()
         self.step_results = []
+        self.guess_log = []
+        if self.guess is not None:
+            self.guess.push(self.gfu_prev.vec)
 
     def _assemble_a(self):
         self.a.Assemble()
@@ -110,11 +118,27 @@ class ImplicitEulerSolver:
         if self.reassemble_each_step and not self.model.nonlinear:
             self._assemble_a()
         self.model.apply_dirichlet_bcs_to(self.gfu)
+        if self.guess is not None and self.guess.ready():
+            self._learned_guess()
         reassemble = self._assemble_a if self.model.nonlinear else None
         res = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble)
         self.step_results.append(res)
         self.gfu_prev.vec.copy_(self.gfu.vec)
+        if self.guess is not None:
+            self.guess.push(self.gfu.vec)
         return res
+
+    def _learned_guess(self):
+        """Replace the plain initial guess (previous solution + Dirichlet lift) by the
+        network's prediction when that has the lower FP64 residual against the
+        current operator; a nonlinear model also linearises about the chosen guess."""
+        from .learned import choose_guess
+        g = C.GridFunction(self.model.fes, self.guess.propose())
+        self.model.apply_dirichlet_bcs_to(g)
+        choice, rp, rl = choose_guess(self.a, self.L.vec, self.gfu.vec, g.vec)
+        if choice == 'learned' and self.model.nonlinear:
+            self.model.W.vec.copy_(self.gfu.vec)
+        self.guess_log.append((choice, rp, rl))
 
     def solve(self, nsteps: Optional[int] = None) -> C.GridFunction:
         if nsteps is None:

@@ -1,0 +1,88 @@
+"""Transient INS on the channel with holes (implicit Euler + Oseen Picard,
+opencmp/models/ins.py:323-359) with the Schur-diagonal preconditioner, and what
+the learned initial guess gains over the plain one on the same steps."""
+import numpy as np
+import pytest
+import torch
+
+from opencmp_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print('[gpu-test]', *a, flush=True)
+
+
+def channel_model(ctx, n, precision='fp64', tol=1e-8):
+    from opencmp_amd.mesh import channel_with_holes
+    from opencmp_amd.models import INS, const
+    hm = channel_with_holes(n)
+    inflow = [(L.FN_POLY2, [0, 0, 6.0, 0, 0, -6.0]), const(0.0)]     # u = (4 * 1.5 y (1 - y), 0)
+    zero = [const(0.0), const(0.0)]
+    return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
+               linear_solver='GMRes', preconditioner='schur', linear_tolerance=tol, restart=100,
+               nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision, ctx=ctx)
+
+
+def test_channel_ins_transient_schur(gpu_ctx):
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    say('channel INS n=16, 4 implicit Euler steps, GMRES + Schur diagonal')
+    m = channel_model(gpu_ctx, 16)
+    s = ImplicitEulerSolver(m, dt=1e-2)
+    for k in range(4):
+        r = s.step()
+        say(f'  step {k}: picard {m.picard_iterations} err {m.picard_error:.2e} lin iters {m.linear_iterations} '
+            f'res {r.rel_residual:.2e}')
+        assert r.converged and r.rel_residual <= 1e-8
+    u = s.gfu_prev.vec
+    assert torch.isfinite(u).all()
+    # inflow profile imposed: max velocity 1.5 at the inlet
+    assert abs(float(u.abs().max()) - 1.5) < 0.6
+
+
+def test_learned_guess_reduces_work(gpu_ctx):
+    """Train the MLP on 40 plain steps, then run 8 more steps from the same state
+    with the plain and with the learned guess: same answer, fewer iterations."""
+    from opencmp_amd.learned import GuessPolicy, MLPGuess
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    say('learned guess: 40 recorded steps, train, 8 compared steps')
+    m = channel_model(gpu_ctx, 16)
+    n = m.fes.ndof
+    pol = GuessPolicy(gpu_ctx, n, max_snapshots=64)
+    s = ImplicitEulerSolver(m, dt=1e-2, guess=pol)
+    for _ in range(40):
+        s.step()
+    state = s.gfu_prev.vec.clone()
+    ring = [h.clone() for h in pol.history()]
+    net = MLPGuess(gpu_ctx, seed=0, max_batch=16384)
+    seg = [int(v) for v in m.fes.block_offsets()]
+    losses = net.train(pol.snapshots(), seg, steps=600, batch=16384, lr=2e-3, seed=1)
+    say('  loss', losses[0], '->', losses[-1])
+    assert losses[-1] < 0.2 * losses[0]
+
+    def run(use_net):
+        mm = channel_model(gpu_ctx, 16)
+        p2 = GuessPolicy(gpu_ctx, n, net=net if use_net else None)
+        for h in reversed(ring[1:]):  # u^{n-3}, u^{n-2}, u^{n-1}; the solver pushes u^n itself
+            p2.push(h)
+        g0 = type(s.gfu_prev)(mm.fes, state.clone())
+        ss = ImplicitEulerSolver(mm, dt=1e-2, gfu0=g0, guess=p2)
+        mm.W.vec.copy_(state)
+        its, pic = 0, 0
+        for _ in range(8):
+            ss.step()
+            its += mm.linear_iterations
+            pic += mm.picard_iterations
+        return ss.gfu_prev.vec.clone(), its, pic, ss.guess_log
+
+    u_plain, it_plain, pic_plain, _ = run(False)
+    u_learn, it_learn, pic_learn, log = run(True)
+    say(f'  plain: {it_plain} iters / {pic_plain} picard; learned: {it_learn} iters / {pic_learn} picard')
+    say('  choices', [c for c, _, _ in log])
+    assert all(rl < rp for c, rp, rl in log if c == 'learned')
+    assert sum(c == 'learned' for c, _, _ in log) >= 6
+    assert it_learn < it_plain
+    rel = float(torch.linalg.norm(u_learn - u_plain) / torch.linalg.norm(u_plain))
+    say('  rel diff of the final states', rel)
+    assert rel < 1e-5
