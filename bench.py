@@ -82,7 +82,8 @@ def build_problem(config, n, ctx):
     raise SystemExit(f'unknown config {config}')
 
 
-def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes', inner_tol=None):
+def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes', inner_tol=None,
+             maxit=500000):
     import torch
     pre = C.Preconditioner(a, 'local', precision=precision)
     x = C.GridFunction(gs)
@@ -95,7 +96,7 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
         pre.Update()
         x.vec.zero_()
         return C.solve(a, f.vec, x.vec, pre, solver, tol=tol, restart=restart, precision=precision,
-                       maxit=500000, inner_tol=inner)
+                       maxit=maxit, inner_tol=inner)
 
     for i in range(warmup):
         res = step()
@@ -251,7 +252,7 @@ def bench_learned(args, ctx):
         t1 = time.perf_counter()
         for k in range(args.steps):
             r = s.step()
-            rows.append((m.linear_iterations, m.picard_iterations, r.rel_residual))
+            rows.append((m.linear_iterations, m.picard_iterations, r.rel_residual, m.picard_log))
         torch.cuda.synchronize()
         dt_ = (time.perf_counter() - t1) / args.steps
         rep = None
@@ -289,6 +290,9 @@ def main():
     ap.add_argument('--restart', type=int, default=50)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--inner-tol', type=float, default=None, help='mixed path: relative reduction per inner solve')
+    ap.add_argument('--maxit', type=int, default=500000, help='Krylov iteration cap (short profiling runs)')
+    ap.add_argument('--calib-copy', type=int, default=0, metavar='MB',
+                    help='first copy MB megabytes with a plain device copy (HBM counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
@@ -298,6 +302,16 @@ def main():
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
+    if args.calib_copy:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the HBM counters
+        nb = args.calib_copy * (1 << 20) // 8
+        src = torch.ones(nb, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f'calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per dispatch')
+        del src, dst
     if args.config == 4:
         r = bench_learned(args, ctx)
         rep = r['rep']
@@ -327,7 +341,7 @@ def main():
     t0 = time.perf_counter()
     hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
-    kw = dict(solver=solver, inner_tol=args.inner_tol)
+    kw = dict(solver=solver, inner_tol=args.inner_tol, maxit=args.maxit)
     s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps, args.warmup, **kw)
     smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps, args.warmup,
                                       **kw)
@@ -346,6 +360,14 @@ def main():
     roof = {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
             'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
             'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)}
+    # HBM bytes per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to the algorithmic bytes,
+    # measured with rocprofv3 --pmc on this workload, times this run's algorithmic bytes per launch
+    tf = os.path.join(ROOT, 'profiles', 'hbm_traffic_dg_p2_512.json')
+    if args.config == 2 and n == 512 and os.path.exists(tf):
+        ratios = json.load(open(tf))['hbm_bytes_over_algorithmic']
+        if dom in ratios:
+            roof['traffic'] = round(ratios[dom] * byt / cnt)
+            roof['traffic_source'] = 'profiles/dg_p2_512_gmres_hbm_traffic.md'
     out = {
         'metric': METRIC, 'value': round(smx, 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(smx * 1e3, 3), 'higher_is_better': False, 'scaling': 'weak',

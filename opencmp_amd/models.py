@@ -328,6 +328,7 @@ class INS(Stokes):
         it = 1
         res = None
         self.linear_iterations = 0
+        self.picard_log = []      # (iterations, refinement steps, initial rel residual, Picard error) per iteration
         while True:
             self.apply_dirichlet_bcs_to(gfu)
             if reassemble is not None:
@@ -335,6 +336,7 @@ class INS(Stokes):
             res = self.linear_solve(a, Lf, pre, gfu)
             self.linear_iterations += res.iters
             err = self.l2_diff(self.W, gfu, 'u')
+            self.picard_log.append((res.iters, res.outer, float('%.3g' % res.initial_rel_residual), float('%.3g' % err)))
             unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
             it += 1
             self.W.vec.copy_(gfu.vec)

