@@ -224,7 +224,9 @@ def bench_learned(args, ctx):
     t_record = time.perf_counter() - t0
     net = MLPGuess(ctx, seed=0, max_batch=65536)
     t0 = time.perf_counter()
-    losses = net.train(pol.snapshots(), m0.fes.block_offsets(), steps=args.train_steps, batch=65536, lr=2e-3,
+    # the network predicts the velocity blocks; the pressure keeps the plain guess
+    seg = m0.fes.block_offsets()[:len(m0.fes.comp_blocks[0]) + 1]
+    losses = net.train(pol.snapshots(), seg, steps=args.train_steps, batch=65536, lr=2e-3,
                        seed=1, log_every=max(1, args.train_steps // 10))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -237,7 +239,7 @@ def bench_learned(args, ctx):
 
     def branch(precision, use_net, timing=False):
         m = model(precision)
-        p = GuessPolicy(ctx, ndof, net=net if use_net else None)
+        p = GuessPolicy(ctx, ndof, net=net if use_net else None, n_learned=seg[-1])
         for h in reversed(ring[1:]):
             p.push(h)
         s = ImplicitEulerSolver(m, dt=dt, gfu0=type(m.W)(m.fes, state.clone()), guess=p)

@@ -232,6 +232,9 @@ int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *re
  * learned-guess fallback compares it between candidate initial guesses. */
 int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double *out);
 
+/* r = b - A x on the free dofs, 0 on the Dirichlet dofs (device f64), *norm = ||r||_2 (host, syncs). */
+int ocmp_residual(ocmp_matrix *A, const double *b, const double *x, double *r, double *norm);
+
 /* ----------------------------------------------------- preconditioner, solve */
 
 enum { OCMP_PRE_NONE = 0, OCMP_PRE_JACOBI = 1, OCMP_PRE_BLOCK_JACOBI = 2 };
@@ -255,7 +258,7 @@ int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_bloc
 int ocmp_precond_update(ocmp_precond *P);
 int ocmp_precond_destroy(ocmp_precond *P);
 
-enum { OCMP_SOLVER_CG = 0, OCMP_SOLVER_GMRES = 1, OCMP_SOLVER_RICHARDSON = 2 };
+enum { OCMP_SOLVER_CG = 0, OCMP_SOLVER_GMRES = 1, OCMP_SOLVER_RICHARDSON = 2, OCMP_SOLVER_FGMRES = 3 };
 
 typedef struct {
   int32_t solver;      /* OCMP_SOLVER_*                                          */
@@ -273,7 +276,7 @@ typedef struct {
   int32_t iters;        /* Krylov iterations (all inner solves)           */
   int32_t outer;        /* refinement steps (1 for FP64)                  */
   int32_t converged;
-  int32_t reserved;
+  int32_t fallback;     /* FGMRES with a learned correction: 1 if it fell back to the plain one */
   double rel_residual;  /* final ||b-Ax||/||b||, FP64, free dofs          */
   double initial_rel_residual;
 } ocmp_solve_stats;
@@ -330,6 +333,47 @@ int ocmp_mlp_train_step(ocmp_mlp *mlp, int64_t n, const float *X, const float *T
 /* out = u0 + s * f(features(u0, u1, u2, u3)) over n dofs (device f64); out may alias none of u0..u3 */
 int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1, const double *u2,
                    const double *u3, double *out);
+
+/* ----------------------------------------------- learned preconditioner correction */
+
+/* 3-level U-net on the vertex grid of an N^3 x 6-tet structured mesh (vertex id =
+ * grid index, x fastest), 4 channels: the vertex dofs of four scalar blocks (the
+ * velocity components and the pressure of the diffuse-interface Stokes model,
+ * opencmp/models/stokes_dim.py:32-153). Used by OCMP_SOLVER_FGMRES as the
+ * preconditioner M v = D^{-1} v + E U(R v) when attached to a point-Jacobi /
+ * Schur-diagonal preconditioner; FGMRES falls back to D^{-1} alone when the
+ * learned cycles reduce the residual more slowly. The reference has no
+ * counterpart (its preconditioners: opencmp/models/base_model.py:365-383).
+ * Layers (3x3x3, padding 1): conv 4->8, conv/2 8->16, conv 16->16, conv/2 16->32,
+ * conv 32->32, tconv*2 32->16 (+skip), tconv*2 16->8 (+skip), conv 8->4; ReLU
+ * except the last; bf16 weights and activations, fp32 sums. Parameters float32,
+ * per layer W then b, W [out][in][27] (conv) or [in][out][27] (tconv). */
+#define OCMP_UNET_NPARAM 70980
+typedef struct ocmp_unet ocmp_unet;
+
+int ocmp_unet_create(ocmp_ctx *ctx, int N, const float *params, ocmp_unet **out);
+int ocmp_unet_destroy(ocmp_unet *u);
+int ocmp_unet_get_params(ocmp_unet *u, float *params);
+int ocmp_unet_set_params(ocmp_unet *u, const float *params);
+/* channel c reads/writes dofs dof_off[c] + vertex; input x_c = in_scale[c] * v, output z += out_scale[c] * y_c */
+int ocmp_unet_set_channels(ocmp_unet *u, const int64_t *dof_off, const double *in_scale, const double *out_scale);
+/* channel offsets and scales fitted to samples (device f64 rows, stride ld, e.g. residuals of
+ * plain solves): in_scale = 1/rms(v_c), out_scale = rms((D^{-1} v)_c); P: A's Jacobi preconditioner */
+int ocmp_unet_fit_channels(ocmp_unet *u, ocmp_precond *P, const double *samples, int nsamples, int64_t ld,
+                           const int64_t *dof_off);
+int ocmp_unet_get_channels(ocmp_unet *u, int64_t *dof_off, double *in_scale, double *out_scale);
+/* grid-level forward / parameter gradient of sum(gy * y) (device float32 [4][(N+1)^3]; grad [NPARAM]) */
+int ocmp_unet_forward(ocmp_unet *u, const float *x, float *y);
+int ocmp_unet_grad(ocmp_unet *u, const float *x, const float *gy, float *grad);
+/* the activations of the last forward pass as float32 (device out, may be NULL to query *count):
+ * layer -1 the bf16-rounded input, 0..6 a0..a6 ([C][n_level^3]), 7 the output y */
+int ocmp_unet_activation(ocmp_unet *u, int layer, float *out, int64_t *count);
+/* one Adam step on 0.5/B sum_b ||r_b - A E U(R v_b)||^2, r_b = v_b - A D^{-1} v_b, v_b = samples row idx[b]
+ * (device f64, row stride ld); P the (updated) Jacobi preconditioner of A; loss: host */
+int ocmp_unet_train_step(ocmp_unet *u, ocmp_matrix *A, ocmp_precond *P, const double *samples, int nsamples,
+                         int64_t ld, const int32_t *idx, int nbatch, float lr, double *loss);
+/* attach (or, with u = NULL, detach) the learned correction used by OCMP_SOLVER_FGMRES */
+int ocmp_precond_attach_unet(ocmp_precond *P, ocmp_unet *u);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,8 @@ LFI_SOURCE, LFI_DG_DIRICHLET = 0, 1
 FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD = 0, 1, 2, 3
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI = 0, 1, 2
 FP64, MIXED = 0, 1
-SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON = 0, 1, 2
+SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON, SOLVER_FGMRES = 0, 1, 2, 3
+UNET_NPARAM = 70980
 MLP_HIDDEN, MLP_NPARAM, MLP_MAXSEG = 128, 33793, 8
 
 
@@ -54,7 +55,7 @@ class SolveOpts(C.Structure):
 
 
 class SolveStats(C.Structure):
-    _fields_ = [('iters', C.c_int32), ('outer', C.c_int32), ('converged', C.c_int32), ('reserved', C.c_int32),
+    _fields_ = [('iters', C.c_int32), ('outer', C.c_int32), ('converged', C.c_int32), ('fallback', C.c_int32),
                 ('rel_residual', C.c_double), ('initial_rel_residual', C.c_double)]
 
 
@@ -91,6 +92,7 @@ SIGNATURES = {
     'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
+    'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],
     'ocmp_precond_update': [P],
@@ -108,6 +110,18 @@ SIGNATURES = {
     'ocmp_mlp_grad': [P, C.c_int64, P, P, P, C.POINTER(C.c_double)],
     'ocmp_mlp_train_step': [P, C.c_int64, P, P, C.c_float, C.POINTER(C.c_double)],
     'ocmp_mlp_guess': [P, C.c_int64, P, P, P, P, P],
+    'ocmp_unet_create': [P, C.c_int, P, C.POINTER(P)],
+    'ocmp_unet_destroy': [P],
+    'ocmp_unet_get_params': [P, P],
+    'ocmp_unet_set_params': [P, P],
+    'ocmp_unet_set_channels': [P, I64P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    'ocmp_unet_forward': [P, P, P],
+    'ocmp_unet_grad': [P, P, P, P],
+    'ocmp_unet_train_step': [P, P, P, P, C.c_int, C.c_int64, P, C.c_int, C.c_float, C.POINTER(C.c_double)],
+    'ocmp_unet_fit_channels': [P, P, P, C.c_int, C.c_int64, I64P],
+    'ocmp_unet_get_channels': [P, I64P, C.POINTER(C.c_double), C.POINTER(C.c_double)],
+    'ocmp_unet_activation': [P, C.c_int, P, I64P],
+    'ocmp_precond_attach_unet': [P, P],
 }
 RESTYPES = {'ocmp_last_error': C.c_char_p, 'ocmp_ctx_stream': C.c_void_p}
 

@@ -488,13 +488,15 @@ class SolveResult:
     converged: bool
     rel_residual: float
     initial_rel_residual: float
+    fallback: bool = False        # FGMRES: the learned correction was dropped for the plain preconditioner
 
 
 def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preconditioner], solver: str = 'CG',
           tol: float = 1e-10, maxit: int = 100000, precision: str = 'fp64', restart: int = 50,
           inner_tol: float = 0.0, check_every: int = 0, max_outer: int = 0) -> SolveResult:
     o = L.SolveOpts()
-    o.solver = {'CG': L.SOLVER_CG, 'GMRes': L.SOLVER_GMRES, 'Richardson': L.SOLVER_RICHARDSON}[solver]
+    o.solver = {'CG': L.SOLVER_CG, 'GMRes': L.SOLVER_GMRES, 'Richardson': L.SOLVER_RICHARDSON,
+                'FGMRes': L.SOLVER_FGMRES}[solver]
     o.precision = L.MIXED if precision == 'mixed' else L.FP64
     o.restart = restart
     o.maxit = maxit
@@ -505,4 +507,5 @@ def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preco
     s = L.SolveStats()
     L.check(a.ctx.lib.ocmp_solve(a.h, pre.h if pre is not None else None, C.c_void_p(_ptr(b)),
                                  C.c_void_p(_ptr(x)), C.byref(o), C.byref(s)), 'solve')
-    return SolveResult(s.iters, s.outer, bool(s.converged), s.rel_residual, s.initial_rel_residual)
+    return SolveResult(s.iters, s.outer, bool(s.converged), s.rel_residual, s.initial_rel_residual,
+                       bool(s.fallback))

@@ -856,6 +856,16 @@ int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double
   });
 }
 
+int ocmp_residual(ocmp_matrix *A, const double *b, const double *x, double *r, double *norm) {
+  return guard([&] {
+    OCMP_REQUIRE(A && b && x && r && norm, "NULL argument");
+    OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
+    OCMP_REQUIRE(r != b && r != x, "r must not alias b or x");
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    *norm = residual_vector(&A->A, b, x, r);
+  });
+}
+
 int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **out) {
   return guard([&] {
     OCMP_REQUIRE(A && out, "NULL argument");
@@ -908,7 +918,8 @@ int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x, cons
     OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
     OCMP_REQUIRE(!P || P->P.A == &A->A, "preconditioner built for another matrix");
     OCMP_REQUIRE(opts->tol > 0 && opts->maxit > 0, "tol and maxit must be positive");
-    OCMP_REQUIRE(opts->solver != OCMP_SOLVER_GMRES || (opts->restart >= 1 && opts->restart <= 200),
+    OCMP_REQUIRE((opts->solver != OCMP_SOLVER_GMRES && opts->solver != OCMP_SOLVER_FGMRES) ||
+                     (opts->restart >= 1 && opts->restart <= 200),
                  "GMRES restart must be in [1, 200]");
     OCMP_HIP(hipSetDevice(A->A.ctx->device));
     if (P && !P->P.ready) precond_update(&P->P);
@@ -1020,6 +1031,129 @@ int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1,
     MLP_ENTER(mlp);
     OCMP_REQUIRE(out != u0 && out != u1 && out != u2 && out != u3, "out must not alias the history");
     ocmp::mlp_guess(mlp->M, n, u0, u1, u2, u3, out);
+  });
+}
+
+// ------------------------------------------------------------------ learned preconditioner correction
+
+#define UNET_ENTER(u)                                  \
+  OCMP_REQUIRE((u) && (u)->U, "NULL unet handle");    \
+  OCMP_HIP(hipSetDevice((u)->ctx->device))
+
+int ocmp_unet_create(ocmp_ctx *ctx, int N, const float *params, ocmp_unet **out) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && params && out, "NULL argument");
+    OCMP_HIP(hipSetDevice(ctx->c.device));
+    auto *h = new ocmp_unet();
+    h->ctx = &ctx->c;
+    try {
+      h->U = ocmp::unet_create(&ctx->c, N, params);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ocmp_unet_destroy(ocmp_unet *u) {
+  return guard([&] {
+    if (!u) return;
+    ocmp::unet_destroy(u->U);
+    delete u;
+  });
+}
+
+int ocmp_unet_get_params(ocmp_unet *u, float *params) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(params, "NULL argument");
+    ocmp::unet_get_params(u->U, params);
+  });
+}
+
+int ocmp_unet_set_params(ocmp_unet *u, const float *params) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(params, "NULL argument");
+    ocmp::unet_set_params(u->U, params);
+  });
+}
+
+int ocmp_unet_set_channels(ocmp_unet *u, const int64_t *dof_off, const double *in_scale, const double *out_scale) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(dof_off && in_scale && out_scale, "NULL argument");
+    ocmp::unet_set_channels(u->U, dof_off, in_scale, out_scale);
+  });
+}
+
+int ocmp_unet_forward(ocmp_unet *u, const float *x, float *y) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(x && y, "NULL argument");
+    ocmp::unet_forward_grid(u->U, x, y);
+  });
+}
+
+int ocmp_unet_grad(ocmp_unet *u, const float *x, const float *gy, float *grad) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(x && gy && grad, "NULL argument");
+    ocmp::unet_grad_grid(u->U, x, gy, grad);
+  });
+}
+
+int ocmp_unet_train_step(ocmp_unet *u, ocmp_matrix *A, ocmp_precond *P, const double *samples, int nsamples,
+                         int64_t ld, const int32_t *idx, int nbatch, float lr, double *loss) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(A && P && samples && idx && loss, "NULL argument");
+    OCMP_REQUIRE(P->P.A == &A->A, "preconditioner built for another matrix");
+    OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
+    OCMP_REQUIRE(ld >= A->A.n && nsamples >= 1, "bad sample layout");
+    OCMP_REQUIRE(lr > 0.f, "learning rate must be > 0");
+    for (int b = 0; b < nbatch; ++b) OCMP_REQUIRE(idx[b] >= 0 && idx[b] < nsamples, "sample index out of range");
+    OCMP_REQUIRE(ocmp::unet_max_index(u->U) <= A->A.n, "U-net channels reach beyond the system's dofs");
+    if (!P->P.ready) ocmp::precond_update(&P->P);
+    *loss = ocmp::unet_train_step(u->U, &A->A, &P->P, samples, ld, idx, nbatch, lr);
+  });
+}
+
+int ocmp_unet_fit_channels(ocmp_unet *u, ocmp_precond *P, const double *samples, int nsamples, int64_t ld,
+                           const int64_t *dof_off) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(P && samples && dof_off && nsamples >= 1, "bad arguments");
+    OCMP_REQUIRE(ld >= P->P.A->n, "sample stride shorter than the system");
+    if (!P->P.ready) ocmp::precond_update(&P->P);
+    ocmp::unet_fit_channels(u->U, &P->P, samples, nsamples, ld, dof_off);
+  });
+}
+
+int ocmp_unet_get_channels(ocmp_unet *u, int64_t *dof_off, double *in_scale, double *out_scale) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(dof_off && in_scale && out_scale, "NULL argument");
+    ocmp::unet_get_channels(u->U, dof_off, in_scale, out_scale);
+  });
+}
+
+int ocmp_unet_activation(ocmp_unet *u, int layer, float *out, int64_t *count) {
+  return guard([&] {
+    UNET_ENTER(u);
+    OCMP_REQUIRE(layer >= -1 && layer <= 7, "layer out of range [-1, 7]");
+    OCMP_REQUIRE(count, "NULL argument");
+    *count = ocmp::unet_activation(u->U, layer, out);
+  });
+}
+
+int ocmp_precond_attach_unet(ocmp_precond *P, ocmp_unet *u) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.kind == OCMP_PRE_JACOBI, "the learned correction extends a point-Jacobi preconditioner");
+    if (u) OCMP_REQUIRE(ocmp::unet_max_index(u->U) <= P->P.A->n, "U-net channels reach beyond the system's dofs");
+    P->P.unet = u ? u->U : nullptr;
   });
 }
 

@@ -218,6 +218,7 @@ struct Precond {
   // saddle-point systems: rows of these scalar blocks get the diagonal Schur complement
   // approximation -1/(C_ii + sum_j B_ij^2 / A_jj) instead of 1/a_ii (kind stays Jacobi)
   uint64_t schur_blocks = 0;
+  struct Unet *unet = nullptr;  // learned correction added to the Jacobi part by FGMRES
   DBuf<uint8_t> schur_flag;  // n: 1 on rows of the Schur blocks
   DBuf<double> dinv;     // Jacobi: n (0 on Dirichlet dofs)
   DBuf<float> dinv32;
@@ -243,6 +244,7 @@ void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
 void residual_norms(Matrix *A, const double *b, const double *x, double *out);
+double residual_vector(Matrix *A, const double *b, const double *x, double *r);
 void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 
@@ -263,6 +265,25 @@ void mlp_train_step(Mlp *M, int64_t n, const float *X, const float *T, float lr,
 void mlp_guess(Mlp *M, int64_t n, const double *u0, const double *u1, const double *u2, const double *u3,
                double *out);
 
+// learned U-net preconditioner correction (unet.hip)
+struct Unet;
+Unet *unet_create(Ctx *ctx, int N, const float *params);
+void unet_destroy(Unet *U);
+int64_t unet_grid_points(const Unet *U);  // (N+1)^3
+int64_t unet_max_index(const Unet *U);    // 1 + the largest dof index a channel touches
+void unet_fit_channels(Unet *U, Precond *P, const double *samples, int ns, int64_t ld, const int64_t *off);
+void unet_get_channels(const Unet *U, int64_t *off, double *in_scale, double *out_scale);
+int64_t unet_activation(Unet *U, int layer, float *out);  // element count; copies if out
+void unet_set_params(Unet *U, const float *params);
+void unet_get_params(Unet *U, float *params);
+void unet_set_channels(Unet *U, const int64_t *off, const double *in_scale, const double *out_scale);
+void unet_forward_grid(Unet *U, const float *x, float *y);
+void unet_grad_grid(Unet *U, const float *x, const float *gy, float *grad);
+template <typename T>
+void unet_correct(Unet *U, const uint8_t *freem, const T *v, const double *sv, T *z, int accumulate);
+double unet_train_step(Unet *U, Matrix *A, Precond *P, const double *samples, int64_t ld, const int *idx, int nb,
+                       float lr);
+
 // greedy colouring + slot map (host)
 void build_h1_schedule(Matrix *A);
 void build_colouring(Space &s);
@@ -279,3 +300,4 @@ struct ocmp_space { ocmp::Space s; };
 struct ocmp_matrix { ocmp::Matrix A; };
 struct ocmp_precond { ocmp::Precond P; };
 struct ocmp_mlp { ocmp::Mlp *M; ocmp::Ctx *ctx; };
+struct ocmp_unet { ocmp::Unet *U; ocmp::Ctx *ctx; };

@@ -789,7 +789,16 @@ __global__ void __launch_bounds__(256) k_gmres_solve(int k, int m, const double 
     for (int r = threadIdx.x; r < i; r += blockDim.x) gs[r] -= R[(int64_t)i * ld + r] * yi;
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < k; i += blockDim.x) coef[i] *= vs[i];
+  if (vs)
+    for (int i = threadIdx.x; i < k; i += blockDim.x) coef[i] *= vs[i];
+}
+
+// z = dinv .* (s * v): the point-Jacobi half of the flexible preconditioner (dinv is 0 off the free dofs)
+template <typename T>
+__global__ void k_prec_jacobi(int64_t n, const T *__restrict__ dinv, const T *__restrict__ v,
+                              const double *__restrict__ sv, T *__restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = (T)((double)dinv[i] * (double)v[i] * sv[0]);
 }
 
 // u = scale * sum_l coef[l] V[l]   (in FP64)
@@ -1309,6 +1318,175 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
   return k_used;
 }
 
+// Flexible GMRES cycle (right preconditioning that may vary between iterations):
+// z_j = M_j v_j with M = D^{-1} (+ the learned correction when `learned`), w = A z_j,
+// x += Z y. Same device-side Gram-Schmidt / Givens machinery as gmres_cycle.
+template <typename T>
+static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, double beta, double *x, int m,
+                        double stop_abs, int maxit, int check, Work &w) {
+  Ctx *ctx = A->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = A->n;
+  const uint8_t *mask = A->space->d_free.p;
+  constexpr int VW = Vec<T>::w;
+  const int64_t ldv = (n + VW - 1) / VW * VW;
+  DBuf<T> *Vbuf;
+  if constexpr (sizeof(T) == 8) {
+    w.vd.alloc(std::max<size_t>(w.vd.n, (size_t)(2 * m + 1) * ldv));
+    Vbuf = &w.vd;
+  } else {
+    w.vf.alloc(std::max<size_t>(w.vf.n, (size_t)(2 * m + 1) * ldv));
+    Vbuf = &w.vf;
+  }
+  T *V = Vbuf->p;
+  T *Z = V + (size_t)(m + 1) * ldv;
+  const size_t ns = (size_t)(m + 2) * 2 + (size_t)(m + 1) * m + 2 * m + (m + 1) + (m + 8) + m;
+  w.small.alloc(std::max(w.small.n, ns));
+  double *dot = w.small.p, *vs = dot + (m + 2), *R = vs + (m + 2), *csn = R + (size_t)(m + 1) * m,
+         *g = csn + 2 * m, *res = g + (m + 1), *coef = res + (m + 8);
+  const int grid = stream_grid((n + VW - 1) / VW);
+  ensure_part(w, (size_t)(m + 2 + KC) * grid);
+  k_scale_copy<double, T><<<nblk(n), NTHR, 0, st>>>(n, r0, 1.0, V);
+  k_gmres_init<<<1, 64, 0, st>>>(m, 1.0, g, vs, 1.0 / beta);
+  double *hres = pinned(w, m + 8);
+  const T *dinv;
+  const T *vals;
+  if constexpr (sizeof(T) == 8) {
+    dinv = P->dinv.p;
+    vals = A->d_val.p;
+  } else {
+    dinv = P->dinv32.p;
+    vals = A->d_val32.p;
+  }
+  const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_csr_f64_fgmres" : "spmv_csr_f32_fgmres");
+  const int c_pre = ctx->cls(sizeof(T) == 8 ? "fgmres_jacobi_f64" : "fgmres_jacobi_f32");
+  const int c_dot = ctx->cls(sizeof(T) == 8 ? "gmres_mdot_f64" : "gmres_mdot_f32");
+  const int c_upd = ctx->cls(sizeof(T) == 8 ? "gmres_update_f64" : "gmres_update_f32");
+  const int limit = std::min(m, maxit);
+  GState G{dot, vs, R, csn, g, res, m, beta};
+  struct Batch {
+    int j0, nb, ev;
+  };
+  std::deque<Batch> inflight;
+  int j = 0, bidx = 0, k_used = -1;
+  auto enqueue = [&]() {
+    const int j_end = std::min(limit, j + check);
+    for (int jj = j; jj < j_end; ++jj) {
+      T *vj = V + (int64_t)jj * ldv;
+      T *zj = Z + (int64_t)jj * ldv;
+      T *wv = V + (int64_t)(jj + 1) * ldv;
+      {
+        KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
+        k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
+      }
+      if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
+      {
+        KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
+        spmv_dispatch<T, T>(A, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid(A), st);
+      }
+      {
+        KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
+        dim3 gd(grid, (jj + 2 + KC - 1) / KC);
+        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p);
+      }
+      reduce_to(w.part.p, grid, jj + 1, dot, st);
+      {
+        KTimer tm(ctx, c_upd, (double)(jj + 3) * n * sizeof(T));
+        k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p);
+      }
+      k_norm_givens<<<1, 256, 0, st>>>(w.part.p, grid, jj, G);
+    }
+    const int nb = j_end - j;
+    OCMP_HIP(hipMemcpyAsync(hres + j, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    const int ev = bidx & 1;
+    OCMP_HIP(hipEventRecord(work_event(w, ev), st));
+    inflight.push_back({j, nb, ev});
+    ++bidx;
+    j = j_end;
+  };
+  while (k_used < 0) {
+    while (inflight.size() < 2 && j < limit) enqueue();
+    if (inflight.empty()) {
+      k_used = j;
+      break;
+    }
+    Batch b = inflight.front();
+    inflight.pop_front();
+    OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
+    for (int s = 0; s < b.nb; ++s)
+      if (!(hres[b.j0 + s] > stop_abs)) {
+        k_used = b.j0 + s + 1;
+        break;
+      }
+  }
+  // x += Z y  (the z_j already carry the scale of v_j)
+  k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, R, g, nullptr, coef);
+  w.u64.alloc(n);
+  k_combine<T><<<nblk(n), NTHR, 0, st>>>(n, Z, ldv, k_used, coef, beta, w.u64.p);
+  k_diag_apply_add<<<nblk(n), NTHR, 0, st>>>(n, nullptr, w.u64.p, x);
+  OCMP_HIP(hipGetLastError());
+  return k_used;
+}
+
+// FGMRES with the learned correction and its fallback: the first cycle runs with the
+// learned preconditioner, the second with the plain one; the rest use whichever reduced
+// the FP64 residual faster per iteration. A learned cycle that does not reduce the
+// residual switches to the plain preconditioner for good.
+static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o,
+                         ocmp_solve_stats &st_) {
+  OCMP_REQUIRE(P && P->kind == OCMP_PRE_JACOBI, "FGMRES needs a point-Jacobi (or Schur-diagonal) preconditioner");
+  OCMP_REQUIRE(!A->dg, "FGMRES is for H1 systems");
+  Work &w = work(A);
+  const int64_t n = A->n;
+  ensure_part(w, 0);
+  w.r64.alloc(n);
+  const bool mixed = o.precision == OCMP_MIXED;
+  if (mixed) OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
+  const double bn = norm_masked(A, b, w);
+  double rn = residual64(A, P, b, x, w.r64.p, w);
+  const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
+  st_.initial_rel_residual = rn / bref;
+  st_.rel_residual = rn / bref;
+  const int check = o.check_every > 0 ? o.check_every : 5;
+  const int m = o.restart;
+  int total = 0, outer = 0;
+  const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
+  const bool have_learned = P->unet != nullptr;
+  double rate_learned = 0.0, rate_plain = 0.0;  // log residual reduction per iteration (< 0 is progress)
+  bool known_l = false, known_p = false, fell_back = false;
+  while (rn > o.tol * bref && total < o.maxit && std::isfinite(rn)) {
+    ++outer;
+    bool use = have_learned && !fell_back;
+    if (use && known_l && !known_p) use = false;          // the probe cycle with the plain preconditioner
+    if (use && known_l && known_p && rate_plain < rate_learned) use = false, fell_back = true;
+    const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
+    int used;
+    if (mixed)
+      used = fgmres_cycle<float>(A, P, use, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+    else
+      used = fgmres_cycle<double>(A, P, use, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+    total += used;
+    const double rn_old = rn;
+    rn = residual64(A, P, b, x, w.r64.p, w);
+    st_.rel_residual = rn / bref;
+    if (used > 0 && rn > 0 && rn_old > 0) {
+      const double rate = std::log(rn / rn_old) / used;
+      if (use) {
+        rate_learned = rate, known_l = true;
+        if (!(rn < rn_old)) fell_back = true;
+      } else {
+        rate_plain = rate, known_p = true;
+      }
+    }
+    if (used == 0 || (mixed && o.max_outer > 0 && outer >= o.max_outer)) break;
+    if (!(rn < rn_old) && used < m && !use) break;
+  }
+  st_.iters = total;
+  st_.outer = outer;
+  st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
+  st_.fallback = have_learned && fell_back ? 1 : 0;
+}
+
 static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o,
                         ocmp_solve_stats &st_) {
   Work &w = work(A);
@@ -1397,6 +1575,12 @@ static void solve_richardson(Matrix *A, Precond *P, const double *b, double *x, 
   st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
 }
 
+// r = mask (b - A x); returns ||r||_2 (FP64)
+double residual_vector(Matrix *A, const double *b, const double *x, double *r) {
+  Work &w = work(A);
+  return residual64(A, nullptr, b, x, r, w);
+}
+
 // out[0] = ||b - A x||_2, out[1] = ||b||_2, both over the free dofs in FP64
 void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
   Work &w = work(A);
@@ -1410,6 +1594,7 @@ void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_o
     case OCMP_SOLVER_CG: solve_cg(A, P, b, x, o, st); break;
     case OCMP_SOLVER_GMRES: solve_gmres(A, P, b, x, o, st); break;
     case OCMP_SOLVER_RICHARDSON: solve_richardson(A, P, b, x, o, st); break;
+    case OCMP_SOLVER_FGMRES: solve_fgmres(A, P, b, x, o, st); break;
     default: throw Error(OCMP_ERR_UNSUPPORTED, "unknown solver");
   }
 }

@@ -226,9 +226,13 @@ class GuessPolicy:
     snapshot recording for training, and the trained network that turns the
     history into the next step's initial guess."""
 
-    def __init__(self, ctx: Context, n: int, net: Optional[MLPGuess] = None, max_snapshots: int = 0):
+    def __init__(self, ctx: Context, n: int, net: Optional[MLPGuess] = None, max_snapshots: int = 0,
+                 n_learned: Optional[int] = None):
+        """n_learned: the network predicts dofs [0, n_learned) (e.g. the velocity blocks of a
+        saddle-point model); the rest keep the plain guess, the newest state."""
         self.ctx = ctx
         self.n = int(n)
+        self.n_learned = self.n if n_learned is None else int(n_learned)
         self.net = net
         self.ring = [ctx.zeros(self.n) for _ in range(4)]
         self.count = 0
@@ -252,7 +256,164 @@ class GuessPolicy:
         return self.net is not None and self.count >= 4
 
     def propose(self) -> torch.Tensor:
-        return self.net.guess(self.history(), out=self.out)
+        k = self.n_learned
+        h = self.history()
+        self.net.guess([v[:k] for v in h], out=self.out[:k])
+        if k < self.n:
+            self.out[k:].copy_(h[0][k:])
+        return self.out
 
     def snapshots(self) -> torch.Tensor:
         return self.snap[:self.nsnap]
+
+
+class UNetPrecond:
+    """Learned U-net correction of a point-Jacobi / Schur-diagonal preconditioner
+    (ocmp_unet_* in include/ocmp.h), used by the 'FGMRes' solver once attached.
+
+    The network lives on the vertex grid of an N^3 x 6-tet structured mesh; its
+    four channels are the vertex dofs of four scalar blocks."""
+
+    NPARAM = L.UNET_NPARAM
+
+    def __init__(self, N: int, ctx: Optional[Context] = None, params: Optional[np.ndarray] = None, seed: int = 0):
+        self.ctx = ctx or Context.default()
+        self.lib = self.ctx.lib
+        self.N = int(N)
+        p = self.init_params(seed) if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.NPARAM:
+            raise ValueError(f'expected {self.NPARAM} parameters, got {p.size}')
+        h = C.c_void_p()
+        L.check(self.lib.ocmp_unet_create(self.ctx.h, self.N, p.ctypes.data_as(C.c_void_p), C.byref(h)),
+                'unet_create')
+        self.h = h
+        self.losses: List[float] = []
+
+    LAYERS = [(4, 8, 'c'), (8, 16, 'c'), (16, 16, 'c'), (16, 32, 'c'), (32, 32, 'c'), (32, 16, 't'), (16, 8, 't'),
+              (8, 4, 'c')]
+
+    @classmethod
+    def init_params(cls, seed: int = 0) -> np.ndarray:
+        """He-normal 3x3x3 kernels (fan-in / 8 for the stride-2 transposed ones), zero
+        biases, the output layer scaled down so the untrained correction is small."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for k, (ci, co, kind) in enumerate(cls.LAYERS):
+            fan = ci * 27 if kind == 'c' else ci * 27 / 8.0
+            w = rng.standard_normal(ci * co * 27) * math.sqrt(2.0 / fan)
+            if k == len(cls.LAYERS) - 1:
+                w *= 0.1
+            parts += [w, np.zeros(co)]
+        return np.concatenate(parts).astype(np.float32)
+
+    def __del__(self):
+        h = getattr(self, 'h', None)
+        if h is not None and h.value:
+            try:
+                self.lib.ocmp_unet_destroy(h)
+            except Exception:
+                pass
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.empty(self.NPARAM, dtype=np.float32)
+        L.check(self.lib.ocmp_unet_get_params(self.h, out.ctypes.data_as(C.c_void_p)), 'unet_get_params')
+        return out
+
+    @params.setter
+    def params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        L.check(self.lib.ocmp_unet_set_params(self.h, p.ctypes.data_as(C.c_void_p)), 'unet_set_params')
+
+    def set_channels(self, dof_off: Sequence[int], in_scale: Sequence[float], out_scale: Sequence[float]) -> None:
+        L.check(self.lib.ocmp_unet_set_channels(self.h, (C.c_int64 * 4)(*[int(v) for v in dof_off]),
+                                                (C.c_double * 4)(*in_scale), (C.c_double * 4)(*out_scale)),
+                'unet_set_channels')
+
+    def channels(self):
+        off = (C.c_int64 * 4)()
+        a = (C.c_double * 4)()
+        b = (C.c_double * 4)()
+        L.check(self.lib.ocmp_unet_get_channels(self.h, off, a, b), 'unet_get_channels')
+        return list(off), list(a), list(b)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        """x: (4, (N+1)^3) float32 device grid -> y of the same shape."""
+        n3 = (self.N + 1) ** 3
+        if x.shape != (4, n3) or x.dtype != torch.float32 or not x.is_cuda:
+            raise ValueError(f'x must be a (4, {n3}) float32 device tensor')
+        x = x.contiguous()
+        y = torch.empty_like(x)
+        L.check(self.lib.ocmp_unet_forward(self.h, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(y))), 'unet_forward')
+        return y
+
+    def grad(self, x: torch.Tensor, gy: torch.Tensor) -> torch.Tensor:
+        x, gy = x.contiguous(), gy.contiguous()
+        g = torch.empty(self.NPARAM, dtype=torch.float32, device=x.device)
+        L.check(self.lib.ocmp_unet_grad(self.h, C.c_void_p(_ptr(x)), C.c_void_p(_ptr(gy)), C.c_void_p(_ptr(g))),
+                'unet_grad')
+        return g
+
+    def activation(self, layer: int) -> torch.Tensor:
+        """Activation of the last forward pass (-1: input, 0..6: a0..a6, 7: output), float32."""
+        cnt = C.c_int64()
+        L.check(self.lib.ocmp_unet_activation(self.h, int(layer), None, C.byref(cnt)), 'unet_activation')
+        out = torch.empty(cnt.value, dtype=torch.float32, device=self.ctx.torch_device)
+        L.check(self.lib.ocmp_unet_activation(self.h, int(layer), C.c_void_p(_ptr(out)), C.byref(cnt)),
+                'unet_activation')
+        return out
+
+    def fit_channels(self, pre, samples: torch.Tensor, dof_off: Sequence[int]) -> None:
+        L.check(self.lib.ocmp_unet_fit_channels(self.h, pre.h, C.c_void_p(_ptr(samples)), samples.shape[0],
+                                                samples.stride(0), (C.c_int64 * 4)(*[int(v) for v in dof_off])),
+                'unet_fit_channels')
+
+    def train(self, a, pre, samples: torch.Tensor, dof_off: Sequence[int], steps: int = 200, batch: int = 4,
+              lr: float = 1e-3, seed: int = 0) -> List[float]:
+        """Fit the channel scales, then `steps` Adam steps of the self-supervised loss on
+        random batches of the sample rows; returns the loss per step."""
+        if samples.dim() != 2 or samples.dtype != torch.float64 or samples.stride(1) != 1:
+            raise ValueError('samples must be a (ns, n) float64 device tensor')
+        self.fit_channels(pre, samples, dof_off)
+        rng = np.random.default_rng(seed)
+        ns = samples.shape[0]
+        out = []
+        for _ in range(steps):
+            idx = (C.c_int32 * batch)(*[int(v) for v in rng.integers(0, ns, batch)])
+            loss = C.c_double()
+            L.check(self.lib.ocmp_unet_train_step(self.h, a.h, pre.h, C.c_void_p(_ptr(samples)), ns,
+                                                  samples.stride(0), idx, batch, C.c_float(lr), C.byref(loss)),
+                    'unet_train_step')
+            out.append(loss.value)
+        self.losses.extend(out)
+        return out
+
+    def attach(self, pre) -> None:
+        L.check(self.lib.ocmp_precond_attach_unet(pre.h, self.h), 'precond_attach_unet')
+        pre.unet = self
+
+    @staticmethod
+    def detach(pre) -> None:
+        L.check(pre.a.ctx.lib.ocmp_precond_attach_unet(pre.h, None), 'precond_attach_unet')
+        pre.unet = None
+
+
+def residual_samples(a, pre, b: torch.Tensor, n_samples: int, every: int, x0: Optional[torch.Tensor] = None,
+                     solver: str = 'GMRes', restart: int = 50, precision: str = 'fp64') -> torch.Tensor:
+    """Normalised residuals r_k / ||r_k|| of a plain solve from x0 (default 0; pass the
+    Dirichlet lift for a problem driven by boundary data), taken every `every`
+    iterations: the vectors a preconditioner meets during that solve (training data)."""
+    from .core import solve as _solve
+    n = b.numel()
+    out = torch.zeros((n_samples, n), dtype=torch.float64, device=b.device)
+    x = torch.zeros_like(b) if x0 is None else x0.clone()
+    r = torch.empty_like(b)
+    rn = C.c_double()
+    for k in range(n_samples):
+        if k > 0:
+            _solve(a, b, x, pre, solver, tol=1e-300, maxit=every, restart=restart, precision=precision)
+        L.check(a.ctx.lib.ocmp_residual(a.h, C.c_void_p(_ptr(b)), C.c_void_p(_ptr(x)), C.c_void_p(_ptr(r)),
+                                        C.byref(rn)), 'residual')
+        L.check(a.ctx.lib.ocmp_axpby(a.ctx.h, n, 1.0 / rn.value if rn.value > 0 else 1.0, C.c_void_p(_ptr(r)), 0.0,
+                                     C.c_void_p(_ptr(out[k]))), 'axpby')
+    return out

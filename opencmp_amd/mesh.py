@@ -292,3 +292,26 @@ def channel_with_holes(n: int = 228, n_holes: int = 8, r_range=(0.05, 0.12), see
     m.holes = holes
     m.hole_fit_error = float(fit_err)
     return m
+
+
+def sphere_obstacles(n_spheres: int = 20, r_range=(0.03, 0.1), seed: int = 0, lo: float = 0.1, hi: float = 0.9):
+    """Spheres with r ~ U[r_range] and centres ~ U[lo, hi]^3 drawn from default_rng(seed)
+    (radius first, then the centre, sphere by sphere)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_spheres):
+        r = rng.uniform(*r_range)
+        c = rng.uniform(lo, hi, 3)
+        out.append((c, r))
+    return out
+
+
+def obstacle_phi(coords: np.ndarray, spheres, h: float) -> np.ndarray:
+    """Diffuse-interface indicator phi = 0.5 (1 + tanh(d / (2h))) with d the signed distance
+    to the union of the spheres, positive in the fluid: phi -> 1 in the fluid and 0 inside
+    the obstacles (the convention of opencmp/models/stokes_dim.py:42-48, where the
+    penalty acts on 1 - phi)."""
+    d = np.full(coords.shape[0], np.inf)
+    for c, r in spheres:
+        d = np.minimum(d, np.linalg.norm(coords - c[None, :], axis=1) - r)
+    return 0.5 * (1.0 + np.tanh(d / (2.0 * h)))
