@@ -120,17 +120,36 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
     return (t1 - t0) / steps, res, its, x.vec.clone(), rep
 
 
-def cpu_baseline(config, gpu_iters_fp64, n_full):
-    """The oracle (numpy/scipy FP64, 1 thread) timed on a bounded sample of the same workload.
+def cpu_baseline(config, a, f, gs, gpu_iters_fp64, n_full):
+    """Host-CPU baseline, timed on a bounded sample of the same workload.
 
-    Assembly (+ preconditioner setup) is timed in full on a smaller instance of the
-    same discretisation and a fixed number of Krylov iterations is timed on it;
-    both scale linearly in elements/DOFs, so s/step at the full size is
-    t_asm * scale + t_iter * scale * (iterations of the FP64 GPU path)."""
-    from oracle import fem
+    Solve: the oracle's C++/OpenMP restatement (oracle/cpu_solver.cpp, FP64, all host
+    cores) runs a fixed number of Krylov iterations of the same algorithm on the same
+    full-size CSR matrix (copied from the GPU); its time per iteration times the FP64 GPU
+    path's iteration count is the solve time per step. Assembly: the numpy oracle
+    (1 thread) on a smaller instance of the same discretisation, scaled linearly in
+    elements. Both parts are stated in `sample`."""
+    import ctypes as C
+    from oracle import fem, cbuild
     from opencmp_amd.mesh import unit_square, unit_cube
+    lib = cbuild.load()
+    cores = len(os.sched_getaffinity(0))
+    lib.ocmp_cpu_threads(cores)
+    rp, ci = a.pattern()
+    va = np.ascontiguousarray(a.values(), dtype=np.float64)
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    b = f.vec.detach().cpu().numpy().astype(np.float64)
+    n = b.size
+    x = np.zeros(n)
+    rel = C.c_double()
     if config == 2:
-        ns, nit = 128, 100
+        nit = 100
+        t0 = time.perf_counter()
+        it = lib.ocmp_cpu_gmres_bj(n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, 6, b.ctypes.data,
+                                   x.ctypes.data, 50, 1e-30, nit, C.byref(rel))
+        t_it = (time.perf_counter() - t0) / max(it, 1)
+        ns = 128
         eps, w = 1e-3, np.array([np.cos(0.3), np.sin(0.3)])
         m = unit_square(ns)
         t = fem.Topology(m)
@@ -140,45 +159,46 @@ def cpu_baseline(config, gpu_iters_fp64, n_full):
         fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=eps),
                                  dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
         fem.assemble_dg_facets(A_, 0, eps, 40.0, w, m.marker_mask(m.bnd_names), 0)
-        A = A_.matrix()
-        b = fem.assemble_source(cs, 0, 1.0)
-        Bi = fem.block_jacobi_inverse(A, 6)
+        A_.matrix()
+        fem.assemble_source(cs, 0, 1.0)
         t_asm = time.perf_counter() - t0
-        M = lambda v: np.einsum('eij,ej->ei', Bi, v.reshape(-1, 6)).ravel()
-        t0 = time.perf_counter()
-        fem.gmres_right(A, b, np.zeros(cs.ndof), M, 1e-30, m=50, maxit=nit)
-        t_it = (time.perf_counter() - t0) / nit
         scale = (n_full / ns) ** 2
-        what = f'{ns}x{ns} instance: assembly + block-Jacobi setup ({t_asm:.2f}s), {nit} GMRES(50) iterations'
+        what = f'GMRES(50) + block Jacobi, {it} iterations'
     else:
+        nit = 100 if config == 3 else 200
+        free = np.ascontiguousarray(gs.FreeDofs().astype(np.uint8))
+        t0 = time.perf_counter()
+        it = lib.ocmp_cpu_pcg_jacobi(n, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, free.ctypes.data,
+                                     b.ctypes.data, x.ctypes.data, 1e-30, nit, C.byref(rel))
+        t_it = (time.perf_counter() - t0) / max(it, 1)
         if config == 3:
-            ns, nit = 8, 200
+            ns = 8
             m = unit_cube(ns)
-            p, coef = 3, np.exp(np.random.default_rng(0).standard_normal(m.nel))
-            f = 1.0
+            p, coef, fsrc = 3, np.exp(np.random.default_rng(0).standard_normal(m.nel)), 1.0
         else:
-            ns, nit = n_full, 200
+            ns = n_full
             m = unit_square(ns)
             p, coef = 2, 1.0
-            f = lambda x, y: 2 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y)
+            fsrc = lambda x_, y_: 2 * np.pi ** 2 * np.sin(np.pi * x_) * np.sin(np.pi * y_)
         t = fem.Topology(m)
         cs = fem.Compound([fem.Space(t, 'H1', p, dirichlet_mask=m.marker_mask(m.bnd_names))])
         A_ = fem.Assembler(cs)
         t0 = time.perf_counter()
         fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=coef)])
-        A = A_.matrix()
-        b = fem.assemble_source(cs, 0, f)
+        A_.matrix()
+        fem.assemble_source(cs, 0, fsrc)
         t_asm = time.perf_counter() - t0
-        t0 = time.perf_counter()
-        fem.pcg_jacobi(A, b, np.zeros(cs.ndof), cs.freedofs, 1e-30, maxit=nit)
-        t_it = (time.perf_counter() - t0) / nit
-        scale = (n_full / ns) ** (m.dim)
-        what = f'{ns}^{m.dim} instance: assembly ({t_asm:.2f}s), {nit} Jacobi-PCG iterations'
-    s_step = t_asm * scale + t_it * scale * gpu_iters_fp64
-    return {'value': s_step, 'unit': 's/step', 'cores': 1, 'kind': 'port',
-            'sample': f'oracle/fem.py (numpy+scipy, FP64, 1 thread) on the {what} ({t_it * 1e3:.3f} ms/iter), '
-                      f'scaled by DOFs ({scale:.0f}x) to the full size and by the FP64 GPU path\'s '
-                      f'{gpu_iters_fp64} iterations'}
+        scale = (n_full / ns) ** m.dim
+        what = f'Jacobi-PCG, {it} iterations'
+    s_solve = t_it * gpu_iters_fp64
+    s_asm = t_asm * scale
+    return {'value': s_asm + s_solve, 'unit': 's/step', 'cores': cores, 'kind': 'port',
+            'solve_s': round(s_solve, 3), 'assembly_s': round(s_asm, 3),
+            'sample': f'solve: oracle/cpu_solver.cpp (C++ -O3 OpenMP, FP64, {cores} threads) on the full-size '
+                      f'CSR ({n} dofs, {va.size} nnz): {what} at {t_it * 1e3:.2f} ms/iteration, times the FP64 '
+                      f'GPU path\'s {gpu_iters_fp64} iterations; assembly: oracle/fem.py (numpy, 1 thread) on '
+                      f'the {ns}^{m.dim} instance ({t_asm:.2f} s) scaled by elements ({scale:.0f}x). '
+                      f'Host-CPU restatement, not NGSolve.'}
 
 
 def bench_learned(args, ctx):
@@ -279,12 +299,132 @@ def bench_learned(args, ctx):
                              residual_plain_vs_learned=[(rp, rl) for _, rp, rl in glog]))
 
 
+def bench_unet(args, ctx):
+    """Config 5: diffuse-interface Stokes on the unit cube, n^3 cubes x 6 tets, Taylor-Hood
+    P2/P1, 20 seeded spheres (phi = 0.5 (1 + tanh(d / 2h)), d > 0 in the fluid), penalty
+    1e4 (1 - phi), unit inflow in x at x = 0, no-slip on y/z walls, do-nothing at x = 1.
+    Plain path: FP64 FGMRES(m) with the Schur-diagonal preconditioner. Learned path: FP32
+    inner FGMRES cycles in FP64 iterative refinement with the U-net correction (trained on
+    residuals of a plain solve by the library's kernels) and its fallback."""
+    import torch
+    from opencmp_amd import core as C
+    from opencmp_amd.learned import UNetPrecond, residual_samples, residual_norms
+    from opencmp_amd.mesh import unit_cube, sphere_obstacles, obstacle_phi
+    from opencmp_amd.models import StokesDIM, const
+    n = args.n or 64
+    t0 = time.perf_counter()
+    hm = unit_cube(n)
+    phi = obstacle_phi(hm.coords, sphere_obstacles(), 1.0 / n)
+    inflow = [const(1.0), const(0.0), const(0.0)]
+    zero = [const(0.0)] * 3
+    walls = {b: zero for b in ('front', 'back', 'bottom', 'top')}
+
+    def model(precision):
+        return StokesDIM(hm, torch.tensor(phi), phi_order=1, penalty=1e4, kv=1.0, interp_ord=2,
+                         dirichlet={'u': {'left': inflow, **walls}}, linear_solver='FGMRes', preconditioner='schur',
+                         linear_tolerance=args.tol, restart=args.restart, precision=precision,
+                         inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, ctx=ctx)
+
+    m64 = model('fp64')
+    a = C.BilinearForm(m64.fes, m64.construct_bilinear_time_coefficient(1.0) + m64.construct_bilinear_time_ODE(1.0))
+    f = C.LinearForm(m64.fes, m64.construct_linear(1.0))
+    pre64 = m64.construct_preconditioners(a)
+    pre32 = C.Preconditioner(a, 'schur', precision='mixed', schur_blocks=m64.schur_blocks())
+    ndof = m64.fes.ndof
+    log(f'config 5 setup {time.perf_counter() - t0:.1f}s: {hm.nel} tets, ndof {ndof}, nnz {a.nnz}')
+    x = C.GridFunction(m64.fes)
+
+    def step(pre, precision):
+        """One assemble+solve step. The solve is issued in slices of 20 restart cycles (each
+        cycle restarts from the FP64 residual, so slicing at cycle boundaries changes nothing
+        for the plain preconditioner) and prints its progress after each slice."""
+        a.Assemble()
+        f.Assemble()
+        pre.Update()
+        x.vec.zero_()
+        m64.apply_dirichlet_bcs_to(x)
+        inner = (args.inner_tol or 1e-4) if precision == 'mixed' else 0.0
+        learned = getattr(pre, 'unet', None) is not None
+        chunk = args.maxit if learned else 20 * args.restart
+        # the solve measures ||b - Ax|| against ||b|| on the free dofs, or against its own initial
+        # residual when b vanishes there (flow driven by the inflow data): keep that reference fixed
+        r0, bn = residual_norms(a, f.vec, x.vec)
+        bref = bn if bn > 0 else r0
+        total, outer, r, rc = 0, 0, None, r0
+        while total < args.maxit:
+            tol = args.tol if bn > 0 else args.tol * r0 / rc
+            r = C.solve(a, f.vec, x.vec, pre, 'FGMRes', tol=tol, restart=args.restart,
+                        maxit=min(chunk, args.maxit - total), precision=precision, inner_tol=inner)
+            total += r.iters
+            outer += r.outer
+            rc = residual_norms(a, f.vec, x.vec)[0]
+            if rc <= args.tol * bref or r.iters == 0:
+                break
+            log(f'  ... {total} iterations, rel residual {rc / bref:.2e}')
+        r.iters, r.outer, r.rel_residual = total, outer, rc / bref
+        r.converged = rc <= args.tol * bref
+        return r
+
+    def timed(pre, precision, label, timing=False):
+        for _ in range(args.warmup):
+            step(pre, precision)
+        torch.cuda.synchronize()
+        if timing:
+            ctx.timing(True)
+            ctx.timing_report()
+        t1 = time.perf_counter()
+        rs = [step(pre, precision) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t1) / args.steps
+        rep = None
+        if timing:
+            rep = ctx.timing_report()
+            ctx.timing(False)
+        log(f'{label}: {dt:.3f} s/step, iters {[r.iters for r in rs]}, res {rs[-1].rel_residual:.2e}, '
+            f'fallback {[r.fallback for r in rs]}')
+        return dt, rs[-1], x.vec.clone(), rep
+
+    s64, r64, u64, _ = timed(pre64, 'fp64', 'fp64 plain')
+    # training data: normalised residuals along a plain solve from the Dirichlet lift
+    a.Assemble()
+    f.Assemble()
+    pre64.Update()
+    x0 = C.GridFunction(m64.fes)
+    m64.apply_dirichlet_bcs_to(x0)
+    t1 = time.perf_counter()
+    S = residual_samples(a, pre64, f.vec, args.unet_samples, 20, x0=x0.vec)
+    torch.cuda.synchronize()
+    t_samples = time.perf_counter() - t1
+    U = UNetPrecond(n, ctx, seed=0)
+    blocks = m64.fes.comp_blocks[0] + m64.fes.comp_blocks[1]
+    off = [m64.fes.blocks[k][0] for k in blocks]
+    t1 = time.perf_counter()
+    losses = U.train(a, pre64, S, off, steps=args.train_steps, batch=4, lr=2e-3, seed=1)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    log(f'U-net: {args.unet_samples} samples in {t_samples:.1f}s, {args.train_steps} Adam steps x 4 in '
+        f'{t_train:.1f}s, loss {losses[0]:.3e} -> {np.mean(losses[-10:]):.3e}')
+    del S
+    U.attach(pre32)
+    sml, rml, uml, rep = timed(pre32, 'mixed', 'mixed+learned', timing=True)
+    U.detach(pre32)
+    smx, rmx, umx, _ = timed(pre32, 'mixed', 'mixed (no correction)')
+    err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
+    return dict(desc=(f'diffuse-interface Stokes, unit cube {n}^3 x 6 tets ({hm.nel} tets), Taylor-Hood P2/P1, '
+                      f'20 spheres, penalty 1e4, FGMRES({args.restart}) + Schur-diagonal preconditioner'),
+                ndof=ndof, nnz=a.nnz, s64=s64, sml=sml, smx=smx, r64=r64, rml=rml, rmx=rmx, err=err, rep=rep,
+                learned=dict(samples=args.unet_samples, sample_s=round(t_samples, 2), train_steps=args.train_steps,
+                             train_s=round(t_train, 2), loss_first=losses[0], loss_last10=float(np.mean(losses[-10:])),
+                             fallback=rml.fallback))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4))
+    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4, 5))
+    ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--train-steps', type=int, default=400, help='config 4: Adam steps')
     ap.add_argument('--n', type=int, default=None, help='mesh cells per direction (defaults: 32 / 512 / 64)')
@@ -300,7 +440,7 @@ def main():
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228}[args.config]
+    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 64}[args.config]
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
@@ -314,6 +454,35 @@ def main():
         torch.cuda.synchronize()
         log(f'calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written per dispatch')
         del src, dst
+    if args.config == 5:
+        r = bench_unet(args, ctx)
+        rep = r['rep']
+        dom = max((k for k in rep if rep[k][2] > 0), key=lambda k: rep[k][1])
+        cnt, ms, byt = rep[dom]
+        achieved = byt / (ms * 1e-3) / 1e9
+        out = {
+            'metric': METRIC, 'value': round(r['sml'], 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
+            'warmup': args.warmup, 'ms_per_step': round(r['sml'] * 1e3, 3), 'higher_is_better': False,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32+bf16', 'data': 'synthetic',
+            'config': {'workload': f"{r['desc']} to rel residual {args.tol:g}", 'config_index': 5,
+                       'ndof': r['ndof'], 'nnz': r['nnz'], 'tol': args.tol},
+            'plain_fp64': {'s_per_step': round(r['s64'], 5), 'iters': r['r64'].iters,
+                           'rel_residual': r['r64'].rel_residual},
+            'mixed_learned': {'s_per_step': round(r['sml'], 5), 'iters': r['rml'].iters, 'outer': r['rml'].outer,
+                              'fallback': r['rml'].fallback, 'rel_residual': r['rml'].rel_residual,
+                              'rel_error_vs_fp64': r['err']},
+            'mixed_plain': {'s_per_step': round(r['smx'], 5), 'iters': r['rmx'].iters},
+            'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
+            'dofs_per_s': round(r['ndof'] / r['sml'], 1),
+            'learned': r['learned'],
+            'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
+                         'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
+                         'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)},
+            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
+                                      for k, (c, m_, b_) in rep.items()},
+        }
+        print(json.dumps(out), flush=True)
+        return
     if args.config == 4:
         r = bench_learned(args, ctx)
         rep = r['rep']
@@ -387,7 +556,7 @@ def main():
     }
     if not args.no_cpu:
         log('cpu baseline ...')
-        cb = cpu_baseline(args.config, it64[-1], n)
+        cb = cpu_baseline(args.config, a, f, gs, it64[-1], n)
         cb['speedup_gpu_mixed'] = round(cb['value'] / smx, 1)
         cb['speedup_gpu_fp64'] = round(cb['value'] / s64, 1)
         out['cpu_baseline'] = cb

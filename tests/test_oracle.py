@@ -306,3 +306,62 @@ def test_channel_with_holes_mesh():
             assert 0.05 <= r <= 0.12 and 0.25 + r <= cx <= 3.75 - r
     # about 4e5 triangles at the workload size is asserted by the bench; here the count scales as n^2
     assert abs(channel_with_holes(40).nel / channel_with_holes(20).nel - 4.0) < 0.1
+
+
+def test_cpu_restatement_solvers_match_oracle():
+    """The C++/OpenMP Krylov restatement used for the CPU baseline solves the oracle's
+    DG and H1 systems to the requested FP64 residual, like the numpy oracle."""
+    import ctypes as C
+    from oracle import cbuild
+    if not os.path.exists(cbuild.LIB):
+        cbuild.build()
+    lib = cbuild.load()
+    lib.ocmp_cpu_threads(2)
+    P = C.c_void_p
+    m = unit_square(8)
+    t = fem.Topology(m)
+    # DG P2 convection-diffusion (config 2 at 8x8)
+    cs = fem.Compound([fem.Space(t, 'L2', 2)], dgjumps=True)
+    A_ = fem.Assembler(cs)
+    w = np.array([np.cos(0.3), np.sin(0.3)])
+    fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=1e-3),
+                             dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
+    fem.assemble_dg_facets(A_, 0, 1e-3, 40.0, w, m.marker_mask(m.bnd_names), 0)
+    A = A_.matrix().tocsr()
+    b = fem.assemble_source(cs, 0, 1.0)
+    rp = A.indptr.astype(np.int32)
+    ci = A.indices.astype(np.int32)
+    va = A.data.astype(np.float64)
+    x = np.zeros(A.shape[0])
+    rel = C.c_double()
+    it = lib.ocmp_cpu_gmres_bj(A.shape[0], rp.ctypes.data, ci.ctypes.data, va.ctypes.data, 6, b.ctypes.data,
+                               x.ctypes.data, 50, 1e-10, 100000, C.byref(rel))
+    assert rel.value <= 1e-10 and 0 < it < 100000
+    assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b) * 1.01
+    xo = spl_solve(A, b)
+    assert np.linalg.norm(x - xo) <= 1e-7 * np.linalg.norm(xo)
+    # H1 P2 Poisson with Dirichlet dofs (Jacobi-PCG)
+    sp = fem.Space(t, 'H1', 2, dirichlet_mask=m.marker_mask(m.bnd_names))
+    cs = fem.Compound([sp])
+    A_ = fem.Assembler(cs)
+    fem.assemble_volume(A_, [dict(kind='grad_grad', trial=0, test=0, coef=1.0)])
+    A = A_.matrix().tocsr()
+    b = fem.assemble_source(cs, 0, 1.0)
+    free = cs.freedofs.astype(np.uint8)
+    rp = A.indptr.astype(np.int32)
+    ci = A.indices.astype(np.int32)
+    va = A.data.astype(np.float64)
+    x = np.zeros(A.shape[0])
+    it = lib.ocmp_cpu_pcg_jacobi(A.shape[0], rp.ctypes.data, ci.ctypes.data, va.ctypes.data, free.ctypes.data,
+                                 b.ctypes.data, x.ctypes.data, 1e-12, 100000, C.byref(rel))
+    f = free.astype(bool)
+    r = (b - A @ x)[f]
+    assert rel.value <= 1e-12 and np.linalg.norm(r) <= 1e-12 * np.linalg.norm(b[f]) * 1.01
+    xo = np.zeros_like(x)
+    xo[f] = spl_solve(A[f][:, f].tocsc(), b[f])
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def spl_solve(A, b):
+    import scipy.sparse.linalg as spla
+    return spla.spsolve(A.tocsc(), b)
