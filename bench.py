@@ -83,7 +83,7 @@ def build_problem(config, n, ctx):
 
 
 def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=False, solver='GMRes', inner_tol=None,
-             maxit=500000):
+             maxit=500000, s_step=0):
     import torch
     pre = C.Preconditioner(a, 'local', precision=precision)
     x = C.GridFunction(gs)
@@ -96,7 +96,7 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
         pre.Update()
         x.vec.zero_()
         return C.solve(a, f.vec, x.vec, pre, solver, tol=tol, restart=restart, precision=precision,
-                       maxit=maxit, inner_tol=inner)
+                       maxit=maxit, inner_tol=inner, s_step=s_step)
 
     for i in range(warmup):
         res = step()
@@ -116,7 +116,8 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
     if timing:
         rep = ctx.timing_report()
         ctx.timing(False)
-    log(f'{precision}: {steps} steps in {t1 - t0:.3f}s, iters {its}, rel_res {res.rel_residual:.3e}')
+    log(f'{precision}{" s-step" if s_step else ""}: {steps} steps in {t1 - t0:.3f}s, iters {its}, '
+        f'rel_res {res.rel_residual:.3e}')
     return (t1 - t0) / steps, res, its, x.vec.clone(), rep
 
 
@@ -433,6 +434,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--inner-tol', type=float, default=None, help='mixed path: relative reduction per inner solve')
     ap.add_argument('--maxit', type=int, default=500000, help='Krylov iteration cap (short profiling runs)')
+    ap.add_argument('--s-step', type=int, default=5, help='GMRES on the DG operator: 5 = s-step cycles in the '
+                    'mixed path (0: standard); the plain FP64 path is always standard GMRES')
     ap.add_argument('--calib-copy', type=int, default=0, metavar='MB',
                     help='first copy MB megabytes with a plain device copy (HBM counter calibration)')
     args = ap.parse_args()
@@ -513,11 +516,16 @@ def main():
     hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
     kw = dict(solver=solver, inner_tol=args.inner_tol, maxit=args.maxit)
+    ss = args.s_step if solver == 'GMRes' else 0
     s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps, args.warmup, **kw)
     smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps, args.warmup,
-                                      **kw)
+                                      s_step=ss, **kw)
+    # the same s-step algorithm in FP64, for reference (not the plain path)
+    s64s = None
+    if ss:
+        s64s, r64s, it64s, _, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, s_step=ss, **kw)
     # one more step of each path with HIP events around every launch: the per-kernel roofline
-    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, 1, 0, timing=True, **kw)
+    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, 1, 0, timing=True, s_step=ss, **kw)
     _, _, _, _, rep_fp = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, timing=True, **kw)
     err = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
 
@@ -546,8 +554,10 @@ def main():
         'config': {'workload': f'{desc} to rel residual {args.tol:g}', 'config_index': args.config,
                    'ndof': gs.ndof, 'nnz': a.nnz, 'tol': args.tol},
         'plain_fp64': {'s_per_step': round(s64, 5), 'iters': it64[-1], 'rel_residual': r64.rel_residual},
-        'mixed': {'s_per_step': round(smx, 5), 'iters': itmx[-1], 'outer': rmx.outer,
+        'mixed': {'s_per_step': round(smx, 5), 'iters': itmx[-1], 'outer': rmx.outer, 's_step': ss,
                   'rel_residual': rmx.rel_residual, 'rel_error_vs_fp64': err},
+        'fp64_s_step': ({'s_per_step': round(s64s, 5), 'iters': it64s[-1], 'rel_residual': r64s.rel_residual}
+                        if s64s is not None else None),
         'speedup_mixed_vs_fp64': round(s64 / smx, 3),
         'dofs_per_s': round(gs.ndof / smx, 1),
         'roofline': roof,

@@ -270,13 +270,18 @@ typedef struct {
   double inner_tol;    /* mixed: relative reduction asked of each inner solve    */
   int32_t check_every; /* CG: residual polled by the host every k iterations     */
   int32_t max_outer;   /* mixed: refinement steps                                */
+  int32_t s_step;      /* GMRES on the DG block-Jacobi operator: 5 = s-step cycles
+                          (blocks of 5 Arnoldi steps, one basis pass per block;
+                          restart a multiple of 5, at most 50), else standard   */
+  int32_t reserved;
 } ocmp_solve_opts;
 
 typedef struct {
   int32_t iters;        /* Krylov iterations (all inner solves)           */
   int32_t outer;        /* refinement steps (1 for FP64)                  */
   int32_t converged;
-  int32_t fallback;     /* FGMRES with a learned correction: 1 if it fell back to the plain one */
+  int32_t fallback;     /* FGMRES with a learned correction: 1 if it fell back to the plain one;
+                           s-step GMRES: 1 if a block lost definiteness and standard cycles took over */
   double rel_residual;  /* final ||b-Ax||/||b||, FP64, free dofs          */
   double initial_rel_residual;
 } ocmp_solve_stats;

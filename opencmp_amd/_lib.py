@@ -51,7 +51,7 @@ class LTerm(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [('solver', C.c_int32), ('precision', C.c_int32), ('restart', C.c_int32), ('maxit', C.c_int32),
                 ('tol', C.c_double), ('inner_tol', C.c_double), ('check_every', C.c_int32),
-                ('max_outer', C.c_int32)]
+                ('max_outer', C.c_int32), ('s_step', C.c_int32), ('reserved', C.c_int32)]
 
 
 class SolveStats(C.Structure):

@@ -493,7 +493,7 @@ class SolveResult:
 
 def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preconditioner], solver: str = 'CG',
           tol: float = 1e-10, maxit: int = 100000, precision: str = 'fp64', restart: int = 50,
-          inner_tol: float = 0.0, check_every: int = 0, max_outer: int = 0) -> SolveResult:
+          inner_tol: float = 0.0, check_every: int = 0, max_outer: int = 0, s_step: int = 0) -> SolveResult:
     o = L.SolveOpts()
     o.solver = {'CG': L.SOLVER_CG, 'GMRes': L.SOLVER_GMRES, 'Richardson': L.SOLVER_RICHARDSON,
                 'FGMRes': L.SOLVER_FGMRES}[solver]
@@ -504,6 +504,7 @@ def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preco
     o.inner_tol = inner_tol
     o.check_every = check_every
     o.max_outer = max_outer
+    o.s_step = s_step
     s = L.SolveStats()
     L.check(a.ctx.lib.ocmp_solve(a.h, pre.h if pre is not None else None, C.c_void_p(_ptr(b)),
                                  C.c_void_p(_ptr(x)), C.byref(o), C.byref(s)), 'solve')

@@ -15,6 +15,8 @@
 
 #include "internal.h"
 
+#include <type_traits>
+
 namespace ocmp {
 
 constexpr int NTHR = 256;
@@ -274,7 +276,9 @@ void spmv_f64(Matrix *A, const double *x, double *y) {
 template <typename T, int NB>
 __global__ void __launch_bounds__(NTHR) k_dg_ahat_spmv(int64_t nel, const int32_t *__restrict__ nbr,
                                                        const T *__restrict__ ahat, const T *__restrict__ x,
-                                                       const double *__restrict__ sx, T *__restrict__ y) {
+                                                       const double *__restrict__ sx, T *__restrict__ y,
+                                                       T dcoef = (T)1) {
+  // y = sx * ((Ahat - (1 - dcoef) I) x): dcoef = 1 is Ahat itself, dcoef = 0 its off-diagonal part
   constexpr int VW = Vec<T>::w;
   const int64_t n = nel * NB;
   const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VW;
@@ -282,6 +286,8 @@ __global__ void __launch_bounds__(NTHR) k_dg_ahat_spmv(int64_t nel, const int32_
   const T scale = sx ? (T)sx[0] : (T)1;
   T s[VW];
   vload(x + r0, s);
+#pragma unroll
+  for (int l = 0; l < VW; ++l) s[l] *= dcoef;
   int64_t cb[VW][3];
 #pragma unroll
   for (int l = 0; l < VW; ++l) {
@@ -1318,6 +1324,525 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
   return k_used;
 }
 
+// ------------------------------------------------------------------ s-step (communication-avoiding) GMRES
+//
+// One block of CA_S Arnoldi steps reads the basis once instead of CA_S times: the CA_S
+// monomial Krylov vectors w_i = Ahat^i q_k are generated by CA_S operator applications, then
+// orthogonalised against Q[0..k] with one block classical Gram-Schmidt pass (dot products in
+// one kernel, the update fused with the Gram matrix of the result), and against each other by
+// CholQR2 (Cholesky of the FP64 Gram matrix, twice). The Hessenberg columns follow from the
+// change of basis: Ahat [q_k..q_{k+s-1}] = V (Rr[:,1:s+1] - [H_old Rr[0:k,0:s]; 0]) T^{-1}.
+constexpr int CA_S = 5;
+// basis vectors per y-tile of the block dot kernel: with FP32 data each thread's partial sums
+// (a dozen products) accumulate in FP32, which leaves the registers for 16 rows; wave and block
+// sums are FP64 in both cases
+template <typename T>
+struct CaTile {
+  static constexpr int KQ = sizeof(T) == 4 ? 16 : 8;
+  using Acc = typename std::conditional<sizeof(T) == 4, float, double>::type;
+};
+constexpr int CA_KQ_MAX = 16;
+// The block-Jacobi-folded DG operator has identity diagonal blocks, so its spectrum sits around
+// 1: the basis is built with (Ahat - CA_SIGMA I), which keeps the Gram matrix of a block of 5
+// at condition ~1e4 instead of ~1e7 for plain powers (small enough for FP32 storage).
+constexpr double CA_SIGMA = 1.0;
+constexpr int CA_NG = CA_S * (CA_S + 1) / 2;
+
+// part[(q*S + i) * grid + lb] = partial of Q[q] . W[i],  q < nq
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_bdot(int64_t n, const T *__restrict__ Q, int64_t ldv, int nq,
+                                               const T *__restrict__ W, double *__restrict__ part) {
+  constexpr int VW = Vec<T>::w;
+  constexpr int CA_KQ = CaTile<T>::KQ;
+  using Acc = typename CaTile<T>::Acc;
+  __shared__ double sh[NTHR / 64][CA_KQ * CA_S];
+  const int lb = xcd_block();
+  const int64_t nvec = n / VW;
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
+  const int q0 = blockIdx.y * CA_KQ;
+  const int nqt = min(CA_KQ, nq - q0);
+  Acc acc[CA_KQ][CA_S];
+#pragma unroll
+  for (int c = 0; c < CA_KQ; ++c)
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) acc[c][t] = (Acc)0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    T wv[CA_S][VW];
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) vload(W + (int64_t)t * ldv + i * VW, wv[t]);
+#pragma unroll
+    for (int c = 0; c < CA_KQ; ++c) {
+      if (c < nqt) {
+        T qv[VW];
+        vload(Q + (int64_t)(q0 + c) * ldv + i * VW, qv);
+#pragma unroll
+        for (int t = 0; t < CA_S; ++t) {
+          Acc sum = (Acc)0;
+#pragma unroll
+          for (int l = 0; l < VW; ++l) sum += (Acc)qv[l] * (Acc)wv[t][l];
+          acc[c][t] += sum;
+        }
+      }
+    }
+  }
+  if (lb == gridDim.x - 1)
+    for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+      for (int c = 0; c < CA_KQ; ++c)
+        if (c < nqt)
+#pragma unroll
+          for (int t = 0; t < CA_S; ++t)
+            acc[c][t] += (Acc)Q[(int64_t)(q0 + c) * ldv + i] * (Acc)W[(int64_t)t * ldv + i];
+#pragma unroll
+  for (int c = 0; c < CA_KQ; ++c)
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) {
+      const double v = wave_sum((double)acc[c][t]);
+      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][c * CA_S + t] = v;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < nqt * CA_S) {
+    double sum = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < NTHR / 64; ++wv) sum += sh[wv][threadIdx.x];
+    part[(int64_t)(q0 * CA_S + threadIdx.x) * gridDim.x + lb] = sum;
+  }
+}
+
+// W[t] -= sum_q C[q][t] Q[q]  (C: nq x S, q-major), then partials of the Gram matrix of the
+// rounded result, packed upper triangle: part[g * grid + lb]
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_bupdate_gram(int64_t n, const T *__restrict__ Q, int64_t ldv, int nq,
+                                                       T *__restrict__ W, const double *__restrict__ C,
+                                                       double *__restrict__ part) {
+  constexpr int VW = Vec<T>::w;
+  __shared__ double cq[(256 / CA_S) * CA_S];
+  __shared__ double sh[NTHR / 64][CA_NG];
+  for (int e = threadIdx.x; e < nq * CA_S; e += blockDim.x) cq[e] = C[e];
+  __syncthreads();
+  const int lb = xcd_block();
+  const int64_t nvec = n / VW;
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
+  double gacc[CA_NG];
+#pragma unroll
+  for (int g = 0; g < CA_NG; ++g) gacc[g] = 0.0;
+  auto row = [&](double (&w)[CA_S], auto qload) {
+    for (int q = 0; q < nq; ++q) {
+      const double qv = qload(q);
+#pragma unroll
+      for (int t = 0; t < CA_S; ++t) w[t] -= cq[q * CA_S + t] * qv;
+    }
+  };
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    T wv[CA_S][VW];
+    double w[VW][CA_S];
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) {
+      vload(W + (int64_t)t * ldv + i * VW, wv[t]);
+#pragma unroll
+      for (int l = 0; l < VW; ++l) w[l][t] = (double)wv[t][l];
+    }
+    for (int q = 0; q < nq; ++q) {
+      T qv[VW];
+      vload(Q + (int64_t)q * ldv + i * VW, qv);
+#pragma unroll
+      for (int t = 0; t < CA_S; ++t) {
+        const double c = cq[q * CA_S + t];
+#pragma unroll
+        for (int l = 0; l < VW; ++l) w[l][t] -= c * (double)qv[l];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) {
+#pragma unroll
+      for (int l = 0; l < VW; ++l) wv[t][l] = (T)w[l][t];
+      vstore(W + (int64_t)t * ldv + i * VW, wv[t]);
+    }
+#pragma unroll
+    for (int l = 0; l < VW; ++l) {
+      int g = 0;
+#pragma unroll
+      for (int a = 0; a < CA_S; ++a)
+#pragma unroll
+        for (int b = a; b < CA_S; ++b, ++g) gacc[g] += (double)wv[a][l] * (double)wv[b][l];
+    }
+  }
+  if (lb == gridDim.x - 1)
+    for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x) {
+      double w[CA_S];
+      for (int t = 0; t < CA_S; ++t) w[t] = (double)W[(int64_t)t * ldv + i];
+      row(w, [&](int q) { return (double)Q[(int64_t)q * ldv + i]; });
+      T o[CA_S];
+      for (int t = 0; t < CA_S; ++t) {
+        o[t] = (T)w[t];
+        W[(int64_t)t * ldv + i] = o[t];
+      }
+      int g = 0;
+      for (int a = 0; a < CA_S; ++a)
+        for (int b = a; b < CA_S; ++b, ++g) gacc[g] += (double)o[a] * (double)o[b];
+    }
+#pragma unroll
+  for (int g = 0; g < CA_NG; ++g) {
+    const double v = wave_sum(gacc[g]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][g] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < CA_NG) {
+    double sum = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < NTHR / 64; ++wv) sum += sh[wv][threadIdx.x];
+    part[(int64_t)threadIdx.x * gridDim.x + lb] = sum;
+  }
+}
+
+// W <- W Rinv (Rinv upper triangular S x S, row-major), Gram partials of the result if part
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_btrsm_gram(int64_t n, T *__restrict__ W, int64_t ldv,
+                                                     const double *__restrict__ Rinv, double *__restrict__ part) {
+  constexpr int VW = Vec<T>::w;
+  __shared__ double ri[CA_S * CA_S];
+  __shared__ double sh[NTHR / 64][CA_NG];
+  if ((int)threadIdx.x < CA_S * CA_S) ri[threadIdx.x] = Rinv[threadIdx.x];
+  __syncthreads();
+  const int lb = xcd_block();
+  const int64_t nvec = n / VW;
+  const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
+  double gacc[CA_NG];
+#pragma unroll
+  for (int g = 0; g < CA_NG; ++g) gacc[g] = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    T wv[CA_S][VW];
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) vload(W + (int64_t)t * ldv + i * VW, wv[t]);
+    T ov[CA_S][VW];
+#pragma unroll
+    for (int l = 0; l < VW; ++l)
+#pragma unroll
+      for (int t = 0; t < CA_S; ++t) {
+        double sum = 0.0;
+#pragma unroll
+        for (int a = 0; a <= t; ++a) sum += (double)wv[a][l] * ri[a * CA_S + t];
+        ov[t][l] = (T)sum;
+      }
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) vstore(W + (int64_t)t * ldv + i * VW, ov[t]);
+    if (part) {
+#pragma unroll
+      for (int l = 0; l < VW; ++l) {
+        int g = 0;
+#pragma unroll
+        for (int a = 0; a < CA_S; ++a)
+#pragma unroll
+          for (int b = a; b < CA_S; ++b, ++g) gacc[g] += (double)ov[a][l] * (double)ov[b][l];
+      }
+    }
+  }
+  if (lb == gridDim.x - 1)
+    for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x) {
+      double w[CA_S];
+      T o[CA_S];
+      for (int t = 0; t < CA_S; ++t) w[t] = (double)W[(int64_t)t * ldv + i];
+      for (int t = 0; t < CA_S; ++t) {
+        double sum = 0.0;
+        for (int a = 0; a <= t; ++a) sum += w[a] * ri[a * CA_S + t];
+        o[t] = (T)sum;
+        W[(int64_t)t * ldv + i] = o[t];
+      }
+      if (part) {
+        int g = 0;
+        for (int a = 0; a < CA_S; ++a)
+          for (int b = a; b < CA_S; ++b, ++g) gacc[g] += (double)o[a] * (double)o[b];
+      }
+    }
+  if (!part) return;
+#pragma unroll
+  for (int g = 0; g < CA_NG; ++g) {
+    const double v = wave_sum(gacc[g]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][g] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < CA_NG) {
+    double sum = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < NTHR / 64; ++wv) sum += sh[wv][threadIdx.x];
+    part[(int64_t)threadIdx.x * gridDim.x + lb] = sum;
+  }
+}
+
+// Cholesky G = R^T R of the packed Gram matrix; R and R^{-1} (row-major, upper). A non-positive
+// pivot sets *flag (the block is then discarded by the host) and returns the identity.
+__global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R, double *__restrict__ Rinv,
+                          double *__restrict__ flag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double G[CA_S][CA_S], U[CA_S][CA_S], Ui[CA_S][CA_S];
+  int g = 0;
+  for (int a = 0; a < CA_S; ++a)
+    for (int b = a; b < CA_S; ++b, ++g) G[a][b] = G[b][a] = gp[g];
+  bool ok = true;
+  for (int a = 0; a < CA_S; ++a)
+    for (int b = 0; b < CA_S; ++b) U[a][b] = 0.0;
+  for (int j = 0; j < CA_S && ok; ++j) {
+    double d = G[j][j];
+    for (int k = 0; k < j; ++k) d -= U[k][j] * U[k][j];
+    if (!(d > 0.0)) {
+      ok = false;
+      break;
+    }
+    U[j][j] = sqrt(d);
+    for (int c = j + 1; c < CA_S; ++c) {
+      double v = G[j][c];
+      for (int k = 0; k < j; ++k) v -= U[k][j] * U[k][c];
+      U[j][c] = v / U[j][j];
+    }
+  }
+  if (!ok) {
+    flag[0] = 1.0;
+    for (int a = 0; a < CA_S; ++a)
+      for (int b = 0; b < CA_S; ++b) {
+        R[a * CA_S + b] = a == b ? 1.0 : 0.0;
+        Rinv[a * CA_S + b] = a == b ? 1.0 : 0.0;
+      }
+    return;
+  }
+  // inverse of the upper triangular U, column by column (back substitution)
+  for (int c = 0; c < CA_S; ++c)
+    for (int r = CA_S - 1; r >= 0; --r) {
+      double v = (r == c) ? 1.0 : 0.0;
+      for (int k = r + 1; k < CA_S; ++k) v -= U[r][k] * Ui[k][c];
+      Ui[r][c] = v / U[r][r];
+    }
+  for (int a = 0; a < CA_S; ++a)
+    for (int b = 0; b < CA_S; ++b) {
+      R[a * CA_S + b] = U[a][b];
+      Rinv[a * CA_S + b] = (b < a) ? 0.0 : Ui[a][b];
+    }
+}
+
+// New Hessenberg columns k..k+S-1 from the change of basis (shifted monomial basis:
+// Ahat u_i = u_{i+1} + sigma u_i), then their Givens rotations.
+// Hf: unrotated Hessenberg, Hr: rotated (both column-major, ld = m+1). C: (k+1) x S q-major.
+__global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__restrict__ C,
+                                                 const double *__restrict__ R1, const double *__restrict__ R2,
+                                                 double *__restrict__ Hf, GState G) {
+  __shared__ double Rr[(256 / CA_S + CA_S + 1) * (CA_S + 1)];  // (k+1+S) x (S+1), column-major
+  __shared__ double HC[(256 / CA_S + 1) * CA_S];               // H_old (k+1 x k) @ Rr[0:k, 0:S]
+  __shared__ double Rt[CA_S * CA_S];
+  const int ld = m + 1, nr = k + 1 + CA_S;
+  if (threadIdx.x == 0) {
+    for (int a = 0; a < CA_S; ++a)
+      for (int b = 0; b < CA_S; ++b) {
+        double v = 0.0;
+        for (int t = 0; t < CA_S; ++t) v += R2[a * CA_S + t] * R1[t * CA_S + b];
+        Rt[a * CA_S + b] = v;
+      }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nr * (CA_S + 1); e += blockDim.x) {
+    const int r = e % nr, c = e / nr;
+    double v;
+    if (c == 0)
+      v = (r == k) ? 1.0 : 0.0;
+    else if (r <= k)
+      v = C[r * CA_S + (c - 1)];
+    else
+      v = Rt[(r - k - 1) * CA_S + (c - 1)];
+    Rr[c * nr + r] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (k + 1) * CA_S; e += blockDim.x) {
+    const int r = e % (k + 1), c = e / (k + 1);
+    double v = 0.0;
+    for (int t = (r > 0 ? r - 1 : 0); t < k; ++t) v += Hf[(int64_t)t * ld + r] * Rr[c * nr + t];  // H_old is upper Hessenberg
+    HC[c * (k + 1) + r] = v;
+  }
+  __syncthreads();
+  // Hnew = (Rr[:,1:S+1] + sigma Rr[:,0:S] - [HC; 0]) T^{-1},  T = Rr[k:k+S, 0:S] upper triangular;
+  // one thread per row r solves the row system x T = rhs
+  __shared__ double Hn[(256 / CA_S + CA_S + 1) * CA_S];
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    double x[CA_S];
+#pragma unroll
+    for (int c = 0; c < CA_S; ++c) {
+      double v = Rr[(c + 1) * nr + r] + CA_SIGMA * Rr[c * nr + r] - (r <= k ? HC[c * (k + 1) + r] : 0.0);
+#pragma unroll
+      for (int t = 0; t < c; ++t) v -= x[t] * Rr[c * nr + (k + t)];
+      x[c] = v / Rr[c * nr + (k + c)];
+    }
+#pragma unroll
+    for (int c = 0; c < CA_S; ++c) Hn[r * CA_S + c] = x[c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < CA_S * ld; e += blockDim.x) {
+    const int c = e / ld, r = e % ld;
+    Hf[(int64_t)(k + c) * ld + r] = (r < nr) ? Hn[r * CA_S + c] : 0.0;
+  }
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < CA_S; ++c) {
+    const int j = k + c;
+    const double *hf = Hn;
+    // rotated copy: apply the previous rotations, form the new one
+    double col[256 / CA_S + CA_S + 1];
+    for (int r = 0; r <= j + 1; ++r) col[r] = (r < nr) ? hf[r * CA_S + c] : 0.0;
+    for (int t = 0; t < j; ++t) {
+      const double cs = G.csn[2 * t], sn = G.csn[2 * t + 1];
+      const double a = col[t], b = col[t + 1];
+      col[t] = cs * a + sn * b;
+      col[t + 1] = -sn * a + cs * b;
+    }
+    const double a = col[j], b = col[j + 1];
+    const double den = hypot(a, b);
+    const double cs = den > 0.0 ? a / den : 1.0, sn = den > 0.0 ? b / den : 0.0;
+    G.csn[2 * j] = cs;
+    G.csn[2 * j + 1] = sn;
+    col[j] = den;
+    col[j + 1] = 0.0;
+    const double gj = G.g[j];
+    G.g[j] = cs * gj;
+    G.g[j + 1] = -sn * gj;
+    G.res[j] = fabs(G.g[j + 1]) * G.gscale;
+    double *R = G.R + (int64_t)j * ld;
+    for (int r = 0; r <= j + 1; ++r) R[r] = col[r];
+  }
+}
+
+// One CA-GMRES(m) cycle on the DG operator Ahat (block Jacobi folded in); same contract as
+// gmres_cycle. Returns -1 (and changes nothing in x) if the first block breaks down.
+template <typename T>
+static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, double *x, int m, double stop_abs,
+                          int maxit, Work &w, bool &breakdown) {
+  Ctx *ctx = A->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = A->n;
+  constexpr int VW = Vec<T>::w;
+  const int64_t ldv = (n + VW - 1) / VW * VW;
+  DBuf<T> *Vbuf;
+  if constexpr (sizeof(T) == 8) {
+    w.vd.alloc(std::max<size_t>(w.vd.n, (size_t)(m + 1) * ldv));
+    Vbuf = &w.vd;
+  } else {
+    w.vf.alloc(std::max<size_t>(w.vf.n, (size_t)(m + 1) * ldv));
+    Vbuf = &w.vf;
+  }
+  T *Q = Vbuf->p;
+  const size_t ns = (size_t)(m + 1) * CA_S + 4 * CA_S * CA_S + 2 * (size_t)(m + 1) * m + 2 * m + (m + 1) + (m + 8) +
+                    m + 8;
+  w.small.alloc(std::max(w.small.n, ns));
+  double *Cm = w.small.p, *R1 = Cm + (size_t)(m + 1) * CA_S, *R1i = R1 + CA_S * CA_S, *R2 = R1i + CA_S * CA_S,
+         *R2i = R2 + CA_S * CA_S, *Hf = R2i + CA_S * CA_S, *Hr = Hf + (size_t)(m + 1) * m, *csn = Hr + (size_t)(m + 1) * m,
+         *g = csn + 2 * m, *res = g + (m + 1), *coef = res + (m + 8), *flag = coef + m, *one = flag + 1;
+  const int grid = stream_grid((n + VW - 1) / VW);
+  // the block dot reduces CA_KQ x CA_S sums per block: fewer, longer blocks amortise that
+  const int dgrid = std::max(NXCD, grid / 4 / NXCD * NXCD);
+  ensure_part(w, (size_t)(m + 1) * CA_S * grid + 64);
+  k_scale_copy<double, T><<<nblk(n), NTHR, 0, st>>>(n, r0, 1.0 / beta, Q);
+  // g = e_0 (relative to beta), flag = 0, one = 1
+  k_gmres_init<<<1, 64, 0, st>>>(m, 1.0, g, flag, 0.0);
+  k_fill_t<double><<<1, 64, 0, st>>>(one, 1, 1.0);
+  double *hres = pinned(w, m + 16);
+  GState G{nullptr, nullptr, Hr, csn, g, res, m, beta};
+  const int c_op = ctx->cls(sizeof(T) == 8 ? "dg_ahat_spmv_f64" : "dg_ahat_spmv_f32");
+  const int c_dot = ctx->cls(sizeof(T) == 8 ? "ca_bdot_f64" : "ca_bdot_f32");
+  const int c_upd = ctx->cls(sizeof(T) == 8 ? "ca_bupdate_f64" : "ca_bupdate_f32");
+  const int c_qr = ctx->cls(sizeof(T) == 8 ? "ca_cholqr_f64" : "ca_cholqr_f32");
+  const int64_t nel = A->space->mesh->nel;
+  const unsigned gb = nblk((n + VW - 1) / VW);
+  const T *ah;
+  if constexpr (sizeof(T) == 8)
+    ah = P->ahat.p;
+  else
+    ah = P->ahat32.p;
+  struct Batch {
+    int j0, ev;
+  };
+  std::deque<Batch> inflight;
+  const int limit = std::min(m, std::max(CA_S, (maxit + CA_S - 1) / CA_S * CA_S)) / CA_S * CA_S;
+  int k = 0, bidx = 0, k_used = -1;
+  breakdown = false;
+  const T dc = (T)(1.0 - CA_SIGMA);
+  auto enqueue = [&]() {
+    for (int i = 0; i < CA_S; ++i) {
+      KTimer tm(ctx, c_op, (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * nel + 2.0 * n * sizeof(T));
+      const T *xin = Q + (int64_t)(k + i) * ldv;
+      T *yout = Q + (int64_t)(k + i + 1) * ldv;
+      switch (A->dg_nb) {
+        case 1: k_dg_ahat_spmv<T, 1><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
+        case 3: k_dg_ahat_spmv<T, 3><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
+        case 6: k_dg_ahat_spmv<T, 6><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
+        case 10: k_dg_ahat_spmv<T, 10><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
+        default: throw Error(OCMP_ERR_UNSUPPORTED, "DG block size");
+      }
+    }
+    T *W = Q + (int64_t)(k + 1) * ldv;
+    const int nq = k + 1;
+    {
+      constexpr int KQ = CaTile<T>::KQ;
+      KTimer tm(ctx, c_dot, (double)(nq + CA_S * ((nq + KQ - 1) / KQ)) * n * sizeof(T));
+      dim3 gd(dgrid, (nq + KQ - 1) / KQ);
+      k_bdot<T><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p);
+    }
+    reduce_to(w.part.p, dgrid, nq * CA_S, Cm, st);
+    {
+      KTimer tm(ctx, c_upd, (double)(nq + 2 * CA_S) * n * sizeof(T));
+      k_bupdate_gram<T><<<grid, NTHR, 0, st>>>(n, Q, ldv, nq, W, Cm, w.part.p);
+    }
+    double *gp = Cm + (size_t)(m + 1) * CA_S - CA_NG;  // packed Gram (the last rows of Cm are unused: nq <= m+1-S)
+    reduce_to(w.part.p, grid, CA_NG, gp, st);
+    {
+      KTimer tm(ctx, c_qr, 4.0 * CA_S * n * sizeof(T));
+      k_ca_chol<<<1, 64, 0, st>>>(gp, R1, R1i, flag);
+      k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, w.part.p);
+      reduce_to(w.part.p, grid, CA_NG, gp, st);
+      k_ca_chol<<<1, 64, 0, st>>>(gp, R2, R2i, flag);
+      k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R2i, nullptr);
+    }
+    k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G);
+    OCMP_HIP(hipMemcpyAsync(hres + k, res + k, CA_S * sizeof(double), hipMemcpyDeviceToHost, st));
+    OCMP_HIP(hipMemcpyAsync(hres + m + 8 + (bidx & 1), flag, sizeof(double), hipMemcpyDeviceToHost, st));
+    const int ev = bidx & 1;
+    OCMP_HIP(hipEventRecord(work_event(w, ev), st));
+    inflight.push_back({k, ev});
+    ++bidx;
+    k += CA_S;
+  };
+  while (k_used < 0) {
+    while (inflight.size() < 2 && k < limit) enqueue();
+    if (inflight.empty()) {
+      k_used = k;
+      break;
+    }
+    Batch b = inflight.front();
+    inflight.pop_front();
+    OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
+    if (hres[m + 8 + (b.ev)] != 0.0) {  // CholQR broke down in this block: keep the columns before it
+      breakdown = true;
+      k_used = b.j0;
+      break;
+    }
+    for (int s = 0; s < CA_S; ++s)
+      if (!(hres[b.j0 + s] > stop_abs)) {
+        k_used = b.j0 + s + 1;
+        break;
+      }
+  }
+  // drain the look-ahead before the buffers are reused
+  OCMP_HIP(hipStreamSynchronize(st));
+  if (k_used <= 0) return breakdown ? -1 : 0;
+  k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, Hr, g, nullptr, coef);
+  w.u64.alloc(n);
+  k_combine<T><<<nblk(n), NTHR, 0, st>>>(n, Q, ldv, k_used, coef, beta, w.u64.p);
+  switch (A->dg_nb) {
+    case 1: k_dg_block_apply_add<1><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+    case 3: k_dg_block_apply_add<3><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+    case 6: k_dg_block_apply_add<6><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+    case 10: k_dg_block_apply_add<10><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+  }
+  OCMP_HIP(hipGetLastError());
+  return k_used;
+}
+
 // Flexible GMRES cycle (right preconditioning that may vary between iterations):
 // z_j = M_j v_j with M = D^{-1} (+ the learned correction when `learned`), w = A z_j,
 // x += Z y. Same device-side Gram-Schmidt / Givens machinery as gmres_cycle.
@@ -1516,14 +2041,26 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
   const int m = o.restart;
   int total = 0, outer = 0;
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
+  // s-step cycles on the DG block-Jacobi operator when asked for and the restart fits the block size
+  bool ca = o.s_step == CA_S && A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && m % CA_S == 0 && m <= 50;
   while (rn > o.tol * bref && total < o.maxit && std::isfinite(rn)) {
     ++outer;
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
-    int used;
-    if (mixed)
-      used = gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
-    else
-      used = gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+    int used = -1;
+    if (ca) {
+      bool broke = false;
+      if (mixed)
+        used = ca_gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke);
+      else
+        used = ca_gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke);
+      if (broke) ca = false, st_.fallback = 1;  // CholQR lost definiteness: standard cycles from here on
+    }
+    if (used < 0) {
+      if (mixed)
+        used = gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+      else
+        used = gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+    }
     total += used;
     const double rn_old = rn;
     rn = residual64(A, P, b, x, w.r64.p, w);

@@ -37,7 +37,7 @@ def test_struct_layouts_match_header():
     # sizes of the ABI structs as laid out by the C compiler (x86-64 SysV)
     assert ctypes.sizeof(_lib.FeDesc) == 24
     assert ctypes.sizeof(_lib.Field) == 24
-    assert ctypes.sizeof(_lib.SolveOpts) == 40
+    assert ctypes.sizeof(_lib.SolveOpts) == 48
     assert ctypes.sizeof(_lib.SolveStats) == 32
     assert ctypes.sizeof(_lib.Term) == 104
     assert ctypes.sizeof(_lib.LTerm) == 160

@@ -417,3 +417,42 @@ def test_3d_p3_kappa_mixed(gpu_ctx):
         res[prec] = x.numpy()
     d = np.abs(res['fp64'] - res['mixed']).max() / np.abs(res['fp64']).max()
     assert d < 1e-8
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_s_step_gmres_matches_standard(gpu_ctx, precision):
+    """s-step GMRES(50) (blocks of 5 Arnoldi steps, block Gram-Schmidt + CholQR2) on the DG
+    block-Jacobi operator: same solution and about the same iteration count as GMRES(50)."""
+    from opencmp_amd import core as C, _lib as L
+    from opencmp_amd.mesh import unit_square
+    print('[gpu-test] s-step gmres', precision, flush=True)
+    hm = unit_square(64)
+    gm = C.Mesh(hm, gpu_ctx)
+    gs = C.FESpace(gm, [C.L2(2)], dgjumps=True)
+    w = (np.cos(0.3), np.sin(0.3))
+    a = C.BilinearForm(gs)
+    a += C.Vol('gradgrad', 0, 0, coef=1e-3)
+    a += C.Vol('conv', 0, 0, coef=-1.0, wind=(w[0], w[1], 0.0))
+    a += C.DGFacet(0, eps=1e-3, alpha=40.0, wind=(w[0], w[1]), dirichlet=hm.marker_mask(hm.bnd_names))
+    a.Assemble()
+    f = C.LinearForm(gs)
+    f += C.Source(0, (L.FN_CONST, (1.0,)))
+    f.Assemble()
+    pre = C.Preconditioner(a, 'local', precision=precision)
+    pre.Update()
+    x1 = C.GridFunction(gs)
+    x2 = C.GridFunction(gs)
+    r1 = C.solve(a, f.vec, x1.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, maxit=100000)
+    r2 = C.solve(a, f.vec, x2.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, maxit=100000,
+                 s_step=5)
+    print('[gpu-test]   standard', r1, 's-step', r2, flush=True)
+    assert r1.converged and r2.converged and r2.rel_residual <= 1e-8
+    assert not r2.fallback
+    assert r2.iters <= 1.2 * r1.iters + 10
+    rel = float(torch.linalg.norm(x1.vec - x2.vec) / torch.linalg.norm(x1.vec))
+    assert rel < 1e-6
+    # bitwise repeatable
+    x3 = C.GridFunction(gs)
+    r3 = C.solve(a, f.vec, x3.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, maxit=100000,
+                 s_step=5)
+    assert r3.iters == r2.iters and torch.equal(x3.vec, x2.vec)
