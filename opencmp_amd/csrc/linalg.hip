@@ -1621,6 +1621,11 @@ __global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R,
     }
 }
 
+__global__ void k_ca_ident(double *__restrict__ R) {
+  const int i = threadIdx.x;
+  if (i < CA_S * CA_S) R[i] = (i / CA_S == i % CA_S) ? 1.0 : 0.0;
+}
+
 // New Hessenberg columns k..k+S-1 from the change of basis (shifted monomial basis:
 // Ahat u_i = u_{i+1} + sigma u_i), then their Givens rotations.
 // Hf: unrotated Hessenberg, Hr: rotated (both column-major, ld = m+1). C: (k+1) x S q-major.
@@ -1778,6 +1783,8 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
     T *W = Q + (int64_t)(k + 1) * ldv;
     const int nq = k + 1;
     {
+      // (an MFMA version of this kernel, v_mfma_f32_16x16x4_f32 tiles reading Q and W once, measured
+      // 156 us against 121 us here: its 16-row operand gathers coalesce worse than these row streams)
       constexpr int KQ = CaTile<T>::KQ;
       KTimer tm(ctx, c_dot, (double)(nq + CA_S * ((nq + KQ - 1) / KQ)) * n * sizeof(T));
       dim3 gd(dgrid, (nq + KQ - 1) / KQ);
@@ -1791,12 +1798,19 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
     double *gp = Cm + (size_t)(m + 1) * CA_S - CA_NG;  // packed Gram (the last rows of Cm are unused: nq <= m+1-S)
     reduce_to(w.part.p, grid, CA_NG, gp, st);
     {
-      KTimer tm(ctx, c_qr, 4.0 * CA_S * n * sizeof(T));
+      // CholQR, and a second pass (CholQR2) for FP64 bases only: the shifted blocks have Gram
+      // condition ~1e4, so one pass leaves FP32 vectors orthogonal to ~1e-6, below their precision
+      KTimer tm(ctx, c_qr, (sizeof(T) == 8 ? 4.0 : 2.0) * CA_S * n * sizeof(T));
       k_ca_chol<<<1, 64, 0, st>>>(gp, R1, R1i, flag);
-      k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, w.part.p);
-      reduce_to(w.part.p, grid, CA_NG, gp, st);
-      k_ca_chol<<<1, 64, 0, st>>>(gp, R2, R2i, flag);
-      k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R2i, nullptr);
+      if (sizeof(T) == 8) {
+        k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, w.part.p);
+        reduce_to(w.part.p, grid, CA_NG, gp, st);
+        k_ca_chol<<<1, 64, 0, st>>>(gp, R2, R2i, flag);
+        k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R2i, nullptr);
+      } else {
+        k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, nullptr);
+        k_ca_ident<<<1, 64, 0, st>>>(R2);
+      }
     }
     k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G);
     OCMP_HIP(hipMemcpyAsync(hres + k, res + k, CA_S * sizeof(double), hipMemcpyDeviceToHost, st));
