@@ -1349,11 +1349,12 @@ constexpr double CA_SIGMA = 1.0;
 constexpr int CA_NG = CA_S * (CA_S + 1) / 2;
 
 // part[(q*S + i) * grid + lb] = partial of Q[q] . W[i],  q < nq
-template <typename T>
+// Rows past nq in the last y-tile read row nq-1 again (no branch around the loads, so all of a
+// thread's row loads are in flight together) and their sums are dropped.
+template <typename T, int CA_KQ>
 __global__ void __launch_bounds__(NTHR) k_bdot(int64_t n, const T *__restrict__ Q, int64_t ldv, int nq,
                                                const T *__restrict__ W, double *__restrict__ part) {
   constexpr int VW = Vec<T>::w;
-  constexpr int CA_KQ = CaTile<T>::KQ;
   using Acc = typename CaTile<T>::Acc;
   __shared__ double sh[NTHR / 64][CA_KQ * CA_S];
   const int lb = xcd_block();
@@ -1371,29 +1372,26 @@ __global__ void __launch_bounds__(NTHR) k_bdot(int64_t n, const T *__restrict__ 
     T wv[CA_S][VW];
 #pragma unroll
     for (int t = 0; t < CA_S; ++t) vload(W + (int64_t)t * ldv + i * VW, wv[t]);
+    T qv[CA_KQ][VW];
 #pragma unroll
-    for (int c = 0; c < CA_KQ; ++c) {
-      if (c < nqt) {
-        T qv[VW];
-        vload(Q + (int64_t)(q0 + c) * ldv + i * VW, qv);
+    for (int c = 0; c < CA_KQ; ++c) vload(Q + (int64_t)min(q0 + c, nq - 1) * ldv + i * VW, qv[c]);
 #pragma unroll
-        for (int t = 0; t < CA_S; ++t) {
-          Acc sum = (Acc)0;
+    for (int c = 0; c < CA_KQ; ++c)
 #pragma unroll
-          for (int l = 0; l < VW; ++l) sum += (Acc)qv[l] * (Acc)wv[t][l];
-          acc[c][t] += sum;
-        }
+      for (int t = 0; t < CA_S; ++t) {
+        Acc sum = (Acc)0;
+#pragma unroll
+        for (int l = 0; l < VW; ++l) sum += (Acc)qv[c][l] * (Acc)wv[t][l];
+        acc[c][t] += sum;
       }
-    }
   }
   if (lb == gridDim.x - 1)
     for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x)
 #pragma unroll
       for (int c = 0; c < CA_KQ; ++c)
-        if (c < nqt)
 #pragma unroll
-          for (int t = 0; t < CA_S; ++t)
-            acc[c][t] += (Acc)Q[(int64_t)(q0 + c) * ldv + i] * (Acc)W[(int64_t)t * ldv + i];
+        for (int t = 0; t < CA_S; ++t)
+          acc[c][t] += (Acc)Q[(int64_t)min(q0 + c, nq - 1) * ldv + i] * (Acc)W[(int64_t)t * ldv + i];
 #pragma unroll
   for (int c = 0; c < CA_KQ; ++c)
 #pragma unroll
@@ -1573,57 +1571,62 @@ __global__ void __launch_bounds__(NTHR) k_btrsm_gram(int64_t n, T *__restrict__ 
 }
 
 // Cholesky G = R^T R of the packed Gram matrix; R and R^{-1} (row-major, upper). A non-positive
-// pivot sets *flag (the block is then discarded by the host) and returns the identity.
+// pivot sets *flag (the block is then discarded by the host) and returns the identity. Every loop
+// has a constant trip count so the 5x5 arrays stay in registers. ident, if given, is set to the
+// identity (the second factor of a single-pass CholQR).
 __global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R, double *__restrict__ Rinv,
-                          double *__restrict__ flag) {
+                          double *__restrict__ flag, double *__restrict__ ident) {
+  if (ident && threadIdx.x < CA_S * CA_S) ident[threadIdx.x] = (threadIdx.x / CA_S == threadIdx.x % CA_S) ? 1.0 : 0.0;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double G[CA_S][CA_S], U[CA_S][CA_S], Ui[CA_S][CA_S];
-  int g = 0;
+#pragma unroll
   for (int a = 0; a < CA_S; ++a)
-    for (int b = a; b < CA_S; ++b, ++g) G[a][b] = G[b][a] = gp[g];
+#pragma unroll
+    for (int b = a; b < CA_S; ++b) G[a][b] = G[b][a] = gp[a * CA_S - a * (a - 1) / 2 + (b - a)];
   bool ok = true;
-  for (int a = 0; a < CA_S; ++a)
-    for (int b = 0; b < CA_S; ++b) U[a][b] = 0.0;
-  for (int j = 0; j < CA_S && ok; ++j) {
+#pragma unroll
+  for (int j = 0; j < CA_S; ++j) {
     double d = G[j][j];
+#pragma unroll
     for (int k = 0; k < j; ++k) d -= U[k][j] * U[k][j];
-    if (!(d > 0.0)) {
-      ok = false;
-      break;
-    }
-    U[j][j] = sqrt(d);
+    ok = ok && d > 0.0;
+    U[j][j] = sqrt(d > 0.0 ? d : 1.0);
+#pragma unroll
+    for (int c = 0; c < j; ++c) U[j][c] = 0.0;
+#pragma unroll
     for (int c = j + 1; c < CA_S; ++c) {
       double v = G[j][c];
+#pragma unroll
       for (int k = 0; k < j; ++k) v -= U[k][j] * U[k][c];
       U[j][c] = v / U[j][j];
     }
   }
   if (!ok) {
     flag[0] = 1.0;
-    for (int a = 0; a < CA_S; ++a)
-      for (int b = 0; b < CA_S; ++b) {
-        R[a * CA_S + b] = a == b ? 1.0 : 0.0;
-        Rinv[a * CA_S + b] = a == b ? 1.0 : 0.0;
-      }
+#pragma unroll
+    for (int i = 0; i < CA_S * CA_S; ++i) {
+      R[i] = (i / CA_S == i % CA_S) ? 1.0 : 0.0;
+      Rinv[i] = (i / CA_S == i % CA_S) ? 1.0 : 0.0;
+    }
     return;
   }
   // inverse of the upper triangular U, column by column (back substitution)
+#pragma unroll
   for (int c = 0; c < CA_S; ++c)
+#pragma unroll
     for (int r = CA_S - 1; r >= 0; --r) {
       double v = (r == c) ? 1.0 : 0.0;
+#pragma unroll
       for (int k = r + 1; k < CA_S; ++k) v -= U[r][k] * Ui[k][c];
       Ui[r][c] = v / U[r][r];
     }
+#pragma unroll
   for (int a = 0; a < CA_S; ++a)
+#pragma unroll
     for (int b = 0; b < CA_S; ++b) {
       R[a * CA_S + b] = U[a][b];
       Rinv[a * CA_S + b] = (b < a) ? 0.0 : Ui[a][b];
     }
-}
-
-__global__ void k_ca_ident(double *__restrict__ R) {
-  const int i = threadIdx.x;
-  if (i < CA_S * CA_S) R[i] = (i / CA_S == i % CA_S) ? 1.0 : 0.0;
 }
 
 // New Hessenberg columns k..k+S-1 from the change of basis (shifted monomial basis:
@@ -1632,20 +1635,25 @@ __global__ void k_ca_ident(double *__restrict__ R) {
 __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__restrict__ C,
                                                  const double *__restrict__ R1, const double *__restrict__ R2,
                                                  double *__restrict__ Hf, GState G) {
-  __shared__ double Rr[(256 / CA_S + CA_S + 1) * (CA_S + 1)];  // (k+1+S) x (S+1), column-major
-  __shared__ double HC[(256 / CA_S + 1) * CA_S];               // H_old (k+1 x k) @ Rr[0:k, 0:S]
+  constexpr int MAXR = 256 / CA_S + CA_S + 1;
+  __shared__ double Rr[MAXR * (CA_S + 1)];          // (k+1+S) x (S+1), column-major
+  __shared__ double HC[(256 / CA_S + 1) * CA_S];    // H_old (k+1 x k) @ Rr[0:k, 0:S]
   __shared__ double Rt[CA_S * CA_S];
+  __shared__ double Hn[MAXR * CA_S];
+  __shared__ double rot[2 * MAXR];                  // the k rotations of the earlier columns
+  __shared__ double tl[CA_S][CA_S + 2];             // rows k..k+c+1 of column k+c after those rotations
   const int ld = m + 1, nr = k + 1 + CA_S;
-  if (threadIdx.x == 0) {
-    for (int a = 0; a < CA_S; ++a)
-      for (int b = 0; b < CA_S; ++b) {
-        double v = 0.0;
-        for (int t = 0; t < CA_S; ++t) v += R2[a * CA_S + t] * R1[t * CA_S + b];
-        Rt[a * CA_S + b] = v;
-      }
+  const int tid = threadIdx.x;
+  if (tid < CA_S * CA_S) {
+    const int a = tid / CA_S, b = tid % CA_S;
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < CA_S; ++t) v += R2[a * CA_S + t] * R1[t * CA_S + b];
+    Rt[a * CA_S + b] = v;
   }
+  for (int e = tid; e < 2 * k; e += blockDim.x) rot[e] = G.csn[e];
   __syncthreads();
-  for (int e = threadIdx.x; e < nr * (CA_S + 1); e += blockDim.x) {
+  for (int e = tid; e < nr * (CA_S + 1); e += blockDim.x) {
     const int r = e % nr, c = e / nr;
     double v;
     if (c == 0)
@@ -1657,7 +1665,7 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
     Rr[c * nr + r] = v;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < (k + 1) * CA_S; e += blockDim.x) {
+  for (int e = tid; e < (k + 1) * CA_S; e += blockDim.x) {
     const int r = e % (k + 1), c = e / (k + 1);
     double v = 0.0;
     for (int t = (r > 0 ? r - 1 : 0); t < k; ++t) v += Hf[(int64_t)t * ld + r] * Rr[c * nr + t];  // H_old is upper Hessenberg
@@ -1666,8 +1674,7 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
   __syncthreads();
   // Hnew = (Rr[:,1:S+1] + sigma Rr[:,0:S] - [HC; 0]) T^{-1},  T = Rr[k:k+S, 0:S] upper triangular;
   // one thread per row r solves the row system x T = rhs
-  __shared__ double Hn[(256 / CA_S + CA_S + 1) * CA_S];
-  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+  for (int r = tid; r < nr; r += blockDim.x) {
     double x[CA_S];
 #pragma unroll
     for (int c = 0; c < CA_S; ++c) {
@@ -1680,36 +1687,57 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
     for (int c = 0; c < CA_S; ++c) Hn[r * CA_S + c] = x[c];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < CA_S * ld; e += blockDim.x) {
+  for (int e = tid; e < CA_S * ld; e += blockDim.x) {
     const int c = e / ld, r = e % ld;
     Hf[(int64_t)(k + c) * ld + r] = (r < nr) ? Hn[r * CA_S + c] : 0.0;
   }
-  if (threadIdx.x != 0) return;
+  // rotated copy: thread c sweeps the k earlier rotations down column k+c (rows 0..k-1 are
+  // final after them), carrying the running entry in a register
+  if (tid < CA_S) {
+    const int c = tid;
+    double *Rc = G.R + (int64_t)(k + c) * ld;
+    double a = Hn[c];
+    for (int t = 0; t < k; ++t) {
+      const double b = Hn[(t + 1) * CA_S + c], cs = rot[2 * t], sn = rot[2 * t + 1];
+      Rc[t] = cs * a + sn * b;
+      a = -sn * a + cs * b;
+    }
+    tl[c][0] = a;
+#pragma unroll
+    for (int i = 1; i < CA_S + 2; ++i) tl[c][i] = (i <= c + 1) ? Hn[(k + i) * CA_S + c] : 0.0;
+  }
+  __syncthreads();
+  if (tid != 0) return;
+  // the S new rotations, each applied to the later columns of the block
+  double cb[CA_S], sb[CA_S];
+  double gcur = G.g[k];
+#pragma unroll
   for (int c = 0; c < CA_S; ++c) {
     const int j = k + c;
-    const double *hf = Hn;
-    // rotated copy: apply the previous rotations, form the new one
-    double col[256 / CA_S + CA_S + 1];
-    for (int r = 0; r <= j + 1; ++r) col[r] = (r < nr) ? hf[r * CA_S + c] : 0.0;
-    for (int t = 0; t < j; ++t) {
-      const double cs = G.csn[2 * t], sn = G.csn[2 * t + 1];
-      const double a = col[t], b = col[t + 1];
-      col[t] = cs * a + sn * b;
-      col[t + 1] = -sn * a + cs * b;
+    double v[CA_S + 2];
+#pragma unroll
+    for (int i = 0; i < CA_S + 2; ++i) v[i] = tl[c][i];
+#pragma unroll
+    for (int i = 0; i < c; ++i) {
+      const double a = v[i], b = v[i + 1];
+      v[i] = cb[i] * a + sb[i] * b;
+      v[i + 1] = -sb[i] * a + cb[i] * b;
     }
-    const double a = col[j], b = col[j + 1];
+    const double a = v[c], b = v[c + 1];
     const double den = hypot(a, b);
-    const double cs = den > 0.0 ? a / den : 1.0, sn = den > 0.0 ? b / den : 0.0;
-    G.csn[2 * j] = cs;
-    G.csn[2 * j + 1] = sn;
-    col[j] = den;
-    col[j + 1] = 0.0;
-    const double gj = G.g[j];
-    G.g[j] = cs * gj;
-    G.g[j + 1] = -sn * gj;
-    G.res[j] = fabs(G.g[j + 1]) * G.gscale;
-    double *R = G.R + (int64_t)j * ld;
-    for (int r = 0; r <= j + 1; ++r) R[r] = col[r];
+    cb[c] = den > 0.0 ? a / den : 1.0;
+    sb[c] = den > 0.0 ? b / den : 0.0;
+    G.csn[2 * j] = cb[c];
+    G.csn[2 * j + 1] = sb[c];
+    v[c] = den;
+    v[c + 1] = 0.0;
+    G.g[j] = cb[c] * gcur;
+    gcur = -sb[c] * gcur;
+    G.g[j + 1] = gcur;
+    G.res[j] = fabs(gcur) * G.gscale;
+    double *Rc = G.R + (int64_t)j * ld + k;
+#pragma unroll
+    for (int i = 0; i <= c + 1; ++i) Rc[i] = v[i];
   }
 }
 
@@ -1740,6 +1768,8 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
          *g = csn + 2 * m, *res = g + (m + 1), *coef = res + (m + 8), *flag = coef + m, *one = flag + 1;
   const int grid = stream_grid((n + VW - 1) / VW);
   // the block dot reduces CA_KQ x CA_S sums per block: fewer, longer blocks amortise that
+  // the block dot runs on a quarter of the streaming grid per y-tile (measured 88 us against
+  // 91 us at half and 117 us at the full grid, DG-P2 512^2 FP32): fewer, longer row streams
   const int dgrid = std::max(NXCD, grid / 4 / NXCD * NXCD);
   ensure_part(w, (size_t)(m + 1) * CA_S * grid + 64);
   k_scale_copy<double, T><<<nblk(n), NTHR, 0, st>>>(n, r0, 1.0 / beta, Q);
@@ -1785,10 +1815,18 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
     {
       // (an MFMA version of this kernel, v_mfma_f32_16x16x4_f32 tiles reading Q and W once, measured
       // 156 us against 121 us here: its 16-row operand gathers coalesce worse than these row streams)
+      // tile height: the fewest y-tiles of at most KQ rows, each as short as the rows allow
       constexpr int KQ = CaTile<T>::KQ;
-      KTimer tm(ctx, c_dot, (double)(nq + CA_S * ((nq + KQ - 1) / KQ)) * n * sizeof(T));
-      dim3 gd(dgrid, (nq + KQ - 1) / KQ);
-      k_bdot<T><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p);
+      const int ny = (nq + KQ - 1) / KQ, per = (nq + ny - 1) / ny;
+      const int kq = per <= 4 ? 4 : per <= 8 ? 8 : per <= 12 ? 12 : 16;
+      KTimer tm(ctx, c_dot, (double)(nq + CA_S * ny) * n * sizeof(T));
+      dim3 gd(dgrid, (nq + kq - 1) / kq);
+      switch (kq) {
+        case 4: k_bdot<T, 4><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p); break;
+        case 8: k_bdot<T, 8><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p); break;
+        case 12: k_bdot<T, (KQ >= 12 ? 12 : KQ)><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p); break;
+        default: k_bdot<T, KQ><<<gd, NTHR, 0, st>>>(n, Q, ldv, nq, W, w.part.p); break;
+      }
     }
     reduce_to(w.part.p, dgrid, nq * CA_S, Cm, st);
     {
@@ -1801,15 +1839,14 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
       // CholQR, and a second pass (CholQR2) for FP64 bases only: the shifted blocks have Gram
       // condition ~1e4, so one pass leaves FP32 vectors orthogonal to ~1e-6, below their precision
       KTimer tm(ctx, c_qr, (sizeof(T) == 8 ? 4.0 : 2.0) * CA_S * n * sizeof(T));
-      k_ca_chol<<<1, 64, 0, st>>>(gp, R1, R1i, flag);
+      k_ca_chol<<<1, 64, 0, st>>>(gp, R1, R1i, flag, sizeof(T) == 8 ? nullptr : R2);
       if (sizeof(T) == 8) {
         k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, w.part.p);
         reduce_to(w.part.p, grid, CA_NG, gp, st);
-        k_ca_chol<<<1, 64, 0, st>>>(gp, R2, R2i, flag);
+        k_ca_chol<<<1, 64, 0, st>>>(gp, R2, R2i, flag, nullptr);
         k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R2i, nullptr);
       } else {
         k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, nullptr);
-        k_ca_ident<<<1, 64, 0, st>>>(R2);
       }
     }
     k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G);
