@@ -921,6 +921,7 @@ int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x, cons
     OCMP_REQUIRE((opts->solver != OCMP_SOLVER_GMRES && opts->solver != OCMP_SOLVER_FGMRES) ||
                      (opts->restart >= 1 && opts->restart <= 200),
                  "GMRES restart must be in [1, 200]");
+    OCMP_REQUIRE(opts->s_step == 0 || opts->s_step == 5, "s_step must be 0 (standard) or 5");
     OCMP_HIP(hipSetDevice(A->A.ctx->device));
     if (P && !P->P.ready) precond_update(&P->P);
     std::memset(stats, 0, sizeof(*stats));

@@ -40,7 +40,8 @@ class Model:
     (interior penalty coefficient), boundary conditions as dicts
     {marker_name: Fn}, and the [SOLVER] keys (``linear_solver``,
     ``preconditioner``, ``linear_tolerance``, ``linear_max_iterations``) plus the
-    GPU-path keys ``precision`` (fp64|mixed) and ``restart``.
+    GPU-path keys ``precision`` (fp64|mixed), ``restart`` and ``s_step`` (5: s-step GMRES
+    cycles where the operator allows them, the DG space with element-block Jacobi; 0: standard).
     """
 
     #: names of the model components in compound-space order
@@ -51,7 +52,7 @@ class Model:
                  dirichlet: Optional[Dict[str, Dict[str, Fn]]] = None, ctx: Optional[C.Context] = None,
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
-                 inner_tol: float = 0.0):
+                 inner_tol: float = 0.0, s_step: int = 5):
         self.ctx = ctx or C.Context.default()
         self.host_mesh = mesh
         self.mesh = C.Mesh(mesh, self.ctx)
@@ -76,6 +77,7 @@ class Model:
         self.precision = precision
         self.restart = restart
         self.inner_tol = inner_tol
+        self.s_step = s_step
         self.fes = self._construct_fes()
 
     # ---------------------------------------------------------------- hooks
@@ -140,7 +142,7 @@ class Model:
         from gfu (which carries the Dirichlet lift). Unlike the reference, returns the solver status."""
         return C.solve(a_assembled, L_assembled.vec, gfu.vec, precond, self.linear_solver,
                        tol=self.linear_tolerance, maxit=self.linear_max_iterations, precision=self.precision,
-                       restart=self.restart, inner_tol=self.inner_tol)
+                       restart=self.restart, inner_tol=self.inner_tol, s_step=self.s_step)
 
     def solve_single_step(self, a: C.BilinearForm, Lf: C.LinearForm, pre, gfu: C.GridFunction,
                           reassemble=None) -> C.SolveResult:

@@ -456,3 +456,17 @@ def test_s_step_gmres_matches_standard(gpu_ctx, precision):
     r3 = C.solve(a, f.vec, x3.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, maxit=100000,
                  s_step=5)
     assert r3.iters == r2.iters and torch.equal(x3.vec, x2.vec)
+    # a shorter restart (blocks k = 0..25) converges to the same solution
+    x4 = C.GridFunction(gs)
+    r4 = C.solve(a, f.vec, x4.vec, pre, 'GMRes', tol=1e-8, restart=30, precision=precision, maxit=100000,
+                 s_step=5)
+    assert r4.converged and r4.rel_residual <= 1e-8
+    assert float(torch.linalg.norm(x4.vec - x1.vec) / torch.linalg.norm(x1.vec)) < 1e-6
+    # a restart that is not a multiple of 5 runs standard cycles: bitwise the standard solve
+    x5, x6 = C.GridFunction(gs), C.GridFunction(gs)
+    r5 = C.solve(a, f.vec, x5.vec, pre, 'GMRes', tol=1e-8, restart=48, precision=precision, maxit=100000,
+                 s_step=5)
+    r6 = C.solve(a, f.vec, x6.vec, pre, 'GMRes', tol=1e-8, restart=48, precision=precision, maxit=100000)
+    assert r5.iters == r6.iters and torch.equal(x5.vec, x6.vec)
+    with pytest.raises(L.OcmpError, match='s_step'):
+        C.solve(a, f.vec, x5.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, s_step=3)
