@@ -409,25 +409,60 @@ void build_pattern(Matrix *A) {
   A->d_col.upload(A->col, st);
   A->d_val.alloc(A->nnz);
   OCMP_HIP(hipMemsetAsync(A->d_val.p, 0, A->nnz * sizeof(double), st));
-  build_row_blocks(A);
+  build_row_blocks(A, A->rb64, SPMV_NNZB64);
 }
 
-void build_row_blocks(Matrix *A) {
-  // greedy: extend a block while its nonzeros fit in SPMV_NNZB; a row longer than that is a block
-  // of its own (the kernel then loops over it in chunks)
-  A->rblk.clear();
-  A->rblk.push_back(0);
+void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb) {
+  // greedy: extend a block while its nonzeros fit in nnzb; a row longer than that is a block of
+  // its own (the kernel then loops over it in chunks and keeps global columns)
+  rb.rblk.clear();
+  rb.rblk.push_back(0);
   int64_t start_nz = 0;
   for (int64_t r = 0; r < A->n; ++r) {
     const int64_t end_nz = A->row_ptr[r + 1];
-    if (end_nz - start_nz > SPMV_NNZB && A->rblk.back() != r) {
-      A->rblk.push_back((int32_t)r);
+    if (end_nz - start_nz > nnzb && rb.rblk.back() != r) {
+      rb.rblk.push_back((int32_t)r);
       start_nz = A->row_ptr[r];
     }
   }
-  A->rblk.push_back((int32_t)A->n);
-  A->nrblk = (int64_t)A->rblk.size() - 1;
-  A->d_rblk.upload(A->rblk, A->ctx->stream);
+  rb.rblk.push_back((int32_t)A->n);
+  rb.nrblk = (int64_t)rb.rblk.size() - 1;
+  rb.d_rblk.upload(rb.rblk, A->ctx->stream);
+  // per block: its distinct columns, sorted, and for every nonzero the 16-bit position of its
+  // column in that list; the SpMV gathers x once per distinct column instead of once per nonzero
+  const int64_t nb = rb.nrblk;
+  std::vector<int32_t> nu(nb + 1, 0);
+  std::vector<uint16_t> lcol(A->nnz);
+  std::vector<std::vector<int32_t>> ub(nb);
+#pragma omp parallel
+  {
+    std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t b = 0; b < nb; ++b) {
+      const int64_t j0 = A->row_ptr[rb.rblk[b]], j1 = A->row_ptr[rb.rblk[b + 1]];
+      if (j1 - j0 > nnzb) continue;
+      buf.assign(A->col.begin() + j0, A->col.begin() + j1);
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      for (int64_t j = j0; j < j1; ++j)
+        lcol[j] = (uint16_t)(std::lower_bound(buf.begin(), buf.end(), A->col[j]) - buf.begin());
+      nu[b + 1] = (int32_t)buf.size();
+      ub[b] = buf;
+    }
+  }
+  for (int64_t b = 0; b < nb; ++b) nu[b + 1] += nu[b];
+  std::vector<int32_t> ucol(nu[nb]);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) std::copy(ub[b].begin(), ub[b].end(), ucol.begin() + nu[b]);
+  rb.nucol = nu[nb];
+  rb.d_ubase.upload(nu, A->ctx->stream);
+  rb.d_ucol.upload(ucol, A->ctx->stream);
+  rb.d_lcol.upload(lcol, A->ctx->stream);
+  rb.nnzb = nnzb;
+}
+
+void ensure_row_blocks32(Matrix *A) {
+  if (!A->rb32.nnzb) build_row_blocks(A, A->rb32, SPMV_NNZB32);
 }
 
 void build_colouring(Space &s) {

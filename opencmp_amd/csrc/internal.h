@@ -188,6 +188,17 @@ void eval_basis(int kind, int dim, int order, const double *lam, double *phi, do
 
 // ------------------------------------------------------------------ matrix
 
+// Rows grouped into blocks of at most nnzb nonzeros (block b = rows [rblk[b], rblk[b+1])), the
+// distinct columns of each block (ucol[ubase[b] .. ubase[b+1]), sorted) and every nonzero's
+// 16-bit position in its block's list.
+struct RowBlocks {
+  int nnzb = 0;  // 0: not built
+  std::vector<int32_t> rblk;
+  DBuf<int32_t> d_rblk, d_ubase, d_ucol;
+  DBuf<uint16_t> d_lcol;
+  int64_t nrblk = 0, nucol = 0;
+};
+
 struct Matrix {
   Space *space;
   Ctx *ctx;
@@ -202,11 +213,9 @@ struct Matrix {
   DBuf<int8_t> d_dg_pos;             // nel*(dim+2): pos of diag block, then of each facet neighbour (-1 none)
   DBuf<int32_t> d_dg_bcol;           // nel*4 block columns (-1 padded), sorted
   int dg_nb = 0;                     // local dofs per element (block size)
-  // SpMV launch parameters
-  // LDS-staged SpMV: row blocks of at most SPMV_NNZB nonzeros (block b = rows [rblk[b], rblk[b+1]))
-  std::vector<int32_t> rblk;
-  DBuf<int32_t> d_rblk;
-  int64_t nrblk = 0;
+  // LDS-staged SpMV row blocks: FP64 values use blocks of SPMV_NNZB64 nonzeros, FP32 values
+  // (built with the FP32 copy) blocks of SPMV_NNZB32
+  RowBlocks rb64, rb32;
   bool assembled = false;
   // mixed precision copy
   DBuf<float> d_val32;
@@ -288,9 +297,13 @@ double unet_train_step(Unet *U, Matrix *A, Precond *P, const double *samples, in
 void build_h1_schedule(Matrix *A);
 void build_colouring(Space &s);
 void build_pattern(Matrix *A);
-// rows grouped into blocks of at most SPMV_NNZB nonzeros for the LDS-staged SpMV
-constexpr int SPMV_NNZB = 2048;
-void build_row_blocks(Matrix *A);
+// nonzeros per row block of the LDS-staged SpMV (LDS: 16 KB per block in both precisions;
+// measured on H1 P3, 7.2M dofs: FP64 1.01 ms at 2048 against 1.04 at 3072 and 1.20 at 4096,
+// FP32 0.69 ms at 4096 against 0.83 at 2048 and 0.85 at 8192)
+constexpr int SPMV_NNZB64 = 2048;
+constexpr int SPMV_NNZB32 = 4096;
+void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb);
+void ensure_row_blocks32(Matrix *A);
 
 }  // namespace ocmp
 

@@ -126,16 +126,17 @@ __global__ void k_reduce(const double *__restrict__ part, int nparts, double *__
 // ------------------------------------------------------------------ CSR SpMV
 
 // LDS-staged CSR SpMV. Block b owns rows [rblk[b], rblk[b+1]) whose nonzeros form one
-// contiguous range of at most SPMV_NNZB entries: every thread loads 8 (col, val) pairs with
-// fully coalesced loads, gathers x, and stages the products in LDS; rows are then summed from
-// LDS by 4 lanes each. Two dependent memory levels (col -> x) instead of three
-// (row_ptr -> col -> x), and 8 independent loads in flight per thread.
+// contiguous range of at most NNZB entries. The block first gathers x at its distinct
+// columns (ucol, sorted: neighbouring lanes read neighbouring entries) into LDS; every thread
+// then loads NNZB/256 (16-bit local column, value) pairs with fully coalesced loads and stages the
+// products in LDS; rows are summed from LDS by 4 lanes each. The rows of a finite-element block
+// share most of their columns, so the x gathers (the texture-unit bound part of a plain CSR
+// SpMV) drop several-fold, and the 16-bit local columns save 2 bytes per nonzero.
 //
 // mode 0: y = A x
 // mode 1: y = mask*(b - A x), partial sum y^2
 // mode 2: y = mask*(A x), partial sum x.y                (CG: q = A p, p.q)
 // mode 3: y = mask*(A (cs*sx*x))   cs: column scaling (right Jacobi), sx: device scalar  (GMRES on H1)
-constexpr int SPMV_PER_THREAD = SPMV_NNZB / NTHR;
 
 template <typename VT, typename XT>
 __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT *__restrict__ y,
@@ -156,16 +157,25 @@ __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT 
   }
 }
 
-template <typename VT, typename XT>
+template <typename VT, typename XT, int NNZB>
 __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t *__restrict__ rblk,
                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                   const int32_t *__restrict__ ubase,
+                                                   const int32_t *__restrict__ ucol,
+                                                   const uint16_t *__restrict__ lcol,
                                                    const VT *__restrict__ val, const XT *__restrict__ x,
                                                    XT *__restrict__ y, const double *__restrict__ b,
                                                    const uint8_t *__restrict__ mask, const XT *__restrict__ cs,
                                                    const double *__restrict__ sx, int mode,
                                                    double *__restrict__ part) {
-  __shared__ double prod[SPMV_NNZB];
-  __shared__ double sh[NTHR / 64];
+  // one LDS buffer: first the x values at the distinct columns, then (after every thread has
+  // read its operands) the products, then the block-sum scratch, so the kernel's LDS is
+  // exactly NNZB values (a power of two: whole blocks per CU)
+  constexpr int SPMV_PER_THREAD = NNZB / NTHR;
+  __shared__ double lds[NNZB * sizeof(XT) / sizeof(double)];
+  XT *xs = reinterpret_cast<XT *>(lds);
+  XT *prod = xs;
+  double *sh = lds;
   const int lb = xcd_block();
   double acc = 0.0;
   const XT scale = (mode == 3 && sx) ? (XT)sx[0] : (XT)1;
@@ -174,7 +184,7 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
     const int32_t j0 = rp[r0], j1 = rp[r1];
     const int nz = j1 - j0;
     const int nrows = r1 - r0;
-    if (nz <= SPMV_NNZB) {
+    if (nz <= NNZB) {
       // rows of the first pass: fetch their bounds early so the loads overlap the staging
       const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
       int a0 = 0, e0 = 0;
@@ -182,26 +192,40 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
         a0 = rp[r0 + ri0] - j0;
         e0 = rp[r0 + ri0 + 1] - j0;
       }
-      int32_t ci[SPMV_PER_THREAD];
+      int li[SPMV_PER_THREAD];
       VT vv[SPMV_PER_THREAD];
 #pragma unroll
       for (int k = 0; k < SPMV_PER_THREAD; ++k) {
         const int t = threadIdx.x + k * NTHR;
-        ci[k] = -1;
+        li[k] = -1;
         vv[k] = (VT)0;
         if (t < nz) {
-          ci[k] = col[j0 + t];
+          li[k] = lcol[j0 + t];
           vv[k] = val[j0 + t];
         }
       }
-#pragma unroll
-      for (int k = 0; k < SPMV_PER_THREAD; ++k) {
-        if (ci[k] >= 0) {
-          XT xv = x[ci[k]];
-          if (mode == 3 && cs) xv *= cs[ci[k]];
-          prod[threadIdx.x + k * NTHR] = (double)((XT)vv[k] * xv);
+      // x at the block's distinct columns (nu >= 1 since nz >= 1; indices past nu are clamped
+      // so both gathers of a round are unconditional)
+      const int u0 = ubase[lb], nu = ubase[lb + 1] - u0;
+      for (int base = 0; base < nu; base += 2 * NTHR) {
+        const int t0 = base + threadIdx.x, t1 = t0 + NTHR;
+        const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
+        XT x0 = x[c0], x1 = x[c1];
+        if (mode == 3 && cs) {
+          x0 *= cs[c0];
+          x1 *= cs[c1];
         }
+        if (t0 < nu) xs[t0] = x0;
+        if (t1 < nu) xs[t1] = x1;
       }
+      __syncthreads();
+      XT pv[SPMV_PER_THREAD];
+#pragma unroll
+      for (int k = 0; k < SPMV_PER_THREAD; ++k) pv[k] = li[k] >= 0 ? (XT)vv[k] * xs[li[k]] : (XT)0;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < SPMV_PER_THREAD; ++k)
+        if (li[k] >= 0) prod[threadIdx.x + k * NTHR] = pv[k];
       __syncthreads();
       for (int base = 0; base < nrows; base += NTHR / 4) {
         const int ri = base + ri0;
@@ -212,7 +236,7 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
         }
         double s = 0.0;
         if (ri < nrows)
-          for (int t = a + q; t < e; t += 4) s += prod[t];
+          for (int t = a + q; t < e; t += 4) s += (double)prod[t];
         s += __shfl_xor(s, 1, 4);
         s += __shfl_xor(s, 2, 4);
         if (ri < nrows && q == 0) spmv_epilogue<VT, XT>(mode, r0 + ri, s, y, x, b, mask, scale, acc);
@@ -231,27 +255,46 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
     }
   }
   if (part) {
+    __syncthreads();  // sh aliases the products
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[lb] = acc;
   }
 }
 
+// FP32 values and vectors use the larger row blocks (same 16 KB of LDS)
+template <typename VT, typename XT>
+static const RowBlocks &row_blocks(const Matrix *A) {
+  return (sizeof(VT) == 4 && sizeof(XT) == 4 && A->rb32.nnzb) ? A->rb32 : A->rb64;
+}
+
+template <typename VT, typename XT>
 static int spmv_grid(Matrix *A) {
-  int64_t g = std::max<int64_t>(A->nrblk, NXCD);
+  int64_t g = std::max<int64_t>(row_blocks<VT, XT>(A).nrblk, NXCD);
   g = (g + NXCD - 1) / NXCD * NXCD;
   return (int)g;
 }
+static int spmv_grid(Matrix *A) { return spmv_grid<double, double>(A); }
 
 template <typename VT, typename XT>
 static void spmv_dispatch(Matrix *A, const VT *val, const XT *x, XT *y, const double *b, const uint8_t *mask,
                           const XT *cs, const double *sx, int mode, double *part, int grid, hipStream_t st) {
-  k_spmv_csr<VT, XT><<<grid, NTHR, 0, st>>>(A->nrblk, A->d_rblk.p, A->d_row_ptr.p, A->d_col.p, val, x, y, b, mask,
-                                            cs, sx, mode, part);
+  const RowBlocks &rb = row_blocks<VT, XT>(A);
+  OCMP_REQUIRE((grid == spmv_grid<VT, XT>(A)), "SpMV grid does not match the row blocks");
+  if (rb.nnzb == SPMV_NNZB32)
+    k_spmv_csr<VT, XT, SPMV_NNZB32><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
+                                                          rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
+                                                          mask, cs, sx, mode, part);
+  else
+    k_spmv_csr<VT, XT, SPMV_NNZB64><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
+                                                          rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
+                                                          mask, cs, sx, mode, part);
 }
 
-// algorithmic bytes of one CSR SpMV: values + column indices + row pointers + x once + y once
+// algorithmic bytes of one CSR SpMV: values + 16-bit local columns + the blocks' distinct
+// columns + row pointers + x once + y once
 static double csr_bytes(const Matrix *A, int wv, int wx) {
-  return (double)A->nnz * (wv + 4) + 4.0 * (A->n + 1) + 2.0 * A->n * wx;
+  const RowBlocks &rb = (wv == 4 && wx == 4 && A->rb32.nnzb) ? A->rb32 : A->rb64;
+  return (double)A->nnz * (wv + 2) + 4.0 * rb.nucol + 4.0 * (A->n + 1) + 2.0 * A->n * wx;
 }
 
 void spmv_f64(Matrix *A, const double *x, double *y) {
@@ -1001,6 +1044,7 @@ void precond_update(Precond *P) {
     if (mixed) {
       A->d_val32.alloc(A->nnz);
       k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+      ensure_row_blocks32(A);
     }
   } else {
     const Mesh &m = *A->space->mesh;
@@ -1045,7 +1089,7 @@ static int cg_inner(Matrix *A, const VT *val, const T *dinv, T *x, T *r, T *z, T
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = A->n;
-  const int sgrid = spmv_grid(A);
+  const int sgrid = spmv_grid<VT, T>(A);
   ensure_part(w, (size_t)sgrid + SEG_BLOCKS + 64);
   const int vgrid = red_grid(n);
   double *sc = ctx->scal.p;  // [0],[1]: rz ring, [2]: pq
@@ -1266,7 +1310,7 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
             vals = A->d_val32.p;
             cs = (P && P->kind == OCMP_PRE_JACOBI) ? P->dinv32.p : nullptr;
           }
-          spmv_dispatch<T, T>(A, vals, vj, wv, nullptr, mask, cs, vs + jj, 3, nullptr, spmv_grid(A), st);
+          spmv_dispatch<T, T>(A, vals, vj, wv, nullptr, mask, cs, vs + jj, 3, nullptr, spmv_grid<T, T>(A), st);
         }
       }
       {
@@ -1958,7 +2002,7 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
         KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
-        spmv_dispatch<T, T>(A, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid(A), st);
+        spmv_dispatch<T, T>(A, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid<T, T>(A), st);
       }
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
@@ -2080,6 +2124,7 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     // plain GMRES in FP32 needs an FP32 copy of A
     A->d_val32.alloc(A->nnz);
     k_to_f32<<<nblk(A->nnz), NTHR, 0, A->ctx->stream>>>(A->nnz, A->d_val.p, A->d_val32.p);
+    ensure_row_blocks32(A);
   }
   const double bn = norm_masked(A, b, w);
   double rn = residual64(A, P, b, x, w.r64.p, w);
