@@ -31,7 +31,8 @@ enum {
   OCMP_ERR_HIP = -2,         /* HIP runtime failure                         */
   OCMP_ERR_STATE = -3,       /* call sequence error (e.g. not assembled)    */
   OCMP_ERR_UNSUPPORTED = -4, /* valid request outside the implemented set   */
-  OCMP_ERR_INTERNAL = -5
+  OCMP_ERR_INTERNAL = -5,
+  OCMP_ERR_NUMERIC = -6      /* an iteration did not converge               */
 };
 
 typedef struct ocmp_ctx ocmp_ctx;
@@ -292,6 +293,19 @@ typedef struct {
  * also when not converged; check stats->converged. */
 int ocmp_solve(ocmp_matrix *A, ocmp_precond *P, const double *b, double *x,
                const ocmp_solve_opts *opts, ocmp_solve_stats *stats);
+
+/* s-step GMRES basis shifts. No counterpart in the reference (NGSolve's GMRes
+ * has no s-step variant); ocmp_solve uses them internally, they are exported
+ * so that callers and tests can inspect them. Host-only, no device calls.
+ *
+ * Eigenvalues (wr + i wi) of the n x n upper Hessenberg matrix H, column-major
+ * with leading dimension ldh (the Ritz values of a GMRES cycle): Francis
+ * double-shift QR. Returns OCMP_ERR_NUMERIC if it does not converge. */
+int ocmp_hessenberg_eigvals(int32_t n, const double *H, int32_t ldh, double *wr, double *wi);
+/* s Newton-basis shifts from n Ritz values in modified Leja order, complex
+ * conjugates adjacent: basis step t is v_{t+1} = (A - d[t] I) v_t + c[t] v_{t-1},
+ * with c[t] = (Im z)^2 on the second member of a pair and 0 otherwise. */
+int ocmp_newton_shifts(int32_t n, const double *wr, const double *wi, int32_t s, double *d, double *c);
 
 /* ----------------------------------------------------- learned initial guess */
 

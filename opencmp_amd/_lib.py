@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libocmp_mi355x.so')
 
 OCMP_OK = 0
+OCMP_ERR_ARG, OCMP_ERR_HIP, OCMP_ERR_STATE, OCMP_ERR_UNSUPPORTED, OCMP_ERR_INTERNAL, OCMP_ERR_NUMERIC = -1, -2, -3, -4, -5, -6
 OCMP_H1, OCMP_L2 = 0, 1
 BFI_GRADGRAD, BFI_MASS, BFI_CONV, BFI_DTRIAL, BFI_DTEST, BFI_DG_FACET = 0, 1, 2, 3, 4, 16
 WEIGHT_NONE, WEIGHT_FIELD, WEIGHT_ONE_MINUS_FIELD = 0, 1, 2
@@ -98,6 +99,10 @@ SIGNATURES = {
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
+    'ocmp_hessenberg_eigvals': [C.c_int32, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)],
+    'ocmp_newton_shifts': [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32,
+                           C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_mlp_create': [P, P, C.c_int64, C.POINTER(P)],
     'ocmp_mlp_destroy': [P],
     'ocmp_mlp_get_params': [P, P],

@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libocmp_mi355x.so')
 
-SOURCES = ['api.cpp', 'tables.cpp', 'assemble.hip', 'linalg.hip', 'mlp.hip', 'unet.hip']
+SOURCES = ['api.cpp', 'tables.cpp', 'ritz.cpp', 'assemble.hip', 'linalg.hip', 'mlp.hip', 'unet.hip']
 
 # Packed-FP32 instructions are disabled for all device code: hipcc would
 # otherwise form v_pk_* FP32 ops from scalar float code.

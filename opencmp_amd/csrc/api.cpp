@@ -891,6 +891,22 @@ int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double
   });
 }
 
+int ocmp_hessenberg_eigvals(int32_t n, const double *H, int32_t ldh, double *wr, double *wi) {
+  return guard([&] {
+    OCMP_REQUIRE(H && wr && wi, "NULL argument");
+    OCMP_REQUIRE(n >= 1 && ldh >= n, "need n >= 1 and ldh >= n");
+    if (!hessenberg_eigvals(n, H, ldh, wr, wi)) throw Error(OCMP_ERR_NUMERIC, "Hessenberg QR did not converge");
+  });
+}
+
+int ocmp_newton_shifts(int32_t n, const double *wr, const double *wi, int32_t s, double *d, double *c) {
+  return guard([&] {
+    OCMP_REQUIRE(wr && wi && d && c, "NULL argument");
+    OCMP_REQUIRE(n >= 0 && s >= 1, "need n >= 0 and s >= 1");
+    newton_shifts(n, wr, wi, s, d, c);
+  });
+}
+
 int ocmp_residual(ocmp_matrix *A, const double *b, const double *x, double *r, double *norm) {
   return guard([&] {
     OCMP_REQUIRE(A && b && x && r && norm, "NULL argument");

@@ -253,6 +253,8 @@ void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
 void residual_norms(Matrix *A, const double *b, const double *x, double *out);
+bool hessenberg_eigvals(int n, const double *H, int ldh, double *wr, double *wi);
+void newton_shifts(int n, const double *wr, const double *wi, int s, double *d, double *c);
 double residual_vector(Matrix *A, const double *b, const double *x, double *r);
 void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);

@@ -320,8 +320,10 @@ template <typename T, int NB>
 __global__ void __launch_bounds__(NTHR) k_dg_ahat_spmv(int64_t nel, const int32_t *__restrict__ nbr,
                                                        const T *__restrict__ ahat, const T *__restrict__ x,
                                                        const double *__restrict__ sx, T *__restrict__ y,
-                                                       T dcoef = (T)1) {
-  // y = sx * ((Ahat - (1 - dcoef) I) x): dcoef = 1 is Ahat itself, dcoef = 0 its off-diagonal part
+                                                       T dcoef = (T)1, const T *__restrict__ xp = nullptr,
+                                                       T pc = (T)0) {
+  // y = sx * ((Ahat - (1 - dcoef) I) x + pc xp): dcoef = 1 is Ahat itself, dcoef = 0 its
+  // off-diagonal part; xp (may be null) is the previous basis vector of a Newton-basis step
   constexpr int VW = Vec<T>::w;
   const int64_t n = nel * NB;
   const int64_t r0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VW;
@@ -331,6 +333,12 @@ __global__ void __launch_bounds__(NTHR) k_dg_ahat_spmv(int64_t nel, const int32_
   vload(x + r0, s);
 #pragma unroll
   for (int l = 0; l < VW; ++l) s[l] *= dcoef;
+  if (xp) {
+    T pv[VW];
+    vload(xp + r0, pv);
+#pragma unroll
+    for (int l = 0; l < VW; ++l) s[l] += pc * pv[l];
+  }
   int64_t cb[VW][3];
 #pragma unroll
   for (int l = 0; l < VW; ++l) {
@@ -1392,6 +1400,16 @@ constexpr int CA_KQ_MAX = 16;
 constexpr double CA_SIGMA = 1.0;
 constexpr int CA_NG = CA_S * (CA_S + 1) / 2;
 
+// Basis of one block: v_{t+1} = (Ahat - d[t] I) v_t + c[t] v_{t-1}, so that
+// Ahat v_t = v_{t+1} + d[t] v_t - c[t] v_{t-1}. The first cycle of a solve uses the shifted
+// monomial basis (d = CA_SIGMA, c = 0); later cycles use Leja-ordered Ritz values of the first
+// cycle (a Newton basis), which keeps the block well conditioned for FP32 storage. d and c are
+// held as the basis precision applies them.
+struct CaShift {
+  double d[CA_S];
+  double c[CA_S];
+};
+
 // part[(q*S + i) * grid + lb] = partial of Q[q] . W[i],  q < nq
 // Rows past nq in the last y-tile read row nq-1 again (no branch around the loads, so all of a
 // thread's row loads are in flight together) and their sums are dropped.
@@ -1673,12 +1691,12 @@ __global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R,
     }
 }
 
-// New Hessenberg columns k..k+S-1 from the change of basis (shifted monomial basis:
-// Ahat u_i = u_{i+1} + sigma u_i), then their Givens rotations.
+// New Hessenberg columns k..k+S-1 from the change of basis (Ahat v_t = v_{t+1} + d_t v_t -
+// c_t v_{t-1}), then their Givens rotations.
 // Hf: unrotated Hessenberg, Hr: rotated (both column-major, ld = m+1). C: (k+1) x S q-major.
 __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__restrict__ C,
                                                  const double *__restrict__ R1, const double *__restrict__ R2,
-                                                 double *__restrict__ Hf, GState G) {
+                                                 double *__restrict__ Hf, GState G, CaShift B) {
   constexpr int MAXR = 256 / CA_S + CA_S + 1;
   __shared__ double Rr[MAXR * (CA_S + 1)];          // (k+1+S) x (S+1), column-major
   __shared__ double HC[(256 / CA_S + 1) * CA_S];    // H_old (k+1 x k) @ Rr[0:k, 0:S]
@@ -1716,13 +1734,14 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
     HC[c * (k + 1) + r] = v;
   }
   __syncthreads();
-  // Hnew = (Rr[:,1:S+1] + sigma Rr[:,0:S] - [HC; 0]) T^{-1},  T = Rr[k:k+S, 0:S] upper triangular;
-  // one thread per row r solves the row system x T = rhs
+  // Hnew = (Rr Bc - [HC; 0]) T^{-1},  T = Rr[k:k+S, 0:S] upper triangular, Bc the change of basis
+  // (column t: 1 at row t+1, d[t] at row t, -c[t] at row t-1); one thread per row r solves x T = rhs
   for (int r = tid; r < nr; r += blockDim.x) {
     double x[CA_S];
 #pragma unroll
     for (int c = 0; c < CA_S; ++c) {
-      double v = Rr[(c + 1) * nr + r] + CA_SIGMA * Rr[c * nr + r] - (r <= k ? HC[c * (k + 1) + r] : 0.0);
+      double v = Rr[(c + 1) * nr + r] + B.d[c] * Rr[c * nr + r] - (r <= k ? HC[c * (k + 1) + r] : 0.0);
+      if (c > 0) v -= B.c[c] * Rr[(c - 1) * nr + r];
 #pragma unroll
       for (int t = 0; t < c; ++t) v -= x[t] * Rr[c * nr + (k + t)];
       x[c] = v / Rr[c * nr + (k + c)];
@@ -1789,7 +1808,7 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
 // gmres_cycle. Returns -1 (and changes nothing in x) if the first block breaks down.
 template <typename T>
 static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, double *x, int m, double stop_abs,
-                          int maxit, Work &w, bool &breakdown) {
+                          int maxit, Work &w, bool &breakdown, CaShift &shift, bool &newton) {
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = A->n;
@@ -1840,17 +1859,20 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   const int limit = std::min(m, std::max(CA_S, (maxit + CA_S - 1) / CA_S * CA_S)) / CA_S * CA_S;
   int k = 0, bidx = 0, k_used = -1;
   breakdown = false;
-  const T dc = (T)(1.0 - CA_SIGMA);
   auto enqueue = [&]() {
     for (int i = 0; i < CA_S; ++i) {
-      KTimer tm(ctx, c_op, (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * nel + 2.0 * n * sizeof(T));
+      // v_{i+1} = (Ahat - d_i I) v_i + c_i v_{i-1}: the diagonal blocks of Ahat are the identity,
+      // so the kernel's diagonal coefficient is 1 - d_i (c_0 = 0 always)
+      const T dc = (T)(1.0 - shift.d[i]), pc = (T)shift.c[i];
+      const T *xp = (i > 0 && shift.c[i] != 0.0) ? Q + (int64_t)(k + i - 1) * ldv : nullptr;
+      KTimer tm(ctx, c_op, (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * nel + (xp ? 3.0 : 2.0) * n * sizeof(T));
       const T *xin = Q + (int64_t)(k + i) * ldv;
       T *yout = Q + (int64_t)(k + i + 1) * ldv;
       switch (A->dg_nb) {
-        case 1: k_dg_ahat_spmv<T, 1><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
-        case 3: k_dg_ahat_spmv<T, 3><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
-        case 6: k_dg_ahat_spmv<T, 6><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
-        case 10: k_dg_ahat_spmv<T, 10><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc); break;
+        case 1: k_dg_ahat_spmv<T, 1><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
+        case 3: k_dg_ahat_spmv<T, 3><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
+        case 6: k_dg_ahat_spmv<T, 6><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
+        case 10: k_dg_ahat_spmv<T, 10><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
         default: throw Error(OCMP_ERR_UNSUPPORTED, "DG block size");
       }
     }
@@ -1893,7 +1915,7 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
         k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, nullptr);
       }
     }
-    k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G);
+    k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G, shift);
     OCMP_HIP(hipMemcpyAsync(hres + k, res + k, CA_S * sizeof(double), hipMemcpyDeviceToHost, st));
     OCMP_HIP(hipMemcpyAsync(hres + m + 8 + (bidx & 1), flag, sizeof(double), hipMemcpyDeviceToHost, st));
     const int ev = bidx & 1;
@@ -1925,6 +1947,22 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   // drain the look-ahead before the buffers are reused
   OCMP_HIP(hipStreamSynchronize(st));
   if (k_used <= 0) return breakdown ? -1 : 0;
+  if (!newton && !breakdown && k_used >= 2 * CA_S) {
+    // Newton-basis shifts for the later cycles of this solve: Leja-ordered Ritz values of this
+    // cycle's (unrotated) Hessenberg matrix, rounded to what the basis precision applies
+    std::vector<double> Hh((size_t)(m + 1) * k_used), wr(k_used), wi(k_used);
+    OCMP_HIP(hipMemcpy(Hh.data(), Hf, Hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (hessenberg_eigvals(k_used, Hh.data(), m + 1, wr.data(), wi.data())) {
+      CaShift sh;
+      newton_shifts(k_used, wr.data(), wi.data(), CA_S, sh.d, sh.c);
+      for (int i = 0; i < CA_S; ++i) {
+        sh.d[i] = 1.0 - (double)(T)(1.0 - sh.d[i]);
+        sh.c[i] = (double)(T)sh.c[i];
+      }
+      shift = sh;
+      newton = true;
+    }
+  }
   k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, Hr, g, nullptr, coef);
   w.u64.alloc(n);
   k_combine<T><<<nblk(n), NTHR, 0, st>>>(n, Q, ldv, k_used, coef, beta, w.u64.p);
@@ -2139,6 +2177,13 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
   // s-step cycles on the DG block-Jacobi operator when asked for and the restart fits the block size
   bool ca = o.s_step == CA_S && A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && m % CA_S == 0 && m <= 50;
+  // the first s-step cycle uses the shifted monomial basis; it leaves Newton shifts for the rest
+  CaShift ca_shift;
+  for (int i = 0; i < CA_S; ++i) {
+    ca_shift.d[i] = CA_SIGMA;
+    ca_shift.c[i] = 0.0;
+  }
+  bool newton = false;
   while (rn > o.tol * bref && total < o.maxit && std::isfinite(rn)) {
     ++outer;
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
@@ -2146,9 +2191,10 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     if (ca) {
       bool broke = false;
       if (mixed)
-        used = ca_gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke);
+        used = ca_gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke, ca_shift, newton);
       else
-        used = ca_gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke);
+        used = ca_gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke, ca_shift,
+                                      newton);
       if (broke) ca = false, st_.fallback = 1;  // CholQR lost definiteness: standard cycles from here on
     }
     if (used < 0) {
