@@ -243,11 +243,12 @@ def bench_learned(args, ctx):
                 f'({time.perf_counter() - t0:.1f}s)')
     torch.cuda.synchronize()
     t_record = time.perf_counter() - t0
-    net = MLPGuess(ctx, seed=0, max_batch=65536)
+    # the network predicts the velocity blocks' correction to the linear extrapolation 2 u^n - u^{n-1};
+    # the pressure keeps the plain guess
+    net = MLPGuess(ctx, seed=0, max_batch=65536, base=1)
     t0 = time.perf_counter()
-    # the network predicts the velocity blocks; the pressure keeps the plain guess
     seg = m0.fes.block_offsets()[:len(m0.fes.comp_blocks[0]) + 1]
-    losses = net.train(pol.snapshots(), seg, steps=args.train_steps, batch=65536, lr=2e-3,
+    losses = net.train(pol.snapshots(), seg, steps=args.train_steps, batch=65536, lr=5e-4,
                        seed=1, log_every=max(1, args.train_steps // 10))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
@@ -427,7 +428,7 @@ def main():
     ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4, 5))
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
-    ap.add_argument('--train-steps', type=int, default=400, help='config 4: Adam steps')
+    ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
     ap.add_argument('--n', type=int, default=None, help='mesh cells per direction (defaults: 32 / 512 / 64)')
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=50)

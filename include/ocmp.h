@@ -318,9 +318,9 @@ int ocmp_newton_shifts(int32_t n, const double *wr, const double *wi, int32_t s,
  * Parameters (float32, this order): W1[128][4], b1[128], W2[128][128], b2[128],
  * W3[128][128], b3[128], W4[128], b4[1].
  * Features of DOF i in segment k (a segment = the DOFs of one scalar block):
- *   x = ((u0-u1)/s_k, (u1-u2)/s_k, (u2-u3)/s_k, u0/U_k),  guess = u0 + s_k * f(x)
+ *   x = ((u0-u1)/s_k, (u1-u2)/s_k, (u2-u3)/s_k, u0/U_k),  guess = u0 + base (u0-u1) + s_k * f(x)
  * with u0 the newest state. Training minimises 0.5/B sum (f(x) - t)^2 with
- * t = (u_next - u0)/s_k by Adam (beta 0.9/0.999, eps 1e-8).
+ * t = (u_next - u0 - base (u0 - u1))/s_k by Adam (beta 0.9/0.999, eps 1e-8); base 0 unless set.
  * X: device float32[n][4] (16-byte aligned), T/Y: device float32[n]. */
 #define OCMP_MLP_HIDDEN 128
 #define OCMP_MLP_NPARAM 33793
@@ -340,7 +340,7 @@ int ocmp_mlp_get_scales(ocmp_mlp *mlp, int *nseg, int64_t *seg_off, double *s, d
  * snap[j*ld + i] (device f64, j < nsnap); a segment that never moves gets 1 */
 int ocmp_mlp_fit_scales(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int nseg,
                         const int64_t *seg_off);
-/* batch random samples (snapshot j in [3, nsnap-2], dof i < n) of (X, T), seeded */
+/* batch random samples (snapshot j in [3, nsnap-2], dof i < n) of (X, T), seeded; T follows the base */
 int ocmp_mlp_sample(ocmp_mlp *mlp, const double *snap, int nsnap, int64_t ld, int64_t n, uint64_t seed,
                     int64_t batch, float *X, float *T);
 /* Y = f(X) */
@@ -349,7 +349,13 @@ int ocmp_mlp_forward(ocmp_mlp *mlp, int64_t n, const float *X, float *Y);
 int ocmp_mlp_grad(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float *grad, double *loss);
 /* one Adam step on (X, T); loss (host, may be NULL: then nothing synchronises) is the pre-step loss */
 int ocmp_mlp_train_step(ocmp_mlp *mlp, int64_t n, const float *X, const float *T, float lr, double *loss);
-/* out = u0 + s * f(features(u0, u1, u2, u3)) over n dofs (device f64); out may alias none of u0..u3 */
+/* What the network predicts the increment from: base 0 (default) the newest state u0, base 1 the
+ * linear extrapolation 2 u0 - u1. With base 1 training targets are (u_next - 2 u0 + u1)/s_k and the
+ * guess is 2 u0 - u1 + s_k f(x): an untrained (near-zero) network already extrapolates. */
+int ocmp_mlp_set_base(ocmp_mlp *mlp, int base);
+int ocmp_mlp_get_base(ocmp_mlp *mlp, int *base);
+/* out = u0 + base (u0 - u1) + s * f(features(u0, u1, u2, u3)) over n dofs (device f64); out may
+ * alias none of u0..u3 */
 int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1, const double *u2,
                    const double *u3, double *out);
 

@@ -115,6 +115,8 @@ SIGNATURES = {
     'ocmp_mlp_grad': [P, C.c_int64, P, P, P, C.POINTER(C.c_double)],
     'ocmp_mlp_train_step': [P, C.c_int64, P, P, C.c_float, C.POINTER(C.c_double)],
     'ocmp_mlp_guess': [P, C.c_int64, P, P, P, P, P],
+    'ocmp_mlp_set_base': [P, C.c_int],
+    'ocmp_mlp_get_base': [P, C.POINTER(C.c_int)],
     'ocmp_unet_create': [P, C.c_int, P, C.POINTER(P)],
     'ocmp_unet_destroy': [P],
     'ocmp_unet_get_params': [P, P],

@@ -1077,6 +1077,21 @@ int ocmp_mlp_train_step(ocmp_mlp *mlp, int64_t n, const float *X, const float *T
   });
 }
 
+int ocmp_mlp_set_base(ocmp_mlp *mlp, int base) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    mlp_set_base(mlp->M, base);
+  });
+}
+
+int ocmp_mlp_get_base(ocmp_mlp *mlp, int *base) {
+  return guard([&] {
+    MLP_ENTER(mlp);
+    OCMP_REQUIRE(base, "NULL argument");
+    *base = mlp_get_base(mlp->M);
+  });
+}
+
 int ocmp_mlp_guess(ocmp_mlp *mlp, int64_t n, const double *u0, const double *u1, const double *u2, const double *u3,
                    double *out) {
   return guard([&] {

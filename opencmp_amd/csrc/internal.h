@@ -273,6 +273,8 @@ void mlp_sample(Mlp *M, const double *snap, int nsnap, int64_t ld, int64_t n, ui
 void mlp_forward(Mlp *M, int64_t n, const float *X, float *Y);
 void mlp_grad(Mlp *M, int64_t n, const float *X, const float *T, float *grad, double *loss);
 void mlp_train_step(Mlp *M, int64_t n, const float *X, const float *T, float lr, double *loss);
+void mlp_set_base(Mlp *M, int base);
+int mlp_get_base(const Mlp *M);
 void mlp_guess(Mlp *M, int64_t n, const double *u0, const double *u1, const double *u2, const double *u3,
                double *out);
 

@@ -57,6 +57,7 @@ struct SegArgs {
 struct GuessArgs {
   const double *u0, *u1, *u2, *u3;
   double *out;
+  int base;  // 1: the network predicts the correction to the linear extrapolation 2 u0 - u1
 };
 
 __device__ __forceinline__ int seg_of(const SegArgs &sg, int64_t i) {
@@ -110,7 +111,8 @@ __device__ __forceinline__ s4 relu_bf(const f4 v) {
   return o;
 }
 
-// MODE 0: Y = f(X); MODE 1: same and store a1..a3; MODE 2: out = u0 + s * f(features(u0..u3))
+// MODE 0: Y = f(X); MODE 1: same and store a1..a3;
+// MODE 2: out = u0 + base * (u0 - u1) + s * f(features(u0..u3))
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mlp_fwd(const u16 *__restrict__ pb, int64_t n, int64_t npad,
                                                  const float4 *__restrict__ X, float *__restrict__ Y,
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const u16 *__restrict__ pb, int
     const int64_t rb = base + wv * 32;
     s4 act_[8][2];
     float su[2] = {1.f, 1.f};
-    double u0v[2] = {0.0, 0.0};
+    double u0v[2] = {0.0, 0.0}, u1v[2] = {0.0, 0.0};
     // layer 1 (4 -> 128) on VALU, written straight into the B-operand layout
 #pragma unroll
     for (int nt = 0; nt < 2; ++nt) {
@@ -144,8 +146,9 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const u16 *__restrict__ pb, int
         if (MODE == 2) {
           const int k = seg_of(sg, row);
           u0v[nt] = ga.u0[row];
+          u1v[nt] = ga.u1[row];
           su[nt] = sg.s[k];
-          x = features(u0v[nt], ga.u1[row], ga.u2[row], ga.u3[row], sg.s[k], sg.U[k]);
+          x = features(u0v[nt], u1v[nt], ga.u2[row], ga.u3[row], sg.s[k], sg.U[k]);
         } else {
           x = X[row];
         }
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(const u16 *__restrict__ pb, int
       if (g == 0 && row < n) {
         const float y = p + sB4;
         if (MODE == 2)
-          ga.out[row] = u0v[nt] + (double)su[nt] * (double)y;
+          ga.out[row] = u0v[nt] + (double)ga.base * (u0v[nt] - u1v[nt]) + (double)su[nt] * (double)y;
         else
           Y[row] = y;
       }
@@ -438,9 +441,11 @@ __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
   return z ^ (z >> 31);
 }
 
-// random (snapshot j, dof i) samples: X = features of u^j..u^{j-3}, T = (u^{j+1} - u^j) / s
+// random (snapshot j, dof i) samples: X = features of u^j..u^{j-3},
+// T = (u^{j+1} - u^j - base (u^j - u^{j-1})) / s
 __global__ void k_mlp_sample(const double *__restrict__ snap, int nsnap, int64_t ld, int64_t n, SegArgs sg,
-                             uint64_t seed, int batch, float4 *__restrict__ X, float *__restrict__ T) {
+                             uint64_t seed, int batch, int base, float4 *__restrict__ X,
+                             float *__restrict__ T) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= batch) return;
   const uint64_t h1 = splitmix(seed * 0x2545F4914F6CDD1Dull + (uint64_t)b);
@@ -448,10 +453,9 @@ __global__ void k_mlp_sample(const double *__restrict__ snap, int nsnap, int64_t
   const int j = 3 + (int)(h1 % (uint64_t)(nsnap - 4));
   const int64_t i = (int64_t)(h2 % (uint64_t)n);
   const int k = seg_of(sg, i);
-  const double u0 = snap[(size_t)j * ld + i];
-  X[b] = features(u0, snap[(size_t)(j - 1) * ld + i], snap[(size_t)(j - 2) * ld + i],
-                  snap[(size_t)(j - 3) * ld + i], sg.s[k], sg.U[k]);
-  T[b] = (float)((snap[(size_t)(j + 1) * ld + i] - u0) / (double)sg.s[k]);
+  const double u0 = snap[(size_t)j * ld + i], u1 = snap[(size_t)(j - 1) * ld + i];
+  X[b] = features(u0, u1, snap[(size_t)(j - 2) * ld + i], snap[(size_t)(j - 3) * ld + i], sg.s[k], sg.U[k]);
+  T[b] = (float)((snap[(size_t)(j + 1) * ld + i] - u0 - (double)base * (u0 - u1)) / (double)sg.s[k]);
 }
 
 // per (segment, block) sums of squared increments and squared values over the snapshots
@@ -503,6 +507,7 @@ struct Mlp {
   DBuf<double> loss, spart;
   int64_t step = 0;
   SegArgs sg{};
+  int base = 0;  // guess relative to u0 (0) or to the linear extrapolation 2 u0 - u1 (1)
 };
 
 static void mlp_alloc(Mlp *M, int64_t max_batch) {
@@ -626,7 +631,7 @@ void mlp_sample(Mlp *M, const double *snap, int nsnap, int64_t ld, int64_t n, ui
   OCMP_REQUIRE(batch >= 1 && batch <= ((int64_t)1 << 30), "mlp: batch out of range");
   hipStream_t st = M->ctx->stream;
   k_mlp_sample<<<(unsigned)((batch + 255) / 256), 256, 0, st>>>(snap, nsnap, ld, n, M->sg, seed, (int)batch,
-                                                               (float4 *)X, T);
+                                                               M->base, (float4 *)X, T);
   OCMP_HIP(hipGetLastError());
 }
 
@@ -652,9 +657,16 @@ void mlp_guess(Mlp *M, int64_t n, const double *u0, const double *u1, const doub
   Ctx *c = M->ctx;
   KTimer t(c, c->cls("mlp_guess"), (double)n * 40.0 + NP * 2.0);
   k_mlp_fwd<2><<<fwd_grid(npad), 256, 0, c->stream>>>(M->pb.p, n, npad, nullptr, nullptr, nullptr, M->sg,
-                                                       GuessArgs{u0, u1, u2, u3, out});
+                                                       GuessArgs{u0, u1, u2, u3, out, M->base});
   OCMP_HIP(hipGetLastError());
 }
+
+void mlp_set_base(Mlp *M, int base) {
+  OCMP_REQUIRE(base == 0 || base == 1, "mlp: base must be 0 (previous state) or 1 (linear extrapolation)");
+  M->base = base;
+}
+
+int mlp_get_base(const Mlp *M) { return M->base; }
 
 // gradient of the batch loss (and optionally one Adam step); loss (device) written to M->loss
 static void mlp_backprop(Mlp *M, int64_t n, const float *X, const float *T, float *grad, bool update, float lr) {

@@ -51,7 +51,9 @@ class MLPGuess:
     """Device-resident network, its Adam state and the per-segment feature scales."""
 
     def __init__(self, ctx: Optional[Context] = None, params: Optional[np.ndarray] = None, seed: int = 0,
-                 max_batch: int = 65536):
+                 max_batch: int = 65536, base: int = 0):
+        """base 1: the network predicts the correction to the linear extrapolation 2 u0 - u1
+        (training targets and the guess follow, ocmp_mlp_set_base); base 0: to u0."""
         self.ctx = ctx or Context.default()
         self.lib = self.ctx.lib
         self.max_batch = int(max_batch)
@@ -63,6 +65,13 @@ class MLPGuess:
                 'mlp_create')
         self.h = h
         self.losses: List[float] = []
+        L.check(self.lib.ocmp_mlp_set_base(self.h, int(base)), 'mlp_set_base')
+
+    @property
+    def base(self) -> int:
+        b = C.c_int()
+        L.check(self.lib.ocmp_mlp_get_base(self.h, C.byref(b)), 'mlp_get_base')
+        return b.value
 
     def __del__(self):
         h = getattr(self, 'h', None)
@@ -221,25 +230,44 @@ def choose_guess(a, b: torch.Tensor, plain: torch.Tensor, learned: torch.Tensor)
     return 'plain', rp, rl
 
 
+def choose_best(a, b: torch.Tensor, plain: torch.Tensor, candidates) -> Tuple[str, float, float]:
+    """Fallback rule over several candidates: keep the one with the lowest FP64 residual
+    if it beats the plain guess. Copies the winner into `plain` and returns (choice,
+    plain residual, residual of the best candidate)."""
+    rp, _ = residual_norms(a, b, plain)
+    best, rb, vb = 'plain', math.inf, None
+    for name, v in candidates:
+        r, _ = residual_norms(a, b, v)
+        if math.isfinite(r) and r < rb:
+            best, rb, vb = name, r, v
+    if vb is not None and rb < rp:
+        plain.copy_(vb)
+        return best, rp, rb
+    return 'plain', rp, rb
+
+
 class GuessPolicy:
     """The last four accepted states of a transient run (device ring), optional
-    snapshot recording for training, and the trained network that turns the
-    history into the next step's initial guess."""
+    snapshot recording for training, and the initial-guess candidates of the next
+    step: the trained network's prediction and, optionally, the linear extrapolation."""
 
     def __init__(self, ctx: Context, n: int, net: Optional[MLPGuess] = None, max_snapshots: int = 0,
-                 n_learned: Optional[int] = None):
-        """n_learned: the network predicts dofs [0, n_learned) (e.g. the velocity blocks of a
-        saddle-point model); the rest keep the plain guess, the newest state."""
+                 n_learned: Optional[int] = None, extrapolate: bool = False):
+        """n_learned: the network (and the extrapolation) cover dofs [0, n_learned) (e.g. the
+        velocity blocks of a saddle-point model); the rest keep the plain guess, the newest
+        state. extrapolate: also offer 2 u^n - u^{n-1} on those dofs."""
         self.ctx = ctx
         self.n = int(n)
         self.n_learned = self.n if n_learned is None else int(n_learned)
         self.net = net
+        self.extrapolate = bool(extrapolate)
         self.ring = [ctx.zeros(self.n) for _ in range(4)]
         self.count = 0
         self.snap = (torch.empty((max_snapshots, self.n), dtype=torch.float64, device=ctx.torch_device)
                      if max_snapshots else None)
         self.nsnap = 0
         self.out = ctx.empty(self.n)
+        self.out_x = ctx.empty(self.n) if self.extrapolate else None
 
     def push(self, u: torch.Tensor) -> None:
         self.ring[self.count % 4].copy_(u)
@@ -253,15 +281,30 @@ class GuessPolicy:
         return [self.ring[(self.count - 1 - k) % 4] for k in range(4)]
 
     def ready(self) -> bool:
-        return self.net is not None and self.count >= 4
+        return (self.net is not None and self.count >= 4) or (self.extrapolate and self.count >= 2)
 
     def propose(self) -> torch.Tensor:
+        """The network's guess."""
         k = self.n_learned
         h = self.history()
         self.net.guess([v[:k] for v in h], out=self.out[:k])
         if k < self.n:
             self.out[k:].copy_(h[0][k:])
         return self.out
+
+    def candidates(self) -> List[Tuple[str, torch.Tensor]]:
+        """(name, vector) of every guess the policy can offer now."""
+        out = []
+        if self.extrapolate and self.count >= 2:
+            k = self.n_learned
+            h = self.history()
+            torch.add(h[0][:k], h[0][:k] - h[1][:k], out=self.out_x[:k])
+            if k < self.n:
+                self.out_x[k:].copy_(h[0][k:])
+            out.append(('extrapolated', self.out_x))
+        if self.net is not None and self.count >= 4:
+            out.append(('learned', self.propose()))
+        return out
 
     def snapshots(self) -> torch.Tensor:
         return self.snap[:self.nsnap]

@@ -145,7 +145,7 @@ class Model:
                        restart=self.restart, inner_tol=self.inner_tol, s_step=self.s_step)
 
     def solve_single_step(self, a: C.BilinearForm, Lf: C.LinearForm, pre, gfu: C.GridFunction,
-                          reassemble=None) -> C.SolveResult:
+                          reassemble=None, assembled: bool = False) -> C.SolveResult:
         return self.linear_solve(a, Lf, pre, gfu)
 
     # ---------------------------------------------------------------- error norms
@@ -324,16 +324,17 @@ class INS(Stokes):
             terms.append(C.Vol('conv', b, b, coef=-dt, wind_gf=(self.W, ub)))
         return terms
 
-    def solve_single_step(self, a, Lf, pre, gfu, reassemble=None):
+    def solve_single_step(self, a, Lf, pre, gfu, reassemble=None, assembled=False):
         """Picard iteration (opencmp/models/ins.py:323-359): re-assemble with the current wind,
-        solve, stop when ||W - u||_L2 < abs + rel * ||u||_L2."""
+        solve, stop when ||W - u||_L2 < abs + rel * ||u||_L2. assembled: `a` and `pre` already
+        hold the first iteration's operator (the caller assembled it with the current wind)."""
         it = 1
         res = None
         self.linear_iterations = 0
         self.picard_log = []      # (iterations, refinement steps, initial rel residual, Picard error) per iteration
         while True:
             self.apply_dirichlet_bcs_to(gfu)
-            if reassemble is not None:
+            if reassemble is not None and not (assembled and it == 1):
                 reassemble()
             res = self.linear_solve(a, Lf, pre, gfu)
             self.linear_iterations += res.iters

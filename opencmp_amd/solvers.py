@@ -69,13 +69,18 @@ class ImplicitEulerSolver:
     """Fixed-step implicit Euler (time_integration_schemes.py:79-132)."""
 
     def __init__(self, model: Model, dt: float, t_range=(0.0, 1.0), gfu0: Optional[C.GridFunction] = None,
-                 reassemble_each_step: Optional[bool] = None, guess=None):
+                 reassemble_each_step: Optional[bool] = None, guess=None, linearize_at_guess: bool = False):
         """guess: an optional :class:`opencmp_amd.learned.GuessPolicy` that records the
-        solution history (and snapshots for training) and, once it holds a trained
-        network, proposes the initial guess of each step (kept only if it lowers the
-        FP64 residual)."""
+        solution history (and snapshots for training) and proposes initial-guess candidates
+        for each step (one is kept only if it lowers the FP64 residual of the step's first
+        system). For a nonlinear model that first system is linearised about the previous
+        solution, as in the reference, and the kept guess is only the Krylov start vector;
+        linearize_at_guess makes the kept guess also the first linearisation point (the
+        Picard iterate can then start closer to the fixed point, at the risk of a harder
+        first system)."""
         self.model = model
         self.guess = guess
+        self.linearize_at_guess = bool(linearize_at_guess)
         self.dt = dt
         self.t0, self.t1 = t_range
         self.gfu = C.GridFunction(model.fes)
@@ -118,27 +123,37 @@ class ImplicitEulerSolver:
         if self.reassemble_each_step and not self.model.nonlinear:
             self._assemble_a()
         self.model.apply_dirichlet_bcs_to(self.gfu)
+        assembled = False
         if self.guess is not None and self.guess.ready():
-            self._learned_guess()
+            if self.model.nonlinear:
+                # the step's first Picard system, linearised about the previous solution (W holds it)
+                self._assemble_a()
+                assembled = True
+            if self._learned_guess() != 'plain' and self.model.nonlinear and self.linearize_at_guess:
+                self.model.W.vec.copy_(self.gfu.vec)
+                assembled = False
         reassemble = self._assemble_a if self.model.nonlinear else None
-        res = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble)
+        res = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble,
+                                           assembled=assembled)
         self.step_results.append(res)
         self.gfu_prev.vec.copy_(self.gfu.vec)
         if self.guess is not None:
             self.guess.push(self.gfu.vec)
         return res
 
-    def _learned_guess(self):
+    def _learned_guess(self) -> str:
         """Replace the plain initial guess (previous solution + Dirichlet lift) by the
-        network's prediction when that has the lower FP64 residual against the
-        current operator; a nonlinear model also linearises about the chosen guess."""
-        from .learned import choose_guess
-        g = C.GridFunction(self.model.fes, self.guess.propose())
-        self.model.apply_dirichlet_bcs_to(g)
-        choice, rp, rl = choose_guess(self.a, self.L.vec, self.gfu.vec, g.vec)
-        if choice == 'learned' and self.model.nonlinear:
-            self.model.W.vec.copy_(self.gfu.vec)
-        self.guess_log.append((choice, rp, rl))
+        policy's candidate with the lowest FP64 residual against the current operator,
+        if that beats the plain one; returns the choice."""
+        from .learned import choose_best
+        cands = []
+        for name, v in self.guess.candidates():
+            g = C.GridFunction(self.model.fes, v)
+            self.model.apply_dirichlet_bcs_to(g)
+            cands.append((name, g.vec))
+        choice, rp, rb = choose_best(self.a, self.L.vec, self.gfu.vec, cands)
+        self.guess_log.append((choice, rp, rb))
+        return choice
 
     def solve(self, nsteps: Optional[int] = None) -> C.GridFunction:
         if nsteps is None:

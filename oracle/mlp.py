@@ -140,8 +140,8 @@ def scales(snap, seg_off):
     return np.array(s), np.array(U)
 
 
-def guess(p, hist, seg_off, s, U, bf16=True):
-    """u0 + s_k * f(features) per DOF; hist = [u0, u1, u2, u3]."""
+def guess(p, hist, seg_off, s, U, bf16=True, base=0):
+    """u0 + base (u0 - u1) + s_k * f(features) per DOF; hist = [u0, u1, u2, u3]."""
     n = hist[0].size
     sd = np.empty(n)
     Ud = np.empty(n)
@@ -149,4 +149,9 @@ def guess(p, hist, seg_off, s, U, bf16=True):
         sd[seg_off[k]:seg_off[k + 1]] = np.float32(s[k])
         Ud[seg_off[k]:seg_off[k + 1]] = np.float32(U[k])
     X = features(*[np.asarray(h, dtype=np.float64) for h in hist], sd, Ud).astype(np.float32)
-    return hist[0] + sd * forward(p, X, bf16=bf16)
+    return hist[0] + base * (hist[0] - hist[1]) + sd * forward(p, X, bf16=bf16)
+
+
+def target(u_next, u0, u1, s, base=0):
+    """Training target of a sample: (u_next - u0 - base (u0 - u1)) / s."""
+    return (u_next - u0 - base * (u0 - u1)) / s

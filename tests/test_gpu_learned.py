@@ -73,7 +73,8 @@ def test_learned_guess_reduces_work(gpu_ctx):
         for h in reversed(ring[1:]):  # u^{n-3}, u^{n-2}, u^{n-1}; the solver pushes u^n itself
             p2.push(h)
         g0 = type(s.gfu_prev)(mm.fes, state.clone())
-        ss = ImplicitEulerSolver(mm, dt=1e-3, gfu0=g0, guess=p2)
+        # the chosen guess is also the first Picard linearisation point
+        ss = ImplicitEulerSolver(mm, dt=1e-3, gfu0=g0, guess=p2, linearize_at_guess=True)
         mm.W.vec.copy_(state)
         its, pic, perr = 0, 0, []
         for _ in range(8):
@@ -98,3 +99,54 @@ def test_learned_guess_reduces_work(gpu_ctx):
     rel = float(torch.linalg.norm(u_learn - u_plain) / torch.linalg.norm(u_plain))
     say('  rel diff of the final states', rel)
     assert rel < 0.1
+
+
+def test_guess_candidates_keep_the_reference_iteration(gpu_ctx):
+    """Default policy: the linear extrapolation and a network trained on the correction to it
+    compete with the plain guess on the FP64 residual of the step's first Picard system, which
+    stays linearised about the previous solution. The Picard iteration is then the plain one:
+    the same states to the linear tolerance (1e-10 here, so that the solutions of the two runs
+    agree to ~1e-6 on this saddle-point system), at no more linear work."""
+    from opencmp_amd.learned import GuessPolicy, MLPGuess
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    say('guess candidates: 30 recorded steps, train (base 1), 6 compared steps')
+    m = channel_model(gpu_ctx, 24, tol=1e-10)
+    n = m.fes.ndof
+    pol = GuessPolicy(gpu_ctx, n, max_snapshots=40)
+    s = ImplicitEulerSolver(m, dt=1e-3, guess=pol)
+    for _ in range(30):
+        s.step()
+    state = s.gfu_prev.vec.clone()
+    ring = [h.clone() for h in pol.history()]
+    wind = m.W.vec.clone()
+    nvel = len(m.fes.comp_blocks[0])
+    seg = [int(v) for v in m.fes.block_offsets()[:nvel + 1]]
+    net = MLPGuess(gpu_ctx, seed=0, max_batch=16384, base=1)
+    losses = net.train(pol.snapshots(), seg, steps=300, batch=16384, lr=5e-4, seed=1)
+    say('  loss', losses[0], '->', losses[-1])
+    assert np.isfinite(losses).all()
+
+    def run(policy):
+        mm = channel_model(gpu_ctx, 24, tol=1e-10)
+        for h in reversed(ring[1:]):
+            policy.push(h)
+        ss = ImplicitEulerSolver(mm, dt=1e-3, gfu0=type(s.gfu_prev)(mm.fes, state.clone()), guess=policy)
+        mm.W.vec.copy_(wind)
+        its, first = 0, []
+        for _ in range(6):
+            ss.step()
+            its += mm.linear_iterations
+            first.append(mm.picard_log[0][0])
+        return ss.gfu_prev.vec.clone(), its, first, ss.guess_log
+
+    u_p, it_p, first_p, _ = run(GuessPolicy(gpu_ctx, n, n_learned=seg[-1]))
+    u_g, it_g, first_g, log = run(GuessPolicy(gpu_ctx, n, net=net, n_learned=seg[-1], extrapolate=True))
+    say(f'  plain: {it_p} iters (first Picard {first_p}); candidates: {it_g} iters (first Picard {first_g})')
+    say('  choices', [(c, f'{rp:.2e}', f'{rb:.2e}') for c, rp, rb in log])
+    assert len(log) == 6
+    assert all(rb < rp for c, rp, rb in log if c != 'plain')
+    assert sum(c != 'plain' for c, _, _ in log) >= 3
+    rel = float(torch.linalg.norm(u_g - u_p) / torch.linalg.norm(u_p))
+    say('  rel diff of the final states', rel)
+    assert rel < 1e-5
+    assert it_g <= 1.05 * it_p

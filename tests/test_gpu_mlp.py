@@ -138,11 +138,13 @@ def test_scales_match_oracle(gpu_ctx):
     np.testing.assert_allclose(U, Uo, rtol=1e-7)
 
 
-def test_samples_come_from_the_snapshots(gpu_ctx):
+@pytest.mark.parametrize('base', [0, 1])
+def test_samples_come_from_the_snapshots(gpu_ctx, base):
     from opencmp_amd.learned import MLPGuess
     snap = snapshots(n=60, K=9)
     off = [0, 60]
-    net = MLPGuess(gpu_ctx, max_batch=128)
+    net = MLPGuess(gpu_ctx, max_batch=128, base=base)
+    assert net.base == base
     net.fit_scales(dev(snap, torch.float64), off)
     _, s, U = net.scales()
     X, T = net.sample(dev(snap, torch.float64), 60, seed=5, batch=500)
@@ -151,7 +153,7 @@ def test_samples_come_from_the_snapshots(gpu_ctx):
     cand = {}
     for j in range(3, 8):
         f = om.features(snap[j], snap[j - 1], snap[j - 2], snap[j - 3], float(s32), float(U32))
-        t = (snap[j + 1] - snap[j]) / float(s32)
+        t = om.target(snap[j + 1], snap[j], snap[j - 1], float(s32), base)
         for i in range(60):
             cand[(j, i)] = (f[i], t[i])
     keys = list(cand)
@@ -166,19 +168,25 @@ def test_samples_come_from_the_snapshots(gpu_ctx):
     assert seen == {3, 4, 5, 6, 7}     # every admissible snapshot index is drawn
 
 
-def test_guess_matches_oracle(gpu_ctx):
+@pytest.mark.parametrize('base', [0, 1])
+def test_guess_matches_oracle(gpu_ctx, base):
     from opencmp_amd.learned import MLPGuess
     p = rand_params(21)
-    net = MLPGuess(gpu_ctx, params=p, max_batch=128)
+    net = MLPGuess(gpu_ctx, params=p, max_batch=128, base=base)
     snap = snapshots(n=1000, K=6, seed=2)
     off = [0, 400, 1000]
     s, U = [0.02, 0.05], [1.3, 0.7]
     net.set_scales(off, s, U)
     hist = [dev(snap[5 - k], torch.float64) for k in range(4)]
     g = net.guess(hist).cpu().numpy()
-    go = om.guess(p, [snap[5 - k] for k in range(4)], off, s, U, bf16=True)
-    # the increment s*y carries the network's error; u0 is exact
-    assert np.linalg.norm((g - snap[5]) - (go - snap[5])) <= Y_RTOL * np.linalg.norm(go - snap[5]) + 1e-9
+    go = om.guess(p, [snap[5 - k] for k in range(4)], off, s, U, bf16=True, base=base)
+    # the increment s*y carries the network's error; u0 + base (u0 - u1) is exact
+    b0 = snap[5] + base * (snap[5] - snap[4])
+    assert np.linalg.norm((g - b0) - (go - b0)) <= Y_RTOL * np.linalg.norm(go - b0) + 1e-9
+    # a network whose output layer is zero returns the base exactly
+    net.params = np.where(np.arange(len(p)) >= len(p) - 129, 0.0, p).astype(np.float32)   # W4, b4 = 0
+    g0 = net.guess(hist).cpu().numpy()
+    assert np.abs(g0 - b0).max() <= 1e-12 * np.abs(b0).max()
 
 
 def test_fallback_keeps_the_better_guess(gpu_ctx):

@@ -104,3 +104,23 @@ def test_features_and_scales_closed_form():
     # a constant segment gets scale 1
     s2, _ = mlp.scales(np.ones((4, 3)), [0, 3])
     assert s2[0] == 1.0
+
+
+def test_extrapolation_base_closed_forms():
+    """base 1: the target is the scaled second difference and a zero network returns the
+    linear extrapolation 2 u0 - u1; base 0 keeps the first difference and u0."""
+    rng = np.random.default_rng(5)
+    u = [rng.standard_normal(50) for _ in range(5)]    # u[0] newest
+    s = 0.3
+    assert np.allclose(mlp.target(u[0], u[1], u[2], s, 1), (u[0] - 2 * u[1] + u[2]) / s)
+    assert np.allclose(mlp.target(u[0], u[1], u[2], s, 0), (u[0] - u[1]) / s)
+    p = mlp.init_params(0)
+    p[-129:] = 0.0                                     # W4, b4: the output is exactly 0
+    g1 = mlp.guess(p, u[1:5], [0, 50], [s], [1.0], bf16=False, base=1)
+    g0 = mlp.guess(p, u[1:5], [0, 50], [s], [1.0], bf16=False, base=0)
+    assert np.allclose(g1, 2 * u[1] - u[2], rtol=1e-14, atol=1e-14) and np.array_equal(g0, u[1])
+    # a quadratic-in-time field: base 1 leaves the constant second difference as the target
+    t = np.arange(5.0)
+    x = rng.standard_normal(50)
+    q = [x * (1 + 0.5 * tt + 0.25 * tt ** 2) for tt in t[::-1]]   # q[0] at the latest time
+    assert np.allclose(mlp.target(q[0], q[1], q[2], 1.0, 1), 0.5 * x)
