@@ -906,6 +906,7 @@ void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y) 
 struct Work {
   // scratch owned by the matrix' solver state; grown on demand
   DBuf<double> r64, u64, t64;
+  DBuf<double> xbak;  // x before an s-step cycle (restored if the cycle does not lower the residual)
   DBuf<double> vd;   // FP64 work vectors (n * k)
   DBuf<float> vf;    // FP32 work vectors
   DBuf<double> part; // partial sums
@@ -1399,6 +1400,9 @@ constexpr int CA_KQ_MAX = 16;
 // at condition ~1e4 instead of ~1e7 for plain powers (small enough for FP32 storage).
 constexpr double CA_SIGMA = 1.0;
 constexpr int CA_NG = CA_S * (CA_S + 1) / 2;
+// largest restart of an s-step cycle: the block kernels size their LDS for k + 1 + CA_S <= CA_MMAX + 1
+// basis rows (the host only takes the s-step path for m <= CA_MMAX)
+constexpr int CA_MMAX = 50;
 
 // Basis of one block: v_{t+1} = (Ahat - d[t] I) v_t + c[t] v_{t-1}, so that
 // Ahat v_t = v_{t+1} + d[t] v_t - c[t] v_{t-1}. The first cycle of a solve uses the shifted
@@ -1477,7 +1481,7 @@ __global__ void __launch_bounds__(NTHR) k_bupdate_gram(int64_t n, const T *__res
                                                        T *__restrict__ W, const double *__restrict__ C,
                                                        double *__restrict__ part) {
   constexpr int VW = Vec<T>::w;
-  __shared__ double cq[(256 / CA_S) * CA_S];
+  __shared__ double cq[(CA_MMAX + 1) * CA_S];
   __shared__ double sh[NTHR / 64][CA_NG];
   for (int e = threadIdx.x; e < nq * CA_S; e += blockDim.x) cq[e] = C[e];
   __syncthreads();
@@ -1697,9 +1701,9 @@ __global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R,
 __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__restrict__ C,
                                                  const double *__restrict__ R1, const double *__restrict__ R2,
                                                  double *__restrict__ Hf, GState G, CaShift B) {
-  constexpr int MAXR = 256 / CA_S + CA_S + 1;
+  constexpr int MAXR = CA_MMAX + 1;  // rows of the basis: k + 1 + CA_S <= m + 1
   __shared__ double Rr[MAXR * (CA_S + 1)];          // (k+1+S) x (S+1), column-major
-  __shared__ double HC[(256 / CA_S + 1) * CA_S];    // H_old (k+1 x k) @ Rr[0:k, 0:S]
+  __shared__ double HC[(CA_MMAX + 1) * CA_S];       // H_old (k+1 x k) @ Rr[0:k, 0:S]
   __shared__ double Rt[CA_S * CA_S];
   __shared__ double Hn[MAXR * CA_S];
   __shared__ double rot[2 * MAXR];                  // the k rotations of the earlier columns
@@ -2176,7 +2180,7 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
   int total = 0, outer = 0;
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
   // s-step cycles on the DG block-Jacobi operator when asked for and the restart fits the block size
-  bool ca = o.s_step == CA_S && A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && m % CA_S == 0 && m <= 50;
+  bool ca = o.s_step == CA_S && A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && m % CA_S == 0 && m <= CA_MMAX;
   // the first s-step cycle uses the shifted monomial basis; it leaves Newton shifts for the rest
   CaShift ca_shift;
   for (int i = 0; i < CA_S; ++i) {
@@ -2188,7 +2192,10 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     ++outer;
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
     int used = -1;
+    const bool ran_ca = ca;
     if (ca) {
+      w.xbak.alloc(A->n);
+      OCMP_HIP(hipMemcpyAsync(w.xbak.p, x, A->n * sizeof(double), hipMemcpyDeviceToDevice, A->ctx->stream));
       bool broke = false;
       if (mixed)
         used = ca_gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, w, broke, ca_shift, newton);
@@ -2206,6 +2213,16 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     total += used;
     const double rn_old = rn;
     rn = residual64(A, P, b, x, w.r64.p, w);
+    if (ran_ca && ca && used > 0 && !(rn < rn_old)) {
+      // the s-step cycle did not lower the FP64 residual (its basis lost too much to rounding):
+      // undo it and continue with standard cycles
+      OCMP_HIP(hipMemcpyAsync(x, w.xbak.p, A->n * sizeof(double), hipMemcpyDeviceToDevice, A->ctx->stream));
+      rn = residual64(A, P, b, x, w.r64.p, w);
+      ca = false;
+      st_.fallback = 1;
+      st_.rel_residual = rn / bref;
+      continue;
+    }
     st_.rel_residual = rn / bref;
     if (used == 0 || (mixed && o.max_outer > 0 && outer >= o.max_outer)) break;
     if (!(rn < rn_old) && used < m) break;  // no progress and the cycle thought it had converged
