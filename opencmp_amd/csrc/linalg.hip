@@ -1698,9 +1698,14 @@ __global__ void k_ca_chol(const double *__restrict__ gp, double *__restrict__ R,
 // New Hessenberg columns k..k+S-1 from the change of basis (Ahat v_t = v_{t+1} + d_t v_t -
 // c_t v_{t-1}), then their Givens rotations.
 // Hf: unrotated Hessenberg, Hr: rotated (both column-major, ld = m+1). C: (k+1) x S q-major.
+// The residual estimates of the new columns and the CholQR breakdown flag also go straight to the
+// host's pinned status buffer hst (res at [j], the flag at [m + 8 + slot]); the host reads them
+// after the event recorded behind this kernel.
 __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__restrict__ C,
                                                  const double *__restrict__ R1, const double *__restrict__ R2,
-                                                 double *__restrict__ Hf, GState G, CaShift B) {
+                                                 double *__restrict__ Hf, GState G, CaShift B,
+                                                 const double *__restrict__ flag, double *__restrict__ hst,
+                                                 int slot) {
   constexpr int MAXR = CA_MMAX + 1;  // rows of the basis: k + 1 + CA_S <= m + 1
   __shared__ double Rr[MAXR * (CA_S + 1)];          // (k+1+S) x (S+1), column-major
   __shared__ double HC[(CA_MMAX + 1) * CA_S];       // H_old (k+1 x k) @ Rr[0:k, 0:S]
@@ -1802,10 +1807,12 @@ __global__ void __launch_bounds__(256) k_ca_hess(int k, int m, const double *__r
     gcur = -sb[c] * gcur;
     G.g[j + 1] = gcur;
     G.res[j] = fabs(gcur) * G.gscale;
+    hst[j] = G.res[j];
     double *Rc = G.R + (int64_t)j * ld + k;
 #pragma unroll
     for (int i = 0; i <= c + 1; ++i) Rc[i] = v[i];
   }
+  hst[m + 8 + slot] = flag[0];
 }
 
 // One CA-GMRES(m) cycle on the DG operator Ahat (block Jacobi folded in); same contract as
@@ -1844,6 +1851,8 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   k_gmres_init<<<1, 64, 0, st>>>(m, 1.0, g, flag, 0.0);
   k_fill_t<double><<<1, 64, 0, st>>>(one, 1, 1.0);
   double *hres = pinned(w, m + 16);
+  double *hres_d = nullptr;  // the same pinned buffer as the device addresses it
+  OCMP_HIP(hipHostGetDevicePointer((void **)&hres_d, hres, 0));
   GState G{nullptr, nullptr, Hr, csn, g, res, m, beta};
   const int c_op = ctx->cls(sizeof(T) == 8 ? "dg_ahat_spmv_f64" : "dg_ahat_spmv_f32");
   const int c_dot = ctx->cls(sizeof(T) == 8 ? "ca_bdot_f64" : "ca_bdot_f32");
@@ -1919,9 +1928,7 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
         k_btrsm_gram<T><<<grid, NTHR, 0, st>>>(n, W, ldv, R1i, nullptr);
       }
     }
-    k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G, shift);
-    OCMP_HIP(hipMemcpyAsync(hres + k, res + k, CA_S * sizeof(double), hipMemcpyDeviceToHost, st));
-    OCMP_HIP(hipMemcpyAsync(hres + m + 8 + (bidx & 1), flag, sizeof(double), hipMemcpyDeviceToHost, st));
+    k_ca_hess<<<1, 256, 0, st>>>(k, m, Cm, R1, R2, Hf, G, shift, flag, hres_d, bidx & 1);
     const int ev = bidx & 1;
     OCMP_HIP(hipEventRecord(work_event(w, ev), st));
     inflight.push_back({k, ev});
