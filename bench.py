@@ -313,7 +313,7 @@ def bench_unet(args, ctx):
     from opencmp_amd.learned import UNetPrecond, residual_samples, residual_norms
     from opencmp_amd.mesh import unit_cube, sphere_obstacles, obstacle_phi
     from opencmp_amd.models import StokesDIM, const
-    n = args.n or 64
+    n = args.n or 32
     t0 = time.perf_counter()
     hm = unit_cube(n)
     phi = obstacle_phi(hm.coords, sphere_obstacles(), 1.0 / n)
@@ -429,9 +429,13 @@ def main():
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
-    ap.add_argument('--n', type=int, default=None, help='mesh cells per direction (defaults: 32 / 512 / 64)')
+    ap.add_argument('--n', type=int, default=None,
+                    help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 32; config 5 at 64^3 does not '
+                         'converge with the diagonal Schur preconditioner)')
     ap.add_argument('--tol', type=float, default=1e-8)
-    ap.add_argument('--restart', type=int, default=50)
+    ap.add_argument('--restart', type=int, default=None,
+                    help='GMRES restart (defaults: 50; 200 for config 5, whose diffuse-interface saddle point '
+                         'stagnates under GMRES(50) and GMRES(100) from 32^3 on)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--inner-tol', type=float, default=None, help='mixed path: relative reduction per inner solve')
     ap.add_argument('--maxit', type=int, default=500000, help='Krylov iteration cap (short profiling runs)')
@@ -444,7 +448,9 @@ def main():
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 64}[args.config]
+    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 32}[args.config]
+    if args.restart is None:
+        args.restart = 200 if args.config == 5 else 50
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)

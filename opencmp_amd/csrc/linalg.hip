@@ -541,22 +541,65 @@ __global__ void k_jacobi_diag(int64_t n, const int32_t *__restrict__ rp, const i
 // Schur rows r (flag[r] = 1): dinv[r] = -1 / (-a_rr + sum_{j not Schur, free} a_rj^2 / a_jj),
 // the diagonal of -(C + B diag(A)^{-1} B^T) for the system [[A, B^T], [B, -C]]. Reads
 // dinv only on non-Schur columns, which k_jacobi_diag has filled.
-__global__ void k_schur_diag(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-                             const double *__restrict__ val, const uint8_t *__restrict__ mask,
-                             const uint8_t *__restrict__ flag, double *__restrict__ dinv,
-                             float *__restrict__ dinv32) {
+// Schur-diagonal rows (flag set): s = -(C_ii) + sum_j B_ij^2 / A_jj over the velocity columns j.
+// Pass 1 stores s in dinv and per-block maxima of s in part; k_schur_finish turns s into -1/s, or 0
+// where s is not above SCHUR_REL times the largest s: pressure dofs that the phi weighting has all
+// but removed (inside solid obstacles) stay out of the Krylov update instead of getting a huge scale.
+constexpr double SCHUR_REL = 1e-10;
+__global__ void __launch_bounds__(NTHR) k_schur_diag(int64_t n, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ col, const double *__restrict__ val,
+                                                     const uint8_t *__restrict__ mask,
+                                                     const uint8_t *__restrict__ flag, double *__restrict__ dinv,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[NTHR / 64];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (r < n && flag[r]) {
+    for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+      const int32_t c = col[j];
+      const double a = val[j];
+      if (c == r)
+        s -= a;
+      else if (!flag[c] && mask[c])
+        s += a * a * dinv[c];
+    }
+    if (!mask[r] || !(s > 0.0)) s = 0.0;
+    dinv[r] = s;
+  }
+  double m = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = sh[0];
+    for (int w = 1; w < NTHR / 64; ++w) bm = fmax(bm, sh[w]);
+    part[blockIdx.x] = bm;
+  }
+}
+
+__global__ void __launch_bounds__(NTHR) k_max_reduce(const double *__restrict__ part, int nparts,
+                                                     double *__restrict__ out) {
+  __shared__ double sh[NTHR / 64];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, part[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = sh[0];
+    for (int w = 1; w < NTHR / 64; ++w) bm = fmax(bm, sh[w]);
+    out[0] = bm;
+  }
+}
+
+__global__ void k_schur_finish(int64_t n, const uint8_t *__restrict__ flag, const double *__restrict__ smax,
+                               double *__restrict__ dinv, float *__restrict__ dinv32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n || !flag[r]) return;
-  double s = 0.0;
-  for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
-    const int32_t c = col[j];
-    const double a = val[j];
-    if (c == r)
-      s -= a;
-    else if (!flag[c] && mask[c])
-      s += a * a * dinv[c];
-  }
-  const double v = !mask[r] || !(s > 0.0) ? 0.0 : -1.0 / s;
+  const double s = dinv[r];
+  const double v = s > SCHUR_REL * smax[0] ? -1.0 / s : 0.0;
   dinv[r] = v;
   if (dinv32) dinv32[r] = (float)v;
 }
@@ -1047,8 +1090,16 @@ void precond_update(Precond *P) {
         P->schur_flag.upload(f, st);
         OCMP_HIP(hipStreamSynchronize(st));
       }
-      k_schur_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_row_ptr.p, A->d_col.p, A->d_val.p, A->space->d_free.p,
-                                                P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+      {
+        Work &w = work(A);
+        const unsigned nb = nblk(A->n);
+        ensure_part(w, (size_t)nb + 8);
+        double *smax = w.part.p + nb;
+        k_schur_diag<<<nb, NTHR, 0, st>>>(A->n, A->d_row_ptr.p, A->d_col.p, A->d_val.p, A->space->d_free.p,
+                                          P->schur_flag.p, P->dinv.p, w.part.p);
+        k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nb, smax);
+        k_schur_finish<<<nb, NTHR, 0, st>>>(A->n, P->schur_flag.p, smax, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+      }
     }
     if (mixed) {
       A->d_val32.alloc(A->nnz);
