@@ -2174,17 +2174,22 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   const bool have_learned = P->unet != nullptr;
   double rate_learned = 0.0, rate_plain = 0.0;  // log residual reduction per iteration (< 0 is progress)
   bool known_l = false, known_p = false, fell_back = false;
+  // the two probe cycles (learned, then plain) are at most FGMRES_PROBE iterations long, so that a
+  // correction that does not help costs little; full cycles follow with the faster one
+  constexpr int FGMRES_PROBE = 40;
   while (rn > o.tol * bref && total < o.maxit && std::isfinite(rn)) {
     ++outer;
     bool use = have_learned && !fell_back;
     if (use && known_l && !known_p) use = false;          // the probe cycle with the plain preconditioner
     if (use && known_l && known_p && rate_plain < rate_learned) use = false, fell_back = true;
+    const bool probe = have_learned && (!known_l || !known_p) && !fell_back;
+    const int lim = probe ? std::min(o.maxit - total, std::min(m, FGMRES_PROBE)) : o.maxit - total;
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
     int used;
     if (mixed)
-      used = fgmres_cycle<float>(A, P, use, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+      used = fgmres_cycle<float>(A, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
     else
-      used = fgmres_cycle<double>(A, P, use, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w);
+      used = fgmres_cycle<double>(A, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
     total += used;
     const double rn_old = rn;
     rn = residual64(A, P, b, x, w.r64.p, w);
@@ -2199,7 +2204,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
       }
     }
     if (used == 0 || (mixed && o.max_outer > 0 && outer >= o.max_outer)) break;
-    if (!(rn < rn_old) && used < m && !use) break;
+    if (!(rn < rn_old) && used < std::min(m, lim) && !use) break;  // no progress, the cycle thought it had converged
   }
   st_.iters = total;
   st_.outer = outer;
