@@ -2,7 +2,7 @@
 
 Only the test-suite (and the smoke/bench checker legs) may import this module.
 
-Network (the learned initial guess of the north star; no reference counterpart):
+Network (the learned initial guess of a transient solve; no reference counterpart):
     x in R^4 -> 128 -> 128 -> 128 -> 1, ReLU between layers,
     y = W4 relu(W3 relu(W2 relu(W1 x + b1) + b2) + b3) + b4
 Loss: mean squared error 0.5/B * sum_s (y_s - t_s)^2.
