@@ -88,7 +88,7 @@ def run_path(C, ctx, a, f, gs, precision, tol, restart, steps, warmup, timing=Fa
     pre = C.Preconditioner(a, 'local', precision=precision)
     x = C.GridFunction(gs)
     res = None
-    inner = (inner_tol or (1e-5 if solver == 'GMRes' else 1e-6)) if precision == 'mixed' else 0.0
+    inner = (inner_tol or 1e-5) if precision == 'mixed' else 0.0
 
     def step():
         a.Assemble()

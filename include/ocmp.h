@@ -268,7 +268,8 @@ typedef struct {
   int32_t maxit;       /* total inner iterations                                 */
   double tol;          /* ||b - A x||_2 / ||b||_2 on free dofs, computed in FP64
                           (relative to the initial residual when b is 0 on the free dofs) */
-  double inner_tol;    /* mixed: relative reduction asked of each inner solve    */
+  double inner_tol;    /* mixed: relative reduction asked of each inner solve
+                          (0: 1e-5)                                             */
   int32_t check_every; /* CG: residual polled by the host every k iterations     */
   int32_t max_outer;   /* mixed: refinement steps                                */
   int32_t s_step;      /* GMRES on the DG block-Jacobi operator: 5 = s-step cycles
