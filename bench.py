@@ -222,11 +222,12 @@ def bench_learned(args, ctx):
     inflow = [(L.FN_POLY2, [0, 0, 6.0, 0, 0, -6.0]), const(0.0)]
     zero = [const(0.0), const(0.0)]
 
-    def model(precision):
+    def model(precision, s_step=5):
+        # the plain FP64 path is standard GMRES (s_step 0), as in the other configs
         return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
                    linear_solver='GMRes', preconditioner='schur', linear_tolerance=args.tol, restart=args.restart,
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
-                   inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, ctx=ctx)
+                   inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step, ctx=ctx)
 
     m0 = model('fp64')
     ndof = m0.fes.ndof
@@ -260,7 +261,7 @@ def bench_learned(args, ctx):
     del s0, m0, pol
 
     def branch(precision, use_net, timing=False):
-        m = model(precision)
+        m = model(precision, s_step=0 if precision == 'fp64' else 5)
         p = GuessPolicy(ctx, ndof, net=net if use_net else None, n_learned=seg[-1])
         for h in reversed(ring[1:]):
             p.push(h)

@@ -137,11 +137,13 @@ __global__ void k_reduce(const double *__restrict__ part, int nparts, double *__
 // mode 1: y = mask*(b - A x), partial sum y^2
 // mode 2: y = mask*(A x), partial sum x.y                (CG: q = A p, p.q)
 // mode 3: y = mask*(A (cs*sx*x))   cs: column scaling (right Jacobi), sx: device scalar  (GMRES on H1)
+// mode 4: y = mask*(A (cs*x)) - dsh*x + csh*xp                  (a Newton-basis step of s-step GMRES)
 
 template <typename VT, typename XT>
 __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT *__restrict__ y,
                                               const XT *__restrict__ x, const double *__restrict__ b,
-                                              const uint8_t *__restrict__ mask, XT scale, double &acc) {
+                                              const uint8_t *__restrict__ mask, XT scale, double &acc,
+                                              XT dsh, const XT *__restrict__ xp, XT csh) {
   if (mode == 0) {
     y[r] = (XT)s;
   } else if (mode == 1) {
@@ -152,8 +154,12 @@ __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT 
     const XT v = (mask && !mask[r]) ? (XT)0 : (XT)s;
     y[r] = v;
     acc += (double)x[r] * (double)v;
-  } else {
+  } else if (mode == 3) {
     y[r] = (mask && !mask[r]) ? (XT)0 : (XT)s * scale;
+  } else {
+    XT v = (XT)s - dsh * x[r];
+    if (xp) v += csh * xp[r];
+    y[r] = (mask && !mask[r]) ? (XT)0 : v;
   }
 }
 
@@ -167,7 +173,8 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
                                                    XT *__restrict__ y, const double *__restrict__ b,
                                                    const uint8_t *__restrict__ mask, const XT *__restrict__ cs,
                                                    const double *__restrict__ sx, int mode,
-                                                   double *__restrict__ part) {
+                                                   double *__restrict__ part, XT dsh, const XT *__restrict__ xp,
+                                                   XT csh) {
   // one LDS buffer: first the x values at the distinct columns, then (after every thread has
   // read its operands) the products, then the block-sum scratch, so the kernel's LDS is
   // exactly NNZB values (a power of two: whole blocks per CU)
@@ -211,7 +218,7 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
         const int t0 = base + threadIdx.x, t1 = t0 + NTHR;
         const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
         XT x0 = x[c0], x1 = x[c1];
-        if (mode == 3 && cs) {
+        if (mode >= 3 && cs) {
           x0 *= cs[c0];
           x1 *= cs[c1];
         }
@@ -239,7 +246,7 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
           for (int t = a + q; t < e; t += 4) s += (double)prod[t];
         s += __shfl_xor(s, 1, 4);
         s += __shfl_xor(s, 2, 4);
-        if (ri < nrows && q == 0) spmv_epilogue<VT, XT>(mode, r0 + ri, s, y, x, b, mask, scale, acc);
+        if (ri < nrows && q == 0) spmv_epilogue<VT, XT>(mode, r0 + ri, s, y, x, b, mask, scale, acc, dsh, xp, csh);
       }
     } else {
       // a single row longer than the staging buffer
@@ -247,11 +254,11 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
       for (int t = threadIdx.x; t < nz; t += NTHR) {
         const int32_t c = col[j0 + t];
         XT xv = x[c];
-        if (mode == 3 && cs) xv *= cs[c];
+        if (mode >= 3 && cs) xv *= cs[c];
         s += (double)((XT)val[j0 + t] * xv);
       }
       s = block_sum(s, sh);
-      if (threadIdx.x == 0) spmv_epilogue<VT, XT>(mode, r0, s, y, x, b, mask, scale, acc);
+      if (threadIdx.x == 0) spmv_epilogue<VT, XT>(mode, r0, s, y, x, b, mask, scale, acc, dsh, xp, csh);
     }
   }
   if (part) {
@@ -277,17 +284,18 @@ static int spmv_grid(Matrix *A) { return spmv_grid<double, double>(A); }
 
 template <typename VT, typename XT>
 static void spmv_dispatch(Matrix *A, const VT *val, const XT *x, XT *y, const double *b, const uint8_t *mask,
-                          const XT *cs, const double *sx, int mode, double *part, int grid, hipStream_t st) {
+                          const XT *cs, const double *sx, int mode, double *part, int grid, hipStream_t st,
+                          XT dsh = (XT)0, const XT *xp = nullptr, XT csh = (XT)0) {
   const RowBlocks &rb = row_blocks<VT, XT>(A);
   OCMP_REQUIRE((grid == spmv_grid<VT, XT>(A)), "SpMV grid does not match the row blocks");
   if (rb.nnzb == SPMV_NNZB32)
     k_spmv_csr<VT, XT, SPMV_NNZB32><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh);
   else
     k_spmv_csr<VT, XT, SPMV_NNZB64><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh);
 }
 
 // algorithmic bytes of one CSR SpMV: values + 16-bit local columns + the blocks' distinct
@@ -772,6 +780,7 @@ struct GState {
   double *dot, *vs, *R, *csn, *g, *res;
   int m;
   double gscale;
+  double *Hf = nullptr;  // if set: the unrotated Hessenberg columns (ld = m+1), for Ritz values
 };
 
 // Givens step for column j, run by one block: R(:,j) = [vs[k]*dot[k]]_k, h_{j+1,j} = hn;
@@ -784,6 +793,10 @@ __device__ void givens_step(const GState &G, int j, double hn) {
   for (int k = threadIdx.x; k < 2 * j; k += blockDim.x) rot[k] = G.csn[k];
   if (threadIdx.x == 0) col[j + 1] = hn;
   __syncthreads();
+  if (G.Hf) {
+    for (int k = threadIdx.x; k <= j + 1; k += blockDim.x) G.Hf[(int64_t)j * ld + k] = col[k];
+    __syncthreads();
+  }
   if (threadIdx.x == 0) {
     for (int k = 0; k < j; ++k) {
       const double c = rot[2 * k], s = rot[2 * k + 1];
@@ -950,6 +963,7 @@ struct Work {
   // scratch owned by the matrix' solver state; grown on demand
   DBuf<double> r64, u64, t64;
   DBuf<double> xbak;  // x before an s-step cycle (restored if the cycle does not lower the residual)
+  DBuf<double> hf;    // unrotated Hessenberg matrix of a standard cycle (Ritz values for s-step shifts)
   DBuf<double> vd;   // FP64 work vectors (n * k)
   DBuf<float> vf;    // FP32 work vectors
   DBuf<double> part; // partial sums
@@ -1296,7 +1310,7 @@ static void solve_cg(Matrix *A, Precond *P, const double *b, double *x, const oc
 // accumulating the update into x (FP64). Returns iterations used.
 template <typename T>
 static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, double *x, int m, double stop_abs,
-                       int maxit, int check, Work &w) {
+                       int maxit, int check, Work &w, bool capture_h = false) {
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = A->n;
@@ -1332,6 +1346,10 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
   const double gscale = beta;  // g is relative to beta
   const int limit = std::min(m, maxit);
   GState G{dot, vs, R, csn, g, res, m, gscale};
+  if (capture_h) {  // keep the unrotated Hessenberg columns in w.hf (ld = m+1)
+    w.hf.alloc(std::max<size_t>(w.hf.n, (size_t)(m + 1) * m));
+    G.Hf = w.hf.p;
+  }
   struct Batch {
     int j0, nb, ev;
   };
@@ -1466,6 +1484,24 @@ struct CaShift {
   double d[CA_S];
   double c[CA_S];
 };
+
+// Newton shifts from the leading k x k block of an unrotated Hessenberg matrix (device, ld = m+1):
+// Leja-ordered Ritz values, rounded to what the basis precision applies (the DG kernel applies
+// 1 - d, the CSR kernel d). Returns false (sh unchanged) if the eigenvalues are not available.
+static bool newton_from_hessenberg(const double *Hf, int m, int k, bool dg, bool fp32, CaShift &sh) {
+  std::vector<double> Hh((size_t)(m + 1) * k), wr(k), wi(k);
+  OCMP_HIP(hipMemcpy(Hh.data(), Hf, Hh.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (!hessenberg_eigvals(k, Hh.data(), m + 1, wr.data(), wi.data())) return false;
+  CaShift out;
+  newton_shifts(k, wr.data(), wi.data(), CA_S, out.d, out.c);
+  if (fp32)
+    for (int i = 0; i < CA_S; ++i) {
+      out.d[i] = dg ? 1.0 - (double)(float)(1.0 - out.d[i]) : (double)(float)out.d[i];
+      out.c[i] = (double)(float)out.c[i];
+    }
+  sh = out;
+  return true;
+}
 
 // part[(q*S + i) * grid + lb] = partial of Q[q] . W[i],  q < nq
 // Rows past nq in the last y-tile read row nq-1 again (no branch around the loads, so all of a
@@ -1907,17 +1943,28 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   double *hres_d = nullptr;  // the same pinned buffer as the device addresses it
   OCMP_HIP(hipHostGetDevicePointer((void **)&hres_d, hres, 0));
   GState G{nullptr, nullptr, Hr, csn, g, res, m, beta};
-  const int c_op = ctx->cls(sizeof(T) == 8 ? "dg_ahat_spmv_f64" : "dg_ahat_spmv_f32");
+  const bool dg = A->dg;
+  const int c_op = ctx->cls(dg ? (sizeof(T) == 8 ? "dg_ahat_spmv_f64" : "dg_ahat_spmv_f32")
+                               : (sizeof(T) == 8 ? "spmv_csr_f64_gmres" : "spmv_csr_f32_gmres"));
   const int c_dot = ctx->cls(sizeof(T) == 8 ? "ca_bdot_f64" : "ca_bdot_f32");
   const int c_upd = ctx->cls(sizeof(T) == 8 ? "ca_bupdate_f64" : "ca_bupdate_f32");
   const int c_qr = ctx->cls(sizeof(T) == 8 ? "ca_cholqr_f64" : "ca_cholqr_f32");
   const int64_t nel = A->space->mesh->nel;
   const unsigned gb = nblk((n + VW - 1) / VW);
-  const T *ah;
-  if constexpr (sizeof(T) == 8)
-    ah = P->ahat.p;
-  else
-    ah = P->ahat32.p;
+  // DG: the block-Jacobi-folded ELL operator; H1 / saddle point: the CSR matrix with the point
+  // Jacobi (or Schur-diagonal) scaling applied to its columns (right preconditioning)
+  const T *ah = nullptr, *vals = nullptr, *cs = nullptr;
+  if constexpr (sizeof(T) == 8) {
+    if (dg) ah = P->ahat.p;
+    vals = A->d_val.p;
+    if (P && P->kind == OCMP_PRE_JACOBI) cs = P->dinv.p;
+  } else {
+    if (dg) ah = P->ahat32.p;
+    vals = A->d_val32.p;
+    if (P && P->kind == OCMP_PRE_JACOBI) cs = P->dinv32.p;
+  }
+  const uint8_t *mask = dg ? nullptr : A->space->d_free.p;
+  const int sgrid = dg ? 0 : spmv_grid<T, T>(A);
   struct Batch {
     int j0, ev;
   };
@@ -1927,13 +1974,20 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   breakdown = false;
   auto enqueue = [&]() {
     for (int i = 0; i < CA_S; ++i) {
-      // v_{i+1} = (Ahat - d_i I) v_i + c_i v_{i-1}: the diagonal blocks of Ahat are the identity,
-      // so the kernel's diagonal coefficient is 1 - d_i (c_0 = 0 always)
-      const T dc = (T)(1.0 - shift.d[i]), pc = (T)shift.c[i];
+      // v_{i+1} = (Ahat - d_i I) v_i + c_i v_{i-1} (c_0 = 0 always)
+      const T pc = (T)shift.c[i];
       const T *xp = (i > 0 && shift.c[i] != 0.0) ? Q + (int64_t)(k + i - 1) * ldv : nullptr;
-      KTimer tm(ctx, c_op, (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * nel + (xp ? 3.0 : 2.0) * n * sizeof(T));
       const T *xin = Q + (int64_t)(k + i) * ldv;
       T *yout = Q + (int64_t)(k + i + 1) * ldv;
+      if (!dg) {
+        KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)) + (xp ? 2.0 : 1.0) * n * sizeof(T));
+        spmv_dispatch<T, T>(A, vals, xin, yout, nullptr, mask, cs, nullptr, 4, nullptr, sgrid, st, (T)shift.d[i], xp,
+                            pc);
+        continue;
+      }
+      // the diagonal blocks of Ahat are the identity, so the kernel's diagonal coefficient is 1 - d_i
+      const T dc = (T)(1.0 - shift.d[i]);
+      KTimer tm(ctx, c_op, (double)n * 3 * A->dg_nb * sizeof(T) + 12.0 * nel + (xp ? 3.0 : 2.0) * n * sizeof(T));
       switch (A->dg_nb) {
         case 1: k_dg_ahat_spmv<T, 1><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
         case 3: k_dg_ahat_spmv<T, 3><<<gb, NTHR, 0, st>>>(nel, w.nbr.p, ah, xin, one, yout, dc, xp, pc); break;
@@ -2011,30 +2065,22 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
   // drain the look-ahead before the buffers are reused
   OCMP_HIP(hipStreamSynchronize(st));
   if (k_used <= 0) return breakdown ? -1 : 0;
-  if (!newton && !breakdown && k_used >= 2 * CA_S) {
-    // Newton-basis shifts for the later cycles of this solve: Leja-ordered Ritz values of this
-    // cycle's (unrotated) Hessenberg matrix, rounded to what the basis precision applies
-    std::vector<double> Hh((size_t)(m + 1) * k_used), wr(k_used), wi(k_used);
-    OCMP_HIP(hipMemcpy(Hh.data(), Hf, Hh.size() * sizeof(double), hipMemcpyDeviceToHost));
-    if (hessenberg_eigvals(k_used, Hh.data(), m + 1, wr.data(), wi.data())) {
-      CaShift sh;
-      newton_shifts(k_used, wr.data(), wi.data(), CA_S, sh.d, sh.c);
-      for (int i = 0; i < CA_S; ++i) {
-        sh.d[i] = 1.0 - (double)(T)(1.0 - sh.d[i]);
-        sh.c[i] = (double)(T)sh.c[i];
-      }
-      shift = sh;
-      newton = true;
-    }
-  }
+  // Newton-basis shifts for the later cycles of this solve from this cycle's Ritz values
+  if (!newton && !breakdown && k_used >= 2 * CA_S)
+    newton = newton_from_hessenberg(Hf, m, k_used, A->dg, sizeof(T) == 4, shift);
   k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, Hr, g, nullptr, coef);
   w.u64.alloc(n);
   k_combine<T><<<nblk(n), NTHR, 0, st>>>(n, Q, ldv, k_used, coef, beta, w.u64.p);
-  switch (A->dg_nb) {
-    case 1: k_dg_block_apply_add<1><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
-    case 3: k_dg_block_apply_add<3><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
-    case 6: k_dg_block_apply_add<6><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
-    case 10: k_dg_block_apply_add<10><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+  if (!dg) {
+    const double *dinv = (P && P->kind == OCMP_PRE_JACOBI) ? P->dinv.p : nullptr;
+    k_diag_apply_add<<<nblk(n), NTHR, 0, st>>>(n, dinv, w.u64.p, x);
+  } else {
+    switch (A->dg_nb) {
+      case 1: k_dg_block_apply_add<1><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+      case 3: k_dg_block_apply_add<3><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+      case 6: k_dg_block_apply_add<6><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+      case 10: k_dg_block_apply_add<10><<<nblk(n), NTHR, 0, st>>>(nel, P->binv.p, w.u64.p, x); break;
+    }
   }
   OCMP_HIP(hipGetLastError());
   return k_used;
@@ -2244,9 +2290,14 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
   const int m = o.restart;
   int total = 0, outer = 0;
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
-  // s-step cycles on the DG block-Jacobi operator when asked for and the restart fits the block size
-  bool ca = o.s_step == CA_S && A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI && m % CA_S == 0 && m <= CA_MMAX;
-  // the first s-step cycle uses the shifted monomial basis; it leaves Newton shifts for the rest
+  // s-step cycles when asked for and the restart fits the block size: on the DG block-Jacobi
+  // operator, and on a CSR matrix with point-Jacobi / Schur-diagonal (or no) preconditioning
+  const bool ca_dg = A->dg && P && P->kind == OCMP_PRE_BLOCK_JACOBI;
+  const bool ca_csr = !A->dg && (!P || P->kind == OCMP_PRE_JACOBI);
+  bool ca = o.s_step == CA_S && (ca_dg || ca_csr) && m % CA_S == 0 && m <= CA_MMAX;
+  // DG: the first s-step cycle uses the shifted monomial basis (the spectrum of the folded operator
+  // clusters around 1) and leaves Newton shifts for the rest. CSR: a first standard cycle records
+  // its Hessenberg matrix for the shifts.
   CaShift ca_shift;
   for (int i = 0; i < CA_S; ++i) {
     ca_shift.d[i] = CA_SIGMA;
@@ -2257,8 +2308,15 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
     ++outer;
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
     int used = -1;
-    const bool ran_ca = ca;
-    if (ca) {
+    if (ca && ca_csr && !newton) {
+      if (mixed)
+        used = gmres_cycle<float>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w, true);
+      else
+        used = gmres_cycle<double>(A, P, w.r64.p, rn, x, m, stop_abs, o.maxit - total, check, w, true);
+      if (used >= 2 * CA_S) newton = newton_from_hessenberg(w.hf.p, m, used, false, mixed, ca_shift);
+    }
+    const bool ran_ca = ca && used < 0;
+    if (ran_ca) {
       w.xbak.alloc(A->n);
       OCMP_HIP(hipMemcpyAsync(w.xbak.p, x, A->n * sizeof(double), hipMemcpyDeviceToDevice, A->ctx->stream));
       bool broke = false;

@@ -470,3 +470,67 @@ def test_s_step_gmres_matches_standard(gpu_ctx, precision):
     assert r5.iters == r6.iters and torch.equal(x5.vec, x6.vec)
     with pytest.raises(L.OcmpError, match='s_step'):
         C.solve(a, f.vec, x5.vec, pre, 'GMRes', tol=1e-8, restart=50, precision=precision, s_step=3)
+
+
+def _csr_problem(gpu_ctx, problem, precision):
+    C = core()
+    if problem == 'convdiff':
+        # nonsymmetric H1 P2 convection-diffusion, point Jacobi
+        hm = unit_square(48)
+        gm = C.Mesh(hm, gpu_ctx)
+        gs = C.FESpace(gm, [C.H1(2, '|'.join(hm.bnd_names))])
+        a = C.BilinearForm(gs)
+        a += C.Vol('gradgrad', 0, 0, coef=1e-2)
+        a += C.Vol('conv', 0, 0, coef=-1.0, wind=(1.0, 0.5, 0.0))
+        f = C.LinearForm(gs)
+        f += C.Source(0, (L.FN_CONST, (1.0,)))
+        a.Assemble()
+        f.Assemble()
+        pre = C.Preconditioner(a, 'local', precision=precision)
+        pre.Update()
+        return a, f.vec, pre, C.GridFunction(gs)
+    # an implicit-Euler step of Taylor-Hood Stokes (M + dt (kv K + B-coupling), dt = 1e-2) with the
+    # Schur-diagonal preconditioner: the saddle-point system the transient models solve
+    from opencmp_amd.models import Stokes, const
+    hm = unit_square(8)   # (GMRES(50) with the diagonal Schur approximation stalls from about 24^2 on)
+    inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
+    wall = [const(0.0), const(0.0)]
+    m = Stokes(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
+               linear_solver='GMRes', preconditioner='schur', linear_tolerance=1e-10, restart=50,
+               precision=precision, ctx=gpu_ctx)
+    dt = 1e-2
+    a = C.BilinearForm(m.fes, m.time_derivative_terms() + m.construct_bilinear_time_coefficient(dt) +
+                       m.construct_bilinear_time_ODE(dt))
+    f = C.LinearForm(m.fes, m.construct_linear(dt))
+    a.Assemble()
+    f.Assemble()
+    pre = m.construct_preconditioners(a)
+    pre.Update()
+    x0 = C.GridFunction(m.fes)
+    m.apply_dirichlet_bcs_to(x0)
+    return a, f.vec, pre, x0
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+@pytest.mark.parametrize('problem', ['convdiff', 'stokes'])
+def test_s_step_gmres_csr_matches_standard(gpu_ctx, problem, precision):
+    """s-step GMRES(50) on CSR systems (Newton shifts from a first standard cycle; the point-Jacobi
+    or Schur-diagonal scaling folded into the operator's columns): the solution of standard
+    GMRES(50) at about the same iteration count, no fallback, bitwise repeatable."""
+    C = core()
+    say('s-step gmres csr', problem, precision)
+    a, b, pre, x0 = _csr_problem(gpu_ctx, problem, precision)
+    x1 = C.GridFunction(x0.space, x0.vec.clone())
+    x2 = C.GridFunction(x0.space, x0.vec.clone())
+    r1 = C.solve(a, b, x1.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=100000)
+    r2 = C.solve(a, b, x2.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=100000, s_step=5)
+    say('  standard', r1, 's-step', r2)
+    assert r1.converged and r2.converged and r2.rel_residual <= 1e-10
+    assert not r2.fallback
+    assert r2.iters <= 1.25 * r1.iters + 20
+    rel = float(torch.linalg.norm(x1.vec - x2.vec) / torch.linalg.norm(x1.vec))
+    say('  rel diff', rel)
+    assert rel < 1e-6
+    x3 = C.GridFunction(x0.space, x0.vec.clone())
+    r3 = C.solve(a, b, x3.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=100000, s_step=5)
+    assert r3.iters == r2.iters and torch.equal(x3.vec, x2.vec)
