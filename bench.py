@@ -227,7 +227,8 @@ def bench_learned(args, ctx):
         return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
                    linear_solver='GMRes', preconditioner='schur', linear_tolerance=args.tol, restart=args.restart,
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
-                   inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step, ctx=ctx)
+                   inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step,
+                   schur_iters=args.schur_iters, ctx=ctx)
 
     m0 = model('fp64')
     ndof = m0.fes.ndof
@@ -239,9 +240,8 @@ def bench_learned(args, ctx):
     for k in range(args.record):
         s0.step()
         rec_its.append((m0.linear_iterations, m0.picard_iterations))
-        if k % 10 == 0 or k == args.record - 1:
-            log(f'record step {k}: picard {m0.picard_iterations} lin iters {m0.linear_iterations} '
-                f'({time.perf_counter() - t0:.1f}s)')
+        log(f'record step {k}: picard {m0.picard_iterations} lin iters {m0.linear_iterations} '
+            f'({time.perf_counter() - t0:.1f}s)')
     torch.cuda.synchronize()
     t_record = time.perf_counter() - t0
     # the network predicts the velocity blocks' correction to the linear extrapolation 2 u^n - u^{n-1};
@@ -270,6 +270,8 @@ def bench_learned(args, ctx):
         rows = []
         for k in range(args.warmup):
             s.step()
+            if args.verbose_steps:
+                log(f'  {precision} warmup {k}: lin iters {m.linear_iterations}')
         torch.cuda.synchronize()
         if timing:
             ctx.timing(True)
@@ -278,6 +280,8 @@ def bench_learned(args, ctx):
         for k in range(args.steps):
             r = s.step()
             rows.append((m.linear_iterations, m.picard_iterations, r.rel_residual, m.picard_log))
+            if args.verbose_steps:
+                log(f'  {precision} step {k}: lin iters {m.linear_iterations} ({time.perf_counter() - t1:.1f}s)')
         torch.cuda.synchronize()
         dt_ = (time.perf_counter() - t1) / args.steps
         rep = None
@@ -293,8 +297,10 @@ def bench_learned(args, ctx):
     _, _, _, _, rep = branch('mixed', True, timing=True)
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
     return dict(desc=(f'transient INS Taylor-Hood P2/P1, implicit Euler dt 1e-3 + Oseen Picard (<=3, tol 1e-6), '
-                      f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), GMRES({args.restart}) + '
-                      f'Schur-diagonal preconditioner'),
+                      f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), ' +
+                      (f'FGMRES({args.restart}) + Schur preconditioner with {args.schur_iters} inner PCG '
+                       f'iterations on C + B diag(A)^-1 B^T' if args.schur_iters else
+                       f'GMRES({args.restart}) + Schur-diagonal preconditioner')),
                 ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep,
                 learned=dict(record_steps=args.record, record_s=round(t_record, 2), train_steps=args.train_steps,
                              train_batch=65536, train_s=round(t_train, 3), loss_first=losses[0],
@@ -429,6 +435,10 @@ def main():
     ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4, 5))
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
+    ap.add_argument('--verbose-steps', action='store_true', help='config 4: log every step of the timed legs')
+    ap.add_argument('--schur-iters', type=int, default=0,
+                    help='config 4: inner PCG iterations on the pressure Schur complement per FGMRES step '
+                         '(0: the Schur diagonal under GMRES)')
     ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
     ap.add_argument('--n', type=int, default=None,
                     help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 32; config 5 at 64^3 does not '

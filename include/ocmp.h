@@ -254,6 +254,18 @@ int ocmp_precond_create(ocmp_matrix *A, int kind, int precision, ocmp_precond **
  * -(C + B diag(A)^{-1} B^T). The reference solves these systems with a direct
  * factorisation (its 'direct' default); this replaces it on the Krylov path. */
 int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_blocks, ocmp_precond **out);
+/* Schur inner solve for a preconditioner from ocmp_precond_create_schur (same
+ * replacement of the reference's direct factorisation of the saddle-point system,
+ * opencmp/models/ins.py:178-270 with opencmp/solvers/base_solver.py:711-719):
+ * iters > 0 makes the flexible step of OCMP_SOLVER_FGMRES solve the Schur rows
+ * with iters Jacobi-PCG iterations on S = C + B diag(A)^{-1} B^T (the sign of the
+ * Schur diagonal, i.e. -S^{-1} on those rows) instead of scaling them by the
+ * Schur diagonal; 0 turns it off. The Schur blocks must be adjacent in the dof
+ * numbering. S's pattern is built at the next ocmp_precond_update and its values
+ * at every update, so call ocmp_precond_update after turning it on. The step is
+ * nonlinear in its input, so the other solvers refuse such a preconditioner
+ * (OCMP_ERR_ARG). */
+int ocmp_precond_set_schur_solve(ocmp_precond *P, int iters);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);

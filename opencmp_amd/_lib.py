@@ -96,6 +96,7 @@ SIGNATURES = {
     'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],
+    'ocmp_precond_set_schur_solve': [P, C.c_int],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],

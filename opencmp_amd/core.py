@@ -449,10 +449,12 @@ class Preconditioner:
     KINDS = {'local': None, 'jacobi': L.PRE_JACOBI, 'block_jacobi': L.PRE_BLOCK_JACOBI, 'none': L.PRE_NONE}
 
     def __init__(self, a: BilinearForm, kind: str = 'local', precision: str = 'fp64',
-                 schur_blocks: Sequence[int] = ()):
+                 schur_blocks: Sequence[int] = (), schur_iters: int = 0):
         """kind 'schur': point Jacobi on all rows except those of `schur_blocks` (the
         pressure blocks of a saddle-point system), which get the diagonal of the
-        Schur complement (ocmp_precond_create_schur)."""
+        Schur complement (ocmp_precond_create_schur); with `schur_iters` > 0 FGMRES
+        solves those rows by that many PCG iterations on C + B diag(A)^{-1} B^T instead
+        (ocmp_precond_set_schur_solve)."""
         self.a = a
         self.precision = L.MIXED if precision == 'mixed' else L.FP64
         h = C.c_void_p()
@@ -463,6 +465,8 @@ class Preconditioner:
             self.kind = L.PRE_JACOBI
             L.check(a.ctx.lib.ocmp_precond_create_schur(a.h, self.precision, C.c_uint64(mask), C.byref(h)),
                     'precond_create_schur')
+            if schur_iters:
+                L.check(a.ctx.lib.ocmp_precond_set_schur_solve(h, int(schur_iters)), 'precond_set_schur_solve')
         else:
             k = self.KINDS[kind]
             if k is None:

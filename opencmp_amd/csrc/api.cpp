@@ -465,6 +465,82 @@ void ensure_row_blocks32(Matrix *A) {
   if (!A->rb32.nnzb) build_row_blocks(A, A->rb32, SPMV_NNZB32);
 }
 
+void build_schur_matrix(Precond *P) {
+  Matrix *A = P->A;
+  const auto &bl = A->space->blocks;
+  int64_t s0 = -1, s1 = -1;
+  for (size_t k = 0; k < bl.size(); ++k) {
+    if (!(P->schur_blocks >> k & 1)) continue;
+    if (s0 < 0) {
+      s0 = bl[k].dof_off;
+    } else {
+      OCMP_REQUIRE(bl[k].dof_off == s1, "the Schur inner solve needs the Schur blocks to be adjacent");
+    }
+    s1 = bl[k].dof_off + bl[k].n1;
+  }
+  OCMP_REQUIRE(s0 >= 0 && s1 > s0, "no Schur rows");
+  const int64_t n = A->n, ns = s1 - s0;
+  const int32_t *rp = A->row_ptr.data(), *col = A->col.data();
+  const uint8_t *freed = A->space->free.data();
+  // per row: the CSR slots of its Schur columns (sorted columns, so one range)
+  std::vector<int32_t> seg(2 * n);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    seg[2 * r] = (int32_t)(std::lower_bound(col + rp[r], col + rp[r + 1], (int32_t)s0) - col);
+    seg[2 * r + 1] = (int32_t)(std::lower_bound(col + rp[r], col + rp[r + 1], (int32_t)s1) - col);
+  }
+  // row p of S: the Schur columns of row p (C) and of every row u that p couples to through a
+  // free non-Schur column (B diag(A)^{-1} B^T)
+  std::vector<std::vector<int32_t>> rows(ns);
+#pragma omp parallel
+  {
+    std::vector<int32_t> buf;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < ns; ++i) {
+      const int64_t p = s0 + i;
+      buf.clear();
+      for (int32_t j = seg[2 * p]; j < seg[2 * p + 1]; ++j) buf.push_back(col[j] - (int32_t)s0);
+      for (int pass = 0; pass < 2; ++pass) {
+        const int32_t a = pass ? seg[2 * p + 1] : rp[p], e = pass ? rp[p + 1] : seg[2 * p];
+        for (int32_t j = a; j < e; ++j) {
+          const int32_t u = col[j];
+          if (!freed[u]) continue;
+          for (int32_t t = seg[2 * u]; t < seg[2 * u + 1]; ++t) buf.push_back(col[t] - (int32_t)s0);
+        }
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      rows[i] = buf;
+    }
+  }
+  auto *S = new Matrix();
+  S->space = A->space;
+  S->ctx = A->ctx;
+  S->n = ns;
+  S->row_ptr.assign(ns + 1, 0);
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < ns; ++i) {
+    nnz += (int64_t)rows[i].size();
+    OCMP_REQUIRE(nnz < (int64_t)INT32_MAX, "Schur matrix nnz exceeds 32-bit CSR offsets");
+    S->row_ptr[i + 1] = (int32_t)nnz;
+  }
+  S->nnz = nnz;
+  S->col.resize(nnz);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < ns; ++i) std::copy(rows[i].begin(), rows[i].end(), S->col.begin() + S->row_ptr[i]);
+  hipStream_t st = A->ctx->stream;
+  S->d_row_ptr.upload(S->row_ptr, st);
+  S->d_col.upload(S->col, st);
+  S->d_val.alloc(nnz);
+  build_row_blocks(S, S->rb64, SPMV_NNZB64);
+  S->assembled = true;
+  P->d_sseg.upload(seg, st);
+  OCMP_HIP(hipStreamSynchronize(st));
+  P->s0 = s0;
+  P->s1 = s1;
+  P->S = S;
+}
+
 void build_colouring(Space &s) {
   Mesh &m = *s.mesh;
   const int d = m.dim, nvl = d + 1;
@@ -946,6 +1022,15 @@ int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_bloc
     P->P.precision = precision;
     P->P.schur_blocks = schur_blocks;
     *out = P;
+  });
+}
+
+int ocmp_precond_set_schur_solve(ocmp_precond *P, int iters) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.schur_blocks != 0, "the Schur inner solve needs a Schur preconditioner");
+    OCMP_REQUIRE(iters >= 0 && iters <= 1000, "Schur inner iterations must be in [0, 1000]");
+    P->P.schur_iters = iters;
   });
 }
 

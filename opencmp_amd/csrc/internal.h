@@ -239,6 +239,22 @@ struct Precond {
   // A itself in the same ELL layout with the diagonal block as slot 0 (FP64 outer residual)
   DBuf<double> aell;     // n*4*nb
   bool ready = false;
+  // Schur inner solve (FGMRES only; 0: off): on the Schur rows [s0, s1) the flexible
+  // preconditioner takes schur_iters PCG iterations on S = C + B diag(A)^{-1} B^T in place of
+  // the Schur-diagonal step. S's pattern is built once from A's, its values at every update.
+  int schur_iters = 0;
+  int64_t s0 = 0, s1 = 0;
+  Matrix *S = nullptr;
+  DBuf<int32_t> d_sseg;  // per row of A: first and one-past-last CSR slot of its columns in [s0, s1)
+  DBuf<double> sdinv;    // Jacobi of S: 1/S_ii, 0 on rows the Schur diagonal leaves out
+  DBuf<float> sdinv32;
+  DBuf<double> svd;      // PCG vectors r z p q (4 ns)
+  DBuf<float> svf;
+  DBuf<double> spart;    // PCG partial sums
+  Precond() = default;
+  Precond(const Precond &) = delete;
+  Precond &operator=(const Precond &) = delete;
+  ~Precond() { delete S; }
 };
 
 // ------------------------------------------------------------------ kernels (launchers)
@@ -308,6 +324,9 @@ constexpr int SPMV_NNZB64 = 2048;
 constexpr int SPMV_NNZB32 = 4096;
 void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb);
 void ensure_row_blocks32(Matrix *A);
+// the pattern of a Schur preconditioner's S = C + B diag(A)^{-1} B^T and the per-row Schur
+// column ranges of A (host, once per preconditioner)
+void build_schur_matrix(Precond *P);
 
 }  // namespace ocmp
 

@@ -612,6 +612,56 @@ __global__ void k_schur_finish(int64_t n, const uint8_t *__restrict__ flag, cons
   if (dinv32) dinv32[r] = (float)v;
 }
 
+// Values of S = C + B diag(A)^{-1} B^T on the Schur rows [s0, s1) of [[A, B^T], [B, -C]]:
+// S_pq = -a_pq + sum_u a_pu a_uq / a_uu over the free non-Schur columns u of row p (dinv[u] is
+// 1/a_uu there and 0 on Dirichlet dofs). One wavefront per row of S and one lane per entry: the
+// lanes walk row p's columns u together and each looks its own column q up in the Schur part of
+// row u (a few slots, read by every lane at once), so each entry is summed in one fixed order.
+__global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, const int32_t *__restrict__ srp,
+                                                       const int32_t *__restrict__ scol,
+                                                       const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       const int32_t *__restrict__ sseg,
+                                                       const double *__restrict__ dinv, double *__restrict__ sval) {
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= ns) return;
+  const int64_t p = s0 + row;
+  const int32_t pb = sseg[2 * p], pe = sseg[2 * p + 1];
+  const int32_t k1 = srp[row + 1];
+  for (int32_t k0 = srp[row]; k0 < k1; k0 += 64) {
+    const int32_t k = k0 + lane;
+    const int32_t q = k < k1 ? (int32_t)(s0 + scol[k]) : -1;
+    double acc = 0.0;
+    for (int32_t j = pb; j < pe; ++j)
+      if (col[j] == q) acc -= val[j];
+    for (int pass = 0; pass < 2; ++pass) {
+      const int32_t a = pass ? pe : rp[p], e = pass ? rp[p + 1] : pb;
+      for (int32_t j = a; j < e; ++j) {
+        const int32_t u = col[j];
+        const double f = val[j] * dinv[u];
+        if (f == 0.0) continue;
+        const int32_t t1 = sseg[2 * u + 1];
+        for (int32_t t = sseg[2 * u]; t < t1; ++t)
+          if (col[t] == q) acc += f * val[t];
+      }
+    }
+    if (k < k1) sval[k] = acc;
+  }
+}
+
+// Jacobi of S from the Schur diagonal: dinv holds -1/s on the Schur rows (0 where the diagonal
+// pass left a row out), so S's scale is -dinv
+__global__ void k_schur_sdinv(int64_t ns, const double *__restrict__ dinv_s, double *__restrict__ sdinv,
+                              float *__restrict__ sdinv32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const double v = -dinv_s[i];
+  sdinv[i] = v;
+  if (sdinv32) sdinv32[i] = (float)v;
+}
+
 __global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
@@ -1113,6 +1163,26 @@ void precond_update(Precond *P) {
                                           P->schur_flag.p, P->dinv.p, w.part.p);
         k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nb, smax);
         k_schur_finish<<<nb, NTHR, 0, st>>>(A->n, P->schur_flag.p, smax, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+      }
+      if (P->schur_iters > 0) {
+        if (!P->S) build_schur_matrix(P);
+        Matrix *S = P->S;
+        const int64_t ns = S->n;
+        k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
+                                                       A->d_col.p, A->d_val.p, P->d_sseg.p, P->dinv.p, S->d_val.p);
+        P->sdinv.alloc(ns);
+        if (mixed) P->sdinv32.alloc(ns);
+        k_schur_sdinv<<<nblk(ns), NTHR, 0, st>>>(ns, P->dinv.p + P->s0, P->sdinv.p, mixed ? P->sdinv32.p : nullptr);
+        if (mixed) {
+          S->d_val32.alloc(S->nnz);
+          k_to_f32<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, S->d_val.p, S->d_val32.p);
+          ensure_row_blocks32(S);
+          P->svf.alloc((size_t)4 * ns);
+        } else {
+          P->svd.alloc((size_t)4 * ns);
+        }
+        const int sg = std::max(spmv_grid<double, double>(S), spmv_grid<float, float>(S));
+        P->spart.alloc(std::max(P->spart.n, (size_t)2 * red_grid(ns) + sg));
       }
     }
     if (mixed) {
@@ -2089,6 +2159,133 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
 // Flexible GMRES cycle (right preconditioning that may vary between iterations):
 // z_j = M_j v_j with M = D^{-1} (+ the learned correction when `learned`), w = A z_j,
 // x += Z y. Same device-side Gram-Schmidt / Givens machinery as gmres_cycle.
+// ------------------------------------------------------------------ Schur inner PCG
+//
+// Fixed-count Jacobi PCG on S for the Schur rows of an FGMRES preconditioner step. Each block
+// sums the partial dot products it needs itself (the same order in every block, so every block
+// gets the same alpha and beta), which leaves three launches per iteration: SpMV (+ p.q
+// partials), the x/r/z update (+ r.z partials) and the search-direction update. The r.z
+// partials alternate between two buffers by iteration. A zero denominator gives a zero step.
+
+__device__ __forceinline__ double block_sum_all(double v, double *sh) {
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) sh[0] = v;
+  __syncthreads();
+  v = sh[0];
+  __syncthreads();
+  return v;
+}
+
+__device__ __forceinline__ double sum_parts(const double *__restrict__ part, int n, double *sh) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  return block_sum_all(s, sh);
+}
+
+// x = 0, r = -s v, z = dinv r, p = z; partials of r.z
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restrict__ v, const double *__restrict__ sv,
+                                                   const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
+                                                   T *__restrict__ z, T *__restrict__ p, double *__restrict__ rz) {
+  __shared__ double sh[NTHR / 64];
+  const int lb = xcd_block();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
+  const double sc = -sv[0];
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    x[i] = (T)0;
+    const T ri = (T)((double)v[i] * sc);
+    r[i] = ri;
+    const T zi = dinv[i] * ri;
+    z[i] = zi;
+    p[i] = zi;
+    acc += (double)ri * (double)zi;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) rz[lb] = acc;
+}
+
+// alpha = r.z / p.q; x += alpha p; r -= alpha q; z = dinv r; partials of the new r.z
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_scg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
+                                                     T *__restrict__ z, const T *__restrict__ p,
+                                                     const T *__restrict__ q, const T *__restrict__ dinv,
+                                                     const double *__restrict__ pq_part, int npq,
+                                                     const double *__restrict__ rz_old, double *__restrict__ rz_new) {
+  __shared__ double sh[NTHR / 64];
+  const double pq = sum_parts(pq_part, npq, sh);
+  const double rz = sum_parts(rz_old, gridDim.x, sh);
+  const T alpha = (T)(pq != 0.0 ? rz / pq : 0.0);
+  const int lb = xcd_block();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    const T zi = dinv[i] * ri;
+    z[i] = zi;
+    acc += (double)ri * (double)zi;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) rz_new[lb] = acc;
+}
+
+// p = z + beta p, beta = new r.z / old r.z
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_scg_p(int64_t n, T *__restrict__ p, const T *__restrict__ z,
+                                                const double *__restrict__ rz_old,
+                                                const double *__restrict__ rz_new) {
+  __shared__ double sh[NTHR / 64];
+  const double o = sum_parts(rz_old, gridDim.x, sh);
+  const double w = sum_parts(rz_new, gridDim.x, sh);
+  const T beta = (T)(o != 0.0 ? w / o : 0.0);
+  const int lb = xcd_block();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+// z[s0:s1] = -S^{-1} (s v)[s0:s1] by P->schur_iters PCG iterations from zero
+template <typename T>
+static void schur_inner(Precond *P, const T *v, const double *sv, T *z) {
+  Matrix *S = P->S;
+  Ctx *ctx = S->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t n = S->n;
+  const T *val, *dinv;
+  T *buf;
+  if constexpr (sizeof(T) == 8) {
+    val = S->d_val.p;
+    dinv = P->sdinv.p;
+    buf = P->svd.p;
+  } else {
+    val = S->d_val32.p;
+    dinv = P->sdinv32.p;
+    buf = P->svf.p;
+  }
+  T *x = z + P->s0, *r = buf, *zz = r + n, *p = zz + n, *q = p + n;
+  const int vg = red_grid(n), sg = spmv_grid<T, T>(S);
+  double *rz[2] = {P->spart.p, P->spart.p + vg};
+  double *pq = P->spart.p + 2 * vg;
+  const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_schur_f64" : "spmv_schur_f32");
+  const int c_upd = ctx->cls(sizeof(T) == 8 ? "schur_pcg_update_f64" : "schur_pcg_update_f32");
+  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r, zz, p, rz[0]);
+  for (int it = 0; it < P->schur_iters; ++it) {
+    {
+      KTimer tm(ctx, c_op, csr_bytes(S, sizeof(T), sizeof(T)));
+      spmv_dispatch<T, T>(S, val, p, q, nullptr, nullptr, nullptr, nullptr, 2, pq, sg, st);
+    }
+    {
+      KTimer tm(ctx, c_upd, 7.0 * n * sizeof(T));
+      k_scg_update<T><<<vg, NTHR, 0, st>>>(n, x, r, zz, p, q, dinv, pq, sg, rz[it & 1], rz[(it + 1) & 1]);
+    }
+    if (it + 1 < P->schur_iters) k_scg_p<T><<<vg, NTHR, 0, st>>>(n, p, zz, rz[it & 1], rz[(it + 1) & 1]);
+  }
+}
+
 template <typename T>
 static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, double beta, double *x, int m,
                         double stop_abs, int maxit, int check, Work &w) {
@@ -2147,6 +2344,7 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
         KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
         k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
       }
+      if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj);
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
         KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
@@ -2204,6 +2402,8 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
                          ocmp_solve_stats &st_) {
   OCMP_REQUIRE(P && P->kind == OCMP_PRE_JACOBI, "FGMRES needs a point-Jacobi (or Schur-diagonal) preconditioner");
   OCMP_REQUIRE(!A->dg, "FGMRES is for H1 systems");
+  OCMP_REQUIRE(P->schur_iters == 0 || (P->S && P->sdinv.p && (o.precision != OCMP_MIXED || P->sdinv32.p)),
+               "update the preconditioner after turning the Schur inner solve on");
   Work &w = work(A);
   const int64_t n = A->n;
   ensure_part(w, 0);
@@ -2409,6 +2609,8 @@ void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
 }
 
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st) {
+  OCMP_REQUIRE(!P || P->schur_iters == 0 || o.solver == OCMP_SOLVER_FGMRES,
+               "the Schur inner solve is a nonlinear preconditioner step: it needs FGMRES");
   switch (o.solver) {
     case OCMP_SOLVER_CG: solve_cg(A, P, b, x, o, st); break;
     case OCMP_SOLVER_GMRES: solve_gmres(A, P, b, x, o, st); break;

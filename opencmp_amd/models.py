@@ -40,8 +40,11 @@ class Model:
     (interior penalty coefficient), boundary conditions as dicts
     {marker_name: Fn}, and the [SOLVER] keys (``linear_solver``,
     ``preconditioner``, ``linear_tolerance``, ``linear_max_iterations``) plus the
-    GPU-path keys ``precision`` (fp64|mixed), ``restart`` and ``s_step`` (5: s-step GMRES
-    cycles where the operator allows them, the DG space with element-block Jacobi; 0: standard).
+    GPU-path keys ``precision`` (fp64|mixed), ``restart``, ``s_step`` (5: s-step GMRES
+    cycles where the operator allows them, the DG space with element-block Jacobi; 0: standard)
+    and ``schur_iters`` (saddle-point models with preconditioner 'schur': > 0 solves the pressure
+    rows of every preconditioner step by that many PCG iterations on C + B diag(A)^{-1} B^T, which
+    makes GMRes run as FGMRes).
     """
 
     #: names of the model components in compound-space order
@@ -52,7 +55,7 @@ class Model:
                  dirichlet: Optional[Dict[str, Dict[str, Fn]]] = None, ctx: Optional[C.Context] = None,
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
-                 inner_tol: float = 0.0, s_step: int = 5):
+                 inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0):
         self.ctx = ctx or C.Context.default()
         self.host_mesh = mesh
         self.mesh = C.Mesh(mesh, self.ctx)
@@ -68,8 +71,11 @@ class Model:
             # 'direct' maps to a preconditioned Krylov solve to a tolerance near machine precision
             linear_solver = 'GMRes' if self._indefinite_or_nonsymmetric() else 'CG'
             linear_tolerance = min(linear_tolerance, 1e-13)
-        self.linear_solver = linear_solver
         self.preconditioner = 'local' if preconditioner in ('default', 'local') else preconditioner
+        self.schur_iters = schur_iters if self.preconditioner == 'schur' else 0
+        if self.schur_iters and linear_solver == 'GMRes':
+            linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
+        self.linear_solver = linear_solver
         if self.preconditioner in (None, 'None') and linear_solver == 'CG':
             raise ValueError('Preconditioner cannot be None if using CG or MinRes solvers.')
         self.linear_tolerance = linear_tolerance
@@ -129,7 +135,7 @@ class Model:
             return None
         if self.preconditioner == 'schur':
             return C.Preconditioner(a_assembled, 'schur', precision=self.precision,
-                                    schur_blocks=self.schur_blocks())
+                                    schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters)
         return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
 
     def schur_blocks(self) -> List[int]:

@@ -489,26 +489,32 @@ def _csr_problem(gpu_ctx, problem, precision):
         pre = C.Preconditioner(a, 'local', precision=precision)
         pre.Update()
         return a, f.vec, pre, C.GridFunction(gs)
-    # an implicit-Euler step of Taylor-Hood Stokes (M + dt (kv K + B-coupling), dt = 1e-2) with the
-    # Schur-diagonal preconditioner: the saddle-point system the transient models solve
+    # (GMRES(50) with the diagonal Schur approximation stalls from about 24^2 on)
+    m, a, f, x0 = _stokes_step(gpu_ctx, precision, 8)
+    pre = m.construct_preconditioners(a)
+    pre.Update()
+    return a, f.vec, pre, x0
+
+
+def _stokes_step(gpu_ctx, precision, n, dt=1e-2, schur_iters=0):
+    """An implicit-Euler step of Taylor-Hood Stokes on the unit square (M + dt (kv K + B-coupling))
+    with the Schur preconditioner: the saddle-point system the transient models solve."""
+    C = core()
     from opencmp_amd.models import Stokes, const
-    hm = unit_square(8)   # (GMRES(50) with the diagonal Schur approximation stalls from about 24^2 on)
+    hm = unit_square(n)
     inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
     wall = [const(0.0), const(0.0)]
     m = Stokes(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
                linear_solver='GMRes', preconditioner='schur', linear_tolerance=1e-10, restart=50,
-               precision=precision, ctx=gpu_ctx)
-    dt = 1e-2
+               precision=precision, schur_iters=schur_iters, ctx=gpu_ctx)
     a = C.BilinearForm(m.fes, m.time_derivative_terms() + m.construct_bilinear_time_coefficient(dt) +
                        m.construct_bilinear_time_ODE(dt))
     f = C.LinearForm(m.fes, m.construct_linear(dt))
     a.Assemble()
     f.Assemble()
-    pre = m.construct_preconditioners(a)
-    pre.Update()
     x0 = C.GridFunction(m.fes)
     m.apply_dirichlet_bcs_to(x0)
-    return a, f.vec, pre, x0
+    return m, a, f, x0
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
@@ -534,3 +540,94 @@ def test_s_step_gmres_csr_matches_standard(gpu_ctx, problem, precision):
     x3 = C.GridFunction(x0.space, x0.vec.clone())
     r3 = C.solve(a, b, x3.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=100000, s_step=5)
     assert r3.iters == r2.iters and torch.equal(x3.vec, x2.vec)
+
+
+def _schur_step_reference(A, r0, free, s0, s1):
+    """M^{-1} r0 of the Schur preconditioner with an exact inner solve, from the explicit sparse
+    product: diag(A)^{-1} on the free velocity rows, -(C + B diag(A)^{-1} B^T)^{-1} on the pressure
+    rows (B: pressure rows x free velocity columns; -C: the assembled pressure block)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = A.shape[0]
+    vel = np.zeros(n, bool)
+    vel[:s0] = True
+    vel &= free
+    d = A.diagonal()
+    dinv = np.where(vel, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    B = A[s0:s1][:, :s0].tocsr()
+    Bt = A[:s0][:, s0:s1].tocsr()
+    S = -A[s0:s1][:, s0:s1] + B @ sp.diags(dinv[:s0]) @ Bt
+    z = dinv * r0
+    z[s0:s1] = -spla.spsolve(S.tocsc(), r0[s0:s1])
+    return z, S
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_schur_pcg_step_matches_explicit_schur_complement(gpu_ctx, precision):
+    """One FGMRES iteration from the Dirichlet lift moves x along M^{-1} r0. With enough inner PCG
+    iterations to solve the 81 pressure rows exactly, that direction equals diag(A)^{-1} r0 on the
+    velocity and -(C + B diag(A)^{-1} B^T)^{-1} r0 on the pressure, formed here by scipy's sparse
+    product and direct solve: a wrong entry, sign or index of S changes the direction."""
+    C = core()
+    say('schur pcg step', precision)
+    m, a, f, x0 = _stokes_step(gpu_ctx, precision, 8, schur_iters=120)
+    pre = m.construct_preconditioners(a)
+    pre.Update()
+    assert m.linear_solver == 'FGMRes'
+    fes = m.fes
+    off = fes.block_offsets()
+    pb = fes.comp_blocks[1][0]
+    s0, s1 = off[pb], off[pb + 1]
+    A = a.to_scipy()
+    free = fes.FreeDofs()
+    xl = x0.vec.cpu().numpy()
+    r0 = np.where(free, f.vec.cpu().numpy() - A @ xl, 0.0)
+    z, S = _schur_step_reference(A, r0, free, s0, s1)
+    # S is symmetric positive definite here (the PCG needs that)
+    assert abs(S - S.T).max() <= 1e-12 * abs(S).max()
+    x = C.GridFunction(fes, x0.vec.clone())
+    C.solve(a, f.vec, x.vec, pre, 'FGMRes', tol=1e-10, restart=50, precision=precision, maxit=1)
+    dx = x.vec.cpu().numpy() - xl
+    c = float(dx @ z) / float(z @ z)
+    err = np.linalg.norm(dx - c * z) / np.linalg.norm(c * z)
+    errp = np.linalg.norm(dx[s0:s1] - c * z[s0:s1]) / np.linalg.norm(c * z[s0:s1])
+    say('  scale', c, 'rel diff', err, 'pressure rows', errp)
+    tol = 1e-9 if precision == 'fp64' else 1e-4
+    assert err <= tol and errp <= tol
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_schur_pcg_fgmres_solves_where_the_diagonal_stalls(gpu_ctx, precision):
+    """Stokes implicit-Euler step on 32^2 (dt 1e-3, the transient models' regime): FGMRES with 30
+    inner PCG iterations on the Schur rows reaches 1e-10 in a few dozen iterations and matches
+    scipy's direct solve of the free-dof system; GMRES(50) with the Schur diagonal alone does not
+    get there in 2,000 iterations. The other solvers refuse the nonlinear step."""
+    import scipy.sparse.linalg as spla
+    C = core()
+    say('schur pcg fgmres', precision)
+    m, a, f, x0 = _stokes_step(gpu_ctx, precision, 32, dt=1e-3, schur_iters=30)
+    pre = m.construct_preconditioners(a)
+    pre.Update()
+    x = C.GridFunction(m.fes, x0.vec.clone())
+    r = C.solve(a, f.vec, x.vec, pre, 'FGMRes', tol=1e-10, restart=50, precision=precision, maxit=2000)
+    say('  fgmres + schur pcg', r)
+    assert r.converged and r.rel_residual <= 1e-10 and r.iters <= 150
+    A = a.to_scipy().tocsr()
+    free = m.fes.FreeDofs()
+    xl = x0.vec.cpu().numpy()
+    rhs = f.vec.cpu().numpy() - A @ xl
+    Aff = A[free][:, free].tocsc()
+    ref = xl.copy()
+    ref[free] += spla.spsolve(Aff, rhs[free])
+    rel = np.linalg.norm(x.vec.cpu().numpy() - ref) / np.linalg.norm(ref)
+    say('  rel diff vs direct', rel)
+    assert rel <= 1e-7
+    with pytest.raises(RuntimeError):
+        C.solve(a, f.vec, x.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=10)
+    # the Schur diagonal alone
+    m2, a2, f2, x2 = _stokes_step(gpu_ctx, precision, 32, dt=1e-3)
+    pre2 = m2.construct_preconditioners(a2)
+    pre2.Update()
+    r2 = C.solve(a2, f2.vec, x2.vec, pre2, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=2000)
+    say('  gmres + schur diagonal', r2)
+    assert r2.iters > 5 * r.iters
