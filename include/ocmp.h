@@ -257,15 +257,17 @@ int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_bloc
 /* Schur inner solve for a preconditioner from ocmp_precond_create_schur (same
  * replacement of the reference's direct factorisation of the saddle-point system,
  * opencmp/models/ins.py:178-270 with opencmp/solvers/base_solver.py:711-719):
- * iters > 0 makes the flexible step of OCMP_SOLVER_FGMRES solve the Schur rows
- * with iters Jacobi-PCG iterations on S = C + B diag(A)^{-1} B^T (the sign of the
- * Schur diagonal, i.e. -S^{-1} on those rows) instead of scaling them by the
- * Schur diagonal; 0 turns it off. The Schur blocks must be adjacent in the dof
- * numbering. S's pattern is built at the next ocmp_precond_update and its values
- * at every update, so call ocmp_precond_update after turning it on. The step is
+ * max_iters > 0 makes the flexible step of OCMP_SOLVER_FGMRES solve the Schur
+ * rows with Jacobi-PCG on S = C + B diag(A)^{-1} B^T (the sign of the Schur
+ * diagonal, i.e. -S^{-1} on those rows) instead of scaling them by the Schur
+ * diagonal: at most max_iters iterations, fewer once the preconditioned residual
+ * (r, diag(S)^{-1} r) has fallen by tol^2 (tol 0: always max_iters); 0 turns it
+ * off. The Schur blocks must be adjacent in the dof numbering. S's pattern is
+ * built at the next ocmp_precond_update (ocmp_solve runs one first if none has
+ * run since S was turned on) and its values at every update. The step is
  * nonlinear in its input, so the other solvers refuse such a preconditioner
  * (OCMP_ERR_ARG). */
-int ocmp_precond_set_schur_solve(ocmp_precond *P, int iters);
+int ocmp_precond_set_schur_solve(ocmp_precond *P, int max_iters, double tol);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);
@@ -298,6 +300,7 @@ typedef struct {
                            s-step GMRES: 1 if a block lost definiteness and standard cycles took over */
   double rel_residual;  /* final ||b-Ax||/||b||, FP64, free dofs          */
   double initial_rel_residual;
+  int64_t precond_iters; /* FGMRES with a Schur inner solve: PCG iterations done in all its steps */
 } ocmp_solve_stats;
 
 /* Replaces Model.linear_solve (opencmp/models/base_model.py:886-947): solves

@@ -57,7 +57,7 @@ class SolveOpts(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [('iters', C.c_int32), ('outer', C.c_int32), ('converged', C.c_int32), ('fallback', C.c_int32),
-                ('rel_residual', C.c_double), ('initial_rel_residual', C.c_double)]
+                ('rel_residual', C.c_double), ('initial_rel_residual', C.c_double), ('precond_iters', C.c_int64)]
 
 
 # (name, argtypes) for every entry point declared in include/ocmp.h
@@ -96,7 +96,7 @@ SIGNATURES = {
     'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],
-    'ocmp_precond_set_schur_solve': [P, C.c_int],
+    'ocmp_precond_set_schur_solve': [P, C.c_int, C.c_double],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],

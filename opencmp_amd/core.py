@@ -449,11 +449,12 @@ class Preconditioner:
     KINDS = {'local': None, 'jacobi': L.PRE_JACOBI, 'block_jacobi': L.PRE_BLOCK_JACOBI, 'none': L.PRE_NONE}
 
     def __init__(self, a: BilinearForm, kind: str = 'local', precision: str = 'fp64',
-                 schur_blocks: Sequence[int] = (), schur_iters: int = 0):
+                 schur_blocks: Sequence[int] = (), schur_iters: int = 0, schur_tol: float = 0.0):
         """kind 'schur': point Jacobi on all rows except those of `schur_blocks` (the
         pressure blocks of a saddle-point system), which get the diagonal of the
         Schur complement (ocmp_precond_create_schur); with `schur_iters` > 0 FGMRES
-        solves those rows by that many PCG iterations on C + B diag(A)^{-1} B^T instead
+        solves those rows by at most that many PCG iterations on C + B diag(A)^{-1} B^T
+        instead, fewer once the preconditioned residual has fallen by `schur_tol`
         (ocmp_precond_set_schur_solve)."""
         self.a = a
         self.precision = L.MIXED if precision == 'mixed' else L.FP64
@@ -466,7 +467,8 @@ class Preconditioner:
             L.check(a.ctx.lib.ocmp_precond_create_schur(a.h, self.precision, C.c_uint64(mask), C.byref(h)),
                     'precond_create_schur')
             if schur_iters:
-                L.check(a.ctx.lib.ocmp_precond_set_schur_solve(h, int(schur_iters)), 'precond_set_schur_solve')
+                L.check(a.ctx.lib.ocmp_precond_set_schur_solve(h, int(schur_iters), float(schur_tol)),
+                        'precond_set_schur_solve')
         else:
             k = self.KINDS[kind]
             if k is None:
@@ -493,6 +495,7 @@ class SolveResult:
     rel_residual: float
     initial_rel_residual: float
     fallback: bool = False        # FGMRES: the learned correction was dropped for the plain preconditioner
+    precond_iters: int = 0        # FGMRES with a Schur inner solve: its PCG iterations over all steps
 
 
 def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preconditioner], solver: str = 'CG',
@@ -513,4 +516,4 @@ def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preco
     L.check(a.ctx.lib.ocmp_solve(a.h, pre.h if pre is not None else None, C.c_void_p(_ptr(b)),
                                  C.c_void_p(_ptr(x)), C.byref(o), C.byref(s)), 'solve')
     return SolveResult(s.iters, s.outer, bool(s.converged), s.rel_residual, s.initial_rel_residual,
-                       bool(s.fallback))
+                       bool(s.fallback), int(s.precond_iters))

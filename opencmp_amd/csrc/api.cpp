@@ -1025,12 +1025,15 @@ int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_bloc
   });
 }
 
-int ocmp_precond_set_schur_solve(ocmp_precond *P, int iters) {
+int ocmp_precond_set_schur_solve(ocmp_precond *P, int max_iters, double tol) {
   return guard([&] {
     OCMP_REQUIRE(P, "NULL argument");
     OCMP_REQUIRE(P->P.schur_blocks != 0, "the Schur inner solve needs a Schur preconditioner");
-    OCMP_REQUIRE(iters >= 0 && iters <= 1000, "Schur inner iterations must be in [0, 1000]");
-    P->P.schur_iters = iters;
+    OCMP_REQUIRE(max_iters >= 0 && max_iters <= 1000, "Schur inner iterations must be in [0, 1000]");
+    OCMP_REQUIRE(tol >= 0.0 && tol < 1.0, "Schur inner tolerance must be in [0, 1)");
+    P->P.schur_iters = max_iters;
+    P->P.schur_tol = tol;
+    if (max_iters > 0 && !P->P.S) P->P.ready = false;  // S is built by the next update
   });
 }
 

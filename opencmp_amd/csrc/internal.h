@@ -242,15 +242,19 @@ struct Precond {
   // Schur inner solve (FGMRES only; 0: off): on the Schur rows [s0, s1) the flexible
   // preconditioner takes schur_iters PCG iterations on S = C + B diag(A)^{-1} B^T in place of
   // the Schur-diagonal step. S's pattern is built once from A's, its values at every update.
-  int schur_iters = 0;
+  int schur_iters = 0;     // at most this many PCG iterations per step
+  double schur_tol = 0.0;  // stop when (r, dinv r) <= tol^2 (r0, dinv r0) (0: always schur_iters)
+  int schur_cap = 0;       // with a tolerance: iterations enqueued per step, adapted to those needed
+  double schur_seen = 0.0; // inner iterations of the running solve already accounted for
   int64_t s0 = 0, s1 = 0;
   Matrix *S = nullptr;
   DBuf<int32_t> d_sseg;  // per row of A: first and one-past-last CSR slot of its columns in [s0, s1)
   DBuf<double> sdinv;    // Jacobi of S: 1/S_ii, 0 on rows the Schur diagonal leaves out
   DBuf<float> sdinv32;
-  DBuf<double> svd;      // PCG vectors r z p q (4 ns)
+  DBuf<double> svd;      // PCG vectors r u p s w (5 ns)
   DBuf<float> svf;
-  DBuf<double> spart;    // PCG partial sums
+  DBuf<double> spart;    // PCG scalars and partial sums
+  DBuf<int> sflag;       // converged flag of the running inner solve
   Precond() = default;
   Precond(const Precond &) = delete;
   Precond &operator=(const Precond &) = delete;

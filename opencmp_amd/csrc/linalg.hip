@@ -23,6 +23,7 @@ constexpr int NTHR = 256;
 constexpr int NXCD = 8;
 constexpr int MAX_RED_BLOCKS = 2048;
 constexpr int STREAM_BLOCKS = 1024;  // grid of the streaming reduction kernels (4 per CU)
+constexpr int SCHUR_SCAL = 8;        // scalars at the head of the Schur inner solve's partial-sum buffer
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
 
@@ -174,7 +175,8 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
                                                    const uint8_t *__restrict__ mask, const XT *__restrict__ cs,
                                                    const double *__restrict__ sx, int mode,
                                                    double *__restrict__ part, XT dsh, const XT *__restrict__ xp,
-                                                   XT csh) {
+                                                   XT csh, const int *__restrict__ skip) {
+  if (skip && *skip) return;  // a converged inner solve: the rest of its fixed launch sequence is empty
   // one LDS buffer: first the x values at the distinct columns, then (after every thread has
   // read its operands) the products, then the block-sum scratch, so the kernel's LDS is
   // exactly NNZB values (a power of two: whole blocks per CU)
@@ -285,17 +287,17 @@ static int spmv_grid(Matrix *A) { return spmv_grid<double, double>(A); }
 template <typename VT, typename XT>
 static void spmv_dispatch(Matrix *A, const VT *val, const XT *x, XT *y, const double *b, const uint8_t *mask,
                           const XT *cs, const double *sx, int mode, double *part, int grid, hipStream_t st,
-                          XT dsh = (XT)0, const XT *xp = nullptr, XT csh = (XT)0) {
+                          XT dsh = (XT)0, const XT *xp = nullptr, XT csh = (XT)0, const int *skip = nullptr) {
   const RowBlocks &rb = row_blocks<VT, XT>(A);
   OCMP_REQUIRE((grid == spmv_grid<VT, XT>(A)), "SpMV grid does not match the row blocks");
   if (rb.nnzb == SPMV_NNZB32)
     k_spmv_csr<VT, XT, SPMV_NNZB32><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part, dsh, xp, csh);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip);
   else
     k_spmv_csr<VT, XT, SPMV_NNZB64><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part, dsh, xp, csh);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip);
 }
 
 // algorithmic bytes of one CSR SpMV: values + 16-bit local columns + the blocks' distinct
@@ -1173,16 +1175,18 @@ void precond_update(Precond *P) {
         P->sdinv.alloc(ns);
         if (mixed) P->sdinv32.alloc(ns);
         k_schur_sdinv<<<nblk(ns), NTHR, 0, st>>>(ns, P->dinv.p + P->s0, P->sdinv.p, mixed ? P->sdinv32.p : nullptr);
+        // S is small (a pressure space): its FP32 SpMV keeps the 2048-nonzero row blocks for more
+        // blocks per launch
         if (mixed) {
           S->d_val32.alloc(S->nnz);
           k_to_f32<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, S->d_val.p, S->d_val32.p);
-          ensure_row_blocks32(S);
-          P->svf.alloc((size_t)4 * ns);
+          P->svf.alloc((size_t)5 * ns);
         } else {
-          P->svd.alloc((size_t)4 * ns);
+          P->svd.alloc((size_t)5 * ns);
         }
         const int sg = std::max(spmv_grid<double, double>(S), spmv_grid<float, float>(S));
-        P->spart.alloc(std::max(P->spart.n, (size_t)2 * red_grid(ns) + sg));
+        P->spart.alloc(std::max(P->spart.n, (size_t)SCHUR_SCAL + 2 * red_grid(ns) + sg));
+        P->sflag.alloc(1);
       }
     }
     if (mixed) {
@@ -2161,11 +2165,21 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
 // x += Z y. Same device-side Gram-Schmidt / Givens machinery as gmres_cycle.
 // ------------------------------------------------------------------ Schur inner PCG
 //
-// Fixed-count Jacobi PCG on S for the Schur rows of an FGMRES preconditioner step. Each block
-// sums the partial dot products it needs itself (the same order in every block, so every block
-// gets the same alpha and beta), which leaves three launches per iteration: SpMV (+ p.q
-// partials), the x/r/z update (+ r.z partials) and the search-direction update. The r.z
-// partials alternate between two buffers by iteration. A zero denominator gives a zero step.
+// Jacobi PCG on S for the Schur rows of an FGMRES preconditioner step, in the single-reduction
+// form of Chronopoulos and Gear (J. Comput. Appl. Math. 25 (1989) 153-168): w = S u gives
+// delta = (u, w) in the SpMV's epilogue, the previous vector update gave gamma = (r, u), and with
+// beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old) one fused kernel
+// updates p = u + beta p, s = w + beta s (= S p), x += alpha p, r -= alpha s, u = dinv r. Two
+// launches per iteration. Each block sums the partial dot products it needs itself (the same
+// order in every block, so all blocks use the same alpha and beta); gamma partials and the
+// (gamma, alpha) pair of the previous iteration alternate between two slots by iteration. A zero
+// denominator gives a zero step.
+// The host enqueues max_iters iterations without waiting. With a tolerance, the step kernel that
+// finds gamma <= tol^2 gamma_0 sets a device flag instead of updating, and every later launch of
+// the sequence (SpMV included) returns on reading it.
+//
+// Scalars (sc): [0..3] (gamma, alpha) slots, [4] gamma_0, [5] iterations done (a running count
+// over the solve).
 
 __device__ __forceinline__ double block_sum_all(double v, double *sh) {
   v = block_sum(v, sh);
@@ -2182,11 +2196,12 @@ __device__ __forceinline__ double sum_parts(const double *__restrict__ part, int
   return block_sum_all(s, sh);
 }
 
-// x = 0, r = -s v, z = dinv r, p = z; partials of r.z
+// x = 0, r = -s v, u = dinv r; partials of r.u; clears the converged flag
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restrict__ v, const double *__restrict__ sv,
                                                    const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
-                                                   T *__restrict__ z, T *__restrict__ p, double *__restrict__ rz) {
+                                                   T *__restrict__ u, double *__restrict__ gpart,
+                                                   int *__restrict__ flag) {
   __shared__ double sh[NTHR / 64];
   const int lb = xcd_block();
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -2197,60 +2212,73 @@ __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restric
     x[i] = (T)0;
     const T ri = (T)((double)v[i] * sc);
     r[i] = ri;
-    const T zi = dinv[i] * ri;
-    z[i] = zi;
-    p[i] = zi;
-    acc += (double)ri * (double)zi;
+    const T ui = dinv[i] * ri;
+    u[i] = ui;
+    acc += (double)ri * (double)ui;
   }
   acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) rz[lb] = acc;
+  if (threadIdx.x == 0) {
+    gpart[lb] = acc;
+    if (blockIdx.x == 0) flag[0] = 0;
+  }
 }
 
-// alpha = r.z / p.q; x += alpha p; r -= alpha q; z = dinv r; partials of the new r.z
 template <typename T>
-__global__ void __launch_bounds__(NTHR) k_scg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
-                                                     T *__restrict__ z, const T *__restrict__ p,
-                                                     const T *__restrict__ q, const T *__restrict__ dinv,
-                                                     const double *__restrict__ pq_part, int npq,
-                                                     const double *__restrict__ rz_old, double *__restrict__ rz_new) {
+__global__ void __launch_bounds__(NTHR) k_scg_step(int64_t n, int it, double tol2, T *__restrict__ x,
+                                                   T *__restrict__ r, T *__restrict__ u, T *__restrict__ p,
+                                                   T *__restrict__ s, const T *__restrict__ w,
+                                                   const T *__restrict__ dinv, const double *__restrict__ gpart,
+                                                   const double *__restrict__ dpart, int nd, double *__restrict__ gnew,
+                                                   double *__restrict__ sc, int *__restrict__ flag) {
   __shared__ double sh[NTHR / 64];
-  const double pq = sum_parts(pq_part, npq, sh);
-  const double rz = sum_parts(rz_old, gridDim.x, sh);
-  const T alpha = (T)(pq != 0.0 ? rz / pq : 0.0);
+  if (*flag) return;
+  const double g = sum_parts(gpart, gridDim.x, sh);
+  const double g0 = it == 0 ? g : sc[4];
+  if (!(g > tol2 * g0)) {  // converged (or a zero right-hand side): the same decision in every block
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 1;
+    return;
+  }
+  const double d = sum_parts(dpart, nd, sh);
+  double beta = 0.0, den = d;
+  if (it > 0) {
+    const double gp = sc[2 * ((it + 1) & 1)], ap = sc[2 * ((it + 1) & 1) + 1];
+    beta = gp != 0.0 ? g / gp : 0.0;
+    if (ap != 0.0) den = d - beta * g / ap;
+  }
+  const double alpha = den != 0.0 ? g / den : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[2 * (it & 1)] = g;
+    sc[2 * (it & 1) + 1] = alpha;
+    if (it == 0) sc[4] = g;
+    sc[5] += 1.0;
+  }
+  const T a = (T)alpha, b = (T)beta;
   const int lb = xcd_block();
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
   double acc = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - alpha * q[i];
+    T pi = u[i], si = w[i];
+    if (it > 0) {
+      pi += b * p[i];
+      si += b * s[i];
+    }
+    p[i] = pi;
+    s[i] = si;
+    x[i] += a * pi;
+    const T ri = r[i] - a * si;
     r[i] = ri;
-    const T zi = dinv[i] * ri;
-    z[i] = zi;
-    acc += (double)ri * (double)zi;
+    const T ui = dinv[i] * ri;
+    u[i] = ui;
+    acc += (double)ri * (double)ui;
   }
   acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) rz_new[lb] = acc;
+  if (threadIdx.x == 0) gnew[lb] = acc;
 }
 
-// p = z + beta p, beta = new r.z / old r.z
+// z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero
 template <typename T>
-__global__ void __launch_bounds__(NTHR) k_scg_p(int64_t n, T *__restrict__ p, const T *__restrict__ z,
-                                                const double *__restrict__ rz_old,
-                                                const double *__restrict__ rz_new) {
-  __shared__ double sh[NTHR / 64];
-  const double o = sum_parts(rz_old, gridDim.x, sh);
-  const double w = sum_parts(rz_new, gridDim.x, sh);
-  const T beta = (T)(o != 0.0 ? w / o : 0.0);
-  const int lb = xcd_block();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) p[i] = z[i] + beta * p[i];
-}
-
-// z[s0:s1] = -S^{-1} (s v)[s0:s1] by P->schur_iters PCG iterations from zero
-template <typename T>
-static void schur_inner(Precond *P, const T *v, const double *sv, T *z) {
+static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
   Matrix *S = P->S;
   Ctx *ctx = S->ctx;
   hipStream_t st = ctx->stream;
@@ -2266,23 +2294,29 @@ static void schur_inner(Precond *P, const T *v, const double *sv, T *z) {
     dinv = P->sdinv32.p;
     buf = P->svf.p;
   }
-  T *x = z + P->s0, *r = buf, *zz = r + n, *p = zz + n, *q = p + n;
-  const int vg = red_grid(n), sg = spmv_grid<T, T>(S);
-  double *rz[2] = {P->spart.p, P->spart.p + vg};
-  double *pq = P->spart.p + 2 * vg;
+  T *x = z + P->s0, *r = buf, *u = r + n, *p = u + n, *s = p + n, *w = s + n;
+  // the vector kernels run at most 256 blocks (a few rows per thread): every block also sums all
+  // the partials, so fewer blocks means fewer redundant partial loads
+  const int vg = std::min(red_grid(n), 256), sg = spmv_grid<T, T>(S);
+  double *sc = P->spart.p;
+  double *gp[2] = {sc + SCHUR_SCAL, sc + SCHUR_SCAL + vg};
+  double *dp = sc + SCHUR_SCAL + 2 * vg;
+  int *flag = P->sflag.p;
+  const double tol2 = P->schur_tol * P->schur_tol;
   const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_schur_f64" : "spmv_schur_f32");
-  const int c_upd = ctx->cls(sizeof(T) == 8 ? "schur_pcg_update_f64" : "schur_pcg_update_f32");
-  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r, zz, p, rz[0]);
-  for (int it = 0; it < P->schur_iters; ++it) {
+  const int c_upd = ctx->cls(sizeof(T) == 8 ? "schur_pcg_step_f64" : "schur_pcg_step_f32");
+  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r, u, gp[0], flag);
+  for (int it = 0; it < K; ++it) {
     {
       KTimer tm(ctx, c_op, csr_bytes(S, sizeof(T), sizeof(T)));
-      spmv_dispatch<T, T>(S, val, p, q, nullptr, nullptr, nullptr, nullptr, 2, pq, sg, st);
+      spmv_dispatch<T, T>(S, val, u, w, nullptr, nullptr, nullptr, nullptr, 2, dp, sg, st, (T)0, nullptr, (T)0,
+                          flag);
     }
     {
-      KTimer tm(ctx, c_upd, 7.0 * n * sizeof(T));
-      k_scg_update<T><<<vg, NTHR, 0, st>>>(n, x, r, zz, p, q, dinv, pq, sg, rz[it & 1], rz[(it + 1) & 1]);
+      KTimer tm(ctx, c_upd, 12.0 * n * sizeof(T));
+      k_scg_step<T><<<vg, NTHR, 0, st>>>(n, it, tol2, x, r, u, p, s, w, dinv, gp[it & 1], dp, sg,
+                                         gp[(it + 1) & 1], sc, flag);
     }
-    if (it + 1 < P->schur_iters) k_scg_p<T><<<vg, NTHR, 0, st>>>(n, p, zz, rz[it & 1], rz[(it + 1) & 1]);
   }
 }
 
@@ -2330,12 +2364,17 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
   const int limit = std::min(m, maxit);
   GState G{dot, vs, R, csn, g, res, m, beta};
   struct Batch {
-    int j0, nb, ev;
+    int j0, nb, ev, cap;
   };
   std::deque<Batch> inflight;
   int j = 0, bidx = 0, k_used = -1;
+  // Schur inner solve with a tolerance: each step enqueues schur_cap iterations (the rest of a
+  // converged inner solve are empty launches); after each batch the host reads how many ran and
+  // sets the cap to 1.3x that plus 10, doubling it when a step used them all
+  const bool adapt = P->schur_iters > 0 && P->schur_tol > 0.0;
   auto enqueue = [&]() {
     const int j_end = std::min(limit, j + check);
+    const int cap = P->schur_iters > 0 ? (adapt ? P->schur_cap : P->schur_iters) : 0;
     for (int jj = j; jj < j_end; ++jj) {
       T *vj = V + (int64_t)jj * ldv;
       T *zj = Z + (int64_t)jj * ldv;
@@ -2344,7 +2383,7 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
         KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
         k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
       }
-      if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj);
+      if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj, cap);
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
         KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
@@ -2365,8 +2404,10 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
     const int nb = j_end - j;
     OCMP_HIP(hipMemcpyAsync(hres + j, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));
     const int ev = bidx & 1;
+    if (adapt)
+      OCMP_HIP(hipMemcpyAsync(hres + m + 2 + ev, P->spart.p + 5, sizeof(double), hipMemcpyDeviceToHost, st));
     OCMP_HIP(hipEventRecord(work_event(w, ev), st));
-    inflight.push_back({j, nb, ev});
+    inflight.push_back({j, nb, ev, cap});
     ++bidx;
     j = j_end;
   };
@@ -2379,6 +2420,13 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
     Batch b = inflight.front();
     inflight.pop_front();
     OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
+    if (adapt) {
+      const double cnt = hres[m + 2 + b.ev];
+      const double per = (cnt - P->schur_seen) / b.nb;
+      P->schur_seen = cnt;
+      P->schur_cap = per >= b.cap - 0.5 ? std::min(P->schur_iters, 2 * b.cap)
+                                         : std::min(P->schur_iters, (int)(1.3 * per) + 10);
+    }
     for (int s = 0; s < b.nb; ++s)
       if (!(hres[b.j0 + s] > stop_abs)) {
         k_used = b.j0 + s + 1;
@@ -2404,6 +2452,11 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   OCMP_REQUIRE(!A->dg, "FGMRES is for H1 systems");
   OCMP_REQUIRE(P->schur_iters == 0 || (P->S && P->sdinv.p && (o.precision != OCMP_MIXED || P->sdinv32.p)),
                "update the preconditioner after turning the Schur inner solve on");
+  if (P->schur_iters > 0) {
+    k_fill_t<double><<<1, 64, 0, A->ctx->stream>>>(P->spart.p + 5, 1, 0.0);
+    P->schur_seen = 0.0;
+    if (P->schur_cap <= 0 || P->schur_cap > P->schur_iters) P->schur_cap = P->schur_iters;
+  }
   Work &w = work(A);
   const int64_t n = A->n;
   ensure_part(w, 0);
@@ -2415,7 +2468,9 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
   st_.initial_rel_residual = rn / bref;
   st_.rel_residual = rn / bref;
-  const int check = o.check_every > 0 ? o.check_every : 5;
+  // with a Schur inner solve a step costs milliseconds: poll every iteration so that at most a
+  // step or two run past convergence
+  const int check = o.check_every > 0 ? o.check_every : (P->schur_iters > 0 ? 1 : 5);
   const int m = o.restart;
   int total = 0, outer = 0;
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
@@ -2458,6 +2513,12 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   st_.outer = outer;
   st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
   st_.fallback = have_learned && fell_back ? 1 : 0;
+  if (P->schur_iters > 0) {
+    double cnt = 0.0;
+    OCMP_HIP(hipMemcpyAsync(&cnt, P->spart.p + 5, sizeof(double), hipMemcpyDeviceToHost, A->ctx->stream));
+    OCMP_HIP(hipStreamSynchronize(A->ctx->stream));
+    st_.precond_iters = (int64_t)cnt;
+  }
 }
 
 static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o,

@@ -42,8 +42,9 @@ class Model:
     ``preconditioner``, ``linear_tolerance``, ``linear_max_iterations``) plus the
     GPU-path keys ``precision`` (fp64|mixed), ``restart``, ``s_step`` (5: s-step GMRES
     cycles where the operator allows them, the DG space with element-block Jacobi; 0: standard)
-    and ``schur_iters`` (saddle-point models with preconditioner 'schur': > 0 solves the pressure
-    rows of every preconditioner step by that many PCG iterations on C + B diag(A)^{-1} B^T, which
+    and ``schur_iters`` / ``schur_tol`` (saddle-point models with preconditioner 'schur': > 0
+    solves the pressure rows of every preconditioner step by at most that many PCG iterations on
+    C + B diag(A)^{-1} B^T, fewer once the preconditioned residual has fallen by schur_tol; this
     makes GMRes run as FGMRes).
     """
 
@@ -55,7 +56,7 @@ class Model:
                  dirichlet: Optional[Dict[str, Dict[str, Fn]]] = None, ctx: Optional[C.Context] = None,
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
-                 inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0):
+                 inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0):
         self.ctx = ctx or C.Context.default()
         self.host_mesh = mesh
         self.mesh = C.Mesh(mesh, self.ctx)
@@ -73,6 +74,7 @@ class Model:
             linear_tolerance = min(linear_tolerance, 1e-13)
         self.preconditioner = 'local' if preconditioner in ('default', 'local') else preconditioner
         self.schur_iters = schur_iters if self.preconditioner == 'schur' else 0
+        self.schur_tol = schur_tol
         if self.schur_iters and linear_solver == 'GMRes':
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
         self.linear_solver = linear_solver
@@ -135,7 +137,8 @@ class Model:
             return None
         if self.preconditioner == 'schur':
             return C.Preconditioner(a_assembled, 'schur', precision=self.precision,
-                                    schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters)
+                                    schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters,
+                                    schur_tol=self.schur_tol)
         return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
 
     def schur_blocks(self) -> List[int]:
@@ -337,6 +340,7 @@ class INS(Stokes):
         it = 1
         res = None
         self.linear_iterations = 0
+        self.precond_iterations = 0   # inner PCG iterations of a Schur inner solve
         self.picard_log = []      # (iterations, refinement steps, initial rel residual, Picard error) per iteration
         while True:
             self.apply_dirichlet_bcs_to(gfu)
@@ -344,6 +348,7 @@ class INS(Stokes):
                 reassemble()
             res = self.linear_solve(a, Lf, pre, gfu)
             self.linear_iterations += res.iters
+            self.precond_iterations += res.precond_iters
             err = self.l2_diff(self.W, gfu, 'u')
             self.picard_log.append((res.iters, res.outer, float('%.3g' % res.initial_rel_residual), float('%.3g' % err)))
             unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')

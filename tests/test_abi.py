@@ -38,6 +38,6 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.FeDesc) == 24
     assert ctypes.sizeof(_lib.Field) == 24
     assert ctypes.sizeof(_lib.SolveOpts) == 48
-    assert ctypes.sizeof(_lib.SolveStats) == 32
+    assert ctypes.sizeof(_lib.SolveStats) == 40
     assert ctypes.sizeof(_lib.Term) == 104
     assert ctypes.sizeof(_lib.LTerm) == 160

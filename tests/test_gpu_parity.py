@@ -496,7 +496,7 @@ def _csr_problem(gpu_ctx, problem, precision):
     return a, f.vec, pre, x0
 
 
-def _stokes_step(gpu_ctx, precision, n, dt=1e-2, schur_iters=0):
+def _stokes_step(gpu_ctx, precision, n, dt=1e-2, schur_iters=0, schur_tol=0.0):
     """An implicit-Euler step of Taylor-Hood Stokes on the unit square (M + dt (kv K + B-coupling))
     with the Schur preconditioner: the saddle-point system the transient models solve."""
     C = core()
@@ -506,7 +506,7 @@ def _stokes_step(gpu_ctx, precision, n, dt=1e-2, schur_iters=0):
     wall = [const(0.0), const(0.0)]
     m = Stokes(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
                linear_solver='GMRes', preconditioner='schur', linear_tolerance=1e-10, restart=50,
-               precision=precision, schur_iters=schur_iters, ctx=gpu_ctx)
+               precision=precision, schur_iters=schur_iters, schur_tol=schur_tol, ctx=gpu_ctx)
     a = C.BilinearForm(m.fes, m.time_derivative_terms() + m.construct_bilinear_time_coefficient(dt) +
                        m.construct_bilinear_time_ODE(dt))
     f = C.LinearForm(m.fes, m.construct_linear(dt))
@@ -622,8 +622,19 @@ def test_schur_pcg_fgmres_solves_where_the_diagonal_stalls(gpu_ctx, precision):
     rel = np.linalg.norm(x.vec.cpu().numpy() - ref) / np.linalg.norm(ref)
     say('  rel diff vs direct', rel)
     assert rel <= 1e-7
+    # no tolerance: every step runs all 30 (a step or two enqueued past convergence included)
+    assert r.precond_iters % 30 == 0 and 30 * r.iters <= r.precond_iters <= 30 * (r.iters + 2 * r.outer)
     with pytest.raises(RuntimeError):
         C.solve(a, f.vec, x.vec, pre, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=10)
+    # with a tolerance the inner solves stop early: the same answer from fewer PCG iterations
+    m3, a3, f3, x3 = _stokes_step(gpu_ctx, precision, 32, dt=1e-3, schur_iters=200, schur_tol=1e-2)
+    pre3 = m3.construct_preconditioners(a3)
+    pre3.Update()
+    r3 = C.solve(a3, f3.vec, x3.vec, pre3, 'FGMRes', tol=1e-10, restart=50, precision=precision, maxit=2000)
+    say('  fgmres + schur pcg to 1e-2', r3, 'mean inner', r3.precond_iters / max(r3.iters, 1))
+    assert r3.converged and 0 < r3.precond_iters < 200 * r3.iters
+    rel3 = np.linalg.norm(x3.vec.cpu().numpy() - ref) / np.linalg.norm(ref)
+    assert rel3 <= 1e-7
     # the Schur diagonal alone
     m2, a2, f2, x2 = _stokes_step(gpu_ctx, precision, 32, dt=1e-3)
     pre2 = m2.construct_preconditioners(a2)
