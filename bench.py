@@ -226,7 +226,9 @@ def bench_learned(args, ctx):
 
     def model(precision, s_step=5):
         # the plain FP64 path is standard GMRES (s_step 0), as in the other configs
+        # do-nothing outlet: the stress condition with zero traction, which carries the no-backflow term
         return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
+                   stress={'u': {'outlet': zero}},
                    linear_solver='GMRes', preconditioner='schur', linear_tolerance=args.tol, restart=args.restart,
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
                    inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step,
@@ -300,7 +302,8 @@ def bench_learned(args, ctx):
     _, _, _, _, rep = branch('mixed', True, timing=True)
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
     return dict(desc=(f'transient INS Taylor-Hood P2/P1, implicit Euler dt 1e-3 + Oseen Picard (<=3, tol 1e-6), '
-                      f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), ' +
+                      f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), do-nothing (stress) outlet '
+                      f'with the no-backflow term, ' +
                       (f'FGMRES({args.restart}) + Schur preconditioner with inner Jacobi-PCG on '
                        f'C + B diag(A)^-1 B^T (to {args.schur_tol:g}, at most {args.schur_iters} iterations)'
                        if args.schur_iters else

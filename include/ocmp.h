@@ -120,6 +120,9 @@ enum {
   OCMP_BFI_CONV = 2,     /* coef * w * u (wind . grad v)                       */
   OCMP_BFI_DTRIAL = 3,   /* coef * w * d_dir u * v                             */
   OCMP_BFI_DTEST = 4,    /* coef * w * u * d_dir v                             */
+  OCMP_BFI_BND_UPWIND = 5, /* coef * max(wind.n, 0) * u v on boundary facets whose marker is in
+                              dirichlet_mask (triangles): the no-backflow part of the stress
+                              boundary condition, opencmp/models/ins.py:210-216 */
   OCMP_BFI_DG_FACET = 16 /* interior + boundary facet terms of an L2 block, see below */
 };
 

@@ -227,6 +227,17 @@ class Vol:
 
 
 @dataclass
+class BndUpwind:
+    """coef * max(w.n, 0) * u v on the boundary facets whose marker is in `markers` (bit mask), one
+    H1 block (trial == test): the no-backflow part of the stress boundary condition."""
+    block: int
+    markers: int
+    coef: float = 1.0
+    wind: Tuple[float, ...] = (0.0, 0.0, 0.0)
+    wind_gf: Optional[Tuple['GridFunction', Sequence[int]]] = None   # (gf, blocks per dimension)
+
+
+@dataclass
 class DGFacet:
     """Interior-penalty + upwind skeleton terms of an L2 block."""
     block: int
@@ -310,7 +321,20 @@ class BilinearForm:
             tt = L.Term()
             tt.wind_field = -1
             tt.weight_field = -1
-            if isinstance(t, DGFacet):
+            if isinstance(t, BndUpwind):
+                tt.kind = L.BFI_BND_UPWIND
+                tt.trial = tt.test = t.block
+                tt.coef = t.coef
+                tt.dirichlet_mask = t.markers
+                for c in range(min(3, len(t.wind))):
+                    tt.wind[c] = t.wind[c]
+                if t.wind_gf is not None:
+                    gf, blocks = t.wind_gf
+                    idx = [ft.index(gf, b) for b in blocks]
+                    if idx != list(range(idx[0], idx[0] + len(idx))):
+                        raise ValueError('wind components must be consecutive fields')
+                    tt.wind_field = idx[0]
+            elif isinstance(t, DGFacet):
                 tt.kind = L.BFI_DG_FACET
                 tt.trial = tt.test = t.block
                 tt.coef = t.coef
@@ -349,7 +373,7 @@ class BilinearForm:
         ft = _FieldTable()
         # register wind fields first so each wind gets consecutive indices
         for t in self.terms:
-            if isinstance(t, Vol) and t.wind_gf is not None:
+            if isinstance(t, (Vol, BndUpwind)) and t.wind_gf is not None:
                 gf, blocks = t.wind_gf
                 for b in blocks:
                     ft.index(gf, b)

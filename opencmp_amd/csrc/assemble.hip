@@ -45,6 +45,7 @@ struct KField {
   int loc_off;           // block offset within the element dof list
   int nl;                // block local dofs
   int tab_phi;           // offset of phi table (nq x nl) at the volume rule
+  int tab_fphi;          // boundary terms: phi at the facet rule ((dim+1) x nqf x nl)
 };
 
 struct KBlocks {
@@ -53,6 +54,7 @@ struct KBlocks {
   int nloc[MAXB];
   int tab_phi[MAXB];   // volume rule: phi (nq x nl)
   int tab_grad[MAXB];  // volume rule: reference gradients (nq x nl x dim)
+  int tab_fphi[MAXB];  // boundary terms: phi at the facet rule of each local facet ((dim+1) x nqf x nl)
 };
 
 struct KArgs {
@@ -63,6 +65,10 @@ struct KArgs {
   int tab_w;   // quadrature weights of the volume rule
   int tab_lam; // barycentric points (nq x (dim+1))
   const double *tab;
+  // boundary terms: facet rule on [0, 1] (nqf points, weights at tab_fw) and the element-facet
+  // neighbours (-(marker+1) on the boundary)
+  int nqf, tab_fw;
+  const int32_t *elem_nb;
 };
 
 // ------------------------------------------------------------------ geometry
@@ -142,6 +148,64 @@ __device__ __forceinline__ double field_at(const KField &f, const double *__rest
   return s;
 }
 
+// value of a field at facet-rule point q of local facet k of element e
+__device__ __forceinline__ double field_at_facet(const KField &f, const double *__restrict__ tab, int64_t e, int k,
+                                                 int q, int nqf) {
+  const double *phi = tab + f.tab_fphi + (k * nqf + q) * f.nl;
+  const int32_t *ed = f.edofs + e * f.ld + f.loc_off;
+  double s = 0.0;
+  for (int l = 0; l < f.nl; ++l) s += phi[l] * f.vec[ed[l]];
+  return s;
+}
+
+// Row il (block bi) of coef * max(w.n, 0) u v over the element's boundary facets whose marker is
+// in T.dmask (triangles): the no-backflow part of the stress boundary condition. Facet k is the
+// edge opposite vertex k, n its outward unit normal, the rule maps [0, 1] onto the edge.
+template <int D>
+__device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
+                               double ce, double *r) {
+  if constexpr (D == 2) {
+    const int nlt = A.b.nloc[bi], nlj = A.b.nloc[bj];
+    double P[3][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      P[0][c] = g.x0[c];
+      P[1][c] = g.x0[c] + g.J[c][0];
+      P[2][c] = g.x0[c] + g.J[c][1];
+    }
+    const double *fw = A.tab + A.tab_fw;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int32_t nb = A.elem_nb[e * 3 + k];
+      if (nb >= 0) continue;
+      const int marker = -nb - 1;
+      if (marker >= 64 || !((T.dmask >> marker) & 1ull)) continue;
+      // edge k runs from vertex fa to vertex fb: (1, 2), (0, 2), (0, 1)
+      const int fa = k == 0 ? 1 : 0, fb = k == 2 ? 1 : 2;
+      const double tx = P[fb][0] - P[fa][0], ty = P[fb][1] - P[fa][1];
+      const double len = sqrt(tx * tx + ty * ty);
+      double nx = ty / len, ny = -tx / len;
+      if ((P[k][0] - P[fa][0]) * nx + (P[k][1] - P[fa][1]) * ny > 0.0) {
+        nx = -nx;
+        ny = -ny;
+      }
+      const double *phit = A.tab + A.b.tab_fphi[bi] + (size_t)k * A.nqf * nlt;
+      const double *phij = A.tab + A.b.tab_fphi[bj] + (size_t)k * A.nqf * nlj;
+      for (int q = 0; q < A.nqf; ++q) {
+        double wx = T.wind[0], wy = T.wind[1];
+        if (T.wind_field >= 0) {
+          wx = field_at_facet(A.f[T.wind_field], A.tab, e, k, q, A.nqf);
+          wy = field_at_facet(A.f[T.wind_field + 1], A.tab, e, k, q, A.nqf);
+        }
+        const double wn = wx * nx + wy * ny;
+        if (!(wn > 0.0)) continue;
+        const double c = ce * fw[q] * len * wn * phit[q * nlt + il];
+        for (int j = 0; j < nlj; ++j) r[j] += c * phij[q * nlj + j];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ H1 row kernel
 
 template <int D>
@@ -162,6 +226,10 @@ __device__ void h1_row(const KArgs &A, const double *__restrict__ coords, const 
     const int nlj = A.b.nloc[bj];
     double *r = row + A.b.loc_off[bj];
     double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
+    if (T.kind == OCMP_BFI_BND_UPWIND) {
+      bnd_upwind_row<D>(A, T, g, e, bi, il, bj, ce, r);
+      continue;
+    }
     if (!T.quad) {
       // reference-tensor contraction
       double fac[9];
@@ -826,6 +894,32 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
   fill_block_tables(tb, s, rule, A.b);
   A.tab_w = tb.add(rule.w);
   A.tab_lam = tb.add(rule.lam);
+  bool bnd = false;
+  for (auto &t : terms) bnd |= t.kind == OCMP_BFI_BND_UPWIND;
+  // facet rule for boundary terms: exact for the product of three functions of the highest order
+  std::vector<double> ft, fwt;
+  if (bnd) {
+    OCMP_REQUIRE(d == 2, "boundary-facet terms are implemented on triangles");
+    line_rule(2 * max_order(s) + std::max(pf, 1), ft, fwt);
+    A.nqf = (int)ft.size();
+    A.tab_fw = tb.add(fwt);
+    A.elem_nb = s.mesh->d_elem_nb.p;
+  }
+  // phi of a (kind, order) block at the facet rule of the three edges (edge k opposite vertex k)
+  auto facet_phi = [&](int kind, int order, int nl) {
+    std::vector<double> phi((size_t)3 * A.nqf * nl), gr((size_t)3 * A.nqf * nl * 2);
+    const int fa[3] = {1, 0, 0}, fb[3] = {2, 2, 1};
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < A.nqf; ++q) {
+        double lam[3] = {0, 0, 0};
+        lam[fa[k]] = 1.0 - ft[q];
+        lam[fb[k]] = ft[q];
+        eval_basis(kind, 2, order, lam, &phi[((size_t)k * A.nqf + q) * nl], &gr[((size_t)k * A.nqf + q) * nl * 2]);
+      }
+    return tb.add(phi);
+  };
+  if (bnd)
+    for (int b = 0; b < A.b.n; ++b) A.b.tab_fphi[b] = facet_phi(s.blocks[b].kind, s.blocks[b].order, s.blocks[b].nloc);
   for (size_t i = 0; i < fields.size(); ++i) {
     const Space &fs = fields[i].space->s;
     const Block &fb = fs.blocks[fields[i].block];
@@ -840,6 +934,7 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
       eval_basis(fb.kind, d, fb.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * fb.nloc],
                  &gr[(size_t)q * fb.nloc * d]);
     kf.tab_phi = tb.add(phi);
+    if (bnd) kf.tab_fphi = facet_phi(fb.kind, fb.order, fb.nloc);
   }
   for (size_t i = 0; i < terms.size(); ++i) {
     const ocmp_term &t = terms[i];
@@ -862,6 +957,10 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
     k.dmask = t.dirichlet_mask;
     if (t.kind == OCMP_BFI_DG_FACET) {
       k.test = -1;  // never matches a row block in the volume pass
+      continue;
+    }
+    if (t.kind == OCMP_BFI_BND_UPWIND) {
+      OCMP_REQUIRE(t.wind_field < 0 || t.wind_field + 1 < (int)fields.size(), "wind field out of range");
       continue;
     }
     if (!k.quad) k.tab = add_tensors(tb, s, t, k.ntens);

@@ -240,8 +240,9 @@ struct Precond {
   DBuf<double> aell;     // n*4*nb
   bool ready = false;
   // Schur inner solve (FGMRES only; 0: off): on the Schur rows [s0, s1) the flexible
-  // preconditioner takes schur_iters PCG iterations on S = C + B diag(A)^{-1} B^T in place of
-  // the Schur-diagonal step. S's pattern is built once from A's, its values at every update.
+  // preconditioner takes schur_iters PCG iterations on S = C + B D^{-1} B^T in place of the
+  // Schur-diagonal step, D_uu = max(|a_uu|, 1e-2 max_j |a_uj|). S's pattern is built once from
+  // A's, its values at every update.
   int schur_iters = 0;     // at most this many PCG iterations per step
   double schur_tol = 0.0;  // stop when (r, dinv r) <= tol^2 (r0, dinv r0) (0: always schur_iters)
   int schur_cap = 0;       // with a tolerance: iterations enqueued per step, adapted to those needed
@@ -249,7 +250,8 @@ struct Precond {
   int64_t s0 = 0, s1 = 0;
   Matrix *S = nullptr;
   DBuf<int32_t> d_sseg;  // per row of A: first and one-past-last CSR slot of its columns in [s0, s1)
-  DBuf<double> sdinv;    // Jacobi of S: 1/S_ii, 0 on rows the Schur diagonal leaves out
+  DBuf<double> sdpos;    // n: the positive velocity scale S is built with (0 off the free velocity rows)
+  DBuf<double> sdinv;    // Jacobi of S: 1/S_ii, 0 on rows left out (non-free, or S_ii negligible)
   DBuf<float> sdinv32;
   DBuf<double> svd;      // PCG vectors r u p s w (5 ns)
   DBuf<float> svf;

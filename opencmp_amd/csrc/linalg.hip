@@ -614,9 +614,34 @@ __global__ void k_schur_finish(int64_t n, const uint8_t *__restrict__ flag, cons
   if (dinv32) dinv32[r] = (float)v;
 }
 
-// Values of S = C + B diag(A)^{-1} B^T on the Schur rows [s0, s1) of [[A, B^T], [B, -C]]:
-// S_pq = -a_pq + sum_u a_pu a_uq / a_uu over the free non-Schur columns u of row p (dinv[u] is
-// 1/a_uu there and 0 on Dirichlet dofs). One wavefront per row of S and one lane per entry: the
+// Velocity scale of S: dpos[u] = 1 / max(|a_uu|, 1e-2 max_j |a_uj|) on the free non-Schur rows, 0
+// elsewhere. Positive by construction, so S stays symmetric positive semidefinite where the
+// convection term has made a velocity diagonal small or negative (the P2 vertex functions' mass
+// is small, and the discrete wind is only weakly divergence-free).
+__global__ void k_schur_dpos(int64_t n, int64_t s0, int64_t s1, const int32_t *__restrict__ rp,
+                             const int32_t *__restrict__ col, const double *__restrict__ val,
+                             const uint8_t *__restrict__ mask, double *__restrict__ dpos) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double v = 0.0;
+  if (mask[r] && (r < s0 || r >= s1)) {
+    double d = 0.0, mx = 0.0;
+    for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+      const int32_t c = col[j];
+      if (c >= s0 && c < s1) continue;
+      const double a = fabs(val[j]);
+      if (c == r) d = a;
+      mx = fmax(mx, a);
+    }
+    const double dd = fmax(d, 1e-2 * mx);
+    v = dd > 0.0 ? 1.0 / dd : 0.0;
+  }
+  dpos[r] = v;
+}
+
+// Values of S = C + B D^{-1} B^T on the Schur rows [s0, s1) of [[A, B^T], [B, -C]], D the
+// positive velocity scale above: S_pq = -a_pq + sum_u a_pu a_uq dpos[u] over the non-Schur
+// columns u of row p (dpos is 0 on Dirichlet dofs). One wavefront per row of S and one lane per entry: the
 // lanes walk row p's columns u together and each looks its own column q up in the Schur part of
 // row u (a few slots, read by every lane at once), so each entry is summed in one fixed order.
 __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, const int32_t *__restrict__ srp,
@@ -625,7 +650,7 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
                                                        const int32_t *__restrict__ col,
                                                        const double *__restrict__ val,
                                                        const int32_t *__restrict__ sseg,
-                                                       const double *__restrict__ dinv, double *__restrict__ sval) {
+                                                       const double *__restrict__ dpos, double *__restrict__ sval) {
   const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= ns) return;
@@ -642,7 +667,7 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
       const int32_t a = pass ? pe : rp[p], e = pass ? rp[p + 1] : pb;
       for (int32_t j = a; j < e; ++j) {
         const int32_t u = col[j];
-        const double f = val[j] * dinv[u];
+        const double f = val[j] * dpos[u];
         if (f == 0.0) continue;
         const int32_t t1 = sseg[2 * u + 1];
         for (int32_t t = sseg[2 * u]; t < t1; ++t)
@@ -653,13 +678,41 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
   }
 }
 
-// Jacobi of S from the Schur diagonal: dinv holds -1/s on the Schur rows (0 where the diagonal
-// pass left a row out), so S's scale is -dinv
-__global__ void k_schur_sdinv(int64_t ns, const double *__restrict__ dinv_s, double *__restrict__ sdinv,
+// Jacobi of S, in two passes like the Schur diagonal: S_ii of the free rows and per-block maxima,
+// then 1/S_ii, or 0 where S_ii is not above SCHUR_REL times the largest (rows the phi weighting
+// has all but removed stay out of the inner solve)
+__global__ void __launch_bounds__(NTHR) k_schur_sdiag(int64_t ns, const int32_t *__restrict__ srp,
+                                                      const int32_t *__restrict__ scol,
+                                                      const double *__restrict__ sval,
+                                                      const uint8_t *__restrict__ mask_s, double *__restrict__ sd,
+                                                      double *__restrict__ part) {
+  __shared__ double sh[NTHR / 64];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (i < ns) {
+    for (int32_t k = srp[i]; k < srp[i + 1]; ++k)
+      if (scol[k] == i) d = sval[k];
+    if (!mask_s[i] || !(d > 0.0)) d = 0.0;
+    sd[i] = d;
+  }
+  double m = d;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = sh[0];
+    for (int w = 1; w < NTHR / 64; ++w) bm = fmax(bm, sh[w]);
+    part[blockIdx.x] = bm;
+  }
+}
+
+__global__ void k_schur_sdinv(int64_t ns, const double *__restrict__ smax, double *__restrict__ sdinv,
                               float *__restrict__ sdinv32) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
-  const double v = -dinv_s[i];
+  const double d = sdinv[i];
+  const double v = d > SCHUR_REL * smax[0] ? 1.0 / d : 0.0;
   sdinv[i] = v;
   if (sdinv32) sdinv32[i] = (float)v;
 }
@@ -1170,11 +1223,23 @@ void precond_update(Precond *P) {
         if (!P->S) build_schur_matrix(P);
         Matrix *S = P->S;
         const int64_t ns = S->n;
+        P->sdpos.alloc(A->n);
+        k_schur_dpos<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->s0, P->s1, A->d_row_ptr.p, A->d_col.p, A->d_val.p,
+                                                  A->space->d_free.p, P->sdpos.p);
         k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
-                                                       A->d_col.p, A->d_val.p, P->d_sseg.p, P->dinv.p, S->d_val.p);
+                                                       A->d_col.p, A->d_val.p, P->d_sseg.p, P->sdpos.p, S->d_val.p);
         P->sdinv.alloc(ns);
         if (mixed) P->sdinv32.alloc(ns);
-        k_schur_sdinv<<<nblk(ns), NTHR, 0, st>>>(ns, P->dinv.p + P->s0, P->sdinv.p, mixed ? P->sdinv32.p : nullptr);
+        {
+          Work &w = work(A);
+          const unsigned nbs = nblk(ns);
+          ensure_part(w, (size_t)nbs + 8);
+          double *smax = w.part.p + nbs;
+          k_schur_sdiag<<<nbs, NTHR, 0, st>>>(ns, S->d_row_ptr.p, S->d_col.p, S->d_val.p, A->space->d_free.p + P->s0,
+                                              P->sdinv.p, w.part.p);
+          k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nbs, smax);
+          k_schur_sdinv<<<nbs, NTHR, 0, st>>>(ns, smax, P->sdinv.p, mixed ? P->sdinv32.p : nullptr);
+        }
         // S is small (a pressure space): its FP32 SpMV keeps the 2048-nonzero row blocks for more
         // blocks per launch
         if (mixed) {

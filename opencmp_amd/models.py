@@ -315,13 +315,21 @@ class Stokes(Model):
 
 class INS(Stokes):
     """opencmp/models/ins.py (CG, Oseen linearisation): adds -(u (x) w) : grad v with the wind w the
-    previous iterate (:191-195); solved by the Picard loop of solve_single_step (:323-359)."""
+    previous iterate (:191-195); solved by the Picard loop of solve_single_step (:323-359).
+    ``stress``: {'u': {marker: traction}} boundaries with the stress condition; each gets the
+    no-backflow term dt max(w.n, 0) u.v (:210-216). Only zero traction (the do-nothing outlet)
+    is supported."""
     nonlinear = True
 
     def __init__(self, mesh, kv: float = 1e-3, nonlinear_tolerance=(1e-8, 1e-8), nonlinear_max_iters: int = 10,
-                 **kw):
+                 stress: Optional[Dict[str, Dict[str, Fn]]] = None, **kw):
         self.abs_tol, self.rel_tol = nonlinear_tolerance
         self.nonlinear_max_iters = nonlinear_max_iters
+        self.stress = stress or {}
+        for comp, bcs in self.stress.items():
+            for marker, h in bcs.items():
+                if h is not None and any(float(c) != 0.0 for fn in h for c in fn[1]):
+                    raise ValueError('only zero traction is supported on stress boundaries')
         super().__init__(mesh, kv=kv, **kw)
         self.W = C.GridFunction(self.fes)    # linearisation point (opencmp/models/ins.py:109-116)
 
@@ -331,6 +339,11 @@ class INS(Stokes):
         for b in ub:
             # -(u (x) w) : grad v = -sum_i u_i (w . grad v_i)
             terms.append(C.Vol('conv', b, b, coef=-dt, wind_gf=(self.W, ub)))
+        names = list(self.stress.get('u', {}).keys())
+        if names:
+            mask = self.host_mesh.marker_mask(names)
+            for b in ub:
+                terms.append(C.BndUpwind(b, mask, coef=dt, wind_gf=(self.W, ub)))
         return terms
 
     def solve_single_step(self, a, Lf, pre, gfu, reassemble=None, assembled=False):

@@ -151,3 +151,54 @@ def test_oseen_picard_converges(gpu_ctx):
     # inner solves are measured against their own initial residual (no body force), so the
     # criterion that matters is the Picard one: ||W - u||_L2 < abs + rel ||u||_L2
     assert m.picard_converged and 2 <= m.picard_iterations < 20
+
+
+def test_boundary_upwind_term_closed_forms(gpu_ctx):
+    """The stress-BC no-backflow term coef * max(w.n, 0) u v (opencmp/models/ins.py:210-216) on the
+    unit square, H1 P2: it couples boundary dofs only, and P2 traces interpolate x and y exactly, so
+    u^T A u = sum over edges of int max(w.n, 0) u^2 ds in closed form. u = x separates the right edge
+    (x = 1) from the left one (x = 0), so a flipped normal fails; a wind field checks the field path."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import const
+    say('boundary upwind term')
+    hm = unit_square(8)
+    gm = C.Mesh(hm, gpu_ctx)
+    names = '|'.join(hm.bnd_names)
+    allm = hm.marker_mask(hm.bnd_names)
+    sp = C.FESpace(gm, [C.H1(2, '')])
+
+    def trace(c):
+        g = C.GridFunction(sp)
+        g.Set((L.FN_POLY2, c), names)
+        return g.numpy()
+
+    one, x, y = trace([1, 0, 0, 0, 0, 0]), trace([0, 1, 0, 0, 0, 0]), trace([0, 0, 1, 0, 0, 0])
+
+    def form(**kw):
+        a = C.BilinearForm(sp, [C.BndUpwind(0, kw.pop('markers', allm), **kw)])
+        a.Assemble()
+        return a.to_scipy()
+
+    A = form(coef=1.0, wind=(1.0, 0.0, 0.0))
+    vals = [float(v @ (A @ v)) for v in (one, x, y)]
+    say('  wind (1,0): u=1, x, y ->', vals)
+    assert np.allclose(vals, [1.0, 1.0, 1.0 / 3.0], rtol=0, atol=1e-12)
+    A = form(coef=2.0, wind=(0.6, 0.8, 0.0))
+    vals = [float(v @ (A @ v)) for v in (one, x)]
+    say('  wind (0.6,0.8), coef 2: u=1, x ->', vals)
+    assert np.allclose(vals, [2 * 1.4, 2 * (0.6 + 0.8 / 3.0)], rtol=0, atol=1e-12)
+    # only the marked boundary: the top edge alone
+    A = form(coef=1.0, wind=(0.6, 0.8, 0.0), markers=hm.marker_mask(['top']))
+    assert abs(float(one @ (A @ one)) - 0.8) <= 1e-12 and abs(float(x @ (A @ x)) - 0.8 / 3.0) <= 1e-12
+    # wind field w = (x, 0) set on the boundary (the trace on an edge depends on its own dofs only)
+    ws = C.FESpace(gm, [C.H1(2, '', ncomp=2)])
+    W = C.GridFunction(ws)
+    W.Set([(L.FN_POLY2, [0, 1, 0, 0, 0, 0]), const(0.0)], names, component=0)
+    A = form(coef=1.0, wind_gf=(W, [0, 1]))
+    vals = [float(v @ (A @ v)) for v in (one, y)]
+    say('  wind field (x,0): u=1, y ->', vals)
+    assert np.allclose(vals, [1.0, 1.0 / 3.0], rtol=0, atol=1e-12)
+    # symmetric, and zero off the boundary dofs (those a space with Dirichlet data everywhere fixes)
+    assert abs(A - A.T).max() <= 1e-14
+    bd = ~C.FESpace(gm, [C.H1(2, names)]).FreeDofs()
+    assert abs(A[~bd]).max() == 0.0 and abs(A[:, ~bd]).max() == 0.0

@@ -544,8 +544,9 @@ def test_s_step_gmres_csr_matches_standard(gpu_ctx, problem, precision):
 
 def _schur_step_reference(A, r0, free, s0, s1):
     """M^{-1} r0 of the Schur preconditioner with an exact inner solve, from the explicit sparse
-    product: diag(A)^{-1} on the free velocity rows, -(C + B diag(A)^{-1} B^T)^{-1} on the pressure
-    rows (B: pressure rows x free velocity columns; -C: the assembled pressure block)."""
+    product: diag(A)^{-1} on the free velocity rows, -(C + B D^{-1} B^T)^{-1} on the pressure rows
+    (B: pressure rows x free velocity columns; -C: the assembled pressure block; D_uu =
+    max(|a_uu|, 1e-2 max_j |a_uj|) over the velocity columns j)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     n = A.shape[0]
@@ -554,9 +555,12 @@ def _schur_step_reference(A, r0, free, s0, s1):
     vel &= free
     d = A.diagonal()
     dinv = np.where(vel, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    mx = np.asarray(abs(A[:s0][:, :s0]).max(axis=1).todense()).ravel()
+    dd = np.maximum(np.abs(d[:s0]), 1e-2 * mx)
+    dpos = np.where(vel[:s0], 1.0 / np.where(dd > 0, dd, 1.0), 0.0)
     B = A[s0:s1][:, :s0].tocsr()
     Bt = A[:s0][:, s0:s1].tocsr()
-    S = -A[s0:s1][:, s0:s1] + B @ sp.diags(dinv[:s0]) @ Bt
+    S = -A[s0:s1][:, s0:s1] + B @ sp.diags(dpos) @ Bt
     z = dinv * r0
     z[s0:s1] = -spla.spsolve(S.tocsc(), r0[s0:s1])
     return z, S
@@ -566,8 +570,9 @@ def _schur_step_reference(A, r0, free, s0, s1):
 def test_schur_pcg_step_matches_explicit_schur_complement(gpu_ctx, precision):
     """One FGMRES iteration from the Dirichlet lift moves x along M^{-1} r0. With enough inner PCG
     iterations to solve the 81 pressure rows exactly, that direction equals diag(A)^{-1} r0 on the
-    velocity and -(C + B diag(A)^{-1} B^T)^{-1} r0 on the pressure, formed here by scipy's sparse
-    product and direct solve: a wrong entry, sign or index of S changes the direction."""
+    velocity and -(C + B D^{-1} B^T)^{-1} r0 on the pressure (D the positive velocity scale), formed
+    here by scipy's sparse product and direct solve: a wrong entry, sign or index of S changes the
+    direction."""
     C = core()
     say('schur pcg step', precision)
     m, a, f, x0 = _stokes_step(gpu_ctx, precision, 8, schur_iters=120)
