@@ -2255,9 +2255,18 @@ __device__ __forceinline__ double block_sum_all(double v, double *sh) {
   return v;
 }
 
+// sum of n >= 1 partials: the first 8 per thread are loaded unconditionally (index clamped, the
+// extras dropped) so that all of them are in flight at once, then any rest
 __device__ __forceinline__ double sum_parts(const double *__restrict__ part, int n, double *sh) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+  double a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * blockDim.x;
+    const double v = part[min(i, n - 1)];
+    a[k] = i < n ? v : 0.0;
+  }
+  for (int i = threadIdx.x + 8 * blockDim.x; i < n; i += blockDim.x) a[0] += part[i];
+  const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
   return block_sum_all(s, sh);
 }
 
