@@ -2246,30 +2246,6 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
 // Scalars (sc): [0..3] (gamma, alpha) slots, [4] gamma_0, [5] iterations done (a running count
 // over the solve).
 
-__device__ __forceinline__ double block_sum_all(double v, double *sh) {
-  v = block_sum(v, sh);
-  if (threadIdx.x == 0) sh[0] = v;
-  __syncthreads();
-  v = sh[0];
-  __syncthreads();
-  return v;
-}
-
-// sum of n >= 1 partials: the first 8 per thread are loaded unconditionally (index clamped, the
-// extras dropped) so that all of them are in flight at once, then any rest
-__device__ __forceinline__ double sum_parts(const double *__restrict__ part, int n, double *sh) {
-  double a[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int i = threadIdx.x + k * blockDim.x;
-    const double v = part[min(i, n - 1)];
-    a[k] = i < n ? v : 0.0;
-  }
-  for (int i = threadIdx.x + 8 * blockDim.x; i < n; i += blockDim.x) a[0] += part[i];
-  const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-  return block_sum_all(s, sh);
-}
-
 // x = 0, r = -s v, u = dinv r; partials of r.u; clears the converged flag
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restrict__ v, const double *__restrict__ sv,
@@ -2297,6 +2273,40 @@ __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restric
   }
 }
 
+// the two sums of sum_parts in one pass: both sets of loads in flight together, one shared round
+__device__ __forceinline__ void sum_parts2(const double *__restrict__ pa, int na, const double *__restrict__ pb,
+                                           int nb, double *sh, double &sa, double &sb) {
+  double a[8], b[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * blockDim.x;
+    const double va = pa[min(i, na - 1)], vb = pb[min(i, nb - 1)];
+    a[k] = i < na ? va : 0.0;
+    b[k] = i < nb ? vb : 0.0;
+  }
+  for (int i = threadIdx.x + 8 * blockDim.x; i < na; i += blockDim.x) a[0] += pa[i];
+  for (int i = threadIdx.x + 8 * blockDim.x; i < nb; i += blockDim.x) b[0] += pb[i];
+  double x = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  double y = ((b[0] + b[1]) + (b[2] + b[3])) + ((b[4] + b[5]) + (b[6] + b[7]));
+  x = wave_sum(x);
+  y = wave_sum(y);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if (lane == 0) {
+    sh[wid] = x;
+    sh[nw + wid] = y;
+  }
+  __syncthreads();
+  x = 0.0;
+  y = 0.0;
+  for (int w = 0; w < nw; ++w) {
+    x += sh[w];
+    y += sh[nw + w];
+  }
+  __syncthreads();
+  sa = x;
+  sb = y;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_scg_step(int64_t n, int it, double tol2, T *__restrict__ x,
                                                    T *__restrict__ r, T *__restrict__ u, T *__restrict__ p,
@@ -2304,15 +2314,15 @@ __global__ void __launch_bounds__(NTHR) k_scg_step(int64_t n, int it, double tol
                                                    const T *__restrict__ dinv, const double *__restrict__ gpart,
                                                    const double *__restrict__ dpart, int nd, double *__restrict__ gnew,
                                                    double *__restrict__ sc, int *__restrict__ flag) {
-  __shared__ double sh[NTHR / 64];
+  __shared__ double sh[2 * (NTHR / 64)];
   if (*flag) return;
-  const double g = sum_parts(gpart, gridDim.x, sh);
+  double g, d;
+  sum_parts2(gpart, gridDim.x, dpart, nd, sh, g, d);
   const double g0 = it == 0 ? g : sc[4];
   if (!(g > tol2 * g0)) {  // converged (or a zero right-hand side): the same decision in every block
     if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 1;
     return;
   }
-  const double d = sum_parts(dpart, nd, sh);
   double beta = 0.0, den = d;
   if (it > 0) {
     const double gp = sc[2 * ((it + 1) & 1)], ap = sc[2 * ((it + 1) & 1) + 1];
