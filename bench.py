@@ -443,7 +443,7 @@ def main():
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--verbose-steps', action='store_true', help='config 4: log every step of the timed legs')
-    ap.add_argument('--schur-tol', type=float, default=1e-2,
+    ap.add_argument('--schur-tol', type=float, default=1e-1,
                     help='config 4: the inner PCG stops once its preconditioned residual has fallen by this')
     ap.add_argument('--schur-iters', type=int, default=None,
                     help='config 4: at most this many inner PCG iterations on the pressure Schur complement '
