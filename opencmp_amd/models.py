@@ -328,7 +328,8 @@ class INS(Stokes):
         self.stress = stress or {}
         for comp, bcs in self.stress.items():
             for marker, h in bcs.items():
-                if h is not None and any(float(c) != 0.0 for fn in h for c in fn[1]):
+                fns = [] if h is None else (h if isinstance(h, list) else [h])   # one Fn or one per component
+                if any(float(c) != 0.0 for fn in fns for c in fn[1]):
                     raise ValueError('only zero traction is supported on stress boundaries')
         super().__init__(mesh, kv=kv, **kw)
         self.W = C.GridFunction(self.fes)    # linearisation point (opencmp/models/ins.py:109-116)
