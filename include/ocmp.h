@@ -230,6 +230,17 @@ int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, do
 int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *ref, int fn, const double *c,
                      double *out);
 
+/* out[0] = sqrt(int (div u_h)^2) for the vector field in the dim consecutive scalar
+ * blocks block0 .. block0+dim-1 of x (one H1 vector component): _divergence of
+ * opencmp/helpers/error.py:151-167 (mass conservation check). Synchronises. */
+int ocmp_divergence_norm(ocmp_space *s, int block0, const double *x, double *out);
+
+/* out[0] = max_i |x_i - ref_i| over the dofs of scalar block `block` (ref NULL: max |x_i|):
+ * _l_inf of opencmp/helpers/error.py:80-88, which interpolates the difference into the
+ * space and takes the largest coefficient; for a reference grid function on the same
+ * space that is exactly this. Synchronises. */
+int ocmp_linf_norm(ocmp_space *s, int block, const double *x, const double *ref, double *out);
+
 /* out[0] = ||b - A x||_2 and out[1] = ||b||_2 over the free dofs, FP64 (host
  * doubles, syncs). The residual a solve starts from: NGSolve's solvers compute
  * it inside (opencmp/models/base_model.py:886-947) without exposing it; the

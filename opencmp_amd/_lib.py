@@ -93,6 +93,8 @@ SIGNATURES = {
     'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
+    'ocmp_divergence_norm': [P, C.c_int, P, C.POINTER(C.c_double)],
+    'ocmp_linf_norm': [P, C.c_int, P, P, C.POINTER(C.c_double)],
     'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],

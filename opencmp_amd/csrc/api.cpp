@@ -945,6 +945,25 @@ int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *re
   });
 }
 
+int ocmp_divergence_norm(ocmp_space *s, int block0, const double *x, double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && x && out, "NULL argument");
+    OCMP_REQUIRE(block0 >= 0 && block0 + s->s.mesh->dim <= (int)s->s.blocks.size(), "block out of range");
+    OCMP_REQUIRE(s->s.blocks[block0].kind == OCMP_H1, "the divergence norm is for H1 blocks");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    divergence_norm(&s->s, block0, x, out);
+  });
+}
+
+int ocmp_linf_norm(ocmp_space *s, int block, const double *x, const double *ref, double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && x && out, "NULL argument");
+    OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    linf_norm(&s->s, block, x, ref, out);
+  });
+}
+
 int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, double *y) {
   return guard([&] {
     OCMP_REQUIRE(ctx && x && y && n >= 0, "bad arguments");

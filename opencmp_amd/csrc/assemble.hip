@@ -759,6 +759,72 @@ __global__ void __launch_bounds__(256) k_error_norms(KNorm K, int64_t nel, const
   }
 }
 
+// per-block partial sums of int (div u_h)^2 for the vector field in the D consecutive scalar blocks
+// starting at loc_off (same kind and order each, nl local dofs); the pair's second entry is 0
+template <int D>
+__global__ void __launch_bounds__(256) k_div_norm(KNorm K, int64_t nel, const double *__restrict__ coords,
+                                                  const int32_t *__restrict__ elems,
+                                                  const int32_t *__restrict__ edofs, const double *__restrict__ x,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[4];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s2 = 0.0;
+  if (e < nel) {
+    Geo<D> g;
+    geometry<D>(coords, elems + e * (D + 1), g);
+    const double *gr = K.tab + K.tab_grad;
+    const double *w = K.tab + K.tab_w;
+    for (int q = 0; q < K.nq; ++q) {
+      double dv = 0.0;
+      for (int c = 0; c < D; ++c) {
+        const int32_t *ed = edofs + e * K.ld + K.loc_off + c * K.nl;
+        // d u_c / d x_c = sum_a Jinv[a][c] d u_c / d xhat_a
+        double gref[D];
+        for (int a = 0; a < D; ++a) gref[a] = 0.0;
+        for (int l = 0; l < K.nl; ++l) {
+          const double xv = x[ed[l]];
+          for (int a = 0; a < D; ++a) gref[a] += gr[(q * K.nl + l) * D + a] * xv;
+        }
+        for (int a = 0; a < D; ++a) dv += g.Jinv[a][c] * gref[a];
+      }
+      s2 += w[q] * g.adet * dv * dv;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s2 += __shfl_down(s2, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+// per-block maxima of |x_i - ref_i| over i in [0, n) (ref may be NULL)
+__global__ void __launch_bounds__(256) k_absmax_part(int64_t n, const double *__restrict__ x,
+                                                     const double *__restrict__ ref, double *__restrict__ part) {
+  __shared__ double sh[4];
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(x[i] - (ref ? ref[i] : 0.0)));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ void k_max_of(const double *__restrict__ part, int np, double *__restrict__ out) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmax(m, part[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
 __global__ void k_sum_pairs(const double *__restrict__ part, int64_t np, double *__restrict__ out) {
   __shared__ double sh[2][256];
   double a = 0.0, b = 0.0;
@@ -1309,6 +1375,68 @@ void error_norms(Space *sp, int block, const double *x, const double *ref, int f
   k_sum_pairs<<<1, 256, 0, st>>>(part, nb, res);
   OCMP_HIP(hipGetLastError());
   OCMP_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace ocmp
+
+namespace ocmp {
+
+void divergence_norm(Space *sp, int block0, const double *x, double *out) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const int d = m.dim;
+  const Block &b = s.blocks[block0];
+  for (int c = 1; c < d; ++c) {
+    const Block &bc = s.blocks[block0 + c];
+    OCMP_REQUIRE(bc.kind == b.kind && bc.order == b.order && bc.loc_off == b.loc_off + c * b.nloc,
+                 "the divergence needs dim consecutive blocks of one vector component");
+  }
+  KNorm K;
+  std::memset(&K, 0, sizeof K);
+  TableBuilder tb;
+  Rule rule = simplex_rule(d, 2 * b.order);
+  K.nq = rule.nq;
+  K.nl = b.nloc;
+  K.ld = s.nloc;
+  K.loc_off = b.loc_off;
+  std::vector<double> phi((size_t)rule.nq * b.nloc), gr((size_t)rule.nq * b.nloc * d);
+  for (int q = 0; q < rule.nq; ++q)
+    eval_basis(b.kind, d, b.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b.nloc], &gr[(size_t)q * b.nloc * d]);
+  K.tab_grad = tb.add(gr);
+  K.tab_w = tb.add(rule.w);
+  DBuf<double> *tab = cached_table(s, "D" + std::to_string(block0), tb, st);
+  K.tab = tab->p;
+  const unsigned nb = nblk(m.nel);
+  double *part = ctx->red.p;
+  OCMP_REQUIRE(ctx->red.n >= 2 * (size_t)nb + 2, "reduction scratch too small for this mesh");
+  KTimer tm(ctx, ctx->cls("error_norms"));
+  if (d == 2)
+    k_div_norm<2><<<nb, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, part);
+  else
+    k_div_norm<3><<<nb, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, part);
+  double *res = ctx->scal.p + 110;
+  k_sum_pairs<<<1, 256, 0, st>>>(part, nb, res);
+  OCMP_HIP(hipGetLastError());
+  OCMP_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, st));
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+void linf_norm(Space *sp, int block, const double *x, const double *ref, double *out) {
+  Space &s = *sp;
+  Ctx *ctx = s.mesh->ctx;
+  hipStream_t st = ctx->stream;
+  const Block &b = s.blocks[block];
+  const unsigned nb = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, nblk(b.n1)));
+  double *part = ctx->red.p;
+  OCMP_REQUIRE(ctx->red.n >= (size_t)nb + 2, "reduction scratch too small");
+  k_absmax_part<<<nb, 256, 0, st>>>(b.n1, x + b.dof_off, ref ? ref + b.dof_off : nullptr, part);
+  double *res = ctx->scal.p + 110;
+  k_max_of<<<1, 256, 0, st>>>(part, (int)nb, res);
+  OCMP_HIP(hipGetLastError());
+  OCMP_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, st));
   OCMP_HIP(hipStreamSynchronize(st));
 }
 

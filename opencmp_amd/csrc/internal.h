@@ -279,6 +279,8 @@ bool hessenberg_eigvals(int n, const double *H, int ldh, double *wr, double *wi)
 void newton_shifts(int n, const double *wr, const double *wi, int s, double *d, double *c);
 double residual_vector(Matrix *A, const double *b, const double *x, double *r);
 void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
+void divergence_norm(Space *s, int block0, const double *x, double *out);
+void linf_norm(Space *s, int block, const double *x, const double *ref, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 
 // per-DOF MLP initial guess (mlp.hip)

@@ -168,6 +168,28 @@ class Model:
                                               c, out), 'error_norms')
         return out[0], out[1]
 
+    def divergence(self, gfu: C.GridFunction, comp: str) -> float:
+        """sqrt(int (div u)^2) of a vector component (opencmp/helpers/error.py:151-167)."""
+        import ctypes
+        b0 = self.fes.comp_blocks[self.components.index(comp)][0]
+        out = (ctypes.c_double * 1)()
+        L.check(self.ctx.lib.ocmp_divergence_norm(self.fes.h, b0, ctypes.c_void_p(gfu.vec.data_ptr()), out),
+                'divergence_norm')
+        return out[0]
+
+    def linf_diff(self, a: C.GridFunction, b: Optional[C.GridFunction], comp: str) -> float:
+        """max |a - b| over the component's dof coefficients (opencmp/helpers/error.py:80-88 with a
+        grid-function reference on the same space; b None: max |a|)."""
+        import ctypes
+        out = (ctypes.c_double * 1)()
+        m = 0.0
+        for blk in self.fes.comp_blocks[self.components.index(comp)]:
+            L.check(self.ctx.lib.ocmp_linf_norm(self.fes.h, blk, ctypes.c_void_p(a.vec.data_ptr()),
+                                                None if b is None else ctypes.c_void_p(b.vec.data_ptr()), out),
+                    'linf_norm')
+            m = max(m, out[0])
+        return m
+
     def l2_diff(self, a: C.GridFunction, b: C.GridFunction, comp: str) -> float:
         """sqrt(sum over the component's blocks of int (a - b)^2) -- norm('l2_norm', W, u) of the Picard loop."""
         import ctypes

@@ -202,3 +202,37 @@ def test_boundary_upwind_term_closed_forms(gpu_ctx):
     assert abs(A - A.T).max() <= 1e-14
     bd = ~C.FESpace(gm, [C.H1(2, names)]).FreeDofs()
     assert abs(A[~bd]).max() == 0.0 and abs(A[:, ~bd]).max() == 0.0
+
+
+def test_divergence_and_linf_norms_closed_forms(gpu_ctx):
+    """_divergence and _l_inf of opencmp/helpers/error.py:80-88, :151-167 on linear P2 vector fields
+    on the unit square (vertex coefficients = the field, edge coefficients 0 reproduce it exactly):
+    sqrt(int (div u)^2) = |div u| for the unit area, and the largest coefficient difference is the
+    largest vertex difference."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import Stokes
+    say('divergence and L-inf norms')
+    hm = unit_square(8)
+    m = Stokes(hm, kv=1.0, interp_ord=2, ctx=gpu_ctx)
+    off = m.fes.block_offsets()
+    bx, by = m.fes.comp_blocks[0]
+    X, Y = hm.coords[:, 0], hm.coords[:, 1]
+    nv = len(X)
+
+    def field(ux, uy):
+        v = np.zeros(m.fes.ndof)
+        v[off[bx]:off[bx] + nv] = ux
+        v[off[by]:off[by] + nv] = uy
+        return C.GridFunction(m.fes, torch.tensor(v, device=gpu_ctx.torch_device))
+
+    a = field(X, Y)                       # div 2
+    b = field(X + 2 * Y, 3 * X - Y)       # div 0
+    c = field(3 * X, -Y)                  # div 2
+    d = [m.divergence(g, 'u') for g in (a, b, c)]
+    say('  div norms', d)
+    assert np.allclose(d, [2.0, 0.0, 2.0], rtol=0, atol=1e-12)
+    # max |a - b| over the vertex coefficients: |-2y| <= 2, |y - 3x| <= 3 (at (1, 0))
+    li = m.linf_diff(a, b, 'u')
+    l0 = m.linf_diff(a, None, 'u')
+    say('  linf', li, l0)
+    assert abs(li - 3.0) <= 1e-14 and abs(l0 - 1.0) <= 1e-14
