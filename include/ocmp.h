@@ -123,6 +123,8 @@ enum {
   OCMP_BFI_BND_UPWIND = 5, /* coef * max(wind.n, 0) * u v on boundary facets whose marker is in
                               dirichlet_mask (triangles): the no-backflow part of the stress
                               boundary condition, opencmp/models/ins.py:210-216 */
+  OCMP_BFI_BND_MASS = 6,   /* coef * u v on boundary facets whose marker is in dirichlet_mask
+                              (triangles): the Robin term, opencmp/models/poisson.py:94-96 */
   OCMP_BFI_DG_FACET = 16 /* interior + boundary facet terms of an L2 block, see below */
 };
 
@@ -175,7 +177,10 @@ typedef struct {
 int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, int nfields,
                            const ocmp_field *fields, int accumulate);
 
-enum { OCMP_LFI_SOURCE = 0, OCMP_LFI_DG_DIRICHLET = 1 };
+enum { OCMP_LFI_SOURCE = 0, OCMP_LFI_DG_DIRICHLET = 1,
+       OCMP_LFI_BND_SOURCE = 2 /* coef * g * v on boundary facets whose marker is in dirichlet_mask
+                                  (triangles, H1, closed-form g): Neumann and Robin data,
+                                  opencmp/models/poisson.py:151-158 */ };
 enum {
   OCMP_FN_CONST = 0,     /* c[0]                                               */
   OCMP_FN_SINPROD = 1,   /* c[0] * prod_d sin(pi x_d)                           */

@@ -238,6 +238,25 @@ class BndUpwind:
 
 
 @dataclass
+class BndMass:
+    """coef * u v on the boundary facets whose marker is in `markers` (bit mask), one H1 block:
+    the Robin term."""
+    block: int
+    markers: int
+    coef: float = 1.0
+
+
+@dataclass
+class BndSource:
+    """coef * g * v on the boundary facets whose marker is in `markers`, g = (FN_*, params):
+    Neumann and Robin data."""
+    test: int
+    markers: int
+    fn: Tuple[int, Sequence[float]] = (L.FN_CONST, (0.0,))
+    coef: float = 1.0
+
+
+@dataclass
 class DGFacet:
     """Interior-penalty + upwind skeleton terms of an L2 block."""
     block: int
@@ -321,7 +340,12 @@ class BilinearForm:
             tt = L.Term()
             tt.wind_field = -1
             tt.weight_field = -1
-            if isinstance(t, BndUpwind):
+            if isinstance(t, BndMass):
+                tt.kind = L.BFI_BND_MASS
+                tt.trial = tt.test = t.block
+                tt.coef = t.coef
+                tt.dirichlet_mask = t.markers
+            elif isinstance(t, BndUpwind):
                 tt.kind = L.BFI_BND_UPWIND
                 tt.trial = tt.test = t.block
                 tt.coef = t.coef
@@ -449,6 +473,14 @@ class LinearForm:
                         gf, blk, mode = t.weight_gf
                         lt.weight_field = ft.index(gf, blk)
                         lt.weight_mode = mode
+                elif isinstance(t, BndSource):
+                    lt.kind = L.LFI_BND_SOURCE
+                    lt.test = t.test
+                    lt.coef = t.coef
+                    lt.dirichlet_mask = t.markers
+                    lt.fn = t.fn[0]
+                    for k, v in enumerate(t.fn[1]):
+                        lt.c[k] = v
                 else:
                     lt.kind = L.LFI_DG_DIRICHLET
                     lt.test = t.test

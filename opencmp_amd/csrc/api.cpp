@@ -872,10 +872,10 @@ int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, i
     for (auto &x : t) {
       OCMP_REQUIRE(x.trial >= 0 && x.trial < (int)s.blocks.size() && x.test >= 0 && x.test < (int)s.blocks.size(),
                    "term block index out of range");
-      OCMP_REQUIRE(x.kind == OCMP_BFI_DG_FACET || (x.kind >= 0 && x.kind <= OCMP_BFI_BND_UPWIND),
+      OCMP_REQUIRE(x.kind == OCMP_BFI_DG_FACET || (x.kind >= 0 && x.kind <= OCMP_BFI_BND_MASS),
                    "unknown term kind");
-      OCMP_REQUIRE(x.kind != OCMP_BFI_BND_UPWIND || (!A->A.dg && x.trial == x.test),
-                   "the boundary upwind term is for one H1 block (trial == test)");
+      OCMP_REQUIRE((x.kind != OCMP_BFI_BND_UPWIND && x.kind != OCMP_BFI_BND_MASS) || (!A->A.dg && x.trial == x.test),
+                   "the boundary terms are for one H1 block (trial == test)");
       OCMP_REQUIRE(x.weight_mode == OCMP_WEIGHT_NONE || (x.weight_field >= 0 && x.weight_field < nfields),
                    "weight field index out of range");
       OCMP_REQUIRE(x.wind_field < 0 || x.wind_field + s.mesh->dim <= nfields, "wind field index out of range");
@@ -909,8 +909,11 @@ int ocmp_assemble_linear(ocmp_space *s, double *b, int nterms, const ocmp_lterm 
       OCMP_REQUIRE(x.fn != OCMP_FN_FIELD || (x.field >= 0 && x.field < nfields), "field index out of range");
       OCMP_REQUIRE(x.weight_mode == OCMP_WEIGHT_NONE || (x.weight_field >= 0 && x.weight_field < nfields),
                    "weight field index out of range");
+      OCMP_REQUIRE(x.kind >= OCMP_LFI_SOURCE && x.kind <= OCMP_LFI_BND_SOURCE, "unknown linear term kind");
       OCMP_REQUIRE(x.kind != OCMP_LFI_DG_DIRICHLET || s->s.blocks[x.test].kind == OCMP_L2,
                    "DG Dirichlet data needs an L2 block");
+      OCMP_REQUIRE(x.kind != OCMP_LFI_BND_SOURCE || s->s.blocks[x.test].kind == OCMP_H1,
+                   "boundary sources need an H1 block");
     }
     for (auto &x : f) {
       OCMP_REQUIRE(x.space && x.vec, "field without space or vector");

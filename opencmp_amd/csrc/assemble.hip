@@ -158,9 +158,10 @@ __device__ __forceinline__ double field_at_facet(const KField &f, const double *
   return s;
 }
 
-// Row il (block bi) of coef * max(w.n, 0) u v over the element's boundary facets whose marker is
-// in T.dmask (triangles): the no-backflow part of the stress boundary condition. Facet k is the
-// edge opposite vertex k, n its outward unit normal, the rule maps [0, 1] onto the edge.
+// Row il (block bi) of coef * max(w.n, 0) u v (OCMP_BFI_BND_UPWIND: the no-backflow part of the
+// stress boundary condition) or of coef * u v (OCMP_BFI_BND_MASS: the Robin term) over the
+// element's boundary facets whose marker is in T.dmask (triangles). Facet k is the edge opposite
+// vertex k, n its outward unit normal, the rule maps [0, 1] onto the edge.
 template <int D>
 __device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
                                double ce, double *r) {
@@ -192,13 +193,16 @@ __device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, 
       const double *phit = A.tab + A.b.tab_fphi[bi] + (size_t)k * A.nqf * nlt;
       const double *phij = A.tab + A.b.tab_fphi[bj] + (size_t)k * A.nqf * nlj;
       for (int q = 0; q < A.nqf; ++q) {
-        double wx = T.wind[0], wy = T.wind[1];
-        if (T.wind_field >= 0) {
-          wx = field_at_facet(A.f[T.wind_field], A.tab, e, k, q, A.nqf);
-          wy = field_at_facet(A.f[T.wind_field + 1], A.tab, e, k, q, A.nqf);
+        double wn = 1.0;
+        if (T.kind == OCMP_BFI_BND_UPWIND) {
+          double wx = T.wind[0], wy = T.wind[1];
+          if (T.wind_field >= 0) {
+            wx = field_at_facet(A.f[T.wind_field], A.tab, e, k, q, A.nqf);
+            wy = field_at_facet(A.f[T.wind_field + 1], A.tab, e, k, q, A.nqf);
+          }
+          wn = wx * nx + wy * ny;
+          if (!(wn > 0.0)) continue;
         }
-        const double wn = wx * nx + wy * ny;
-        if (!(wn > 0.0)) continue;
         const double c = ce * fw[q] * len * wn * phit[q * nlt + il];
         for (int j = 0; j < nlj; ++j) r[j] += c * phij[q * nlj + j];
       }
@@ -226,7 +230,7 @@ __device__ void h1_row(const KArgs &A, const double *__restrict__ coords, const 
     const int nlj = A.b.nloc[bj];
     double *r = row + A.b.loc_off[bj];
     double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
-    if (T.kind == OCMP_BFI_BND_UPWIND) {
+    if (T.kind == OCMP_BFI_BND_UPWIND || T.kind == OCMP_BFI_BND_MASS) {
       bnd_upwind_row<D>(A, T, g, e, bi, il, bj, ce, r);
       continue;
     }
@@ -503,6 +507,9 @@ struct KLArgs {
   int tab_w, tab_lam;
   // DG facet data (L2 spaces)
   int fnq, tab_fphi, tab_fgrad, tab_fw, tab_ft;
+  // H1 boundary sources: edge rule (bnq points, weights at tab_bfw, points at tab_bft) with the
+  // blocks' phi at it in KBlocks::tab_fphi
+  int bnq, tab_bfw, tab_bft;
   const double *tab;
 };
 
@@ -537,6 +544,31 @@ __device__ void lin_local(const KLArgs &A, const double *__restrict__ coords, co
           c *= T.weight_mode == OCMP_WEIGHT_FIELD ? ph : (1.0 - ph);
         }
         for (int j = 0; j < nl; ++j) r[j] += c * phi[q * nl + j];
+      }
+    } else if (T.kind == OCMP_LFI_BND_SOURCE) {
+      // coef * g * v on the element's marked boundary edges (edge k opposite vertex k, from vertex
+      // fa to vertex fb as in the facet tables)
+      if constexpr (D == 2) {
+        const double *fw = A.tab + A.tab_bfw;
+        const double *ft = A.tab + A.tab_bft;
+        const double *phi0 = A.tab + A.b.tab_fphi[bi];
+        const int32_t *v = elems + e * 3;
+        for (int k = 0; k < 3; ++k) {
+          const int32_t nbv = elem_nb[e * 3 + k];
+          if (nbv >= 0) continue;
+          const int mk = -1 - nbv;
+          if (mk >= 64 || !((T.dmask >> mk) & 1ull)) continue;
+          const int fa = k == 0 ? 1 : 0, fb = k == 2 ? 1 : 2;
+          const double ax = coords[(int64_t)v[fa] * 2], ay = coords[(int64_t)v[fa] * 2 + 1];
+          const double bx = coords[(int64_t)v[fb] * 2], by = coords[(int64_t)v[fb] * 2 + 1];
+          const double len = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+          for (int q = 0; q < A.bnq; ++q) {
+            const double x[2] = {(1.0 - ft[q]) * ax + ft[q] * bx, (1.0 - ft[q]) * ay + ft[q] * by};
+            const double c = T.coef * eval_fn(T.fn, T.c, x, 2) * fw[q] * len;
+            const double *ph = phi0 + (k * A.bnq + q) * nl;
+            for (int j = 0; j < nl; ++j) r[j] += c * ph[j];
+          }
+        }
       }
     } else if constexpr (D == 2) {
       // DG Dirichlet data on boundary facets of this element
@@ -961,7 +993,7 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
   A.tab_w = tb.add(rule.w);
   A.tab_lam = tb.add(rule.lam);
   bool bnd = false;
-  for (auto &t : terms) bnd |= t.kind == OCMP_BFI_BND_UPWIND;
+  for (auto &t : terms) bnd |= t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS;
   // facet rule for boundary terms: exact for the product of three functions of the highest order
   std::vector<double> ft, fwt;
   if (bnd) {
@@ -1025,8 +1057,9 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
       k.test = -1;  // never matches a row block in the volume pass
       continue;
     }
-    if (t.kind == OCMP_BFI_BND_UPWIND) {
-      OCMP_REQUIRE(t.wind_field < 0 || t.wind_field + 1 < (int)fields.size(), "wind field out of range");
+    if (t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS) {
+      OCMP_REQUIRE(t.kind == OCMP_BFI_BND_MASS || t.wind_field < 0 || t.wind_field + 1 < (int)fields.size(),
+                   "wind field out of range");
       continue;
     }
     if (!k.quad) k.tab = add_tensors(tb, s, t, k.ntens);
@@ -1189,7 +1222,7 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
                  &gr[(size_t)q * fb.nloc * d]);
     kf.tab_phi = tb.add(phi);
   }
-  bool has_dg = false;
+  bool has_dg = false, has_bnd = false;
   for (size_t i = 0; i < terms.size(); ++i) {
     const ocmp_lterm &t = terms[i];
     KLTerm &k = A.t[i];
@@ -1210,6 +1243,32 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
       OCMP_REQUIRE(d == 2, "DG Dirichlet data on triangles only");
       OCMP_REQUIRE(t.fn != OCMP_FN_FIELD, "DG Dirichlet data must be a closed-form function");
       has_dg = true;
+    }
+    if (t.kind == OCMP_LFI_BND_SOURCE) {
+      OCMP_REQUIRE(d == 2, "boundary sources on triangles only");
+      OCMP_REQUIRE(t.fn != OCMP_FN_FIELD, "boundary source data must be a closed-form function");
+      has_bnd = true;
+    }
+  }
+  if (has_bnd) {
+    std::vector<double> ft, fw;
+    line_rule(2 * max_order(s) + 4, ft, fw);
+    A.bnq = (int)ft.size();
+    A.tab_bfw = tb.add(fw);
+    A.tab_bft = tb.add(ft);
+    const int fa[3] = {1, 0, 0}, fb[3] = {2, 2, 1};
+    for (int bb = 0; bb < A.b.n; ++bb) {
+      const Block &blk = s.blocks[bb];
+      std::vector<double> phi((size_t)3 * A.bnq * blk.nloc), gr((size_t)3 * A.bnq * blk.nloc * 2);
+      for (int k = 0; k < 3; ++k)
+        for (int q = 0; q < A.bnq; ++q) {
+          double lam[3] = {0, 0, 0};
+          lam[fa[k]] = 1.0 - ft[q];
+          lam[fb[k]] = ft[q];
+          eval_basis(blk.kind, 2, blk.order, lam, &phi[((size_t)k * A.bnq + q) * blk.nloc],
+                     &gr[((size_t)k * A.bnq + q) * blk.nloc * 2]);
+        }
+      A.b.tab_fphi[bb] = tb.add(phi);
     }
   }
   if (has_dg) {

@@ -205,13 +205,20 @@ class Model:
 
 
 class Poisson(Model):
-    """opencmp/models/poisson.py:28-168: -div(dc grad u) = f, H1 (CG) or L2 with SIPG (DG)."""
+    """opencmp/models/poisson.py:28-168: -div(dc grad u) = f, H1 (CG) or L2 with SIPG (DG).
+    ``neumann``: {'u': {marker: h}} adds dt dc h v ds to the load (:151-154); ``robin``:
+    {'u': {marker: (r, q)}} adds -dt dc r u v ds to the form (:94-96) and -dt dc r q v ds to the
+    load (:156-158); h and q closed forms, r a number; H1 on triangles."""
     components = ['u']
 
-    def __init__(self, mesh, dc: float = 1.0, f: Fn = ZERO, **kw):
+    def __init__(self, mesh, dc: float = 1.0, f: Fn = ZERO, neumann=None, robin=None, **kw):
         self.dc = dc
         self.f = f
+        self.neumann = neumann or {}
+        self.robin = robin or {}
         super().__init__(mesh, **kw)
+        if (self.neumann.get('u') or self.robin.get('u')) and self.DG:
+            raise ValueError('Neumann/Robin data are implemented for the H1 (CG) Poisson model')
 
     def _construct_fes(self):
         if self.DG:
@@ -223,6 +230,8 @@ class Poisson(Model):
         if self.DG and self.dirichlet.get('u'):
             # Dirichlet Nitsche terms (poisson.py:81-91) are part of the facet term below
             pass
+        for marker, (r, q) in self.robin.get('u', {}).items():                # poisson.py:94-96
+            a.append(C.BndMass(0, self.mesh.marker_mask(marker), coef=-dt * self.dc * float(r)))
         return a
 
     def construct_bilinear_time_coefficient(self, dt=1.0):
@@ -238,6 +247,10 @@ class Poisson(Model):
             for marker, g in self.dirichlet.get('u', {}).items():             # poisson.py:136-146
                 Lf.append(C.DGDirichletData(0, eps=dt * self.dc, alpha=self.nu, wind=(0.0, 0.0),
                                             dirichlet=self.mesh.marker_mask(marker), fn=g))
+        for marker, h in self.neumann.get('u', {}).items():                   # poisson.py:151-154
+            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=h, coef=dt * self.dc))
+        for marker, (r, q) in self.robin.get('u', {}).items():                # poisson.py:156-158
+            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=q, coef=-dt * self.dc * float(r)))
         return Lf
 
 

@@ -236,3 +236,43 @@ def test_divergence_and_linf_norms_closed_forms(gpu_ctx):
     l0 = m.linf_diff(a, None, 'u')
     say('  linf', li, l0)
     assert abs(li - 3.0) <= 1e-14 and abs(l0 - 1.0) <= 1e-14
+
+
+def test_neumann_robin_terms_and_exact_solutions(gpu_ctx):
+    """Neumann and Robin data of the Poisson model (opencmp/models/poisson.py:94-96, :151-158): the
+    boundary source and boundary mass against closed forms on the unit square (b . interpolant of p
+    = int g p ds for p = 1, y), and -lap u = 0 with u = x + 1 reproduced to rounding by P2 with
+    Dirichlet data on the left and, on the right, Neumann data h = du/dn = 1 or Robin data
+    du/dn = r (u - q) with r = -1, q = 3 (top and bottom: the natural condition du/dn = 0)."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import Poisson, const
+    from opencmp_amd.solvers import StationarySolver
+    say('neumann / robin')
+    hm = unit_square(8)
+    gm = C.Mesh(hm, gpu_ctx)
+    sp = C.FESpace(gm, [C.H1(2, '')])
+    right = hm.marker_mask(['right'])
+    f = C.LinearForm(sp, [C.BndSource(0, right, fn=(L.FN_POLY2, [0, 0, 1, 0, 0, 0]), coef=2.0)])   # 2 y
+    f.Assemble()
+    b = f.vec.cpu().numpy()
+    def trace(c):
+        g = C.GridFunction(sp)
+        g.Set((L.FN_POLY2, c), '|'.join(hm.bnd_names))
+        return g.numpy()
+
+    one, y = trace([1, 0, 0, 0, 0, 0]), trace([0, 0, 1, 0, 0, 0])
+    say('  b.1', b @ one, 'b.y', b @ y)
+    assert abs(b @ one - 1.0) <= 1e-13 and abs(b @ y - 2.0 / 3.0) <= 1e-13
+    a = C.BilinearForm(sp, [C.BndMass(0, right, coef=3.0)])
+    a.Assemble()
+    A = a.to_scipy()
+    assert abs(float(y @ (A @ y)) - 1.0) <= 1e-13   # 3 int_0^1 y^2 dy
+    exact = (L.FN_POLY2, [1, 1, 0, 0, 0, 0])
+    for kw in ({'neumann': {'u': {'right': const(1.0)}}}, {'robin': {'u': {'right': (-1.0, const(3.0))}}}):
+        m = Poisson(hm, dc=1.0, f=const(0.0), interp_ord=2, dirichlet={'u': {'left': const(1.0)}},
+                    linear_solver='CG', linear_tolerance=1e-13, ctx=gpu_ctx, **kw)
+        s = StationarySolver(m)
+        u = s.solve()
+        e_l2, e_h1 = m.l2_error(u, 'u', exact)
+        say('  ', list(kw)[0], s.result, 'L2', e_l2, 'H1', e_h1)
+        assert s.result.converged and e_l2 <= 1e-10 and e_h1 <= 1e-9
