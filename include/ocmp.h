@@ -121,10 +121,10 @@ enum {
   OCMP_BFI_DTRIAL = 3,   /* coef * w * d_dir u * v                             */
   OCMP_BFI_DTEST = 4,    /* coef * w * u * d_dir v                             */
   OCMP_BFI_BND_UPWIND = 5, /* coef * max(wind.n, 0) * u v on boundary facets whose marker is in
-                              dirichlet_mask (triangles): the no-backflow part of the stress
+                              dirichlet_mask (H1, trial == test): the no-backflow part of the stress
                               boundary condition, opencmp/models/ins.py:210-216 */
   OCMP_BFI_BND_MASS = 6,   /* coef * u v on boundary facets whose marker is in dirichlet_mask
-                              (triangles): the Robin term, opencmp/models/poisson.py:94-96 */
+                              (H1, trial == test): the Robin term, opencmp/models/poisson.py:94-96 */
   OCMP_BFI_DG_FACET = 16 /* interior + boundary facet terms of an L2 block, see below */
 };
 
@@ -179,7 +179,7 @@ int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, i
 
 enum { OCMP_LFI_SOURCE = 0, OCMP_LFI_DG_DIRICHLET = 1,
        OCMP_LFI_BND_SOURCE = 2 /* coef * g * v on boundary facets whose marker is in dirichlet_mask
-                                  (triangles, H1, closed-form g): Neumann and Robin data,
+                                  (H1, closed-form g): Neumann and Robin data,
                                   opencmp/models/poisson.py:151-158 */ };
 enum {
   OCMP_FN_CONST = 0,     /* c[0]                                               */

@@ -581,6 +581,21 @@ void build_colouring(Space &s) {
     for (int64_t e = 0; e < m.nel; ++e) order[fill[colour[e]]++] = (int32_t)e;
   }
   s.d_colour_elems.upload(order, m.ctx->stream);
+  // the elements of each colour with a boundary facet, same order (boundary-facet terms)
+  const int nf = m.dim + 1;
+  std::vector<int32_t> bnd;
+  s.bnd_colour_off.assign(ncol + 1, 0);
+  for (int c = 0; c < ncol; ++c) {
+    for (int64_t i = s.colour_off[c]; i < s.colour_off[c + 1]; ++i) {
+      const int32_t e = order[i];
+      bool b = false;
+      for (int k = 0; k < nf; ++k) b |= m.elem_nb[(size_t)e * nf + k] < 0;
+      if (b) bnd.push_back(e);
+    }
+    s.bnd_colour_off[c + 1] = (int64_t)bnd.size();
+  }
+  s.d_bnd_colour_elems.upload(bnd, m.ctx->stream);
+  OCMP_HIP(hipStreamSynchronize(m.ctx->stream));
 }
 
 void build_h1_schedule(Matrix *A) {

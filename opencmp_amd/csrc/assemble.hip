@@ -160,8 +160,8 @@ __device__ __forceinline__ double field_at_facet(const KField &f, const double *
 
 // Row il (block bi) of coef * max(w.n, 0) u v (OCMP_BFI_BND_UPWIND: the no-backflow part of the
 // stress boundary condition) or of coef * u v (OCMP_BFI_BND_MASS: the Robin term) over the
-// element's boundary facets whose marker is in T.dmask (triangles). Facet k is the edge opposite
-// vertex k, n its outward unit normal, the rule maps [0, 1] onto the edge.
+// element's boundary facets whose marker is in T.dmask. Facet k is the edge (face) opposite vertex
+// k, n its outward unit normal; in 2-D the rule maps [0, 1] onto the edge.
 template <int D>
 __device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
                                double ce, double *r) {
@@ -207,6 +207,58 @@ __device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, 
         for (int j = 0; j < nlj; ++j) r[j] += c * phij[q * nlj + j];
       }
     }
+  } else {
+    // tetrahedra: face k has the vertices other than k, (a, b, c) in increasing order; the face rule's
+    // weights sum to 1, so a face integral is its area times the weighted sum
+    const int nlt = A.b.nloc[bi], nlj = A.b.nloc[bj];
+    double P[4][3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      P[0][cc] = g.x0[cc];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) P[i + 1][cc] = g.x0[cc] + g.J[cc][i];
+    }
+    const double *fw = A.tab + A.tab_fw;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t nb = A.elem_nb[e * 4 + k];
+      if (nb >= 0) continue;
+      const int marker = -nb - 1;
+      if (marker >= 64 || !((T.dmask >> marker) & 1ull)) continue;
+      const int a = k == 0 ? 1 : 0, b = k <= 1 ? 2 : 1, c = k == 3 ? 2 : 3;
+      double u[3], v[3], nrm[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        u[cc] = P[b][cc] - P[a][cc];
+        v[cc] = P[c][cc] - P[a][cc];
+      }
+      nrm[0] = u[1] * v[2] - u[2] * v[1];
+      nrm[1] = u[2] * v[0] - u[0] * v[2];
+      nrm[2] = u[0] * v[1] - u[1] * v[0];
+      const double a2 = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+      double n3[3] = {nrm[0] / a2, nrm[1] / a2, nrm[2] / a2};
+      if ((P[k][0] - P[a][0]) * n3[0] + (P[k][1] - P[a][1]) * n3[1] + (P[k][2] - P[a][2]) * n3[2] > 0.0) {
+        n3[0] = -n3[0];
+        n3[1] = -n3[1];
+        n3[2] = -n3[2];
+      }
+      const double area = 0.5 * a2;
+      const double *phit = A.tab + A.b.tab_fphi[bi] + (size_t)k * A.nqf * nlt;
+      const double *phij = A.tab + A.b.tab_fphi[bj] + (size_t)k * A.nqf * nlj;
+      for (int q = 0; q < A.nqf; ++q) {
+        double wn = 1.0;
+        if (T.kind == OCMP_BFI_BND_UPWIND) {
+          double w3[3] = {T.wind[0], T.wind[1], T.wind[2]};
+          if (T.wind_field >= 0)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) w3[cc] = field_at_facet(A.f[T.wind_field + cc], A.tab, e, k, q, A.nqf);
+          wn = w3[0] * n3[0] + w3[1] * n3[1] + w3[2] * n3[2];
+          if (!(wn > 0.0)) continue;
+        }
+        const double cq = ce * fw[q] * area * wn * phit[q * nlt + il];
+        for (int j = 0; j < nlj; ++j) r[j] += cq * phij[q * nlj + j];
+      }
+    }
   }
 }
 
@@ -230,10 +282,7 @@ __device__ void h1_row(const KArgs &A, const double *__restrict__ coords, const 
     const int nlj = A.b.nloc[bj];
     double *r = row + A.b.loc_off[bj];
     double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
-    if (T.kind == OCMP_BFI_BND_UPWIND || T.kind == OCMP_BFI_BND_MASS) {
-      bnd_upwind_row<D>(A, T, g, e, bi, il, bj, ce, r);
-      continue;
-    }
+    if (T.kind == OCMP_BFI_BND_UPWIND || T.kind == OCMP_BFI_BND_MASS) continue;  // k_h1_bnd_assemble
     if (!T.quad) {
       // reference-tensor contraction
       double fac[9];
@@ -344,6 +393,39 @@ __global__ void __launch_bounds__(256) k_h1_assemble(KArgs A, const double *__re
   h1_row<D>(A, coords, elems, e, i, row);
   const int32_t *sl = slot + (e * A.nloc + i) * A.nloc;
   for (int j = 0; j < A.nloc; ++j) val[sl[j]] += row[j];
+}
+
+// the boundary-facet terms, over the elements of one colour that have a boundary facet: thread =
+// (element, local row), which owns that row of the element's slots within the launch
+template <int D>
+__global__ void __launch_bounds__(256) k_h1_bnd_assemble(KArgs A, const double *__restrict__ coords,
+                                                         const int32_t *__restrict__ elems,
+                                                         const int32_t *__restrict__ bnd_elems, int64_t cbeg,
+                                                         int64_t ccount, const int32_t *__restrict__ slot,
+                                                         double *__restrict__ val) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ccount * A.nloc) return;
+  const int64_t e = bnd_elems[cbeg + t / A.nloc];
+  const int i = (int)(t % A.nloc);
+  Geo<D> g;
+  geometry<D>(coords, elems + e * (D + 1), g);
+  int bi = 0;
+  while (bi + 1 < A.b.n && i >= A.b.loc_off[bi + 1]) ++bi;
+  const int il = i - A.b.loc_off[bi];
+  double row[MAXL];
+  for (int j = 0; j < A.nloc; ++j) row[j] = 0.0;
+  bool any = false;
+  for (int tt = 0; tt < A.nterms; ++tt) {
+    const KTerm &T = A.t[tt];
+    if (T.test != bi || (T.kind != OCMP_BFI_BND_UPWIND && T.kind != OCMP_BFI_BND_MASS)) continue;
+    const double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
+    bnd_upwind_row<D>(A, T, g, e, bi, il, T.trial, ce, row + A.b.loc_off[T.trial]);
+    any = true;
+  }
+  if (!any) return;
+  const int32_t *sl = slot + (e * A.nloc + i) * A.nloc;
+  for (int j = 0; j < A.nloc; ++j)
+    if (row[j] != 0.0) val[sl[j]] += row[j];
 }
 
 __global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
@@ -567,6 +649,40 @@ __device__ void lin_local(const KLArgs &A, const double *__restrict__ coords, co
             const double c = T.coef * eval_fn(T.fn, T.c, x, 2) * fw[q] * len;
             const double *ph = phi0 + (k * A.bnq + q) * nl;
             for (int j = 0; j < nl; ++j) r[j] += c * ph[j];
+          }
+        }
+      } else {
+        // tetrahedra: face k has the vertices other than k in increasing order; tab_bft holds the
+        // face rule's barycentrics (3 per point), the weights sum to 1
+        const double *fw = A.tab + A.tab_bfw;
+        const double *fl = A.tab + A.tab_bft;
+        const double *phi0 = A.tab + A.b.tab_fphi[bi];
+        const int32_t *v = elems + e * 4;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int32_t nbv = elem_nb[e * 4 + k];
+          if (nbv >= 0) continue;
+          const int mk = -1 - nbv;
+          if (mk >= 64 || !((T.dmask >> mk) & 1ull)) continue;
+          const int a = k == 0 ? 1 : 0, b = k <= 1 ? 2 : 1, c = k == 3 ? 2 : 3;
+          double Pa[3], Pb[3], Pc[3];
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) {
+            Pa[cc] = coords[(int64_t)v[a] * 3 + cc];
+            Pb[cc] = coords[(int64_t)v[b] * 3 + cc];
+            Pc[cc] = coords[(int64_t)v[c] * 3 + cc];
+          }
+          const double u0 = Pb[0] - Pa[0], u1 = Pb[1] - Pa[1], u2 = Pb[2] - Pa[2];
+          const double w0 = Pc[0] - Pa[0], w1 = Pc[1] - Pa[1], w2 = Pc[2] - Pa[2];
+          const double n0 = u1 * w2 - u2 * w1, n1 = u2 * w0 - u0 * w2, n2 = u0 * w1 - u1 * w0;
+          const double area = 0.5 * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+          for (int q = 0; q < A.bnq; ++q) {
+            double x[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) x[cc] = fl[3 * q] * Pa[cc] + fl[3 * q + 1] * Pb[cc] + fl[3 * q + 2] * Pc[cc];
+            const double cq = T.coef * eval_fn(T.fn, T.c, x, 3) * fw[q] * area;
+            const double *ph = phi0 + (k * A.bnq + q) * nl;
+            for (int j = 0; j < nl; ++j) r[j] += cq * ph[j];
           }
         }
       }
@@ -975,6 +1091,31 @@ static std::string term_key(const std::vector<ocmp_term> &terms, const std::vect
   return k;
 }
 
+// phi of a (kind, order) block at the facet rules of all local facets (facet k opposite vertex k):
+// triangles take the edge rule t (lam = 1 - t, t on the edge's two vertices in increasing order),
+// tetrahedra the face rule's barycentrics on the face's three vertices in increasing order
+static int add_facet_phi(TableBuilder &tb, int d, int kind, int order, int nl, int nq, const std::vector<double> &ft,
+                         const std::vector<double> &flam) {
+  const int nf = d + 1;
+  std::vector<double> phi((size_t)nf * nq * nl), gr((size_t)nf * nq * nl * d);
+  for (int k = 0; k < nf; ++k)
+    for (int q = 0; q < nq; ++q) {
+      double lam[4] = {0, 0, 0, 0};
+      if (d == 2) {
+        const int a = k == 0 ? 1 : 0, b = k == 2 ? 1 : 2;
+        lam[a] = 1.0 - ft[q];
+        lam[b] = ft[q];
+      } else {
+        const int a = k == 0 ? 1 : 0, b = k <= 1 ? 2 : 1, c = k == 3 ? 2 : 3;
+        lam[a] = flam[3 * q];
+        lam[b] = flam[3 * q + 1];
+        lam[c] = flam[3 * q + 2];
+      }
+      eval_basis(kind, d, order, lam, &phi[((size_t)k * nq + q) * nl], &gr[((size_t)k * nq + q) * nl * d]);
+    }
+  return tb.add(phi);
+}
+
 static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
                         KArgs &A, TableBuilder &tb, hipStream_t st) {
   const int d = s.mesh->dim;
@@ -995,27 +1136,24 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
   bool bnd = false;
   for (auto &t : terms) bnd |= t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS;
   // facet rule for boundary terms: exact for the product of three functions of the highest order
-  std::vector<double> ft, fwt;
+  std::vector<double> ft, fwt, flam;
   if (bnd) {
-    OCMP_REQUIRE(d == 2, "boundary-facet terms are implemented on triangles");
-    line_rule(2 * max_order(s) + std::max(pf, 1), ft, fwt);
-    A.nqf = (int)ft.size();
+    const int deg = 2 * max_order(s) + std::max(pf, 1);
+    if (d == 2) {
+      line_rule(deg, ft, fwt);
+      A.nqf = (int)ft.size();
+    } else {
+      Rule fr = simplex_rule(2, deg);
+      A.nqf = fr.nq;
+      flam = fr.lam;
+      fwt = fr.w;
+      for (auto &w : fwt) w *= 2.0;  // weights summing to 1: a face integral is area * sum
+    }
     A.tab_fw = tb.add(fwt);
     A.elem_nb = s.mesh->d_elem_nb.p;
   }
-  // phi of a (kind, order) block at the facet rule of the three edges (edge k opposite vertex k)
-  auto facet_phi = [&](int kind, int order, int nl) {
-    std::vector<double> phi((size_t)3 * A.nqf * nl), gr((size_t)3 * A.nqf * nl * 2);
-    const int fa[3] = {1, 0, 0}, fb[3] = {2, 2, 1};
-    for (int k = 0; k < 3; ++k)
-      for (int q = 0; q < A.nqf; ++q) {
-        double lam[3] = {0, 0, 0};
-        lam[fa[k]] = 1.0 - ft[q];
-        lam[fb[k]] = ft[q];
-        eval_basis(kind, 2, order, lam, &phi[((size_t)k * A.nqf + q) * nl], &gr[((size_t)k * A.nqf + q) * nl * 2]);
-      }
-    return tb.add(phi);
-  };
+  // phi of a (kind, order) block at the facet rule of each local facet (facet k opposite vertex k)
+  auto facet_phi = [&](int kind, int order, int nl) { return add_facet_phi(tb, d, kind, order, nl, A.nqf, ft, flam); };
   if (bnd)
     for (int b = 0; b < A.b.n; ++b) A.b.tab_fphi[b] = facet_phi(s.blocks[b].kind, s.blocks[b].order, s.blocks[b].nloc);
   for (size_t i = 0; i < fields.size(); ++i) {
@@ -1082,6 +1220,9 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
             8.0 * M->nnz + 4.0 * s.mesh->nel * (s.mesh->dim + 1) + 8.0 * s.mesh->nv * s.mesh->dim);
   if (!accumulate) k_fill<<<nblk(M->nnz), 256, 0, st>>>(M->d_val.p, M->nnz, 0.0);
   const Mesh &m = *s.mesh;
+  bool has_bnd = false;
+  for (auto &t : terms) has_bnd |= t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS;
+  OCMP_REQUIRE(!has_bnd || (int64_t)s.bnd_colour_off.size() == s.ncolours + 1, "boundary element lists missing");
   for (int c = 0; c < s.ncolours; ++c) {
     int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;
     if (!cn) continue;
@@ -1091,6 +1232,16 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
     else
       k_h1_assemble<3><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb, cn,
                                                            M->d_slot.p, M->d_val.p);
+    if (has_bnd) {
+      const int64_t bb = s.bnd_colour_off[c], bn = s.bnd_colour_off[c + 1] - bb;
+      if (!bn) continue;
+      if (m.dim == 2)
+        k_h1_bnd_assemble<2><<<nblk(bn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_bnd_colour_elems.p,
+                                                                 bb, bn, M->d_slot.p, M->d_val.p);
+      else
+        k_h1_bnd_assemble<3><<<nblk(bn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_bnd_colour_elems.p,
+                                                                 bb, bn, M->d_slot.p, M->d_val.p);
+    }
   }
   OCMP_HIP(hipGetLastError());
 }
@@ -1245,31 +1396,28 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
       has_dg = true;
     }
     if (t.kind == OCMP_LFI_BND_SOURCE) {
-      OCMP_REQUIRE(d == 2, "boundary sources on triangles only");
       OCMP_REQUIRE(t.fn != OCMP_FN_FIELD, "boundary source data must be a closed-form function");
       has_bnd = true;
     }
   }
   if (has_bnd) {
-    std::vector<double> ft, fw;
-    line_rule(2 * max_order(s) + 4, ft, fw);
-    A.bnq = (int)ft.size();
-    A.tab_bfw = tb.add(fw);
-    A.tab_bft = tb.add(ft);
-    const int fa[3] = {1, 0, 0}, fb[3] = {2, 2, 1};
-    for (int bb = 0; bb < A.b.n; ++bb) {
-      const Block &blk = s.blocks[bb];
-      std::vector<double> phi((size_t)3 * A.bnq * blk.nloc), gr((size_t)3 * A.bnq * blk.nloc * 2);
-      for (int k = 0; k < 3; ++k)
-        for (int q = 0; q < A.bnq; ++q) {
-          double lam[3] = {0, 0, 0};
-          lam[fa[k]] = 1.0 - ft[q];
-          lam[fb[k]] = ft[q];
-          eval_basis(blk.kind, 2, blk.order, lam, &phi[((size_t)k * A.bnq + q) * blk.nloc],
-                     &gr[((size_t)k * A.bnq + q) * blk.nloc * 2]);
-        }
-      A.b.tab_fphi[bb] = tb.add(phi);
+    std::vector<double> ft, fw, flam;
+    if (d == 2) {
+      line_rule(2 * max_order(s) + 4, ft, fw);
+      A.bnq = (int)ft.size();
+      A.tab_bft = tb.add(ft);
+    } else {
+      Rule fr = simplex_rule(2, 2 * max_order(s) + 4);
+      A.bnq = fr.nq;
+      flam = fr.lam;
+      fw = fr.w;
+      for (auto &w : fw) w *= 2.0;  // weights summing to 1: a face integral is area * sum
+      A.tab_bft = tb.add(flam);
     }
+    A.tab_bfw = tb.add(fw);
+    for (int bb = 0; bb < A.b.n; ++bb)
+      A.b.tab_fphi[bb] = add_facet_phi(tb, d, s.blocks[bb].kind, s.blocks[bb].order, s.blocks[bb].nloc, A.bnq, ft,
+                                       flam);
   }
   if (has_dg) {
     const Block &b = s.blocks[0];

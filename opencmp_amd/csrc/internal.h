@@ -165,6 +165,8 @@ struct Space {
   int ncolours = 0;
   std::vector<int64_t> colour_off;   // ncolours+1
   DBuf<int32_t> d_colour_elems;      // elements sorted by colour
+  std::vector<int64_t> bnd_colour_off;  // ncolours+1: the elements with a boundary facet, per colour
+  DBuf<int32_t> d_bnd_colour_elems;
   // cached kernel tables keyed by a signature
   std::map<std::string, DBuf<double> *> tables;
   ~Space() {

@@ -208,7 +208,7 @@ class Poisson(Model):
     """opencmp/models/poisson.py:28-168: -div(dc grad u) = f, H1 (CG) or L2 with SIPG (DG).
     ``neumann``: {'u': {marker: h}} adds dt dc h v ds to the load (:151-154); ``robin``:
     {'u': {marker: (r, q)}} adds -dt dc r u v ds to the form (:94-96) and -dt dc r q v ds to the
-    load (:156-158); h and q closed forms, r a number; H1 on triangles."""
+    load (:156-158); h and q closed forms, r a number; H1 (triangles or tetrahedra)."""
     components = ['u']
 
     def __init__(self, mesh, dc: float = 1.0, f: Fn = ZERO, neumann=None, robin=None, **kw):

@@ -276,3 +276,52 @@ def test_neumann_robin_terms_and_exact_solutions(gpu_ctx):
         e_l2, e_h1 = m.l2_error(u, 'u', exact)
         say('  ', list(kw)[0], s.result, 'L2', e_l2, 'H1', e_h1)
         assert s.result.converged and e_l2 <= 1e-10 and e_h1 <= 1e-9
+
+
+def test_boundary_terms_on_tetrahedra(gpu_ctx):
+    """The boundary-facet terms on the unit cube (tetrahedral faces): u^T A u = int over the marked
+    faces in closed form for the Robin mass and the no-backflow term (u = x picks the x = 1 face
+    only, so a flipped normal fails), b . interpolant of p = int g p dA for the boundary source, and
+    -lap u = 0 with u = x + 1 reproduced by P2 with Neumann data du/dn = 1 on x = 1."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import Poisson, const
+    from opencmp_amd.solvers import StationarySolver
+    say('boundary terms on tetrahedra')
+    hm = unit_cube(3)
+    gm = C.Mesh(hm, gpu_ctx)
+    sp = C.FESpace(gm, [C.H1(2, '')])
+    names = '|'.join(hm.bnd_names)
+    right, allm = hm.marker_mask(['right']), hm.marker_mask(hm.bnd_names)
+
+    def trace(c):
+        g = C.GridFunction(sp)
+        g.Set((L.FN_POLY2, c + [0] * (10 - len(c))), names)
+        return g.numpy()
+
+    one, x, y = trace([1]), trace([0, 1]), trace([0, 0, 1])
+
+    def form(t):
+        a = C.BilinearForm(sp, [t])
+        a.Assemble()
+        return a.to_scipy()
+
+    A = form(C.BndMass(0, right, coef=3.0))
+    vals = [float(v @ (A @ v)) for v in (one, y)]
+    say('  robin mass on x=1: u=1, y ->', vals)
+    assert np.allclose(vals, [3.0, 1.0], rtol=0, atol=1e-12)
+    A = form(C.BndUpwind(0, allm, coef=1.0, wind=(1.0, 0.0, 0.0)))
+    vals = [float(v @ (A @ v)) for v in (one, x, y)]
+    say('  no-backflow, wind (1,0,0): u=1, x, y ->', vals)
+    assert np.allclose(vals, [1.0, 1.0, 1.0 / 3.0], rtol=0, atol=1e-12)
+    f = C.LinearForm(sp, [C.BndSource(0, right, fn=(L.FN_POLY2, [0, 0, 1, 0, 0, 0, 0, 0, 0, 0]), coef=2.0)])
+    f.Assemble()
+    b = f.vec.cpu().numpy()
+    say('  boundary source 2y on x=1: b.1, b.y ->', b @ one, b @ y)
+    assert abs(b @ one - 1.0) <= 1e-12 and abs(b @ y - 2.0 / 3.0) <= 1e-12
+    m = Poisson(hm, dc=1.0, f=const(0.0), interp_ord=2, dirichlet={'u': {'left': const(1.0)}},
+                neumann={'u': {'right': const(1.0)}}, linear_solver='CG', linear_tolerance=1e-13, ctx=gpu_ctx)
+    s = StationarySolver(m)
+    u = s.solve()
+    e_l2, e_h1 = m.l2_error(u, 'u', (L.FN_POLY2, [1, 1, 0, 0, 0, 0, 0, 0, 0, 0]))
+    say('  neumann exact', s.result, 'L2', e_l2, 'H1', e_h1)
+    assert s.result.converged and e_l2 <= 1e-10 and e_h1 <= 1e-9
