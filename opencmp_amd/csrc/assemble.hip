@@ -163,7 +163,7 @@ __device__ __forceinline__ double field_at_facet(const KField &f, const double *
 // element's boundary facets whose marker is in T.dmask. Facet k is the edge (face) opposite vertex
 // k, n its outward unit normal; in 2-D the rule maps [0, 1] onto the edge.
 template <int D>
-__device__ void bnd_upwind_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
+__device__ void bnd_facet_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
                                double ce, double *r) {
   if constexpr (D == 2) {
     const int nlt = A.b.nloc[bi], nlj = A.b.nloc[bj];
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(256) k_h1_bnd_assemble(KArgs A, const double *
     const KTerm &T = A.t[tt];
     if (T.test != bi || (T.kind != OCMP_BFI_BND_UPWIND && T.kind != OCMP_BFI_BND_MASS)) continue;
     const double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
-    bnd_upwind_row<D>(A, T, g, e, bi, il, T.trial, ce, row + A.b.loc_off[T.trial]);
+    bnd_facet_row<D>(A, T, g, e, bi, il, T.trial, ce, row + A.b.loc_off[T.trial]);
     any = true;
   }
   if (!any) return;
