@@ -780,6 +780,63 @@ def assemble_dg_dirichlet_rhs(cs, comp, eps, alpha_coef, wind, g, dirichlet_mask
 # =============================================================== Dirichlet lift and solves
 
 
+def _bnd_facets(cs, comp, marker_mask, qdeg):
+    """Marked boundary edges of an H1 component (triangles): owner element, basis values at the edge
+    rule, rule weights times edge length, points and outward normals."""
+    s = cs.spaces[comp]
+    topo = cs.topo
+    if topo.dim != 2:
+        raise NotImplementedError
+    t, wq = line_rule(qdeg)
+    mk = topo.f_marker
+    sel = np.nonzero((topo.f_e1 < 0) & (mk >= 0) & (((marker_mask >> np.maximum(mk, 0)) & 1) == 1))[0]
+    e0, k0 = topo.f_e0[sel], topo.f_k0[sel]
+    X = topo.coords
+    fv = topo.facets[sel]
+    L = np.linalg.norm(X[fv[:, 1]] - X[fv[:, 0]], axis=1)
+    P = X[fv[:, 0]][:, None, :] * (1 - t)[None, :, None] + X[fv[:, 1]][:, None, :] * t[None, :, None]
+    gl = topo.grad_lam[e0, k0]
+    n = -gl / np.linalg.norm(gl, axis=1, keepdims=True)
+    phis = np.stack([ref_basis(s.kind, 2, s.order, _facet_lam(2, k, t))[0] for k in range(3)])
+    return s, e0, phis[k0], wq[None, :] * L[:, None], P, n
+
+
+def assemble_bnd_facets(asm, comp, coef, marker_mask, wind=None, qdeg=None):
+    """coef * max(w.n, 0) u v (wind given: the stress-BC no-backflow term,
+    opencmp/models/ins.py:210-216) or coef * u v (wind None: the Robin term,
+    opencmp/models/poisson.py:94-96) on the boundary edges whose marker is in marker_mask, every
+    copy of H1 component ``comp`` on its own block (triangles)."""
+    s = asm.cs.spaces[comp]
+    q = qdeg if qdeg is not None else 2 * s.order + 2
+    s, e0, phi, ww, P, n = _bnd_facets(asm.cs, comp, marker_mask, q)
+    if len(e0) == 0:
+        return
+    if wind is not None:
+        W = _wind_at(wind, asm.cs.topo, None, P)
+        ww = ww * np.maximum(np.einsum('fqd,fd->fq', W, n), 0.0)
+    Ke = coef * np.einsum('fq,fqi,fqj->fij', ww, phi, phi)
+    for c in range(s.ncomp):
+        d = s.edofs1[e0] + asm.cs.offsets[comp] + c * s.n1
+        nl = d.shape[1]
+        asm.add(np.repeat(d, nl, axis=1), np.tile(d, (1, nl)), Ke.reshape(len(e0), -1))
+
+
+def assemble_bnd_source(cs, comp, g, marker_mask, coef=1.0, qdeg=None):
+    """coef * int g v ds over the boundary edges whose marker is in marker_mask (Neumann and Robin
+    data, opencmp/models/poisson.py:151-158), g a number or g(x, y), first copy of H1 component
+    ``comp`` (triangles)."""
+    s = cs.spaces[comp]
+    q = qdeg if qdeg is not None else 2 * s.order + 4
+    s, e0, phi, ww, P, n = _bnd_facets(cs, comp, marker_mask, q)
+    b = np.zeros(cs.ndof)
+    if len(e0) == 0:
+        return b
+    gv = g(P[..., 0], P[..., 1]) if callable(g) else np.full(P.shape[:2], float(g))
+    loc = coef * np.einsum('fq,fqi->fi', ww * gv, phi)
+    np.add.at(b, s.edofs1[e0] + cs.offsets[comp], loc)
+    return b
+
+
 def h1_interpolate_vertices(space, g):
     """Values of g at the vertex dofs (higher-order dofs set to 0): exact for P1-representable data."""
     topo = space.topo

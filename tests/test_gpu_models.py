@@ -325,3 +325,39 @@ def test_boundary_terms_on_tetrahedra(gpu_ctx):
     e_l2, e_h1 = m.l2_error(u, 'u', (L.FN_POLY2, [1, 1, 0, 0, 0, 0, 0, 0, 0, 0]))
     say('  neumann exact', s.result, 'L2', e_l2, 'H1', e_h1)
     assert s.result.converged and e_l2 <= 1e-10 and e_h1 <= 1e-9
+
+
+def test_boundary_terms_match_oracle_on_channel(gpu_ctx):
+    """The boundary-facet terms against the oracle's restatement on the channel with holes: a straight
+    outlet and curved, body-fitted hole boundaries give every facet orientation (matrix and load
+    vector within relative Frobenius 1e-12)."""
+    from opencmp_amd import core as C
+    from opencmp_amd.mesh import channel_with_holes
+    say('boundary terms vs oracle on the channel')
+    hm = channel_with_holes(10)
+    gm = C.Mesh(hm, gpu_ctx)
+    sp = C.FESpace(gm, [C.H1(2, '')])
+    t = fem.Topology(hm)
+    cs = fem.Compound([fem.Space(t, 'H1', 2)])
+    mask = hm.marker_mask(['outlet', 'cyl'])
+    for kw, okw in (({'wind': (1.0, 0.3, 0.0)}, {'wind': (1.0, 0.3)}), ({}, {})):
+        term = C.BndUpwind(0, mask, coef=0.7, **kw) if kw else C.BndMass(0, mask, coef=0.7)
+        a = C.BilinearForm(sp, [term])
+        a.Assemble()
+        A = a.to_scipy()
+        asm = fem.Assembler(cs)
+        fem.assemble_bnd_facets(asm, 0, 0.7, mask, **okw)
+        R = asm.matrix()
+        rel = sp_norm(A - R) / sp_norm(R)
+        say('  ', 'upwind' if kw else 'mass', 'rel', rel)
+        assert rel <= 1e-12
+    f = C.LinearForm(sp, [C.BndSource(0, mask, fn=(L.FN_POLY2, [0.5, 1.0, -2.0, 0.3, 0.0, 1.0]), coef=1.3)])
+    f.Assemble()
+    ref = fem.assemble_bnd_source(cs, 0, lambda X, Y: 0.5 + X - 2 * Y + 0.3 * X * X + Y * Y, mask, coef=1.3)
+    rel = np.linalg.norm(f.vec.cpu().numpy() - ref) / np.linalg.norm(ref)
+    say('  source rel', rel)
+    assert rel <= 1e-12
+
+
+def sp_norm(M):
+    return float(np.sqrt((M.multiply(M)).sum()))

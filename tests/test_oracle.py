@@ -365,3 +365,35 @@ def test_cpu_restatement_solvers_match_oracle():
 def spl_solve(A, b):
     import scipy.sparse.linalg as spla
     return spla.spsolve(A.tocsc(), b)
+
+
+def test_boundary_terms_closed_forms():
+    """The oracle's boundary-facet terms (no-backflow, Robin mass, Neumann/Robin load) on the unit
+    square against closed forms: u^T A u = sum over edges of int max(w.n, 0) u^2 ds (u = x picks the
+    x = 1 edge, so a flipped normal fails) and b . u = int g u ds, for P2 vectors holding linear
+    functions (vertex values, edge coefficients 0)."""
+    hm = unit_square(6)
+    t = fem.Topology(hm)
+    cs = fem.Compound([fem.Space(t, 'H1', 2)])
+
+    def lin(f):
+        v = np.zeros(cs.ndof)
+        v[:t.nv] = f(t.coords[:, 0], t.coords[:, 1])
+        return v
+
+    one, x, y = lin(lambda a, b: np.ones_like(a)), lin(lambda a, b: a), lin(lambda a, b: b)
+    allm, right, top = hm.marker_mask(hm.bnd_names), hm.marker_mask(['right']), hm.marker_mask(['top'])
+    asm = fem.Assembler(cs)
+    fem.assemble_bnd_facets(asm, 0, 1.0, allm, wind=(1.0, 0.0))
+    A = asm.matrix()
+    assert np.allclose([v @ A @ v for v in (one, x, y)], [1.0, 1.0, 1.0 / 3.0], atol=1e-13)
+    asm = fem.Assembler(cs)
+    fem.assemble_bnd_facets(asm, 0, 2.0, allm, wind=(0.6, 0.8))
+    A = asm.matrix()
+    assert np.allclose([v @ A @ v for v in (one, x)], [2 * 1.4, 2 * (0.6 + 0.8 / 3.0)], atol=1e-13)
+    asm = fem.Assembler(cs)
+    fem.assemble_bnd_facets(asm, 0, 3.0, top)
+    A = asm.matrix()
+    assert abs(x @ A @ x - 1.0) <= 1e-13 and abs(abs(A - A.T).max()) <= 1e-15
+    b = fem.assemble_bnd_source(cs, 0, lambda a, c: 2 * c, right)
+    assert abs(b @ one - 1.0) <= 1e-13 and abs(b @ y - 2.0 / 3.0) <= 1e-13
