@@ -351,9 +351,10 @@ class Stokes(Model):
 class INS(Stokes):
     """opencmp/models/ins.py (CG, Oseen linearisation): adds -(u (x) w) : grad v with the wind w the
     previous iterate (:191-195); solved by the Picard loop of solve_single_step (:323-359).
-    ``stress``: {'u': {marker: traction}} boundaries with the stress condition; each gets the
-    no-backflow term dt max(w.n, 0) u.v (:210-216). Only zero traction (the do-nothing outlet)
-    is supported."""
+    ``stress``: {'u': {marker: traction}} boundaries with the stress condition: each gets the
+    no-backflow term dt max(w.n, 0) u.v (:210-216) and the load dt h.v ds of its traction h (one
+    closed form per velocity component, or one for all; None or zero: the do-nothing outlet,
+    :299-305)."""
     nonlinear = True
 
     def __init__(self, mesh, kv: float = 1e-3, nonlinear_tolerance=(1e-8, 1e-8), nonlinear_max_iters: int = 10,
@@ -361,11 +362,6 @@ class INS(Stokes):
         self.abs_tol, self.rel_tol = nonlinear_tolerance
         self.nonlinear_max_iters = nonlinear_max_iters
         self.stress = stress or {}
-        for comp, bcs in self.stress.items():
-            for marker, h in bcs.items():
-                fns = [] if h is None else (h if isinstance(h, list) else [h])   # one Fn or one per component
-                if any(float(c) != 0.0 for fn in fns for c in fn[1]):
-                    raise ValueError('only zero traction is supported on stress boundaries')
         super().__init__(mesh, kv=kv, **kw)
         self.W = C.GridFunction(self.fes)    # linearisation point (opencmp/models/ins.py:109-116)
 
@@ -381,6 +377,18 @@ class INS(Stokes):
             for b in ub:
                 terms.append(C.BndUpwind(b, mask, coef=dt, wind_gf=(self.W, ub)))
         return terms
+
+    def construct_linear(self, dt=1.0):
+        Lf = super().construct_linear(dt)
+        ub = self.fes.comp_blocks[0]
+        for marker, h in self.stress.get('u', {}).items():                 # ins.py:299-305
+            if h is None:
+                continue
+            fns = h if isinstance(h, list) else [h] * len(ub)             # one Fn or one per component
+            for b, fn in zip(ub, fns):
+                if any(float(c) != 0.0 for c in fn[1]):
+                    Lf.append(C.BndSource(b, self.host_mesh.marker_mask(marker), fn=fn, coef=dt))
+        return Lf
 
     def solve_single_step(self, a, Lf, pre, gfu, reassemble=None, assembled=False):
         """Picard iteration (opencmp/models/ins.py:323-359): re-assemble with the current wind,

@@ -361,3 +361,28 @@ def test_boundary_terms_match_oracle_on_channel(gpu_ctx):
 
 def sp_norm(M):
     return float(np.sqrt((M.multiply(M)).sum()))
+
+
+def test_ins_stress_traction_load(gpu_ctx):
+    """The load side of the INS stress condition, dt h.v ds (opencmp/models/ins.py:299-305): with
+    h = (1, 2y) on the right edge of the unit square and dt = 0.5, the x block of the load against
+    the interpolant of 1 gives 0.5 * int 1 ds = 0.5 and the y block against the interpolant of y
+    gives 0.5 * int 2 y^2 ds = 1/3; the zero traction adds nothing."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import INS, const
+    say('ins stress traction')
+    hm = unit_square(6)
+    h = [const(1.0), (L.FN_POLY2, [0, 0, 2.0, 0, 0, 0])]
+    m = INS(hm, kv=1.0, interp_ord=2, stress={'u': {'right': h, 'top': [const(0.0), const(0.0)]}}, ctx=gpu_ctx)
+    f = C.LinearForm(m.fes, m.construct_linear(0.5))
+    f.Assemble()
+    b = f.vec.cpu().numpy()
+    off = m.fes.block_offsets()
+    bx, by = m.fes.comp_blocks[0]
+    X, Y = hm.coords[:, 0], hm.coords[:, 1]
+    nv = len(X)
+    vx = b[off[bx]:off[bx] + nv].sum()              # vertex dofs only: the interpolant of 1
+    vy = b[off[by]:off[by] + nv] @ Y
+    say('  x block . 1', vx, 'y block . y', vy)
+    assert abs(vx - 0.5) <= 1e-13 and abs(vy - 1.0 / 3.0) <= 1e-13
+    assert np.abs(b[off[bx + 2]:]).max() == 0.0     # pressure block untouched
