@@ -186,8 +186,34 @@ enum {
   OCMP_FN_SINPROD = 1,   /* c[0] * prod_d sin(pi x_d)                           */
   OCMP_FN_POLY2 = 2,     /* 2D: c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2;
                             3D: c0 + c1 x + c2 y + c3 z + c4 x^2 + c5 y^2 + c6 z^2 + c7 xy + c8 xz + c9 yz */
-  OCMP_FN_FIELD = 3      /* the field `field` times c[0]                        */
+  OCMP_FN_FIELD = 3,     /* the field `field` times c[0]                        */
+  OCMP_FN_EXPR = 4       /* c[2] * P(x, y, z, t = c[1]) + c[3] for the program P registered as
+                            c[0] by ocmp_expr_create (the config expressions of
+                            opencmp/config_functions/parse_arithmetic.py:37-268)  */
 };
+
+/* Postfix programs of OCMP_FN_EXPR: one double per opcode, OCMP_OP_CONST followed by its value.
+ * Unary ops replace the top of the stack, binary ops pop b then a and push a op b; HS2 pops
+ * (t, w), RAMP4 pops (t, p1, p2, tr). sig, tanh, H and ramp are opencmp/helpers/math.py:23-131
+ * (tanh clamps its argument at 350, H_s shifts by 9.21025, ramp is ramp_cos); sgn uses the
+ * 1e-12 dead band of parse_arithmetic.py; round is round-half-even, as Python's. */
+enum {
+  OCMP_OP_END = 0, OCMP_OP_CONST = 1, OCMP_OP_X = 2, OCMP_OP_Y = 3, OCMP_OP_Z = 4, OCMP_OP_T = 5,
+  OCMP_OP_NEG = 10, OCMP_OP_ADD = 11, OCMP_OP_SUB = 12, OCMP_OP_MUL = 13, OCMP_OP_DIV = 14, OCMP_OP_POW = 15,
+  OCMP_OP_SIN = 20, OCMP_OP_COS = 21, OCMP_OP_TAN = 22, OCMP_OP_EXP = 23, OCMP_OP_SQRT = 24,
+  OCMP_OP_TANH = 25, OCMP_OP_SIG = 26, OCMP_OP_ABS = 27, OCMP_OP_SGN = 28, OCMP_OP_TRUNC = 29,
+  OCMP_OP_ROUND = 30,
+  OCMP_OP_HS = 40, OCMP_OP_HS2 = 41,    /* H(t), H(t, w)                         */
+  OCMP_OP_RAMP = 50, OCMP_OP_RAMP4 = 53 /* ramp(t), ramp(t, p1, p2, tr)          */
+};
+
+/* Registers a postfix program (n doubles ending in OCMP_OP_END, at most 8 stack slots deep) on the
+ * context's device and returns its id for c[0] of OCMP_FN_EXPR. Replaces the evaluation of
+ * parsed config expressions into NGSolve CoefficientFunctions
+ * (opencmp/config_functions/parse_arithmetic.py:122-268) for the GPU path: the kernels evaluate the
+ * program at their own quadrature and interpolation points. Programs stay registered for the
+ * life of the process (64K doubles per device in all). */
+int ocmp_expr_create(ocmp_ctx *ctx, int32_t n, const double *code, int32_t *id);
 
 /* One linear-form term: coef * w * f * v (SOURCE), or the DG Dirichlet data
  * terms eps*g*(alpha/h v - grad v.n) - v g min(w.n,0) on boundary facets in
@@ -214,10 +240,19 @@ int ocmp_assemble_linear(ocmp_space *s, double *b, int nterms, const ocmp_lterm 
 /* Replaces gfu.Set(g, definedon=mesh.Boundaries(...)) of
  * Model.apply_dirichlet_bcs_to (opencmp/models/base_model.py:321-341) for one
  * scalar H1 block: vertex dofs on the given markers take g's vertex values,
- * edge dofs the 1D L2 projection of g minus its vertex interpolant; 3D face
- * dofs (order 3) are set to zero. x: device f64[ndof]. */
+ * edge dofs the 1D L2 projection of g minus its vertex interpolant, 3D face
+ * dofs (order 3) the L2 projection over the face of g minus its vertex and edge
+ * parts onto the face bubble. g: closed form or expression (OCMP_FN_EXPR).
+ * x: device f64[ndof]. */
 int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double *c,
                        uint64_t marker_mask);
+
+/* gfu.Set(g) on one scalar block over the whole domain (the initial conditions of
+ * opencmp/models/base_model.py:216-221 and the interpolation of _l_inf, opencmp/helpers/error.py:
+ * 80-88): on H1 blocks (order 1-3) the vertex values, then per edge and per triangle with a
+ * face-bubble dof (3D faces, 2D cells) the L2 projection of what the lower entities leave; on
+ * L2 blocks (triangles) the element-wise L2 projection. g: closed form or expression. Syncs. */
+int ocmp_set(ocmp_space *s, int block, double *x, int fn, const double *c);
 
 /* y = A x; x, y device f64[n]. Replaces a.mat * vec (opencmp/models/base_model.py:920). */
 int ocmp_matvec(ocmp_matrix *A, const double *x, double *y);
@@ -234,6 +269,15 @@ int ocmp_axpby(ocmp_ctx *ctx, int64_t n, double a, const double *x, double b, do
  * The H1-seminorm has no counterpart in the reference. Synchronises. */
 int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *ref, int fn, const double *c,
                      double *out);
+
+/* Norms of one vector component (nblocks = 1..3 scalar blocks of one kind and order) against
+ * closed forms or expressions (fns[k], c[10 k .. 10 k + 9] per block) or, with ref != NULL,
+ * against the vector ref on the same space: out[0] = sqrt(int |u_h - g|^2) (_l_2), out[1] =
+ * int |u_h - g| with the pointwise Euclidean norm (_l_1 of opencmp/helpers/error.py:70-72,
+ * listed by calc_error :252), out[2] = sqrt(int |grad(u_h - g)|^2) (no reference counterpart;
+ * NaN against an expression, whose derivative is not available). Synchronises. */
+int ocmp_error_norms_vec(ocmp_space *s, int nblocks, const int32_t *blocks, const double *x, const double *ref,
+                         const int32_t *fns, const double *c, double *out);
 
 /* out[0] = sqrt(int (div u_h)^2) for the vector field in the dim consecutive scalar
  * blocks block0 .. block0+dim-1 of x (one H1 vector component): _divergence of

@@ -17,7 +17,7 @@ OCMP_H1, OCMP_L2 = 0, 1
 BFI_GRADGRAD, BFI_MASS, BFI_CONV, BFI_DTRIAL, BFI_DTEST, BFI_BND_UPWIND, BFI_BND_MASS, BFI_DG_FACET = 0, 1, 2, 3, 4, 5, 6, 16
 WEIGHT_NONE, WEIGHT_FIELD, WEIGHT_ONE_MINUS_FIELD = 0, 1, 2
 LFI_SOURCE, LFI_DG_DIRICHLET, LFI_BND_SOURCE = 0, 1, 2
-FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD = 0, 1, 2, 3
+FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD, FN_EXPR = 0, 1, 2, 3, 4
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI = 0, 1, 2
 FP64, MIXED = 0, 1
 SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON, SOLVER_FGMRES = 0, 1, 2, 3
@@ -93,6 +93,10 @@ SIGNATURES = {
     'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
+    'ocmp_error_norms_vec': [P, C.c_int, C.POINTER(C.c_int32), P, P, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                             C.POINTER(C.c_double)],
+    'ocmp_set': [P, C.c_int, P, C.c_int, C.POINTER(C.c_double)],
+    'ocmp_expr_create': [P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)],
     'ocmp_divergence_norm': [P, C.c_int, P, C.POINTER(C.c_double)],
     'ocmp_linf_norm': [P, C.c_int, P, P, C.POINTER(C.c_double)],
     'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],

@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from .expr import resolve_fn
 from .mesh import Mesh as HostMesh
 
 
@@ -192,18 +193,26 @@ class GridFunction:
     def field(self, block: int) -> L.Field:
         return L.Field(self.space.h, block, 0, _ptr(self.vec))
 
-    def Set(self, fn: Tuple[int, Sequence[float]], boundaries: str, component: int = 0):
-        """gfu.Set(g, definedon=mesh.Boundaries(...)) for a closed-form g on each block of a component.
+    def Set(self, fn, boundaries: Optional[str] = None, component: int = 0):
+        """gfu.Set(g, definedon=mesh.Boundaries(...)) for a closed-form or expression g on each block of a
+        component; without ``boundaries``, gfu.Set(g) over the whole domain (projection-based
+        interpolation on H1 blocks, element-wise L2 projection on L2 blocks).
 
-        fn = (FN_*, params); for a vector component pass a list of fns, one per block.
+        fn = (FN_*, params) or a bound expression; for a vector component pass a list, one per block.
         """
-        mask = self.space.mesh.marker_mask(boundaries)
         blocks = self.space.comp_blocks[component]
         fns = fn if isinstance(fn, list) else [fn] * len(blocks)
-        for blk, (fid, params) in zip(blocks, fns):
+        if len(fns) != len(blocks):
+            raise ValueError(f'{len(fns)} functions for a component of {len(blocks)} scalar blocks')
+        mask = None if boundaries is None else self.space.mesh.marker_mask(boundaries)
+        for blk, f in zip(blocks, fns):
+            fid, params = resolve_fn(f)
             c = (C.c_double * 10)(*([float(v) for v in params] + [0.0] * (10 - len(params))))
-            L.check(self.space.ctx.lib.ocmp_dirichlet_set(self.space.h, blk, C.c_void_p(_ptr(self.vec)), fid, c,
-                                                          mask), 'dirichlet_set')
+            if mask is None:
+                L.check(self.space.ctx.lib.ocmp_set(self.space.h, blk, C.c_void_p(_ptr(self.vec)), fid, c), 'set')
+            else:
+                L.check(self.space.ctx.lib.ocmp_dirichlet_set(self.space.h, blk, C.c_void_p(_ptr(self.vec)), fid,
+                                                              c, mask), 'dirichlet_set')
 
     def numpy(self) -> np.ndarray:
         return self.vec.detach().cpu().numpy()
@@ -466,8 +475,9 @@ class LinearForm:
                         lt.field = ft.index(gf, blk)
                         lt.c[0] = 1.0
                     else:
-                        lt.fn = t.fn[0]
-                        for k, v in enumerate(t.fn[1]):
+                        fid, prm = resolve_fn(t.fn)
+                        lt.fn = fid
+                        for k, v in enumerate(prm):
                             lt.c[k] = v
                     if t.weight_gf is not None:
                         gf, blk, mode = t.weight_gf
@@ -478,8 +488,9 @@ class LinearForm:
                     lt.test = t.test
                     lt.coef = t.coef
                     lt.dirichlet_mask = t.markers
-                    lt.fn = t.fn[0]
-                    for k, v in enumerate(t.fn[1]):
+                    fid, prm = resolve_fn(t.fn)
+                    lt.fn = fid
+                    for k, v in enumerate(prm):
                         lt.c[k] = v
                 else:
                     lt.kind = L.LFI_DG_DIRICHLET
@@ -489,8 +500,9 @@ class LinearForm:
                     lt.alpha = t.alpha
                     lt.wind[0], lt.wind[1] = t.wind[0], t.wind[1]
                     lt.dirichlet_mask = t.dirichlet
-                    lt.fn = t.fn[0]
-                    for k, v in enumerate(t.fn[1]):
+                    fid, prm = resolve_fn(t.fn)
+                    lt.fn = fid
+                    for k, v in enumerate(prm):
                         lt.c[k] = v
                 arr[i] = lt
             farr, nf = ft.array()

@@ -946,9 +946,27 @@ int ocmp_dirichlet_set(ocmp_space *s, int block, double *x, int fn, const double
     OCMP_REQUIRE(s && x && c, "NULL argument");
     OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
     OCMP_REQUIRE(s->s.blocks[block].kind == OCMP_H1, "Dirichlet values apply to H1 blocks");
-    OCMP_REQUIRE(fn == OCMP_FN_CONST || fn == OCMP_FN_SINPROD || fn == OCMP_FN_POLY2, "unsupported function");
+    OCMP_REQUIRE(fn != OCMP_FN_FIELD, "Dirichlet data must be a closed form or an expression");
     OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
     launch_dirichlet_set(&s->s, block, x, fn, c, marker_mask);
+  });
+}
+
+int ocmp_set(ocmp_space *s, int block, double *x, int fn, const double *c) {
+  return guard([&] {
+    OCMP_REQUIRE(s && x && c, "NULL argument");
+    OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
+    OCMP_REQUIRE(fn != OCMP_FN_FIELD, "Set takes a closed form or an expression");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    launch_set(&s->s, block, x, fn, c);
+  });
+}
+
+int ocmp_expr_create(ocmp_ctx *ctx, int32_t n, const double *code, int32_t *id) {
+  return guard([&] {
+    OCMP_REQUIRE(ctx && code && id, "NULL argument");
+    OCMP_HIP(hipSetDevice(ctx->c.device));
+    *id = expr_register(&ctx->c, n, code);
   });
 }
 
@@ -957,9 +975,18 @@ int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *re
   return guard([&] {
     OCMP_REQUIRE(s && x && c && out, "NULL argument");
     OCMP_REQUIRE(block >= 0 && block < (int)s->s.blocks.size(), "block out of range");
-    OCMP_REQUIRE(fn == OCMP_FN_CONST || fn == OCMP_FN_SINPROD || fn == OCMP_FN_POLY2, "unsupported function");
+    OCMP_REQUIRE(ref || fn != OCMP_FN_FIELD, "a field reference is passed as ref");
     OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
     error_norms(&s->s, block, x, ref, fn, c, out);
+  });
+}
+
+int ocmp_error_norms_vec(ocmp_space *s, int nblocks, const int32_t *blocks, const double *x, const double *ref,
+                         const int32_t *fns, const double *c, double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && blocks && x && out && (ref || (fns && c)), "NULL argument");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    error_norms_vec(&s->s, nblocks, blocks, x, ref, fns, c, out);
   });
 }
 

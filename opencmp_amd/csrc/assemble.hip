@@ -14,6 +14,10 @@
 //    dofs. Each value is written exactly once; no colouring, no atomics.
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
 
 #include "internal.h"
 
@@ -122,6 +126,80 @@ __device__ __forceinline__ void geometry(const double *__restrict__ coords, cons
   }
 }
 
+// ------------------------------------------------------------------ expressions (OCMP_FN_EXPR)
+//
+// Coefficient, boundary-data and reference-solution expressions of the OpenCMP config language
+// (opencmp/config_functions/parse_arithmetic.py:37-268) arrive compiled to postfix programs
+// (opcodes OCMP_OP_* with inline constants, see ocmp.h). Programs live in one table per device
+// and are evaluated per point by a stack machine whose 8 stack slots are named registers: a push
+// or pop shifts them, so the stack never goes to scratch memory.
+
+constexpr int EXPR_CAP = 1 << 16;  // doubles of program text per device
+constexpr int EXPR_DEPTH = 8;
+static __device__ double expr_code[EXPR_CAP];
+
+// sigmoid and hyperbolic tangent as the reference writes them (opencmp/helpers/math.py:23-59)
+__device__ __forceinline__ double ref_sig(double x) { return 1.0 / (1.0 + exp(-x)); }
+__device__ __forceinline__ double ref_tanh(double x) {
+  const double xa = x - 350.0 > 0.0 ? 350.0 : x;
+  const double e2 = exp(2.0 * xa);
+  return (e2 - 1.0) / (e2 + 1.0);
+}
+
+__device__ double eval_expr(int pc, double t, const double *x, int dim) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
+  for (;;) {
+    const int op = (int)expr_code[pc++];
+    if (op == OCMP_OP_END) return s0;
+    if (op <= OCMP_OP_T) {
+      double v;
+      if (op == OCMP_OP_CONST) v = expr_code[pc++];
+      else if (op == OCMP_OP_X) v = x[0];
+      else if (op == OCMP_OP_Y) v = dim > 1 ? x[1] : 0.0;
+      else if (op == OCMP_OP_Z) v = dim > 2 ? x[2] : 0.0;
+      else v = t;
+      s7 = s6; s6 = s5; s5 = s4; s4 = s3; s3 = s2; s2 = s1; s1 = s0; s0 = v;
+    } else if (op >= OCMP_OP_ADD && op <= OCMP_OP_POW) {
+      const double a = s1, b = s0;
+      double r;
+      if (op == OCMP_OP_ADD) r = a + b;
+      else if (op == OCMP_OP_SUB) r = a - b;
+      else if (op == OCMP_OP_MUL) r = a * b;
+      else if (op == OCMP_OP_DIV) r = a / b;
+      else r = pow(a, b);
+      s0 = r; s1 = s2; s2 = s3; s3 = s4; s4 = s5; s5 = s6; s6 = s7;
+    } else if (op == OCMP_OP_NEG || (op >= OCMP_OP_SIN && op <= OCMP_OP_ROUND) || op == OCMP_OP_HS ||
+               op == OCMP_OP_RAMP) {
+      const double a = s0;
+      double r;
+      switch (op) {
+        case OCMP_OP_NEG: r = -a; break;
+        case OCMP_OP_SIN: r = sin(a); break;
+        case OCMP_OP_COS: r = cos(a); break;
+        case OCMP_OP_TAN: r = tan(a); break;
+        case OCMP_OP_EXP: r = exp(a); break;
+        case OCMP_OP_SQRT: r = sqrt(a); break;
+        case OCMP_OP_TANH: r = ref_tanh(a); break;
+        case OCMP_OP_SIG: r = ref_sig(a); break;
+        case OCMP_OP_ABS: r = fabs(a); break;
+        case OCMP_OP_SGN: r = a < -1e-12 ? -1.0 : (a > 1e-12 ? 1.0 : 0.0); break;
+        case OCMP_OP_TRUNC: r = trunc(a); break;
+        case OCMP_OP_ROUND: r = rint(a); break;
+        case OCMP_OP_HS: r = ref_sig(2.0 * 9.21025 / 0.1 * a - 9.21025); break;  // H_s(t), w = 0.1
+        default: r = 1.0 - a > 0.0 ? 0.5 * (0.0 - 1.0) * cos(a * M_PI) + 0.5 : 1.0;  // ramp_cos(t)
+      }
+      s0 = r;
+    } else if (op == OCMP_OP_HS2) {  // H_s(t, w)
+      const double r = ref_sig(2.0 * 9.21025 / s0 * s1 - 9.21025);
+      s0 = r; s1 = s2; s2 = s3; s3 = s4; s4 = s5; s5 = s6; s6 = s7;
+    } else {  // OCMP_OP_RAMP4: ramp_cos(t, p1, p2, tr), opencmp/helpers/math.py:116-131
+      const double tt = s3, p1 = s2, p2 = s1, tr = s0;
+      const double r = tr - tt > 0.0 ? 0.5 * (p1 - p2) * cos(tt * M_PI / tr) + 0.5 * (p1 + p2) : p2;
+      s0 = r; s1 = s4; s2 = s5; s3 = s6; s4 = s7;
+    }
+  }
+}
+
 __device__ __forceinline__ double eval_fn(int fn, const double *c, const double *x, int dim) {
   if (fn == OCMP_FN_CONST) return c[0];
   if (fn == OCMP_FN_SINPROD) {
@@ -129,6 +207,7 @@ __device__ __forceinline__ double eval_fn(int fn, const double *c, const double 
     for (int d = 0; d < dim; ++d) s *= sin(M_PI * x[d]);
     return s;
   }
+  if (fn == OCMP_FN_EXPR) return eval_expr((int)c[0], c[1], x, dim) * c[2] + c[3];
   // POLY2
   if (dim == 2) {
     double X = x[0], Y = x[1];
@@ -756,13 +835,19 @@ __global__ void __launch_bounds__(256) k_linear(KLArgs A, const double *__restri
 
 // ------------------------------------------------------------------ Dirichlet lift
 
+// 6-point Gauss-Legendre rule on [0, 1]
+__constant__ double GL6_T[6] = {0.033765242898423986, 0.16939530676686776, 0.38069040695840156,
+                                0.61930959304159844, 0.83060469323313224, 0.96623475710157603};
+__constant__ double GL6_W[6] = {0.085662246189585172, 0.18038078652406930, 0.23395696728634552,
+                                0.23395696728634552, 0.18038078652406930, 0.085662246189585172};
+
 __global__ void k_dir_verts(const int32_t *__restrict__ verts, int64_t nv, const double *__restrict__ coords, int dim,
                             int fn, const double *cc, int64_t off, double *__restrict__ x) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nv) return;
   double c[10];
   for (int i = 0; i < 10; ++i) c[i] = cc[i];
-  const int32_t v = verts[t];
+  const int32_t v = verts ? verts[t] : (int32_t)t;
   x[off + v] = eval_fn(fn, c, coords + (int64_t)v * dim, dim);
 }
 
@@ -774,7 +859,7 @@ __global__ void k_dir_edges(const int32_t *__restrict__ eids, int64_t ne, const 
   if (t >= ne) return;
   double c[10];
   for (int i = 0; i < 10; ++i) c[i] = cc[i];
-  const int32_t ei = eids[t];
+  const int32_t ei = eids ? eids[t] : (int32_t)t;
   const int32_t a = edges[2 * ei], b = edges[2 * ei + 1];
   double xa[3] = {0, 0, 0}, xb[3] = {0, 0, 0};
   for (int d = 0; d < dim; ++d) {
@@ -783,10 +868,7 @@ __global__ void k_dir_edges(const int32_t *__restrict__ eids, int64_t ne, const 
   }
   const double ga = eval_fn(fn, c, xa, dim), gb = eval_fn(fn, c, xb, dim);
   // 6-point Gauss-Legendre on [0,1]
-  const double gt[6] = {0.033765242898423986, 0.16939530676686776, 0.38069040695840156,
-                        0.61930959304159844, 0.83060469323313224, 0.96623475710157603};
-  const double gw[6] = {0.085662246189585172, 0.18038078652406930, 0.23395696728634552,
-                        0.23395696728634552, 0.18038078652406930, 0.085662246189585172};
+  const double *gt = GL6_T, *gw = GL6_W;
   double r2 = 0.0, r3 = 0.0, m2 = 0.0, m3 = 0.0;
   for (int q = 0; q < 6; ++q) {
     const double s = gt[q];
@@ -806,17 +888,108 @@ __global__ void k_dir_edges(const int32_t *__restrict__ eids, int64_t ne, const 
   if (order >= 3) x[base + 1] = r3 / m3;
 }
 
-__global__ void k_dir_faces(const int32_t *__restrict__ fids, int64_t nf, int64_t base, double *__restrict__ x) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The face-bubble dof (lambda_a lambda_b lambda_c, global vertex ids a < b < c) of triangles whose
+// vertex and edge dofs are already set: the L2 projection over the triangle of g minus its vertex
+// and edge parts onto the bubble. The 3D faces of a P3 lift or set, and the cell bubbles of a 2D P3
+// set. fids: the triangles (nullptr: 0..nf-1), fverts: their vertex triples, fedges[3 t + k]: the
+// edge ids of (a,b), (a,c), (b,c) of the t-th listed triangle. Collapsed 6x6 Gauss rule (exact to
+// degree 10); the constant Jacobian of the triangle cancels in the quotient.
+__global__ void k_set_faces(const int32_t *__restrict__ fids, int64_t nf, const int32_t *__restrict__ fverts,
+                            const int32_t *__restrict__ fedges, const double *__restrict__ coords, int dim, int fn,
+                            const double *cc, int64_t off, int64_t nvtx, int64_t face_base, double *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nf) return;
-  x[base + fids[t]] = 0.0;
+  double c[10];
+  for (int i = 0; i < 10; ++i) c[i] = cc[i];
+  const int64_t f = fids ? fids[t] : t;
+  const int32_t va = fverts[3 * f], vb = fverts[3 * f + 1], vc = fverts[3 * f + 2];
+  double X[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int d = 0; d < dim; ++d) {
+    X[0][d] = coords[(int64_t)va * dim + d];
+    X[1][d] = coords[(int64_t)vb * dim + d];
+    X[2][d] = coords[(int64_t)vc * dim + d];
+  }
+  const double ga = x[off + va], gb = x[off + vb], gc = x[off + vc];
+  double e2[3], e3[3];
+  for (int k = 0; k < 3; ++k) {
+    const int64_t eb = off + nvtx + 2 * (int64_t)fedges[3 * t + k];
+    e2[k] = x[eb];
+    e3[k] = x[eb + 1];
+  }
+  const double *gt = GL6_T, *gw = GL6_W;
+  double num = 0.0, den = 0.0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      const double lb = gt[i], lc = gt[j] * (1.0 - gt[i]), la = 1.0 - lb - lc;
+      const double w = gw[i] * gw[j] * (1.0 - gt[i]);
+      double xq[3];
+      for (int d = 0; d < 3; ++d) xq[d] = la * X[0][d] + lb * X[1][d] + lc * X[2][d];
+      double r = eval_fn(fn, c, xq, dim) - (ga * la + gb * lb + gc * lc);
+      r -= la * lb * (e2[0] + e3[0] * (lb - la));
+      r -= la * lc * (e2[1] + e3[1] * (lc - la));
+      r -= lb * lc * (e2[2] + e3[2] * (lc - lb));
+      const double B = la * lb * lc;
+      num += w * r * B;
+      den += w * B * B;
+    }
+  x[face_base + f] = num / den;
+}
+
+// Element-wise L2 projection of g onto an L2 block (GridFunction.Set on a DG space):
+// coefficients M_ref^{-1} sum_q w_q g(x_q) phi_j(q) per element (the element's |det J| cancels)
+struct KL2Set {
+  int nq, nl, ld, loc_off;
+  int tab_phi, tab_w, tab_lam, tab_minv;
+  int fn;
+  double c[10];
+  const double *tab;
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_l2_set(KL2Set K, int64_t nel, const double *__restrict__ coords,
+                                                const int32_t *__restrict__ elems, const int32_t *__restrict__ edofs,
+                                                double *__restrict__ x) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  Geo<D> g;
+  geometry<D>(coords, elems + e * (D + 1), g);
+  const double *phi = K.tab + K.tab_phi;
+  const double *w = K.tab + K.tab_w;
+  const double *lam = K.tab + K.tab_lam;
+  const double *minv = K.tab + K.tab_minv;
+  double r[MAXNB];
+#pragma unroll
+  for (int j = 0; j < MAXNB; ++j) r[j] = 0.0;
+  for (int q = 0; q < K.nq; ++q) {
+    double xq[D];
+    for (int cc = 0; cc < D; ++cc) {
+      double sx = g.x0[cc];
+      for (int a = 0; a < D; ++a) sx += g.J[cc][a] * lam[q * (D + 1) + a + 1];
+      xq[cc] = sx;
+    }
+    const double gv = w[q] * eval_fn(K.fn, K.c, xq, D);
+#pragma unroll
+    for (int j = 0; j < MAXNB; ++j)
+      if (j < K.nl) r[j] += gv * phi[q * K.nl + j];
+  }
+  const int32_t *ed = edofs + e * K.ld + K.loc_off;
+  for (int i = 0; i < K.nl; ++i) {
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXNB; ++j)
+      if (j < K.nl) v += minv[i * K.nl + j] * r[j];
+    x[ed[i]] = v;
+  }
 }
 
 // ------------------------------------------------------------------ error norms
 
 __device__ __forceinline__ void eval_fn_grad(int fn, const double *c, const double *x, int dim, double *g) {
   for (int d = 0; d < dim; ++d) g[d] = 0.0;
-  if (fn == OCMP_FN_SINPROD) {
+  if (fn == OCMP_FN_EXPR) {
+    // programs carry no derivative: the H1 seminorm against an expression is undefined
+    for (int d = 0; d < dim; ++d) g[d] = __builtin_nan("");
+  } else if (fn == OCMP_FN_SINPROD) {
     for (int d = 0; d < dim; ++d) {
       double s = c[0] * M_PI * cos(M_PI * x[d]);
       for (int e = 0; e < dim; ++e)
@@ -904,6 +1077,113 @@ __global__ void __launch_bounds__(256) k_error_norms(KNorm K, int64_t nel, const
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
     part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// A vector component (ncomp <= 3 scalar blocks of one kind and order) against closed forms or a
+// reference vector: per-block partial sums of int |u_h - g|^2, int |u_h - g| (the pointwise
+// Euclidean norm: opencmp/helpers/error.py:70-72 takes sqrt((u - g).(u - g))) and
+// int |grad(u_h - g)|^2
+struct KNormV {
+  int nq, nl, ld, ncomp;
+  int loc_off[3];
+  int tab_phi, tab_grad, tab_w, tab_lam;
+  int fn[3];
+  double c[3][10];
+  const double *tab;
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_error_norms_vec(KNormV K, int64_t nel, const double *__restrict__ coords,
+                                                         const int32_t *__restrict__ elems,
+                                                         const int32_t *__restrict__ edofs,
+                                                         const double *__restrict__ x, const double *__restrict__ ref,
+                                                         double *__restrict__ part) {
+  __shared__ double sh[3][4];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e2 = 0.0, l1 = 0.0, h2 = 0.0;
+  if (e < nel) {
+    Geo<D> g;
+    geometry<D>(coords, elems + e * (D + 1), g);
+    const double *phi = K.tab + K.tab_phi;
+    const double *gr = K.tab + K.tab_grad;
+    const double *w = K.tab + K.tab_w;
+    const double *lam = K.tab + K.tab_lam;
+    for (int q = 0; q < K.nq; ++q) {
+      double xq[D];
+      for (int c = 0; c < D; ++c) {
+        double sx = g.x0[c];
+        for (int a = 0; a < D; ++a) sx += g.J[c][a] * lam[q * (D + 1) + a + 1];
+        xq[c] = sx;
+      }
+      double sq = 0.0, hq = 0.0;
+      for (int k = 0; k < K.ncomp; ++k) {
+        const int32_t *ed = edofs + e * K.ld + K.loc_off[k];
+        double u = 0.0, gref[D];
+        for (int a = 0; a < D; ++a) gref[a] = 0.0;
+        for (int l = 0; l < K.nl; ++l) {
+          const double dv = x[ed[l]] - (ref ? ref[ed[l]] : 0.0);
+          u += phi[q * K.nl + l] * dv;
+          for (int a = 0; a < D; ++a) gref[a] += gr[(q * K.nl + l) * D + a] * dv;
+        }
+        double gu[D];
+        for (int c = 0; c < D; ++c) {
+          double sg = 0.0;
+          for (int a = 0; a < D; ++a) sg += g.Jinv[a][c] * gref[a];
+          gu[c] = sg;
+        }
+        if (!ref) {
+          u -= eval_fn(K.fn[k], K.c[k], xq, D);
+          double gg[3];
+          eval_fn_grad(K.fn[k], K.c[k], xq, D, gg);
+          for (int c = 0; c < D; ++c) gu[c] -= gg[c];
+        }
+        sq += u * u;
+        for (int c = 0; c < D; ++c) hq += gu[c] * gu[c];
+      }
+      const double wq = w[q] * g.adet;
+      e2 += wq * sq;
+      l1 += wq * sqrt(sq);
+      h2 += wq * hq;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    e2 += __shfl_down(e2, o, 64);
+    l1 += __shfl_down(l1, o, 64);
+    h2 += __shfl_down(h2, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = e2;
+    sh[1][threadIdx.x >> 6] = l1;
+    sh[2][threadIdx.x >> 6] = h2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
+}
+
+// out = (sqrt(sum part[3i]), sum part[3i+1], sqrt(sum part[3i+2])), one block, fixed order
+__global__ void k_sum_triples(const double *__restrict__ part, int64_t np, double *__restrict__ out) {
+  __shared__ double sh[3][256];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += blockDim.x) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  sh[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(sh[0][0]);
+    out[1] = sh[1][0];
+    out[2] = sqrt(sh[2][0]);
   }
 }
 
@@ -1389,7 +1669,7 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
     k.alpha = t.alpha;
     for (int c = 0; c < 3; ++c) k.wind[c] = t.wind[c];
     k.dmask = t.dirichlet_mask;
-    OCMP_REQUIRE(t.fn >= OCMP_FN_CONST && t.fn <= OCMP_FN_FIELD, "unknown function id");
+    check_fn(s.mesh->ctx, t.fn, t.c);
     if (t.kind == OCMP_LFI_DG_DIRICHLET) {
       OCMP_REQUIRE(d == 2, "DG Dirichlet data on triangles only");
       OCMP_REQUIRE(t.fn != OCMP_FN_FIELD, "DG Dirichlet data must be a closed-form function");
@@ -1471,12 +1751,56 @@ void launch_assemble_linear(Space *sp, double *bvec, const std::vector<ocmp_lter
   OCMP_HIP(hipGetLastError());
 }
 
+// edge id of the vertex pair (a, b) (edges are sorted pairs)
+static int64_t find_edge(const Mesh &m, int32_t a, int32_t b) {
+  if (a > b) std::swap(a, b);
+  int64_t lo = 0, hi = m.nedge;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (m.edges[2 * mid] < a || (m.edges[2 * mid] == a && m.edges[2 * mid + 1] < b))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  OCMP_REQUIRE(lo < m.nedge && m.edges[2 * lo] == a && m.edges[2 * lo + 1] == b, "edge not in the mesh");
+  return lo;
+}
+
+// edge ids of (a,b), (a,c), (b,c) of the triangles tri[3 t ..] (ascending vertex ids), for the
+// listed triangles (ids) or all ntri of them (ids empty)
+static std::vector<int32_t> triangle_edges(const Mesh &m, const int32_t *tri, int64_t ntri,
+                                           const std::vector<int32_t> &ids) {
+  const int64_t n = ids.empty() ? ntri : (int64_t)ids.size();
+  std::vector<int32_t> out((size_t)n * 3);
+  for (int64_t t = 0; t < n; ++t) {
+    const int32_t *v = tri + 3 * (int64_t)(ids.empty() ? t : ids[t]);
+    OCMP_REQUIRE(v[0] < v[1] && v[1] < v[2], "triangle vertices must be ascending");
+    out[3 * t] = (int32_t)find_edge(m, v[0], v[1]);
+    out[3 * t + 1] = (int32_t)find_edge(m, v[0], v[2]);
+    out[3 * t + 2] = (int32_t)find_edge(m, v[1], v[2]);
+  }
+  return out;
+}
+
+static void ensure_tri_tables(Space &s) {
+  if (s.d_tri_verts.p) return;
+  const Mesh &m = *s.mesh;
+  hipStream_t st = m.ctx->stream;
+  const std::vector<int32_t> &tri = m.dim == 3 ? m.faces : m.elems;
+  const int64_t ntri = m.dim == 3 ? m.nface : m.nel;
+  s.d_tri_verts.upload(tri, st);
+  s.d_tri_edges.upload(triangle_edges(m, tri.data(), ntri, {}), st);
+  s.ntri = ntri;
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
 void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double *c, uint64_t mask) {
   Space &s = *sp;
   const Mesh &m = *s.mesh;
   Ctx *ctx = m.ctx;
   hipStream_t st = ctx->stream;
   const Block &b = s.blocks[block];
+  check_fn(ctx, fn, c);
   auto key = std::make_pair(block, mask);
   Space::Lift *L;
   auto it = s.lifts.find(key);
@@ -1485,19 +1809,6 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
     std::vector<uint8_t> vflag(m.nv, 0), eflag(m.nedge, 0);
     std::vector<int32_t> fl;
     const int d = m.dim;
-    // edge lookup by vertex pair (3D)
-    auto find_edge = [&](int32_t a, int32_t bb) -> int64_t {
-      if (a > bb) std::swap(a, bb);
-      int64_t lo = 0, hi = m.nedge;
-      while (lo < hi) {
-        int64_t mid = (lo + hi) / 2;
-        if (m.edges[2 * mid] < a || (m.edges[2 * mid] == a && m.edges[2 * mid + 1] < bb))
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      return lo;
-    };
     for (int64_t f = 0; f < m.nfacet; ++f) {
       int mk = m.f_marker[f];
       if (m.f_e1[f] >= 0 || mk < 0 || !((mask >> mk) & 1)) continue;
@@ -1506,9 +1817,9 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
       if (d == 2)
         eflag[f] = 1;
       else {
-        eflag[find_edge(fv[0], fv[1])] = 1;
-        eflag[find_edge(fv[0], fv[2])] = 1;
-        eflag[find_edge(fv[1], fv[2])] = 1;
+        eflag[find_edge(m, fv[0], fv[1])] = 1;
+        eflag[find_edge(m, fv[0], fv[2])] = 1;
+        eflag[find_edge(m, fv[1], fv[2])] = 1;
         fl.push_back((int32_t)f);
       }
     }
@@ -1520,6 +1831,7 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
     L->verts.upload(vl, st);
     L->edges.upload(el, st);
     L->faces.upload(fl, st);
+    if (d == 3 && !fl.empty()) L->fedges.upload(triangle_edges(m, m.faces.data(), m.nface, fl), st);
     L->nverts = (int64_t)vl.size();
     L->nedges = (int64_t)el.size();
     L->nfaces = (int64_t)fl.size();
@@ -1528,6 +1840,7 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
   } else {
     L = it->second;
   }
+  if (b.order >= 3 && m.dim == 3 && L->nfaces) ensure_tri_tables(s);
   // function parameters live in the context's device scalar area (slots 200..209)
   double *dc = ctx->scal.p + 200;
   OCMP_HIP(hipMemcpyAsync(dc, c, 10 * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1538,9 +1851,96 @@ void launch_dirichlet_set(Space *sp, int block, double *x, int fn, const double 
     k_dir_edges<<<nblk(L->nedges), 256, 0, st>>>(L->edges.p, L->nedges, s.d_edges.p, m.d_coords.p, m.dim, fn, dc,
                                                  b.order, b.dof_off, m.nv, x);
   if (b.order >= 3 && m.dim == 3 && L->nfaces)
-    k_dir_faces<<<nblk(L->nfaces), 256, 0, st>>>(L->faces.p, L->nfaces, b.dof_off + m.nv + 2 * m.nedge, x);
+    k_set_faces<<<nblk(L->nfaces), 256, 0, st>>>(L->faces.p, L->nfaces, s.d_tri_verts.p, L->fedges.p, m.d_coords.p,
+                                                 m.dim, fn, dc, b.dof_off, m.nv, b.dof_off + m.nv + 2 * m.nedge, x);
   OCMP_HIP(hipGetLastError());
   // the parameter copy reads host memory that the caller may free on return
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+void launch_set(Space *sp, int block, double *x, int fn, const double *c) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const Block &b = s.blocks[block];
+  const int d = m.dim;
+  check_fn(ctx, fn, c);
+  OCMP_REQUIRE(fn != OCMP_FN_FIELD, "Set takes a closed-form function or an expression");
+  KTimer tm(ctx, ctx->cls("set"));
+  if (b.kind == OCMP_H1) {
+    OCMP_REQUIRE(b.order >= 1 && b.order <= 3, "Set on H1 blocks of order 1-3");
+    double *dc = ctx->scal.p + 200;
+    OCMP_HIP(hipMemcpyAsync(dc, c, 10 * sizeof(double), hipMemcpyHostToDevice, st));
+    k_dir_verts<<<nblk(m.nv), 256, 0, st>>>(nullptr, m.nv, m.d_coords.p, d, fn, dc, b.dof_off, x);
+    if (b.order >= 2)
+      k_dir_edges<<<nblk(m.nedge), 256, 0, st>>>(nullptr, m.nedge, s.d_edges.p, m.d_coords.p, d, fn, dc, b.order,
+                                                 b.dof_off, m.nv, x);
+    if (b.order >= 3) {
+      ensure_tri_tables(s);
+      k_set_faces<<<nblk(s.ntri), 256, 0, st>>>(nullptr, s.ntri, s.d_tri_verts.p, s.d_tri_edges.p, m.d_coords.p, d,
+                                                fn, dc, b.dof_off, m.nv, b.dof_off + m.nv + 2 * m.nedge, x);
+    }
+    OCMP_HIP(hipGetLastError());
+    OCMP_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  OCMP_REQUIRE(b.kind == OCMP_L2 && d == 2 && b.nloc <= MAXNB, "Set on L2 blocks: triangles up to P3");
+  KL2Set K;
+  std::memset(&K, 0, sizeof K);
+  TableBuilder tb;
+  const int qdeg = fn == OCMP_FN_CONST ? 2 * b.order : (fn == OCMP_FN_POLY2 ? b.order + 2 : 2 * b.order + 6);
+  Rule rule = simplex_rule(d, qdeg);
+  K.nq = rule.nq;
+  K.nl = b.nloc;
+  K.ld = s.nloc;
+  K.loc_off = b.loc_off;
+  std::vector<double> phi((size_t)rule.nq * b.nloc), gr((size_t)rule.nq * b.nloc * d);
+  for (int q = 0; q < rule.nq; ++q)
+    eval_basis(b.kind, d, b.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b.nloc], &gr[(size_t)q * b.nloc * d]);
+  // reference mass matrix on an exact rule, inverted by Gauss-Jordan (nl <= 10)
+  Rule mr = simplex_rule(d, 2 * b.order);
+  const int nl = b.nloc;
+  std::vector<double> M((size_t)nl * nl, 0.0), Mi((size_t)nl * nl, 0.0), pm((size_t)nl), gm((size_t)nl * d);
+  for (int q = 0; q < mr.nq; ++q) {
+    eval_basis(b.kind, d, b.order, &mr.lam[q * (d + 1)], pm.data(), gm.data());
+    for (int i = 0; i < nl; ++i)
+      for (int j = 0; j < nl; ++j) M[i * nl + j] += mr.w[q] * pm[i] * pm[j];
+  }
+  for (int i = 0; i < nl; ++i) Mi[i * nl + i] = 1.0;
+  for (int col = 0; col < nl; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < nl; ++r)
+      if (std::fabs(M[r * nl + col]) > std::fabs(M[piv * nl + col])) piv = r;
+    for (int j = 0; j < nl; ++j) {
+      std::swap(M[col * nl + j], M[piv * nl + j]);
+      std::swap(Mi[col * nl + j], Mi[piv * nl + j]);
+    }
+    const double dv = M[col * nl + col];
+    OCMP_REQUIRE(std::fabs(dv) > 0.0, "singular reference mass matrix");
+    for (int j = 0; j < nl; ++j) {
+      M[col * nl + j] /= dv;
+      Mi[col * nl + j] /= dv;
+    }
+    for (int r = 0; r < nl; ++r) {
+      if (r == col) continue;
+      const double fct = M[r * nl + col];
+      for (int j = 0; j < nl; ++j) {
+        M[r * nl + j] -= fct * M[col * nl + j];
+        Mi[r * nl + j] -= fct * Mi[col * nl + j];
+      }
+    }
+  }
+  K.tab_phi = tb.add(phi);
+  K.tab_w = tb.add(rule.w);
+  K.tab_lam = tb.add(rule.lam);
+  K.tab_minv = tb.add(Mi);
+  K.fn = fn;
+  for (int i = 0; i < 10; ++i) K.c[i] = c[i];
+  DBuf<double> *tab = cached_table(s, "S" + std::to_string(block) + "," + std::to_string(qdeg), tb, st);
+  K.tab = tab->p;
+  k_l2_set<2><<<nblk(m.nel), 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x);
+  OCMP_HIP(hipGetLastError());
   OCMP_HIP(hipStreamSynchronize(st));
 }
 
@@ -1551,6 +1951,7 @@ void error_norms(Space *sp, int block, const double *x, const double *ref, int f
   hipStream_t st = ctx->stream;
   const Block &b = s.blocks[block];
   const int d = m.dim;
+  if (!ref) check_fn(ctx, fn, c);
   KNorm K;
   std::memset(&K, 0, sizeof K);
   TableBuilder tb;
@@ -1582,6 +1983,59 @@ void error_norms(Space *sp, int block, const double *x, const double *ref, int f
   k_sum_pairs<<<1, 256, 0, st>>>(part, nb, res);
   OCMP_HIP(hipGetLastError());
   OCMP_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+void error_norms_vec(Space *sp, int nb, const int *blocks, const double *x, const double *ref, const int *fns,
+                     const double *cs, double *out) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const int d = m.dim;
+  OCMP_REQUIRE(nb >= 1 && nb <= 3, "a vector component has 1 to 3 scalar blocks");
+  const Block &b = s.blocks[blocks[0]];
+  KNormV K;
+  std::memset(&K, 0, sizeof K);
+  for (int k = 0; k < nb; ++k) {
+    OCMP_REQUIRE(blocks[k] >= 0 && blocks[k] < (int)s.blocks.size(), "block out of range");
+    const Block &bk = s.blocks[blocks[k]];
+    OCMP_REQUIRE(bk.kind == b.kind && bk.order == b.order, "the blocks of a vector norm must share kind and order");
+    K.loc_off[k] = bk.loc_off;
+    if (!ref) {
+      check_fn(ctx, fns[k], cs + 10 * k);
+      OCMP_REQUIRE(fns[k] != OCMP_FN_FIELD, "a field reference is passed as ref, not as a function");
+    }
+    K.fn[k] = ref ? OCMP_FN_CONST : fns[k];
+    for (int i = 0; i < 10; ++i) K.c[k][i] = ref ? 0.0 : cs[10 * k + i];
+  }
+  TableBuilder tb;
+  Rule rule = simplex_rule(d, 2 * b.order + 6);
+  K.nq = rule.nq;
+  K.nl = b.nloc;
+  K.ld = s.nloc;
+  K.ncomp = nb;
+  std::vector<double> phi((size_t)rule.nq * b.nloc), gr((size_t)rule.nq * b.nloc * d);
+  for (int q = 0; q < rule.nq; ++q)
+    eval_basis(b.kind, d, b.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b.nloc], &gr[(size_t)q * b.nloc * d]);
+  K.tab_phi = tb.add(phi);
+  K.tab_grad = tb.add(gr);
+  K.tab_w = tb.add(rule.w);
+  K.tab_lam = tb.add(rule.lam);
+  DBuf<double> *tab = cached_table(s, "N" + std::to_string(blocks[0]), tb, st);
+  K.tab = tab->p;
+  const unsigned nblkk = nblk(m.nel);
+  double *part = ctx->red.p;
+  OCMP_REQUIRE(ctx->red.n >= 3 * (size_t)nblkk + 3, "reduction scratch too small for this mesh");
+  KTimer tm(ctx, ctx->cls("error_norms"));
+  if (d == 2)
+    k_error_norms_vec<2><<<nblkk, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, ref, part);
+  else
+    k_error_norms_vec<3><<<nblkk, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, x, ref, part);
+  double *res = ctx->scal.p + 110;
+  k_sum_triples<<<1, 256, 0, st>>>(part, nblkk, res);
+  OCMP_HIP(hipGetLastError());
+  OCMP_HIP(hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
   OCMP_HIP(hipStreamSynchronize(st));
 }
 
@@ -1645,6 +2099,79 @@ void linf_norm(Space *sp, int block, const double *x, const double *ref, double 
   OCMP_HIP(hipGetLastError());
   OCMP_HIP(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, st));
   OCMP_HIP(hipStreamSynchronize(st));
+}
+
+
+// ------------------------------------------------------------------ expression programs (host)
+
+namespace {
+std::mutex g_expr_mu;
+struct ExprTable {
+  int next = 0;
+  std::set<int> starts;
+};
+std::map<int, ExprTable> g_expr_tables;  // device -> its program table's bookkeeping
+}  // namespace
+
+// stack effect of an opcode: (pops, pushes, inline constants); pops < 0: invalid opcode
+static void expr_effect(int op, int &pops, int &pushes, int &consts) {
+  pops = -1;
+  pushes = 1;
+  consts = 0;
+  if (op == OCMP_OP_CONST) { pops = 0; consts = 1; }
+  else if (op >= OCMP_OP_X && op <= OCMP_OP_T) pops = 0;
+  else if (op == OCMP_OP_NEG || (op >= OCMP_OP_SIN && op <= OCMP_OP_ROUND) || op == OCMP_OP_HS || op == OCMP_OP_RAMP)
+    pops = 1;
+  else if ((op >= OCMP_OP_ADD && op <= OCMP_OP_POW) || op == OCMP_OP_HS2) pops = 2;
+  else if (op == OCMP_OP_RAMP4) pops = 4;
+}
+
+int expr_register(Ctx *ctx, int n, const double *code) {
+  OCMP_REQUIRE(n >= 2 && n <= EXPR_CAP, "expression program length out of range");
+  // validate: every opcode known and integral, the stack never under- or overflows, exactly one
+  // value left at the final OCMP_OP_END, and no END before it (so every evaluation terminates
+  // inside the program)
+  int depth = 0;
+  for (int pc = 0; pc < n;) {
+    const double od = code[pc];
+    OCMP_REQUIRE(od == std::floor(od) && od >= 0 && od < 64, "expression program: bad opcode");
+    const int op = (int)od;
+    if (op == OCMP_OP_END) {
+      OCMP_REQUIRE(pc == n - 1, "expression program: END before the end");
+      OCMP_REQUIRE(depth == 1, "expression program must leave exactly one value");
+      break;
+    }
+    OCMP_REQUIRE(pc < n - 1, "expression program must end with OCMP_OP_END");
+    int pops, pushes, consts;
+    expr_effect(op, pops, pushes, consts);
+    OCMP_REQUIRE(pops >= 0, "expression program: unknown opcode " + std::to_string(op));
+    OCMP_REQUIRE(depth >= pops, "expression program: stack underflow");
+    depth += pushes - pops;
+    OCMP_REQUIRE(depth <= EXPR_DEPTH, "expression program needs more than 8 stack slots");
+    pc += 1 + consts;
+    OCMP_REQUIRE(pc < n, "expression program: constant past the end");
+  }
+  std::lock_guard<std::mutex> lock(g_expr_mu);
+  ExprTable &T = g_expr_tables[ctx->device];
+  OCMP_REQUIRE(T.next + n <= EXPR_CAP, "expression program table full (64K doubles per device)");
+  const int start = T.next;
+  OCMP_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(expr_code), code, (size_t)n * sizeof(double),
+                                  (size_t)start * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  OCMP_HIP(hipStreamSynchronize(ctx->stream));
+  T.starts.insert(start);
+  T.next += n;
+  return start;
+}
+
+void check_fn(const Ctx *ctx, int fn, const double *c) {
+  OCMP_REQUIRE(fn >= OCMP_FN_CONST && fn <= OCMP_FN_EXPR, "unknown function id");
+  if (fn != OCMP_FN_EXPR) return;
+  std::lock_guard<std::mutex> lock(g_expr_mu);
+  auto it = g_expr_tables.find(ctx->device);
+  const double id = c[0];
+  OCMP_REQUIRE(it != g_expr_tables.end() && id == std::floor(id) && id >= 0 && id < EXPR_CAP &&
+                   it->second.starts.count((int)id),
+               "OCMP_FN_EXPR: c[0] is not a program registered on this device");
 }
 
 }  // namespace ocmp

@@ -157,10 +157,15 @@ struct Space {
     DBuf<int32_t> verts;   // vertex ids
     DBuf<int32_t> edges;   // edge ids
     DBuf<int32_t> faces;   // face ids (3D)
+    DBuf<int32_t> fedges;  // per listed face: edge ids of (a,b), (a,c), (b,c), a < b < c
     int64_t nverts = 0, nedges = 0, nfaces = 0;
   };
   std::map<std::pair<int, uint64_t>, Lift *> lifts;
   DBuf<int32_t> d_edges;   // mesh edges (for lifts)
+  // whole-domain set of order-3 H1 blocks: the triangles carrying a face-bubble dof (3D faces,
+  // 2D cells), their vertex triples and edge ids (built on first use)
+  DBuf<int32_t> d_tri_verts, d_tri_edges;
+  int64_t ntri = 0;
   // deterministic element schedule for H1 scatter: elements sharing a vertex get different colours
   int ncolours = 0;
   std::vector<int64_t> colour_off;   // ncolours+1
@@ -201,6 +206,9 @@ struct RowBlocks {
   int64_t nrblk = 0, nucol = 0;
 };
 
+struct Work;                  // a matrix' Krylov scratch (linalg.hip), created on first solve
+void work_release(Work *w);   // frees it (device buffers, pinned host buffer, events)
+
 struct Matrix {
   Space *space;
   Ctx *ctx;
@@ -221,6 +229,12 @@ struct Matrix {
   bool assembled = false;
   // mixed precision copy
   DBuf<float> d_val32;
+  // solver scratch owned by this matrix (one per matrix, so distinct matrices never share it)
+  Work *work = nullptr;
+  Matrix() = default;
+  Matrix(const Matrix &) = delete;
+  Matrix &operator=(const Matrix &) = delete;
+  ~Matrix() { work_release(work); }
 };
 
 struct Precond {
@@ -273,6 +287,13 @@ void launch_assemble_dg(Matrix *A, const std::vector<ocmp_term> &terms, bool acc
 void launch_assemble_linear(Space *s, double *b, const std::vector<ocmp_lterm> &terms,
                             const std::vector<ocmp_field> &fields, bool accumulate);
 void launch_dirichlet_set(Space *s, int block, double *x, int fn, const double *c, uint64_t mask);
+// GridFunction.Set(g) on one scalar block over the whole domain: projection-based interpolation
+// (vertex values, edge and face projections) on H1, element-wise L2 projection on L2
+void launch_set(Space *s, int block, double *x, int fn, const double *c);
+// config expressions (OCMP_FN_EXPR): register a validated postfix program on ctx's device (returns
+// its id), and check a closed-form function id and its parameters before a launch
+int expr_register(Ctx *ctx, int n, const double *code);
+void check_fn(const Ctx *ctx, int fn, const double *c);
 void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
@@ -282,6 +303,8 @@ void newton_shifts(int n, const double *wr, const double *wi, int s, double *d, 
 double residual_vector(Matrix *A, const double *b, const double *x, double *r);
 void error_norms(Space *s, int block, const double *x, const double *ref, int fn, const double *c, double *out);
 void divergence_norm(Space *s, int block0, const double *x, double *out);
+void error_norms_vec(Space *s, int nb, const int *blocks, const double *x, const double *ref, const int *fns,
+                     const double *cs, double *out);
 void linf_norm(Space *s, int block, const double *x, const double *ref, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 

@@ -12,6 +12,9 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "internal.h"
 
@@ -135,7 +138,7 @@ __global__ void k_reduce(const double *__restrict__ part, int nparts, double *__
 // SpMV) drop several-fold, and the 16-bit local columns save 2 bytes per nonzero.
 //
 // mode 0: y = A x
-// mode 1: y = mask*(b - A x), partial sum y^2
+// mode 1: y = mask*(b - A x), partial sum y^2 (y == nullptr: the partial sum only)
 // mode 2: y = mask*(A x), partial sum x.y                (CG: q = A p, p.q)
 // mode 3: y = mask*(A (cs*sx*x))   cs: column scaling (right Jacobi), sx: device scalar  (GMRES on H1)
 // mode 4: y = mask*(A (cs*x)) - dsh*x + csh*xp                  (a Newton-basis step of s-step GMRES)
@@ -149,7 +152,7 @@ __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT 
     y[r] = (XT)s;
   } else if (mode == 1) {
     const double v = (mask && !mask[r]) ? 0.0 : b[r] - s;
-    y[r] = (XT)v;
+    if (y) y[r] = (XT)v;
     acc += v * v;
   } else if (mode == 2) {
     const XT v = (mask && !mask[r]) ? (XT)0 : (XT)s;
@@ -1077,16 +1080,21 @@ struct Work {
   size_t hcap = 0;
   DBuf<int32_t> nbr;       // DG neighbour table for the ELL operators
   std::vector<hipEvent_t> events;
+  Work() = default;
+  Work(const Work &) = delete;
+  Work &operator=(const Work &) = delete;
+  ~Work() {
+    if (hbuf) (void)hipHostFree(hbuf);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
 };
 
-static std::map<Matrix *, Work *> g_work;
+void work_release(Work *w) { delete w; }
 
+// the matrix owns its scratch: created on first use, freed with the matrix
 static Work &work(Matrix *A) {
-  auto it = g_work.find(A);
-  if (it != g_work.end()) return *it->second;
-  auto *w = new Work();
-  g_work[A] = w;
-  return *w;
+  if (!A->work) A->work = new Work();
+  return *A->work;
 }
 
 static double *pinned(Work &w, size_t n) {
