@@ -296,6 +296,17 @@ int ocmp_linf_norm(ocmp_space *s, int block, const double *x, const double *ref,
  * learned-guess fallback compares it between candidate initial guesses. */
 int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double *out);
 
+/* Allocation-free variant of ocmp_residual_norms. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc). out: DEVICE f64[2],
+ * out[0] = ||b - A x||_2, out[1] = ||b||_2 over the free dofs, written on the
+ * context stream; the call neither allocates nor synchronises (the caller
+ * synchronises before reading out). Partial sums live in a static 4 MiB device
+ * array per GPU (one copy per device, held until the process ends), so matrices
+ * with more than about 1.07e9 nonzeros are rejected with OCMP_ERR_ARG. Calls that
+ * overlap on one GPU share that array and race. */
+int ocmp_residual_norms_noalloc(ocmp_matrix *A, const double *b, const double *x, double *out);
+
 /* r = b - A x on the free dofs, 0 on the Dirichlet dofs (device f64), *norm = ||r||_2 (host, syncs). */
 int ocmp_residual(ocmp_matrix *A, const double *b, const double *x, double *r, double *norm);
 

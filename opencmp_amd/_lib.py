@@ -93,6 +93,7 @@ SIGNATURES = {
     'ocmp_axpby': [P, C.c_int64, C.c_double, P, C.c_double, P],
     'ocmp_error_norms': [P, C.c_int, P, P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)],
     'ocmp_residual_norms': [P, P, P, C.POINTER(C.c_double)],
+    'ocmp_residual_norms_noalloc': [P, P, P, P],
     'ocmp_error_norms_vec': [P, C.c_int, C.POINTER(C.c_int32), P, P, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                              C.POINTER(C.c_double)],
     'ocmp_set': [P, C.c_int, P, C.c_int, C.POINTER(C.c_double)],

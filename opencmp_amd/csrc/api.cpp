@@ -639,8 +639,11 @@ int ocmp_ctx_create(int device, void *hip_stream, ocmp_ctx **out) {
     OCMP_HIP(hipGetDeviceCount(&n));
     OCMP_REQUIRE(device >= 0 && device < n, "device index out of range");
     OCMP_HIP(hipSetDevice(device));
+    // this device's copy of the allocation-free entry points' scratch (looked up once per device)
+    double *noalloc = noalloc_scratch_init(device);
     auto *c = new ocmp_ctx();
     c->c.device = device;
+    c->c.noalloc_scratch = noalloc;
     // NULL selects the device's null stream, which is also PyTorch's default stream:
     // work enqueued by the library and by torch is then ordered without extra events.
     c->c.stream = (hipStream_t)hip_stream;
@@ -1031,6 +1034,15 @@ int ocmp_residual_norms(ocmp_matrix *A, const double *b, const double *x, double
     OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
     OCMP_HIP(hipSetDevice(A->A.ctx->device));
     residual_norms(&A->A, b, x, out);
+  });
+}
+
+int ocmp_residual_norms_noalloc(ocmp_matrix *A, const double *b, const double *x, double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(A && b && x && out, "NULL argument");
+    OCMP_REQUIRE(A->A.assembled, "matrix not assembled");
+    OCMP_HIP(hipSetDevice(A->A.ctx->device));
+    residual_norms_noalloc(&A->A, b, x, out);
   });
 }
 

@@ -82,6 +82,8 @@ struct Ctx {
   DBuf<double> red;        // partial sums
   DBuf<double> scal;       // device scalars
   double *host_scal = nullptr;  // pinned
+  // this device's copy of the allocation-free entry points' static scratch array (device address)
+  double *noalloc_scratch = nullptr;
   int cls(const std::string &name);
   void tic(int c, hipEvent_t *a);
   void toc(int c, hipEvent_t a, double bytes);
@@ -307,6 +309,10 @@ void error_norms_vec(Space *s, int nb, const int *blocks, const double *x, const
                      const double *cs, double *out);
 void linf_norm(Space *s, int block, const double *x, const double *ref, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
+// allocation-free entry points: the device address of the static scratch array on `device`
+// (looked up once per device, with that device current), and the residual norms computed in it
+double *noalloc_scratch_init(int device);
+void residual_norms_noalloc(Matrix *A, const double *b, const double *x, double *out_dev);
 
 // per-DOF MLP initial guess (mlp.hip)
 struct Mlp;

@@ -2773,4 +2773,62 @@ void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_o
   }
 }
 
+// ------------------------------------------------------------------ allocation-free entry points
+//
+// Callers that may not allocate device memory (code under stream capture, serving loops,
+// harnesses that forbid hipMalloc) get variants whose partial sums live in one static device
+// array. Each GPU the library runs on holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for that device (the array's
+// name is a host-side stand-in, not a device address). Calls on one GPU share the array, so
+// calls that overlap on one GPU race; every call writes the partial sums it reads.
+
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = (int64_t)1 << 19;  // 4 MiB per device
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_addr;  // device -> address of that device's copy
+
+double *noalloc_scratch_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) return it->second;
+  int cur = -1;
+  OCMP_HIP(hipGetDevice(&cur));
+  OCMP_REQUIRE(cur == device, "noalloc_scratch_init: the device must be current");
+  void *ptr = nullptr;
+  OCMP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  OCMP_REQUIRE(ptr != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+  g_scratch_addr[device] = static_cast<double *>(ptr);
+  return static_cast<double *>(ptr);
+}
+
+// out_dev[0] = ||b - A x||_2, out_dev[1] = ||b||_2 over the free dofs (FP64), written on the
+// context stream without allocating or synchronising. The same kernels as residual_norms,
+// except that the residual vector is not stored (mode 1 of the SpMV with y = nullptr).
+void residual_norms_noalloc(Matrix *A, const double *b, const double *x, double *out_dev) {
+  Ctx *ctx = A->ctx;
+  double *sp = ctx->noalloc_scratch;
+  OCMP_REQUIRE(sp != nullptr, "the context has no allocation-free scratch");
+  OCMP_REQUIRE(A->rb64.nnzb != 0, "the matrix has no SpMV row blocks");
+  hipStream_t st = ctx->stream;
+  const int grid = spmv_grid(A);
+  const int dgrid = red_grid(A->n);
+  // [0, grid): SpMV partials; [grid, grid + SEG_BLOCKS): second-stage sums of a long reduction;
+  // then dgrid partials of ||b||^2
+  const int64_t need = (int64_t)grid + SEG_BLOCKS + dgrid;
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    throw Error(OCMP_ERR_ARG, "residual_norms_noalloc: the matrix needs " + std::to_string(need) +
+                                  " doubles of scratch, more than the " + std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                                  " of the static array (at most about " +
+                                  std::to_string((NOALLOC_SCRATCH_DOUBLES - SEG_BLOCKS - MAX_RED_BLOCKS) * SPMV_NNZB64) +
+                                  " nonzeros)");
+  spmv_dispatch<double, double>(A, A->d_val.p, x, (double *)nullptr, b, A->space->d_free.p, (const double *)nullptr,
+                                nullptr, 1, sp, grid, st);
+  reduce_to(sp, grid, 1, out_dev, st, true);
+  double *bp = sp + grid + SEG_BLOCKS;
+  k_dot<double><<<dgrid, NTHR, 0, st>>>(A->n, b, b, A->space->d_free.p, bp);
+  k_reduce<<<1, NTHR, 0, st>>>(bp, dgrid, out_dev + 1, 1);
+  OCMP_HIP(hipGetLastError());
+}
+
 }  // namespace ocmp
