@@ -125,6 +125,9 @@ enum {
                               boundary condition, opencmp/models/ins.py:210-216 */
   OCMP_BFI_BND_MASS = 6,   /* coef * u v on boundary facets whose marker is in dirichlet_mask
                               (H1, trial == test): the Robin term, opencmp/models/poisson.py:94-96 */
+  OCMP_BFI_BND_FLUX = 7,   /* coef * (wind.n) * u v on boundary facets whose marker is in
+                              dirichlet_mask (H1, trial == test): the convective part of a Neumann
+                              boundary of a transported species, opencmp/models/multi_component_ins.py:277-285 */
   OCMP_BFI_DG_FACET = 16 /* interior + boundary facet terms of an L2 block, see below */
 };
 

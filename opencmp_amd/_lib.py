@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, 'libocmp_mi355x.so')
 OCMP_OK = 0
 OCMP_ERR_ARG, OCMP_ERR_HIP, OCMP_ERR_STATE, OCMP_ERR_UNSUPPORTED, OCMP_ERR_INTERNAL, OCMP_ERR_NUMERIC = -1, -2, -3, -4, -5, -6
 OCMP_H1, OCMP_L2 = 0, 1
-BFI_GRADGRAD, BFI_MASS, BFI_CONV, BFI_DTRIAL, BFI_DTEST, BFI_BND_UPWIND, BFI_BND_MASS, BFI_DG_FACET = 0, 1, 2, 3, 4, 5, 6, 16
+BFI_GRADGRAD, BFI_MASS, BFI_CONV, BFI_DTRIAL, BFI_DTEST, BFI_BND_UPWIND, BFI_BND_MASS, BFI_BND_FLUX = 0, 1, 2, 3, 4, 5, 6, 7
+BFI_DG_FACET = 16
 WEIGHT_NONE, WEIGHT_FIELD, WEIGHT_ONE_MINUS_FIELD = 0, 1, 2
 LFI_SOURCE, LFI_DG_DIRICHLET, LFI_BND_SOURCE = 0, 1, 2
 FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD, FN_EXPR = 0, 1, 2, 3, 4

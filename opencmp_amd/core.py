@@ -238,12 +238,14 @@ class Vol:
 @dataclass
 class BndUpwind:
     """coef * max(w.n, 0) * u v on the boundary facets whose marker is in `markers` (bit mask), one
-    H1 block (trial == test): the no-backflow part of the stress boundary condition."""
+    H1 block (trial == test): the no-backflow part of the stress boundary condition. signed=True
+    gives coef * (w.n) * u v instead: the convective flux of a Neumann boundary."""
     block: int
     markers: int
     coef: float = 1.0
     wind: Tuple[float, ...] = (0.0, 0.0, 0.0)
     wind_gf: Optional[Tuple['GridFunction', Sequence[int]]] = None   # (gf, blocks per dimension)
+    signed: bool = False
 
 
 @dataclass
@@ -355,7 +357,7 @@ class BilinearForm:
                 tt.coef = t.coef
                 tt.dirichlet_mask = t.markers
             elif isinstance(t, BndUpwind):
-                tt.kind = L.BFI_BND_UPWIND
+                tt.kind = L.BFI_BND_FLUX if t.signed else L.BFI_BND_UPWIND
                 tt.trial = tt.test = t.block
                 tt.coef = t.coef
                 tt.dirichlet_mask = t.markers

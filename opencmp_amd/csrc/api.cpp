@@ -890,9 +890,10 @@ int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, i
     for (auto &x : t) {
       OCMP_REQUIRE(x.trial >= 0 && x.trial < (int)s.blocks.size() && x.test >= 0 && x.test < (int)s.blocks.size(),
                    "term block index out of range");
-      OCMP_REQUIRE(x.kind == OCMP_BFI_DG_FACET || (x.kind >= 0 && x.kind <= OCMP_BFI_BND_MASS),
+      OCMP_REQUIRE(x.kind == OCMP_BFI_DG_FACET || (x.kind >= 0 && x.kind <= OCMP_BFI_BND_FLUX),
                    "unknown term kind");
-      OCMP_REQUIRE((x.kind != OCMP_BFI_BND_UPWIND && x.kind != OCMP_BFI_BND_MASS) || (!A->A.dg && x.trial == x.test),
+      OCMP_REQUIRE((x.kind != OCMP_BFI_BND_UPWIND && x.kind != OCMP_BFI_BND_MASS && x.kind != OCMP_BFI_BND_FLUX) ||
+                       (!A->A.dg && x.trial == x.test),
                    "the boundary terms are for one H1 block (trial == test)");
       OCMP_REQUIRE(x.weight_mode == OCMP_WEIGHT_NONE || (x.weight_field >= 0 && x.weight_field < nfields),
                    "weight field index out of range");

@@ -238,9 +238,14 @@ __device__ __forceinline__ double field_at_facet(const KField &f, const double *
 }
 
 // Row il (block bi) of coef * max(w.n, 0) u v (OCMP_BFI_BND_UPWIND: the no-backflow part of the
-// stress boundary condition) or of coef * u v (OCMP_BFI_BND_MASS: the Robin term) over the
+// stress boundary condition), coef * (w.n) u v (OCMP_BFI_BND_FLUX: the convective flux of a
+// Neumann boundary) or of coef * u v (OCMP_BFI_BND_MASS: the Robin term) over the
 // element's boundary facets whose marker is in T.dmask. Facet k is the edge (face) opposite vertex
 // k, n its outward unit normal; in 2-D the rule maps [0, 1] onto the edge.
+__host__ __device__ inline bool is_bnd_kind(int k) {
+  return k == OCMP_BFI_BND_UPWIND || k == OCMP_BFI_BND_MASS || k == OCMP_BFI_BND_FLUX;
+}
+
 template <int D>
 __device__ void bnd_facet_row(const KArgs &A, const KTerm &T, const Geo<D> &g, int64_t e, int bi, int il, int bj,
                                double ce, double *r) {
@@ -273,14 +278,14 @@ __device__ void bnd_facet_row(const KArgs &A, const KTerm &T, const Geo<D> &g, i
       const double *phij = A.tab + A.b.tab_fphi[bj] + (size_t)k * A.nqf * nlj;
       for (int q = 0; q < A.nqf; ++q) {
         double wn = 1.0;
-        if (T.kind == OCMP_BFI_BND_UPWIND) {
+        if (T.kind != OCMP_BFI_BND_MASS) {
           double wx = T.wind[0], wy = T.wind[1];
           if (T.wind_field >= 0) {
             wx = field_at_facet(A.f[T.wind_field], A.tab, e, k, q, A.nqf);
             wy = field_at_facet(A.f[T.wind_field + 1], A.tab, e, k, q, A.nqf);
           }
           wn = wx * nx + wy * ny;
-          if (!(wn > 0.0)) continue;
+          if (T.kind == OCMP_BFI_BND_UPWIND && !(wn > 0.0)) continue;
         }
         const double c = ce * fw[q] * len * wn * phit[q * nlt + il];
         for (int j = 0; j < nlj; ++j) r[j] += c * phij[q * nlj + j];
@@ -326,13 +331,13 @@ __device__ void bnd_facet_row(const KArgs &A, const KTerm &T, const Geo<D> &g, i
       const double *phij = A.tab + A.b.tab_fphi[bj] + (size_t)k * A.nqf * nlj;
       for (int q = 0; q < A.nqf; ++q) {
         double wn = 1.0;
-        if (T.kind == OCMP_BFI_BND_UPWIND) {
+        if (T.kind != OCMP_BFI_BND_MASS) {
           double w3[3] = {T.wind[0], T.wind[1], T.wind[2]};
           if (T.wind_field >= 0)
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) w3[cc] = field_at_facet(A.f[T.wind_field + cc], A.tab, e, k, q, A.nqf);
           wn = w3[0] * n3[0] + w3[1] * n3[1] + w3[2] * n3[2];
-          if (!(wn > 0.0)) continue;
+          if (T.kind == OCMP_BFI_BND_UPWIND && !(wn > 0.0)) continue;
         }
         const double cq = ce * fw[q] * area * wn * phit[q * nlt + il];
         for (int j = 0; j < nlj; ++j) r[j] += cq * phij[q * nlj + j];
@@ -361,7 +366,7 @@ __device__ void h1_row(const KArgs &A, const double *__restrict__ coords, const 
     const int nlj = A.b.nloc[bj];
     double *r = row + A.b.loc_off[bj];
     double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
-    if (T.kind == OCMP_BFI_BND_UPWIND || T.kind == OCMP_BFI_BND_MASS) continue;  // k_h1_bnd_assemble
+    if (is_bnd_kind(T.kind)) continue;  // k_h1_bnd_assemble
     if (!T.quad) {
       // reference-tensor contraction
       double fac[9];
@@ -496,7 +501,7 @@ __global__ void __launch_bounds__(256) k_h1_bnd_assemble(KArgs A, const double *
   bool any = false;
   for (int tt = 0; tt < A.nterms; ++tt) {
     const KTerm &T = A.t[tt];
-    if (T.test != bi || (T.kind != OCMP_BFI_BND_UPWIND && T.kind != OCMP_BFI_BND_MASS)) continue;
+    if (T.test != bi || !is_bnd_kind(T.kind)) continue;
     const double ce = T.coef * (T.elem_coef ? T.elem_coef[e] : 1.0);
     bnd_facet_row<D>(A, T, g, e, bi, il, T.trial, ce, row + A.b.loc_off[T.trial]);
     any = true;
@@ -1414,7 +1419,7 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
   A.tab_w = tb.add(rule.w);
   A.tab_lam = tb.add(rule.lam);
   bool bnd = false;
-  for (auto &t : terms) bnd |= t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS;
+  for (auto &t : terms) bnd |= is_bnd_kind(t.kind);
   // facet rule for boundary terms: exact for the product of three functions of the highest order
   std::vector<double> ft, fwt, flam;
   if (bnd) {
@@ -1475,7 +1480,7 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
       k.test = -1;  // never matches a row block in the volume pass
       continue;
     }
-    if (t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS) {
+    if (is_bnd_kind(t.kind)) {
       OCMP_REQUIRE(t.kind == OCMP_BFI_BND_MASS || t.wind_field < 0 || t.wind_field + 1 < (int)fields.size(),
                    "wind field out of range");
       continue;
@@ -1501,7 +1506,7 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
   if (!accumulate) k_fill<<<nblk(M->nnz), 256, 0, st>>>(M->d_val.p, M->nnz, 0.0);
   const Mesh &m = *s.mesh;
   bool has_bnd = false;
-  for (auto &t : terms) has_bnd |= t.kind == OCMP_BFI_BND_UPWIND || t.kind == OCMP_BFI_BND_MASS;
+  for (auto &t : terms) has_bnd |= is_bnd_kind(t.kind);
   OCMP_REQUIRE(!has_bnd || (int64_t)s.bnd_colour_off.size() == s.ncolours + 1, "boundary element lists missing");
   for (int c = 0; c < s.ncolours; ++c) {
     int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;

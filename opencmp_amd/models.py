@@ -21,6 +21,7 @@ import torch
 
 from . import _lib as L
 from . import core as C
+from . import expr as E
 from .mesh import Mesh as HostMesh
 
 Fn = Tuple[int, Sequence[float]]
@@ -28,6 +29,16 @@ Fn = Tuple[int, Sequence[float]]
 
 def const(v: float) -> Fn:
     return (L.FN_CONST, (float(v),))
+
+
+def _shifted(fn: Fn, shift: float) -> Fn:
+    """fn + shift for a closed form whose constant term is its first parameter."""
+    fid, prm = fn
+    if fid not in (L.FN_CONST, L.FN_POLY2):
+        raise NotImplementedError('a shifted reference needs a constant, a quadratic or an expression')
+    prm = list(prm) + [0.0] * (10 - len(prm))
+    prm[0] += shift
+    return (fid, tuple(prm))
 
 
 ZERO = const(0.0)
@@ -56,8 +67,12 @@ class Model:
                  dirichlet: Optional[Dict[str, Dict[str, Fn]]] = None, ctx: Optional[C.Context] = None,
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
-                 inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0):
+                 inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
+                 t_param: Optional[List[E.Parameter]] = None):
         self.ctx = ctx or C.Context.default()
+        # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
+        # expressions of t (boundary data, sources) are read at the level a form is built for
+        self.t_param = t_param if t_param is not None else [E.Parameter(0.0) for _ in range(3)]
         self.host_mesh = mesh
         self.mesh = C.Mesh(mesh, self.ctx)
         self.interp_ord = interp_ord
@@ -69,9 +84,12 @@ class Model:
             linear_solver = 'direct'
         if linear_solver == 'direct':
             # the GPU path has no sparse direct factorisation; the reference's default
-            # 'direct' maps to a preconditioned Krylov solve to a tolerance near machine precision
-            linear_solver = 'GMRes' if self._indefinite_or_nonsymmetric() else 'CG'
+            # 'direct' maps to a preconditioned Krylov solve to a tolerance near machine precision,
+            # without the iteration cap of the iterative solvers' config key
+            # (DG spaces take the element-block Jacobi preconditioner, which CG does not accept)
+            linear_solver = 'GMRes' if (self._indefinite_or_nonsymmetric() or DG) else 'CG'
             linear_tolerance = min(linear_tolerance, 1e-13)
+            linear_max_iterations = max(linear_max_iterations, 1000000)
         self.preconditioner = 'local' if preconditioner in ('default', 'local') else preconditioner
         self.schur_iters = schur_iters if self.preconditioner == 'schur' else 0
         self.schur_tol = schur_tol
@@ -92,6 +110,15 @@ class Model:
     def _indefinite_or_nonsymmetric(self) -> bool:
         return False
 
+    def _fn(self, v, time_step: int = 0):
+        """A boundary-data / source value as an Fn of the GPU terms: config expressions are bound to
+        the time of step n+1-time_step; closed forms pass through; lists map per component."""
+        if isinstance(v, list):
+            return [self._fn(x, time_step) for x in v]
+        if isinstance(v, E.Expr):
+            return v.bind(self.ctx, self.t_param[time_step])
+        return v
+
     def _construct_fes(self) -> C.FESpace:
         raise NotImplementedError
 
@@ -104,7 +131,7 @@ class Model:
     def construct_bilinear_time_coefficient(self, dt: float = 1.0) -> list:
         return []
 
-    def construct_linear(self, dt: float = 1.0) -> list:
+    def construct_linear(self, dt: float = 1.0, time_step: int = 0) -> list:
         return []
 
     def time_derivative_terms(self) -> list:
@@ -122,14 +149,17 @@ class Model:
         return None
 
     # ---------------------------------------------------------------- boundary conditions, solve
-    def apply_dirichlet_bcs_to(self, gfu: C.GridFunction) -> None:
-        """gfu.Set(g, definedon=Boundaries(...)) per component (opencmp/models/base_model.py:321-341)."""
+    def apply_dirichlet_bcs_to(self, gfu: C.GridFunction, time_step: int = 0) -> None:
+        """gfu.Set(g, definedon=Boundaries(...)) per component (opencmp/models/base_model.py:321-341),
+        with time-dependent data read at the time of step n+1-time_step."""
         for comp, bcs in self.dirichlet.items():
+            if comp not in self.components:
+                continue          # e.g. pressure data, which the reference does not impose either
             ci = self.components.index(comp)
             if not isinstance(self.fes.comps[ci], C.H1):
                 continue
             for marker, g in bcs.items():
-                gfu.Set(g, marker, component=ci)
+                gfu.Set(self._fn(g, time_step), marker, component=ci)
 
     def construct_preconditioners(self, a_assembled: C.BilinearForm) -> Optional[C.Preconditioner]:
         """ngs.Preconditioner(a, 'local') (opencmp/models/base_model.py:365-383)."""
@@ -167,6 +197,66 @@ class Model:
         L.check(self.ctx.lib.ocmp_error_norms(self.fes.h, b, ctypes.c_void_p(gfu.vec.data_ptr()), None, exact[0],
                                               c, out), 'error_norms')
         return out[0], out[1]
+
+    def _vec_norms(self, blocks, x: torch.Tensor, fns=None, ref: Optional[torch.Tensor] = None):
+        """(L2, L1, H1-seminorm) of the vector of the given scalar blocks against per-block Fns or a
+        reference vector on the same space (ocmp_error_norms_vec)."""
+        import ctypes
+        nb = len(blocks)
+        bl = (ctypes.c_int32 * nb)(*blocks)
+        fid = (ctypes.c_int32 * nb)()
+        c = (ctypes.c_double * (10 * nb))()
+        for k, f in enumerate(fns or []):
+            i, prm = E.resolve_fn(f)
+            fid[k] = i
+            for j, v in enumerate(prm):
+                c[10 * k + j] = v
+        out = (ctypes.c_double * 3)()
+        L.check(self.ctx.lib.ocmp_error_norms_vec(self.fes.h, nb, bl, ctypes.c_void_p(x.data_ptr()),
+                                                  None if ref is None else ctypes.c_void_p(ref.data_ptr()),
+                                                  fid, c, out), 'error_norms_vec')
+        return out[0], out[1], out[2]
+
+    def norm(self, norm_type: str, gfu: C.GridFunction, comp: str, ref, average: bool = False) -> float:
+        """norm() of opencmp/helpers/error.py:91-128 for one component against its reference (one
+        bound expression or closed form per scalar block). l1_norm integrates the pointwise Euclidean
+        norm (:70-72), l2_norm is the L2 norm (:75-77), linfinity_norm the largest coefficient of
+        the interpolant of the difference (:80-88). average: both sides first offset by their
+        ``mean`` (:35-68), which the reference computes as sqrt(int u^2), and Set into the space."""
+        ci = self.components.index(comp)
+        blocks = self.fes.comp_blocks[ci]
+        ref = ref if isinstance(ref, list) else [ref]
+        if len(ref) != len(blocks):
+            raise ValueError(f'{len(ref)} reference components for {comp!r}, which has {len(blocks)}')
+        ref_vec = None
+        if average or norm_type == 'linfinity_norm':
+            shift = 0.0
+            if average:
+                if len(blocks) != 1:
+                    raise NotImplementedError('error_average for vector components')
+                zero = torch.zeros_like(gfu.vec)
+                a_h = self._vec_norms(blocks, gfu.vec, [ZERO])[0]
+                a_g = self._vec_norms(blocks, zero, ref)[0]
+                shift = a_h - a_g
+            tmp = C.GridFunction(self.fes)
+            tmp.Set([r.scaled(shift=shift) if hasattr(r, 'scaled') else _shifted(r, shift) for r in ref],
+                    component=ci)
+            ref_vec = tmp.vec
+        if norm_type == 'linfinity_norm':
+            import ctypes
+            out = (ctypes.c_double * 1)()
+            m = 0.0
+            for blk in blocks:
+                L.check(self.ctx.lib.ocmp_linf_norm(self.fes.h, blk, ctypes.c_void_p(gfu.vec.data_ptr()),
+                                                    ctypes.c_void_p(ref_vec.data_ptr()), out), 'linf_norm')
+                m = max(m, out[0])
+            return m
+        l2, l1, _ = self._vec_norms(blocks, gfu.vec, None if ref_vec is not None else ref, ref=ref_vec)
+        if norm_type == 'l2_norm':
+            return l2
+        if norm_type == 'l1_norm':
+            return l1
+        raise ValueError(f'{norm_type} has not been implemented yet.')
 
     def divergence(self, gfu: C.GridFunction, comp: str) -> float:
         """sqrt(int (div u)^2) of a vector component (opencmp/helpers/error.py:151-167)."""
@@ -241,16 +331,18 @@ class Poisson(Model):
         # poisson.py:106-121 (interior SIPG) and :81-91 (boundary Nitsche), no convection
         return [C.DGFacet(0, eps=dt * self.dc, alpha=self.nu, dirichlet=dmask, upwind=False)]
 
-    def construct_linear(self, dt=1.0):
-        Lf = [C.Source(0, self.f, coef=dt)]                                   # poisson.py:133
+    def construct_linear(self, dt=1.0, time_step=0):
+        f = self._fn
+        Lf = [C.Source(0, f(self.f, time_step), coef=dt)]                     # poisson.py:133
         if self.DG:
             for marker, g in self.dirichlet.get('u', {}).items():             # poisson.py:136-146
                 Lf.append(C.DGDirichletData(0, eps=dt * self.dc, alpha=self.nu, wind=(0.0, 0.0),
-                                            dirichlet=self.mesh.marker_mask(marker), fn=g))
+                                            dirichlet=self.mesh.marker_mask(marker), fn=f(g, time_step)))
         for marker, h in self.neumann.get('u', {}).items():                   # poisson.py:151-154
-            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=h, coef=dt * self.dc))
+            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=f(h, time_step), coef=dt * self.dc))
         for marker, (r, q) in self.robin.get('u', {}).items():                # poisson.py:156-158
-            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=q, coef=-dt * self.dc * float(r)))
+            Lf.append(C.BndSource(0, self.mesh.marker_mask(marker), fn=f(q, time_step),
+                                  coef=-dt * self.dc * float(r)))
         return Lf
 
 
@@ -289,21 +381,25 @@ class DGTransport(Model):
         dmask = self.mesh.marker_mask(self.dirichlet_names('c')) if self.dirichlet.get('c') else 0
         return [C.DGFacet(0, eps=self.D, alpha=self.nu, wind=self.wind, dirichlet=dmask, coef=dt)]
 
-    def construct_linear(self, dt=1.0):
-        Lf = [C.Source(0, self.f, coef=dt)]
+    def construct_linear(self, dt=1.0, time_step=0):
+        Lf = [C.Source(0, self._fn(self.f, time_step), coef=dt)]
         for marker, g in self.dirichlet.get('c', {}).items():
             Lf.append(C.DGDirichletData(0, eps=self.D, alpha=self.nu, wind=self.wind,
-                                        dirichlet=self.mesh.marker_mask(marker), fn=g, coef=dt))
+                                        dirichlet=self.mesh.marker_mask(marker), fn=self._fn(g, time_step),
+                                        coef=dt))
         return Lf
 
 
 class Stokes(Model):
-    """opencmp/models/stokes.py:28-147 (CG Taylor-Hood): kv grad u : grad v - div(u) q - div(v) p - 1e-10 p q."""
+    """opencmp/models/stokes.py:28-147 (CG Taylor-Hood): kv grad u : grad v - div(u) q - div(v) p - 1e-10 p q.
+    ``stress``: {'u': {marker: traction}} adds dt h.v ds of each traction h (one Fn per velocity
+    component, or one for all; None: the do-nothing condition), stokes.py:131-137."""
     components = ['u', 'p']
 
-    def __init__(self, mesh, kv: float = 1.0, f=None, **kw):
+    def __init__(self, mesh, kv: float = 1.0, f=None, stress: Optional[Dict[str, Dict[str, Fn]]] = None, **kw):
         self.kv = kv
         self.f = f
+        self.stress = stress or {}
         kw.setdefault('linear_solver', 'GMRes')
         super().__init__(mesh, **kw)
 
@@ -341,11 +437,25 @@ class Stokes(Model):
     def construct_bilinear_time_coefficient(self, dt=1.0):
         return self._pressure_terms(dt)
 
-    def construct_linear(self, dt=1.0):
-        if self.f is None:
-            return []
+    def construct_linear(self, dt=1.0, time_step=0):
         ub = self.fes.comp_blocks[0]
-        return [C.Source(b, fk, coef=dt) for b, fk in zip(ub, self.f)]
+        Lf = []
+        if self.f is not None:
+            fs = self.f if isinstance(self.f, list) else [self.f] * len(ub)
+            if len(fs) != len(ub):
+                raise ValueError(f'source has {len(fs)} components, the velocity has {len(ub)}')
+            Lf += [C.Source(b, self._fn(fk, time_step), coef=dt) for b, fk in zip(ub, fs)]
+        for marker, h in self.stress.get('u', {}).items():                 # stokes.py:131-137, ins.py:299-305
+            if h is None:
+                continue
+            fns = h if isinstance(h, list) else [h] * len(ub)             # one Fn or one per component
+            if len(fns) != len(ub):
+                raise ValueError(f'traction on {marker!r} has {len(fns)} components, the velocity has {len(ub)}')
+            for b, fn in zip(ub, fns):
+                if isinstance(fn, E.Expr) or hasattr(fn, 'fn_args') or any(float(c) != 0.0 for c in fn[1]):
+                    Lf.append(C.BndSource(b, self.host_mesh.marker_mask(marker), fn=self._fn(fn, time_step),
+                                          coef=dt))
+        return Lf
 
 
 class INS(Stokes):
@@ -361,9 +471,12 @@ class INS(Stokes):
                  stress: Optional[Dict[str, Dict[str, Fn]]] = None, **kw):
         self.abs_tol, self.rel_tol = nonlinear_tolerance
         self.nonlinear_max_iters = nonlinear_max_iters
-        self.stress = stress or {}
-        super().__init__(mesh, kv=kv, **kw)
+        super().__init__(mesh, kv=kv, stress=stress, **kw)
         self.W = C.GridFunction(self.fes)    # linearisation point (opencmp/models/ins.py:109-116)
+
+    def set_linearisation_point(self, gfu: C.GridFunction) -> None:
+        """W = the initial condition's velocity (opencmp/models/ins.py:125-128)."""
+        self.W.vec.copy_(gfu.vec)
 
     def construct_bilinear_time_ODE(self, dt=1.0):
         ub = self.fes.comp_blocks[0]
@@ -377,18 +490,6 @@ class INS(Stokes):
             for b in ub:
                 terms.append(C.BndUpwind(b, mask, coef=dt, wind_gf=(self.W, ub)))
         return terms
-
-    def construct_linear(self, dt=1.0):
-        Lf = super().construct_linear(dt)
-        ub = self.fes.comp_blocks[0]
-        for marker, h in self.stress.get('u', {}).items():                 # ins.py:299-305
-            if h is None:
-                continue
-            fns = h if isinstance(h, list) else [h] * len(ub)             # one Fn or one per component
-            for b, fn in zip(ub, fns):
-                if any(float(c) != 0.0 for c in fn[1]):
-                    Lf.append(C.BndSource(b, self.host_mesh.marker_mask(marker), fn=fn, coef=dt))
-        return Lf
 
     def solve_single_step(self, a, Lf, pre, gfu, reassemble=None, assembled=False):
         """Picard iteration (opencmp/models/ins.py:323-359): re-assemble with the current wind,
@@ -451,11 +552,116 @@ class StokesDIM(Stokes):
             t.append(C.Vol('dtest', pb, b, coef=-dt, dir=k, weight_gf=(self.phi, 0, L.WEIGHT_ONE_MINUS_FIELD)))
         return t
 
-    def construct_linear(self, dt=1.0):
+    def construct_linear(self, dt=1.0, time_step=0):
         if self.f is None:
             return []
         ub = self.fes.comp_blocks[0]
-        return [C.Source(b, fk, coef=dt, weight_gf=(self.phi, 0, L.WEIGHT_FIELD)) for b, fk in zip(ub, self.f)]
+        return [C.Source(b, self._fn(fk, time_step), coef=dt, weight_gf=(self.phi, 0, L.WEIGHT_FIELD))
+                for b, fk in zip(ub, self.f)]
 
     def _time_weight(self):
         return (self.phi, 0, L.WEIGHT_FIELD)                 # opencmp/solvers/time_integration_schemes.py:678-681
+
+
+class MultiComponentINS(Model):
+    """opencmp/models/multi_component_ins.py with ``velocity_fixed = True``: species carried by a fixed
+    velocity w (the initial condition's; a constant vector here). Per species c with test function
+    r, every term times dt:
+
+        D grad c . grad r  -  c w . grad r                                 (:223-224, :258-260)
+        r c max(w.n, 0) on total-flux boundaries (neither Dirichlet nor Neumann)   (:262-275)
+        r c (w.n) on Neumann boundaries                                    (:277-285)
+        DG: skeleton SIPG (D != 0) + upwind [r](c max(w.n,0) + c' min(w.n,0))    (:171-197),
+            Dirichlet facets D (alpha r c - r grad c.n - c grad r.n) + r c max(w.n, 0)   (:287-301)
+        L: f r;  -h r on total-flux and Neumann boundaries (:357-367);
+           DG Dirichlet data D g (alpha r - grad r.n) - r g min(w.n, 0)  (:371-386)
+
+    H1 species take their Dirichlet data as a lift (strongly), L2 species through the facet terms.
+    """
+    nonlinear = False
+
+    def __init__(self, mesh, components: Sequence[str], kinds: Dict[str, str], D: Dict[str, float],
+                 f: Dict[str, Fn], wind=(0.0, 0.0), dirichlet=None, neumann=None, total_flux=None, **kw):
+        self.components = list(components)
+        self.kinds = dict(kinds)
+        for c in self.components:
+            if self.kinds[c] not in ('H1', 'L2'):
+                raise NotImplementedError(f'species {c!r} in a {self.kinds[c]} space')
+        self.D = dict(D)
+        self.f = dict(f)
+        self.wind = tuple(float(v) for v in wind) + (0.0,) * (3 - len(wind))
+        self.neumann = neumann or {}
+        self.total_flux = total_flux or {}
+        kw.setdefault('linear_solver', 'GMRes')
+        super().__init__(mesh, dirichlet=dirichlet, **kw)
+        for c in self.components:
+            if self.kinds[c] == 'L2' and self.neumann.get(c):
+                raise NotImplementedError('Neumann boundaries of a DG species (its facet term upwinds every '
+                                          'boundary facet)')
+
+    def _indefinite_or_nonsymmetric(self):
+        return True
+
+    def _construct_fes(self):
+        comps = []
+        for c in self.components:
+            if self.kinds[c] == 'H1':
+                comps.append(C.H1(self.interp_ord, self.dirichlet_names(c)))
+            else:
+                comps.append(C.L2(self.interp_ord))
+        return C.FESpace(self.mesh, comps, dgjumps=self.DG)
+
+    def _markers(self, c: str):
+        names = list(self.host_mesh.bnd_names)
+        d = set(self.dirichlet.get(c, {}).keys())
+        n = set(self.neumann.get(c, {}).keys())
+        return [m for m in names if m not in d and m not in n], sorted(n), sorted(d)
+
+    def construct_bilinear_time_ODE(self, dt=1.0):
+        w = self.wind
+        terms = []
+        for k, c in enumerate(self.components):
+            b = self.fes.comp_blocks[k][0]
+            if self.D[c] > 0:
+                terms.append(C.Vol('gradgrad', b, b, coef=dt * self.D[c]))
+            terms.append(C.Vol('conv', b, b, coef=-dt, wind=w))
+            if self.kinds[c] == 'H1':
+                tf, nm, _ = self._markers(c)
+                if tf:
+                    terms.append(C.BndUpwind(b, self.host_mesh.marker_mask(tf), coef=dt, wind=w))
+                if nm:
+                    if self.D[c] == 0:
+                        raise ValueError('Trying to apply a neuman boundary condition for purely advective flow '
+                                         '(diffusion coefficient is 0).')
+                    terms.append(C.BndUpwind(b, self.host_mesh.marker_mask(nm), coef=dt, wind=w, signed=True))
+        return terms
+
+    def construct_bilinear_time_coefficient(self, dt=1.0):
+        terms = []
+        for k, c in enumerate(self.components):
+            if self.kinds[c] != 'L2':
+                continue
+            b = self.fes.comp_blocks[k][0]
+            dm = self.host_mesh.marker_mask(list(self.dirichlet.get(c, {}).keys())) if self.dirichlet.get(c) else 0
+            terms.append(C.DGFacet(b, eps=self.D[c], alpha=self.nu, wind=self.wind[:2], dirichlet=dm,
+                                   sipg=self.D[c] != 0, upwind=True, coef=dt))
+        return terms
+
+    def construct_linear(self, dt=1.0, time_step=0):
+        Lf = []
+        for k, c in enumerate(self.components):
+            b = self.fes.comp_blocks[k][0]
+            Lf.append(C.Source(b, self._fn(self.f.get(c, ZERO), time_step), coef=dt))
+            for marker, h in list(self.total_flux.get(c, {}).items()) + list(self.neumann.get(c, {}).items()):
+                if not isinstance(h, E.Expr) and not hasattr(h, 'fn_args') and \
+                        all(float(v) == 0.0 for v in h[1]):
+                    continue
+                if self.kinds[c] == 'L2':
+                    raise NotImplementedError('non-zero flux data on a DG species')
+                Lf.append(C.BndSource(b, self.host_mesh.marker_mask(marker), fn=self._fn(h, time_step), coef=-dt))
+            if self.kinds[c] == 'L2':
+                for marker, g in self.dirichlet.get(c, {}).items():
+                    Lf.append(C.DGDirichletData(b, eps=self.D[c], alpha=self.nu, wind=self.wind[:2],
+                                                dirichlet=self.host_mesh.marker_mask(marker),
+                                                fn=self._fn(g, time_step), coef=dt))
+        return Lf

@@ -27,9 +27,12 @@ from .models import Model
 
 
 class StationarySolver:
-    def __init__(self, model: Model):
+    def __init__(self, model: Model, gfu0: Optional[C.GridFunction] = None):
+        """gfu0: the initial condition, the solve's starting point (base_solver.py:813)."""
         self.model = model
         self.gfu = C.GridFunction(model.fes)
+        if gfu0 is not None:
+            self.gfu.vec.copy_(gfu0.vec)
         self.a = None
         self.L = None
         self.pre = None
@@ -161,3 +164,141 @@ class ImplicitEulerSolver:
         for _ in range(nsteps):
             self.step()
         return self.gfu_prev
+
+
+SCHEMES = ('implicit euler', 'explicit euler', 'crank nicolson')
+
+
+class TransientMultiStepSolver:
+    """Fixed-step time integration (opencmp/solvers/transient_multistep.py:32-169) with the one-step
+    schemes of opencmp/solvers/time_integration_schemes.py, each a combination of device-resident
+    operators: the mass matrix M of the time-differentiated components, the time-coefficient form
+    A_c and the ODE form A_o (both built for the step's dt), and the linear forms L(t) at the new and
+    the previous time:
+
+        implicit euler (:79-132):  (M + A_c + A_o) u = L(t_n+1) + M u_n
+        explicit euler (:25-76):   (M + A_c) u = L(t_n) - A_o u_n + M u_n
+        crank nicolson (:135-190): (M + A_c + A_o/2) u = (L(t_n+1) + L(t_n))/2 - A_o u_n/2 + M u_n
+
+    The step size is clipped so the last step lands on the end time (_dt_for_next_time_to_hit);
+    a step of another size rebuilds the operators. Boundary data are applied at t_n+1. A linear model
+    keeps its assembled operators between steps; a nonlinear (Oseen) model re-assembles in its Picard
+    loop and supports implicit euler only, as a wind that changes within the step has no explicit part
+    here. Differences from the reference: a DG model's boundary facet terms belong to its facet term,
+    which the explicit schemes treat as part of A_c (implicitly).
+    """
+
+    def __init__(self, model: Model, scheme: str, dt: float, t_range=(0.0, 1.0), t_param=None,
+                 gfu0: Optional[C.GridFunction] = None):
+        if scheme not in SCHEMES:
+            raise NotImplementedError(f'time scheme {scheme!r} (the GPU path has {", ".join(SCHEMES)})')
+        if model.nonlinear and scheme != 'implicit euler':
+            raise NotImplementedError(f'{scheme} for a nonlinear (Oseen) model')
+        self.model = model
+        self.scheme = scheme
+        self.dt0 = float(dt)
+        self.t0, self.t1 = float(t_range[0]), float(t_range[1])
+        from .expr import Parameter
+        self.t_param = t_param if t_param is not None else model.t_param
+        self.t_param[0].Set(self.t0)
+        self.t_param[1].Set(self.t0 - self.dt0)
+        self.gfu = C.GridFunction(model.fes)
+        self.gfu_prev = C.GridFunction(model.fes)
+        if gfu0 is not None:
+            self.gfu_prev.vec.copy_(gfu0.vec)
+        self.gfu.vec.copy_(self.gfu_prev.vec)
+        self.tmp = model.ctx.zeros(model.fes.ndof)
+        self.steps = 0
+        self.linear_iterations = 0
+        self.step_results = []
+        self.M = C.BilinearForm(model.fes, model.time_derivative_terms())
+        self.M.Assemble()
+        self.dt = None
+        self._build(self.dt0)
+
+    def _build(self, dt: float):
+        m = self.model
+        fes = m.fes
+        mass = m.time_derivative_terms()
+        coef = m.construct_bilinear_time_coefficient(dt)
+        if self.scheme == 'implicit euler':
+            a_terms = mass + coef + m.construct_bilinear_time_ODE(dt)
+            self.L_new, self.L_old, self.Aode = C.LinearForm(fes, m.construct_linear(dt, 0)), None, None
+        elif self.scheme == 'explicit euler':
+            a_terms = mass + coef
+            self.L_new, self.L_old = None, C.LinearForm(fes, m.construct_linear(dt, 1))
+            self.Aode = C.BilinearForm(fes, m.construct_bilinear_time_ODE(dt))
+        else:
+            a_terms = mass + coef + m.construct_bilinear_time_ODE(0.5 * dt)
+            self.L_new = C.LinearForm(fes, m.construct_linear(0.5 * dt, 0))
+            self.L_old = C.LinearForm(fes, m.construct_linear(0.5 * dt, 1))
+            self.Aode = C.BilinearForm(fes, m.construct_bilinear_time_ODE(0.5 * dt))
+        self.a = C.BilinearForm(fes, a_terms)
+        self.pre = m.construct_preconditioners(self.a)
+        self.a.Assemble()
+        if self.pre is not None:
+            self.pre.Update()
+        if self.Aode is not None:
+            self.Aode.Assemble()
+        self.dt = dt
+
+    def _assemble_a(self):
+        self.a.Assemble()
+        if self.pre is not None:
+            self.pre.Update()
+
+    def _axpy(self, a: float, x, y):
+        L.check(self.model.ctx.lib.ocmp_axpby(self.model.ctx.h, self.model.fes.ndof, a,
+                                              ctypes.c_void_p(x.data_ptr()), 1.0, ctypes.c_void_p(y.data_ptr())),
+                'axpby')
+
+    def _rhs(self) -> torch.Tensor:
+        """The scheme's right-hand side in a fresh device vector."""
+        rhs = self.model.ctx.zeros(self.model.fes.ndof)
+        if self.L_new is not None:
+            self.L_new.Assemble()
+            self._axpy(1.0, self.L_new.vec, rhs)
+        if self.L_old is not None:
+            self.L_old.Assemble()
+            self._axpy(1.0, self.L_old.vec, rhs)
+        if self.Aode is not None:
+            self.Aode.mult(self.gfu_prev.vec, self.tmp)
+            self._axpy(-1.0, self.tmp, rhs)
+        self.M.mult(self.gfu_prev.vec, self.tmp)
+        self._axpy(1.0, self.tmp, rhs)
+        return rhs
+
+    def step(self, dt: float):
+        if self.dt is None or abs(dt - self.dt) > 1e-12 * self.dt0:
+            self._build(dt)
+        tn = self.t_param[0].value
+        self.t_param[1].Set(tn)
+        self.t_param[0].Set(tn + dt)
+        rhs = self._rhs()
+        self.gfu.vec.copy_(self.gfu_prev.vec)
+        self.model.apply_dirichlet_bcs_to(self.gfu)
+        Lwrap = _VecForm(rhs)
+        reassemble = self._assemble_a if self.model.nonlinear else None
+        res = self.model.solve_single_step(self.a, Lwrap, self.pre, self.gfu, reassemble=reassemble)
+        self.step_results.append(res)
+        self.linear_iterations += getattr(self.model, 'linear_iterations', None) or res.iters
+        self.gfu_prev.vec.copy_(self.gfu.vec)
+        self.steps += 1
+        return res
+
+    def solve(self) -> C.GridFunction:
+        import numpy as np
+        # opencmp/solvers/base_solver.py:815: step while t < t_end and t is not close to t_end
+        while self.t_param[0].value < self.t1 and not np.isclose(self.t_param[0].value, self.t1):
+            self.step(min(self.dt0, self.t1 - self.t_param[0].value))
+        return self.gfu
+
+
+class _VecForm:
+    """A linear form already assembled into a device vector (what solve_single_step reads)."""
+
+    def __init__(self, vec: torch.Tensor):
+        self.vec = vec
+
+    def Assemble(self):
+        return self

@@ -849,8 +849,11 @@ def h1_interpolate_vertices(space, g):
 def h1_lift(space, g, dirichlet_mask):
     """Boundary values of an H1 space on the facets with the given markers:
     vertex dofs take g at the vertex; edge dofs the 1D L2 projection of g minus its
-    linear interpolant onto the edge bubbles; face dofs (3D, order 3) zero.
-    Stands in for GridFunction.Set(g, definedon=Boundaries) (opencmp/models/base_model.py:321-341)."""
+    linear interpolant onto the edge bubbles; face dofs (3D, order 3) the L2 projection over
+    the face of g minus its vertex and edge parts onto the face bubble lambda_a lambda_b
+    lambda_c. Stands in for GridFunction.Set(g, definedon=Boundaries)
+    (opencmp/models/base_model.py:321-341); NGSolve's own projection is parity unpinned, the
+    tests pin exactness on the trace space (a cubic's lift reproduces it on every face)."""
     topo = space.topo
     d = topo.dim
     out = np.zeros(space.ndof)
@@ -871,6 +874,22 @@ def h1_lift(space, g, dirichlet_mask):
         out[topo.nv + (p - 1) * eids] = (res * b2) @ w / (b2 * b2 @ w)
         if p >= 3:
             out[topo.nv + (p - 1) * eids + 1] = (res * b3) @ w / (b3 * b3 @ w)
+    if p >= 3 and d == 3 and len(sel):
+        F = topo.faces[sel]                                   # ascending vertex ids a < b < c
+        nv = topo.nv
+        ekey = topo.edges[:, 0] * nv + topo.edges[:, 1]
+        E = [np.searchsorted(ekey, F[:, i] * nv + F[:, j]) for (i, j) in [(0, 1), (0, 2), (1, 2)]]
+        lam, w = simplex_rule(2, 12)                          # barycentrics (a, b, c) of each point
+        P = np.einsum('qk,fkd->fqd', lam, X[F])
+        la, lb, lc = lam[:, 0], lam[:, 1], lam[:, 2]
+        part = out[F] @ lam.T                                 # vertex part, (faces, points)
+        for k, (l1, l2) in enumerate([(la, lb), (la, lc), (lb, lc)]):
+            e2 = out[topo.nv + 2 * E[k]]
+            e3 = out[topo.nv + 2 * E[k] + 1]
+            part += np.outer(e2, l1 * l2) + np.outer(e3, l1 * l2 * (l2 - l1))
+        res = gv(P) - part
+        B = la * lb * lc
+        out[topo.nv + 2 * topo.nedge + sel] = (res * B) @ w / ((B * B) @ w)
     return out
 
 

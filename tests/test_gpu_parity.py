@@ -17,7 +17,7 @@ from opencmp_amd.io import load_mesh
 
 pytestmark = pytest.mark.gpu
 
-GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'mesh_files')
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'pytests', 'mesh_files')
 PI = np.pi
 TOL_FROB = 1e-13
 

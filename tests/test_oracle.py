@@ -10,7 +10,7 @@ from oracle import fem
 from opencmp_amd.mesh import unit_square, unit_cube, Mesh
 from opencmp_amd.io import load_mesh
 
-GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'mesh_files')
+GOLD = os.path.join(os.path.dirname(__file__), 'golden', 'pytests', 'mesh_files')
 PI = np.pi
 
 
@@ -164,10 +164,16 @@ def test_config1_counts_and_rates():
     assert 2.85 < l2 < 3.15 and 1.9 < h1 < 2.1
 
 
-@pytest.mark.parametrize('dim,p', [(2, 2), (2, 3), (3, 2)])
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 3), (3, 2), (3, 3)])
 def test_polynomial_reproduction_with_lift(dim, p):
-    """u in P_p with Dirichlet data u on the whole boundary: Galerkin returns u exactly."""
-    if dim == 2:
+    """u in P_p with Dirichlet data u on the whole boundary: Galerkin returns u exactly. The 3D
+    cubic's traces (e.g. y z - 2 y^2 z + 2 on x = 1) have face-bubble components, so a lift that
+    left the face dofs at zero would miss them."""
+    if dim == 3 and p == 3:
+        m = unit_cube(2)
+        u = lambda x, y, z: x * y * z + x ** 3 - 2 * y * y * z + 1
+        f = lambda x, y, z: -(6 * x - 4 * z)
+    elif dim == 2:
         m = unit_square(4)
         if p == 2:
             u = lambda x, y: 1 + x - 2 * y + x * x + 0.5 * x * y - y * y
