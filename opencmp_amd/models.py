@@ -500,6 +500,7 @@ class INS(Stokes):
         self.linear_iterations = 0
         self.precond_iterations = 0   # inner PCG iterations of a Schur inner solve
         self.picard_log = []      # (iterations, refinement steps, initial rel residual, Picard error) per iteration
+        self.picard_errors = []   # ||W - u||_L2 per iteration, full precision
         while True:
             self.apply_dirichlet_bcs_to(gfu)
             if reassemble is not None and not (assembled and it == 1):
@@ -508,6 +509,7 @@ class INS(Stokes):
             self.linear_iterations += res.iters
             self.precond_iterations += res.precond_iters
             err = self.l2_diff(self.W, gfu, 'u')
+            self.picard_errors.append(err)
             self.picard_log.append((res.iters, res.outer, float('%.3g' % res.initial_rel_residual), float('%.3g' % err)))
             unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
             it += 1

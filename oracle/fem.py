@@ -812,7 +812,15 @@ def assemble_bnd_facets(asm, comp, coef, marker_mask, wind=None, qdeg=None):
     if len(e0) == 0:
         return
     if wind is not None:
-        W = _wind_at(wind, asm.cs.topo, None, P)
+        if isinstance(wind, GridFunctionEval):
+            # a field on this component's space: its trace from the facet basis values
+            ws = wind.space
+            if ws.order != s.order or ws.kind != s.kind:
+                raise ValueError('a field wind must live on the component space')
+            W = np.stack([np.einsum('fql,fl->fq', phi, wind.vec[ws.edofs1[e0] + c * ws.n1])
+                          for c in range(ws.ncomp)], 2)
+        else:
+            W = _wind_at(wind, asm.cs.topo, None, P)
         ww = ww * np.maximum(np.einsum('fqd,fd->fq', W, n), 0.0)
     Ke = coef * np.einsum('fq,fqi,fqj->fij', ww, phi, phi)
     for c in range(s.ncomp):
@@ -1015,6 +1023,9 @@ def l2_error(space, vec, exact, comp=0, qdeg=None, grad_exact=None):
 # =============================================================== parity status
 
 PARITY = {
+    'oseen_picard': 'pinned: Poiseuille flow (an exact Navier-Stokes solution in the P2/P1 space) is '
+                    'reproduced to round-off, the Picard errors fall monotonically at low Reynolds number',
+    'h1_lift (3D P3 faces)': 'pinned: a cubic lift reproduces the cubic exactly (a zero face dof does not)',
     'simplex_rule': 'pinned: closed-form simplex monomial integrals',
     'h1_eval': 'pinned: partition of unity of the vertex part, polynomial reproduction (P2/P3 exact solves)',
     'dubiner_polys': 'pinned: orthonormality, reproduction of P2 polynomials',
@@ -1028,3 +1039,40 @@ PARITY = {
     'gmres_right/pcg_jacobi': 'pinned: true residual check against solve_free',
     'l2_error': 'pinned: manufactured solution rates (P2: L2 3, H1 2) and the reference transient Poisson value',
 }
+
+
+def oseen_picard(cs, kv, lift, b, abs_tol, rel_tol, max_iter, W0=None, qdeg=None, stress_mask=0):
+    """Stationary Oseen/Picard iteration of opencmp/models/ins.py:323-359 (linearized_solve :361-379)
+    for Taylor-Hood spaces cs = Compound([vector H1 u, H1 p]):
+
+        repeat: assemble kv grad u:grad v - (u (x) W):grad v - div(u) q - div(v) p - 1e-10 p q
+                (ins.py:191-195, :240-244) + max(W.n, 0) u.v on the stress boundaries in
+                stress_mask (ins.py:210-216), solve on the free dofs from the Dirichlet lift ``lift``
+                (scipy direct), err = ||W - u||_L2 (velocity), W <- u,
+        until err < abs_tol + rel_tol ||u||_L2 or max_iter iterations.
+
+    b: the load vector (sources, tractions). Returns (x, [err per iteration])."""
+    su = cs.spaces[0]
+    W = np.zeros(cs.ndof) if W0 is None else np.asarray(W0, dtype=float).copy()
+    errs = []
+    x = lift.copy()
+    for _ in range(max_iter):
+        A = Assembler(cs)
+        wind = GridFunctionEval(su, W[:su.ndof])
+        assemble_volume(A, [dict(kind='grad_grad', trial=0, test=0, coef=kv),
+                            dict(kind='conv', trial=0, test=0, coef=-1.0, wind=wind),
+                            dict(kind='dtrial', trial=0, test=1, coef=-1.0, dir=-1),
+                            dict(kind='dtest', trial=1, test=0, coef=-1.0, dir=-1),
+                            dict(kind='mass', trial=1, test=1, coef=-1e-10)], qdeg=qdeg)
+        if stress_mask:
+            assemble_bnd_facets(A, 0, 1.0, stress_mask, wind=wind)
+        K = A.matrix()
+        x = solve_free(K, b, lift, cs.freedofs)
+        du = x[:su.ndof] - W[:su.ndof]
+        err = np.sqrt(sum(l2_error(su, du, lambda *X: 0 * X[0], comp=c) ** 2 for c in range(su.ncomp)))
+        un = np.sqrt(sum(l2_error(su, x[:su.ndof], lambda *X: 0 * X[0], comp=c) ** 2 for c in range(su.ncomp)))
+        errs.append(err)
+        W = x.copy()
+        if err < abs_tol + rel_tol * un:
+            break
+    return x, errs

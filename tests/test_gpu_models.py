@@ -386,3 +386,40 @@ def test_ins_stress_traction_load(gpu_ctx):
     say('  x block . 1', vx, 'y block . y', vy)
     assert abs(vx - 0.5) <= 1e-13 and abs(vy - 1.0 / 3.0) <= 1e-13
     assert np.abs(b[off[bx + 2]:]).max() == 0.0     # pressure block untouched
+
+
+def test_oseen_picard_matches_oracle(gpu_ctx):
+    """The Picard loop of opencmp/models/ins.py:323-359 on the GPU against the oracle's restatement
+    (oracle/fem.py oseen_picard: the same Oseen form, scipy direct solves): unit square, P2/P1,
+    kv = 0.1, parabolic inflow, walls, a zero-traction stress outlet, body force (0, 5 sin(pi x)
+    sin(pi y)). The Picard errors ||W - u|| agree iteration by iteration, the loops stop after the
+    same number of iterations, and the solutions agree."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle import _th_space, th_lift
+    from opencmp_amd.models import INS, const
+    from opencmp_amd.solvers import StationarySolver
+    say('oseen picard vs oracle')
+    hm = unit_square(4)
+    t, cs, mask = _th_space(hm)
+    x0 = th_lift(t, cs, mask, lambda x, y: 4 * y * (1 - y) + 0 * x, lambda x, y: 0 * x)
+    b = fem.assemble_source(cs, 0, [lambda x, y: 0 * x, lambda x, y: 5 * np.sin(np.pi * x) * np.sin(np.pi * y)])
+    xo, errs_o = fem.oseen_picard(cs, 0.1, x0, b, 1e-10, 1e-10, 40, stress_mask=hm.marker_mask(['right']))
+    inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
+    wall = [const(0.0), const(0.0)]
+    m = INS(hm, kv=0.1, interp_ord=2, f=[const(0.0), (L.FN_SINPROD, (5.0,))],
+            dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}}, stress={'u': {'right': None}},
+            linear_solver='GMRes', preconditioner='None', linear_tolerance=1e-13, restart=200,
+            nonlinear_tolerance=(1e-10, 1e-10), nonlinear_max_iters=40, ctx=gpu_ctx)
+    s = StationarySolver(m)
+    s.solve()
+    eg = np.array(m.picard_errors)
+    say('  GPU', len(eg), 'iterations, oracle', len(errs_o))
+    say('  GPU errors   ', ' '.join('%.6e' % e for e in eg[:6]))
+    say('  oracle errors', ' '.join('%.6e' % e for e in errs_o[:6]))
+    assert len(eg) == len(errs_o)
+    big = np.array(errs_o) > 1e-7
+    np.testing.assert_allclose(eg[big], np.array(errs_o)[big], rtol=1e-6)
+    x = s.gfu.numpy()
+    nu = cs.spaces[0].ndof
+    assert np.abs(x[:nu] - xo[:nu]).max() <= 1e-8 * np.abs(xo[:nu]).max()

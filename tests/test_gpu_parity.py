@@ -647,3 +647,30 @@ def test_schur_pcg_fgmres_solves_where_the_diagonal_stalls(gpu_ctx, precision):
     r2 = C.solve(a2, f2.vec, x2.vec, pre2, 'GMRes', tol=1e-10, restart=50, precision=precision, maxit=2000)
     say('  gmres + schur diagonal', r2)
     assert r2.iters > 5 * r.iters
+
+
+@pytest.mark.parametrize('mesh_name', ['unit_square', 'channel'])
+def test_dg_upwind_energy_identity(gpu_ctx, mesh_name):
+    """The GPU's DG convection operator (bulk -u w.grad v, upwind facets; eps = 0) satisfies the
+    upwind energy identity u^T A u = 1/2 sum_F int |w.n| [u]^2 + 1/2 int_boundary |w.n| u^2 for
+    divergence-free w, with the right side computed from facet traces independently of any
+    assembly. A downwind or centred flux fails it."""
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    from test_oracle import dg_jump_energy
+    C = core()
+    say('DG upwind energy identity', mesh_name)
+    hm = unit_square(5) if mesh_name == 'unit_square' else load_mesh(os.path.join(GOLD, 'channel_3bcs.vol'))
+    w = np.array([np.cos(0.3), np.sin(0.3)])
+    gm = C.Mesh(hm, gpu_ctx)
+    gs = C.FESpace(gm, [C.L2(2)], dgjumps=True)
+    a = C.BilinearForm(gs)
+    a += C.Vol('conv', 0, 0, coef=-1.0, wind=(w[0], w[1], 0.0))
+    a += C.DGFacet(0, eps=0.0, alpha=40.0, wind=(w[0], w[1]), dirichlet=0, sipg=False, upwind=True)
+    a.Assemble()
+    K = a.to_scipy()
+    s = fem.Space(fem.Topology(hm), 'L2', 2)
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        u = rng.standard_normal(gs.ndof)
+        assert u @ (K @ u) == pytest.approx(dg_jump_energy(s, u, w), rel=1e-11)

@@ -403,3 +403,106 @@ def test_boundary_terms_closed_forms():
     assert abs(x @ A @ x - 1.0) <= 1e-13 and abs(abs(A - A.T).max()) <= 1e-15
     b = fem.assemble_bnd_source(cs, 0, lambda a, c: 2 * c, right)
     assert abs(b @ one - 1.0) <= 1e-13 and abs(b @ y - 2.0 / 3.0) <= 1e-13
+
+
+def dg_jump_energy(space, vec, w):
+    """1/2 sum_F int_F |w.n| [u]^2 + 1/2 int_{boundary} |w.n| u^2 for a DG function on triangles
+    with a constant wind, from the traces of u on each facet (Gauss rule on the edge)."""
+    topo = space.topo
+    t, wq = fem.line_rule(12)
+    X = topo.coords
+    tot = 0.0
+    for f in range(topo.nfacet):
+        a, b = topo.facets[f]
+        tang = X[b] - X[a]
+        length = np.hypot(*tang)
+        n = np.array([tang[1], -tang[0]]) / length
+        wn = abs(n @ w)
+        vals = []
+        for e, k in ((topo.f_e0[f], topo.f_k0[f]), (topo.f_e1[f], topo.f_k1[f])):
+            if e < 0:
+                continue
+            lam = fem._facet_lam(2, k, t)
+            phi, _ = fem.ref_basis('L2', 2, space.order, lam)
+            vals.append(phi @ vec[space.edofs1[e]])
+        jump2 = (vals[0] - vals[1]) ** 2 if len(vals) == 2 else vals[0] ** 2
+        tot += 0.5 * wn * length * (wq @ jump2)
+    return tot
+
+
+@pytest.mark.parametrize('mesh_name', ['unit_square', 'channel'])
+def test_upwind_energy_identity(mesh_name):
+    """For a divergence-free wind the DG convection form (bulk -u w.grad v, interior upwind flux,
+    upwind on every boundary facet; opencmp/models/multi_component_ins.py:193-197, :258-275) satisfies
+    u^T A u = 1/2 sum_F int |w.n| [u]^2 + 1/2 int_boundary |w.n| u^2 for every u. A downwind or a
+    centred flux gives a different (non-positive or zero) jump term, so this pins the upwind
+    direction, which consistency with a smooth solution does not."""
+    from opencmp_amd.io import load_mesh
+    m = unit_square(5) if mesh_name == 'unit_square' else load_mesh(os.path.join(GOLD, 'channel_3bcs.vol'))
+    w = np.array([np.cos(0.3), np.sin(0.3)])
+    t = fem.Topology(m)
+    s = fem.Space(t, 'L2', 2)
+    cs = fem.Compound([s], dgjumps=True)
+    A = fem.Assembler(cs)
+    fem.assemble_volume(A, [dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
+    fem.assemble_dg_facets(A, 0, 0.0, 40.0, w, 0, 0, sipg=False)
+    K = A.matrix()
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        u = rng.standard_normal(cs.ndof)
+        lhs = u @ (K @ u)
+        rhs = dg_jump_energy(s, u, w)
+        assert lhs == pytest.approx(rhs, rel=1e-11)
+    # the facet term is what makes the identity hold: without it the form is not positive
+    A2 = fem.Assembler(cs)
+    fem.assemble_volume(A2, [dict(kind='conv', trial=0, test=0, coef=-1.0, wind=w)])
+    K2 = A2.matrix()
+    assert abs(u @ (K2 @ u) - dg_jump_energy(s, u, w)) > 1e-3 * abs(dg_jump_energy(s, u, w))
+
+
+def _th_space(m, p=2):
+    t = fem.Topology(m)
+    mask = m.marker_mask(['left', 'top', 'bottom'])
+    su = fem.Space(t, 'H1', p, ncomp=2, dirichlet_mask=mask)
+    sp = fem.Space(t, 'H1', p - 1)
+    return t, fem.Compound([su, sp]), mask
+
+
+def th_lift(t, cs, mask, gx, gy, p=2):
+    """Dirichlet lift of a Taylor-Hood velocity, one scalar lift per component."""
+    s1 = fem.Space(t, 'H1', p, dirichlet_mask=mask)
+    n1 = s1.ndof
+    x0 = np.zeros(cs.ndof)
+    x0[:n1] = fem.h1_lift(s1, gx, mask)
+    x0[n1:2 * n1] = fem.h1_lift(s1, gy, mask)
+    return x0
+
+
+def test_oseen_picard_reproduces_poiseuille():
+    """Parabolic inflow, walls, a zero-traction stress outlet: Poiseuille flow u = (4y(1-y), 0),
+    p = 8 kv (1 - x) solves the stationary Navier-Stokes equations ((u.grad)u = 0) and lies in the
+    P2/P1 space, so the Picard loop returns it to round-off (the -1e-10 p q term perturbs p at
+    that level). The outlet's max(w.n, 0) u.v term is what makes the skew form -(u (x) w):grad v
+    consistent there: without it the boundary term of the integration by parts remains."""
+    m = unit_square(4)
+    t, cs, mask = _th_space(m)
+    kv = 0.1
+    x0 = th_lift(t, cs, mask, lambda x, y: 4 * y * (1 - y) + 0 * x, lambda x, y: 0 * x)
+    x, errs = fem.oseen_picard(cs, kv, x0, np.zeros(cs.ndof), 1e-12, 1e-12, 20,
+                               stress_mask=m.marker_mask(['right']))
+    su, sp = cs.spaces
+    assert fem.l2_error(su, x[:su.ndof], lambda x, y: 4 * y * (1 - y), comp=0) < 1e-10
+    assert fem.l2_error(su, x[:su.ndof], lambda x, y: 0 * x, comp=1) < 1e-10
+    assert fem.l2_error(sp, x[su.ndof:], lambda x, y: 8 * kv * (1 - x)) < 1e-7
+    assert len(errs) <= 3
+
+
+def test_oseen_picard_converges_with_a_body_force():
+    m = unit_square(4)
+    t, cs, mask = _th_space(m)
+    su = cs.spaces[0]
+    x0 = th_lift(t, cs, mask, lambda x, y: 4 * y * (1 - y) + 0 * x, lambda x, y: 0 * x)
+    b = fem.assemble_source(cs, 0, [lambda x, y: 0 * x, lambda x, y: 5 * np.sin(np.pi * x) * np.sin(np.pi * y)])
+    x, errs = fem.oseen_picard(cs, 0.1, x0, b, 1e-10, 1e-10, 40, stress_mask=m.marker_mask(['right']))
+    assert len(errs) >= 4 and errs[-1] < 1e-9 * (1 + np.abs(x).max())
+    assert all(e1 < e0 for e0, e1 in zip(errs[1:], errs[2:]))
