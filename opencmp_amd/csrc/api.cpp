@@ -598,6 +598,107 @@ void build_colouring(Space &s) {
   OCMP_HIP(hipStreamSynchronize(m.ctx->stream));
 }
 
+// Row-owner gather structures of an H1 matrix: for every row r the entries (element e, local row
+// i) with edofs[e][i] == r, in increasing e, and for each entry the position within row r of every
+// column edofs[e][j]. Built once per pattern (OpenMP over rows for the positions).
+void build_h1_gather(Matrix *A) {
+  Space &s = *A->space;
+  Mesh &m = *s.mesh;
+  const int nl = s.nloc;
+  OCMP_REQUIRE(nl <= 64, "the row gather takes at most 64 local dofs per element");
+  const int64_t n = A->n;
+  // row order: every row's entries (element e, local row i) with edofs[e][i] == r, increasing e
+  std::vector<int32_t> pptr(n + 1, 0);
+  for (int64_t t = 0; t < m.nel * nl; ++t) pptr[s.edofs[t] + 1]++;
+  for (int64_t r = 0; r < n; ++r) pptr[r + 1] += pptr[r];
+  const int64_t np = pptr[n];
+  OCMP_REQUIRE(np == m.nel * nl && np < INT32_MAX, "patch count mismatch");
+  std::vector<int32_t> pe(np);
+  std::vector<uint8_t> pli(np);
+  {
+    std::vector<int32_t> fill(pptr.begin(), pptr.end() - 1);
+    for (int64_t e = 0; e < m.nel; ++e)
+      for (int i = 0; i < nl; ++i) {
+        const int32_t r = s.edofs[e * nl + i];
+        pe[fill[r]] = (int32_t)e;
+        pli[fill[r]] = (uint8_t)i;
+        fill[r]++;
+      }
+  }
+  int maxrow = 0, maxent = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    maxrow = std::max(maxrow, A->row_ptr[r + 1] - A->row_ptr[r]);
+    maxent = std::max(maxent, pptr[r + 1] - pptr[r]);
+  }
+  OCMP_REQUIRE(maxrow <= 65536, "a row with more than 65536 nonzeros");
+  const bool narrow = maxrow <= 256;
+  const int sb = narrow ? 1 : 2;
+  // row blocks: staged values (entries x nl doubles + positions) of about 28 KB and at most 2048
+  // nonzeros per block, greedily over consecutive rows; a row that alone exceeds that gets a block
+  // of its own and the buffers grow to it, up to 48 KB of LDS in all
+  const int emax = std::max(std::max(1, 28672 / (nl * (8 + sb) + 2)), maxent);
+  const int accmax = std::max(2048, maxrow);
+  A->gblk.clear();
+  A->maxrow = maxrow;
+  A->gemax = emax;
+  A->gaccmax = accmax;
+  A->npatch = np;
+  if ((size_t)emax * (nl * (8 + sb) + 2) + (size_t)accmax * 8 > 49152 || emax > 65535) return;  // colour path
+  A->gblk.push_back(0);
+  for (int64_t r = 0; r < n;) {
+    int64_t r1 = r + 1;
+    while (r1 < n && pptr[r1 + 1] - pptr[r] <= emax && A->row_ptr[r1 + 1] - A->row_ptr[r] <= accmax) ++r1;
+    A->gblk.push_back((int32_t)r1);
+    r = r1;
+  }
+  const int64_t nblk = (int64_t)A->gblk.size() - 1;
+  // per block: entries sorted by local row (stable), their positions, and each entry's column
+  // positions within its row
+  std::vector<int32_t> spe(np);
+  std::vector<uint16_t> gpos(np), gloff((size_t)nblk * (nl + 1));
+  std::vector<uint8_t> s8(narrow ? (size_t)np * nl : 0);
+  std::vector<uint16_t> s16(narrow ? 0 : (size_t)np * nl);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int32_t r0 = A->gblk[b], r1 = A->gblk[b + 1];
+    const int32_t e0 = pptr[r0], ne = pptr[r1] - e0;
+    std::vector<int> cnt(nl + 1, 0);
+    for (int32_t k = 0; k < ne; ++k) cnt[pli[e0 + k] + 1]++;
+    for (int L = 0; L < nl; ++L) cnt[L + 1] += cnt[L];
+    for (int L = 0; L <= nl; ++L) gloff[(size_t)b * (nl + 1) + L] = (uint16_t)cnt[L];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int32_t r = r0; r < r1; ++r) {
+      const int32_t *c0 = A->col.data() + A->row_ptr[r];
+      const int32_t *c1 = A->col.data() + A->row_ptr[r + 1];
+      for (int32_t p = pptr[r]; p < pptr[r + 1]; ++p) {
+        const int q = fill[pli[p]]++;             // sorted position within the block
+        gpos[p] = (uint16_t)q;
+        const int64_t sp = e0 + q;
+        spe[sp] = pe[p];
+        const int32_t *ed = &s.edofs[(int64_t)pe[p] * nl];
+        for (int j = 0; j < nl; ++j) {
+          const int pos = (int)(std::lower_bound(c0, c1, ed[j]) - c0);
+          if (narrow)
+            s8[(size_t)sp * nl + j] = (uint8_t)pos;
+          else
+            s16[(size_t)sp * nl + j] = (uint16_t)pos;
+        }
+      }
+    }
+  }
+  hipStream_t st = A->ctx->stream;
+  A->d_pptr.upload(pptr, st);
+  A->d_pe.upload(spe, st);
+  A->d_gpos.upload(gpos, st);
+  A->d_gloff.upload(gloff, st);
+  A->d_gblk.upload(A->gblk, st);
+  if (narrow)
+    A->d_pslot8.upload(s8, st);
+  else
+    A->d_pslot16.upload(s16, st);
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
 void build_h1_schedule(Matrix *A) {
   Space &s = *A->space;
   Mesh &m = *s.mesh;
@@ -836,7 +937,7 @@ int ocmp_matrix_create(ocmp_space *s, ocmp_matrix **out) {
       A->A.ctx = s->s.mesh->ctx;
       OCMP_HIP(hipSetDevice(A->A.ctx->device));
       build_pattern(&A->A);
-      if (!A->A.dg) build_h1_schedule(&A->A);
+      if (!A->A.dg) build_h1_gather(&A->A);
       OCMP_HIP(hipStreamSynchronize(A->A.ctx->stream));
     } catch (...) {
       delete A;

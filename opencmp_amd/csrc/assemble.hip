@@ -12,7 +12,9 @@
 //    whole CSR row: the volume block plus, for every facet of e, its own side of
 //    the facet integral (diagonal block) and the coupling to the neighbour's
 //    dofs. Each value is written exactly once; no colouring, no atomics.
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -515,6 +517,229 @@ __global__ void __launch_bounds__(256) k_h1_bnd_assemble(KArgs A, const double *
 __global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ H1 volume terms: element factors + row gather
+//
+// Every volume term on affine simplices is a contraction K_e = sum_f F_e[f] T_f of per-element
+// factors with reference tables (nl_test x nl_trial each):
+//   grad-grad   F = c |det J| (J^-1 J^-T)_ab, a <= b        T = int (dphi_i/dxh_a dphi_j/dxh_b
+//                                                                  + [a < b] dphi_i/dxh_b dphi_j/dxh_a)
+//   mass        F = c |det J|                               T = int phi_i phi_j
+//   convection / d_dir test / d_dir trial (constant vector v: the wind, or e_dir)
+//               F = c |det J| (J^-1 v)_a                    T = int dphi_i/dxh_a phi_j  (test)
+//                                                           T = int phi_i dphi_j/dxh_a  (trial)
+//   a weight field w = sum_m w_m psi_m (diffuse-interface phi, or 1 - phi) multiplies F by w_m and
+//   T by psi_m; a wind field W = sum_m W_m psi_m gives F = c |det J| (J^-1 W_m)_a,
+//   T = int psi_m dphi_i/dxh_a phi_j.
+// The tables are integrated exactly on the reference element, so field coefficients carry no
+// quadrature error beyond the field's own representation.
+//
+// k_h1_factors: one thread per element computes every factor set once and writes it into the
+// element's factor row, where each (test block, trial block) pair finds its factors contiguous.
+// k_h1_gather: one workgroup per row block; its element entries are stored sorted by local row
+// index L. Phase A, for each L: a thread (column j) keeps the pair's table column T[:, L, j] in
+// registers and forms sum_f F_e[f] T[f, L, j] for the entries with that L, staging value and
+// column position in LDS; the tables are read once per (block, L) instead of once per entry.
+// Phase B: a group of nloc lanes per row walks the row's entries in element order, adding each
+// lane's value at its column's position in an LDS copy of the row (the columns of one entry are
+// distinct). The block's rows, one contiguous CSR range, are then written once. No colouring, no
+// read-modify-write of CSR values, a fixed summation order (bitwise reproducible).
+
+constexpr int MAXFS = 16;  // factor sets per launch
+enum { FS_GG = 0, FS_MASS = 1, FS_VEC = 2, FS_WIND = 3 };
+
+struct KFactorSet {
+  int kind, nf, ndst;
+  int dst[4];        // offsets in the factor row where this set's nf values go
+  int weight_mode;   // OCMP_WEIGHT_*
+  int wfield;        // weight field (weight modes) or first wind field (FS_WIND)
+  double coef;
+  const double *elem_coef;
+  double vec[3];
+};
+
+struct KFactors {
+  int nsets, NF;
+  KFactorSet fs[MAXFS];
+  KField f[MAXF];
+};
+
+struct KPairG {
+  int nfp, fbase, toff, nli, nlj;
+};
+
+struct KGather {
+  int NL, NF, nb, accumulate, emax, accmax;
+  int loc_off[MAXB + 1];
+  KPairG pg[MAXB * MAXB];
+  const double *tab;
+};
+
+// block id with consecutive ids on one XCD (gridDim.x is a multiple of 8; blocks b, b+8, ... of a
+// launch run on one XCD), so neighbouring row blocks share an L2
+__device__ __forceinline__ int64_t xcd_block_id() {
+  const int64_t per = gridDim.x / 8;
+  return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+}
+
+// local coefficient m of a field on element e
+__device__ __forceinline__ double field_coef(const KField &f, int64_t e, int m) {
+  return f.vec[f.edofs[e * f.ld + f.loc_off + m]];
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_h1_factors(const KFactors *__restrict__ Kp, int64_t nel,
+                                                    const double *__restrict__ coords,
+                                                    const int32_t *__restrict__ elems, double *__restrict__ F) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  const KFactors &K = *Kp;
+  Geo<D> g;
+  geometry<D>(coords, elems + e * (D + 1), g);
+  double *Fe = F + e * K.NF;
+  constexpr int NS = D * (D + 1) / 2;
+  for (int si = 0; si < K.nsets; ++si) {
+    const KFactorSet &S = K.fs[si];
+    const double c = S.coef * (S.elem_coef ? S.elem_coef[e] : 1.0) * g.adet;
+    if (S.kind == FS_WIND) {  // F[m D + a] = c (J^-1 W_m)_a
+      const KField &f0 = K.f[S.wfield];
+      for (int m = 0; m < f0.nl; ++m) {
+        double w[D];
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc) w[cc] = field_coef(K.f[S.wfield + cc], e, m);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          double sacc = 0.0;
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) sacc += g.Jinv[a][cc] * w[cc];
+          for (int dd = 0; dd < S.ndst; ++dd) Fe[S.dst[dd] + m * D + a] = c * sacc;
+        }
+      }
+      continue;
+    }
+    double base[NS];
+    int nb;
+    if (S.kind == FS_GG) {
+      nb = NS;
+      int k = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int b = a; b < D; ++b) {
+          double mm = 0.0;
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) mm += g.Jinv[a][cc] * g.Jinv[b][cc];
+          base[k++] = c * mm;
+        }
+    } else if (S.kind == FS_MASS) {
+      nb = 1;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) base[k] = k == 0 ? c : 0.0;
+    } else {  // FS_VEC
+      nb = D;
+#pragma unroll
+      for (int a = 0; a < NS; ++a) {
+        double mm = 0.0;
+        if (a < D)
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) mm += g.Jinv[a][cc] * S.vec[cc];
+        base[a] = c * mm;
+      }
+    }
+    const int nw = S.weight_mode == OCMP_WEIGHT_NONE ? 1 : K.f[S.wfield].nl;
+    for (int m = 0; m < nw; ++m) {
+      double wm = 1.0;
+      if (S.weight_mode != OCMP_WEIGHT_NONE) {
+        wm = field_coef(K.f[S.wfield], e, m);
+        // 1 - w: the constant 1 is the sum of the vertex functions (the first D+1 of an H1 basis)
+        if (S.weight_mode == OCMP_WEIGHT_ONE_MINUS_FIELD) wm = (m <= D ? 1.0 : 0.0) - wm;
+      }
+      for (int dd = 0; dd < S.ndst; ++dd)
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+          if (k < nb) Fe[S.dst[dd] + m * nb + k] = wm * base[k];
+    }
+  }
+}
+
+template <typename SlotT, int MAXPF>
+__global__ void __launch_bounds__(256) k_h1_gather(const KGather *__restrict__ Kp, int64_t nblk,
+                                                   const int32_t *__restrict__ gblk, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ pptr, const int32_t *__restrict__ pe,
+                                                   const uint16_t *__restrict__ gpos,
+                                                   const uint16_t *__restrict__ gloff,
+                                                   const SlotT *__restrict__ pslot, const double *__restrict__ F,
+                                                   double *__restrict__ val) {
+  extern __shared__ double lds_rows[];
+  const KGather &K = *Kp;
+  const int64_t b = xcd_block_id();
+  if (b >= nblk) return;
+  const int NL = K.NL;
+  const int r0 = gblk[b], r1 = gblk[b + 1];
+  const int e0 = pptr[r0], ne = pptr[r1] - e0;
+  const int z0 = rp[r0], nz = rp[r1] - z0;
+  double *V = lds_rows;                                     // ne * NL staged values (sorted entries)
+  double *acc = lds_rows + (size_t)K.emax * NL;             // nz row values
+  SlotT *S = reinterpret_cast<SlotT *>(acc + K.accmax);     // ne * NL positions within the row
+  uint16_t *Gp = reinterpret_cast<uint16_t *>(S + (size_t)K.emax * NL);  // ne sorted positions
+  for (int t = threadIdx.x; t < nz; t += blockDim.x) acc[t] = 0.0;
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) Gp[t] = gpos[e0 + t];
+  // phase A
+  {
+    const int G = blockDim.x / NL;
+    const int g = threadIdx.x / NL, lj = threadIdx.x - g * NL;
+    int bj = 0;
+    while (bj + 1 < K.nb && lj >= K.loc_off[bj + 1]) ++bj;
+    const int ljb = lj - K.loc_off[bj];
+    const uint16_t *off = gloff + b * (NL + 1);
+    for (int L = 0; L < NL; ++L) {
+      const int qa = off[L], qb = off[L + 1];
+      if (qa == qb) continue;
+      int bi = 0;
+      while (bi + 1 < K.nb && L >= K.loc_off[bi + 1]) ++bi;
+      const KPairG P = K.pg[bi * K.nb + bj];
+      double Tr[MAXPF];
+      const double *__restrict__ T = K.tab + P.toff + (L - K.loc_off[bi]) * P.nlj + ljb;
+      const int stride = P.nli * P.nlj;
+#pragma unroll
+      for (int k = 0; k < MAXPF; ++k) Tr[k] = k < P.nfp ? T[k * stride] : 0.0;
+      if (g < G)
+        for (int q = qa + g; q < qb; q += G) {
+          const int64_t sp = e0 + q;
+          const double *__restrict__ Fe = F + (int64_t)pe[sp] * K.NF + P.fbase;
+          double v = 0.0;
+#pragma unroll
+          for (int k = 0; k < MAXPF; ++k)
+            if (k < P.nfp) v += Fe[k] * Tr[k];
+          V[q * NL + lj] = v;
+          S[q * NL + lj] = pslot[sp * NL + lj];
+        }
+    }
+  }
+  __syncthreads();
+  // phase B: groups of NL lanes within a wave, one row each
+  {
+    const int wl = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int Gw = 64 / NL;
+    const int grp = lane / NL, lj = lane - grp * NL;
+    const int nw = blockDim.x >> 6;
+    if (grp < Gw)
+      for (int r = r0 + wl * Gw + grp; r < r1; r += nw * Gw) {
+        double *ar = acc + (rp[r] - z0);
+        const int p1 = pptr[r + 1] - e0;
+        for (int p = pptr[r] - e0; p < p1; ++p) {
+          const int t = Gp[p] * NL + lj;
+          ar[S[t]] += V[t];
+        }
+      }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < nz; t += blockDim.x) {
+    double x = acc[t];
+    if (K.accumulate) x += val[z0 + t];
+    val[z0 + t] = x;
+  }
 }
 
 // ------------------------------------------------------------------ DG owner-computes kernel (triangles)
@@ -1489,19 +1714,284 @@ static void build_kargs(Space &s, const std::vector<ocmp_term> &terms, const std
   }
 }
 
-void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
-                        bool accumulate) {
+// reference tables of one volume term in the contraction form (nf x nl_test x nl_trial), exact;
+// grad-grad in the symmetric form (a <= b)
+static std::vector<double> gather_tables(const Space &s, const ocmp_term &t, const std::vector<ocmp_field> &fields,
+                                         int &nf) {
+  const int d = s.mesh->dim;
+  const Block &bt = s.blocks[t.test], &bj = s.blocks[t.trial];
+  const bool wind_field = t.kind == OCMP_BFI_CONV && t.wind_field >= 0;
+  const bool weighted = t.weight_mode != OCMP_WEIGHT_NONE;
+  int fo = 0, nw = 1, fkind = 0;
+  if (wind_field || weighted) {
+    const ocmp_field &f = fields[wind_field ? t.wind_field : t.weight_field];
+    const Block &fb = f.space->s.blocks[f.block];
+    fo = fb.order;
+    nw = fb.nloc;
+    fkind = fb.kind;
+  }
+  const int ns = d * (d + 1) / 2;
+  int nb = 0;
+  switch (t.kind) {
+    case OCMP_BFI_GRADGRAD: nb = ns; break;
+    case OCMP_BFI_MASS: nb = 1; break;
+    default: nb = d; break;
+  }
+  nf = nw * nb;
+  const int ni = bt.nloc, nj = bj.nloc;
+  std::vector<double> T((size_t)nf * ni * nj, 0.0);
+  Rule rule = simplex_rule(d, bt.order + bj.order + fo);
+  std::vector<double> pt(ni), gt((size_t)ni * d), pj(nj), gj((size_t)nj * d), pw(nw), gw((size_t)nw * d);
+  const size_t tsz = (size_t)ni * nj;
+  for (int q = 0; q < rule.nq; ++q) {
+    const double *lam = &rule.lam[q * (d + 1)];
+    eval_basis(bt.kind, d, bt.order, lam, pt.data(), gt.data());
+    eval_basis(bj.kind, d, bj.order, lam, pj.data(), gj.data());
+    if (wind_field || weighted)
+      eval_basis(fkind, d, fo, lam, pw.data(), gw.data());
+    else
+      pw[0] = 1.0;
+    const double w = rule.w[q];
+    for (int m = 0; m < nw; ++m) {
+      const double wm = w * pw[m];
+      for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j) {
+          double *o = &T[(size_t)m * nb * tsz + (size_t)i * nj + j];
+          if (t.kind == OCMP_BFI_GRADGRAD) {
+            int k = 0;
+            for (int a = 0; a < d; ++a)
+              for (int bb = a; bb < d; ++bb, ++k) {
+                double v = gt[i * d + a] * gj[j * d + bb];
+                if (bb != a) v += gt[i * d + bb] * gj[j * d + a];
+                o[k * tsz] += wm * v;
+              }
+          } else if (t.kind == OCMP_BFI_MASS) {
+            o[0] += wm * pt[i] * pj[j];
+          } else if (t.kind == OCMP_BFI_DTRIAL) {
+            for (int a = 0; a < d; ++a) o[a * tsz] += wm * pt[i] * gj[j * d + a];
+          } else {  // CONV, DTEST: the test function carries the derivative
+            for (int a = 0; a < d; ++a) o[a * tsz] += wm * gt[i * d + a] * pj[j];
+          }
+        }
+    }
+  }
+  return T;
+}
+
+// can the row gather assemble these volume terms (field weights need H1 fields; a convection
+// term takes either a wind field or a weight, not both)
+static bool gather_supported(const Matrix *M, const std::vector<ocmp_term> &terms,
+                             const std::vector<ocmp_field> &fields) {
+  if (!M->d_pptr.p || M->gblk.empty() || M->space->nloc > 64) return false;
+  for (auto &t : terms) {
+    if (is_bnd_kind(t.kind)) continue;
+    if (t.kind == OCMP_BFI_DG_FACET) return false;
+    if (t.kind == OCMP_BFI_CONV && t.wind_field >= 0 && t.weight_mode != OCMP_WEIGHT_NONE) return false;
+    if (t.weight_mode != OCMP_WEIGHT_NONE &&
+        fields[t.weight_field].space->s.blocks[fields[t.weight_field].block].kind != OCMP_H1)
+      return false;
+  }
+  return true;
+}
+
+// returns false (nothing launched) when some block pair needs more than 64 factors
+static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const std::vector<ocmp_field> &fields,
+                             bool accumulate) {
   Space &s = *M->space;
   Ctx *ctx = M->ctx;
   hipStream_t st = ctx->stream;
-  for (auto &t : terms) OCMP_REQUIRE(t.kind != OCMP_BFI_DG_FACET, "facet terms need an L2 (DG) space");
+  const Mesh &m = *s.mesh;
+  const int d = m.dim;
+  OCMP_REQUIRE((int)vol.size() <= MAXT && (int)fields.size() <= MAXF, "too many terms or fields in one call");
+  KFactors KF;
+  KGather KG;
+  std::memset(&KF, 0, sizeof KF);
+  std::memset(&KG, 0, sizeof KG);
+  for (size_t i = 0; i < fields.size(); ++i) {
+    const Space &fs = fields[i].space->s;
+    const Block &fb = fs.blocks[fields[i].block];
+    KField &kf = KF.f[i];
+    kf.vec = fields[i].vec;
+    kf.edofs = fs.d_edofs.p;
+    kf.ld = fs.nloc;
+    kf.loc_off = fb.loc_off;
+    kf.nl = fb.nloc;
+  }
+  // factor sets (terms with equal factors share one) and each term's tables
+  std::vector<std::string> set_keys;
+  std::vector<int> term_set(vol.size());
+  std::vector<std::vector<double>> term_tab(vol.size());
+  for (size_t ti = 0; ti < vol.size(); ++ti) {
+    const ocmp_term &t = vol[ti];
+    int nf = 0;
+    term_tab[ti] = gather_tables(s, t, fields, nf);
+    KFactorSet S;
+    std::memset(&S, 0, sizeof S);
+    S.coef = t.coef;
+    S.elem_coef = t.elem_coef;
+    S.weight_mode = t.weight_mode;
+    S.wfield = t.weight_mode != OCMP_WEIGHT_NONE ? t.weight_field : -1;
+    if (t.kind == OCMP_BFI_GRADGRAD) {
+      S.kind = FS_GG;
+    } else if (t.kind == OCMP_BFI_MASS) {
+      S.kind = FS_MASS;
+    } else if (t.kind == OCMP_BFI_CONV && t.wind_field >= 0) {
+      S.kind = FS_WIND;
+      S.wfield = t.wind_field;
+      S.weight_mode = OCMP_WEIGHT_NONE;
+    } else {
+      S.kind = FS_VEC;
+      if (t.kind == OCMP_BFI_CONV)
+        for (int c = 0; c < 3; ++c) S.vec[c] = t.wind[c];
+      else
+        S.vec[t.dir] = 1.0;
+    }
+    S.nf = nf;
+    char key[256];
+    std::snprintf(key, sizeof key, "%d|%d|%d|%a|%p|%a|%a|%a|%d", S.kind, S.weight_mode, S.wfield, S.coef,
+                  (const void *)S.elem_coef, S.vec[0], S.vec[1], S.vec[2], nf);
+    auto it = std::find(set_keys.begin(), set_keys.end(), std::string(key));
+    if (it == set_keys.end()) {
+      OCMP_REQUIRE(KF.nsets < MAXFS, "too many factor sets");
+      KF.fs[KF.nsets++] = S;
+      set_keys.emplace_back(key);
+      term_set[ti] = KF.nsets - 1;
+    } else {
+      term_set[ti] = (int)(it - set_keys.begin());
+    }
+  }
+  // per (test block, trial block): the distinct sets its terms use (tables of terms sharing a set
+  // add up); pairs with the same set list share one contiguous range of the factor row
+  const int nb = (int)s.blocks.size();
+  std::vector<std::vector<int>> plist(nb * nb);
+  std::vector<std::vector<double>> ptab(nb * nb);
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = 0; bj < nb; ++bj) {
+      const int pi = bi * nb + bj;
+      const size_t tsz = (size_t)s.blocks[bi].nloc * s.blocks[bj].nloc;
+      std::vector<std::vector<double>> chunk;
+      for (size_t ti = 0; ti < vol.size(); ++ti) {
+        if (vol[ti].test != bi || vol[ti].trial != bj) continue;
+        auto f = std::find(plist[pi].begin(), plist[pi].end(), term_set[ti]);
+        if (f == plist[pi].end()) {
+          plist[pi].push_back(term_set[ti]);
+          chunk.push_back(term_tab[ti]);
+        } else {
+          auto &c = chunk[f - plist[pi].begin()];
+          for (size_t k = 0; k < c.size(); ++k) c[k] += term_tab[ti][k];
+        }
+      }
+      for (auto &c : chunk) ptab[pi].insert(ptab[pi].end(), c.begin(), c.end());
+      (void)tsz;
+    }
+  std::vector<std::vector<int>> lists;
+  std::vector<int> list_base;
+  int NF = 0, maxpf = 0;
+  TableBuilder tb;
+  for (int pi = 0; pi < nb * nb; ++pi) {
+    KPairG &P = KG.pg[pi];
+    P.nli = s.blocks[pi / nb].nloc;
+    P.nlj = s.blocks[pi % nb].nloc;
+    if (plist[pi].empty()) continue;
+    int nfp = 0;
+    for (int si : plist[pi]) nfp += KF.fs[si].nf;
+    maxpf = std::max(maxpf, nfp);
+    auto it = std::find(lists.begin(), lists.end(), plist[pi]);
+    int base;
+    if (it == lists.end()) {
+      base = NF;
+      int o = NF;
+      for (int si : plist[pi]) {
+        KFactorSet &S = KF.fs[si];
+        OCMP_REQUIRE(S.ndst < 4, "a factor set used by more than four factor lists");
+        S.dst[S.ndst++] = o;
+        o += S.nf;
+      }
+      NF = o;
+      lists.push_back(plist[pi]);
+      list_base.push_back(base);
+    } else {
+      base = list_base[it - lists.begin()];
+    }
+    P.nfp = nfp;
+    P.fbase = base;
+    P.toff = tb.add(ptab[pi]);
+  }
+  if (maxpf > 64) return false;
+  KF.NF = NF;
+  DBuf<double> *tab = cached_table(s, "G" + term_key(vol, fields), tb, st);
+  KG.NL = s.nloc;
+  KG.NF = NF;
+  KG.nb = nb;
+  KG.emax = M->gemax;
+  KG.accmax = M->gaccmax;
+  KG.accumulate = accumulate ? 1 : 0;
+  for (int bb = 0; bb < nb; ++bb) KG.loc_off[bb] = s.blocks[bb].loc_off;
+  KG.loc_off[nb] = s.nloc;
+  KG.tab = tab->p;
+  // descriptors go to device memory (the kernels index them at run time)
+  std::vector<char> desc(sizeof KF + sizeof KG);
+  std::memcpy(desc.data(), &KF, sizeof KF);
+  std::memcpy(desc.data() + sizeof KF, &KG, sizeof KG);
+  M->d_gdesc.alloc(desc.size());
+  OCMP_HIP(hipMemcpyAsync(M->d_gdesc.p, desc.data(), desc.size(), hipMemcpyHostToDevice, st));
+  const KFactors *dKF = reinterpret_cast<const KFactors *>(M->d_gdesc.p);
+  const KGather *dKG = reinterpret_cast<const KGather *>(M->d_gdesc.p + sizeof KF);
+  M->d_factors.alloc((size_t)m.nel * std::max(NF, 1));
+  {
+    KTimer tm(ctx, ctx->cls("assemble_h1_factors"), (double)m.nel * (8.0 * NF + 4.0 * (d + 1)) + 8.0 * m.nv * d);
+    if (d == 2)
+      k_h1_factors<2><<<nblk(m.nel), 256, 0, st>>>(dKF, m.nel, m.d_coords.p, m.d_elems.p, M->d_factors.p);
+    else
+      k_h1_factors<3><<<nblk(m.nel), 256, 0, st>>>(dKF, m.nel, m.d_coords.p, m.d_elems.p, M->d_factors.p);
+  }
+  {
+    const int sb = M->d_pslot8.p ? 1 : 2;
+    // algorithmic: every CSR value written once (read too when accumulating), the entries' elements,
+    // sorted positions and column positions, every element's factor row once, the row pointers
+    KTimer tm(ctx, ctx->cls("assemble_h1_gather"),
+              8.0 * M->nnz * (accumulate ? 2 : 1) + (double)M->npatch * (6.0 + s.nloc * sb) +
+                  8.0 * m.nel * NF + 8.0 * (M->n + 1));
+    const int64_t nb_rows = (int64_t)M->gblk.size() - 1;
+    const unsigned grid = (unsigned)((nb_rows + 7) / 8 * 8);
+    const size_t shm = (size_t)M->gemax * s.nloc * (sizeof(double) + sb) + (size_t)M->gaccmax * sizeof(double) +
+                       (size_t)M->gemax * sizeof(uint16_t);
+#define OCMP_GATHER(ST, SLOTS, MPF)                                                                               \
+  k_h1_gather<ST, MPF><<<grid, 256, shm, st>>>(dKG, nb_rows, M->d_gblk.p, M->d_row_ptr.p, M->d_pptr.p, M->d_pe.p, \
+                                               M->d_gpos.p, M->d_gloff.p, SLOTS, M->d_factors.p, M->d_val.p)
+#define OCMP_GATHER_PF(ST, SLOTS)              \
+  if (maxpf <= 8) OCMP_GATHER(ST, SLOTS, 8);   \
+  else if (maxpf <= 16) OCMP_GATHER(ST, SLOTS, 16); \
+  else if (maxpf <= 32) OCMP_GATHER(ST, SLOTS, 32); \
+  else OCMP_GATHER(ST, SLOTS, 64);
+    if (sb == 1) {
+      OCMP_GATHER_PF(uint8_t, M->d_pslot8.p)
+    } else {
+      OCMP_GATHER_PF(uint16_t, M->d_pslot16.p)
+    }
+#undef OCMP_GATHER_PF
+#undef OCMP_GATHER
+  }
+  // the descriptor copy reads host memory that goes out of scope on return
+  OCMP_HIP(hipStreamSynchronize(st));
+  OCMP_HIP(hipGetLastError());
+  return true;
+}
+
+// the colour-scheduled element scatter: boundary-facet terms, and volume terms the gather does
+// not cover
+static void launch_h1_colour(Matrix *M, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
+                             bool accumulate, bool volume_terms) {
+  Space &s = *M->space;
+  Ctx *ctx = M->ctx;
+  hipStream_t st = ctx->stream;
+  if (!M->d_slot.p) build_h1_schedule(M);
   KArgs A;
   TableBuilder tb;
   build_kargs(s, terms, fields, A, tb, st);
   DBuf<double> *tab = cached_table(s, term_key(terms, fields), tb, st);
   A.tab = tab->p;
-  // algorithmic: every CSR value written once + element table + coordinates
-  KTimer tm(ctx, ctx->cls("assemble_h1"),
+  KTimer tm(ctx, ctx->cls(volume_terms ? "assemble_h1" : "assemble_h1_bnd"),
             8.0 * M->nnz + 4.0 * s.mesh->nel * (s.mesh->dim + 1) + 8.0 * s.mesh->nv * s.mesh->dim);
   if (!accumulate) k_fill<<<nblk(M->nnz), 256, 0, st>>>(M->d_val.p, M->nnz, 0.0);
   const Mesh &m = *s.mesh;
@@ -1511,12 +2001,14 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
   for (int c = 0; c < s.ncolours; ++c) {
     int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;
     if (!cn) continue;
-    if (m.dim == 2)
-      k_h1_assemble<2><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb, cn,
-                                                           M->d_slot.p, M->d_val.p);
-    else
-      k_h1_assemble<3><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb, cn,
-                                                           M->d_slot.p, M->d_val.p);
+    if (volume_terms) {
+      if (m.dim == 2)
+        k_h1_assemble<2><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb,
+                                                             cn, M->d_slot.p, M->d_val.p);
+      else
+        k_h1_assemble<3><<<nblk(cn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, s.d_colour_elems.p, cb,
+                                                             cn, M->d_slot.p, M->d_val.p);
+    }
     if (has_bnd) {
       const int64_t bb = s.bnd_colour_off[c], bn = s.bnd_colour_off[c + 1] - bb;
       if (!bn) continue;
@@ -1529,6 +2021,23 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
     }
   }
   OCMP_HIP(hipGetLastError());
+}
+
+void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
+                        bool accumulate) {
+  for (auto &t : terms) OCMP_REQUIRE(t.kind != OCMP_BFI_DG_FACET, "facet terms need an L2 (DG) space");
+  std::vector<ocmp_term> vol, bnd;
+  for (auto &t : terms) (is_bnd_kind(t.kind) ? bnd : vol).push_back(t);
+  const bool gather = !vol.empty() && gather_supported(M, vol, fields);
+  if (!gather) {
+    launch_h1_colour(M, terms, fields, accumulate, !vol.empty());
+    return;
+  }
+  if (!launch_h1_gather(M, vol, fields, accumulate)) {
+    launch_h1_colour(M, terms, fields, accumulate, true);
+    return;
+  }
+  if (!bnd.empty()) launch_h1_colour(M, bnd, fields, true, false);
 }
 
 void launch_assemble_dg(Matrix *M, const std::vector<ocmp_term> &terms, bool accumulate) {

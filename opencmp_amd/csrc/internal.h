@@ -219,8 +219,29 @@ struct Matrix {
   std::vector<int32_t> row_ptr, col;
   DBuf<int32_t> d_row_ptr, d_col;
   DBuf<double> d_val;
-  // H1 scatter schedule: element -> CSR slot map (colours live in the space)
+  // H1 boundary-facet scatter: element -> CSR slot map (colours live in the space), built on the
+  // first assembly with boundary terms
   DBuf<int32_t> d_slot;              // nel*nloc*nloc
+  // H1 row-owner gather (volume terms): per CSR row its element entries in element order
+  // (patch), every entry's compound local row index and the position in the row of each of the
+  // element's nloc columns (1 byte when every row has at most 256 nonzeros, else 2)
+  DBuf<int32_t> d_pptr;              // n+1
+  DBuf<int32_t> d_pe;                // entries: element
+  DBuf<uint8_t> d_pslot8;            // entries x nloc (maxrow <= 256)
+  DBuf<uint16_t> d_pslot16;          // entries x nloc (maxrow > 256)
+  int64_t npatch = 0;
+  int maxrow = 0;
+  // row blocks of the gather: rows [gblk[b], gblk[b+1]) with at most gemax element entries and
+  // gaccmax nonzeros (staged in LDS); empty when some row alone would not fit. Within a block the
+  // entries are stored sorted by local row index (d_pe, d_pslot in that order): d_gloff[b*(nl+1)+L]
+  // is where local row L starts (relative to the block's first entry), d_gpos[p] the sorted position
+  // of the p-th entry in row order (pptr indexes row order)
+  std::vector<int32_t> gblk;
+  DBuf<int32_t> d_gblk;
+  DBuf<uint16_t> d_gpos, d_gloff;
+  int gemax = 0, gaccmax = 0;
+  DBuf<double> d_factors;            // nel x (factors per element) of the last assembly
+  DBuf<char> d_gdesc;                // the gather's kernel descriptors
   // DG: per element-facet block position in the block row (0..3), diag block position
   DBuf<int8_t> d_dg_pos;             // nel*(dim+2): pos of diag block, then of each facet neighbour (-1 none)
   DBuf<int32_t> d_dg_bcol;           // nel*4 block columns (-1 padded), sorted
@@ -354,6 +375,7 @@ double unet_train_step(Unet *U, Matrix *A, Precond *P, const double *samples, in
 
 // greedy colouring + slot map (host)
 void build_h1_schedule(Matrix *A);
+void build_h1_gather(Matrix *A);
 void build_colouring(Space &s);
 void build_pattern(Matrix *A);
 // nonzeros per row block of the LDS-staged SpMV (LDS: 16 KB per block in both precisions;

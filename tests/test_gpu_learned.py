@@ -105,8 +105,9 @@ def test_guess_candidates_keep_the_reference_iteration(gpu_ctx):
     """Default policy: the linear extrapolation and a network trained on the correction to it
     compete with the plain guess on the FP64 residual of the step's first Picard system, which
     stays linearised about the previous solution. The Picard iteration is then the plain one:
-    the same states to the linear tolerance (1e-10 here, so that the solutions of the two runs
-    agree to ~1e-6 on this saddle-point system), at no more linear work."""
+    the same states to the linear tolerance (1e-12 here: the two runs start their Krylov solves
+    from different vectors, so their solutions differ by up to the condition number of this
+    saddle-point system times that tolerance), at no more linear work."""
     from opencmp_amd.learned import GuessPolicy, MLPGuess
     from opencmp_amd.solvers import ImplicitEulerSolver
     say('guess candidates: 30 recorded steps, train (base 1), 6 compared steps')
@@ -127,7 +128,7 @@ def test_guess_candidates_keep_the_reference_iteration(gpu_ctx):
     assert np.isfinite(losses).all()
 
     def run(policy):
-        mm = channel_model(gpu_ctx, 24, tol=1e-10)
+        mm = channel_model(gpu_ctx, 24, tol=1e-12)
         for h in reversed(ring[1:]):
             policy.push(h)
         ss = ImplicitEulerSolver(mm, dt=1e-3, gfu0=type(s.gfu_prev)(mm.fes, state.clone()), guess=policy)
