@@ -461,6 +461,19 @@ void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb) {
   rb.nnzb = nnzb;
 }
 
+void ensure_diag_slots(Matrix *A) {
+  if (A->d_dslot.p) return;
+  std::vector<int32_t> d(A->n, -1);
+  const int32_t *rp = A->row_ptr.data(), *col = A->col.data();
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < A->n; ++r) {
+    const int32_t *q = std::lower_bound(col + rp[r], col + rp[r + 1], (int32_t)r);
+    if (q != col + rp[r + 1] && *q == r) d[r] = (int32_t)(q - col);
+  }
+  A->d_dslot.upload(d, A->ctx->stream);
+  OCMP_HIP(hipStreamSynchronize(A->ctx->stream));
+}
+
 void ensure_row_blocks32(Matrix *A) {
   if (!A->rb32.nnzb) build_row_blocks(A, A->rb32, SPMV_NNZB32);
 }
@@ -533,8 +546,72 @@ void build_schur_matrix(Precond *P) {
   S->d_col.upload(S->col, st);
   S->d_val.alloc(nnz);
   build_row_blocks(S, S->rb64, SPMV_NNZB64);
+  build_row_blocks(S, P->srb, SCG_NNZB);
   S->assembled = true;
   P->d_sseg.upload(seg, st);
+  // the product lists of S's values: for entry (p, q), the velocity columns u of row p in slot
+  // order, each with the slot of column q in row u (one per u, since u's Schur columns are sorted)
+  {
+    std::vector<int64_t> rcnt(ns + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < ns; ++i) {
+      const int64_t p = s0 + i;
+      int64_t c = 0;
+      for (int32_t j = rp[p]; j < rp[p + 1]; ++j) {
+        const int32_t u = col[j];
+        if (u >= s0 && u < s1) continue;
+        if (freed[u]) c += seg[2 * u + 1] - seg[2 * u];
+      }
+      rcnt[i + 1] = c;
+    }
+    for (int64_t i = 0; i < ns; ++i) rcnt[i + 1] += rcnt[i];
+    if (rcnt[ns] <= SCHUR_MAX_PAIRS) {
+      std::vector<int32_t> cslot(nnz, -1), pptr(nnz + 1, 0), pj(rcnt[ns]), pt(rcnt[ns]);
+#pragma omp parallel
+      {
+        std::vector<int32_t> cnt;
+#pragma omp for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < ns; ++i) {
+          const int64_t p = s0 + i;
+          const int32_t k0 = S->row_ptr[i], len = S->row_ptr[i + 1] - k0;
+          const int32_t *sc = S->col.data() + k0;
+          auto pos = [&](int32_t q) { return (int32_t)(std::lower_bound(sc, sc + len, q) - sc); };
+          for (int32_t j = seg[2 * p]; j < seg[2 * p + 1]; ++j) cslot[k0 + pos(col[j] - (int32_t)s0)] = j;
+          cnt.assign(len, 0);
+          auto each = [&](auto &&fn) {
+            for (int pass = 0; pass < 2; ++pass) {
+              const int32_t a = pass ? seg[2 * p + 1] : rp[p], e = pass ? rp[p + 1] : seg[2 * p];
+              for (int32_t j = a; j < e; ++j) {
+                const int32_t u = col[j];
+                if (!freed[u]) continue;
+                for (int32_t t = seg[2 * u]; t < seg[2 * u + 1]; ++t) fn(j, t, pos(col[t] - (int32_t)s0));
+              }
+            }
+          };
+          each([&](int32_t, int32_t, int32_t k) { ++cnt[k]; });
+          // the row's lists start at the row's first pair; entry k's list after entries < k
+          int64_t off = rcnt[i];
+          std::vector<int64_t> at(len);
+          for (int32_t k = 0; k < len; ++k) {
+            at[k] = off;
+            pptr[k0 + k] = (int32_t)off;
+            off += cnt[k];
+          }
+          each([&](int32_t j, int32_t t, int32_t k) {
+            pj[at[k]] = j;
+            pt[at[k]] = t;
+            ++at[k];
+          });
+        }
+      }
+      pptr[nnz] = (int32_t)rcnt[ns];
+      P->d_scslot.upload(cslot, st);
+      P->d_spptr.upload(pptr, st);
+      P->d_spj.upload(pj, st);
+      P->d_spt.upload(pt, st);
+      OCMP_HIP(hipStreamSynchronize(st));  // the host vectors end with this scope
+    }
+  }
   OCMP_HIP(hipStreamSynchronize(st));
   P->s0 = s0;
   P->s1 = s1;
@@ -636,8 +713,8 @@ void build_h1_gather(Matrix *A) {
   // row blocks: staged values (entries x nl doubles + positions) of about 28 KB and at most 2048
   // nonzeros per block, greedily over consecutive rows; a row that alone exceeds that gets a block
   // of its own and the buffers grow to it, up to 48 KB of LDS in all
-  const int emax = std::max(std::max(1, 28672 / (nl * (8 + sb) + 2)), maxent);
-  const int accmax = std::max(2048, maxrow);
+  const int emax = std::max(std::max(1, 12288 / (nl * (8 + sb) + 2)), maxent);
+  const int accmax = std::max(1024, maxrow);
   A->gblk.clear();
   A->maxrow = maxrow;
   A->gemax = emax;
@@ -655,6 +732,7 @@ void build_h1_gather(Matrix *A) {
   // per block: entries sorted by local row (stable), their positions, and each entry's column
   // positions within its row
   std::vector<int32_t> spe(np);
+  std::vector<uint8_t> spli(np);
   std::vector<uint16_t> gpos(np), gloff((size_t)nblk * (nl + 1));
   std::vector<uint8_t> s8(narrow ? (size_t)np * nl : 0);
   std::vector<uint16_t> s16(narrow ? 0 : (size_t)np * nl);
@@ -675,6 +753,7 @@ void build_h1_gather(Matrix *A) {
         gpos[p] = (uint16_t)q;
         const int64_t sp = e0 + q;
         spe[sp] = pe[p];
+        spli[sp] = pli[p];
         const int32_t *ed = &s.edofs[(int64_t)pe[p] * nl];
         for (int j = 0; j < nl; ++j) {
           const int pos = (int)(std::lower_bound(c0, c1, ed[j]) - c0);
@@ -689,6 +768,7 @@ void build_h1_gather(Matrix *A) {
   hipStream_t st = A->ctx->stream;
   A->d_pptr.upload(pptr, st);
   A->d_pe.upload(spe, st);
+  A->d_pli.upload(spli, st);
   A->d_gpos.upload(gpos, st);
   A->d_gloff.upload(gloff, st);
   A->d_gblk.upload(A->gblk, st);

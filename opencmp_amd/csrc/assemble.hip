@@ -667,8 +667,8 @@ template <typename SlotT, int MAXPF>
 __global__ void __launch_bounds__(256) k_h1_gather(const KGather *__restrict__ Kp, int64_t nblk,
                                                    const int32_t *__restrict__ gblk, const int32_t *__restrict__ rp,
                                                    const int32_t *__restrict__ pptr, const int32_t *__restrict__ pe,
+                                                   const uint8_t *__restrict__ pli,
                                                    const uint16_t *__restrict__ gpos,
-                                                   const uint16_t *__restrict__ gloff,
                                                    const SlotT *__restrict__ pslot, const double *__restrict__ F,
                                                    double *__restrict__ val) {
   extern __shared__ double lds_rows[];
@@ -685,35 +685,44 @@ __global__ void __launch_bounds__(256) k_h1_gather(const KGather *__restrict__ K
   uint16_t *Gp = reinterpret_cast<uint16_t *>(S + (size_t)K.emax * NL);  // ne sorted positions
   for (int t = threadIdx.x; t < nz; t += blockDim.x) acc[t] = 0.0;
   for (int t = threadIdx.x; t < ne; t += blockDim.x) Gp[t] = gpos[e0 + t];
-  // phase A
+  // phase A: (entry, column) items, four independent ones in flight per thread
   {
-    const int G = blockDim.x / NL;
-    const int g = threadIdx.x / NL, lj = threadIdx.x - g * NL;
-    int bj = 0;
-    while (bj + 1 < K.nb && lj >= K.loc_off[bj + 1]) ++bj;
-    const int ljb = lj - K.loc_off[bj];
-    const uint16_t *off = gloff + b * (NL + 1);
-    for (int L = 0; L < NL; ++L) {
-      const int qa = off[L], qb = off[L + 1];
-      if (qa == qb) continue;
-      int bi = 0;
-      while (bi + 1 < K.nb && L >= K.loc_off[bi + 1]) ++bi;
-      const KPairG P = K.pg[bi * K.nb + bj];
-      double Tr[MAXPF];
-      const double *__restrict__ T = K.tab + P.toff + (L - K.loc_off[bi]) * P.nlj + ljb;
-      const int stride = P.nli * P.nlj;
+    constexpr int U = 4;
+    const int nv = ne * NL;
+    for (int base = threadIdx.x; base < nv; base += U * blockDim.x) {
+      int64_t e[U];
+      int tq[U], L[U], lj[U];
 #pragma unroll
-      for (int k = 0; k < MAXPF; ++k) Tr[k] = k < P.nfp ? T[k * stride] : 0.0;
-      if (g < G)
-        for (int q = qa + g; q < qb; q += G) {
-          const int64_t sp = e0 + q;
-          const double *__restrict__ Fe = F + (int64_t)pe[sp] * K.NF + P.fbase;
-          double v = 0.0;
+      for (int u = 0; u < U; ++u) {
+        tq[u] = base + u * blockDim.x;
+        const int t = min(tq[u], nv - 1);
+        const int q = t / NL;
+        lj[u] = t - q * NL;
+        e[u] = pe[e0 + q];
+        L[u] = pli[e0 + q];
+      }
+      double v[U];
 #pragma unroll
-          for (int k = 0; k < MAXPF; ++k)
-            if (k < P.nfp) v += Fe[k] * Tr[k];
-          V[q * NL + lj] = v;
-          S[q * NL + lj] = pslot[sp * NL + lj];
+      for (int u = 0; u < U; ++u) {
+        int bi = 0, bj = 0;
+        while (bi + 1 < K.nb && L[u] >= K.loc_off[bi + 1]) ++bi;
+        while (bj + 1 < K.nb && lj[u] >= K.loc_off[bj + 1]) ++bj;
+        const KPairG P = K.pg[bi * K.nb + bj];
+        const double *__restrict__ T =
+            K.tab + P.toff + (L[u] - K.loc_off[bi]) * P.nlj + (lj[u] - K.loc_off[bj]);
+        const double *__restrict__ Fe = F + e[u] * K.NF + P.fbase;
+        const int stride = P.nli * P.nlj;
+        double acc_v = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXPF; ++k)
+          if (k < P.nfp) acc_v += Fe[k] * T[k * stride];
+        v[u] = acc_v;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (tq[u] < nv) {
+          V[tq[u]] = v[u];
+          S[tq[u]] = pslot[(int64_t)e0 * NL + tq[u]];
         }
     }
   }
@@ -1958,7 +1967,7 @@ static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const
                        (size_t)M->gemax * sizeof(uint16_t);
 #define OCMP_GATHER(ST, SLOTS, MPF)                                                                               \
   k_h1_gather<ST, MPF><<<grid, 256, shm, st>>>(dKG, nb_rows, M->d_gblk.p, M->d_row_ptr.p, M->d_pptr.p, M->d_pe.p, \
-                                               M->d_gpos.p, M->d_gloff.p, SLOTS, M->d_factors.p, M->d_val.p)
+                                               M->d_pli.p, M->d_gpos.p, SLOTS, M->d_factors.p, M->d_val.p)
 #define OCMP_GATHER_PF(ST, SLOTS)              \
   if (maxpf <= 8) OCMP_GATHER(ST, SLOTS, 8);   \
   else if (maxpf <= 16) OCMP_GATHER(ST, SLOTS, 16); \

@@ -239,8 +239,10 @@ struct Matrix {
   std::vector<int32_t> gblk;
   DBuf<int32_t> d_gblk;
   DBuf<uint16_t> d_gpos, d_gloff;
+  DBuf<uint8_t> d_pli;               // sorted entries: local row index
   int gemax = 0, gaccmax = 0;
   DBuf<double> d_factors;            // nel x (factors per element) of the last assembly
+  DBuf<int32_t> d_dslot;             // n: CSR slot of the diagonal entry (-1: none), built on first use
   DBuf<char> d_gdesc;                // the gather's kernel descriptors
   // DG: per element-facet block position in the block row (0..3), diag block position
   DBuf<int8_t> d_dg_pos;             // nel*(dim+2): pos of diag block, then of each facet neighbour (-1 none)
@@ -290,9 +292,14 @@ struct Precond {
   Matrix *S = nullptr;
   DBuf<int32_t> d_sseg;  // per row of A: first and one-past-last CSR slot of its columns in [s0, s1)
   DBuf<double> sdpos;    // n: the positive velocity scale S is built with (0 off the free velocity rows)
+  // S's values as lists of products: for entry k of S, the A slot of its C entry (d_scslot, -1:
+  // none) and the (a_pu slot, a_uq slot) pairs [d_spptr[k], d_spptr[k+1]) in row order of A;
+  // empty (values by search instead) when the lists would exceed SCHUR_MAX_PAIRS
+  DBuf<int32_t> d_scslot, d_spptr, d_spj, d_spt;
+  RowBlocks srb;         // S's row blocks for the fused inner iteration (SCG_NNZB nonzeros each)
   DBuf<double> sdinv;    // Jacobi of S: 1/S_ii, 0 on rows left out (non-free, or S_ii negligible)
   DBuf<float> sdinv32;
-  DBuf<double> svd;      // PCG vectors r u p s w (5 ns)
+  DBuf<double> svd;      // PCG vectors: r, s, w (two each, by iteration parity), p, u (8 ns)
   DBuf<float> svf;
   DBuf<double> spart;    // PCG scalars and partial sums
   DBuf<int> sflag;       // converged flag of the running inner solve
@@ -383,11 +390,16 @@ void build_pattern(Matrix *A);
 // FP32 0.69 ms at 4096 against 0.83 at 2048 and 0.85 at 8192)
 constexpr int SPMV_NNZB64 = 2048;
 constexpr int SPMV_NNZB32 = 4096;
+// the fused Schur PCG iteration: row blocks of S of this many nonzeros, SCG_NT threads a block
+constexpr int SCG_NNZB = 8192;
+constexpr int SCG_NT = 1024;
 void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb);
 void ensure_row_blocks32(Matrix *A);
 // the pattern of a Schur preconditioner's S = C + B diag(A)^{-1} B^T and the per-row Schur
 // column ranges of A (host, once per preconditioner)
 void build_schur_matrix(Precond *P);
+void ensure_diag_slots(Matrix *A);
+constexpr int64_t SCHUR_MAX_PAIRS = (int64_t)1 << 28;
 
 }  // namespace ocmp
 

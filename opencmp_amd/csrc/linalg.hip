@@ -538,14 +538,13 @@ __global__ void k_dg_block_apply_add(int64_t nel, const double *__restrict__ bin
 
 // ------------------------------------------------------------------ Jacobi
 
-__global__ void k_jacobi_diag(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
-                              const double *__restrict__ val, const uint8_t *__restrict__ mask,
-                              double *__restrict__ dinv, float *__restrict__ dinv32) {
+__global__ void k_jacobi_diag(int64_t n, const int32_t *__restrict__ dslot, const double *__restrict__ val,
+                              const uint8_t *__restrict__ mask, double *__restrict__ dinv,
+                              float *__restrict__ dinv32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  double d = 0.0;
-  for (int32_t j = rp[r]; j < rp[r + 1]; ++j)
-    if (col[j] == r) d = val[j];
+  const int32_t j = dslot[r];
+  const double d = j >= 0 ? val[j] : 0.0;
   const double v = (mask && !mask[r]) || d == 0.0 ? 0.0 : 1.0 / d;
   dinv[r] = v;
   if (dinv32) dinv32[r] = (float)v;
@@ -621,25 +620,35 @@ __global__ void k_schur_finish(int64_t n, const uint8_t *__restrict__ flag, cons
 // elsewhere. Positive by construction, so S stays symmetric positive semidefinite where the
 // convection term has made a velocity diagonal small or negative (the P2 vertex functions' mass
 // is small, and the discrete wind is only weakly divergence-free).
-__global__ void k_schur_dpos(int64_t n, int64_t s0, int64_t s1, const int32_t *__restrict__ rp,
-                             const int32_t *__restrict__ col, const double *__restrict__ val,
-                             const uint8_t *__restrict__ mask, double *__restrict__ dpos) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double v = 0.0;
-  if (mask[r] && (r < s0 || r >= s1)) {
-    double d = 0.0, mx = 0.0;
-    for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+// DPOS_LANES lanes per row, each over every DPOS_LANES-th slot, then a max across the group
+constexpr int DPOS_LANES = 16;
+__global__ void __launch_bounds__(NTHR) k_schur_dpos(int64_t n, int64_t s0, int64_t s1,
+                                                     const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ col,
+                                                     const int32_t *__restrict__ dslot,
+                                                     const double *__restrict__ val,
+                                                     const uint8_t *__restrict__ mask, double *__restrict__ dpos) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / DPOS_LANES;
+  const int q = threadIdx.x % DPOS_LANES;
+  const bool on = r < n && mask[r] && (r < s0 || r >= s1);
+  double mx = 0.0;
+  if (on)
+    for (int32_t j = rp[r] + q; j < rp[r + 1]; j += DPOS_LANES) {
       const int32_t c = col[j];
-      if (c >= s0 && c < s1) continue;
-      const double a = fabs(val[j]);
-      if (c == r) d = a;
-      mx = fmax(mx, a);
+      if (c < s0 || c >= s1) mx = fmax(mx, fabs(val[j]));
     }
-    const double dd = fmax(d, 1e-2 * mx);
-    v = dd > 0.0 ? 1.0 / dd : 0.0;
+#pragma unroll
+  for (int o = DPOS_LANES / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, DPOS_LANES));
+  if (r < n && q == 0) {
+    double v = 0.0;
+    if (on) {
+      const int32_t jd = dslot[r];
+      const double d = jd >= 0 ? fabs(val[jd]) : 0.0;
+      const double dd = fmax(d, 1e-2 * mx);
+      v = dd > 0.0 ? 1.0 / dd : 0.0;
+    }
+    dpos[r] = v;
   }
-  dpos[r] = v;
 }
 
 // Values of S = C + B D^{-1} B^T on the Schur rows [s0, s1) of [[A, B^T], [B, -C]], D the
@@ -681,11 +690,33 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
   }
 }
 
+// The same values from the product lists built with S's pattern: one thread per entry of S,
+// -a_pq then the products in the order of row p's slots, the order of k_schur_values.
+__global__ void __launch_bounds__(NTHR) k_schur_values_pairs(int64_t snnz, const int32_t *__restrict__ cslot,
+                                                             const int32_t *__restrict__ pptr,
+                                                             const int32_t *__restrict__ pj,
+                                                             const int32_t *__restrict__ pt,
+                                                             const int32_t *__restrict__ col,
+                                                             const double *__restrict__ val,
+                                                             const double *__restrict__ dpos,
+                                                             double *__restrict__ sval) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= snnz) return;
+  const int32_t c = cslot[k];
+  double acc = c >= 0 ? -val[c] : 0.0;
+  const int32_t m1 = pptr[k + 1];
+  for (int32_t m = pptr[k]; m < m1; ++m) {
+    const int32_t j = pj[m];
+    const double f = val[j] * dpos[col[j]];
+    acc += f * val[pt[m]];
+  }
+  sval[k] = acc;
+}
+
 // Jacobi of S, in two passes like the Schur diagonal: S_ii of the free rows and per-block maxima,
 // then 1/S_ii, or 0 where S_ii is not above SCHUR_REL times the largest (rows the phi weighting
 // has all but removed stay out of the inner solve)
-__global__ void __launch_bounds__(NTHR) k_schur_sdiag(int64_t ns, const int32_t *__restrict__ srp,
-                                                      const int32_t *__restrict__ scol,
+__global__ void __launch_bounds__(NTHR) k_schur_sdiag(int64_t ns, const int32_t *__restrict__ sdslot,
                                                       const double *__restrict__ sval,
                                                       const uint8_t *__restrict__ mask_s, double *__restrict__ sd,
                                                       double *__restrict__ part) {
@@ -693,8 +724,8 @@ __global__ void __launch_bounds__(NTHR) k_schur_sdiag(int64_t ns, const int32_t 
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.0;
   if (i < ns) {
-    for (int32_t k = srp[i]; k < srp[i + 1]; ++k)
-      if (scol[k] == i) d = sval[k];
+    const int32_t k = sdslot[i];
+    d = k >= 0 ? sval[k] : 0.0;
     if (!mask_s[i] || !(d > 0.0)) d = 0.0;
     sd[i] = d;
   }
@@ -1204,9 +1235,11 @@ void precond_update(Precond *P) {
   if (P->kind == OCMP_PRE_JACOBI || P->kind == OCMP_PRE_NONE) {
     P->dinv.alloc(A->n);
     if (mixed) P->dinv32.alloc(A->n);
-    if (P->kind == OCMP_PRE_JACOBI)
-      k_jacobi_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_row_ptr.p, A->d_col.p, A->d_val.p, A->space->d_free.p,
-                                                 P->dinv.p, mixed ? P->dinv32.p : nullptr);
+    if (P->kind == OCMP_PRE_JACOBI) {
+      ensure_diag_slots(A);
+      k_jacobi_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_dslot.p, A->d_val.p, A->space->d_free.p, P->dinv.p,
+                                                 mixed ? P->dinv32.p : nullptr);
+    }
     if (P->kind == OCMP_PRE_JACOBI && P->schur_blocks) {
       if (!P->schur_flag.p) {
         std::vector<uint8_t> f(A->n, 0);
@@ -1232,10 +1265,17 @@ void precond_update(Precond *P) {
         Matrix *S = P->S;
         const int64_t ns = S->n;
         P->sdpos.alloc(A->n);
-        k_schur_dpos<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->s0, P->s1, A->d_row_ptr.p, A->d_col.p, A->d_val.p,
-                                                  A->space->d_free.p, P->sdpos.p);
-        k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
-                                                       A->d_col.p, A->d_val.p, P->d_sseg.p, P->sdpos.p, S->d_val.p);
+        k_schur_dpos<<<nblk(A->n * DPOS_LANES), NTHR, 0, st>>>(A->n, P->s0, P->s1, A->d_row_ptr.p, A->d_col.p,
+                                                               A->d_dslot.p, A->d_val.p, A->space->d_free.p,
+                                                               P->sdpos.p);
+        if (P->d_spptr.p)
+          k_schur_values_pairs<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, P->d_scslot.p, P->d_spptr.p, P->d_spj.p,
+                                                              P->d_spt.p, A->d_col.p, A->d_val.p, P->sdpos.p,
+                                                              S->d_val.p);
+        else
+          k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
+                                                         A->d_col.p, A->d_val.p, P->d_sseg.p, P->sdpos.p, S->d_val.p);
+        ensure_diag_slots(S);
         P->sdinv.alloc(ns);
         if (mixed) P->sdinv32.alloc(ns);
         {
@@ -1243,8 +1283,8 @@ void precond_update(Precond *P) {
           const unsigned nbs = nblk(ns);
           ensure_part(w, (size_t)nbs + 8);
           double *smax = w.part.p + nbs;
-          k_schur_sdiag<<<nbs, NTHR, 0, st>>>(ns, S->d_row_ptr.p, S->d_col.p, S->d_val.p, A->space->d_free.p + P->s0,
-                                              P->sdinv.p, w.part.p);
+          k_schur_sdiag<<<nbs, NTHR, 0, st>>>(ns, S->d_dslot.p, S->d_val.p, A->space->d_free.p + P->s0, P->sdinv.p,
+                                              w.part.p);
           k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nbs, smax);
           k_schur_sdinv<<<nbs, NTHR, 0, st>>>(ns, smax, P->sdinv.p, mixed ? P->sdinv32.p : nullptr);
         }
@@ -1253,12 +1293,13 @@ void precond_update(Precond *P) {
         if (mixed) {
           S->d_val32.alloc(S->nnz);
           k_to_f32<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, S->d_val.p, S->d_val32.p);
-          P->svf.alloc((size_t)5 * ns);
+          P->svf.alloc((size_t)8 * ns);
         } else {
-          P->svd.alloc((size_t)5 * ns);
+          P->svd.alloc((size_t)8 * ns);
         }
         const int sg = std::max(spmv_grid<double, double>(S), spmv_grid<float, float>(S));
-        P->spart.alloc(std::max(P->spart.n, (size_t)SCHUR_SCAL + 2 * red_grid(ns) + sg));
+        const int fg = (int)((std::max<int64_t>(P->srb.nrblk, NXCD) + NXCD - 1) / NXCD * NXCD);
+        P->spart.alloc(std::max(P->spart.n, (size_t)SCHUR_SCAL + red_grid(ns) + 4 * std::max(sg, fg)));
         P->sflag.alloc(1);
       }
     }
@@ -2239,17 +2280,17 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
 // ------------------------------------------------------------------ Schur inner PCG
 //
 // Jacobi PCG on S for the Schur rows of an FGMRES preconditioner step, in the single-reduction
-// form of Chronopoulos and Gear (J. Comput. Appl. Math. 25 (1989) 153-168): w = S u gives
-// delta = (u, w) in the SpMV's epilogue, the previous vector update gave gamma = (r, u), and with
-// beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old) one fused kernel
-// updates p = u + beta p, s = w + beta s (= S p), x += alpha p, r -= alpha s, u = dinv r. Two
-// launches per iteration. Each block sums the partial dot products it needs itself (the same
-// order in every block, so all blocks use the same alpha and beta); gamma partials and the
-// (gamma, alpha) pair of the previous iteration alternate between two slots by iteration. A zero
-// denominator gives a zero step.
-// The host enqueues max_iters iterations without waiting. With a tolerance, the step kernel that
-// finds gamma <= tol^2 gamma_0 sets a device flag instead of updating, and every later launch of
-// the sequence (SpMV included) returns on reading it.
+// form of Chronopoulos and Gear (J. Comput. Appl. Math. 25 (1989) 153-168): with
+// gamma = (r, u), delta = (S u, u), beta = gamma / gamma_old and
+// alpha = gamma / (delta - beta gamma / alpha_old), one launch per iteration updates
+// p = u + beta p, s = w + beta s (= S p), x += alpha p, r -= alpha s, u = dinv r and computes the
+// next w = S u with both partial dot products (k_scg_fused). Each block sums the partial dot
+// products it needs itself (the same order in every block, so all blocks use the same alpha and
+// beta); the (gamma, alpha) pair of the previous iteration alternates between two slots by
+// iteration. A zero denominator gives a zero step.
+// The host enqueues max_iters iterations without waiting. With a tolerance, the launch that finds
+// gamma <= tol^2 gamma_0 sets a device flag instead of updating, and every later launch of the
+// sequence returns on reading it.
 //
 // Scalars (sc): [0..3] (gamma, alpha) slots, [4] gamma_0, [5] iterations done (a running count
 // over the solve).
@@ -2315,17 +2356,48 @@ __device__ __forceinline__ void sum_parts2(const double *__restrict__ pa, int na
   sb = y;
 }
 
+// One Chronopoulos-Gear iteration in one launch. The previous launch left r_i, w_i = S u_i,
+// s_{i-1} and the partials of gamma_i = (r_i, u_i), delta_i = (w_i, u_i); this one forms alpha_i
+// and beta_i from them and, for the rows of its CSR row block, updates
+//   s_i = w_i + beta s_{i-1},  p_i = u_i + beta p_{i-1},  x += alpha p_i,
+//   r_{i+1} = r_i - alpha s_i,  u_{i+1} = dinv r_{i+1}
+// and computes w_{i+1} = S u_{i+1} with the partials of gamma_{i+1} and delta_{i+1}. The SpMV needs
+// u_{i+1} at columns owned by other blocks, so each block recomputes it there from r_i, w_i and
+// s_{i-1} (pointwise, by the same inline code as the owner); r, s and w are double-buffered by
+// iteration parity, so every value a block reads was written by an earlier launch. u is never
+// stored and p, x are only touched by their owner. u_0 = dinv r_0 (it = 0: beta = 0, no s_{-1}).
 template <typename T>
-__global__ void __launch_bounds__(NTHR) k_scg_step(int64_t n, int it, double tol2, T *__restrict__ x,
-                                                   T *__restrict__ r, T *__restrict__ u, T *__restrict__ p,
-                                                   T *__restrict__ s, const T *__restrict__ w,
-                                                   const T *__restrict__ dinv, const double *__restrict__ gpart,
-                                                   const double *__restrict__ dpart, int nd, double *__restrict__ gnew,
-                                                   double *__restrict__ sc, int *__restrict__ flag) {
-  __shared__ double sh[2 * (NTHR / 64)];
+struct ScgNext {
+  T s, r, u;
+};
+template <typename T>
+__device__ __forceinline__ ScgNext<T> scg_next(const T *__restrict__ rin, const T *__restrict__ win,
+                                               const T *__restrict__ sin, const T *__restrict__ dinv, int64_t c,
+                                               T a, T b, bool first) {
+  ScgNext<T> o;
+  o.s = first ? win[c] : win[c] + b * sin[c];
+  o.r = rin[c] - a * o.s;
+  o.u = dinv[c] * o.r;
+  return o;
+}
+
+template <typename T, int NNZB, int NT>
+__global__ void __launch_bounds__(NT) k_scg_fused(
+    int64_t nrblk, const int32_t *__restrict__ rblk, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+    const int32_t *__restrict__ ubase, const int32_t *__restrict__ ucol, const uint16_t *__restrict__ lcol,
+    const T *__restrict__ val, int it, double tol2, T *__restrict__ x, T *__restrict__ p, const T *__restrict__ rin,
+    T *__restrict__ rout, const T *__restrict__ sin, T *__restrict__ sout, const T *__restrict__ win,
+    T *__restrict__ wout, const T *__restrict__ dinv, const double *__restrict__ gin, int ng,
+    const double *__restrict__ din, int nd, double *__restrict__ gout, double *__restrict__ dout,
+    double *__restrict__ sc, int *__restrict__ flag) {
+  constexpr int PER = NNZB / NT;
+  __shared__ double lds[NNZB * sizeof(T) / sizeof(double)];
+  T *xs = reinterpret_cast<T *>(lds);
+  T *prod = xs;
+  double *sh = lds;
   if (*flag) return;
   double g, d;
-  sum_parts2(gpart, gridDim.x, dpart, nd, sh, g, d);
+  sum_parts2(gin, ng, din, nd, sh, g, d);
   const double g0 = it == 0 ? g : sc[4];
   if (!(g > tol2 * g0)) {  // converged (or a zero right-hand side): the same decision in every block
     if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 1;
@@ -2345,30 +2417,102 @@ __global__ void __launch_bounds__(NTHR) k_scg_step(int64_t n, int it, double tol
     sc[5] += 1.0;
   }
   const T a = (T)alpha, b = (T)beta;
+  const bool first = it == 0;
   const int lb = xcd_block();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
-  double acc = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-    T pi = u[i], si = w[i];
-    if (it > 0) {
-      pi += b * p[i];
-      si += b * s[i];
+  double ag = 0.0, ad = 0.0;
+  // the owner's update of row k, given its new S u value
+  auto finish = [&](int64_t k, double sv) {
+    const ScgNext<T> o = scg_next<T>(rin, win, sin, dinv, k, a, b, first);
+    const T uo = dinv[k] * rin[k];
+    const T pk = first ? uo : uo + b * p[k];
+    p[k] = pk;
+    sout[k] = o.s;
+    x[k] += a * pk;
+    rout[k] = o.r;
+    const T wk = (T)sv;
+    wout[k] = wk;
+    ag += (double)o.r * (double)o.u;
+    ad += (double)wk * (double)o.u;
+  };
+  if (lb < nrblk) {
+    const int32_t r0 = rblk[lb], r1 = rblk[lb + 1];
+    const int32_t j0 = rp[r0], j1 = rp[r1];
+    const int nz = j1 - j0;
+    const int nrows = r1 - r0;
+    __syncthreads();  // sh (the partial sums above) aliases the staging buffer
+    if (nz <= NNZB) {
+      const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
+      int a0 = 0, e0 = 0;
+      if (ri0 < nrows) {
+        a0 = rp[r0 + ri0] - j0;
+        e0 = rp[r0 + ri0 + 1] - j0;
+      }
+      int li[PER];
+      T vv[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int t = threadIdx.x + k * NT;
+        li[k] = -1;
+        vv[k] = (T)0;
+        if (t < nz) {
+          li[k] = lcol[j0 + t];
+          vv[k] = val[j0 + t];
+        }
+      }
+      const int u0 = ubase[lb], nu = ubase[lb + 1] - u0;
+      for (int base = 0; base < nu; base += 2 * NT) {
+        const int t0 = base + threadIdx.x, t1 = t0 + NT;
+        const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
+        const T x0 = scg_next<T>(rin, win, sin, dinv, c0, a, b, first).u;
+        const T x1 = scg_next<T>(rin, win, sin, dinv, c1, a, b, first).u;
+        if (t0 < nu) xs[t0] = x0;
+        if (t1 < nu) xs[t1] = x1;
+      }
+      __syncthreads();
+      T pv[PER];
+#pragma unroll
+      for (int k = 0; k < PER; ++k) pv[k] = li[k] >= 0 ? vv[k] * xs[li[k]] : (T)0;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PER; ++k)
+        if (li[k] >= 0) prod[threadIdx.x + k * NT] = pv[k];
+      __syncthreads();
+      for (int base = 0; base < nrows; base += NT / 4) {
+        const int ri = base + ri0;
+        int ja = a0, je = e0;
+        if (base > 0 && ri < nrows) {
+          ja = rp[r0 + ri] - j0;
+          je = rp[r0 + ri + 1] - j0;
+        }
+        double sv = 0.0;
+        if (ri < nrows)
+          for (int t = ja + q; t < je; t += 4) sv += (double)prod[t];
+        sv += __shfl_xor(sv, 1, 4);
+        sv += __shfl_xor(sv, 2, 4);
+        if (ri < nrows && q == 0) finish(r0 + ri, sv);
+      }
+    } else {
+      // a single row longer than the staging buffer
+      double sv = 0.0;
+      for (int t = threadIdx.x; t < nz; t += NT) {
+        const int32_t c = col[j0 + t];
+        sv += (double)(val[j0 + t] * scg_next<T>(rin, win, sin, dinv, c, a, b, first).u);
+      }
+      sv = block_sum(sv, sh);
+      if (threadIdx.x == 0) finish(r0, sv);
     }
-    p[i] = pi;
-    s[i] = si;
-    x[i] += a * pi;
-    const T ri = r[i] - a * si;
-    r[i] = ri;
-    const T ui = dinv[i] * ri;
-    u[i] = ui;
-    acc += (double)ri * (double)ui;
   }
-  acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) gnew[lb] = acc;
+  __syncthreads();  // sh aliases the products
+  ag = block_sum(ag, sh);
+  ad = block_sum(ad, sh);
+  if (threadIdx.x == 0) {
+    gout[lb] = ag;
+    dout[lb] = ad;
+  }
 }
 
-// z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero
+// z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero: the start (x = 0,
+// r_0, u_0, gamma_0 partials; then w_0 = S u_0 with the delta_0 partials) and K fused iterations.
 template <typename T>
 static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
   Matrix *S = P->S;
@@ -2386,29 +2530,35 @@ static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
     dinv = P->sdinv32.p;
     buf = P->svf.p;
   }
-  T *x = z + P->s0, *r = buf, *u = r + n, *p = u + n, *s = p + n, *w = s + n;
-  // the vector kernels run at most 256 blocks (a few rows per thread): every block also sums all
-  // the partials, so fewer blocks means fewer redundant partial loads
+  T *x = z + P->s0;
+  T *r[2] = {buf, buf + n}, *s[2] = {buf + 2 * n, buf + 3 * n}, *w[2] = {buf + 4 * n, buf + 5 * n};
+  T *p = buf + 6 * n, *u = buf + 7 * n;
   const int vg = std::min(red_grid(n), 256), sg = spmv_grid<T, T>(S);
+  const RowBlocks &rb = P->srb;
+  const int fg = (int)((std::max<int64_t>(rb.nrblk, NXCD) + NXCD - 1) / NXCD * NXCD);
+  const int pm = std::max(sg, fg);
   double *sc = P->spart.p;
-  double *gp[2] = {sc + SCHUR_SCAL, sc + SCHUR_SCAL + vg};
-  double *dp = sc + SCHUR_SCAL + 2 * vg;
+  double *g0 = sc + SCHUR_SCAL;
+  double *gq[2] = {g0 + vg, g0 + vg + pm}, *dq[2] = {g0 + vg + 2 * pm, g0 + vg + 3 * pm};
   int *flag = P->sflag.p;
   const double tol2 = P->schur_tol * P->schur_tol;
   const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_schur_f64" : "spmv_schur_f32");
-  const int c_upd = ctx->cls(sizeof(T) == 8 ? "schur_pcg_step_f64" : "schur_pcg_step_f32");
-  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r, u, gp[0], flag);
+  const int c_it = ctx->cls(sizeof(T) == 8 ? "schur_pcg_fused_f64" : "schur_pcg_fused_f32");
+  const double it_bytes = csr_bytes(S, sizeof(T), sizeof(T)) + 10.0 * n * sizeof(T);
+  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r[0], u, g0, flag);
+  {
+    KTimer tm(ctx, c_op, csr_bytes(S, sizeof(T), sizeof(T)));
+    spmv_dispatch<T, T>(S, val, u, w[0], nullptr, nullptr, nullptr, nullptr, 2, dq[0], sg, st, (T)0, nullptr, (T)0,
+                        flag);
+  }
   for (int it = 0; it < K; ++it) {
-    {
-      KTimer tm(ctx, c_op, csr_bytes(S, sizeof(T), sizeof(T)));
-      spmv_dispatch<T, T>(S, val, u, w, nullptr, nullptr, nullptr, nullptr, 2, dp, sg, st, (T)0, nullptr, (T)0,
-                          flag);
-    }
-    {
-      KTimer tm(ctx, c_upd, 12.0 * n * sizeof(T));
-      k_scg_step<T><<<vg, NTHR, 0, st>>>(n, it, tol2, x, r, u, p, s, w, dinv, gp[it & 1], dp, sg,
-                                         gp[(it + 1) & 1], sc, flag);
-    }
+    KTimer tm(ctx, c_it, it_bytes);
+    const int a = it & 1, b = a ^ 1;
+    const double *gin = it == 0 ? g0 : gq[a];
+    const int ng = it == 0 ? vg : fg, nd = it == 0 ? sg : fg;
+    k_scg_fused<T, SCG_NNZB, SCG_NT><<<fg, SCG_NT, 0, st>>>(
+        rb.nrblk, rb.d_rblk.p, S->d_row_ptr.p, S->d_col.p, rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, it, tol2, x,
+        p, r[a], r[b], s[a], s[b], w[a], w[b], dinv, gin, ng, dq[a], nd, gq[b], dq[b], sc, flag);
   }
 }
 

@@ -130,9 +130,11 @@ def test_fgmres_without_correction_matches_gmres(gpu_ctx):
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_learned_correction_in_fgmres(gpu_ctx, precision):
-    """Train the U-net on residuals of a plain solve, attach it: FGMRES converges to the
-    same solution, never much slower than the plain path (fallback), and reports its gain."""
+def test_learned_correction_keeps_fgmres_converging(gpu_ctx, precision):
+    """Train the U-net on residuals of a plain solve, attach it: FGMRES converges to the same
+    solution and is never much slower than the plain path (the fallback bounds the loss to about
+    two restart cycles). No reduction of iterations is claimed: the correction trained this way
+    does not yet gain over the plain preconditioner."""
     from opencmp_amd import core as C
     from opencmp_amd.learned import UNetPrecond, residual_samples
     N = 8
