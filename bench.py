@@ -1,19 +1,24 @@
-"""Assemble+solve seconds per step on one MI355X: mixed precision vs plain FP64.
+"""Assemble+solve seconds per step on one MI355X: mixed precision + learned guess vs FP64.
 
-Workload (the single-GPU configuration the headline metric is quoted on):
-steady convection-diffusion -eps Lap c + w.grad c = 1 on the unit square,
-512x512 squares split into 524,288 triangles, DG P2 (3,145,728 DOFs),
-symmetric interior penalty (sigma = 10 p^2 / h) + upwind flux, eps = 1e-3,
-w = (cos 0.3, sin 0.3), c = 0 on the boundary (Nitsche), GMRES(50) with
-element-block Jacobi to ||b - A x|| / ||b|| <= 1e-8.
+Headline workload (the default, config 4 of the benchmark set): transient incompressible
+Navier-Stokes, Taylor-Hood P2/P1, implicit Euler dt 1e-3 with the Oseen-Picard linearisation
+(<= 3 iterations, tolerance 1e-6), nu 1e-3, on the channel [0,4]x[0,1] with 8 seeded circular
+holes (n=228: 396,953 triangles, 1,794,910 DOFs), parabolic inflow Umax 1.5, do-nothing (stress)
+outlet with the no-backflow term; FGMRES(50) with the Schur preconditioner (velocity Jacobi,
+Jacobi-PCG on C + B D^-1 B^T for the pressure to 1e-1) to ||b - A x|| / ||b|| <= 1e-8.
+A plain FP64 run records 40 steps, the per-DOF MLP is trained on them, then from the same
+state both legs time the same steps:
+  fp64           : FP64 assembly, FP64 FGMRES + FP64 inner PCG (the best FP64 path: the
+                   same FGMRES/Schur algorithm as the mixed leg)
+  mixed+learned  : FP64 assembly and outer residual, FP32 FGMRES + FP32 inner PCG in
+                   iterative refinement, MLP initial guess with the residual fallback
+A step = the right-hand side, every Picard iteration's bilinear assembly, preconditioner
+setup and solve; inputs (mesh, pattern, schedules) are resident in HBM before the timed
+region. The line also carries config 2 (DG-P2 512^2 convection-diffusion, GMRES(50) +
+block Jacobi; mixed s-step vs FP64 s-step vs FP64 standard) as extra keys, the measured
+device-copy bandwidth, and a CPU baseline timed on the host cores.
 
-A step = bilinear + linear assembly, preconditioner setup, and the Krylov
-solve from a zero initial guess; inputs (mesh, pattern, schedules) are resident
-in HBM before the timed region. Paths:
-  fp64  : FP64 assembly, FP64 GMRES on the block-Jacobi-scaled operator
-  mixed : FP64 assembly and outer residual, FP32 inner GMRES cycles (iterative refinement)
-
-    python bench.py [--steps K] [--warmup W] [--n 512] [--tol 1e-8]
+    python bench.py [--steps K] [--warmup W] [--config 1..5] [--no-extra] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -202,6 +207,86 @@ def cpu_baseline(config, a, f, gs, gpu_iters_fp64, n_full):
                       f'Host-CPU restatement, not NGSolve.'}
 
 
+def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
+    """Measured HBM bandwidth of a plain device copy (bytes read + written per second, GB/s)."""
+    import torch
+    nb = mib * (1 << 20) // 8
+    src = torch.ones(nb, dtype=torch.float64, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nb * 8 * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def cpu_baseline_ins(r, gpu_outer_fp64, n_iter=30):
+    """Config 4 on the host cores, timed on the full-size problem: the oracle's C++/OpenMP
+    restatement (oracle/cpu_solver.cpp, FP64) assembles the Oseen operator with the wind of the
+    last timed step (textbook element loop: quadrature per element, atomic scatter) and runs the
+    same FGMRES(50) + Schur preconditioner (velocity Jacobi, Jacobi-PCG on C + B D^-1 B^T to 1e-1)
+    on the FP64 path's last system: its preconditioner setup (S built by rows) alone, then
+    `n_iter` outer iterations. Per step: (assembly + setup) x the Picard iterations of the FP64
+    leg + time per outer iteration x the FP64 leg's outer iterations."""
+    import ctypes as C
+    from oracle import cbuild
+    from oracle.cpu_baseline import OseenTables
+    lib = cbuild.load()
+    cores = len(os.sched_getaffinity(0))
+    lib.ocmp_cpu_threads(cores)
+    m, s = r['fp64_model'], r['fp64_solver']
+    rp, ci = s.a.pattern()
+    rp = np.ascontiguousarray(rp, dtype=np.int32)
+    ci = np.ascontiguousarray(ci, dtype=np.int32)
+    va = np.ascontiguousarray(s.a.values(), dtype=np.float64)
+    b = s.L.vec.detach().cpu().numpy().astype(np.float64)
+    free = np.ascontiguousarray(m.fes.FreeDofs().astype(np.uint8))
+    off = [int(v) for v in m.fes.block_offsets()]
+    pb = m.fes.comp_blocks[1][0]
+    s0, s1 = off[pb], off[pb + 1]
+    W = m.W.vec.detach().cpu().numpy().astype(np.float64)
+    n = b.size
+    T = OseenTables(r['hm'], 2)
+    if T.ndof != n or T.poff != s0:
+        raise RuntimeError('oracle and GPU compound layouts differ')
+    T.assemble(lib, W, r['dt'], r['nu'], rp, ci)            # first touch of the host arrays
+    t0 = time.perf_counter()
+    T.assemble(lib, W, r['dt'], r['nu'], rp, ci)
+    t_asm = time.perf_counter() - t0
+    rel, inner = C.c_double(), C.c_int64()
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    x = np.zeros(n)
+    t0 = time.perf_counter()
+    lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, 0, 1e-1, 200,
+                              C.byref(rel), C.byref(inner))
+    t_setup = time.perf_counter() - t0
+    x[:] = 0.0
+    t0 = time.perf_counter()
+    it = lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, n_iter, 1e-1,
+                                   200, C.byref(rel), C.byref(inner))
+    t_it = (time.perf_counter() - t0 - t_setup) / max(it, 1)
+    picard = float(np.mean([row[1] for row in r['rows64']]))
+    outer = float(np.mean(gpu_outer_fp64))
+    s_asm = (t_asm + t_setup) * picard
+    s_solve = t_it * outer
+    return {'value': round(s_asm + s_solve, 3), 'unit': 's/step', 'cores': cores, 'kind': 'port',
+            'assembly_and_setup_s': round(s_asm, 3), 'solve_s': round(s_solve, 3),
+            'sample': f'oracle/cpu_solver.cpp (C++ -O3 OpenMP, FP64, {cores} threads) on the full-size problem '
+                      f'({n} dofs, {va.size} nnz): Oseen assembly {t_asm:.3f} s and Schur setup {t_setup:.3f} s, '
+                      f'times {picard:.1f} Picard iterations per step; FGMRES(50) + Schur (inner Jacobi-PCG to '
+                      f'1e-1) {it} outer iterations ({inner.value} inner) at {t_it * 1e3:.1f} ms per outer '
+                      f'iteration, times the FP64 GPU leg\'s {outer:.0f} outer iterations per step. The outlet\'s '
+                      f'boundary term and the right-hand side are not in the CPU assembly. Host-CPU '
+                      f'restatement, not NGSolve.'}
+
+
 def bench_learned(args, ctx):
     """Config 4: transient INS (Taylor-Hood P2/P1, implicit Euler, Oseen Picard) on the
     channel [0,4]x[0,1] with 8 seeded holes. A plain FP64 run records `--record`
@@ -230,6 +315,7 @@ def bench_learned(args, ctx):
         return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
                    stress={'u': {'outlet': zero}},
                    linear_solver='GMRes', preconditioner='schur', linear_tolerance=args.tol, restart=args.restart,
+                   linear_max_iterations=args.maxit,
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
                    inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step,
                    schur_iters=args.schur_iters, schur_tol=args.schur_tol, ctx=ctx)
@@ -281,8 +367,12 @@ def bench_learned(args, ctx):
             ctx.timing(True)
             ctx.timing_report()
         t1 = time.perf_counter()
+        per = []
         for k in range(args.steps):
+            tk = time.perf_counter()
             r = s.step()
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - tk)
             rows.append((m.linear_iterations, m.picard_iterations, r.rel_residual, m.picard_log))
             if args.verbose_steps:
                 log(f'  {precision} step {k}: lin iters {m.linear_iterations} (inner {m.precond_iterations}) '
@@ -295,11 +385,11 @@ def bench_learned(args, ctx):
             ctx.timing(False)
         log(f'{precision}{"+learned" if use_net else ""}: {dt_:.4f} s/step, (iters, picard, res) {rows}, '
             f'choices {[c for c, _, _ in s.guess_log]}')
-        return dt_, rows, s.gfu_prev.vec.clone(), s.guess_log, rep
+        return dt_, rows, s.gfu_prev.vec.clone(), s.guess_log, rep, float(np.median(per)), (m, s)
 
-    s64, rows64, u64, _, _ = branch('fp64', False)
-    sml, rowsml, uml, glog, _ = branch('mixed', True)
-    _, _, _, _, rep = branch('mixed', True, timing=True)
+    s64, rows64, u64, _, _, med64, ms64 = branch('fp64', False)
+    sml, rowsml, uml, glog, _, medml, _ = branch('mixed', True)
+    _, _, _, _, rep, _, _ = branch('mixed', True, timing=True)
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
     return dict(desc=(f'transient INS Taylor-Hood P2/P1, implicit Euler dt 1e-3 + Oseen Picard (<=3, tol 1e-6), '
                       f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), do-nothing (stress) outlet '
@@ -308,7 +398,8 @@ def bench_learned(args, ctx):
                        f'C + B diag(A)^-1 B^T (to {args.schur_tol:g}, at most {args.schur_iters} iterations)'
                        if args.schur_iters else
                        f'GMRES({args.restart}) + Schur-diagonal preconditioner')),
-                ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep,
+                ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep, med64=med64,
+                medml=medml, fp64_model=ms64[0], fp64_solver=ms64[1], hm=hm, dt=dt, nu=1e-3, n=n,
                 learned=dict(record_steps=args.record, record_s=round(t_record, 2), train_steps=args.train_steps,
                              train_batch=65536, train_s=round(t_train, 3), loss_first=losses[0],
                              loss_last=losses[-1], choices=[c for c, _, _ in glog],
@@ -439,7 +530,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--config', type=int, default=2, choices=(1, 2, 3, 4, 5))
+    ap.add_argument('--config', type=int, default=4, choices=(1, 2, 3, 4, 5))
+    ap.add_argument('--no-extra', action='store_true', help='config 4: skip the config-2 extra keys')
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--verbose-steps', action='store_true', help='config 4: log every step of the timed legs')
@@ -519,33 +611,65 @@ def main():
         dom = max((k for k in rep if rep[k][2] > 0), key=lambda k: rep[k][1])
         cnt, ms, byt = rep[dom]
         achieved = byt / (ms * 1e-3) / 1e9
+        copy_gbs = copy_bandwidth()
+        it64 = [row[0] for row in r['rows64']]
+        itml = [row[0] for row in r['rowsml']]
         out = {
             'metric': METRIC, 'value': round(r['sml'], 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
             'warmup': args.warmup, 'ms_per_step': round(r['sml'] * 1e3, 3), 'higher_is_better': False,
             'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32+bf16', 'data': 'synthetic',
             'config': {'workload': f"{r['desc']} to rel residual {args.tol:g}", 'config_index': 4,
                        'ndof': r['ndof'], 'tol': args.tol},
-            'plain_fp64': {'s_per_step': round(r['s64'], 5), 'iters_picard_res': r['rows64']},
-            'mixed_learned': {'s_per_step': round(r['sml'], 5), 'iters_picard_res': r['rowsml'],
+            'plain_fp64': {'s_per_step': round(r['s64'], 5), 'median_s_per_step': round(r['med64'], 5),
+                           'algorithm': 'FP64 FGMRES(50) + Schur (FP64 inner PCG): the same algorithm as the '
+                                        'mixed leg, the best FP64 path of this build',
+                           'outer_iters_per_step': it64, 'iters_picard_res': r['rows64']},
+            'mixed_learned': {'s_per_step': round(r['sml'], 5), 'median_s_per_step': round(r['medml'], 5),
+                              'outer_iters_per_step': itml, 'iters_picard_res': r['rowsml'],
                               'rel_error_vs_fp64': r['err']},
             'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
             'dofs_per_s': round(r['ndof'] / r['sml'], 1),
             'learned': r['learned'],
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
                          'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
-                         'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)},
+                         'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2),
+                         'measured_copy_GBps': round(copy_gbs, 1),
+                         'frac_of_measured_copy': round(achieved / copy_gbs, 4)},
             'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
                                       for k, (c, m_, b_) in rep.items()},
         }
+        if not args.no_cpu:
+            log('cpu baseline (config 4) ...')
+            cb = cpu_baseline_ins(r, it64)
+            cb['speedup_gpu_mixed_learned'] = round(cb['value'] / r['sml'], 1)
+            cb['speedup_gpu_fp64'] = round(cb['value'] / r['s64'], 1)
+            out['cpu_baseline'] = cb
+        del r
+        if not args.no_extra:
+            log('config 2 (extra keys) ...')
+            c2 = bench_plain(args, ctx, 2, 512, 2, 1, False)
+            out['config2'] = {k: c2[k] for k in ('config', 'plain_fp64', 'mixed', 'fp64_s_step', 'roofline')}
+            out['config2']['speedup_mixed_vs_fp64_standard'] = c2['speedup_mixed_vs_fp64']
+            if c2.get('fp64_s_step'):
+                out['config2']['speedup_mixed_vs_fp64_s_step'] = round(
+                    c2['fp64_s_step']['s_per_step'] / c2['mixed']['s_per_step'], 3)
         print(json.dumps(out), flush=True)
         return
+    print(json.dumps(bench_plain(args, ctx, args.config, n, args.steps, args.warmup, not args.no_cpu)), flush=True)
+
+
+def bench_plain(args, ctx, config, n, steps, warmup, with_cpu):
+    """Configs 1-3 (and config 2 as the headline line's extra keys): the plain FP64 path, the
+    mixed path, the FP64 s-step path where it applies, per-kernel timings and the CPU baseline."""
+    import torch
+    from opencmp_amd import core as C
     t0 = time.perf_counter()
-    hm, gm, gs, a, f, solver, desc = build_problem(args.config, n, ctx)
+    hm, gm, gs, a, f, solver, desc = build_problem(config, n, ctx)
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
     kw = dict(solver=solver, inner_tol=args.inner_tol, maxit=args.maxit)
     ss = args.s_step if solver == 'GMRes' else 0
-    s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, args.steps, args.warmup, **kw)
-    smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, args.steps, args.warmup,
+    s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, steps, warmup, **kw)
+    smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, steps, warmup,
                                       s_step=ss, **kw)
     # the same s-step algorithm in FP64, for reference (not the plain path)
     s64s = None
@@ -569,16 +693,16 @@ def main():
     # HBM bytes per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to the algorithmic bytes,
     # measured with rocprofv3 --pmc on this workload, times this run's algorithmic bytes per launch
     tf = os.path.join(ROOT, 'profiles', 'hbm_traffic_dg_p2_512.json')
-    if args.config == 2 and n == 512 and os.path.exists(tf):
+    if config == 2 and n == 512 and os.path.exists(tf):
         ratios = json.load(open(tf))['hbm_bytes_over_algorithmic']
         if dom in ratios:
             roof['traffic'] = round(ratios[dom] * byt / cnt)
             roof['traffic_source'] = 'profiles/dg_p2_512_gmres_hbm_traffic.md'
     out = {
-        'metric': METRIC, 'value': round(smx, 5), 'unit': 's/step', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': round(smx * 1e3, 3), 'higher_is_better': False, 'scaling': 'weak',
+        'metric': METRIC, 'value': round(smx, 5), 'unit': 's/step', 'n_gpus': 1, 'steps': steps,
+        'warmup': warmup, 'ms_per_step': round(smx * 1e3, 3), 'higher_is_better': False, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f64+f32', 'data': 'synthetic',
-        'config': {'workload': f'{desc} to rel residual {args.tol:g}', 'config_index': args.config,
+        'config': {'workload': f'{desc} to rel residual {args.tol:g}', 'config_index': config,
                    'ndof': gs.ndof, 'nnz': a.nnz, 'tol': args.tol},
         'plain_fp64': {'s_per_step': round(s64, 5), 'iters': it64[-1], 'rel_residual': r64.rel_residual},
         'mixed': {'s_per_step': round(smx, 5), 'iters': itmx[-1], 'outer': rmx.outer, 's_step': ss,
@@ -591,13 +715,15 @@ def main():
         'kernels_mixed': table(rep),
         'kernels_fp64': table(rep_fp),
     }
-    if not args.no_cpu:
+    if with_cpu:
         log('cpu baseline ...')
-        cb = cpu_baseline(args.config, a, f, gs, it64[-1], n)
+        cb = cpu_baseline(config, a, f, gs, it64[-1], n)
         cb['speedup_gpu_mixed'] = round(cb['value'] / smx, 1)
         cb['speedup_gpu_fp64'] = round(cb['value'] / s64, 1)
         out['cpu_baseline'] = cb
-    print(json.dumps(out), flush=True)
+    return out
+
+
 
 
 if __name__ == '__main__':

@@ -1,5 +1,6 @@
-"""Builds the oracle's C++ restatement of the Krylov solves (oracle/cpu_solver.cpp) into
-oracle/libocmp_cpu.so -- test infrastructure for the CPU baseline, never the product."""
+"""Builds the oracle's C++ restatement of the assembly and the Krylov solves
+(oracle/cpu_solver.cpp) into oracle/libocmp_cpu.so -- test infrastructure for the CPU
+baseline, never the product."""
 import os
 import subprocess
 import sys
@@ -31,6 +32,13 @@ def load():
     lib.ocmp_cpu_gmres_bj.restype = C.c_int
     lib.ocmp_cpu_pcg_jacobi.argtypes = [C.c_int64, P, P, P, P, P, P, C.c_double, C.c_int, C.POINTER(C.c_double)]
     lib.ocmp_cpu_pcg_jacobi.restype = C.c_int
+    I64, I32, D = C.c_int64, C.c_int, C.c_double
+    lib.ocmp_cpu_assemble_oseen.argtypes = [I32, I64, P, I32, P, I32, P, I64, I64, I32, P, P, P, P, P, P, D, D, D,
+                                            P, P, P]
+    lib.ocmp_cpu_assemble_oseen.restype = C.c_int
+    lib.ocmp_cpu_fgmres_schur.argtypes = [I64, P, P, P, P, I64, I64, P, P, I32, D, I32, D, I32,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.ocmp_cpu_fgmres_schur.restype = C.c_int
     return lib
 
 

@@ -1,13 +1,18 @@
-// Host-CPU restatement of the Krylov solves, for the CPU baseline -- TEST INFRASTRUCTURE.
+// Host-CPU restatement of the assembly and the Krylov solves, for the CPU baseline -- TEST
+// INFRASTRUCTURE.
 //
 // Only bench.py's CPU-baseline leg (and the tests) load this library. It runs the
 // same algorithms as the GPU path on the same CSR matrix, in FP64 with OpenMP over
 // all host cores: GMRES(m) right-preconditioned with element-block Jacobi (the DG
-// workload, opencmp/models/base_model.py:886-947 with Preconditioner 'local'), and
-// Jacobi-PCG (the H1 workloads). Classical Gram-Schmidt, Givens rotations, the
-// FP64 true residual ||b - Ax|| / ||b|| on the free dofs as the stopping test.
+// workload, opencmp/models/base_model.py:886-947 with Preconditioner 'local'),
+// Jacobi-PCG (the H1 workloads), and flexible GMRES with the Schur preconditioner
+// (velocity Jacobi, Jacobi-PCG on C + B D^-1 B^T for the pressure) for the saddle
+// points. Classical Gram-Schmidt, Givens rotations, the FP64 true residual
+// ||b - Ax|| / ||b|| on the free dofs as the stopping test. The Oseen assembly is the
+// textbook element loop: quadrature per element, then a scatter with atomic adds.
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -208,6 +213,310 @@ int ocmp_cpu_pcg_jacobi(int64_t n, const int32_t *rp, const int32_t *col, const 
     ++it;
   }
   *rel = bn > 0 ? rn / bn : rn;
+  return it;
+}
+
+// Implicit-Euler Oseen operator of a Taylor-Hood pair on simplices (opencmp/models/ins.py:
+// 174-216 without the outlet's boundary term, with the time derivative of
+// opencmp/solvers/time_integration_schemes.py:135-170): per velocity component the block
+//   M + dt nu K - dt C(w),  C(w)_ij = int phi_j (w . grad phi_i),
+// and the couplings -dt int psi_k d_c phi_j (pressure rows) and -dt int d_c phi_i psi_k
+// (velocity rows), plus -peps dt int psi_k psi_l. Reference basis values and their
+// barycentric derivatives come tabulated at the quadrature points; the wind is the
+// velocity part of W. Element matrices are added to the CSR values with atomic adds.
+int ocmp_cpu_assemble_oseen(int dim, int64_t nel, const double *X, int nlu, const int32_t *udofs, int nlp,
+                            const int32_t *pdofs, int64_t nu1, int64_t poff, int nq, const double *qw,
+                            const double *phiu, const double *dlu, const double *phip, const double *dlp,
+                            const double *W, double dt, double nu, double peps, const int32_t *rp,
+                            const int32_t *col, double *val) {
+  const int nv = dim + 1;
+  const int nloc = dim * nlu + nlp;
+  int bad = 0;
+#pragma omp parallel
+  {
+    std::vector<double> gu((size_t)nq * nlu * 3), gp((size_t)nq * nlp * 3), K((size_t)nloc * nloc);
+    std::vector<int64_t> g(nloc);
+#pragma omp for schedule(static) reduction(+ : bad)
+    for (int64_t e = 0; e < nel; ++e) {
+      const double *xe = X + e * nv * dim;
+      // J columns x_k - x_0; grad lambda_k (k >= 1) = rows of J^{-1}; grad lambda_0 = -sum
+      double J[3][3] = {{0}}, Ji[3][3] = {{0}}, det;
+      for (int a = 0; a < dim; ++a)
+        for (int k = 0; k < dim; ++k) J[a][k] = xe[(k + 1) * dim + a] - xe[a];
+      if (dim == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        Ji[0][0] = J[1][1] / det;
+        Ji[0][1] = -J[0][1] / det;
+        Ji[1][0] = -J[1][0] / det;
+        Ji[1][1] = J[0][0] / det;
+      } else {
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const int a1 = (b + 1) % 3, a2 = (b + 2) % 3, b1 = (a + 1) % 3, b2 = (a + 2) % 3;
+            Ji[a][b] = (J[a1][b1] * J[a2][b2] - J[a1][b2] * J[a2][b1]) / det;
+          }
+      }
+      double gl[4][3] = {{0}};
+      for (int k = 1; k <= dim; ++k)
+        for (int a = 0; a < dim; ++a) gl[k][a] = Ji[k - 1][a];
+      for (int a = 0; a < dim; ++a) {
+        double s = 0.0;
+        for (int k = 1; k <= dim; ++k) s -= gl[k][a];
+        gl[0][a] = s;
+      }
+      const double vol = std::fabs(det);
+      for (int q = 0; q < nq; ++q) {
+        for (int i = 0; i < nlu; ++i)
+          for (int a = 0; a < dim; ++a) {
+            double s = 0.0;
+            for (int k = 0; k < nv; ++k) s += dlu[((size_t)q * nlu + i) * nv + k] * gl[k][a];
+            gu[((size_t)q * nlu + i) * 3 + a] = s;
+          }
+        for (int i = 0; i < nlp; ++i)
+          for (int a = 0; a < dim; ++a) {
+            double s = 0.0;
+            for (int k = 0; k < nv; ++k) s += dlp[((size_t)q * nlp + i) * nv + k] * gl[k][a];
+            gp[((size_t)q * nlp + i) * 3 + a] = s;
+          }
+      }
+      std::fill(K.begin(), K.end(), 0.0);
+      for (int q = 0; q < nq; ++q) {
+        const double wq = qw[q] * vol;
+        const double *pu = phiu + (size_t)q * nlu, *pp = phip + (size_t)q * nlp;
+        double w[3] = {0, 0, 0};
+        for (int c = 0; c < dim; ++c)
+          for (int j = 0; j < nlu; ++j) w[c] += W[c * nu1 + udofs[e * nlu + j]] * pu[j];
+        for (int i = 0; i < nlu; ++i) {
+          const double *gi = &gu[((size_t)q * nlu + i) * 3];
+          double wgi = 0.0;
+          for (int a = 0; a < dim; ++a) wgi += w[a] * gi[a];
+          for (int j = 0; j < nlu; ++j) {
+            const double *gj = &gu[((size_t)q * nlu + j) * 3];
+            double gg = 0.0;
+            for (int a = 0; a < dim; ++a) gg += gi[a] * gj[a];
+            const double v = wq * (pu[i] * pu[j] + dt * nu * gg - dt * pu[j] * wgi);
+            for (int c = 0; c < dim; ++c) K[(size_t)(c * nlu + i) * nloc + c * nlu + j] += v;
+          }
+          for (int k = 0; k < nlp; ++k)
+            for (int c = 0; c < dim; ++c) {
+              const double v = -dt * wq * gi[c] * pp[k];
+              K[(size_t)(c * nlu + i) * nloc + dim * nlu + k] += v;  // velocity row, pressure column
+              K[(size_t)(dim * nlu + k) * nloc + c * nlu + i] += v;  // pressure row, velocity column
+            }
+        }
+        for (int k = 0; k < nlp; ++k)
+          for (int l = 0; l < nlp; ++l) K[(size_t)(dim * nlu + k) * nloc + dim * nlu + l] -= peps * dt * wq * pp[k] * pp[l];
+      }
+      for (int c = 0; c < dim; ++c)
+        for (int i = 0; i < nlu; ++i) g[c * nlu + i] = c * nu1 + udofs[e * nlu + i];
+      for (int k = 0; k < nlp; ++k) g[dim * nlu + k] = poff + pdofs[e * nlp + k];
+      for (int i = 0; i < nloc; ++i) {
+        const int32_t *c0 = col + rp[g[i]], *c1 = col + rp[g[i] + 1];
+        for (int j = 0; j < nloc; ++j) {
+          const int32_t *at = std::lower_bound(c0, c1, (int32_t)g[j]);
+          if (at == c1 || *at != g[j]) {
+            ++bad;
+            continue;
+          }
+#pragma omp atomic
+          val[at - col] += K[(size_t)i * nloc + j];
+        }
+      }
+    }
+  }
+  return bad;
+}
+
+// Flexible GMRES(m) with the Schur preconditioner of the GPU path: on the rows outside [s0, s1)
+// z = v / a_ii; on [s0, s1) z = -S^{-1} v by Jacobi-PCG from zero until (r, Mr) <= stol^2 (r0, Mr0)
+// or smax iterations, S = C + B D^{-1} B^T with D_uu = max(|a_uu|, 1e-2 max_j |a_uj|) (columns
+// outside [s0, s1)) over the free velocity rows, and S_ii below 1e-10 max S_ii left out. Returns the
+// outer iterations; *inner gets the inner PCG iterations summed.
+int ocmp_cpu_fgmres_schur(int64_t n, const int32_t *rp, const int32_t *col, const double *val, const uint8_t *freem,
+                          int64_t s0, int64_t s1, const double *b, double *x, int m, double tol, int maxit,
+                          double stol, int smax, double *rel, int64_t *inner) {
+  const int64_t ns = s1 - s0;
+  std::vector<double> dinv(n, 0.0), dpos(n, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    if (!freem[r] || (r >= s0 && r < s1)) continue;
+    double d = 0.0, mx = 0.0;
+    for (int32_t j = rp[r]; j < rp[r + 1]; ++j) {
+      if (col[j] >= s0 && col[j] < s1) continue;
+      const double a = std::fabs(val[j]);
+      if (col[j] == r) d = val[j];
+      mx = std::max(mx, a);
+    }
+    dinv[r] = d != 0.0 ? 1.0 / d : 0.0;
+    const double dd = std::max(std::fabs(d), 1e-2 * mx);
+    dpos[r] = dd > 0.0 ? 1.0 / dd : 0.0;
+  }
+  // S by rows: a dense accumulator per thread over the ns pressure columns
+  std::vector<std::vector<int32_t>> scol(ns);
+  std::vector<std::vector<double>> sval(ns);
+#pragma omp parallel
+  {
+    std::vector<double> acc(ns, 0.0);
+    std::vector<char> on(ns, 0);
+    std::vector<int32_t> touched;
+#pragma omp for schedule(dynamic, 512)
+    for (int64_t i = 0; i < ns; ++i) {
+      const int64_t p = s0 + i;
+      touched.clear();
+      auto add = [&](int64_t q, double v) {
+        if (!on[q]) {
+          on[q] = 1;
+          touched.push_back((int32_t)q);
+        }
+        acc[q] += v;
+      };
+      for (int32_t j = rp[p]; j < rp[p + 1]; ++j) {
+        const int64_t u = col[j];
+        if (u >= s0 && u < s1) {
+          add(u - s0, -val[j]);
+          continue;
+        }
+        const double f = val[j] * dpos[u];
+        if (f == 0.0) continue;
+        for (int32_t t = rp[u]; t < rp[u + 1]; ++t)
+          if (col[t] >= s0 && col[t] < s1) add(col[t] - s0, f * val[t]);
+      }
+      std::sort(touched.begin(), touched.end());
+      scol[i] = touched;
+      sval[i].resize(touched.size());
+      for (size_t k = 0; k < touched.size(); ++k) {
+        sval[i][k] = acc[touched[k]];
+        acc[touched[k]] = 0.0;
+        on[touched[k]] = 0;
+      }
+    }
+  }
+  std::vector<double> sd(ns, 0.0);
+  double smx = 0.0;
+  for (int64_t i = 0; i < ns; ++i) {
+    for (size_t k = 0; k < scol[i].size(); ++k)
+      if (scol[i][k] == i) sd[i] = sval[i][k];
+    if (!freem[s0 + i] || !(sd[i] > 0.0)) sd[i] = 0.0;
+    smx = std::max(smx, sd[i]);
+  }
+  for (int64_t i = 0; i < ns; ++i) sd[i] = sd[i] > 1e-10 * smx ? 1.0 / sd[i] : 0.0;
+  std::vector<double> pr(ns), pu(ns), pp(ns), pq(ns);
+  int64_t inner_total = 0;
+  // z[s0:s1] = -S^{-1} v[s0:s1]
+  auto schur_solve = [&](const double *v, double *z) {
+    double *xz = z + s0;
+    double g = 0.0;
+#pragma omp parallel for reduction(+ : g) schedule(static)
+    for (int64_t i = 0; i < ns; ++i) {
+      xz[i] = 0.0;
+      pr[i] = -v[s0 + i];
+      pu[i] = sd[i] * pr[i];
+      pp[i] = pu[i];
+      g += pr[i] * pu[i];
+    }
+    const double g0 = g;
+    for (int it = 0; it < smax && g > stol * stol * g0; ++it) {
+      double d = 0.0;
+#pragma omp parallel for reduction(+ : d) schedule(static)
+      for (int64_t i = 0; i < ns; ++i) {
+        double s = 0.0;
+        for (size_t k = 0; k < scol[i].size(); ++k) s += sval[i][k] * pp[scol[i][k]];
+        pq[i] = s;
+        d += pp[i] * s;
+      }
+      const double a = d != 0.0 ? g / d : 0.0;
+      double g2 = 0.0;
+#pragma omp parallel for reduction(+ : g2) schedule(static)
+      for (int64_t i = 0; i < ns; ++i) {
+        xz[i] += a * pp[i];
+        pr[i] -= a * pq[i];
+        pu[i] = sd[i] * pr[i];
+        g2 += pr[i] * pu[i];
+      }
+      const double be = g != 0.0 ? g2 / g : 0.0;
+      g = g2;
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < ns; ++i) pp[i] = pu[i] + be * pp[i];
+      ++inner_total;
+    }
+  };
+  auto masked_spmv = [&](const double *xv, double *y) {
+    spmv(n, rp, col, val, xv, y);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i)
+      if (!freem[i]) y[i] = 0.0;
+  };
+  std::vector<double> V((size_t)(m + 1) * n), Z((size_t)m * n), w(n), r(n);
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  double bn2 = 0.0;
+#pragma omp parallel for reduction(+ : bn2) schedule(static)
+  for (int64_t i = 0; i < n; ++i) bn2 += freem[i] ? b[i] * b[i] : 0.0;
+  const double bn = std::sqrt(bn2);
+  int it = 0;
+  double rn = 0.0;
+  for (;;) {
+    spmv(n, rp, col, val, x, r.data());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) r[i] = freem[i] ? b[i] - r[i] : 0.0;
+    rn = std::sqrt(dot(n, r.data(), r.data()));
+    if (rn <= tol * bn || it >= maxit) break;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) V[i] = r[i] / rn;
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = rn;
+    int k = 0;
+    for (; k < m && it < maxit; ++k, ++it) {
+      const double *vk = &V[(size_t)k * n];
+      double *zk = &Z[(size_t)k * n];
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) zk[i] = dinv[i] * vk[i];
+      schur_solve(vk, zk);
+      masked_spmv(zk, w.data());
+      for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = dot(n, &V[(size_t)j * n], w.data());
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) {
+        double s = w[i];
+        for (int j = 0; j <= k; ++j) s -= H[(size_t)j * m + k] * V[(size_t)j * n + i];
+        w[i] = s;
+      }
+      const double hn = std::sqrt(dot(n, w.data(), w.data()));
+      H[(size_t)(k + 1) * m + k] = hn;
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < n; ++i) V[(size_t)(k + 1) * n + i] = hn > 0.0 ? w[i] / hn : 0.0;
+      for (int j = 0; j < k; ++j) {
+        const double a = H[(size_t)j * m + k], c = H[(size_t)(j + 1) * m + k];
+        H[(size_t)j * m + k] = cs[j] * a + sn[j] * c;
+        H[(size_t)(j + 1) * m + k] = -sn[j] * a + cs[j] * c;
+      }
+      const double a = H[(size_t)k * m + k], c = H[(size_t)(k + 1) * m + k];
+      const double d = std::hypot(a, c);
+      cs[k] = a / d;
+      sn[k] = c / d;
+      H[(size_t)k * m + k] = d;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      if (std::fabs(g[k + 1]) <= tol * bn) {
+        ++k;
+        ++it;
+        break;
+      }
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
+      y[i] = s / H[(size_t)i * m + i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += y[j] * Z[(size_t)j * n + i];
+      x[i] += s;
+    }
+  }
+  *rel = bn > 0 ? rn / bn : rn;
+  *inner = inner_total;
   return it;
 }
 

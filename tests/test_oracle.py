@@ -506,3 +506,57 @@ def test_oseen_picard_converges_with_a_body_force():
     x, errs = fem.oseen_picard(cs, 0.1, x0, b, 1e-10, 1e-10, 40, stress_mask=m.marker_mask(['right']))
     assert len(errs) >= 4 and errs[-1] < 1e-9 * (1 + np.abs(x).max())
     assert all(e1 < e0 for e0, e1 in zip(errs[1:], errs[2:]))
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_cpu_oseen_assembly_matches_oracle(dim):
+    """The C++ element loop of the CPU baseline (implicit-Euler Oseen operator, Taylor-Hood P2/P1,
+    wind from a P2 field) equals the oracle's assemble_volume with the same terms to round-off,
+    and its FGMRES with the Schur preconditioner solves the system like a direct solve."""
+    import ctypes as C
+    from oracle import cbuild
+    from oracle.cpu_baseline import OseenTables
+    if not os.path.exists(cbuild.LIB):
+        cbuild.build()
+    lib = cbuild.load()
+    lib.ocmp_cpu_threads(2)
+    m = unit_square(6) if dim == 2 else unit_cube(2)
+    T = OseenTables(m, 2)
+    rng = np.random.default_rng(3)
+    W = rng.standard_normal(T.ndof)
+    dt, nu = 0.05, 0.1
+    A_ = fem.Assembler(T.cs)
+    wind = fem.GridFunctionEval(T.su, W[:T.su.ndof])
+    fem.assemble_volume(A_, [dict(kind='mass', trial=0, test=0, coef=1.0),
+                             dict(kind='grad_grad', trial=0, test=0, coef=dt * nu),
+                             dict(kind='conv', trial=0, test=0, coef=-dt, wind=wind),
+                             dict(kind='dtrial', trial=0, test=1, coef=-dt, dir=-1),
+                             dict(kind='dtest', trial=1, test=0, coef=-dt, dir=-1),
+                             dict(kind='mass', trial=1, test=1, coef=-1e-10 * dt)])
+    A = A_.matrix().tocsr()
+    A.sort_indices()
+    val = T.assemble(lib, W, dt, nu, A.indptr, A.indices)
+    assert np.abs(val - A.data).max() <= 1e-13 * np.abs(A.data).max()
+    # a plausible slip (the convection sign) is caught
+    bad = T.assemble(lib, W, -dt, -nu, A.indptr, A.indices)
+    assert np.abs(bad - A.data).max() > 1e-3 * np.abs(A.data).max()
+    # FGMRES + Schur on the free dofs of the unit-box problem with u = 0 on the whole boundary
+    mask = m.marker_mask(m.bnd_names)
+    su = fem.Space(T.topo, 'H1', 2, ncomp=dim, dirichlet_mask=mask)
+    free = np.concatenate([~su.dirichlet, np.ones(T.sp.ndof, bool)])
+    free[T.poff] = False                     # pin one pressure (the -1e-10 mass alone leaves it near-singular)
+    b = np.where(free, rng.standard_normal(T.ndof), 0.0)
+    x = np.zeros(T.ndof)
+    rel, inner = C.c_double(), C.c_int64()
+    fr = free.astype(np.uint8)
+    va = np.ascontiguousarray(A.data)
+    rp = A.indptr.astype(np.int32)
+    ci = A.indices.astype(np.int32)
+    it = lib.ocmp_cpu_fgmres_schur(T.ndof, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, fr.ctypes.data,
+                                   T.poff, T.ndof, b.ctypes.data, x.ctypes.data, 30, 1e-10, 5000, 1e-1, 200,
+                                   C.byref(rel), C.byref(inner))
+    assert rel.value <= 1e-10 and 0 < it < 5000 and inner.value >= it
+    xo = np.zeros(T.ndof)
+    Af = A[free][:, free].tocsc()
+    xo[free] = spl_solve(Af, b[free])
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
