@@ -388,6 +388,7 @@ def bench_learned(args, ctx):
         return dt_, rows, s.gfu_prev.vec.clone(), s.guess_log, rep, float(np.median(per)), (m, s)
 
     s64, rows64, u64, _, _, med64, ms64 = branch('fp64', False)
+    smp, rowsmp, _, _, _, medmp, _ = branch('mixed', False)
     sml, rowsml, uml, glog, _, medml, _ = branch('mixed', True)
     _, _, _, _, rep, _, _ = branch('mixed', True, timing=True)
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
@@ -400,6 +401,7 @@ def bench_learned(args, ctx):
                        f'GMRES({args.restart}) + Schur-diagonal preconditioner')),
                 ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep, med64=med64,
                 medml=medml, fp64_model=ms64[0], fp64_solver=ms64[1], hm=hm, dt=dt, nu=1e-3, n=n,
+                smp=smp, medmp=medmp, rowsmp=rowsmp,
                 learned=dict(record_steps=args.record, record_s=round(t_record, 2), train_steps=args.train_steps,
                              train_batch=65536, train_s=round(t_train, 3), loss_first=losses[0],
                              loss_last=losses[-1], choices=[c for c, _, _ in glog],
@@ -437,6 +439,14 @@ def bench_unet(args, ctx):
     f = C.LinearForm(m64.fes, m64.construct_linear(1.0))
     pre64 = m64.construct_preconditioners(a)
     pre32 = C.Preconditioner(a, 'schur', precision='mixed', schur_blocks=m64.schur_blocks())
+    if args.vel_iters:
+        # block-triangular step: lumped phi-weighted pressure mass for the Schur complement, inner
+        # Jacobi-PCG on the velocity block (BF16-stored in the mixed leg with --bf16)
+        d = m64.schur_diagonal()
+        for p_ in (pre64, pre32):
+            p_.set_block_triangular(d, args.vel_iters, args.vel_tol)
+        if args.bf16:
+            pre32.set_bf16(True)
     ndof = m64.fes.ndof
     log(f'config 5 setup {time.perf_counter() - t0:.1f}s: {hm.nel} tets, ndof {ndof}, nnz {a.nnz}')
     x = C.GridFunction(m64.fes)
@@ -517,8 +527,12 @@ def bench_unet(args, ctx):
     U.detach(pre32)
     smx, rmx, umx, _ = timed(pre32, 'mixed', 'mixed (no correction)')
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
+    pdesc = (f'block-triangular preconditioner (lumped phi-weighted pressure mass, inner Jacobi-PCG on the '
+             f'velocity block to {args.vel_tol:g}, at most {args.vel_iters} iterations'
+             f'{", BF16-stored in the mixed leg" if args.bf16 else ""})' if args.vel_iters else
+             'Schur-diagonal preconditioner')
     return dict(desc=(f'diffuse-interface Stokes, unit cube {n}^3 x 6 tets ({hm.nel} tets), Taylor-Hood P2/P1, '
-                      f'20 spheres, penalty 1e4, FGMRES({args.restart}) + Schur-diagonal preconditioner'),
+                      f'20 spheres, penalty 1e4, FGMRES({args.restart}) + {pdesc}'),
                 ndof=ndof, nnz=a.nnz, s64=s64, sml=sml, smx=smx, r64=r64, rml=rml, rmx=rmx, err=err, rep=rep,
                 learned=dict(samples=args.unet_samples, sample_s=round(t_samples, 2), train_steps=args.train_steps,
                              train_s=round(t_train, 2), loss_first=losses[0], loss_last10=float(np.mean(losses[-10:])),
@@ -532,6 +546,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', type=int, default=4, choices=(1, 2, 3, 4, 5))
     ap.add_argument('--no-extra', action='store_true', help='config 4: skip the config-2 extra keys')
+    ap.add_argument('--vel-iters', type=int, default=0,
+                    help='config 5: block-triangular preconditioner with at most this many velocity PCG iterations')
+    ap.add_argument('--vel-tol', type=float, default=1e-2, help='config 5: velocity PCG reduction per step')
+    ap.add_argument('--bf16', action='store_true', help='config 5: BF16-stored velocity block in the mixed leg')
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--verbose-steps', action='store_true', help='config 4: log every step of the timed legs')
@@ -562,7 +580,7 @@ def main():
         sys.exit(2)
     n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 32}[args.config]
     if args.restart is None:
-        args.restart = 200 if args.config == 5 else 50
+        args.restart = 200 if (args.config == 5 and not args.vel_iters) else 50
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
@@ -594,7 +612,14 @@ def main():
                               'fallback': r['rml'].fallback, 'rel_residual': r['rml'].rel_residual,
                               'rel_error_vs_fp64': r['err']},
             'mixed_plain': {'s_per_step': round(r['smx'], 5), 'iters': r['rmx'].iters},
+            'mixed_plain': {'s_per_step': round(r['smp'], 5), 'median_s_per_step': round(r['medmp'], 5),
+                            'outer_iters_per_step': [row[0] for row in r['rowsmp']],
+                            'first_picard_outer_iters': [row[3][0][0] for row in r['rowsmp']]},
             'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
+            'learned_gain': {'s_per_step_mixed_plain_over_learned': round(r['smp'] / r['sml'], 3),
+                             'outer_iters_plain': sum(row[0] for row in r['rowsmp']),
+                             'outer_iters_learned': sum(row[0] for row in r['rowsml']),
+                             'first_picard_outer_iters_learned': [row[3][0][0] for row in r['rowsml']]},
             'dofs_per_s': round(r['ndof'] / r['sml'], 1),
             'learned': r['learned'],
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
@@ -638,6 +663,14 @@ def main():
             'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
                                       for k, (c, m_, b_) in rep.items()},
         }
+        # HBM bytes per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to the algorithmic bytes,
+        # measured with rocprofv3 --pmc on this workload, times this run's algorithmic bytes per launch
+        tf = os.path.join(ROOT, 'profiles', 'hbm_traffic_ins_channel_n228.json')
+        if (args.n or 228) == 228 and os.path.exists(tf):
+            ratios = json.load(open(tf))['hbm_bytes_over_algorithmic']
+            if dom in ratios:
+                out['roofline']['traffic'] = round(ratios[dom] * byt / cnt)
+                out['roofline']['traffic_source'] = 'profiles/ins_channel_n228_fgmres_schur_hbm_traffic.md'
         if not args.no_cpu:
             log('cpu baseline (config 4) ...')
             cb = cpu_baseline_ins(r, it64)

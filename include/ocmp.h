@@ -345,12 +345,42 @@ int ocmp_precond_create_schur(ocmp_matrix *A, int precision, uint64_t schur_bloc
  * nonlinear in its input, so the other solvers refuse such a preconditioner
  * (OCMP_ERR_ARG). */
 int ocmp_precond_set_schur_solve(ocmp_precond *P, int max_iters, double tol);
+/* Block-triangular step for a preconditioner from ocmp_precond_create_schur
+ * (another replacement of the reference's direct factorisation of the saddle-point
+ * system, opencmp/models/stokes_dim.py:28-153 with
+ * opencmp/models/base_model.py:365-383 and :886-947), for systems whose velocity
+ * block is not diagonally dominated (steady Stokes, diffuse-interface penalties).
+ * The Schur blocks must be the last dofs [s0, n). The flexible step of
+ * OCMP_SOLVER_FGMRES becomes z_p = -v_p / d_p on the Schur rows (d: schur_diag,
+ * a device array of n - s0 positive entries approximating the Schur complement,
+ * e.g. the lumped pressure mass over the viscosity, copied at this call; rows
+ * where d is not above 1e-10 max d, or not free, get 0), then
+ * z_u = A_uu^{-1} (v_u - A_up z_p) by at most vel_iters Jacobi-PCG iterations on
+ * the velocity block A_uu, fewer once (r, diag(A_uu)^{-1} r) has fallen by
+ * vel_tol^2. The velocity blocks' pattern is built at the next
+ * ocmp_precond_update, their values at every update. Replaces the Schur inner
+ * solve (turn that off first); vel_iters = 0 turns it off. */
+int ocmp_precond_set_block_triangular(ocmp_precond *P, const double *schur_diag, int vel_iters, double vel_tol);
+/* BF16 storage of the inner solves' matrices (the Schur matrix S of the Schur
+ * inner solve, the velocity block of the block-triangular step) for a
+ * preconditioner built with OCMP_MIXED: the inner Jacobi-PCG of the FP32 FGMRES
+ * cycles reads BF16 values (rounded to nearest even from FP64, widened exactly
+ * to FP32; FP32 vectors, FP64 sums). This realises the BF16-stored
+ * preconditioner copy inside iterative refinement: the outer FP64 residual and
+ * the FGMRES operator keep FP32/FP64 values, and FGMRES admits a preconditioner
+ * that differs from the exact inner solve. No reference counterpart (NGSolve
+ * runs in FP64). on = 0 turns it off; the copies are made at the next update. */
+int ocmp_precond_set_bf16(ocmp_precond *P, int on);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);
 int ocmp_precond_destroy(ocmp_precond *P);
 
-enum { OCMP_SOLVER_CG = 0, OCMP_SOLVER_GMRES = 1, OCMP_SOLVER_RICHARDSON = 2, OCMP_SOLVER_FGMRES = 3 };
+/* OCMP_SOLVER_MINRES replaces ngs.solvers.MinRes (opencmp/models/base_model.py:929-932):
+ * preconditioned MINRES for symmetric A, FP64 only, with |D|^{-1} of a point-Jacobi or
+ * Schur-diagonal preconditioner (symmetric positive definite, as MINRES needs) or none. */
+enum { OCMP_SOLVER_CG = 0, OCMP_SOLVER_GMRES = 1, OCMP_SOLVER_RICHARDSON = 2, OCMP_SOLVER_FGMRES = 3,
+       OCMP_SOLVER_MINRES = 4 };
 
 typedef struct {
   int32_t solver;      /* OCMP_SOLVER_*                                          */

@@ -21,7 +21,7 @@ LFI_SOURCE, LFI_DG_DIRICHLET, LFI_BND_SOURCE = 0, 1, 2
 FN_CONST, FN_SINPROD, FN_POLY2, FN_FIELD, FN_EXPR = 0, 1, 2, 3, 4
 PRE_NONE, PRE_JACOBI, PRE_BLOCK_JACOBI = 0, 1, 2
 FP64, MIXED = 0, 1
-SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON, SOLVER_FGMRES = 0, 1, 2, 3
+SOLVER_CG, SOLVER_GMRES, SOLVER_RICHARDSON, SOLVER_FGMRES, SOLVER_MINRES = 0, 1, 2, 3, 4
 UNET_NPARAM = 70980
 MLP_HIDDEN, MLP_NPARAM, MLP_MAXSEG = 128, 33793, 8
 
@@ -105,6 +105,8 @@ SIGNATURES = {
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],
     'ocmp_precond_create_schur': [P, C.c_int, C.c_uint64, C.POINTER(P)],
     'ocmp_precond_set_schur_solve': [P, C.c_int, C.c_double],
+    'ocmp_precond_set_block_triangular': [P, P, C.c_int, C.c_double],
+    'ocmp_precond_set_bf16': [P, C.c_int],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],

@@ -547,6 +547,21 @@ class Preconditioner:
             L.check(a.ctx.lib.ocmp_precond_create(a.h, k, self.precision, C.byref(h)), 'precond_create')
         self.h = h
 
+    def set_block_triangular(self, schur_diag: torch.Tensor, vel_iters: int, vel_tol: float) -> None:
+        """Block-triangular step for FGMRES (ocmp_precond_set_block_triangular; 'schur' kind with the
+        Schur blocks last): z_p = -v_p / schur_diag on the Schur rows, then z_u from at most
+        `vel_iters` Jacobi-PCG iterations on the velocity block for v_u - A_up z_p, stopped once the
+        preconditioned residual has fallen by `vel_tol`. `schur_diag`: a device tensor over the Schur
+        rows, a positive diagonal approximation of the Schur complement (0 iters: off)."""
+        d = schur_diag.to(self.a.ctx.torch_device, torch.float64).contiguous()
+        L.check(self.a.ctx.lib.ocmp_precond_set_block_triangular(self.h, C.c_void_p(_ptr(d)), int(vel_iters),
+                                                                 float(vel_tol)), 'precond_set_block_triangular')
+
+    def set_bf16(self, on: bool = True) -> None:
+        """Keep the inner solves' matrices (Schur S, the block-triangular velocity block) in BF16 for the
+        mixed path (ocmp_precond_set_bf16); the next Update makes the copies."""
+        L.check(self.a.ctx.lib.ocmp_precond_set_bf16(self.h, 1 if on else 0), 'precond_set_bf16')
+
     def Update(self):
         L.check(self.a.ctx.lib.ocmp_precond_update(self.h), 'precond_update')
 
@@ -573,7 +588,7 @@ def solve(a: BilinearForm, b: torch.Tensor, x: torch.Tensor, pre: Optional[Preco
           inner_tol: float = 0.0, check_every: int = 0, max_outer: int = 0, s_step: int = 0) -> SolveResult:
     o = L.SolveOpts()
     o.solver = {'CG': L.SOLVER_CG, 'GMRes': L.SOLVER_GMRES, 'Richardson': L.SOLVER_RICHARDSON,
-                'FGMRes': L.SOLVER_FGMRES}[solver]
+                'FGMRes': L.SOLVER_FGMRES, 'MinRes': L.SOLVER_MINRES}[solver]
     o.precision = L.MIXED if precision == 'mixed' else L.FP64
     o.restart = restart
     o.maxit = maxit

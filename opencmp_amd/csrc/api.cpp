@@ -478,10 +478,11 @@ void ensure_row_blocks32(Matrix *A) {
   if (!A->rb32.nnzb) build_row_blocks(A, A->rb32, SPMV_NNZB32);
 }
 
-void build_schur_matrix(Precond *P) {
-  Matrix *A = P->A;
-  const auto &bl = A->space->blocks;
-  int64_t s0 = -1, s1 = -1;
+// the dof range [s0, s1) of the Schur blocks, which must be adjacent
+static void schur_range(const Precond *P, int64_t &s0, int64_t &s1) {
+  const auto &bl = P->A->space->blocks;
+  s0 = -1;
+  s1 = -1;
   for (size_t k = 0; k < bl.size(); ++k) {
     if (!(P->schur_blocks >> k & 1)) continue;
     if (s0 < 0) {
@@ -492,6 +493,12 @@ void build_schur_matrix(Precond *P) {
     s1 = bl[k].dof_off + bl[k].n1;
   }
   OCMP_REQUIRE(s0 >= 0 && s1 > s0, "no Schur rows");
+}
+
+void build_schur_matrix(Precond *P) {
+  Matrix *A = P->A;
+  int64_t s0, s1;
+  schur_range(P, s0, s1);
   const int64_t n = A->n, ns = s1 - s0;
   const int32_t *rp = A->row_ptr.data(), *col = A->col.data();
   const uint8_t *freed = A->space->free.data();
@@ -616,6 +623,61 @@ void build_schur_matrix(Precond *P) {
   P->s0 = s0;
   P->s1 = s1;
   P->S = S;
+}
+
+// The velocity block A_uu (rows and columns [0, s0)) and the coupling A_up (rows [0, s0), columns
+// [s0, n) shifted to 0) of a system whose Schur rows are the last ones: each row's columns are
+// sorted, so its A_uu part is a prefix and its A_up part the rest. Values are gathered through
+// the slot arrays at every update.
+void build_velocity_blocks(Precond *P) {
+  Matrix *A = P->A;
+  int64_t s0, s1;
+  schur_range(P, s0, s1);
+  const int64_t n = A->n;
+  OCMP_REQUIRE(s1 == n, "the block-triangular step needs the Schur blocks last");
+  P->s0 = s0;
+  P->s1 = s1;
+  const int32_t *rp = A->row_ptr.data(), *col = A->col.data();
+  std::vector<int32_t> cut(s0);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < s0; ++r)
+    cut[r] = (int32_t)(std::lower_bound(col + rp[r], col + rp[r + 1], (int32_t)s0) - col);
+  hipStream_t st = A->ctx->stream;
+  auto make = [&](bool uu, std::vector<int32_t> &slot) {
+    auto *M = new Matrix();
+    M->space = A->space;
+    M->ctx = A->ctx;
+    M->n = s0;
+    M->row_ptr.assign(s0 + 1, 0);
+    for (int64_t r = 0; r < s0; ++r) M->row_ptr[r + 1] = M->row_ptr[r] + (uu ? cut[r] - rp[r] : rp[r + 1] - cut[r]);
+    M->nnz = M->row_ptr[s0];
+    M->col.resize(M->nnz);
+    slot.resize(M->nnz);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < s0; ++r) {
+      const int32_t a = uu ? rp[r] : cut[r], e = uu ? cut[r] : rp[r + 1];
+      for (int32_t j = a; j < e; ++j) {
+        const int32_t k = M->row_ptr[r] + (j - a);
+        M->col[k] = uu ? col[j] : col[j] - (int32_t)s0;
+        slot[k] = j;
+      }
+    }
+    M->d_row_ptr.upload(M->row_ptr, st);
+    M->d_col.upload(M->col, st);
+    M->d_val.alloc(M->nnz);
+    build_row_blocks(M, M->rb64, SPMV_NNZB64);
+    M->assembled = true;
+    return M;
+  };
+  std::vector<int32_t> uus, ups;
+  Matrix *Auu = make(true, uus);
+  Matrix *Aup = make(false, ups);
+  build_row_blocks(Auu, P->vrb, SCG_NNZB);
+  P->d_uuslot.upload(uus, st);
+  P->d_upslot.upload(ups, st);
+  OCMP_HIP(hipStreamSynchronize(st));
+  P->Auu = Auu;
+  P->Aup = Aup;
 }
 
 void build_colouring(Space &s) {
@@ -1295,6 +1357,44 @@ int ocmp_precond_set_schur_solve(ocmp_precond *P, int max_iters, double tol) {
     P->P.schur_iters = max_iters;
     P->P.schur_tol = tol;
     if (max_iters > 0 && !P->P.S) P->P.ready = false;  // S is built by the next update
+  });
+}
+
+int ocmp_precond_set_block_triangular(ocmp_precond *P, const double *schur_diag, int vel_iters, double vel_tol) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.schur_blocks != 0, "the block-triangular step needs a Schur preconditioner");
+    OCMP_REQUIRE(vel_iters >= 0 && vel_iters <= 10000, "velocity inner iterations must be in [0, 10000]");
+    OCMP_REQUIRE(vel_tol >= 0.0 && vel_tol < 1.0, "velocity inner tolerance must be in [0, 1)");
+    if (vel_iters == 0) {
+      P->P.btri = false;
+      return;
+    }
+    OCMP_REQUIRE(schur_diag, "NULL Schur diagonal");
+    OCMP_REQUIRE(P->P.schur_iters == 0, "the block-triangular step replaces the Schur inner solve: turn that off");
+    Matrix *A = P->P.A;
+    int64_t s0, s1;
+    schur_range(&P->P, s0, s1);
+    OCMP_REQUIRE(s0 > 0 && s1 == A->n, "the block-triangular step needs the Schur blocks last");
+    OCMP_HIP(hipSetDevice(A->ctx->device));
+    P->P.pdiag.alloc(A->n - s0);
+    OCMP_HIP(hipMemcpyAsync(P->P.pdiag.p, schur_diag, (A->n - s0) * sizeof(double), hipMemcpyDeviceToDevice,
+                            A->ctx->stream));
+    OCMP_HIP(hipStreamSynchronize(A->ctx->stream));
+    P->P.btri = true;
+    P->P.vel_iters = vel_iters;
+    P->P.vel_tol = vel_tol;
+    P->P.vel_cap = 0;
+    P->P.ready = false;
+  });
+}
+
+int ocmp_precond_set_bf16(ocmp_precond *P, int on) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.precision == OCMP_MIXED || !on, "BF16 storage is for preconditioners built with OCMP_MIXED");
+    if ((on != 0) != P->P.bf16) P->P.ready = false;  // the copies are made by the next update
+    P->P.bf16 = on != 0;
   });
 }
 

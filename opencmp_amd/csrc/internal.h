@@ -197,6 +197,21 @@ void eval_basis(int kind, int dim, int order, const double *lam, double *phi, do
 
 // ------------------------------------------------------------------ matrix
 
+// A bfloat16 storage value: the upper 16 bits of an FP32 number (conversion to float is exact).
+struct bf16s {
+  uint16_t b;
+  bf16s() = default;
+  __host__ __device__ explicit bf16s(int) : b(0) {}
+  __host__ __device__ operator float() const {
+    union {
+      uint32_t u;
+      float f;
+    } c;
+    c.u = (uint32_t)b << 16;
+    return c.f;
+  }
+};
+
 // Rows grouped into blocks of at most nnzb nonzeros (block b = rows [rblk[b], rblk[b+1])), the
 // distinct columns of each block (ucol[ubase[b] .. ubase[b+1]), sorted) and every nonzero's
 // 16-bit position in its block's list.
@@ -254,6 +269,7 @@ struct Matrix {
   bool assembled = false;
   // mixed precision copy
   DBuf<float> d_val32;
+  DBuf<bf16s> d_valbf;   // BF16 copy (the inner solves of a preconditioner stored in BF16)
   // solver scratch owned by this matrix (one per matrix, so distinct matrices never share it)
   Work *work = nullptr;
   Matrix() = default;
@@ -303,10 +319,36 @@ struct Precond {
   DBuf<float> svf;
   DBuf<double> spart;    // PCG scalars and partial sums
   DBuf<int> sflag;       // converged flag of the running inner solve
+  // Block-triangular step (FGMRES; the Schur rows are the last ones, [s0, n)): on the Schur rows
+  // z_p = -s v_p / d_p with d the caller's diagonal Schur approximation (rows where d is not above
+  // 1e-10 max d, or not free, get 0), then z_u = A_uu^{-1} (s v_u - A_up z_p) by at most vel_iters
+  // Jacobi-PCG iterations on the velocity block, stopped at (r, D^-1 r) <= vel_tol^2 (r0, D^-1 r0).
+  bool btri = false;
+  int vel_iters = 0, vel_cap = 0;
+  double vel_tol = 0.0, vel_seen = 0.0;
+  DBuf<double> pdiag;    // the caller's d on the Schur rows
+  DBuf<double> pdinv;    // 1/d (0: left out)
+  DBuf<float> pdinv32;
+  Matrix *Auu = nullptr, *Aup = nullptr;  // A's velocity block and velocity-pressure coupling
+  DBuf<int32_t> d_uuslot, d_upslot;        // the A slot of each of their values
+  RowBlocks vrb;                           // Auu's row blocks for the fused iteration
+  DBuf<double> vdinv;    // Jacobi of Auu (0 on Dirichlet rows)
+  DBuf<float> vdinv32;
+  DBuf<double> vvd;      // PCG vectors (8 n_u) and A_up z_p (n_u)
+  DBuf<float> vvf;
+  DBuf<double> vpart;    // PCG scalars and partial sums
+  DBuf<int> vflag;
+  // mixed path: keep the inner solves' matrices (S, the velocity block) in BF16 (FP32 vectors, FP64
+  // sums); they are part of the preconditioner, which FGMRES lets vary
+  bool bf16 = false;
   Precond() = default;
   Precond(const Precond &) = delete;
   Precond &operator=(const Precond &) = delete;
-  ~Precond() { delete S; }
+  ~Precond() {
+    delete S;
+    delete Auu;
+    delete Aup;
+  }
 };
 
 // ------------------------------------------------------------------ kernels (launchers)
@@ -391,14 +433,15 @@ void build_pattern(Matrix *A);
 constexpr int SPMV_NNZB64 = 2048;
 constexpr int SPMV_NNZB32 = 4096;
 // the fused Schur PCG iteration: row blocks of S of this many nonzeros, SCG_NT threads a block
-constexpr int SCG_NNZB = 8192;
-constexpr int SCG_NT = 1024;
+constexpr int SCG_NNZB = 4096;
+constexpr int SCG_NT = 512;
 void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb);
 void ensure_row_blocks32(Matrix *A);
 // the pattern of a Schur preconditioner's S = C + B diag(A)^{-1} B^T and the per-row Schur
 // column ranges of A (host, once per preconditioner)
 void build_schur_matrix(Precond *P);
 void ensure_diag_slots(Matrix *A);
+void build_velocity_blocks(Precond *P);
 constexpr int64_t SCHUR_MAX_PAIRS = (int64_t)1 << 28;
 
 }  // namespace ocmp

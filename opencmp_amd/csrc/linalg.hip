@@ -756,6 +756,17 @@ __global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restr
   if (i < n) b[i] = (float)a[i];
 }
 
+// round to nearest even from the FP32 value (finite inputs)
+__global__ void k_to_bf16(int64_t n, const double *__restrict__ a, bf16s *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t u = __float_as_uint((float)a[i]);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  bf16s o;
+  o.b = (uint16_t)(u >> 16);
+  b[i] = o;
+}
+
 // ------------------------------------------------------------------ vector kernels
 
 template <typename T>
@@ -1226,6 +1237,84 @@ static double norm_masked(Matrix *A, const double *v, Work &w) {
   return std::sqrt(h[0]);
 }
 
+// out[k] = src[slot[k]] (and its FP32 copy)
+__global__ void k_gather_vals(int64_t nnz, const int32_t *__restrict__ slot, const double *__restrict__ src,
+                              double *__restrict__ out, float *__restrict__ out32) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const double v = src[slot[k]];
+  out[k] = v;
+  if (out32) out32[k] = (float)v;
+}
+
+// d on the free Schur rows (0 elsewhere, and where d is not positive) and per-block maxima
+__global__ void __launch_bounds__(NTHR) k_pdiag_mask(int64_t ns, const double *__restrict__ d,
+                                                     const uint8_t *__restrict__ mask, double *__restrict__ out,
+                                                     double *__restrict__ part) {
+  __shared__ double sh[NTHR / 64];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < ns) {
+    v = mask[i] && d[i] > 0.0 ? d[i] : 0.0;
+    out[i] = v;
+  }
+  double m = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = sh[0];
+    for (int w = 1; w < NTHR / 64; ++w) bm = fmax(bm, sh[w]);
+    part[blockIdx.x] = bm;
+  }
+}
+
+static size_t pcg_part_size(Matrix *M, const RowBlocks &rb);
+
+// the block-triangular step's blocks of A, their Jacobi, and 1/d on the Schur rows
+static void btri_update(Precond *P, bool mixed) {
+  Matrix *A = P->A;
+  hipStream_t st = A->ctx->stream;
+  if (!P->Auu) build_velocity_blocks(P);
+  Matrix *Auu = P->Auu, *Aup = P->Aup;
+  const int64_t nu = P->s0, ns = A->n - P->s0;
+  if (mixed) {
+    Auu->d_val32.alloc(Auu->nnz);
+    Aup->d_val32.alloc(Aup->nnz);
+  }
+  k_gather_vals<<<nblk(Auu->nnz), NTHR, 0, st>>>(Auu->nnz, P->d_uuslot.p, A->d_val.p, Auu->d_val.p,
+                                                  mixed ? Auu->d_val32.p : nullptr);
+  k_gather_vals<<<nblk(Aup->nnz), NTHR, 0, st>>>(Aup->nnz, P->d_upslot.p, A->d_val.p, Aup->d_val.p,
+                                                  mixed ? Aup->d_val32.p : nullptr);
+  if (mixed && P->bf16) {
+    Auu->d_valbf.alloc(Auu->nnz);
+    k_to_bf16<<<nblk(Auu->nnz), NTHR, 0, st>>>(Auu->nnz, Auu->d_val.p, Auu->d_valbf.p);
+  }
+  ensure_diag_slots(Auu);
+  P->vdinv.alloc(nu);
+  if (mixed) P->vdinv32.alloc(nu);
+  k_jacobi_diag<<<nblk(nu), NTHR, 0, st>>>(nu, Auu->d_dslot.p, Auu->d_val.p, A->space->d_free.p, P->vdinv.p,
+                                           mixed ? P->vdinv32.p : nullptr);
+  P->pdinv.alloc(ns);
+  if (mixed) P->pdinv32.alloc(ns);
+  {
+    Work &w = work(A);
+    const unsigned nbs = nblk(ns);
+    ensure_part(w, (size_t)nbs + 8);
+    double *smax = w.part.p + nbs;
+    k_pdiag_mask<<<nbs, NTHR, 0, st>>>(ns, P->pdiag.p, A->space->d_free.p + nu, P->pdinv.p, w.part.p);
+    k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nbs, smax);
+    k_schur_sdinv<<<nbs, NTHR, 0, st>>>(ns, smax, P->pdinv.p, mixed ? P->pdinv32.p : nullptr);
+  }
+  if (mixed)
+    P->vvf.alloc((size_t)9 * nu);
+  else
+    P->vvd.alloc((size_t)9 * nu);
+  P->vpart.alloc(std::max(P->vpart.n, pcg_part_size(Auu, P->vrb)));
+  P->vflag.alloc(1);
+}
+
 void precond_update(Precond *P) {
   Matrix *A = P->A;
   Ctx *ctx = A->ctx;
@@ -1293,15 +1382,18 @@ void precond_update(Precond *P) {
         if (mixed) {
           S->d_val32.alloc(S->nnz);
           k_to_f32<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, S->d_val.p, S->d_val32.p);
+          if (P->bf16) {
+            S->d_valbf.alloc(S->nnz);
+            k_to_bf16<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, S->d_val.p, S->d_valbf.p);
+          }
           P->svf.alloc((size_t)8 * ns);
         } else {
           P->svd.alloc((size_t)8 * ns);
         }
-        const int sg = std::max(spmv_grid<double, double>(S), spmv_grid<float, float>(S));
-        const int fg = (int)((std::max<int64_t>(P->srb.nrblk, NXCD) + NXCD - 1) / NXCD * NXCD);
-        P->spart.alloc(std::max(P->spart.n, (size_t)SCHUR_SCAL + red_grid(ns) + 4 * std::max(sg, fg)));
+        P->spart.alloc(std::max(P->spart.n, pcg_part_size(S, P->srb)));
         P->sflag.alloc(1);
       }
+      if (P->btri) btri_update(P, mixed);
     }
     if (mixed) {
       A->d_val32.alloc(A->nnz);
@@ -2295,9 +2387,10 @@ static int ca_gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, 
 // Scalars (sc): [0..3] (gamma, alpha) slots, [4] gamma_0, [5] iterations done (a running count
 // over the solve).
 
-// x = 0, r = -s v, u = dinv r; partials of r.u; clears the converged flag
+// x = 0, r = sign s v - sub (sub may be NULL), u = dinv r; partials of r.u; clears the converged flag
 template <typename T>
 __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restrict__ v, const double *__restrict__ sv,
+                                                   double sign, const T *__restrict__ sub,
                                                    const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
                                                    T *__restrict__ u, double *__restrict__ gpart,
                                                    int *__restrict__ flag) {
@@ -2305,11 +2398,12 @@ __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restric
   const int lb = xcd_block();
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
-  const double sc = -sv[0];
+  const double sc = sign * sv[0];
   double acc = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     x[i] = (T)0;
-    const T ri = (T)((double)v[i] * sc);
+    T ri = (T)((double)v[i] * sc);
+    if (sub) ri -= sub[i];
     r[i] = ri;
     const T ui = dinv[i] * ri;
     u[i] = ui;
@@ -2356,7 +2450,9 @@ __device__ __forceinline__ void sum_parts2(const double *__restrict__ pa, int na
   sb = y;
 }
 
-// One Chronopoulos-Gear iteration in one launch. The previous launch left r_i, w_i = S u_i,
+// One Chronopoulos-Gear iteration in one launch (row blocks of SCG_NNZB = 4096 nonzeros, 512 threads:
+// measured 7 % faster per launch than 8192/1024, 4096/256 and 2048/256 on the 200,198 pressure rows
+// of the INS channel). The previous launch left r_i, w_i = S u_i,
 // s_{i-1} and the partials of gamma_i = (r_i, u_i), delta_i = (w_i, u_i); this one forms alpha_i
 // and beta_i from them and, for the rows of its CSR row block, updates
 //   s_i = w_i + beta s_{i-1},  p_i = u_i + beta p_{i-1},  x += alpha p_i,
@@ -2381,11 +2477,11 @@ __device__ __forceinline__ ScgNext<T> scg_next(const T *__restrict__ rin, const 
   return o;
 }
 
-template <typename T, int NNZB, int NT>
+template <typename T, typename VT, int NNZB, int NT>
 __global__ void __launch_bounds__(NT) k_scg_fused(
     int64_t nrblk, const int32_t *__restrict__ rblk, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
     const int32_t *__restrict__ ubase, const int32_t *__restrict__ ucol, const uint16_t *__restrict__ lcol,
-    const T *__restrict__ val, int it, double tol2, T *__restrict__ x, T *__restrict__ p, const T *__restrict__ rin,
+    const VT *__restrict__ val, int it, double tol2, T *__restrict__ x, T *__restrict__ p, const T *__restrict__ rin,
     T *__restrict__ rout, const T *__restrict__ sin, T *__restrict__ sout, const T *__restrict__ win,
     T *__restrict__ wout, const T *__restrict__ dinv, const double *__restrict__ gin, int ng,
     const double *__restrict__ din, int nd, double *__restrict__ gout, double *__restrict__ dout,
@@ -2434,12 +2530,14 @@ __global__ void __launch_bounds__(NT) k_scg_fused(
     ag += (double)o.r * (double)o.u;
     ad += (double)wk * (double)o.u;
   };
-  if (lb < nrblk) {
-    const int32_t r0 = rblk[lb], r1 = rblk[lb + 1];
+  // a grid of at most SCG_MAX_GRID blocks walks all row blocks, so the partials every block sums
+  // stay few however large the matrix is
+  for (int64_t rbk = lb; rbk < nrblk; rbk += gridDim.x) {
+    const int32_t r0 = rblk[rbk], r1 = rblk[rbk + 1];
     const int32_t j0 = rp[r0], j1 = rp[r1];
     const int nz = j1 - j0;
     const int nrows = r1 - r0;
-    __syncthreads();  // sh (the partial sums above) aliases the staging buffer
+    __syncthreads();  // sh (the partial sums above) and the last row block's products alias the staging buffer
     if (nz <= NNZB) {
       const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
       int a0 = 0, e0 = 0;
@@ -2456,10 +2554,10 @@ __global__ void __launch_bounds__(NT) k_scg_fused(
         vv[k] = (T)0;
         if (t < nz) {
           li[k] = lcol[j0 + t];
-          vv[k] = val[j0 + t];
+          vv[k] = (T)val[j0 + t];
         }
       }
-      const int u0 = ubase[lb], nu = ubase[lb + 1] - u0;
+      const int u0 = ubase[rbk], nu = ubase[rbk + 1] - u0;
       for (int base = 0; base < nu; base += 2 * NT) {
         const int t0 = base + threadIdx.x, t1 = t0 + NT;
         const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
@@ -2496,7 +2594,7 @@ __global__ void __launch_bounds__(NT) k_scg_fused(
       double sv = 0.0;
       for (int t = threadIdx.x; t < nz; t += NT) {
         const int32_t c = col[j0 + t];
-        sv += (double)(val[j0 + t] * scg_next<T>(rin, win, sin, dinv, c, a, b, first).u);
+        sv += (double)((T)val[j0 + t] * scg_next<T>(rin, win, sin, dinv, c, a, b, first).u);
       }
       sv = block_sum(sv, sh);
       if (threadIdx.x == 0) finish(r0, sv);
@@ -2511,54 +2609,171 @@ __global__ void __launch_bounds__(NT) k_scg_fused(
   }
 }
 
-// z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero: the start (x = 0,
-// r_0, u_0, gamma_0 partials; then w_0 = S u_0 with the delta_0 partials) and K fused iterations.
-template <typename T>
-static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
-  Matrix *S = P->S;
+// blocks of the fused iteration: one per row block, at most SCG_MAX_GRID (a multiple of the XCD count)
+constexpr int SCG_MAX_GRID = 1024;
+static int scg_grid(const RowBlocks &rb) {
+  const int64_t g = (std::max<int64_t>(rb.nrblk, NXCD) + NXCD - 1) / NXCD * NXCD;
+  return (int)std::min<int64_t>(g, SCG_MAX_GRID);
+}
+
+// One inner Jacobi-PCG solve M x = sign s v - sub from x = 0: its matrix (values in T), row blocks
+// for the fused iteration, Jacobi, vectors (8 n), scalars + partial sums, converged flag, and the
+// timing classes of its two kernels.
+template <typename T, typename VT>
+struct PcgArgs {
+  Matrix *M;
+  const RowBlocks *rb;
+  const VT *val;
+  const T *dinv;
+  T *buf;
+  double *sc;
+  int *flag;
+  double tol;
+  int c_op, c_it;
+};
+
+// at most K iterations: the start (x = 0, r_0, u_0, gamma_0 partials; then w_0 = M u_0 with the
+// delta_0 partials) and K fused iterations
+template <typename T, typename VT>
+static void inner_pcg(const PcgArgs<T, VT> &q, const T *v, const double *sv, double sign, const T *sub, T *x,
+                      int K) {
+  Matrix *S = q.M;
   Ctx *ctx = S->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = S->n;
-  const T *val, *dinv;
-  T *buf;
-  if constexpr (sizeof(T) == 8) {
-    val = S->d_val.p;
-    dinv = P->sdinv.p;
-    buf = P->svd.p;
-  } else {
-    val = S->d_val32.p;
-    dinv = P->sdinv32.p;
-    buf = P->svf.p;
-  }
-  T *x = z + P->s0;
+  const VT *val = q.val;
+  const T *dinv = q.dinv;
+  T *buf = q.buf;
   T *r[2] = {buf, buf + n}, *s[2] = {buf + 2 * n, buf + 3 * n}, *w[2] = {buf + 4 * n, buf + 5 * n};
   T *p = buf + 6 * n, *u = buf + 7 * n;
-  const int vg = std::min(red_grid(n), 256), sg = spmv_grid<T, T>(S);
-  const RowBlocks &rb = P->srb;
-  const int fg = (int)((std::max<int64_t>(rb.nrblk, NXCD) + NXCD - 1) / NXCD * NXCD);
+  const int vg = std::min(red_grid(n), 256), sg = spmv_grid<VT, T>(S);
+  const RowBlocks &rb = *q.rb;
+  const int fg = scg_grid(rb);
   const int pm = std::max(sg, fg);
-  double *sc = P->spart.p;
+  double *sc = q.sc;
   double *g0 = sc + SCHUR_SCAL;
   double *gq[2] = {g0 + vg, g0 + vg + pm}, *dq[2] = {g0 + vg + 2 * pm, g0 + vg + 3 * pm};
-  int *flag = P->sflag.p;
-  const double tol2 = P->schur_tol * P->schur_tol;
-  const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_schur_f64" : "spmv_schur_f32");
-  const int c_it = ctx->cls(sizeof(T) == 8 ? "schur_pcg_fused_f64" : "schur_pcg_fused_f32");
-  const double it_bytes = csr_bytes(S, sizeof(T), sizeof(T)) + 10.0 * n * sizeof(T);
-  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v + P->s0, sv, dinv, x, r[0], u, g0, flag);
+  int *flag = q.flag;
+  const double tol2 = q.tol * q.tol;
+  const double it_bytes = csr_bytes(S, sizeof(VT), sizeof(T)) + 10.0 * n * sizeof(T);
+  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v, sv, sign, sub, dinv, x, r[0], u, g0, flag);
   {
-    KTimer tm(ctx, c_op, csr_bytes(S, sizeof(T), sizeof(T)));
-    spmv_dispatch<T, T>(S, val, u, w[0], nullptr, nullptr, nullptr, nullptr, 2, dq[0], sg, st, (T)0, nullptr, (T)0,
-                        flag);
+    KTimer tm(ctx, q.c_op, csr_bytes(S, sizeof(VT), sizeof(T)));
+    spmv_dispatch<VT, T>(S, val, u, w[0], nullptr, nullptr, nullptr, nullptr, 2, dq[0], sg, st, (T)0, nullptr, (T)0,
+                         flag);
   }
+  // delta_0: one value (sc[6]) from the SpMV's partials, one per row block of the plain SpMV
+  reduce_to(dq[0], sg, 1, sc + 6, st);
   for (int it = 0; it < K; ++it) {
-    KTimer tm(ctx, c_it, it_bytes);
+    KTimer tm(ctx, q.c_it, it_bytes);
     const int a = it & 1, b = a ^ 1;
     const double *gin = it == 0 ? g0 : gq[a];
-    const int ng = it == 0 ? vg : fg, nd = it == 0 ? sg : fg;
-    k_scg_fused<T, SCG_NNZB, SCG_NT><<<fg, SCG_NT, 0, st>>>(
+    const double *din = it == 0 ? sc + 6 : dq[a];
+    const int ng = it == 0 ? vg : fg, nd = it == 0 ? 1 : fg;
+    k_scg_fused<T, VT, SCG_NNZB, SCG_NT><<<fg, SCG_NT, 0, st>>>(
         rb.nrblk, rb.d_rblk.p, S->d_row_ptr.p, S->d_col.p, rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, it, tol2, x,
-        p, r[a], r[b], s[a], s[b], w[a], w[b], dinv, gin, ng, dq[a], nd, gq[b], dq[b], sc, flag);
+        p, r[a], r[b], s[a], s[b], w[a], w[b], dinv, gin, ng, din, nd, gq[b], dq[b], sc, flag);
+  }
+}
+
+// scalars + partials an inner PCG on M needs (the two-stage reduction of the first SpMV's partials
+// uses SEG_BLOCKS more behind them)
+static size_t pcg_part_size(Matrix *M, const RowBlocks &rb) {
+  const int sg = std::max(spmv_grid<double, double>(M), spmv_grid<float, float>(M));
+  return (size_t)SCHUR_SCAL + red_grid(M->n) + 4 * std::max(sg, scg_grid(rb)) + SEG_BLOCKS;
+}
+
+// z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero
+template <typename T, typename VT>
+static void schur_inner_t(Precond *P, const VT *val, const T *v, const double *sv, T *z, int K) {
+  Ctx *ctx = P->S->ctx;
+  PcgArgs<T, VT> q;
+  q.M = P->S;
+  q.rb = &P->srb;
+  q.val = val;
+  if constexpr (sizeof(T) == 8) {
+    q.dinv = P->sdinv.p;
+    q.buf = P->svd.p;
+  } else {
+    q.dinv = P->sdinv32.p;
+    q.buf = P->svf.p;
+  }
+  q.sc = P->spart.p;
+  q.flag = P->sflag.p;
+  q.tol = P->schur_tol;
+  q.c_op = ctx->cls(sizeof(T) == 8 ? "spmv_schur_f64" : (sizeof(VT) == 2 ? "spmv_schur_bf16" : "spmv_schur_f32"));
+  q.c_it = ctx->cls(sizeof(T) == 8 ? "schur_pcg_fused_f64"
+                                   : (sizeof(VT) == 2 ? "schur_pcg_fused_bf16" : "schur_pcg_fused_f32"));
+  inner_pcg<T, VT>(q, v + P->s0, sv, -1.0, nullptr, z + P->s0, K);
+}
+
+template <typename T>
+static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
+  if constexpr (sizeof(T) == 8) {
+    schur_inner_t<double, double>(P, P->S->d_val.p, v, sv, z, K);
+  } else {
+    if (P->bf16)
+      schur_inner_t<float, bf16s>(P, P->S->d_valbf.p, v, sv, z, K);
+    else
+      schur_inner_t<float, float>(P, P->S->d_val32.p, v, sv, z, K);
+  }
+}
+
+// z_p = -(s v_p) pdinv on the Schur rows
+template <typename T>
+__global__ void k_btri_p(int64_t ns, const T *__restrict__ pdinv, const T *__restrict__ v,
+                         const double *__restrict__ sv, T *__restrict__ z) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) z[i] = (T)(-(double)pdinv[i] * (double)v[i] * sv[0]);
+}
+
+// the block-triangular step: z_p = -(s v_p) / d_p, then z_u = A_uu^{-1} (s v_u - A_up z_p)
+template <typename T, typename VT>
+static void btri_apply_t(Precond *P, const VT *uuval, const T *v, const double *sv, T *z, int K) {
+  Ctx *ctx = P->A->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t nu = P->s0, ns = P->A->n - P->s0;
+  const T *pdinv, *upval;
+  PcgArgs<T, VT> q;
+  q.M = P->Auu;
+  q.rb = &P->vrb;
+  q.val = uuval;
+  if constexpr (sizeof(T) == 8) {
+    pdinv = P->pdinv.p;
+    upval = P->Aup->d_val.p;
+    q.dinv = P->vdinv.p;
+    q.buf = P->vvd.p;
+  } else {
+    pdinv = P->pdinv32.p;
+    upval = P->Aup->d_val32.p;
+    q.dinv = P->vdinv32.p;
+    q.buf = P->vvf.p;
+  }
+  q.sc = P->vpart.p;
+  q.flag = P->vflag.p;
+  q.tol = P->vel_tol;
+  q.c_op = ctx->cls(sizeof(T) == 8 ? "spmv_vel_f64" : (sizeof(VT) == 2 ? "spmv_vel_bf16" : "spmv_vel_f32"));
+  q.c_it = ctx->cls(sizeof(T) == 8 ? "vel_pcg_fused_f64" : (sizeof(VT) == 2 ? "vel_pcg_fused_bf16" : "vel_pcg_fused_f32"));
+  T *y = q.buf + 8 * nu;
+  k_btri_p<T><<<nblk(ns), NTHR, 0, st>>>(ns, pdinv, v + nu, sv, z + nu);
+  {
+    KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "spmv_coupling_f64" : "spmv_coupling_f32"),
+              csr_bytes(P->Aup, sizeof(T), sizeof(T)));
+    spmv_dispatch<T, T>(P->Aup, upval, z + nu, y, nullptr, nullptr, nullptr, nullptr, 0, nullptr,
+                        spmv_grid<T, T>(P->Aup), st);
+  }
+  inner_pcg<T, VT>(q, v, sv, 1.0, y, z, K);
+}
+
+template <typename T>
+static void btri_apply(Precond *P, const T *v, const double *sv, T *z, int K) {
+  if constexpr (sizeof(T) == 8) {
+    btri_apply_t<double, double>(P, P->Auu->d_val.p, v, sv, z, K);
+  } else {
+    if (P->bf16)
+      btri_apply_t<float, bf16s>(P, P->Auu->d_valbf.p, v, sv, z, K);
+    else
+      btri_apply_t<float, float>(P, P->Auu->d_val32.p, v, sv, z, K);
   }
 }
 
@@ -2610,22 +2825,32 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
   };
   std::deque<Batch> inflight;
   int j = 0, bidx = 0, k_used = -1;
-  // Schur inner solve with a tolerance: each step enqueues schur_cap iterations (the rest of a
-  // converged inner solve are empty launches); after each batch the host reads how many ran and
-  // sets the cap to 1.3x that plus 10, doubling it when a step used them all
-  const bool adapt = P->schur_iters > 0 && P->schur_tol > 0.0;
+  // inner solve (Schur PCG, or the velocity PCG of the block-triangular step) with a tolerance:
+  // each step enqueues `cap` iterations (the rest of a converged inner solve are empty launches);
+  // after each batch the host reads how many ran and sets the cap to 1.3x that plus 10, doubling
+  // it when a step used them all
+  const bool btri = P->btri;
+  const int in_iters = btri ? P->vel_iters : P->schur_iters;
+  const bool adapt = in_iters > 0 && (btri ? P->vel_tol : P->schur_tol) > 0.0;
+  int &in_cap = btri ? P->vel_cap : P->schur_cap;
+  double &in_seen = btri ? P->vel_seen : P->schur_seen;
+  const double *in_count = btri ? P->vpart.p + 5 : P->spart.p + 5;
   auto enqueue = [&]() {
     const int j_end = std::min(limit, j + check);
-    const int cap = P->schur_iters > 0 ? (adapt ? P->schur_cap : P->schur_iters) : 0;
+    const int cap = in_iters > 0 ? (adapt ? in_cap : in_iters) : 0;
     for (int jj = j; jj < j_end; ++jj) {
       T *vj = V + (int64_t)jj * ldv;
       T *zj = Z + (int64_t)jj * ldv;
       T *wv = V + (int64_t)(jj + 1) * ldv;
-      {
-        KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
-        k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
+      if (btri) {
+        btri_apply<T>(P, vj, vs + jj, zj, cap);
+      } else {
+        {
+          KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
+          k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
+        }
+        if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj, cap);
       }
-      if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj, cap);
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
         KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
@@ -2647,7 +2872,7 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
     OCMP_HIP(hipMemcpyAsync(hres + j, res + j, nb * sizeof(double), hipMemcpyDeviceToHost, st));
     const int ev = bidx & 1;
     if (adapt)
-      OCMP_HIP(hipMemcpyAsync(hres + m + 2 + ev, P->spart.p + 5, sizeof(double), hipMemcpyDeviceToHost, st));
+      OCMP_HIP(hipMemcpyAsync(hres + m + 2 + ev, in_count, sizeof(double), hipMemcpyDeviceToHost, st));
     OCMP_HIP(hipEventRecord(work_event(w, ev), st));
     inflight.push_back({j, nb, ev, cap});
     ++bidx;
@@ -2664,10 +2889,9 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
     OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
     if (adapt) {
       const double cnt = hres[m + 2 + b.ev];
-      const double per = (cnt - P->schur_seen) / b.nb;
-      P->schur_seen = cnt;
-      P->schur_cap = per >= b.cap - 0.5 ? std::min(P->schur_iters, 2 * b.cap)
-                                         : std::min(P->schur_iters, (int)(1.3 * per) + 10);
+      const double per = (cnt - in_seen) / b.nb;
+      in_seen = cnt;
+      in_cap = per >= b.cap - 0.5 ? std::min(in_iters, 2 * b.cap) : std::min(in_iters, (int)(1.3 * per) + 10);
     }
     for (int s = 0; s < b.nb; ++s)
       if (!(hres[b.j0 + s] > stop_abs)) {
@@ -2694,10 +2918,20 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   OCMP_REQUIRE(!A->dg, "FGMRES is for H1 systems");
   OCMP_REQUIRE(P->schur_iters == 0 || (P->S && P->sdinv.p && (o.precision != OCMP_MIXED || P->sdinv32.p)),
                "update the preconditioner after turning the Schur inner solve on");
-  if (P->schur_iters > 0) {
+  OCMP_REQUIRE(!P->btri || (P->Auu && P->vdinv.p && P->pdinv.p && (o.precision != OCMP_MIXED || P->vdinv32.p)),
+               "update the preconditioner after turning the block-triangular step on");
+  OCMP_REQUIRE(!P->bf16 || o.precision != OCMP_MIXED ||
+                   ((P->schur_iters == 0 || P->btri || P->S->d_valbf.p) && (!P->btri || P->Auu->d_valbf.p)),
+               "update the preconditioner after turning BF16 storage on");
+  if (P->schur_iters > 0 && !P->btri) {
     k_fill_t<double><<<1, 64, 0, A->ctx->stream>>>(P->spart.p + 5, 1, 0.0);
     P->schur_seen = 0.0;
     if (P->schur_cap <= 0 || P->schur_cap > P->schur_iters) P->schur_cap = P->schur_iters;
+  }
+  if (P->btri) {
+    k_fill_t<double><<<1, 64, 0, A->ctx->stream>>>(P->vpart.p + 5, 1, 0.0);
+    P->vel_seen = 0.0;
+    if (P->vel_cap <= 0 || P->vel_cap > P->vel_iters) P->vel_cap = P->vel_iters;
   }
   Work &w = work(A);
   const int64_t n = A->n;
@@ -2712,7 +2946,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   st_.rel_residual = rn / bref;
   // with a Schur inner solve a step costs milliseconds: poll every iteration so that at most a
   // step or two run past convergence
-  const int check = o.check_every > 0 ? o.check_every : (P->schur_iters > 0 ? 1 : 5);
+  const int check = o.check_every > 0 ? o.check_every : (P->schur_iters > 0 || P->btri ? 1 : 5);
   const int m = o.restart;
   int total = 0, outer = 0;
   const double inner = o.inner_tol > 0 ? o.inner_tol : 1e-5;
@@ -2755,9 +2989,10 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   st_.outer = outer;
   st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
   st_.fallback = have_learned && fell_back ? 1 : 0;
-  if (P->schur_iters > 0) {
+  if (P->schur_iters > 0 || P->btri) {
     double cnt = 0.0;
-    OCMP_HIP(hipMemcpyAsync(&cnt, P->spart.p + 5, sizeof(double), hipMemcpyDeviceToHost, A->ctx->stream));
+    OCMP_HIP(hipMemcpyAsync(&cnt, (P->btri ? P->vpart.p : P->spart.p) + 5, sizeof(double), hipMemcpyDeviceToHost,
+                            A->ctx->stream));
     OCMP_HIP(hipStreamSynchronize(A->ctx->stream));
     st_.precond_iters = (int64_t)cnt;
   }
@@ -2903,6 +3138,125 @@ double residual_vector(Matrix *A, const double *b, const double *x, double *r) {
   return residual64(A, nullptr, b, x, r, w);
 }
 
+// ------------------------------------------------------------------ MINRES
+//
+// Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12 (1975) 617-629; the recurrence
+// of Elman, Silvester and Wathen, Finite Elements and Fast Iterative Solvers, Alg. 2.4) for a
+// symmetric A on the free dofs, FP64. The preconditioner must be symmetric positive definite: the
+// Jacobi preconditioner's |D|^{-1} (on a saddle point with the Schur-diagonal preconditioner that is
+// the positive block-diagonal preconditioner diag(|A_uu|)^{-1}, |S_diag|^{-1}), the identity without
+// one. The scalar recurrence runs on the host: each iteration reads two dot products. |eta| (the
+// preconditioned residual norm) decides when to check the FP64 true residual; a cycle whose true
+// residual is still above the tolerance restarts from the current x.
+
+// v2 = q - a v1 - b v0; z2 = |dinv| v2 (free rows only: dinv is 0 elsewhere; no preconditioner: mask)
+__global__ void k_minres_v(int64_t n, const double *__restrict__ q, const double *__restrict__ v1,
+                           const double *__restrict__ v0, double a, double b, const double *__restrict__ dinv,
+                           const uint8_t *__restrict__ mask, double *__restrict__ v2, double *__restrict__ z2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double v = q[i] - a * v1[i] - b * v0[i];
+  v2[i] = v;
+  z2[i] = dinv ? fabs(dinv[i]) * v : (mask[i] ? v : 0.0);
+}
+
+// w2 = (z / gamma - a3 w0 - a2 w1) / a1;  x += t w2
+__global__ void k_minres_w(int64_t n, const double *__restrict__ z, double ig, const double *__restrict__ w0,
+                           const double *__restrict__ w1, double a1, double a2, double a3, double *__restrict__ w2,
+                           double t, double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double w = (z[i] * ig - a3 * w0[i] - a2 * w1[i]) / a1;
+  w2[i] = w;
+  x[i] += t * w;
+}
+
+static void solve_minres(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o,
+                         ocmp_solve_stats &st_) {
+  Ctx *ctx = A->ctx;
+  hipStream_t st = ctx->stream;
+  Work &w = work(A);
+  const int64_t n = A->n;
+  OCMP_REQUIRE(o.precision == OCMP_FP64, "MinRes runs in FP64");
+  OCMP_REQUIRE(!P || P->kind == OCMP_PRE_JACOBI || P->kind == OCMP_PRE_NONE,
+               "MinRes needs a point-Jacobi (or Schur-diagonal) preconditioner, or none");
+  OCMP_REQUIRE(!P || (P->schur_iters == 0 && !P->btri), "MinRes needs a fixed preconditioner (no inner solve)");
+  const double *dinv = P && P->kind == OCMP_PRE_JACOBI ? P->dinv.p : nullptr;
+  const uint8_t *mask = A->space->d_free.p;
+  ensure_part(w, 0);
+  w.r64.alloc(n);
+  w.vd.alloc(std::max<size_t>(w.vd.n, (size_t)10 * n));
+  double *base = w.vd.p;
+  double *V[3] = {base, base + n, base + 2 * n};
+  double *Wv[3] = {base + 3 * n, base + 4 * n, base + 5 * n};
+  double *Z[2] = {base + 6 * n, base + 7 * n};
+  double *xs = base + 8 * n, *aq = base + 9 * n;
+  const double bn = norm_masked(A, b, w);
+  double rn = residual64(A, P, b, x, w.r64.p, w);
+  const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
+  st_.initial_rel_residual = rn / bref;
+  st_.rel_residual = rn / bref;
+  const int grid = red_grid(n);
+  double *sc = ctx->scal.p;
+  double *h = pinned(w, 8);
+  auto dot = [&](const double *a, const double *c) {
+    k_dot<double><<<grid, NTHR, 0, st>>>(n, a, c, mask, w.part.p);
+    reduce_to(w.part.p, grid, 1, sc + 102, st);
+    OCMP_HIP(hipMemcpyAsync(h, sc + 102, sizeof(double), hipMemcpyDeviceToHost, st));
+    OCMP_HIP(hipStreamSynchronize(st));
+    return h[0];
+  };
+  const int sgrid = spmv_grid(A);
+  int total = 0, outer = 0;
+  while (rn > o.tol * bref && total < o.maxit && std::isfinite(rn)) {
+    ++outer;
+    // v1 = r (the true residual, masked), z = M v1, gamma1 = sqrt(v1.z); v0 = w0 = w1 = 0
+    k_fill_t<double><<<nblk(n), NTHR, 0, st>>>(V[0], n, 0.0);
+    k_fill_t<double><<<nblk(2 * n), NTHR, 0, st>>>(Wv[0], 2 * n, 0.0);
+    k_minres_v<<<nblk(n), NTHR, 0, st>>>(n, w.r64.p, V[0], V[0], 0.0, 0.0, dinv, mask, V[1], Z[0]);
+    double g1 = std::sqrt(std::max(dot(V[1], Z[0]), 0.0)), g0 = 1.0;
+    if (!(g1 > 0.0)) break;
+    double eta = g1, c0 = 1.0, c1 = 1.0, s0 = 0.0, s1 = 0.0;
+    const double eta0 = g1;
+    // the true residual is about rn |eta| / eta0: stop the cycle at half the tolerance
+    const double eta_stop = 0.5 * o.tol * bref / rn * eta0;
+    int it = 0, zc = 0;
+    double *v0 = V[0], *v1 = V[1], *v2 = V[2], *w0 = Wv[0], *w1 = Wv[1], *w2 = Wv[2];
+    while (total + it < o.maxit) {
+      double *z = Z[zc], *zn = Z[zc ^ 1];
+      k_scale_copy<double, double><<<nblk(n), NTHR, 0, st>>>(n, z, 1.0 / g1, xs);
+      spmv_dispatch<double, double>(A, A->d_val.p, xs, aq, nullptr, mask, nullptr, nullptr, 3, nullptr, sgrid, st);
+      const double delta = dot(aq, xs);
+      k_minres_v<<<nblk(n), NTHR, 0, st>>>(n, aq, v1, v0, delta / g1, it == 0 ? 0.0 : g1 / g0, dinv, mask, v2, zn);
+      const double g2 = std::sqrt(std::max(dot(v2, zn), 0.0));
+      const double a0 = c1 * delta - c0 * s1 * g1;
+      const double a1 = std::sqrt(a0 * a0 + g2 * g2);
+      const double a2 = s1 * delta + c0 * c1 * g1;
+      const double a3 = s0 * g1;
+      if (!(a1 > 0.0)) break;
+      const double c2 = a0 / a1, s2 = g2 / a1;
+      k_minres_w<<<nblk(n), NTHR, 0, st>>>(n, z, 1.0 / g1, w0, w1, a1, a2, a3, w2, c2 * eta, x);
+      eta = -s2 * eta;
+      ++it;
+      double *tv = v0;
+      v0 = v1, v1 = v2, v2 = tv;
+      double *tw = w0;
+      w0 = w1, w1 = w2, w2 = tw;
+      zc ^= 1;
+      g0 = g1, g1 = g2, c0 = c1, c1 = c2, s0 = s1, s1 = s2;
+      if (!(g1 > 0.0) || std::fabs(eta) <= eta_stop) break;
+    }
+    total += it;
+    const double rn_old = rn;
+    rn = residual64(A, P, b, x, w.r64.p, w);
+    st_.rel_residual = rn / bref;
+    if (it == 0 || !(rn < rn_old)) break;
+  }
+  st_.iters = total;
+  st_.outer = outer;
+  st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
+}
+
 // out[0] = ||b - A x||_2, out[1] = ||b||_2, both over the free dofs in FP64
 void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
   Work &w = work(A);
@@ -2919,6 +3273,7 @@ void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_o
     case OCMP_SOLVER_GMRES: solve_gmres(A, P, b, x, o, st); break;
     case OCMP_SOLVER_RICHARDSON: solve_richardson(A, P, b, x, o, st); break;
     case OCMP_SOLVER_FGMRES: solve_fgmres(A, P, b, x, o, st); break;
+    case OCMP_SOLVER_MINRES: solve_minres(A, P, b, x, o, st); break;
     default: throw Error(OCMP_ERR_UNSUPPORTED, "unknown solver");
   }
 }

@@ -155,3 +155,21 @@ def load_mesh(path: str) -> Mesh:
     if path.endswith('.msh'):
         return read_msh(path)
     raise TypeError('Only .vol (Netgen) and .msh (GMSH) meshes can be used.')
+
+
+# ------------------------------------------------------------------ solution files
+
+def save_sol(path: str, vec) -> None:
+    """GridFunction.Save (opencmp/helpers/saving.py:74-93): the coefficient vector as raw native
+    float64 values, no header, in dof order (vertex, edge, face/cell dofs per component block; NGSolve's
+    node order, which the dof numbering follows). `vec`: a device tensor or an array."""
+    a = vec.detach().cpu().numpy() if hasattr(vec, 'detach') else np.asarray(vec)
+    np.ascontiguousarray(a, dtype=np.float64).tofile(path)
+
+
+def load_sol(path: str, ndof: int) -> np.ndarray:
+    """GridFunction.Load: the raw float64 coefficient vector of a space with `ndof` dofs."""
+    a = np.fromfile(path, dtype=np.float64)
+    if a.size != ndof:
+        raise ValueError(f'"{path}" holds {a.size} values; the space has {ndof} dofs')
+    return a

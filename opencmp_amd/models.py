@@ -68,7 +68,7 @@ class Model:
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
                  inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
-                 t_param: Optional[List[E.Parameter]] = None):
+                 t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0):
         self.ctx = ctx or C.Context.default()
         # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
         # expressions of t (boundary data, sources) are read at the level a form is built for
@@ -95,8 +95,14 @@ class Model:
         self.schur_tol = schur_tol
         if self.schur_iters and linear_solver == 'GMRes':
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
+        self.vel_iters = vel_iters if self.preconditioner == 'block_triangular' else 0
+        self.vel_tol = vel_tol
+        if self.preconditioner == 'block_triangular':
+            if not self.vel_iters:
+                raise ValueError("preconditioner 'block_triangular' needs vel_iters > 0")
+            linear_solver = 'FGMRes'
         self.linear_solver = linear_solver
-        if self.preconditioner in (None, 'None') and linear_solver == 'CG':
+        if self.preconditioner in (None, 'None') and linear_solver in ('CG', 'MinRes'):
             raise ValueError('Preconditioner cannot be None if using CG or MinRes solvers.')
         self.linear_tolerance = linear_tolerance
         self.linear_max_iterations = linear_max_iterations
@@ -169,7 +175,16 @@ class Model:
             return C.Preconditioner(a_assembled, 'schur', precision=self.precision,
                                     schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters,
                                     schur_tol=self.schur_tol)
+        if self.preconditioner == 'block_triangular':
+            pre = C.Preconditioner(a_assembled, 'schur', precision=self.precision, schur_blocks=self.schur_blocks())
+            pre.set_block_triangular(self.schur_diagonal(), self.vel_iters, self.vel_tol)
+            return pre
         return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
+
+    def schur_diagonal(self) -> torch.Tensor:
+        """A positive diagonal approximation of the pressure Schur complement over the Schur rows
+        (saddle-point models; the block-triangular preconditioner)."""
+        raise NotImplementedError(f'{type(self).__name__} has no Schur complement approximation')
 
     def schur_blocks(self) -> List[int]:
         """Scalar blocks of the constraint (pressure) components of a saddle-point model."""
@@ -417,6 +432,22 @@ class Stokes(Model):
     def time_derivative_components(self):
         return ['u']
 
+    def _schur_weight(self):
+        return None
+
+    def schur_diagonal(self) -> torch.Tensor:
+        """The lumped pressure mass over kv, int psi_i / kv (weighted by the model's pressure weight):
+        the Schur complement B A^{-1} B^T of the viscous operator is spectrally equivalent to the
+        pressure mass over the viscosity, and the lumped mass (the P1 functions sum to one) is
+        equivalent to the mass matrix."""
+        pb = self.fes.comp_blocks[1][0]
+        if len(self.fes.comp_blocks[1]) != 1 or self.interp_ord != 2:
+            raise NotImplementedError('the lumped pressure mass needs one P1 pressure block')
+        f = C.LinearForm(self.fes, [C.Source(pb, (L.FN_CONST, (1.0 / self.kv,)), weight_gf=self._schur_weight())])
+        f.Assemble()
+        sl = self.fes.block_slice(pb)
+        return f.vec[sl].clone()
+
     def _velocity_terms(self, dt):
         ub = self.fes.comp_blocks[0]
         return [C.Vol('gradgrad', b, b, coef=dt * self.kv) for b in ub]
@@ -563,6 +594,9 @@ class StokesDIM(Stokes):
 
     def _time_weight(self):
         return (self.phi, 0, L.WEIGHT_FIELD)                 # opencmp/solvers/time_integration_schemes.py:678-681
+
+    def _schur_weight(self):
+        return (self.phi, 0, L.WEIGHT_FIELD)
 
 
 class MultiComponentINS(Model):

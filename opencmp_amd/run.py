@@ -16,10 +16,13 @@ NotImplementedError naming what is missing.
 """
 from __future__ import annotations
 
+import logging
 import math
 import os
+import sys
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional
+from pathlib import Path
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 import torch
@@ -68,6 +71,135 @@ def _bc_dict(bc: SubConfig, section: str) -> Dict[str, Dict[str, str]]:
 
 def _all_markers(mesh) -> List[str]:
     return list(mesh.bnd_names)
+
+
+# ------------------------------------------------------------------ checkpoints (.sol files)
+
+# previous solutions each scheme needs to restart (opencmp/solvers/base_solver.py:56-69)
+SCHEME_HISTORY_ORDER = {'explicit euler': 1, 'implicit euler': 1, 'crank nicolson': 1, 'adaptive two step': 1,
+                        'adaptive three step': 1, 'euler IMEX': 1, 'CNLF': 2, 'SBDF': 3, 'adaptive IMEX': 3,
+                        'RK 222': 1, 'RK 232': 1}
+
+
+def model_file_name(model_name: str) -> str:
+    """Model.name (opencmp/models/base_model.py:648-662): the class name in lower case, without a
+    trailing 'dim'."""
+    n = model_name.lower()
+    return n[:-3] if n.endswith('dim') else n
+
+
+def checkpoint_time(checkpoint: Path, model_name: str) -> float:
+    """opencmp/solvers/base_solver.py:91-113: the time in a '<model>_<time>.sol' file name."""
+    prefix = model_name + '_'
+    if not checkpoint.name.startswith(prefix) or checkpoint.suffix != '.sol':
+        raise ValueError(f'Restart checkpoint "{checkpoint}" must be named "{prefix}<time>.sol" for the active model.')
+    text = checkpoint.name[len(prefix):-len(checkpoint.suffix)]
+    try:
+        t = float(text)
+    except ValueError:
+        raise ValueError(f'Could not determine the restart time from checkpoint "{checkpoint}". Expected the '
+                         f'filename format "{prefix}<time>.sol".')
+    if not math.isfinite(t):
+        raise ValueError(f'Restart checkpoint "{checkpoint}" does not contain a finite time.')
+    return t
+
+
+def select_restart_checkpoint(restart_from: str, run_dir: Path, model_name: str) -> Tuple[Path, float]:
+    """opencmp/solvers/base_solver.py:115-146: an explicit path (relative to run_dir) or LATEST, the
+    latest valid checkpoint of the model under run_dir/output/<model>_sol."""
+    if restart_from.upper() == 'LATEST':
+        out_dir = run_dir / 'output' / (model_name + '_sol')
+        found = []
+        if out_dir.exists():
+            for c in out_dir.rglob(model_name + '_*.sol'):
+                try:
+                    found.append((checkpoint_time(c, model_name), c.resolve()))
+                except ValueError:
+                    continue
+        if not found:
+            raise FileNotFoundError(f'No valid .sol checkpoint for model "{model_name}" was found in "{out_dir}". '
+                                    'Set resume_from_previous to False to start a new simulation.')
+        t, c = max(found, key=lambda item: item[0])
+        return c, t
+    c = Path(restart_from).expanduser()
+    if not c.is_absolute():
+        c = run_dir / c
+    c = c.resolve()
+    if not c.is_file():
+        raise FileNotFoundError(f'The restart checkpoint "{c}" does not exist.')
+    return c, checkpoint_time(c, model_name)
+
+
+def resume_point(cfg: ConfigParser, run_dir: Path) -> Optional[Tuple[Path, float]]:
+    """The resume guard of opencmp/solvers/base_solver.py:158-206, run before anything is built:
+    None for a fresh start, else the checkpoint and its time. Multi-step schemes are refused (their
+    history cannot be rebuilt from saved files); a run that is already complete exits."""
+    resume = cfg.get_item(['OTHER', 'resume_from_previous'], bool)
+    restart_from = cfg.get_item(['OTHER', 'restart_from'], str).strip()
+    if restart_from and not resume:
+        raise ValueError(f'restart_from is set to "{restart_from}", but resume_from_previous is False. Enable '
+                         'resume_from_previous or remove restart_from.')
+    if not resume:
+        return None
+    scheme = cfg.get_item(['TRANSIENT', 'scheme'], str)
+    order = SCHEME_HISTORY_ORDER.get(scheme)
+    if order is None:
+        raise NotImplementedError(f'time scheme {scheme!r}')
+    if order > 1:
+        raise NotImplementedError(f'resume_from_previous only supports single-step time discretization schemes. '
+                                  f'The "{scheme}" scheme needs {order} previous time steps, which cannot be '
+                                  'recovered from the saved .sol files since save_frequency may skip time steps.')
+    if not restart_from:
+        restart_from = 'LATEST'
+        logging.warning('resume_from_previous without restart_from is deprecated and currently means '
+                        'restart_from = LATEST. Set restart_from explicitly in the [OTHER] section.')
+    name = model_file_name(cfg.get_item(['OTHER', 'model'], str))
+    c, t = select_restart_checkpoint(restart_from, run_dir, name)
+    t_end = cfg.get_list(['TRANSIENT', 'time_range'], float)[-1]
+    if t >= t_end or math.isclose(t, t_end, rel_tol=1e-9):
+        print(f'Simulation is already complete at t = {t}, the end of time_range. Nothing to resume.')
+        sys.exit(0)
+    print(f'Resuming from checkpoint: {c}')
+    return c, t
+
+
+class SolSaver:
+    """SolutionFileSaver for .sol files (opencmp/helpers/saving.py:28-93) with the save_frequency
+    rule of opencmp/solvers/base_solver.py:560-583: run_dir/output/<model>_sol/<model>_<t>.sol."""
+
+    def __init__(self, cfg: ConfigParser, run_dir: Path, name: str, t_start: float):
+        self.on = cfg.get_item(['VISUALIZATION', 'save_to_file'], bool)
+        if not self.on:
+            logging.warning('save_to_file is set to False — no solution files will be written for this run.')
+            return
+        kind = cfg.get_item(['VISUALIZATION', 'save_type'], str)
+        if kind != '.sol':
+            raise NotImplementedError(f'save_type {kind!r} (the GPU path writes .sol files)')
+        freq = cfg.get_list(['VISUALIZATION', 'save_frequency'], str)
+        self.freq = (float(freq[0]), freq[1])
+        if self.freq[1] not in ('time', 'numit'):
+            raise ValueError('Save frequency must be specified either per amount of time ("time") or per number '
+                             'of iterations ("numit").')
+        self.dir = run_dir / 'output' / (name + '_sol')
+        self.dir.mkdir(parents=True, exist_ok=True)
+        self.base = name + '_'
+        self.t_start = t_start
+
+    def save(self, vec, t: float) -> None:
+        from .io import save_sol
+        save_sol(str(self.dir / f'{self.base}{t}.sol'), vec)
+
+    def after_step(self, vec, t: float, num_iters: int) -> None:
+        if not self.on:
+            return
+        if self.freq[1] == 'numit':
+            if num_iters % self.freq[0] == 0:
+                self.save(vec, t)
+        else:
+            # a save time is a multiple of the period after the start (up to round-off)
+            k = (t - self.t_start) / self.freq[0]
+            if math.isclose(k, round(k), abs_tol=1e-8):
+                self.save(vec, t)
 
 
 # ------------------------------------------------------------------ model construction
@@ -229,18 +361,30 @@ def run(config_file_path: str, config_parser: Optional[ConfigParser] = None, roo
     (mesh file, run_dir) are taken from; the reference takes them from the working directory."""
     from .io import load_mesh
     from . import solvers as S
+    from .io import load_sol
     cfg = config_parser if config_parser is not None else ConfigParser(config_file_path)
     paths = run_paths(cfg, root)
+    run_dir = Path(paths['run_dir']).resolve()
+    restart = resume_point(cfg, run_dir)          # before anything is built, like the reference's solver
     mesh = load_mesh(paths['mesh'])
     bc, ic, mcfg, ref = (SubConfig(paths[k]) for k in ('bc', 'ic', 'model', 'ref_sol'))
     transient = cfg.get_item(['TRANSIENT', 'transient'], bool)
     t0, t1 = [float(v) for v in cfg.get_list(['TRANSIENT', 'time_range'], float)]
+    if restart is not None:
+        t0 = restart[1]
     dt = cfg.get_item(['TRANSIENT', 'dt'], float)
     t_param = [E.Parameter(t0 - k * dt) for k in range(3)]
     model = build_model(cfg, mesh, bc, mcfg, t_param, ctx=ctx, ic=ic)
     name = cfg.get_item(['OTHER', 'model'], str)
     gfu0 = C.GridFunction(model.fes)
-    _initial_condition(model, ic, name, cfg, gfu0)
+    if restart is not None:
+        # the checkpoint replaces the configured initial condition (opencmp/models/base_model.py:216-221)
+        gfu0.vec.copy_(torch.from_numpy(load_sol(str(restart[0]), model.fes.ndof)).to(gfu0.vec.device))
+    else:
+        _initial_condition(model, ic, name, cfg, gfu0)
+    saver = SolSaver(cfg, run_dir, model_file_name(name), t0)
+    if saver.on:
+        saver.save(gfu0.vec, t0)
     if hasattr(model, 'set_linearisation_point'):
         model.set_linearisation_point(gfu0)
     lines: List[str] = []
@@ -256,7 +400,7 @@ def run(config_file_path: str, config_parser: Optional[ConfigParser] = None, roo
     else:
         scheme = cfg.get_item(['TRANSIENT', 'scheme'], str)
         solver = S.TransientMultiStepSolver(model, scheme, dt, (t0, t1), t_param, gfu0)
-        gfu = solver.solve()
+        gfu = solver.solve(on_step=lambda s_: saver.after_step(s_.gfu.vec, s_.t_param[0].value, s_.steps))
         steps, its = solver.steps, solver.linear_iterations
     errors = []
     if cfg.get_item(['ERROR ANALYSIS', 'check_error'], bool):

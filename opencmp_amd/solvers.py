@@ -286,11 +286,14 @@ class TransientMultiStepSolver:
         self.steps += 1
         return res
 
-    def solve(self) -> C.GridFunction:
+    def solve(self, on_step=None) -> C.GridFunction:
+        """Steps to the end time; `on_step(self)` after every step (solution saving)."""
         import numpy as np
         # opencmp/solvers/base_solver.py:815: step while t < t_end and t is not close to t_end
         while self.t_param[0].value < self.t1 and not np.isclose(self.t_param[0].value, self.t1):
             self.step(min(self.dt0, self.t1 - self.t_param[0].value))
+            if on_step is not None:
+                on_step(self)
         return self.gfu
 
 
