@@ -420,7 +420,7 @@ def bench_unet(args, ctx):
     from opencmp_amd.learned import UNetPrecond, residual_samples, residual_norms
     from opencmp_amd.mesh import unit_cube, sphere_obstacles, obstacle_phi
     from opencmp_amd.models import StokesDIM, const
-    n = args.n or 32
+    n = args.n or 64
     t0 = time.perf_counter()
     hm = unit_cube(n)
     phi = obstacle_phi(hm.coords, sphere_obstacles(), 1.0 / n)
@@ -445,7 +445,7 @@ def bench_unet(args, ctx):
         d = m64.schur_diagonal()
         for p_ in (pre64, pre32):
             p_.set_block_triangular(d, args.vel_iters, args.vel_tol)
-        if args.bf16:
+        if not args.no_bf16:
             pre32.set_bf16(True)
     ndof = m64.fes.ndof
     log(f'config 5 setup {time.perf_counter() - t0:.1f}s: {hm.nel} tets, ndof {ndof}, nnz {a.nnz}')
@@ -529,7 +529,7 @@ def bench_unet(args, ctx):
     err = float(torch.linalg.norm(uml - u64) / torch.linalg.norm(u64))
     pdesc = (f'block-triangular preconditioner (lumped phi-weighted pressure mass, inner Jacobi-PCG on the '
              f'velocity block to {args.vel_tol:g}, at most {args.vel_iters} iterations'
-             f'{", BF16-stored in the mixed leg" if args.bf16 else ""})' if args.vel_iters else
+             f'{", BF16-stored in the mixed leg" if not args.no_bf16 else ""})' if args.vel_iters else
              'Schur-diagonal preconditioner')
     return dict(desc=(f'diffuse-interface Stokes, unit cube {n}^3 x 6 tets ({hm.nel} tets), Taylor-Hood P2/P1, '
                       f'20 spheres, penalty 1e4, FGMRES({args.restart}) + {pdesc}'),
@@ -546,10 +546,11 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--config', type=int, default=4, choices=(1, 2, 3, 4, 5))
     ap.add_argument('--no-extra', action='store_true', help='config 4: skip the config-2 extra keys')
-    ap.add_argument('--vel-iters', type=int, default=0,
-                    help='config 5: block-triangular preconditioner with at most this many velocity PCG iterations')
-    ap.add_argument('--vel-tol', type=float, default=1e-2, help='config 5: velocity PCG reduction per step')
-    ap.add_argument('--bf16', action='store_true', help='config 5: BF16-stored velocity block in the mixed leg')
+    ap.add_argument('--vel-iters', type=int, default=2000,
+                    help='config 5: block-triangular preconditioner with at most this many velocity PCG iterations '
+                         '(0: the Schur-diagonal preconditioner, which stalls from 64^3 on)')
+    ap.add_argument('--vel-tol', type=float, default=1e-3, help='config 5: velocity PCG reduction per step')
+    ap.add_argument('--no-bf16', action='store_true', help='config 5: FP32 (not BF16) velocity block in the mixed leg')
     ap.add_argument('--unet-samples', type=int, default=24, help='config 5: residual samples for training')
     ap.add_argument('--record', type=int, default=40, help='config 4: plain steps recorded for training')
     ap.add_argument('--verbose-steps', action='store_true', help='config 4: log every step of the timed legs')
@@ -560,8 +561,8 @@ def main():
                          'per FGMRES step (0: the Schur diagonal under GMRES)')
     ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
     ap.add_argument('--n', type=int, default=None,
-                    help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 32; config 5 at 64^3 does not '
-                         'converge with the diagonal Schur preconditioner)')
+                    help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 64; config 5 at 64^3 needs the '
+                         'block-triangular preconditioner, the default)')
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=None,
                     help='GMRES restart (defaults: 50; 200 for config 5, whose diffuse-interface saddle point '
@@ -578,7 +579,7 @@ def main():
         print('bench.py measures one GPU only: this assemble+solve path is not sharded across GPUs '
               '(see the sharding design); run with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 32}[args.config]
+    n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 64}[args.config]
     if args.restart is None:
         args.restart = 200 if (args.config == 5 and not args.vel_iters) else 50
     import torch
@@ -612,14 +613,7 @@ def main():
                               'fallback': r['rml'].fallback, 'rel_residual': r['rml'].rel_residual,
                               'rel_error_vs_fp64': r['err']},
             'mixed_plain': {'s_per_step': round(r['smx'], 5), 'iters': r['rmx'].iters},
-            'mixed_plain': {'s_per_step': round(r['smp'], 5), 'median_s_per_step': round(r['medmp'], 5),
-                            'outer_iters_per_step': [row[0] for row in r['rowsmp']],
-                            'first_picard_outer_iters': [row[3][0][0] for row in r['rowsmp']]},
             'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
-            'learned_gain': {'s_per_step_mixed_plain_over_learned': round(r['smp'] / r['sml'], 3),
-                             'outer_iters_plain': sum(row[0] for row in r['rowsmp']),
-                             'outer_iters_learned': sum(row[0] for row in r['rowsml']),
-                             'first_picard_outer_iters_learned': [row[3][0][0] for row in r['rowsml']]},
             'dofs_per_s': round(r['ndof'] / r['sml'], 1),
             'learned': r['learned'],
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
@@ -652,7 +646,14 @@ def main():
             'mixed_learned': {'s_per_step': round(r['sml'], 5), 'median_s_per_step': round(r['medml'], 5),
                               'outer_iters_per_step': itml, 'iters_picard_res': r['rowsml'],
                               'rel_error_vs_fp64': r['err']},
+            'mixed_plain': {'s_per_step': round(r['smp'], 5), 'median_s_per_step': round(r['medmp'], 5),
+                            'outer_iters_per_step': [row[0] for row in r['rowsmp']],
+                            'first_picard_outer_iters': [row[3][0][0] for row in r['rowsmp']]},
             'speedup_mixed_learned_vs_fp64': round(r['s64'] / r['sml'], 3),
+            'learned_gain': {'s_per_step_mixed_plain_over_learned': round(r['smp'] / r['sml'], 3),
+                             'outer_iters_plain': sum(row[0] for row in r['rowsmp']),
+                             'outer_iters_learned': sum(row[0] for row in r['rowsml']),
+                             'first_picard_outer_iters_learned': [row[3][0][0] for row in r['rowsml']]},
             'dofs_per_s': round(r['ndof'] / r['sml'], 1),
             'learned': r['learned'],
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,

@@ -68,7 +68,8 @@ class Model:
                  linear_solver: str = 'CG', preconditioner: str = 'default', linear_tolerance: float = 1e-10,
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
                  inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
-                 t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0):
+                 t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0,
+                 bf16_precond: bool = False):
         self.ctx = ctx or C.Context.default()
         # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
         # expressions of t (boundary data, sources) are read at the level a form is built for
@@ -97,6 +98,11 @@ class Model:
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
         self.vel_iters = vel_iters if self.preconditioner == 'block_triangular' else 0
         self.vel_tol = vel_tol
+        # mixed precision: keep the inner solves' matrices (Schur S, the velocity block) in BF16. Suits a
+        # strongly diagonally dominant block (the diffuse-interface velocity block); a Laplacian-like
+        # S (the INS channel's pressure Schur complement at 1.8M dofs) can lose definiteness in BF16
+        # and stall its inner PCG
+        self.bf16_precond = bool(bf16_precond) and precision == 'mixed'
         if self.preconditioner == 'block_triangular':
             if not self.vel_iters:
                 raise ValueError("preconditioner 'block_triangular' needs vel_iters > 0")
@@ -172,12 +178,17 @@ class Model:
         if self.preconditioner in (None, 'None'):
             return None
         if self.preconditioner == 'schur':
-            return C.Preconditioner(a_assembled, 'schur', precision=self.precision,
-                                    schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters,
-                                    schur_tol=self.schur_tol)
+            pre = C.Preconditioner(a_assembled, 'schur', precision=self.precision,
+                                   schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters,
+                                   schur_tol=self.schur_tol)
+            if self.bf16_precond:
+                pre.set_bf16(True)
+            return pre
         if self.preconditioner == 'block_triangular':
             pre = C.Preconditioner(a_assembled, 'schur', precision=self.precision, schur_blocks=self.schur_blocks())
             pre.set_block_triangular(self.schur_diagonal(), self.vel_iters, self.vel_tol)
+            if self.bf16_precond:
+                pre.set_bf16(True)
             return pre
         return C.Preconditioner(a_assembled, self.preconditioner, precision=self.precision)
 

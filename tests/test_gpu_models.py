@@ -313,6 +313,17 @@ def test_boundary_terms_on_tetrahedra(gpu_ctx):
     vals = [float(v @ (A @ v)) for v in (one, x, y)]
     say('  no-backflow, wind (1,0,0): u=1, x, y ->', vals)
     assert np.allclose(vals, [1.0, 1.0, 1.0 / 3.0], rtol=0, atol=1e-12)
+    # the wind as a P2 field on the faces (field_at_facet on tetrahedra): w = (x, 0, z^2); w.n is x on
+    # x = 1, -x (= 0) on x = 0, z^2 on z = 1 and -z^2 (= 0) on z = 0, so for u = 1 the term is
+    # int_{x=1} 1 + int_{z=1} 1 = 2 and for u = y it is 1/3 + 1/3
+    ws = C.FESpace(gm, [C.H1(2, '', ncomp=3)])
+    W = C.GridFunction(ws)
+    W.Set([(L.FN_POLY2, [0, 1] + [0] * 8), const(0.0), (L.FN_POLY2, [0, 0, 0, 0, 0, 0, 1, 0, 0, 0])], names,
+          component=0)
+    A = form(C.BndUpwind(0, allm, coef=1.0, wind_gf=(W, [0, 1, 2])))
+    vals = [float(v @ (A @ v)) for v in (one, y)]
+    say('  no-backflow, wind field (x, 0, z^2): u=1, y ->', vals)
+    assert np.allclose(vals, [2.0, 2.0 / 3.0], rtol=0, atol=1e-12)
     f = C.LinearForm(sp, [C.BndSource(0, right, fn=(L.FN_POLY2, [0, 0, 1, 0, 0, 0, 0, 0, 0, 0]), coef=2.0)])
     f.Assemble()
     b = f.vec.cpu().numpy()
