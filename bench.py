@@ -318,7 +318,7 @@ def bench_learned(args, ctx):
                    linear_max_iterations=args.maxit,
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
                    inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step,
-                   schur_iters=args.schur_iters, schur_tol=args.schur_tol, ctx=ctx)
+                   schur_iters=args.schur_iters, schur_tol=args.schur_tol, schur_amg=args.schur_amg, ctx=ctx)
 
     m0 = model('fp64')
     ndof = m0.fes.ndof
@@ -395,7 +395,9 @@ def bench_learned(args, ctx):
     return dict(desc=(f'transient INS Taylor-Hood P2/P1, implicit Euler dt 1e-3 + Oseen Picard (<=3, tol 1e-6), '
                       f'channel [0,4]x[0,1] with 8 holes, n={n} ({hm.nel} triangles), do-nothing (stress) outlet '
                       f'with the no-backflow term, ' +
-                      (f'FGMRES({args.restart}) + Schur preconditioner with inner Jacobi-PCG on '
+                      (f'FGMRES({args.restart}) + Schur preconditioner with {args.schur_amg} smoothed-aggregation '
+                       f'AMG V-cycle(s) on C + B diag(A)^-1 B^T' if args.schur_amg else
+                       f'FGMRES({args.restart}) + Schur preconditioner with inner Jacobi-PCG on '
                        f'C + B diag(A)^-1 B^T (to {args.schur_tol:g}, at most {args.schur_iters} iterations)'
                        if args.schur_iters else
                        f'GMRES({args.restart}) + Schur-diagonal preconditioner')),
@@ -560,6 +562,9 @@ def main():
                     help='config 4: at most this many inner PCG iterations on the pressure Schur complement '
                          'per FGMRES step (0: the Schur diagonal under GMRES)')
     ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
+    ap.add_argument('--schur-amg', type=int, default=0,
+                    help='config 4: smoothed-aggregation AMG V-cycles on the Schur matrix per FGMRES step in '
+                         'place of the inner PCG (0: the inner PCG)')
     ap.add_argument('--n', type=int, default=None,
                     help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 64; config 5 at 64^3 needs the '
                          'block-triangular preconditioner, the default)')

@@ -371,6 +371,17 @@ int ocmp_precond_set_block_triangular(ocmp_precond *P, const double *schur_diag,
  * that differs from the exact inner solve. No reference counterpart (NGSolve
  * runs in FP64). on = 0 turns it off; the copies are made at the next update. */
 int ocmp_precond_set_bf16(ocmp_precond *P, int on);
+/* Schur AMG step for a preconditioner from ocmp_precond_create_schur (the same
+ * replacement of the reference's direct factorisation as ocmp_precond_set_schur_solve,
+ * opencmp/models/ins.py:178-270 with opencmp/solvers/base_solver.py:711-719): the
+ * flexible step of OCMP_SOLVER_FGMRES applies vcycles V(1,1) cycles of smoothed-
+ * aggregation algebraic multigrid on S = C + B D^{-1} B^T to the Schur rows (the sign
+ * of the Schur diagonal, i.e. -S^{-1}) instead of the inner PCG; the velocity rows
+ * keep 1/a_ii. The hierarchy (aggregates, prolongations, Galerkin patterns) is built
+ * on the host at the next ocmp_precond_update from S's values then, and its values
+ * are recomputed on the device at every update. The step is a fixed linear operator.
+ * Replaces the Schur inner solve (turn that off first); vcycles = 0 turns it off. */
+int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);

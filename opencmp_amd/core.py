@@ -519,13 +519,15 @@ class Preconditioner:
     KINDS = {'local': None, 'jacobi': L.PRE_JACOBI, 'block_jacobi': L.PRE_BLOCK_JACOBI, 'none': L.PRE_NONE}
 
     def __init__(self, a: BilinearForm, kind: str = 'local', precision: str = 'fp64',
-                 schur_blocks: Sequence[int] = (), schur_iters: int = 0, schur_tol: float = 0.0):
+                 schur_blocks: Sequence[int] = (), schur_iters: int = 0, schur_tol: float = 0.0,
+                 schur_amg: int = 0):
         """kind 'schur': point Jacobi on all rows except those of `schur_blocks` (the
         pressure blocks of a saddle-point system), which get the diagonal of the
         Schur complement (ocmp_precond_create_schur); with `schur_iters` > 0 FGMRES
         solves those rows by at most that many PCG iterations on C + B diag(A)^{-1} B^T
         instead, fewer once the preconditioned residual has fallen by `schur_tol`
-        (ocmp_precond_set_schur_solve)."""
+        (ocmp_precond_set_schur_solve); with `schur_amg` > 0 FGMRES applies that many smoothed-
+        aggregation AMG V-cycles on the same S instead (ocmp_precond_set_schur_amg)."""
         self.a = a
         self.precision = L.MIXED if precision == 'mixed' else L.FP64
         h = C.c_void_p()
@@ -539,6 +541,8 @@ class Preconditioner:
             if schur_iters:
                 L.check(a.ctx.lib.ocmp_precond_set_schur_solve(h, int(schur_iters), float(schur_tol)),
                         'precond_set_schur_solve')
+            if schur_amg:
+                L.check(a.ctx.lib.ocmp_precond_set_schur_amg(h, int(schur_amg)), 'precond_set_schur_amg')
         else:
             k = self.KINDS[kind]
             if k is None:

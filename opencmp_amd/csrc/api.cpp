@@ -3,6 +3,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <array>
 #include <cstring>
 #include <numeric>
@@ -629,6 +630,375 @@ void build_schur_matrix(Precond *P) {
 // [s0, n) shifted to 0) of a system whose Schur rows are the last ones: each row's columns are
 // sorted, so its A_uu part is a prefix and its A_up part the rest. Values are gathered through
 // the slot arrays at every update.
+// ------------------------------------------------------------------ AMG hierarchy on S (host)
+//
+// Smoothed aggregation (Vanek, Mandel and Brezina, Computing 56 (1996) 179-196): strength
+// |a_ij| >= theta sqrt(|a_ii a_jj|); aggregates by the three greedy passes (roots whose strong
+// neighbours are all free take them; the rest join a neighbouring first-pass aggregate; leftovers
+// form their own); tentative prolongation = aggregate indicator, smoothed once by damped Jacobi,
+// P = (I - omega D^-1 A) P_t with omega = 4 / (3 lambda_max(D^-1 A)); Galerkin A_c = P^T A P.
+// Built once with the first values of S; the device recomputes every value from the lists at each
+// update, so the aggregates and patterns stay while the values follow the operator.
+
+namespace {
+
+constexpr double AMG_THETA = 0.08;
+constexpr int64_t AMG_COARSE = 300;   // dense inverse on the last level at or below this many rows
+constexpr int AMG_MAX_LEVELS = 10;
+
+struct HCsr {
+  int64_t n = 0;
+  std::vector<int32_t> rp, col;
+  std::vector<double> val;
+};
+
+Matrix *amg_matrix(Matrix *like, const HCsr &h) {
+  auto *M = new Matrix();
+  M->space = like->space;
+  M->ctx = like->ctx;
+  M->n = h.n;
+  M->row_ptr = h.rp;
+  M->col = h.col;
+  M->nnz = (int64_t)h.col.size();
+  hipStream_t st = like->ctx->stream;
+  M->d_row_ptr.upload(M->row_ptr, st);
+  M->d_col.upload(M->col, st);
+  M->d_val.alloc(std::max<int64_t>(M->nnz, 1));
+  build_row_blocks(M, M->rb64, SPMV_NNZB64);
+  M->assembled = true;
+  return M;
+}
+
+// largest eigenvalue of D^-1 A (A symmetric, D its diagonal) on the usable rows, by power iteration
+double amg_lambda_max(const HCsr &A, const std::vector<double> &dg, const std::vector<uint8_t> &use) {
+  const int64_t n = A.n;
+  std::vector<double> x(n), y(n);
+  for (int64_t i = 0; i < n; ++i) x[i] = use[i] ? 1.0 + 0.5 * std::sin(0.7 * (double)i) : 0.0;
+  double lam = 1.0;
+  for (int it = 0; it < 30; ++it) {
+    double nn = 0.0, dot = 0.0;
+#pragma omp parallel for reduction(+ : nn, dot) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      if (use[i] && dg[i] != 0.0)
+        for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+          if (use[A.col[k]]) s += A.val[k] * x[A.col[k]];
+      y[i] = use[i] && dg[i] != 0.0 ? s / dg[i] : 0.0;
+      nn += y[i] * y[i];
+      dot += y[i] * x[i];
+    }
+    double xx = 0.0;
+    for (int64_t i = 0; i < n; ++i) xx += x[i] * x[i];
+    lam = xx > 0.0 ? dot / xx : 1.0;
+    const double inv = nn > 0.0 ? 1.0 / std::sqrt(nn) : 0.0;
+    for (int64_t i = 0; i < n; ++i) x[i] = y[i] * inv;
+  }
+  return std::max(lam, 1e-300);
+}
+
+}  // namespace
+
+Amg::~Amg() {
+  for (auto &L : lv) {
+    if (L->own_A) delete L->A;
+    delete L->P;
+    delete L->R;
+  }
+}
+
+Precond::~Precond() {
+  delete S;
+  delete Auu;
+  delete Aup;
+  delete amg;
+}
+
+void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::vector<double> &sdinv) {
+  Matrix *S = Pc->S;
+  hipStream_t st = S->ctx->stream;
+  auto *amg = new Amg();
+  HCsr A;
+  A.n = S->n;
+  A.rp = S->row_ptr;
+  A.col = S->col;
+  A.val = sval;
+  std::vector<uint8_t> use(A.n);
+  for (int64_t i = 0; i < A.n; ++i) use[i] = sdinv[i] != 0.0;
+  Matrix *Adev = S;
+  for (int l = 0;; ++l) {
+    auto Lp = std::make_unique<AmgLevel>();
+    AmgLevel &L = *Lp;
+    L.A = Adev;
+    L.own_A = l > 0;
+    const int64_t n = A.n;
+    if (n <= AMG_COARSE || l == AMG_MAX_LEVELS - 1) {
+      OCMP_REQUIRE(n <= 4 * AMG_COARSE, "AMG: the coarsest level is too large for its dense inverse");
+      amg->lv.push_back(std::move(Lp));
+      amg->ncoarse = n;
+      break;
+    }
+    std::vector<double> dg(n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t *c0 = A.col.data() + A.rp[i], *c1 = A.col.data() + A.rp[i + 1];
+      const int32_t *q = std::lower_bound(c0, c1, (int32_t)i);
+      if (q != c1 && *q == i) dg[i] = A.val[q - A.col.data()];
+    }
+    auto strong = [&](int64_t i, int32_t k) {
+      const int32_t j = A.col[k];
+      return j != i && use[j] && std::fabs(A.val[k]) >= AMG_THETA * std::sqrt(std::fabs(dg[i] * dg[j]));
+    };
+    std::vector<int32_t> agg(n, -1);
+    int32_t na = 0;
+    for (int64_t i = 0; i < n; ++i) {  // pass 1: roots with all strong neighbours free
+      if (!use[i] || agg[i] >= 0) continue;
+      bool ok = true;
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1] && ok; ++k)
+        if (strong(i, k) && agg[A.col[k]] >= 0) ok = false;
+      if (!ok) continue;
+      agg[i] = na;
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+        if (strong(i, k)) agg[A.col[k]] = na;
+      ++na;
+    }
+    const std::vector<int32_t> agg1 = agg;
+    for (int64_t i = 0; i < n; ++i) {  // pass 2: join a strongly connected first-pass aggregate
+      if (!use[i] || agg[i] >= 0) continue;
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+        if (strong(i, k) && agg1[A.col[k]] >= 0) {
+          agg[i] = agg1[A.col[k]];
+          break;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) {  // pass 3: the rest form aggregates of their own
+      if (!use[i] || agg[i] >= 0) continue;
+      agg[i] = na;
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+        if (strong(i, k) && agg[A.col[k]] < 0) agg[A.col[k]] = na;
+      ++na;
+    }
+    if (na == 0 || na > (int64_t)(0.9 * (double)n)) {
+      OCMP_REQUIRE(n <= 4 * AMG_COARSE, "AMG: coarsening stalled above the dense-inverse size");
+      amg->lv.push_back(std::move(Lp));
+      amg->ncoarse = n;
+      break;
+    }
+    const double omega = 4.0 / (3.0 * amg_lambda_max(A, dg, use));
+    L.omega = omega;
+    // P: row i (usable) has its own aggregate and its usable neighbours' aggregates; each entry
+    // sums the A entries of row i whose column lies in that aggregate
+    HCsr Ph;
+    Ph.n = n;
+    Ph.rp.assign(n + 1, 0);
+    std::vector<std::vector<int32_t>> pc(n), pl(n), pll(n);  // columns, list lengths, A slots
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      if (!use[i]) continue;
+      std::vector<std::pair<int32_t, int32_t>> t;  // (aggregate, A slot)
+      t.emplace_back(agg[i], -1);
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const int32_t j = A.col[k];
+        if (use[j] && agg[j] >= 0) t.emplace_back(agg[j], k);
+      }
+      std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+      for (size_t a = 0; a < t.size();) {
+        size_t b = a;
+        int32_t cnt = 0;
+        while (b < t.size() && t[b].first == t[a].first) {
+          if (t[b].second >= 0) {
+            pll[i].push_back(t[b].second);
+            ++cnt;
+          }
+          ++b;
+        }
+        pc[i].push_back(t[a].first);
+        pl[i].push_back(cnt);
+        a = b;
+      }
+    }
+    for (int64_t i = 0; i < n; ++i) Ph.rp[i + 1] = Ph.rp[i] + (int32_t)pc[i].size();
+    const int64_t np = Ph.rp[n];
+    Ph.col.resize(np);
+    Ph.val.resize(np);
+    std::vector<int32_t> prow(np), plptr(np + 1, 0), plist;
+    std::vector<uint8_t> pident(np);
+    for (int64_t i = 0; i < n; ++i)
+      for (size_t e = 0; e < pc[i].size(); ++e) plptr[Ph.rp[i] + e + 1] = pl[i][e];
+    for (int64_t e = 0; e < np; ++e) plptr[e + 1] += plptr[e];
+    plist.resize(plptr[np]);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      size_t off = 0;
+      for (size_t e = 0; e < pc[i].size(); ++e) {
+        const int64_t pe = Ph.rp[i] + e;
+        Ph.col[pe] = pc[i][e];
+        prow[pe] = (int32_t)i;
+        pident[pe] = pc[i][e] == agg[i] ? 1 : 0;
+        double sum = 0.0;
+        for (int32_t t = 0; t < pl[i][e]; ++t) {
+          const int32_t k = pll[i][off + t];
+          plist[plptr[pe] + t] = k;
+          sum += A.val[k];
+        }
+        off += pl[i][e];
+        Ph.val[pe] = (double)pident[pe] - (dg[i] != 0.0 ? omega / dg[i] : 0.0) * sum;
+      }
+    }
+    std::vector<std::vector<int32_t>>().swap(pc);
+    std::vector<std::vector<int32_t>>().swap(pl);
+    std::vector<std::vector<int32_t>>().swap(pll);
+    // R = P^T (counting sort by column), with each entry's P slot
+    HCsr Rh;
+    Rh.n = na;
+    Rh.rp.assign(na + 1, 0);
+    for (int64_t e = 0; e < np; ++e) Rh.rp[Ph.col[e] + 1]++;
+    for (int32_t I = 0; I < na; ++I) Rh.rp[I + 1] += Rh.rp[I];
+    Rh.col.resize(np);
+    Rh.val.resize(np);
+    std::vector<int32_t> rslot(np);
+    {
+      std::vector<int32_t> fill(Rh.rp.begin(), Rh.rp.end() - 1);
+      for (int64_t i = 0; i < n; ++i)
+        for (int32_t e = Ph.rp[i]; e < Ph.rp[i + 1]; ++e) {
+          const int32_t k = fill[Ph.col[e]]++;
+          Rh.col[k] = (int32_t)i;
+          Rh.val[k] = Ph.val[e];
+          rslot[k] = e;
+        }
+    }
+    // Q = A P: entry (i, J) sums A[i,j] P[j,J] over the usable columns j of row i
+    HCsr Qh;
+    Qh.n = n;
+    Qh.rp.assign(n + 1, 0);
+    std::vector<std::vector<int32_t>> qc(n), qn(n), qa(n), qp(n);
+#pragma omp parallel for schedule(dynamic, 512)
+    for (int64_t i = 0; i < n; ++i) {
+      std::vector<std::array<int32_t, 3>> t;  // (J, A slot, P slot)
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const int32_t j = A.col[k];
+        for (int32_t e = Ph.rp[j]; e < Ph.rp[j + 1]; ++e) t.push_back({Ph.col[e], k, e});
+      }
+      std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a[0] < b[0]; });
+      for (size_t a = 0; a < t.size();) {
+        size_t b = a;
+        while (b < t.size() && t[b][0] == t[a][0]) {
+          qa[i].push_back(t[b][1]);
+          qp[i].push_back(t[b][2]);
+          ++b;
+        }
+        qc[i].push_back(t[a][0]);
+        qn[i].push_back((int32_t)(b - a));
+        a = b;
+      }
+    }
+    for (int64_t i = 0; i < n; ++i) Qh.rp[i + 1] = Qh.rp[i] + (int32_t)qc[i].size();
+    const int64_t nq = Qh.rp[n];
+    Qh.col.resize(nq);
+    Qh.val.assign(nq, 0.0);
+    std::vector<int64_t> qptr(nq + 1, 0);
+    for (int64_t i = 0; i < n; ++i)
+      for (size_t e = 0; e < qc[i].size(); ++e) qptr[Qh.rp[i] + e + 1] = qn[i][e];
+    for (int64_t e = 0; e < nq; ++e) qptr[e + 1] += qptr[e];
+    OCMP_REQUIRE(qptr[nq] < (int64_t)INT32_MAX, "AMG: Q pair lists exceed 32-bit offsets");
+    std::vector<int32_t> qptr32(nq + 1), qa_all(qptr[nq]), qp_all(qptr[nq]);
+    for (int64_t e = 0; e <= nq; ++e) qptr32[e] = (int32_t)qptr[e];
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      size_t off = 0;
+      for (size_t e = 0; e < qc[i].size(); ++e) {
+        const int64_t qe = Qh.rp[i] + e;
+        Qh.col[qe] = qc[i][e];
+        double sum = 0.0;
+        for (int32_t t = 0; t < qn[i][e]; ++t) {
+          qa_all[qptr[qe] + t] = qa[i][off + t];
+          qp_all[qptr[qe] + t] = qp[i][off + t];
+          sum += A.val[qa[i][off + t]] * Ph.val[qp[i][off + t]];
+        }
+        off += qn[i][e];
+        Qh.val[qe] = sum;
+      }
+    }
+    std::vector<std::vector<int32_t>>().swap(qc);
+    std::vector<std::vector<int32_t>>().swap(qn);
+    std::vector<std::vector<int32_t>>().swap(qa);
+    std::vector<std::vector<int32_t>>().swap(qp);
+    // A_c = P^T Q: entry (I, J) sums P[i,I] Q[i,J] over the fine rows i of R's row I
+    HCsr C;
+    C.n = na;
+    C.rp.assign(na + 1, 0);
+    std::vector<std::vector<int32_t>> cc(na), cn(na), cp(na), cq(na);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t I = 0; I < na; ++I) {
+      std::vector<std::array<int32_t, 3>> t;  // (J, P slot, Q slot)
+      for (int32_t k = Rh.rp[I]; k < Rh.rp[I + 1]; ++k) {
+        const int32_t i = Rh.col[k], e = rslot[k];
+        for (int32_t q = Qh.rp[i]; q < Qh.rp[i + 1]; ++q) t.push_back({Qh.col[q], e, q});
+      }
+      std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a[0] < b[0]; });
+      for (size_t a = 0; a < t.size();) {
+        size_t b = a;
+        while (b < t.size() && t[b][0] == t[a][0]) {
+          cp[I].push_back(t[b][1]);
+          cq[I].push_back(t[b][2]);
+          ++b;
+        }
+        cc[I].push_back(t[a][0]);
+        cn[I].push_back((int32_t)(b - a));
+        a = b;
+      }
+    }
+    for (int64_t I = 0; I < na; ++I) C.rp[I + 1] = C.rp[I] + (int32_t)cc[I].size();
+    const int64_t nc = C.rp[na];
+    C.col.resize(nc);
+    C.val.assign(nc, 0.0);
+    std::vector<int64_t> cptr(nc + 1, 0);
+    for (int64_t I = 0; I < na; ++I)
+      for (size_t e = 0; e < cc[I].size(); ++e) cptr[C.rp[I] + e + 1] = cn[I][e];
+    for (int64_t e = 0; e < nc; ++e) cptr[e + 1] += cptr[e];
+    OCMP_REQUIRE(cptr[nc] < (int64_t)INT32_MAX, "AMG: Galerkin pair lists exceed 32-bit offsets");
+    std::vector<int32_t> cptr32(nc + 1), cp_all(cptr[nc]), cq_all(cptr[nc]);
+    for (int64_t e = 0; e <= nc; ++e) cptr32[e] = (int32_t)cptr[e];
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t I = 0; I < na; ++I) {
+      size_t off = 0;
+      for (size_t e = 0; e < cc[I].size(); ++e) {
+        const int64_t ce = C.rp[I] + e;
+        C.col[ce] = cc[I][e];
+        double sum = 0.0;
+        for (int32_t t = 0; t < cn[I][e]; ++t) {
+          cp_all[cptr[ce] + t] = cp[I][off + t];
+          cq_all[cptr[ce] + t] = cq[I][off + t];
+          sum += Ph.val[cp[I][off + t]] * Qh.val[cq[I][off + t]];
+        }
+        off += cn[I][e];
+        C.val[ce] = sum;
+      }
+    }
+    // device copies of the level's structure
+    L.P = amg_matrix(S, Ph);
+    L.R = amg_matrix(S, Rh);
+    L.nq = nq;
+    L.d_prow.upload(prow, st);
+    L.d_plptr.upload(plptr, st);
+    L.d_plist.upload(plist.empty() ? std::vector<int32_t>(1, 0) : plist, st);
+    L.d_pident.upload(pident, st);
+    L.d_rslot.upload(rslot, st);
+    L.d_qptr.upload(qptr32, st);
+    L.d_qa.upload(qa_all.empty() ? std::vector<int32_t>(1, 0) : qa_all, st);
+    L.d_qp.upload(qp_all.empty() ? std::vector<int32_t>(1, 0) : qp_all, st);
+    L.d_cptr.upload(cptr32, st);
+    L.d_cp.upload(cp_all.empty() ? std::vector<int32_t>(1, 0) : cp_all, st);
+    L.d_cq.upload(cq_all.empty() ? std::vector<int32_t>(1, 0) : cq_all, st);
+    L.q.alloc(std::max<int64_t>(nq, 1));
+    OCMP_HIP(hipStreamSynchronize(st));  // the host vectors end with this iteration
+    amg->lv.push_back(std::move(Lp));
+    Adev = amg_matrix(S, C);
+    A = std::move(C);
+    use.assign(A.n, 1);
+  }
+  Pc->amg = amg;
+}
+
 void build_velocity_blocks(Precond *P) {
   Matrix *A = P->A;
   int64_t s0, s1;
@@ -1395,6 +1765,18 @@ int ocmp_precond_set_bf16(ocmp_precond *P, int on) {
     OCMP_REQUIRE(P->P.precision == OCMP_MIXED || !on, "BF16 storage is for preconditioners built with OCMP_MIXED");
     if ((on != 0) != P->P.bf16) P->P.ready = false;  // the copies are made by the next update
     P->P.bf16 = on != 0;
+  });
+}
+
+int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.schur_blocks != 0, "the Schur AMG step needs a Schur preconditioner");
+    OCMP_REQUIRE(vcycles >= 0 && vcycles <= 16, "V-cycles per step must be in [0, 16]");
+    OCMP_REQUIRE(vcycles == 0 || (P->P.schur_iters == 0 && !P->P.btri),
+                 "the Schur AMG step replaces the Schur inner solve and the block-triangular step");
+    if (vcycles > 0 && !P->P.amg) P->P.ready = false;  // the hierarchy is built by the next update
+    P->P.schur_amg = vcycles;
   });
 }
 

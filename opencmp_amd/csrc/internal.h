@@ -7,6 +7,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/ocmp.h"
@@ -341,14 +342,16 @@ struct Precond {
   // mixed path: keep the inner solves' matrices (S, the velocity block) in BF16 (FP32 vectors, FP64
   // sums); they are part of the preconditioner, which FGMRES lets vary
   bool bf16 = false;
+  // Schur rows by smoothed-aggregation AMG V-cycles on S (FGMRES; replaces the inner PCG): the
+  // hierarchy (aggregates, patterns, product lists) is built on the host at the first update with S's
+  // values, its values on the device at every update
+  int schur_amg = 0;       // V-cycles per preconditioner step (0: off)
+  struct Amg *amg = nullptr;
+  int64_t amg_cycles = 0;  // V-cycles of the running solve
   Precond() = default;
   Precond(const Precond &) = delete;
   Precond &operator=(const Precond &) = delete;
-  ~Precond() {
-    delete S;
-    delete Auu;
-    delete Aup;
-  }
+  ~Precond();
 };
 
 // ------------------------------------------------------------------ kernels (launchers)
@@ -440,6 +443,39 @@ void ensure_row_blocks32(Matrix *A);
 // the pattern of a Schur preconditioner's S = C + B diag(A)^{-1} B^T and the per-row Schur
 // column ranges of A (host, once per preconditioner)
 void build_schur_matrix(Precond *P);
+
+// ------------------------------------------------------------------ AMG on the Schur matrix
+//
+// Level l: matrix A (level 0 is P->S, not owned), prolongation P (n x nc) and restriction R = P^T
+// (nc x n) as CSR matrices, the Jacobi of A, and the device lists that give P, R and the next
+// level's A from A's values: P[e] = ident[e] - omega dinv[prow[e]] sum_{s in plist(e)} A[s];
+// R[k] = P[rslot[k]]; Q = A P with Q[q] = sum over (A slot, P slot) pairs; A_next = P^T Q with
+// A_next[c] = sum over (P slot, Q slot) pairs. The last level is solved by a dense inverse.
+struct AmgLevel {
+  Matrix *A = nullptr, *P = nullptr, *R = nullptr;
+  bool own_A = false;
+  double omega = 0.0;                       // prolongation smoothing weight 4 / (3 lambda_max(D^-1 A))
+  int64_t nq = 0;                           // nonzeros of Q = A P
+  DBuf<int32_t> d_prow, d_plptr, d_plist;   // per P entry: row, list range; A slots
+  DBuf<uint8_t> d_pident;                   // per P entry: 1 where the column is the row's own aggregate
+  DBuf<int32_t> d_rslot;                    // per R entry: its P slot
+  DBuf<int32_t> d_qptr, d_qa, d_qp;         // Q entries' (A slot, P slot) pair lists
+  DBuf<int32_t> d_cptr, d_cp, d_cq;         // next A's (P slot, Q slot) pair lists
+  DBuf<double> q;                           // Q values
+  DBuf<double> dinv;                        // Jacobi of A (0 on rows left out)
+  DBuf<float> dinv32;
+  DBuf<double> vd;                          // V-cycle vectors: b, x, y, r (4 n)
+  DBuf<float> vf;
+};
+struct Amg {
+  std::vector<std::unique_ptr<AmgLevel>> lv;
+  int64_t ncoarse = 0;                      // rows of the last level
+  DBuf<double> cinv;                        // its dense inverse (row-major)
+  DBuf<float> cinv32;
+  DBuf<double> cden;                        // scratch for the inversion (ncoarse x 2 ncoarse)
+  ~Amg();
+};
+void build_schur_amg(Precond *P, const std::vector<double> &sval, const std::vector<double> &sdinv);
 void ensure_diag_slots(Matrix *A);
 void build_velocity_blocks(Precond *P);
 constexpr int64_t SCHUR_MAX_PAIRS = (int64_t)1 << 28;

@@ -27,6 +27,9 @@ constexpr int NXCD = 8;
 constexpr int MAX_RED_BLOCKS = 2048;
 constexpr int STREAM_BLOCKS = 1024;  // grid of the streaming reduction kernels (4 per CU)
 constexpr int SCHUR_SCAL = 8;        // scalars at the head of the Schur inner solve's partial-sum buffer
+// rows of the coarsest AMG level inverted densely at most (the inversion keeps one factor per row in
+// LDS; the host stops coarsening at 300 rows and accepts up to 4x that when coarsening stalls)
+constexpr int AMG_DENSE_MAX = 1200;
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
 
@@ -142,6 +145,8 @@ __global__ void k_reduce(const double *__restrict__ part, int nparts, double *__
 // mode 2: y = mask*(A x), partial sum x.y                (CG: q = A p, p.q)
 // mode 3: y = mask*(A (cs*sx*x))   cs: column scaling (right Jacobi), sx: device scalar  (GMRES on H1)
 // mode 4: y = mask*(A (cs*x)) - dsh*x + csh*xp                  (a Newton-basis step of s-step GMRES)
+// mode 5: y = xp - A x                                           (multigrid residual, b in xp)
+// mode 6: y = xp + A x                                           (multigrid prolongation-add)
 
 template <typename VT, typename XT>
 __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT *__restrict__ y,
@@ -160,6 +165,10 @@ __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT 
     acc += (double)x[r] * (double)v;
   } else if (mode == 3) {
     y[r] = (mask && !mask[r]) ? (XT)0 : (XT)s * scale;
+  } else if (mode == 5) {
+    y[r] = xp[r] - (XT)s;
+  } else if (mode == 6) {
+    y[r] = xp[r] + (XT)s;
   } else {
     XT v = (XT)s - dsh * x[r];
     if (xp) v += csh * xp[r];
@@ -1272,6 +1281,256 @@ __global__ void __launch_bounds__(NTHR) k_pdiag_mask(int64_t ns, const double *_
 
 static size_t pcg_part_size(Matrix *M, const RowBlocks &rb);
 
+// ------------------------------------------------------------------ AMG on S (device)
+
+// P[e] = ident[e] - omega dinv[prow[e]] sum_{s in plist(e)} A[s]
+__global__ void k_amg_pvals(int64_t np, const int32_t *__restrict__ prow, const int32_t *__restrict__ plptr,
+                            const int32_t *__restrict__ plist, const uint8_t *__restrict__ ident,
+                            const double *__restrict__ aval, const double *__restrict__ dinv, double omega,
+                            double *__restrict__ pval, float *__restrict__ pval32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  double s = 0.0;
+  for (int32_t t = plptr[e]; t < plptr[e + 1]; ++t) s += aval[plist[t]];
+  const double v = (double)ident[e] - omega * dinv[prow[e]] * s;
+  pval[e] = v;
+  if (pval32) pval32[e] = (float)v;
+}
+
+// out[o] = sum over the pairs [ptr[o], ptr[o+1]) of a[ia[t]] b[ib[t]] (fixed order)
+__global__ void k_amg_pairs(int64_t no, const int32_t *__restrict__ ptr, const int32_t *__restrict__ ia,
+                            const int32_t *__restrict__ ib, const double *__restrict__ a, const double *__restrict__ b,
+                            double *__restrict__ out, float *__restrict__ out32) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= no) return;
+  double s = 0.0;
+  for (int32_t t = ptr[o]; t < ptr[o + 1]; ++t) s += a[ia[t]] * b[ib[t]];
+  out[o] = s;
+  if (out32) out32[o] = (float)s;
+}
+
+// the augmented dense matrix [A | I] (row-major, n x 2n) of the coarsest level. Rows left out of the
+// solve (dinv 0: not free, or a negligible diagonal) become unit rows with a zero right-hand side and
+// leave their columns out of the others, so their inverse rows and columns are zero.
+__global__ void k_amg_dense(int64_t n, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                            const double *__restrict__ val, const double *__restrict__ dinv, double *__restrict__ m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double *row = m + i * 2 * n;
+  const bool use = dinv[i] != 0.0;
+  for (int64_t j = 0; j < 2 * n; ++j) row[j] = (use && j == n + i) ? 1.0 : 0.0;
+  if (!use) {
+    row[i] = 1.0;
+    return;
+  }
+  for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
+    if (dinv[col[k]] != 0.0) row[col[k]] = val[k];
+}
+
+// Gauss-Jordan on [A | I] in one block (A symmetric positive definite: no pivoting), then the
+// inverse (the right half) into inv and its FP32 copy
+__global__ void __launch_bounds__(1024) k_amg_invert(int64_t n, double *__restrict__ m, double *__restrict__ inv,
+                                                     float *__restrict__ inv32) {
+  __shared__ double fac[AMG_DENSE_MAX];
+  const int64_t w = 2 * n;
+  for (int64_t k = 0; k < n; ++k) {
+    const double piv = m[k * w + k];
+    const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
+    __syncthreads();
+    for (int64_t j = threadIdx.x; j < w; j += blockDim.x) m[k * w + j] *= ip;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) fac[i] = i == k ? 0.0 : m[i * w + k];
+    __syncthreads();
+    // 32 lanes along a row, 32 rows at a time (columns left of k are already zero in the rows)
+    for (int64_t i = threadIdx.x >> 5; i < n; i += blockDim.x >> 5) {
+      const double f = fac[i];
+      if (f != 0.0)
+        for (int64_t j = k + (threadIdx.x & 31); j < w; j += 32) m[i * w + j] -= f * m[k * w + j];
+    }
+    __syncthreads();
+  }
+  for (int64_t i = threadIdx.x >> 5; i < n; i += blockDim.x >> 5)
+    for (int64_t j = threadIdx.x & 31; j < n; j += 32) {
+      const double v = m[i * w + n + j];
+      inv[i * n + j] = v;
+      if (inv32) inv32[i * n + j] = (float)v;
+    }
+}
+
+// x = inv b (one wavefront per row)
+template <typename T>
+__global__ void k_amg_dense_mv(int64_t n, const T *__restrict__ inv, const T *__restrict__ b, T *__restrict__ x) {
+  const int64_t i = blockIdx.x;
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < n; j += 64) s += (double)inv[i * n + j] * (double)b[j];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) x[i] = (T)s;
+}
+
+// x = omega dinv b;  x = y + omega dinv r
+template <typename T>
+__global__ void k_amg_jac0(int64_t n, const T *__restrict__ dinv, const T *__restrict__ b, T w, T *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = w * dinv[i] * b[i];
+}
+template <typename T>
+__global__ void k_amg_jac1(int64_t n, const T *__restrict__ dinv, const T *__restrict__ y, const T *__restrict__ r,
+                           T w, T *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = y[i] + w * dinv[i] * r[i];
+}
+
+// b = -s v
+template <typename T>
+__global__ void k_neg_scale(int64_t n, const T *__restrict__ v, const double *__restrict__ sv, T *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (T)(-(double)v[i] * sv[0]);
+}
+
+// the hierarchy's values from S's (and S's Jacobi) at every update
+static void amg_update(Precond *Pc, bool mixed) {
+  Amg &M = *Pc->amg;
+  hipStream_t st = Pc->A->ctx->stream;
+  for (size_t l = 0; l < M.lv.size(); ++l) {
+    AmgLevel &L = *M.lv[l];
+    Matrix *A = L.A;
+    const int64_t n = A->n;
+    if (l == 0) {
+      L.dinv.alloc(n);
+      OCMP_HIP(hipMemcpyAsync(L.dinv.p, Pc->sdinv.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+      ensure_diag_slots(A);
+      L.dinv.alloc(n);
+      if (mixed) L.dinv32.alloc(n);
+      k_jacobi_diag<<<nblk(n), NTHR, 0, st>>>(n, A->d_dslot.p, A->d_val.p, nullptr, L.dinv.p,
+                                              mixed ? L.dinv32.p : nullptr);
+      if (mixed) {
+        A->d_val32.alloc(A->nnz);
+        k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+      }
+    }
+    const size_t nv = (size_t)(l == 0 ? 6 : 4) * n;
+    if (mixed) {
+      L.dinv32.alloc(n);
+      if (l == 0) OCMP_HIP(hipMemcpyAsync(L.dinv32.p, Pc->sdinv32.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      L.vf.alloc(nv);
+    } else {
+      L.vd.alloc(nv);
+    }
+    if (l + 1 == M.lv.size()) {
+      const int64_t nc = n;
+      M.ncoarse = nc;
+      M.cden.alloc((size_t)nc * 2 * nc);
+      M.cinv.alloc((size_t)nc * nc);
+      if (mixed) M.cinv32.alloc((size_t)nc * nc);
+      k_amg_dense<<<nblk(nc), NTHR, 0, st>>>(nc, A->d_row_ptr.p, A->d_col.p, A->d_val.p, L.dinv.p, M.cden.p);
+      k_amg_invert<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
+      break;
+    }
+    Matrix *P = L.P, *R = L.R, *An = M.lv[l + 1]->A;
+    if (mixed) {
+      P->d_val32.alloc(P->nnz);
+      R->d_val32.alloc(R->nnz);
+    }
+    k_amg_pvals<<<nblk(P->nnz), NTHR, 0, st>>>(P->nnz, L.d_prow.p, L.d_plptr.p, L.d_plist.p, L.d_pident.p, A->d_val.p,
+                                               L.dinv.p, L.omega, P->d_val.p, mixed ? P->d_val32.p : nullptr);
+    k_gather_vals<<<nblk(R->nnz), NTHR, 0, st>>>(R->nnz, L.d_rslot.p, P->d_val.p, R->d_val.p,
+                                                 mixed ? R->d_val32.p : nullptr);
+    k_amg_pairs<<<nblk(L.nq), NTHR, 0, st>>>(L.nq, L.d_qptr.p, L.d_qa.p, L.d_qp.p, A->d_val.p, P->d_val.p, L.q.p,
+                                             nullptr);
+    k_amg_pairs<<<nblk(An->nnz), NTHR, 0, st>>>(An->nnz, L.d_cptr.p, L.d_cp.p, L.d_cq.p, P->d_val.p, L.q.p,
+                                                An->d_val.p, nullptr);
+  }
+}
+
+// one V(1,1) cycle with damped Jacobi (2/3) on level l: x ~ A_l^{-1} b
+constexpr double AMG_SMOOTH = 2.0 / 3.0;
+template <typename T>
+static void amg_vcycle(Amg &M, size_t l, const T *b, T *x, hipStream_t st) {
+  AmgLevel &L = *M.lv[l];
+  const int64_t n = L.A->n;
+  const T *dinv;
+  if constexpr (sizeof(T) == 8)
+    dinv = L.dinv.p;
+  else
+    dinv = L.dinv32.p;
+  if (l + 1 == M.lv.size()) {
+    const T *inv;
+    if constexpr (sizeof(T) == 8)
+      inv = M.cinv.p;
+    else
+      inv = M.cinv32.p;
+    k_amg_dense_mv<T><<<(unsigned)n, 64, 0, st>>>(n, inv, b, x);
+    return;
+  }
+  AmgLevel &C = *M.lv[l + 1];
+  T *vb;
+  T *cb;
+  if constexpr (sizeof(T) == 8) {
+    vb = L.vd.p;
+    cb = C.vd.p;
+  } else {
+    vb = L.vf.p;
+    cb = C.vf.p;
+  }
+  T *y = vb + 2 * n, *r = vb + 3 * n;
+  T *bc = cb, *xc = cb + C.A->n;
+  const T w = (T)AMG_SMOOTH;
+  auto vals = [](Matrix *A) -> const T * {
+    if constexpr (sizeof(T) == 8)
+      return A->d_val.p;
+    else
+      return A->d_val32.p;
+  };
+  k_amg_jac0<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, b, w, x);
+  spmv_dispatch<T, T>(L.A, vals(L.A), x, r, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L.A), st,
+                      (T)0, b, (T)0);
+  spmv_dispatch<T, T>(L.R, vals(L.R), r, bc, nullptr, nullptr, nullptr, nullptr, 0, nullptr, spmv_grid<T, T>(L.R), st);
+  amg_vcycle<T>(M, l + 1, bc, xc, st);
+  spmv_dispatch<T, T>(L.P, vals(L.P), xc, y, nullptr, nullptr, nullptr, nullptr, 6, nullptr, spmv_grid<T, T>(L.P), st,
+                      (T)0, x, (T)0);
+  spmv_dispatch<T, T>(L.A, vals(L.A), y, r, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L.A), st,
+                      (T)0, b, (T)0);
+  k_amg_jac1<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, y, r, w, x);
+}
+
+// z[s0:s1] = -(cycles V-cycles on S) (s v)[s0:s1]: the first cycle from zero, each further one on the
+// residual of the current iterate. Level 0's vectors: b, (unused), y, r (the cycle's own), then the
+// residual and the correction of the further cycles.
+template <typename T>
+__global__ void k_add(int64_t n, const T *__restrict__ e, T *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += e[i];
+}
+
+template <typename T>
+static void schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z) {
+  Amg &M = *Pc->amg;
+  Ctx *ctx = Pc->A->ctx;
+  hipStream_t st = ctx->stream;
+  AmgLevel &L0 = *M.lv[0];
+  const int64_t n = L0.A->n;
+  T *vb;
+  const T *val;
+  if constexpr (sizeof(T) == 8) {
+    vb = L0.vd.p;
+    val = L0.A->d_val.p;
+  } else {
+    vb = L0.vf.p;
+    val = L0.A->d_val32.p;
+  }
+  T *b = vb, *rr = vb + 4 * n, *ee = vb + 5 * n;
+  T *x = z + Pc->s0;
+  KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "schur_amg_vcycle_f64" : "schur_amg_vcycle_f32"));
+  k_neg_scale<T><<<nblk(n), NTHR, 0, st>>>(n, v + Pc->s0, sv, b);
+  amg_vcycle<T>(M, 0, b, x, st);
+  for (int c = 1; c < Pc->schur_amg; ++c) {
+    spmv_dispatch<T, T>(L0.A, val, x, rr, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L0.A), st,
+                        (T)0, b, (T)0);
+    amg_vcycle<T>(M, 0, rr, ee, st);
+    k_add<T><<<nblk(n), NTHR, 0, st>>>(n, ee, x);
+  }
+}
+
 // the block-triangular step's blocks of A, their Jacobi, and 1/d on the Schur rows
 static void btri_update(Precond *P, bool mixed) {
   Matrix *A = P->A;
@@ -1349,7 +1608,7 @@ void precond_update(Precond *P) {
         k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nb, smax);
         k_schur_finish<<<nb, NTHR, 0, st>>>(A->n, P->schur_flag.p, smax, P->dinv.p, mixed ? P->dinv32.p : nullptr);
       }
-      if (P->schur_iters > 0) {
+      if (P->schur_iters > 0 || P->schur_amg > 0) {
         if (!P->S) build_schur_matrix(P);
         Matrix *S = P->S;
         const int64_t ns = S->n;
@@ -1392,6 +1651,17 @@ void precond_update(Precond *P) {
         }
         P->spart.alloc(std::max(P->spart.n, pcg_part_size(S, P->srb)));
         P->sflag.alloc(1);
+        if (P->schur_amg > 0) {
+          if (!P->amg) {
+            // the hierarchy is built once, on the host, from this first set of values
+            std::vector<double> hv(S->nnz), hd(ns);
+            OCMP_HIP(hipMemcpyAsync(hv.data(), S->d_val.p, S->nnz * sizeof(double), hipMemcpyDeviceToHost, st));
+            OCMP_HIP(hipMemcpyAsync(hd.data(), P->sdinv.p, ns * sizeof(double), hipMemcpyDeviceToHost, st));
+            OCMP_HIP(hipStreamSynchronize(st));
+            build_schur_amg(P, hv, hd);
+          }
+          amg_update(P, mixed);
+        }
       }
       if (P->btri) btri_update(P, mixed);
     }
@@ -2849,7 +3119,12 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
           KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
           k_prec_jacobi<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, vj, vs + jj, zj);
         }
-        if (P->schur_iters > 0) schur_inner<T>(P, vj, vs + jj, zj, cap);
+        if (P->schur_amg > 0) {
+          schur_amg_apply<T>(P, vj, vs + jj, zj);
+          P->amg_cycles += P->schur_amg;
+        } else if (P->schur_iters > 0) {
+          schur_inner<T>(P, vj, vs + jj, zj, cap);
+        }
       }
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
@@ -2920,6 +3195,9 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
                "update the preconditioner after turning the Schur inner solve on");
   OCMP_REQUIRE(!P->btri || (P->Auu && P->vdinv.p && P->pdinv.p && (o.precision != OCMP_MIXED || P->vdinv32.p)),
                "update the preconditioner after turning the block-triangular step on");
+  OCMP_REQUIRE(P->schur_amg == 0 || (P->amg && (o.precision != OCMP_MIXED || P->amg->lv[0]->dinv32.p)),
+               "update the preconditioner after turning the Schur AMG step on");
+  P->amg_cycles = 0;
   OCMP_REQUIRE(!P->bf16 || o.precision != OCMP_MIXED ||
                    ((P->schur_iters == 0 || P->btri || P->S->d_valbf.p) && (!P->btri || P->Auu->d_valbf.p)),
                "update the preconditioner after turning BF16 storage on");
@@ -2989,7 +3267,9 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   st_.outer = outer;
   st_.converged = st_.rel_residual <= o.tol ? 1 : 0;
   st_.fallback = have_learned && fell_back ? 1 : 0;
-  if (P->schur_iters > 0 || P->btri) {
+  if (P->schur_amg > 0) {
+    st_.precond_iters = P->amg_cycles;
+  } else if (P->schur_iters > 0 || P->btri) {
     double cnt = 0.0;
     OCMP_HIP(hipMemcpyAsync(&cnt, (P->btri ? P->vpart.p : P->spart.p) + 5, sizeof(double), hipMemcpyDeviceToHost,
                             A->ctx->stream));
@@ -3266,8 +3546,8 @@ void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
 }
 
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st) {
-  OCMP_REQUIRE(!P || P->schur_iters == 0 || o.solver == OCMP_SOLVER_FGMRES,
-               "the Schur inner solve is a nonlinear preconditioner step: it needs FGMRES");
+  OCMP_REQUIRE(!P || (P->schur_iters == 0 && P->schur_amg == 0) || o.solver == OCMP_SOLVER_FGMRES,
+               "the Schur inner solve and the Schur AMG step run inside FGMRES's preconditioner step");
   switch (o.solver) {
     case OCMP_SOLVER_CG: solve_cg(A, P, b, x, o, st); break;
     case OCMP_SOLVER_GMRES: solve_gmres(A, P, b, x, o, st); break;

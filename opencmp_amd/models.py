@@ -69,7 +69,7 @@ class Model:
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
                  inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
                  t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0,
-                 bf16_precond: bool = False):
+                 bf16_precond: bool = False, schur_amg: int = 0):
         self.ctx = ctx or C.Context.default()
         # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
         # expressions of t (boundary data, sources) are read at the level a form is built for
@@ -94,7 +94,11 @@ class Model:
         self.preconditioner = 'local' if preconditioner in ('default', 'local') else preconditioner
         self.schur_iters = schur_iters if self.preconditioner == 'schur' else 0
         self.schur_tol = schur_tol
-        if self.schur_iters and linear_solver == 'GMRes':
+        # 'schur' with schur_amg > 0: AMG V-cycles on the Schur matrix instead of the inner PCG
+        self.schur_amg = schur_amg if self.preconditioner == 'schur' else 0
+        if self.schur_amg:
+            self.schur_iters = 0
+        if (self.schur_iters or self.schur_amg) and linear_solver == 'GMRes':
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
         self.vel_iters = vel_iters if self.preconditioner == 'block_triangular' else 0
         self.vel_tol = vel_tol
@@ -180,7 +184,7 @@ class Model:
         if self.preconditioner == 'schur':
             pre = C.Preconditioner(a_assembled, 'schur', precision=self.precision,
                                    schur_blocks=self.schur_blocks(), schur_iters=self.schur_iters,
-                                   schur_tol=self.schur_tol)
+                                   schur_tol=self.schur_tol, schur_amg=self.schur_amg)
             if self.bf16_precond:
                 pre.set_bf16(True)
             return pre

@@ -1,0 +1,89 @@
+"""Smoothed-aggregation AMG on the Schur matrix S = C + B D^-1 B^T (ocmp_precond_set_schur_amg) as the
+pressure part of the FGMRES preconditioner for the Taylor-Hood saddle points: the solution equals the
+oracle's direct solve and the inner-PCG variant's, in FP64 and mixed precision, on a hierarchy of
+several levels (the INS channel) and on a single dense level (a small Stokes problem)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem
+from opencmp_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print('[gpu-test]', *a, flush=True)
+
+
+def channel(ctx, n, precision, **kw):
+    from opencmp_amd.mesh import channel_with_holes
+    from opencmp_amd.models import INS, const
+    hm = channel_with_holes(n)
+    inflow = [(L.FN_POLY2, [0, 0, 6.0, 0, 0, -6.0]), const(0.0)]
+    zero = [const(0.0), const(0.0)]
+    return INS(hm, kv=1e-3, interp_ord=2, dirichlet={'u': {'inlet': inflow, 'wall': zero, 'cyl': zero}},
+               stress={'u': {'outlet': zero}}, linear_solver='GMRes', preconditioner='schur',
+               linear_tolerance=1e-10, restart=50, precision=precision, ctx=ctx, **kw), hm
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_schur_amg_fgmres_matches_inner_pcg(gpu_ctx, precision):
+    from opencmp_amd import core as C
+    say('Schur AMG vs inner PCG on the INS channel n=32,', precision)
+    m, hm = channel(gpu_ctx, 32, precision)
+    dt = 1e-3
+    a = C.BilinearForm(m.fes, m.time_derivative_terms() + m.construct_bilinear_time_coefficient(dt) +
+                       m.construct_bilinear_time_ODE(dt))
+    a.Assemble()
+    g = torch.Generator(device='cpu').manual_seed(3)
+    b = torch.randn(m.fes.ndof, dtype=torch.float64, generator=g).to(gpu_ctx.torch_device)
+    b *= torch.from_numpy(m.fes.FreeDofs().astype(np.float64)).to(b.device)
+    out = {}
+    for tag, kw in (('pcg', dict(schur_iters=200, schur_tol=1e-1)), ('amg1', dict(schur_amg=1)),
+                    ('amg2', dict(schur_amg=2))):
+        pre = C.Preconditioner(a, 'schur', precision=precision, schur_blocks=m.schur_blocks(), **kw)
+        pre.Update()
+        x = torch.zeros_like(b)
+        r = C.solve(a, b, x, pre, 'FGMRes', tol=1e-10, restart=50, maxit=20000, precision=precision)
+        say(f'  {tag}: {r}')
+        assert r.converged and r.rel_residual <= 1e-10 and r.precond_iters > 0
+        out[tag] = (x, r)
+    for tag in ('amg1', 'amg2'):
+        rel = float(torch.linalg.norm(out[tag][0] - out['pcg'][0]) / torch.linalg.norm(out['pcg'][0]))
+        say(f'  {tag} vs pcg: {rel:.2e}')
+        assert rel < 1e-6
+    # one V-cycle is about as good a Schur solve as the inner PCG to 1e-1 (within 30 % more outer iterations)
+    assert out['amg1'][1].iters <= 1.3 * out['pcg'][1].iters + 5
+    assert out['amg2'][1].iters <= out['amg1'][1].iters + 5
+
+
+def test_schur_amg_single_dense_level_matches_oracle(gpu_ctx):
+    """A Stokes problem small enough that S is inverted densely (one level): FGMRES with the exact
+    Schur solve converges in few iterations to the oracle's direct solution."""
+    from opencmp_amd import core as C
+    from opencmp_amd.mesh import unit_square
+    from opencmp_amd.models import Stokes, const
+    say('Schur AMG, one dense level: Stokes 6x6')
+    hm = unit_square(6)
+    inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
+    wall = [const(0.0), const(0.0)]
+    m = Stokes(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
+               linear_solver='GMRes', preconditioner='schur', linear_tolerance=1e-11, schur_amg=1, ctx=gpu_ctx)
+    assert m.linear_solver == 'FGMRes'
+    a = C.BilinearForm(m.fes, m.construct_bilinear_time_coefficient(1.0) + m.construct_bilinear_time_ODE(1.0))
+    a.Assemble()
+    f = C.LinearForm(m.fes, m.construct_linear(1.0))
+    f.Assemble()
+    pre = m.construct_preconditioners(a)
+    pre.Update()
+    x = C.GridFunction(m.fes)
+    m.apply_dirichlet_bcs_to(x)
+    x0 = x.numpy().copy()
+    r = C.solve(a, f.vec, x.vec, pre, 'FGMRes', tol=1e-11, restart=100, maxit=5000)
+    say('  ', r)
+    assert r.converged
+    K = a.to_scipy().tocsr()
+    xo = fem.solve_free(K, f.vec.cpu().numpy(), x0, m.fes.FreeDofs())
+    uu = slice(0, 2 * m.fes.blocks[0][1])
+    assert np.abs(x.numpy()[uu] - xo[uu]).max() <= 1e-8 * np.abs(xo[uu]).max()
