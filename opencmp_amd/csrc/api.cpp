@@ -643,7 +643,8 @@ void build_schur_matrix(Precond *P) {
 namespace {
 
 constexpr double AMG_THETA = 0.08;
-constexpr int64_t AMG_COARSE = 300;   // dense inverse on the last level at or below this many rows
+constexpr int64_t AMG_COARSE = 48;    // dense inverse (in LDS) on the last level at or below this many rows
+constexpr int64_t AMG_DENSE_LIMIT = 1200;  // the largest last level accepted when coarsening stalls
 constexpr int AMG_MAX_LEVELS = 10;
 
 struct HCsr {
@@ -732,7 +733,7 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
     L.own_A = l > 0;
     const int64_t n = A.n;
     if (n <= AMG_COARSE || l == AMG_MAX_LEVELS - 1) {
-      OCMP_REQUIRE(n <= 4 * AMG_COARSE, "AMG: the coarsest level is too large for its dense inverse");
+      OCMP_REQUIRE(n <= AMG_DENSE_LIMIT, "AMG: the coarsest level is too large for its dense inverse");
       amg->lv.push_back(std::move(Lp));
       amg->ncoarse = n;
       break;
@@ -778,7 +779,7 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
       ++na;
     }
     if (na == 0 || na > (int64_t)(0.9 * (double)n)) {
-      OCMP_REQUIRE(n <= 4 * AMG_COARSE, "AMG: coarsening stalled above the dense-inverse size");
+      OCMP_REQUIRE(n <= AMG_DENSE_LIMIT, "AMG: coarsening stalled above the dense-inverse size");
       amg->lv.push_back(std::move(Lp));
       amg->ncoarse = n;
       break;

@@ -28,8 +28,10 @@ constexpr int MAX_RED_BLOCKS = 2048;
 constexpr int STREAM_BLOCKS = 1024;  // grid of the streaming reduction kernels (4 per CU)
 constexpr int SCHUR_SCAL = 8;        // scalars at the head of the Schur inner solve's partial-sum buffer
 // rows of the coarsest AMG level inverted densely at most (the inversion keeps one factor per row in
-// LDS; the host stops coarsening at 300 rows and accepts up to 4x that when coarsening stalls)
+// LDS; the host stops coarsening at 48 rows and accepts up to 1200 when coarsening stalls)
 constexpr int AMG_DENSE_MAX = 1200;
+// a last level of at most this many rows is inverted in LDS (n x 2n doubles: 36 KB)
+constexpr int AMG_LDS_MAX = 48;
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
 
@@ -1356,6 +1358,35 @@ __global__ void __launch_bounds__(1024) k_amg_invert(int64_t n, double *__restri
     }
 }
 
+// the same inversion for a last level of at most AMG_LDS_MAX rows, entirely in LDS
+__global__ void __launch_bounds__(1024) k_amg_invert_lds(int64_t n, const double *__restrict__ m0,
+                                                         double *__restrict__ inv, float *__restrict__ inv32) {
+  __shared__ double m[AMG_LDS_MAX * 2 * AMG_LDS_MAX];
+  __shared__ double fac[AMG_LDS_MAX];
+  const int w = 2 * (int)n;
+  for (int t = threadIdx.x; t < (int)n * w; t += blockDim.x) m[t] = m0[t];
+  __syncthreads();
+  for (int k = 0; k < (int)n; ++k) {
+    const double piv = m[k * w + k];
+    const double ip = piv != 0.0 ? 1.0 / piv : 0.0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < w; j += blockDim.x) m[k * w + j] *= ip;
+    for (int i = threadIdx.x; i < (int)n; i += blockDim.x) fac[i] = i == k ? 0.0 : m[i * w + k];
+    __syncthreads();
+    for (int t = threadIdx.x; t < (int)n * w; t += blockDim.x) {
+      const int i = t / w, j = t - i * w;
+      if (i != k && j >= k) m[t] -= fac[i] * m[k * w + j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < (int)(n * n); t += blockDim.x) {
+    const int i = t / (int)n, j = t - i * (int)n;
+    const double v = m[i * w + (int)n + j];
+    inv[t] = v;
+    if (inv32) inv32[t] = (float)v;
+  }
+}
+
 // x = inv b (one wavefront per row)
 template <typename T>
 __global__ void k_amg_dense_mv(int64_t n, const T *__restrict__ inv, const T *__restrict__ b, T *__restrict__ x) {
@@ -1423,7 +1454,10 @@ static void amg_update(Precond *Pc, bool mixed) {
       M.cinv.alloc((size_t)nc * nc);
       if (mixed) M.cinv32.alloc((size_t)nc * nc);
       k_amg_dense<<<nblk(nc), NTHR, 0, st>>>(nc, A->d_row_ptr.p, A->d_col.p, A->d_val.p, L.dinv.p, M.cden.p);
-      k_amg_invert<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
+      if (nc <= AMG_LDS_MAX)
+        k_amg_invert_lds<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
+      else
+        k_amg_invert<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
       break;
     }
     Matrix *P = L.P, *R = L.R, *An = M.lv[l + 1]->A;

@@ -64,8 +64,8 @@ def test_schur_amg_single_dense_level_matches_oracle(gpu_ctx):
     from opencmp_amd import core as C
     from opencmp_amd.mesh import unit_square
     from opencmp_amd.models import Stokes, const
-    say('Schur AMG, one dense level: Stokes 6x6')
-    hm = unit_square(6)
+    say('Schur AMG, one dense level: Stokes 5x5')
+    hm = unit_square(5)                       # 36 pressure dofs: at most 48 rows stay a single dense level
     inflow = [(L.FN_POLY2, [0, 0, 4.0, 0, 0, -4.0]), const(0.0)]
     wall = [const(0.0), const(0.0)]
     m = Stokes(hm, kv=1.0, interp_ord=2, dirichlet={'u': {'left': inflow, 'top': wall, 'bottom': wall}},
