@@ -5,13 +5,14 @@ Navier-Stokes, Taylor-Hood P2/P1, implicit Euler dt 1e-3 with the Oseen-Picard l
 (<= 3 iterations, tolerance 1e-6), nu 1e-3, on the channel [0,4]x[0,1] with 8 seeded circular
 holes (n=228: 396,953 triangles, 1,794,910 DOFs), parabolic inflow Umax 1.5, do-nothing (stress)
 outlet with the no-backflow term; FGMRES(50) with the Schur preconditioner (velocity Jacobi,
-Jacobi-PCG on C + B D^-1 B^T for the pressure to 1e-1) to ||b - A x|| / ||b|| <= 1e-8.
+one smoothed-aggregation AMG V-cycle on C + B D^-1 B^T for the pressure) to
+||b - A x|| / ||b|| <= 1e-8.
 A plain FP64 run records 40 steps, the per-DOF MLP is trained on them, then from the same
 state both legs time the same steps:
-  fp64           : FP64 assembly, FP64 FGMRES + FP64 inner PCG (the best FP64 path: the
-                   same FGMRES/Schur algorithm as the mixed leg)
-  mixed+learned  : FP64 assembly and outer residual, FP32 FGMRES + FP32 inner PCG in
-                   iterative refinement, MLP initial guess with the residual fallback
+  fp64           : FP64 assembly, FP64 FGMRES + FP64 AMG (the best FP64 path: the same
+                   FGMRES/Schur algorithm as the mixed leg)
+  mixed+learned  : FP64 assembly and outer residual, FP32 FGMRES + FP32 AMG in iterative
+                   refinement, MLP initial guess with the residual fallback
 A step = the right-hand side, every Picard iteration's bilinear assembly, preconditioner
 setup and solve; inputs (mesh, pattern, schedules) are resident in HBM before the timed
 region. The line also carries config 2 (DG-P2 512^2 convection-diffusion, GMRES(50) +
@@ -227,14 +228,15 @@ def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     return gbs
 
 
-def cpu_baseline_ins(r, gpu_outer_fp64, n_iter=30):
+def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, n_iter=30):
     """Config 4 on the host cores, timed on the full-size problem: the oracle's C++/OpenMP
     restatement (oracle/cpu_solver.cpp, FP64) assembles the Oseen operator with the wind of the
     last timed step (textbook element loop: quadrature per element, atomic scatter) and runs the
-    same FGMRES(50) + Schur preconditioner (velocity Jacobi, Jacobi-PCG on C + B D^-1 B^T to 1e-1)
-    on the FP64 path's last system: its preconditioner setup (S built by rows) alone, then
-    `n_iter` outer iterations. Per step: (assembly + setup) x the Picard iterations of the FP64
-    leg + time per outer iteration x the FP64 leg's outer iterations."""
+    same FGMRES(50) + Schur preconditioner (velocity Jacobi; for the pressure `amg`
+    smoothed-aggregation V-cycles on C + B D^-1 B^T, or Jacobi-PCG on it to 1e-1 when amg is 0)
+    on the FP64 path's last system: its preconditioner setup (S built by rows, the AMG hierarchy
+    built) alone, then `n_iter` outer iterations. Per step: (assembly + setup) x the Picard
+    iterations of the FP64 leg + time per outer iteration x the FP64 leg's outer iterations."""
     import ctypes as C
     from oracle import cbuild
     from oracle.cpu_baseline import OseenTables
@@ -265,12 +267,12 @@ def cpu_baseline_ins(r, gpu_outer_fp64, n_iter=30):
     x = np.zeros(n)
     t0 = time.perf_counter()
     lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, 0, 1e-1, 200,
-                              C.byref(rel), C.byref(inner))
+                              C.byref(rel), C.byref(inner), amg, None)
     t_setup = time.perf_counter() - t0
     x[:] = 0.0
     t0 = time.perf_counter()
     it = lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, n_iter, 1e-1,
-                                   200, C.byref(rel), C.byref(inner))
+                                   200, C.byref(rel), C.byref(inner), amg, None)
     t_it = (time.perf_counter() - t0 - t_setup) / max(it, 1)
     picard = float(np.mean([row[1] for row in r['rows64']]))
     outer = float(np.mean(gpu_outer_fp64))
@@ -280,8 +282,10 @@ def cpu_baseline_ins(r, gpu_outer_fp64, n_iter=30):
             'assembly_and_setup_s': round(s_asm, 3), 'solve_s': round(s_solve, 3),
             'sample': f'oracle/cpu_solver.cpp (C++ -O3 OpenMP, FP64, {cores} threads) on the full-size problem '
                       f'({n} dofs, {va.size} nnz): Oseen assembly {t_asm:.3f} s and Schur setup {t_setup:.3f} s, '
-                      f'times {picard:.1f} Picard iterations per step; FGMRES(50) + Schur (inner Jacobi-PCG to '
-                      f'1e-1) {it} outer iterations ({inner.value} inner) at {t_it * 1e3:.1f} ms per outer '
+                      f'times {picard:.1f} Picard iterations per step; FGMRES(50) + Schur ('
+                      + (f'{amg} smoothed-aggregation AMG V-cycle(s) per step, the hierarchy rebuilt at each '
+                         f'setup' if amg else 'inner Jacobi-PCG to 1e-1') +
+                      f') {it} outer iterations ({inner.value} inner) at {t_it * 1e3:.1f} ms per outer '
                       f'iteration, times the FP64 GPU leg\'s {outer:.0f} outer iterations per step. The outlet\'s '
                       f'boundary term and the right-hand side are not in the CPU assembly. Host-CPU '
                       f'restatement, not NGSolve.'}
@@ -562,9 +566,9 @@ def main():
                     help='config 4: at most this many inner PCG iterations on the pressure Schur complement '
                          'per FGMRES step (0: the Schur diagonal under GMRES)')
     ap.add_argument('--train-steps', type=int, default=1000, help='config 4: Adam steps')
-    ap.add_argument('--schur-amg', type=int, default=0,
-                    help='config 4: smoothed-aggregation AMG V-cycles on the Schur matrix per FGMRES step in '
-                         'place of the inner PCG (0: the inner PCG)')
+    ap.add_argument('--schur-amg', type=int, default=1,
+                    help='config 4: smoothed-aggregation AMG V-cycles on the Schur matrix per FGMRES step '
+                         '(0: the inner Jacobi-PCG of --schur-tol / --schur-iters instead)')
     ap.add_argument('--n', type=int, default=None,
                     help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 64; config 5 at 64^3 needs the '
                          'block-triangular preconditioner, the default)')
@@ -645,8 +649,9 @@ def main():
             'config': {'workload': f"{r['desc']} to rel residual {args.tol:g}", 'config_index': 4,
                        'ndof': r['ndof'], 'tol': args.tol},
             'plain_fp64': {'s_per_step': round(r['s64'], 5), 'median_s_per_step': round(r['med64'], 5),
-                           'algorithm': 'FP64 FGMRES(50) + Schur (FP64 inner PCG): the same algorithm as the '
-                                        'mixed leg, the best FP64 path of this build',
+                           'algorithm': ('FP64 FGMRES(50) + Schur (' + (f'FP64 AMG, {args.schur_amg} V-cycle(s)'
+                                                                         if args.schur_amg else 'FP64 inner PCG') +
+                                         '): the same algorithm as the mixed leg, the best FP64 path of this build'),
                            'outer_iters_per_step': it64, 'iters_picard_res': r['rows64']},
             'mixed_learned': {'s_per_step': round(r['sml'], 5), 'median_s_per_step': round(r['medml'], 5),
                               'outer_iters_per_step': itml, 'iters_picard_res': r['rowsml'],
@@ -679,7 +684,7 @@ def main():
                 out['roofline']['traffic_source'] = 'profiles/ins_channel_n228_fgmres_schur_hbm_traffic.md'
         if not args.no_cpu:
             log('cpu baseline (config 4) ...')
-            cb = cpu_baseline_ins(r, it64)
+            cb = cpu_baseline_ins(r, it64, amg=args.schur_amg)
             cb['speedup_gpu_mixed_learned'] = round(cb['value'] / r['sml'], 1)
             cb['speedup_gpu_fp64'] = round(cb['value'] / r['s64'], 1)
             out['cpu_baseline'] = cb

@@ -382,6 +382,12 @@ int ocmp_precond_set_bf16(ocmp_precond *P, int on);
  * are recomputed on the device at every update. The step is a fixed linear operator.
  * Replaces the Schur inner solve (turn that off first); vcycles = 0 turns it off. */
 int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles);
+/* The Schur AMG hierarchy built by the last update (no counterpart in the reference,
+ * whose preconditioners are NGSolve objects: this reports what ocmp_precond_set_schur_amg
+ * built). *levels gets the number of levels (0: none built); rows[l] and nnz[l] the size
+ * of level l's operator for l < max_levels (0 past the last level). */
+int ocmp_precond_schur_amg_levels(const ocmp_precond *P, int64_t *rows, int64_t *nnz, int max_levels,
+                                  int *levels);
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);

@@ -108,6 +108,7 @@ SIGNATURES = {
     'ocmp_precond_set_block_triangular': [P, P, C.c_int, C.c_double],
     'ocmp_precond_set_bf16': [P, C.c_int],
     'ocmp_precond_set_schur_amg': [P, C.c_int],
+    'ocmp_precond_schur_amg_levels': [P, P, P, C.c_int, C.POINTER(C.c_int)],
     'ocmp_precond_update': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],

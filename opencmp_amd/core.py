@@ -569,6 +569,13 @@ class Preconditioner:
     def Update(self):
         L.check(self.a.ctx.lib.ocmp_precond_update(self.h), 'precond_update')
 
+    def amg_levels(self) -> list:
+        """(rows, nonzeros) of each level of the Schur AMG hierarchy the last Update built (empty if none)."""
+        rows, nnz, nl = (C.c_int64 * 16)(), (C.c_int64 * 16)(), C.c_int()
+        L.check(self.a.ctx.lib.ocmp_precond_schur_amg_levels(self.h, rows, nnz, 16, C.byref(nl)),
+                'precond_schur_amg_levels')
+        return [(int(rows[l]), int(nnz[l])) for l in range(min(nl.value, 16))]
+
     def __del__(self):
         try:
             self.a.ctx.lib.ocmp_precond_destroy(self.h)

@@ -703,7 +703,8 @@ Amg::~Amg() {
   for (auto &L : lv) {
     if (L->own_A) delete L->A;
     delete L->P;
-    delete L->R;
+    delete L->W;
+    delete L->Wt;
   }
 }
 
@@ -848,25 +849,6 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
     std::vector<std::vector<int32_t>>().swap(pc);
     std::vector<std::vector<int32_t>>().swap(pl);
     std::vector<std::vector<int32_t>>().swap(pll);
-    // R = P^T (counting sort by column), with each entry's P slot
-    HCsr Rh;
-    Rh.n = na;
-    Rh.rp.assign(na + 1, 0);
-    for (int64_t e = 0; e < np; ++e) Rh.rp[Ph.col[e] + 1]++;
-    for (int32_t I = 0; I < na; ++I) Rh.rp[I + 1] += Rh.rp[I];
-    Rh.col.resize(np);
-    Rh.val.resize(np);
-    std::vector<int32_t> rslot(np);
-    {
-      std::vector<int32_t> fill(Rh.rp.begin(), Rh.rp.end() - 1);
-      for (int64_t i = 0; i < n; ++i)
-        for (int32_t e = Ph.rp[i]; e < Ph.rp[i + 1]; ++e) {
-          const int32_t k = fill[Ph.col[e]]++;
-          Rh.col[k] = (int32_t)i;
-          Rh.val[k] = Ph.val[e];
-          rslot[k] = e;
-        }
-    }
     // Q = A P: entry (i, J) sums A[i,j] P[j,J] over the usable columns j of row i
     HCsr Qh;
     Qh.n = n;
@@ -923,7 +905,20 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
     std::vector<std::vector<int32_t>>().swap(qn);
     std::vector<std::vector<int32_t>>().swap(qa);
     std::vector<std::vector<int32_t>>().swap(qp);
-    // A_c = P^T Q: entry (I, J) sums P[i,I] Q[i,J] over the fine rows i of R's row I
+    // P's entries by column (counting sort): the fine rows of each aggregate with their P slots
+    std::vector<int32_t> tptr(na + 1, 0), trow(np), tslot(np);
+    for (int64_t e = 0; e < np; ++e) tptr[Ph.col[e] + 1]++;
+    for (int32_t I = 0; I < na; ++I) tptr[I + 1] += tptr[I];
+    {
+      std::vector<int32_t> fill(tptr.begin(), tptr.end() - 1);
+      for (int64_t i = 0; i < n; ++i)
+        for (int32_t e = Ph.rp[i]; e < Ph.rp[i + 1]; ++e) {
+          const int32_t k = fill[Ph.col[e]]++;
+          trow[k] = (int32_t)i;
+          tslot[k] = e;
+        }
+    }
+    // A_c = P^T Q: entry (I, J) sums P[i,I] Q[i,J] over the fine rows i of aggregate I
     HCsr C;
     C.n = na;
     C.rp.assign(na + 1, 0);
@@ -931,8 +926,8 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t I = 0; I < na; ++I) {
       std::vector<std::array<int32_t, 3>> t;  // (J, P slot, Q slot)
-      for (int32_t k = Rh.rp[I]; k < Rh.rp[I + 1]; ++k) {
-        const int32_t i = Rh.col[k], e = rslot[k];
+      for (int32_t k = tptr[I]; k < tptr[I + 1]; ++k) {
+        const int32_t i = trow[k], e = tslot[k];
         for (int32_t q = Qh.rp[i]; q < Qh.rp[i + 1]; ++q) t.push_back({Qh.col[q], e, q});
       }
       std::stable_sort(t.begin(), t.end(), [](const auto &a, const auto &b) { return a[0] < b[0]; });
@@ -975,15 +970,54 @@ void build_schur_amg(Precond *Pc, const std::vector<double> &sval, const std::ve
         C.val[ce] = sum;
       }
     }
+    // W = P - w D^-1 Q on the union of P's and Q's patterns (both rows sorted by column), and W^T
+    HCsr Wh;
+    Wh.n = n;
+    Wh.rp.assign(n + 1, 0);
+    std::vector<int32_t> wrow, wps, wqs;
+    for (int64_t i = 0; i < n; ++i) {
+      int32_t a = Ph.rp[i], b = Qh.rp[i];
+      while (a < Ph.rp[i + 1] || b < Qh.rp[i + 1]) {
+        const int32_t ca = a < Ph.rp[i + 1] ? Ph.col[a] : INT32_MAX, cb = b < Qh.rp[i + 1] ? Qh.col[b] : INT32_MAX;
+        const int32_t c = std::min(ca, cb);
+        Wh.col.push_back(c);
+        wrow.push_back((int32_t)i);
+        wps.push_back(ca == c ? a++ : -1);
+        wqs.push_back(cb == c ? b++ : -1);
+      }
+      Wh.rp[i + 1] = (int32_t)Wh.col.size();
+    }
+    const int64_t nw = (int64_t)Wh.col.size();
+    Wh.val.assign(nw, 0.0);
+    HCsr Wth;
+    Wth.n = na;
+    Wth.rp.assign(na + 1, 0);
+    for (int64_t e = 0; e < nw; ++e) Wth.rp[Wh.col[e] + 1]++;
+    for (int32_t I = 0; I < na; ++I) Wth.rp[I + 1] += Wth.rp[I];
+    Wth.col.resize(nw);
+    std::vector<int32_t> wtslot(nw);
+    {
+      std::vector<int32_t> fill(Wth.rp.begin(), Wth.rp.end() - 1);
+      for (int64_t i = 0; i < n; ++i)
+        for (int32_t e = Wh.rp[i]; e < Wh.rp[i + 1]; ++e) {
+          const int32_t k = fill[Wh.col[e]]++;
+          Wth.col[k] = (int32_t)i;
+          wtslot[k] = e;
+        }
+    }
     // device copies of the level's structure
     L.P = amg_matrix(S, Ph);
-    L.R = amg_matrix(S, Rh);
+    L.W = amg_matrix(S, Wh);
+    L.Wt = amg_matrix(S, Wth);
+    L.d_wrow.upload(wrow.empty() ? std::vector<int32_t>(1, 0) : wrow, st);
+    L.d_wps.upload(wps.empty() ? std::vector<int32_t>(1, 0) : wps, st);
+    L.d_wqs.upload(wqs.empty() ? std::vector<int32_t>(1, 0) : wqs, st);
+    L.d_wtslot.upload(wtslot.empty() ? std::vector<int32_t>(1, 0) : wtslot, st);
     L.nq = nq;
     L.d_prow.upload(prow, st);
     L.d_plptr.upload(plptr, st);
     L.d_plist.upload(plist.empty() ? std::vector<int32_t>(1, 0) : plist, st);
     L.d_pident.upload(pident, st);
-    L.d_rslot.upload(rslot, st);
     L.d_qptr.upload(qptr32, st);
     L.d_qa.upload(qa_all.empty() ? std::vector<int32_t>(1, 0) : qa_all, st);
     L.d_qp.upload(qp_all.empty() ? std::vector<int32_t>(1, 0) : qp_all, st);
@@ -1778,6 +1812,21 @@ int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles) {
                  "the Schur AMG step replaces the Schur inner solve and the block-triangular step");
     if (vcycles > 0 && !P->P.amg) P->P.ready = false;  // the hierarchy is built by the next update
     P->P.schur_amg = vcycles;
+  });
+}
+
+int ocmp_precond_schur_amg_levels(const ocmp_precond *P, int64_t *rows, int64_t *nnz, int max_levels,
+                                  int *levels) {
+  return guard([&] {
+    OCMP_REQUIRE(P && levels, "NULL argument");
+    OCMP_REQUIRE(max_levels >= 0 && (max_levels == 0 || (rows && nnz)), "bad level arrays");
+    const Amg *M = P->P.amg;
+    *levels = M ? (int)M->lv.size() : 0;
+    for (int l = 0; l < max_levels; ++l) {
+      const bool on = l < *levels;
+      rows[l] = on ? M->lv[l]->A->n : 0;
+      nnz[l] = on ? M->lv[l]->A->nnz : 0;
+    }
   });
 }
 

@@ -452,19 +452,24 @@ void build_schur_matrix(Precond *P);
 // R[k] = P[rslot[k]]; Q = A P with Q[q] = sum over (A slot, P slot) pairs; A_next = P^T Q with
 // A_next[c] = sum over (P slot, Q slot) pairs. The last level is solved by a dense inverse.
 struct AmgLevel {
-  Matrix *A = nullptr, *P = nullptr, *R = nullptr;
+  // W = (I - w D^-1 A) P (w the smoother's damping) folds the damped-Jacobi smoothing steps of a
+  // V(1,1) cycle into the grid transfers: the restricted residual after pre-smoothing from zero is
+  // W^T b, and the post-smoothed iterate is z + W x_c with z = w D^-1 (2 b - w A D^-1 b)
+  Matrix *A = nullptr, *P = nullptr, *W = nullptr, *Wt = nullptr;
   bool own_A = false;
   double omega = 0.0;                       // prolongation smoothing weight 4 / (3 lambda_max(D^-1 A))
   int64_t nq = 0;                           // nonzeros of Q = A P
   DBuf<int32_t> d_prow, d_plptr, d_plist;   // per P entry: row, list range; A slots
   DBuf<uint8_t> d_pident;                   // per P entry: 1 where the column is the row's own aggregate
-  DBuf<int32_t> d_rslot;                    // per R entry: its P slot
+  DBuf<int32_t> d_wrow, d_wps, d_wqs;       // per W entry: row, P slot and Q slot (-1: none)
+  DBuf<int32_t> d_wtslot;                   // per W^T entry: its W slot
   DBuf<int32_t> d_qptr, d_qa, d_qp;         // Q entries' (A slot, P slot) pair lists
   DBuf<int32_t> d_cptr, d_cp, d_cq;         // next A's (P slot, Q slot) pair lists
   DBuf<double> q;                           // Q values
   DBuf<double> dinv;                        // Jacobi of A (0 on rows left out)
   DBuf<float> dinv32;
-  DBuf<double> vd;                          // V-cycle vectors: b, x, y, r (4 n)
+  DBuf<double> vd;                          // V-cycle vectors: b, x, z (3 n; level 0 also the
+                                            // further cycles' residual and correction)
   DBuf<float> vf;
 };
 struct Amg {

@@ -32,6 +32,7 @@ constexpr int SCHUR_SCAL = 8;        // scalars at the head of the Schur inner s
 constexpr int AMG_DENSE_MAX = 1200;
 // a last level of at most this many rows is inverted in LDS (n x 2n doubles: 36 KB)
 constexpr int AMG_LDS_MAX = 48;
+constexpr double AMG_SMOOTH = 2.0 / 3.0;  // damping of the AMG levels' Jacobi smoothing
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
 
@@ -178,6 +179,90 @@ __device__ __forceinline__ void spmv_epilogue(int mode, int64_t r, double s, XT 
   }
 }
 
+// One row block of a CSR product: the x values at the block's distinct columns (xl(c) loads the
+// value at column c, xs(c, v) then scales it) are staged in LDS, each thread forms its share of the products, and 4 lanes
+// per row sum them in a fixed order; ef(r, s) receives each row's sum. The LDS buffer holds first
+// the staged x values, then (after every thread has read its operands) the products, so the
+// kernel's LDS is exactly NNZB values (a power of two: whole blocks per CU).
+template <typename VT, typename XT, int NNZB, typename XL, typename XS, typename EF>
+__device__ __forceinline__ void csr_row_block(int lb, const int32_t *__restrict__ rblk, const int32_t *__restrict__ rp,
+                                              const int32_t *__restrict__ col, const int32_t *__restrict__ ubase,
+                                              const int32_t *__restrict__ ucol, const uint16_t *__restrict__ lcol,
+                                              const VT *__restrict__ val, double *lds, XL xl, XS xsc, EF ef) {
+  constexpr int SPMV_PER_THREAD = NNZB / NTHR;
+  XT *xs = reinterpret_cast<XT *>(lds);
+  XT *prod = xs;
+  const int32_t r0 = rblk[lb], r1 = rblk[lb + 1];
+  const int32_t j0 = rp[r0], j1 = rp[r1];
+  const int nz = j1 - j0;
+  const int nrows = r1 - r0;
+  if (nz <= NNZB) {
+    // rows of the first pass: fetch their bounds early so the loads overlap the staging
+    const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
+    int a0 = 0, e0 = 0;
+    if (ri0 < nrows) {
+      a0 = rp[r0 + ri0] - j0;
+      e0 = rp[r0 + ri0 + 1] - j0;
+    }
+    int li[SPMV_PER_THREAD];
+    VT vv[SPMV_PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < SPMV_PER_THREAD; ++k) {
+      const int t = threadIdx.x + k * NTHR;
+      li[k] = -1;
+      vv[k] = (VT)0;
+      if (t < nz) {
+        li[k] = lcol[j0 + t];
+        vv[k] = val[j0 + t];
+      }
+    }
+    // x at the block's distinct columns (nu >= 1 since nz >= 1; indices past nu are clamped
+    // so both gathers of a round are unconditional)
+    const int u0 = ubase[lb], nu = ubase[lb + 1] - u0;
+    for (int base = 0; base < nu; base += 2 * NTHR) {
+      const int t0 = base + threadIdx.x, t1 = t0 + NTHR;
+      const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
+      XT x0 = xl(c0), x1 = xl(c1);
+      x0 = xsc(c0, x0);
+      x1 = xsc(c1, x1);
+      if (t0 < nu) xs[t0] = x0;
+      if (t1 < nu) xs[t1] = x1;
+    }
+    __syncthreads();
+    XT pv[SPMV_PER_THREAD];
+#pragma unroll
+    for (int k = 0; k < SPMV_PER_THREAD; ++k) pv[k] = li[k] >= 0 ? (XT)vv[k] * xs[li[k]] : (XT)0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SPMV_PER_THREAD; ++k)
+      if (li[k] >= 0) prod[threadIdx.x + k * NTHR] = pv[k];
+    __syncthreads();
+    for (int base = 0; base < nrows; base += NTHR / 4) {
+      const int ri = base + ri0;
+      int a = a0, e = e0;
+      if (base > 0 && ri < nrows) {
+        a = rp[r0 + ri] - j0;
+        e = rp[r0 + ri + 1] - j0;
+      }
+      double s = 0.0;
+      if (ri < nrows)
+        for (int t = a + q; t < e; t += 4) s += (double)prod[t];
+      s += __shfl_xor(s, 1, 4);
+      s += __shfl_xor(s, 2, 4);
+      if (ri < nrows && q == 0) ef((int64_t)(r0 + ri), s);
+    }
+  } else {
+    // a single row longer than the staging buffer
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nz; t += NTHR) {
+      const int32_t c = col[j0 + t];
+      s += (double)((XT)val[j0 + t] * xsc(c, xl(c)));
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) ef((int64_t)r0, s);
+  }
+}
+
 template <typename VT, typename XT, int NNZB>
 __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t *__restrict__ rblk,
                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
@@ -191,95 +276,19 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
                                                    double *__restrict__ part, XT dsh, const XT *__restrict__ xp,
                                                    XT csh, const int *__restrict__ skip) {
   if (skip && *skip) return;  // a converged inner solve: the rest of its fixed launch sequence is empty
-  // one LDS buffer: first the x values at the distinct columns, then (after every thread has
-  // read its operands) the products, then the block-sum scratch, so the kernel's LDS is
-  // exactly NNZB values (a power of two: whole blocks per CU)
-  constexpr int SPMV_PER_THREAD = NNZB / NTHR;
   __shared__ double lds[NNZB * sizeof(XT) / sizeof(double)];
-  XT *xs = reinterpret_cast<XT *>(lds);
-  XT *prod = xs;
-  double *sh = lds;
   const int lb = xcd_block();
   double acc = 0.0;
   const XT scale = (mode == 3 && sx) ? (XT)sx[0] : (XT)1;
-  if (lb < nrblk) {
-    const int32_t r0 = rblk[lb], r1 = rblk[lb + 1];
-    const int32_t j0 = rp[r0], j1 = rp[r1];
-    const int nz = j1 - j0;
-    const int nrows = r1 - r0;
-    if (nz <= NNZB) {
-      // rows of the first pass: fetch their bounds early so the loads overlap the staging
-      const int ri0 = threadIdx.x >> 2, q = threadIdx.x & 3;
-      int a0 = 0, e0 = 0;
-      if (ri0 < nrows) {
-        a0 = rp[r0 + ri0] - j0;
-        e0 = rp[r0 + ri0 + 1] - j0;
-      }
-      int li[SPMV_PER_THREAD];
-      VT vv[SPMV_PER_THREAD];
-#pragma unroll
-      for (int k = 0; k < SPMV_PER_THREAD; ++k) {
-        const int t = threadIdx.x + k * NTHR;
-        li[k] = -1;
-        vv[k] = (VT)0;
-        if (t < nz) {
-          li[k] = lcol[j0 + t];
-          vv[k] = val[j0 + t];
-        }
-      }
-      // x at the block's distinct columns (nu >= 1 since nz >= 1; indices past nu are clamped
-      // so both gathers of a round are unconditional)
-      const int u0 = ubase[lb], nu = ubase[lb + 1] - u0;
-      for (int base = 0; base < nu; base += 2 * NTHR) {
-        const int t0 = base + threadIdx.x, t1 = t0 + NTHR;
-        const int32_t c0 = ucol[u0 + min(t0, nu - 1)], c1 = ucol[u0 + min(t1, nu - 1)];
-        XT x0 = x[c0], x1 = x[c1];
-        if (mode >= 3 && cs) {
-          x0 *= cs[c0];
-          x1 *= cs[c1];
-        }
-        if (t0 < nu) xs[t0] = x0;
-        if (t1 < nu) xs[t1] = x1;
-      }
-      __syncthreads();
-      XT pv[SPMV_PER_THREAD];
-#pragma unroll
-      for (int k = 0; k < SPMV_PER_THREAD; ++k) pv[k] = li[k] >= 0 ? (XT)vv[k] * xs[li[k]] : (XT)0;
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < SPMV_PER_THREAD; ++k)
-        if (li[k] >= 0) prod[threadIdx.x + k * NTHR] = pv[k];
-      __syncthreads();
-      for (int base = 0; base < nrows; base += NTHR / 4) {
-        const int ri = base + ri0;
-        int a = a0, e = e0;
-        if (base > 0 && ri < nrows) {
-          a = rp[r0 + ri] - j0;
-          e = rp[r0 + ri + 1] - j0;
-        }
-        double s = 0.0;
-        if (ri < nrows)
-          for (int t = a + q; t < e; t += 4) s += (double)prod[t];
-        s += __shfl_xor(s, 1, 4);
-        s += __shfl_xor(s, 2, 4);
-        if (ri < nrows && q == 0) spmv_epilogue<VT, XT>(mode, r0 + ri, s, y, x, b, mask, scale, acc, dsh, xp, csh);
-      }
-    } else {
-      // a single row longer than the staging buffer
-      double s = 0.0;
-      for (int t = threadIdx.x; t < nz; t += NTHR) {
-        const int32_t c = col[j0 + t];
-        XT xv = x[c];
-        if (mode >= 3 && cs) xv *= cs[c];
-        s += (double)((XT)val[j0 + t] * xv);
-      }
-      s = block_sum(s, sh);
-      if (threadIdx.x == 0) spmv_epilogue<VT, XT>(mode, r0, s, y, x, b, mask, scale, acc, dsh, xp, csh);
-    }
-  }
+  if (lb < nrblk)
+    csr_row_block<VT, XT, NNZB>(
+        lb, rblk, rp, col, ubase, ucol, lcol, val, lds,
+        [&](int32_t c) { return x[c]; },
+        [&](int32_t c, XT v) { return (mode >= 3 && cs) ? v * cs[c] : v; },
+        [&](int64_t r, double s) { spmv_epilogue<VT, XT>(mode, r, s, y, x, b, mask, scale, acc, dsh, xp, csh); });
   if (part) {
-    __syncthreads();  // sh aliases the products
-    acc = block_sum(acc, sh);
+    __syncthreads();  // the block-sum scratch aliases the products
+    acc = block_sum(acc, lds);
     if (threadIdx.x == 0) part[lb] = acc;
   }
 }
@@ -1397,17 +1406,61 @@ __global__ void k_amg_dense_mv(int64_t n, const T *__restrict__ inv, const T *__
   if (threadIdx.x == 0) x[i] = (T)s;
 }
 
-// x = omega dinv b;  x = y + omega dinv r
-template <typename T>
-__global__ void k_amg_jac0(int64_t n, const T *__restrict__ dinv, const T *__restrict__ b, T w, T *__restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = w * dinv[i] * b[i];
+// W = P - w D^-1 Q entrywise on W's pattern (the smoothing folded into the prolongation)
+__global__ void k_amg_wvals(int64_t nw, const int32_t *__restrict__ wrow, const int32_t *__restrict__ wps,
+                            const int32_t *__restrict__ wqs, const double *__restrict__ pval,
+                            const double *__restrict__ qval, const double *__restrict__ dinv, double w,
+                            double *__restrict__ wval, float *__restrict__ wval32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nw) return;
+  const int32_t ps = wps[e], qs = wqs[e];
+  const double v = (ps >= 0 ? pval[ps] : 0.0) - w * dinv[wrow[e]] * (qs >= 0 ? qval[qs] : 0.0);
+  wval[e] = v;
+  if (wval32) wval32[e] = (float)v;
 }
+
+// a CSR matrix with its row blocks, as one kernel argument
 template <typename T>
-__global__ void k_amg_jac1(int64_t n, const T *__restrict__ dinv, const T *__restrict__ y, const T *__restrict__ r,
-                           T w, T *__restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = y[i] + w * dinv[i] * r[i];
+struct RbCsr {
+  int64_t nrblk;
+  const int32_t *rblk, *rp, *col, *ubase, *ucol;
+  const uint16_t *lcol;
+  const T *val;
+};
+template <typename T>
+static RbCsr<T> rb_csr(const Matrix *A, const T *val) {
+  const RowBlocks &rb = A->rb64;
+  return {rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p, rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val};
+}
+
+// The descending half of a V(1,1) level with the input b = g u (g = -sig[0], or 1 without sig):
+// the first ga logical blocks form z = w D^-1 (2 b - w A D^-1 b) (the iterate after damped-Jacobi
+// pre-smoothing from zero and post-smoothing without the coarse correction), the rest the
+// restricted residual b_c = W^T b; bout (optional) receives b.
+template <typename T, int NNZB>
+__global__ void __launch_bounds__(NTHR) k_amg_down(int ga, RbCsr<T> A, RbCsr<T> Wt, const T *__restrict__ u,
+                                                   const T *__restrict__ dinv, const double *__restrict__ sig,
+                                                   T w, T *__restrict__ z, T *__restrict__ bc, T *__restrict__ bout) {
+  __shared__ double lds[NNZB * sizeof(T) / sizeof(double)];
+  const int lb = xcd_block();
+  const double g = sig ? -sig[0] : 1.0;
+  if (lb < ga) {
+    if (lb >= A.nrblk) return;
+    csr_row_block<T, T, NNZB>(
+        lb, A.rblk, A.rp, A.col, A.ubase, A.ucol, A.lcol, A.val, lds, [&](int32_t c) { return u[c]; },
+        [&](int32_t c, T v) { return v * dinv[c]; },
+        [&](int64_t r, double s) {
+          const double b = g * (double)u[r];
+          z[r] = (T)((double)w * (double)dinv[r] * (2.0 * b - (double)w * g * s));
+          if (bout) bout[r] = (T)b;
+        });
+  } else {
+    const int wb = lb - ga;
+    if (wb >= Wt.nrblk) return;
+    csr_row_block<T, T, NNZB>(
+        wb, Wt.rblk, Wt.rp, Wt.col, Wt.ubase, Wt.ucol, Wt.lcol, Wt.val, lds, [&](int32_t c) { return u[c]; },
+        [&](int32_t, T v) { return v; }, [&](int64_t r, double s) { bc[r] = (T)(g * s); });
+  }
 }
 
 // b = -s v
@@ -1439,7 +1492,7 @@ static void amg_update(Precond *Pc, bool mixed) {
         k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
       }
     }
-    const size_t nv = (size_t)(l == 0 ? 6 : 4) * n;
+    const size_t nv = (size_t)(l == 0 ? 5 : 3) * n;
     if (mixed) {
       L.dinv32.alloc(n);
       if (l == 0) OCMP_HIP(hipMemcpyAsync(L.dinv32.p, Pc->sdinv32.p, n * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -1460,75 +1513,78 @@ static void amg_update(Precond *Pc, bool mixed) {
         k_amg_invert<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
       break;
     }
-    Matrix *P = L.P, *R = L.R, *An = M.lv[l + 1]->A;
+    Matrix *P = L.P, *W = L.W, *Wt = L.Wt, *An = M.lv[l + 1]->A;
     if (mixed) {
-      P->d_val32.alloc(P->nnz);
-      R->d_val32.alloc(R->nnz);
+      W->d_val32.alloc(W->nnz);
+      Wt->d_val32.alloc(Wt->nnz);
     }
     k_amg_pvals<<<nblk(P->nnz), NTHR, 0, st>>>(P->nnz, L.d_prow.p, L.d_plptr.p, L.d_plist.p, L.d_pident.p, A->d_val.p,
-                                               L.dinv.p, L.omega, P->d_val.p, mixed ? P->d_val32.p : nullptr);
-    k_gather_vals<<<nblk(R->nnz), NTHR, 0, st>>>(R->nnz, L.d_rslot.p, P->d_val.p, R->d_val.p,
-                                                 mixed ? R->d_val32.p : nullptr);
+                                               L.dinv.p, L.omega, P->d_val.p, nullptr);
     k_amg_pairs<<<nblk(L.nq), NTHR, 0, st>>>(L.nq, L.d_qptr.p, L.d_qa.p, L.d_qp.p, A->d_val.p, P->d_val.p, L.q.p,
                                              nullptr);
+    k_amg_wvals<<<nblk(W->nnz), NTHR, 0, st>>>(W->nnz, L.d_wrow.p, L.d_wps.p, L.d_wqs.p, P->d_val.p, L.q.p, L.dinv.p,
+                                               AMG_SMOOTH, W->d_val.p, mixed ? W->d_val32.p : nullptr);
+    k_gather_vals<<<nblk(Wt->nnz), NTHR, 0, st>>>(Wt->nnz, L.d_wtslot.p, W->d_val.p, Wt->d_val.p,
+                                                  mixed ? Wt->d_val32.p : nullptr);
     k_amg_pairs<<<nblk(An->nnz), NTHR, 0, st>>>(An->nnz, L.d_cptr.p, L.d_cp.p, L.d_cq.p, P->d_val.p, L.q.p,
                                                 An->d_val.p, nullptr);
   }
 }
 
-// one V(1,1) cycle with damped Jacobi (2/3) on level l: x ~ A_l^{-1} b
-constexpr double AMG_SMOOTH = 2.0 / 3.0;
+// one V(1,1) cycle with damped Jacobi (weight AMG_SMOOTH) on level l: x ~ A_l^{-1} b with
+// b = -sig[0] u (b = u without sig); bout (optional) receives b. Two launches per level (the
+// smoothing steps ride on the grid transfers through W) and the dense solve on the last.
 template <typename T>
-static void amg_vcycle(Amg &M, size_t l, const T *b, T *x, hipStream_t st) {
+static void amg_vcycle(Amg &M, size_t l, const T *u, const double *sig, T *x, T *bout, hipStream_t st) {
   AmgLevel &L = *M.lv[l];
   const int64_t n = L.A->n;
-  const T *dinv;
-  if constexpr (sizeof(T) == 8)
-    dinv = L.dinv.p;
-  else
-    dinv = L.dinv32.p;
-  if (l + 1 == M.lv.size()) {
-    const T *inv;
+  auto vecs = [](AmgLevel &V) -> T * {
     if constexpr (sizeof(T) == 8)
-      inv = M.cinv.p;
+      return V.vd.p;
     else
-      inv = M.cinv32.p;
-    k_amg_dense_mv<T><<<(unsigned)n, 64, 0, st>>>(n, inv, b, x);
-    return;
-  }
-  AmgLevel &C = *M.lv[l + 1];
-  T *vb;
-  T *cb;
-  if constexpr (sizeof(T) == 8) {
-    vb = L.vd.p;
-    cb = C.vd.p;
-  } else {
-    vb = L.vf.p;
-    cb = C.vf.p;
-  }
-  T *y = vb + 2 * n, *r = vb + 3 * n;
-  T *bc = cb, *xc = cb + C.A->n;
-  const T w = (T)AMG_SMOOTH;
+      return V.vf.p;
+  };
   auto vals = [](Matrix *A) -> const T * {
     if constexpr (sizeof(T) == 8)
       return A->d_val.p;
     else
       return A->d_val32.p;
   };
-  k_amg_jac0<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, b, w, x);
-  spmv_dispatch<T, T>(L.A, vals(L.A), x, r, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L.A), st,
-                      (T)0, b, (T)0);
-  spmv_dispatch<T, T>(L.R, vals(L.R), r, bc, nullptr, nullptr, nullptr, nullptr, 0, nullptr, spmv_grid<T, T>(L.R), st);
-  amg_vcycle<T>(M, l + 1, bc, xc, st);
-  spmv_dispatch<T, T>(L.P, vals(L.P), xc, y, nullptr, nullptr, nullptr, nullptr, 6, nullptr, spmv_grid<T, T>(L.P), st,
-                      (T)0, x, (T)0);
-  spmv_dispatch<T, T>(L.A, vals(L.A), y, r, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L.A), st,
-                      (T)0, b, (T)0);
-  k_amg_jac1<T><<<nblk(n), NTHR, 0, st>>>(n, dinv, y, r, w, x);
+  if (l + 1 == M.lv.size()) {
+    const T *inv;
+    if constexpr (sizeof(T) == 8)
+      inv = M.cinv.p;
+    else
+      inv = M.cinv32.p;
+    const T *b = u;
+    if (sig) {
+      T *bb = bout ? bout : vecs(L);
+      k_neg_scale<T><<<nblk(n), NTHR, 0, st>>>(n, u, sig, bb);
+      b = bb;
+    } else if (bout) {
+      OCMP_HIP(hipMemcpyAsync(bout, u, n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    k_amg_dense_mv<T><<<(unsigned)n, 64, 0, st>>>(n, inv, b, x);
+    return;
+  }
+  AmgLevel &C = *M.lv[l + 1];
+  const T *dinv;
+  if constexpr (sizeof(T) == 8)
+    dinv = L.dinv.p;
+  else
+    dinv = L.dinv32.p;
+  T *z = vecs(L) + 2 * n;
+  T *bc = vecs(C), *xc = vecs(C) + C.A->n;
+  const int ga = spmv_grid<double, double>(L.A), gw = spmv_grid<double, double>(L.Wt);
+  const RbCsr<T> a = rb_csr<T>(L.A, vals(L.A)), wt = rb_csr<T>(L.Wt, vals(L.Wt));
+  k_amg_down<T, SPMV_NNZB64><<<ga + gw, NTHR, 0, st>>>(ga, a, wt, u, dinv, sig, (T)AMG_SMOOTH, z, bc, bout);
+  amg_vcycle<T>(M, l + 1, bc, nullptr, xc, nullptr, st);
+  spmv_dispatch<T, T>(L.W, vals(L.W), xc, x, nullptr, nullptr, nullptr, nullptr, 6, nullptr, spmv_grid<T, T>(L.W), st,
+                      (T)0, z, (T)0);
 }
 
 // z[s0:s1] = -(cycles V-cycles on S) (s v)[s0:s1]: the first cycle from zero, each further one on the
-// residual of the current iterate. Level 0's vectors: b, (unused), y, r (the cycle's own), then the
+// residual of the current iterate. Level 0's vectors: b, (unused), z (the cycle's own), then the
 // residual and the correction of the further cycles.
 template <typename T>
 __global__ void k_add(int64_t n, const T *__restrict__ e, T *__restrict__ x) {
@@ -1552,15 +1608,15 @@ static void schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z) {
     vb = L0.vf.p;
     val = L0.A->d_val32.p;
   }
-  T *b = vb, *rr = vb + 4 * n, *ee = vb + 5 * n;
+  T *b = vb, *rr = vb + 3 * n, *ee = vb + 4 * n;
   T *x = z + Pc->s0;
   KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "schur_amg_vcycle_f64" : "schur_amg_vcycle_f32"));
-  k_neg_scale<T><<<nblk(n), NTHR, 0, st>>>(n, v + Pc->s0, sv, b);
-  amg_vcycle<T>(M, 0, b, x, st);
+  const bool more = Pc->schur_amg > 1;
+  amg_vcycle<T>(M, 0, v + Pc->s0, sv, x, more ? b : nullptr, st);
   for (int c = 1; c < Pc->schur_amg; ++c) {
     spmv_dispatch<T, T>(L0.A, val, x, rr, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L0.A), st,
                         (T)0, b, (T)0);
-    amg_vcycle<T>(M, 0, rr, ee, st);
+    amg_vcycle<T>(M, 0, rr, nullptr, ee, nullptr, st);
     k_add<T><<<nblk(n), NTHR, 0, st>>>(n, ee, x);
   }
 }

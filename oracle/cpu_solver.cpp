@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 namespace {
@@ -80,6 +81,323 @@ void block_apply(int64_t n, int nb, const std::vector<double> &binv, const doubl
       for (int j = 0; j < nb; ++j) s += binv[((size_t)e * nb + i) * nb + j] * v[e * nb + j];
       z[e * nb + i] = s;
     }
+}
+
+// ---- smoothed-aggregation AMG, in the textbook form (sparse products P = (I - w D^-1 A) P_t,
+// A_c = P^T A P, a V(1,1) cycle with pre-smoothing, restriction, coarse correction, prolongation,
+// post-smoothing). Aggregation: strength |a_ij| >= theta sqrt(|a_ii a_jj|); pass 1 roots whose
+// strong neighbours are all free, pass 2 joins a strong first-pass aggregate, pass 3 groups the
+// rest; prolongation damping 4 / (3 lambda_max(D^-1 A)) from 30 power steps; damped Jacobi (2/3)
+// smoothing; a dense inverse on the last level (at most 48 rows, up to 1200 where coarsening
+// stalls). Rows with a zero Jacobi entry on the first level are left out of every operator.
+
+constexpr double AMG_THETA = 0.08, AMG_SMOOTH = 2.0 / 3.0;
+constexpr int64_t AMG_COARSE = 48, AMG_DENSE_LIMIT = 1200;
+constexpr int AMG_MAX_LEVELS = 10;
+
+struct Csr {
+  int64_t n = 0, m = 0;  // rows, columns
+  std::vector<int32_t> rp, col;
+  std::vector<double> val;
+};
+
+void csr_mv(const Csr &A, const double *x, double *y) {
+  spmv(A.n, A.rp.data(), A.col.data(), A.val.data(), x, y);
+}
+
+Csr csr_transpose(const Csr &A) {
+  Csr T;
+  T.n = A.m;
+  T.m = A.n;
+  T.rp.assign(T.n + 1, 0);
+  for (int32_t c : A.col) T.rp[c + 1]++;
+  for (int64_t i = 0; i < T.n; ++i) T.rp[i + 1] += T.rp[i];
+  T.col.resize(A.col.size());
+  T.val.resize(A.col.size());
+  std::vector<int32_t> fill(T.rp.begin(), T.rp.end() - 1);
+  for (int64_t i = 0; i < A.n; ++i)
+    for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+      const int32_t t = fill[A.col[k]]++;
+      T.col[t] = (int32_t)i;
+      T.val[t] = A.val[k];
+    }
+  return T;
+}
+
+// C = A B, row by row with a dense accumulator per thread (columns sorted)
+Csr csr_product(const Csr &A, const Csr &B) {
+  std::vector<std::vector<int32_t>> cc(A.n);
+  std::vector<std::vector<double>> cv(A.n);
+#pragma omp parallel
+  {
+    std::vector<double> acc(B.m, 0.0);
+    std::vector<char> on(B.m, 0);
+    std::vector<int32_t> t;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t i = 0; i < A.n; ++i) {
+      t.clear();
+      for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+        const double a = A.val[k];
+        const int32_t j = A.col[k];
+        for (int32_t e = B.rp[j]; e < B.rp[j + 1]; ++e) {
+          const int32_t c = B.col[e];
+          if (!on[c]) {
+            on[c] = 1;
+            t.push_back(c);
+          }
+          acc[c] += a * B.val[e];
+        }
+      }
+      std::sort(t.begin(), t.end());
+      cc[i] = t;
+      cv[i].resize(t.size());
+      for (size_t q = 0; q < t.size(); ++q) {
+        cv[i][q] = acc[t[q]];
+        acc[t[q]] = 0.0;
+        on[t[q]] = 0;
+      }
+    }
+  }
+  Csr C;
+  C.n = A.n;
+  C.m = B.m;
+  C.rp.assign(A.n + 1, 0);
+  for (int64_t i = 0; i < A.n; ++i) C.rp[i + 1] = C.rp[i] + (int32_t)cc[i].size();
+  C.col.resize(C.rp[A.n]);
+  C.val.resize(C.rp[A.n]);
+  for (int64_t i = 0; i < A.n; ++i) {
+    std::copy(cc[i].begin(), cc[i].end(), C.col.begin() + C.rp[i]);
+    std::copy(cv[i].begin(), cv[i].end(), C.val.begin() + C.rp[i]);
+  }
+  return C;
+}
+
+double csr_diag(const Csr &A, int64_t i) {
+  for (int32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+    if (A.col[k] == i) return A.val[k];
+  return 0.0;
+}
+
+struct AmgLevelCpu {
+  Csr A, P, R;
+  std::vector<double> dinv, x, r, bc;
+};
+
+struct AmgCpu {
+  std::vector<AmgLevelCpu> lv;
+  std::vector<double> inv;  // dense inverse of the last level (row-major)
+  int64_t applications = 0;
+};
+
+// dense inverse of the last level; rows with dinv 0 are left out (zero rows and columns)
+void amg_dense_inverse(const Csr &A, const std::vector<double> &dinv, std::vector<double> &inv) {
+  const int64_t n = A.n;
+  std::vector<int64_t> id;
+  for (int64_t i = 0; i < n; ++i)
+    if (dinv[i] != 0.0) id.push_back(i);
+  const int64_t k = (int64_t)id.size();
+  std::vector<int64_t> pos(n, -1);
+  for (int64_t a = 0; a < k; ++a) pos[id[a]] = a;
+  std::vector<double> M((size_t)k * 2 * k, 0.0);
+  for (int64_t a = 0; a < k; ++a) {
+    for (int32_t q = A.rp[id[a]]; q < A.rp[id[a] + 1]; ++q)
+      if (pos[A.col[q]] >= 0) M[(size_t)a * 2 * k + pos[A.col[q]]] = A.val[q];
+    M[(size_t)a * 2 * k + k + a] = 1.0;
+  }
+  for (int64_t c = 0; c < k; ++c) {  // Gauss-Jordan with partial pivoting
+    int64_t p = c;
+    for (int64_t a = c + 1; a < k; ++a)
+      if (std::fabs(M[(size_t)a * 2 * k + c]) > std::fabs(M[(size_t)p * 2 * k + c])) p = a;
+    if (p != c)
+      for (int64_t j = 0; j < 2 * k; ++j) std::swap(M[(size_t)c * 2 * k + j], M[(size_t)p * 2 * k + j]);
+    const double ip = 1.0 / M[(size_t)c * 2 * k + c];
+    for (int64_t j = 0; j < 2 * k; ++j) M[(size_t)c * 2 * k + j] *= ip;
+    for (int64_t a = 0; a < k; ++a) {
+      const double f = M[(size_t)a * 2 * k + c];
+      if (a == c || f == 0.0) continue;
+      for (int64_t j = 0; j < 2 * k; ++j) M[(size_t)a * 2 * k + j] -= f * M[(size_t)c * 2 * k + j];
+    }
+  }
+  inv.assign((size_t)n * n, 0.0);
+  for (int64_t a = 0; a < k; ++a)
+    for (int64_t c = 0; c < k; ++c) inv[(size_t)id[a] * n + id[c]] = M[(size_t)a * 2 * k + k + c];
+}
+
+// the hierarchy of A (dinv0: its Jacobi, 0 on rows left out)
+void amg_setup(AmgCpu &M, Csr A, std::vector<double> dinv) {
+  std::vector<uint8_t> use(A.n);
+  for (int64_t i = 0; i < A.n; ++i) use[i] = dinv[i] != 0.0;
+  for (int l = 0;; ++l) {
+    const int64_t n = A.n;
+    AmgLevelCpu L;
+    auto finish = [&]() {
+      if (n > AMG_DENSE_LIMIT) throw std::runtime_error("AMG: the last level is too large for a dense inverse");
+      amg_dense_inverse(A, dinv, M.inv);
+      L.A = std::move(A);
+      L.dinv = std::move(dinv);
+      L.x.resize(n);
+      M.lv.push_back(std::move(L));
+    };
+    if (n <= AMG_COARSE || l == AMG_MAX_LEVELS - 1) {
+      finish();
+      return;
+    }
+    std::vector<double> dg(n);
+    for (int64_t i = 0; i < n; ++i) dg[i] = csr_diag(A, i);
+    auto strong = [&](int64_t i, int32_t q) {
+      const int32_t j = A.col[q];
+      return j != i && use[j] && std::fabs(A.val[q]) >= AMG_THETA * std::sqrt(std::fabs(dg[i] * dg[j]));
+    };
+    std::vector<int32_t> agg(n, -1);
+    int32_t na = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      if (!use[i] || agg[i] >= 0) continue;
+      bool free_nb = true;
+      for (int32_t q = A.rp[i]; q < A.rp[i + 1]; ++q) free_nb = free_nb && !(strong(i, q) && agg[A.col[q]] >= 0);
+      if (!free_nb) continue;
+      agg[i] = na;
+      for (int32_t q = A.rp[i]; q < A.rp[i + 1]; ++q)
+        if (strong(i, q)) agg[A.col[q]] = na;
+      ++na;
+    }
+    const std::vector<int32_t> first = agg;
+    for (int64_t i = 0; i < n; ++i) {
+      if (!use[i] || agg[i] >= 0) continue;
+      for (int32_t q = A.rp[i]; q < A.rp[i + 1]; ++q)
+        if (strong(i, q) && first[A.col[q]] >= 0) {
+          agg[i] = first[A.col[q]];
+          break;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      if (!use[i] || agg[i] >= 0) continue;
+      agg[i] = na;
+      for (int32_t q = A.rp[i]; q < A.rp[i + 1]; ++q)
+        if (strong(i, q) && agg[A.col[q]] < 0) agg[A.col[q]] = na;
+      ++na;
+    }
+    if (na == 0 || na > (int64_t)(0.9 * (double)n)) {
+      finish();
+      return;
+    }
+    // lambda_max(D^-1 A) on the usable rows by power steps from a fixed start
+    std::vector<double> v(n), y(n);
+    for (int64_t i = 0; i < n; ++i) v[i] = use[i] ? 1.0 + 0.5 * std::sin(0.7 * (double)i) : 0.0;
+    double lam = 1.0;
+    for (int it = 0; it < 30; ++it) {
+      double yy = 0.0, yv = 0.0, vv = 0.0;
+      for (int64_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        if (use[i] && dg[i] != 0.0) {
+          for (int32_t q = A.rp[i]; q < A.rp[i + 1]; ++q)
+            if (use[A.col[q]]) s += A.val[q] * v[A.col[q]];
+          s /= dg[i];
+        }
+        y[i] = s;
+        yy += s * s;
+        yv += s * v[i];
+        vv += v[i] * v[i];
+      }
+      lam = vv > 0.0 ? yv / vv : 1.0;
+      for (int64_t i = 0; i < n; ++i) v[i] = yy > 0.0 ? y[i] / std::sqrt(yy) : 0.0;
+    }
+    const double omega = 4.0 / (3.0 * std::max(lam, 1e-300));
+    // tentative prolongation, then P = P_t - omega D^-1 A P_t on the usable rows
+    Csr Pt;
+    Pt.n = n;
+    Pt.m = na;
+    Pt.rp.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      if (use[i]) {
+        Pt.col.push_back(agg[i]);
+        Pt.val.push_back(1.0);
+      }
+      Pt.rp[i + 1] = (int32_t)Pt.col.size();
+    }
+    const Csr AP = csr_product(A, Pt);
+    Csr P;
+    P.n = n;
+    P.m = na;
+    P.rp.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      if (use[i]) {
+        const double f = dg[i] != 0.0 ? omega / dg[i] : 0.0;
+        for (int32_t q = AP.rp[i]; q < AP.rp[i + 1]; ++q) {
+          P.col.push_back(AP.col[q]);
+          P.val.push_back((AP.col[q] == agg[i] ? 1.0 : 0.0) - f * AP.val[q]);
+        }
+      }
+      P.rp[i + 1] = (int32_t)P.col.size();
+    }
+    Csr R = csr_transpose(P);
+    Csr Ac = csr_product(R, csr_product(A, P));
+    L.A = std::move(A);
+    L.P = std::move(P);
+    L.R = std::move(R);
+    L.dinv = std::move(dinv);
+    L.x.resize(n);
+    L.r.resize(n);
+    L.bc.resize(na);
+    M.lv.push_back(std::move(L));
+    A = std::move(Ac);
+    dinv.assign(A.n, 0.0);
+    for (int64_t i = 0; i < A.n; ++i) {
+      const double d = csr_diag(A, i);
+      dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+    use.assign(A.n, 1);
+  }
+}
+
+// x = V(1,1)-cycle approximation of A_l^{-1} b
+void amg_vcycle(AmgCpu &M, size_t l, const double *b, double *x) {
+  AmgLevelCpu &L = M.lv[l];
+  const int64_t n = L.A.n;
+  if (l + 1 == M.lv.size()) {
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int64_t j = 0; j < n; ++j) s += M.inv[(size_t)i * n + j] * b[j];
+      x[i] = s;
+    }
+    return;
+  }
+  AmgLevelCpu &Cl = M.lv[l + 1];
+  double *r = L.r.data();
+  const double *dv = L.dinv.data();
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) x[i] = AMG_SMOOTH * dv[i] * b[i];
+  csr_mv(L.A, x, r);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+  csr_mv(L.R, r, L.bc.data());
+  double *xc = Cl.x.data();
+  amg_vcycle(M, l + 1, L.bc.data(), xc);
+  csr_mv(L.P, xc, r);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) x[i] += r[i];
+  csr_mv(L.A, x, r);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) x[i] += AMG_SMOOTH * dv[i] * (b[i] - r[i]);
+}
+
+// x = (cycles V-cycles) b: the first from zero, each further one on the current residual
+void amg_apply(AmgCpu &M, int cycles, const double *b, double *x) {
+  const AmgLevelCpu &L0 = M.lv[0];
+  const int64_t n = L0.A.n;
+  amg_vcycle(M, 0, b, x);
+  std::vector<double> rr(cycles > 1 ? n : 0), ee(cycles > 1 ? n : 0);
+  for (int c = 1; c < cycles; ++c) {
+    csr_mv(L0.A, x, rr.data());
+    for (int64_t i = 0; i < n; ++i) rr[i] = b[i] - rr[i];
+    amg_vcycle(M, 0, rr.data(), ee.data());
+    for (int64_t i = 0; i < n; ++i) x[i] += ee[i];
+  }
+  ++M.applications;
+}
+
+void amg_levels(const AmgCpu &M, int64_t *levels) {
+  if (!levels) return;
+  for (int l = 0; l < AMG_MAX_LEVELS; ++l) levels[l] = l < (int)M.lv.size() ? M.lv[l].A.n : 0;
 }
 
 }  // namespace
@@ -216,6 +534,56 @@ int ocmp_cpu_pcg_jacobi(int64_t n, const int32_t *rp, const int32_t *col, const 
   return it;
 }
 
+// CG on the free dofs preconditioned with `cycles` AMG V-cycles of the matrix (its Jacobi on the free
+// rows, the other rows left out); Dirichlet entries of x are left as given. levels (optional, 10
+// entries) receives the rows of each level, 0 past the last.
+int ocmp_cpu_pcg_amg(int64_t n, const int32_t *rp, const int32_t *col, const double *val, const uint8_t *freem,
+                     const double *b, double *x, double tol, int maxit, int cycles, int64_t *levels,
+                     double *rel) {
+  Csr A;
+  A.n = A.m = n;
+  A.rp.assign(rp, rp + n + 1);
+  A.col.assign(col, col + rp[n]);
+  A.val.assign(val, val + rp[n]);
+  std::vector<double> dinv(n, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = csr_diag(A, i);
+    dinv[i] = freem[i] && d != 0.0 ? 1.0 / d : 0.0;
+  }
+  AmgCpu M;
+  amg_setup(M, A, dinv);
+  amg_levels(M, levels);
+  std::vector<double> r(n), z(n), p(n), q(n);
+  spmv(n, rp, col, val, x, r.data());
+  for (int64_t i = 0; i < n; ++i) r[i] = freem[i] ? b[i] - r[i] : 0.0;
+  double bn2 = 0.0;
+  for (int64_t i = 0; i < n; ++i) bn2 += freem[i] ? b[i] * b[i] : 0.0;
+  const double bn = std::sqrt(bn2);
+  amg_apply(M, cycles, r.data(), z.data());
+  p = z;
+  double g = dot(n, r.data(), z.data());
+  int it = 0;
+  double rn = std::sqrt(dot(n, r.data(), r.data()));
+  while (rn > tol * bn && it < maxit) {
+    spmv(n, rp, col, val, p.data(), q.data());
+    for (int64_t i = 0; i < n; ++i)
+      if (!freem[i]) q[i] = 0.0;
+    const double a = g / dot(n, p.data(), q.data());
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += a * p[i];
+      r[i] -= a * q[i];
+    }
+    amg_apply(M, cycles, r.data(), z.data());
+    const double g2 = dot(n, r.data(), z.data());
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i] + (g2 / g) * p[i];
+    g = g2;
+    rn = std::sqrt(dot(n, r.data(), r.data()));
+    ++it;
+  }
+  *rel = bn > 0.0 ? rn / bn : rn;
+  return it;
+}
+
 // Implicit-Euler Oseen operator of a Taylor-Hood pair on simplices (opencmp/models/ins.py:
 // 174-216 without the outlet's boundary term, with the time derivative of
 // opencmp/solvers/time_integration_schemes.py:135-170): per velocity component the block
@@ -332,11 +700,13 @@ int ocmp_cpu_assemble_oseen(int dim, int64_t nel, const double *X, int nlu, cons
 // Flexible GMRES(m) with the Schur preconditioner of the GPU path: on the rows outside [s0, s1)
 // z = v / a_ii; on [s0, s1) z = -S^{-1} v by Jacobi-PCG from zero until (r, Mr) <= stol^2 (r0, Mr0)
 // or smax iterations, S = C + B D^{-1} B^T with D_uu = max(|a_uu|, 1e-2 max_j |a_uj|) (columns
-// outside [s0, s1)) over the free velocity rows, and S_ii below 1e-10 max S_ii left out. Returns the
-// outer iterations; *inner gets the inner PCG iterations summed.
+// outside [s0, s1)) over the free velocity rows, and S_ii below 1e-10 max S_ii left out. With amg > 0
+// the pressure step is amg smoothed-aggregation V-cycles on S instead of the PCG (levels, optional,
+// gets the hierarchy's rows per level). Returns the outer iterations; *inner gets the inner PCG
+// iterations (or the V-cycle applications) summed.
 int ocmp_cpu_fgmres_schur(int64_t n, const int32_t *rp, const int32_t *col, const double *val, const uint8_t *freem,
                           int64_t s0, int64_t s1, const double *b, double *x, int m, double tol, int maxit,
-                          double stol, int smax, double *rel, int64_t *inner) {
+                          double stol, int smax, double *rel, int64_t *inner, int amg, int64_t *levels) {
   const int64_t ns = s1 - s0;
   std::vector<double> dinv(n, 0.0), dpos(n, 0.0);
 #pragma omp parallel for schedule(static)
@@ -404,9 +774,28 @@ int ocmp_cpu_fgmres_schur(int64_t n, const int32_t *rp, const int32_t *col, cons
   for (int64_t i = 0; i < ns; ++i) sd[i] = sd[i] > 1e-10 * smx ? 1.0 / sd[i] : 0.0;
   std::vector<double> pr(ns), pu(ns), pp(ns), pq(ns);
   int64_t inner_total = 0;
+  AmgCpu amgS;
+  if (amg > 0) {
+    Csr S;
+    S.n = S.m = ns;
+    S.rp.assign(ns + 1, 0);
+    for (int64_t i = 0; i < ns; ++i) S.rp[i + 1] = S.rp[i] + (int32_t)scol[i].size();
+    for (int64_t i = 0; i < ns; ++i) {
+      S.col.insert(S.col.end(), scol[i].begin(), scol[i].end());
+      S.val.insert(S.val.end(), sval[i].begin(), sval[i].end());
+    }
+    amg_setup(amgS, std::move(S), sd);
+    amg_levels(amgS, levels);
+  }
   // z[s0:s1] = -S^{-1} v[s0:s1]
   auto schur_solve = [&](const double *v, double *z) {
     double *xz = z + s0;
+    if (amg > 0) {
+      for (int64_t i = 0; i < ns; ++i) pr[i] = -v[s0 + i];
+      amg_apply(amgS, amg, pr.data(), xz);
+      inner_total = amgS.applications;
+      return;
+    }
     double g = 0.0;
 #pragma omp parallel for reduction(+ : g) schedule(static)
     for (int64_t i = 0; i < ns; ++i) {

@@ -554,9 +554,66 @@ def test_cpu_oseen_assembly_matches_oracle(dim):
     ci = A.indices.astype(np.int32)
     it = lib.ocmp_cpu_fgmres_schur(T.ndof, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, fr.ctypes.data,
                                    T.poff, T.ndof, b.ctypes.data, x.ctypes.data, 30, 1e-10, 5000, 1e-1, 200,
-                                   C.byref(rel), C.byref(inner))
+                                   C.byref(rel), C.byref(inner), 0, None)
     assert rel.value <= 1e-10 and 0 < it < 5000 and inner.value >= it
     xo = np.zeros(T.ndof)
     Af = A[free][:, free].tocsc()
     xo[free] = spl_solve(Af, b[free])
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def _laplacian_2d(k):
+    """5-point Laplacian on a k x k grid of nodes; the boundary rows are the non-free dofs."""
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(k, k))
+    A = (sp.kron(sp.identity(k), T) + sp.kron(T, sp.identity(k))).tocsr()
+    g = np.arange(k * k).reshape(k, k)
+    free = np.ones(k * k, np.uint8)
+    free[np.r_[g[0], g[-1], g[:, 0], g[:, -1]]] = 0
+    return A, free
+
+
+def _cpu_amg_pcg(lib, A, free, b, cycles=1, tol=1e-10):
+    import ctypes as C
+    x = np.zeros(A.shape[0])
+    rp, ci = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    va = np.ascontiguousarray(A.data, dtype=np.float64)
+    lv = np.zeros(10, np.int64)
+    rel = C.c_double()
+    it = lib.ocmp_cpu_pcg_amg(A.shape[0], rp.ctypes.data, ci.ctypes.data, va.ctypes.data, free.ctypes.data,
+                              b.ctypes.data, x.ctypes.data, tol, 500, cycles, lv.ctypes.data, C.byref(rel))
+    return it, x, rel.value, [int(v) for v in lv if v]
+
+
+def test_cpu_amg_restatement():
+    """The oracle's smoothed-aggregation AMG (oracle/cpu_solver.cpp, the textbook form of the
+    GPU's Schur AMG): as a CG preconditioner on the 5-point Laplacian it must (1) be exact when the
+    whole matrix fits the dense last level (one iteration), (2) converge to the direct solution, and
+    (3) keep the iteration count nearly flat under refinement, with each level at least 3x smaller.
+    A wrong sign or a dropped term in the prolongation smoothing, the Galerkin product or the
+    smoother breaks (3) (Jacobi-PCG grows like the grid size); (2) catches a non-symmetric cycle."""
+    from oracle import cbuild
+    if not os.path.exists(cbuild.LIB):
+        cbuild.build()
+    lib = cbuild.load()
+    rng = np.random.default_rng(5)
+    A, free = _laplacian_2d(6)                                   # 36 rows: one dense level
+    b = np.where(free, rng.standard_normal(36), 0.0)
+    it, x, rel, lv = _cpu_amg_pcg(lib, A, free, b, tol=1e-12)
+    assert lv == [36] and it == 1 and rel <= 1e-12
+    its = {}
+    for k in (34, 66, 130):
+        A, free = _laplacian_2d(k)
+        b = np.where(free, rng.standard_normal(k * k), 0.0)
+        it, x, rel, lv = _cpu_amg_pcg(lib, A, free, b)
+        F = free.astype(bool)
+        xo = spl_solve(A[F][:, F].tocsc(), b[F])
+        assert rel <= 1e-10 and np.abs(x[F] - xo).max() <= 1e-8 * np.abs(xo).max()
+        assert not x[~F].any()
+        assert len(lv) >= 3 and all(lv[i + 1] * 3 <= lv[i] for i in range(len(lv) - 1)) and lv[-1] <= 1200
+        its[k] = it
+    assert its[130] <= 25 and its[130] <= its[34] + 6, its
+    # two V-cycles per step: fewer CG iterations
+    A, free = _laplacian_2d(130)
+    b = np.where(free, rng.standard_normal(130 * 130), 0.0)
+    it2 = _cpu_amg_pcg(lib, A, free, b, cycles=2)[0]
+    assert it2 < its[130]
