@@ -282,6 +282,14 @@ int ocmp_error_norms(ocmp_space *s, int block, const double *x, const double *re
 int ocmp_error_norms_vec(ocmp_space *s, int nblocks, const int32_t *blocks, const double *x, const double *ref,
                          const int32_t *fns, const double *c, double *out);
 
+/* The Picard stop test of opencmp/models/ins.py:346-355 in one pass: out[0] =
+ * sqrt(int |a - b|^2) (norm("l2_norm", W, u), error.py:75-77) and out[1] = sqrt(int |b|^2)
+ * (mean(u), error.py:54-67) over one vector component (nblocks = 1..3 scalar blocks of one
+ * kind and order) of two vectors on the space, integrated exactly (rule of degree 2p).
+ * Synchronises. */
+int ocmp_l2_pair_norms(ocmp_space *s, int nblocks, const int32_t *blocks, const double *a, const double *b,
+                       double *out);
+
 /* out[0] = sqrt(int (div u_h)^2) for the vector field in the dim consecutive scalar
  * blocks block0 .. block0+dim-1 of x (one H1 vector component): _divergence of
  * opencmp/helpers/error.py:151-167 (mass conservation check). Synchronises. */

@@ -100,6 +100,7 @@ SIGNATURES = {
     'ocmp_set': [P, C.c_int, P, C.c_int, C.POINTER(C.c_double)],
     'ocmp_expr_create': [P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32)],
     'ocmp_divergence_norm': [P, C.c_int, P, C.POINTER(C.c_double)],
+    'ocmp_l2_pair_norms': [P, C.c_int, P, P, P, C.POINTER(C.c_double)],
     'ocmp_linf_norm': [P, C.c_int, P, P, C.POINTER(C.c_double)],
     'ocmp_residual': [P, P, P, P, C.POINTER(C.c_double)],
     'ocmp_precond_create': [P, C.c_int, C.c_int, C.POINTER(P)],

@@ -1642,6 +1642,15 @@ int ocmp_error_norms_vec(ocmp_space *s, int nblocks, const int32_t *blocks, cons
   });
 }
 
+int ocmp_l2_pair_norms(ocmp_space *s, int nblocks, const int32_t *blocks, const double *a, const double *b,
+                       double *out) {
+  return guard([&] {
+    OCMP_REQUIRE(s && blocks && a && b && out, "NULL argument");
+    OCMP_HIP(hipSetDevice(s->s.mesh->ctx->device));
+    l2_pair_norms(&s->s, nblocks, blocks, a, b, out);
+  });
+}
+
 int ocmp_divergence_norm(ocmp_space *s, int block0, const double *x, double *out) {
   return guard([&] {
     OCMP_REQUIRE(s && x && out, "NULL argument");

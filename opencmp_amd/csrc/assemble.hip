@@ -20,6 +20,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -1401,6 +1402,74 @@ __global__ void __launch_bounds__(256) k_error_norms_vec(KNormV K, int64_t nel, 
     for (int k = 0; k < 3; ++k) part[3 * blockIdx.x + k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
 }
 
+// The Picard stop test's two norms in one pass over the elements: per-block partial sums of
+// int |a - b|^2 and int |b|^2 over the vector component's ncomp scalar blocks (a, b finite element
+// vectors on the same space; the rule integrates their degree-2p products exactly). The element's
+// dof values are held in registers (NLM >= nl) and the tables read with wave-uniform indices.
+struct KPair {
+  int nq, nl, ld, ncomp;
+  int loc_off[3];
+  int tab_phi, tab_w;
+  const double *tab;
+};
+
+template <int D, int NLM>
+__global__ void __launch_bounds__(256) k_l2_pair(KPair K, int64_t nel, const double *__restrict__ coords,
+                                                 const int32_t *__restrict__ elems,
+                                                 const int32_t *__restrict__ edofs, const double *__restrict__ a,
+                                                 const double *__restrict__ b, double *__restrict__ part) {
+  __shared__ double sh[2][4];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s_d = 0.0, s_b = 0.0;
+  if (e < nel) {
+    const double *phi = K.tab + K.tab_phi;
+    const double *w = K.tab + K.tab_w;
+    // int |a - b|^2 = sum over the scalar blocks of int (a_k - b_k)^2: one block at a time
+    for (int k = 0; k < K.ncomp; ++k) {
+      const int32_t *ed = edofs + e * K.ld + K.loc_off[k];
+      double dv[NLM], bv[NLM];
+#pragma unroll
+      for (int l = 0; l < NLM; ++l) {
+        dv[l] = 0.0;
+        bv[l] = 0.0;
+        if (l < K.nl) {
+          const int32_t i = ed[l];
+          bv[l] = b[i];
+          dv[l] = a[i] - bv[l];
+        }
+      }
+      for (int q = 0; q < K.nq; ++q) {
+        double ud = 0.0, ub = 0.0;
+#pragma unroll
+        for (int l = 0; l < NLM; ++l) {
+          const double f = l < K.nl ? phi[q * K.nl + l] : 0.0;
+          ud += f * dv[l];
+          ub += f * bv[l];
+        }
+        s_d += w[q] * ud * ud;
+        s_b += w[q] * ub * ub;
+      }
+    }
+    Geo<D> g;
+    geometry<D>(coords, elems + e * (D + 1), g);
+    s_d *= g.adet;
+    s_b *= g.adet;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s_d += __shfl_down(s_d, o, 64);
+    s_b += __shfl_down(s_b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = s_d;
+    sh[1][threadIdx.x >> 6] = s_b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
 // out = (sqrt(sum part[3i]), sum part[3i+1], sqrt(sum part[3i+2])), one block, fixed order
 __global__ void k_sum_triples(const double *__restrict__ part, int64_t np, double *__restrict__ out) {
   __shared__ double sh[3][256];
@@ -2559,6 +2628,60 @@ void error_norms_vec(Space *sp, int nb, const int *blocks, const double *x, cons
   k_sum_triples<<<1, 256, 0, st>>>(part, nblkk, res);
   OCMP_HIP(hipGetLastError());
   OCMP_HIP(hipMemcpyAsync(out, res, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  OCMP_HIP(hipStreamSynchronize(st));
+}
+
+void l2_pair_norms(Space *sp, int nb, const int *blocks, const double *a, const double *b, double *out) {
+  Space &s = *sp;
+  const Mesh &m = *s.mesh;
+  Ctx *ctx = m.ctx;
+  hipStream_t st = ctx->stream;
+  const int d = m.dim;
+  OCMP_REQUIRE(nb >= 1 && nb <= 3, "a vector component has 1 to 3 scalar blocks");
+  const Block &b0 = s.blocks[blocks[0]];
+  KPair K;
+  std::memset(&K, 0, sizeof K);
+  for (int k = 0; k < nb; ++k) {
+    OCMP_REQUIRE(blocks[k] >= 0 && blocks[k] < (int)s.blocks.size(), "block out of range");
+    const Block &bk = s.blocks[blocks[k]];
+    OCMP_REQUIRE(bk.kind == b0.kind && bk.order == b0.order, "the blocks of a vector norm must share kind and order");
+    K.loc_off[k] = bk.loc_off;
+  }
+  OCMP_REQUIRE(b0.nloc <= 20, "at most 20 local dofs per block");
+  TableBuilder tb;
+  Rule rule = simplex_rule(d, 2 * b0.order);
+  K.nq = rule.nq;
+  K.nl = b0.nloc;
+  K.ld = s.nloc;
+  K.ncomp = nb;
+  std::vector<double> phi((size_t)rule.nq * b0.nloc), gr((size_t)rule.nq * b0.nloc * d);
+  for (int q = 0; q < rule.nq; ++q)
+    eval_basis(b0.kind, d, b0.order, &rule.lam[q * (d + 1)], &phi[(size_t)q * b0.nloc], &gr[(size_t)q * b0.nloc * d]);
+  K.tab_phi = tb.add(phi);
+  K.tab_w = tb.add(rule.w);
+  DBuf<double> *tab = cached_table(s, "P" + std::to_string(blocks[0]), tb, st);
+  K.tab = tab->p;
+  const unsigned nbk = nblk(m.nel);
+  double *part = ctx->red.p;
+  OCMP_REQUIRE(ctx->red.n >= 2 * (size_t)nbk + 2, "reduction scratch too small for this mesh");
+  KTimer tm(ctx, ctx->cls("picard_norms"));
+  auto launch = [&](auto nlm) {
+    constexpr int NLM = decltype(nlm)::value;
+    if (d == 2)
+      k_l2_pair<2, NLM><<<nbk, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, a, b, part);
+    else
+      k_l2_pair<3, NLM><<<nbk, 256, 0, st>>>(K, m.nel, m.d_coords.p, m.d_elems.p, s.d_edofs.p, a, b, part);
+  };
+  if (b0.nloc <= 4)
+    launch(std::integral_constant<int, 4>());
+  else if (b0.nloc <= 10)
+    launch(std::integral_constant<int, 10>());
+  else
+    launch(std::integral_constant<int, 20>());
+  double *res = ctx->scal.p + 110;
+  k_sum_pairs<<<1, 256, 0, st>>>(part, nbk, res);
+  OCMP_HIP(hipGetLastError());
+  OCMP_HIP(hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   OCMP_HIP(hipStreamSynchronize(st));
 }
 

@@ -309,6 +309,7 @@ struct Precond {
   Matrix *S = nullptr;
   DBuf<int32_t> d_sseg;  // per row of A: first and one-past-last CSR slot of its columns in [s0, s1)
   DBuf<double> sdpos;    // n: the positive velocity scale S is built with (0 off the free velocity rows)
+  DBuf<double> sbf;      // per slot of the Schur rows: a_pu sdpos[u], the first factor of S's products
   // S's values as lists of products: for entry k of S, the A slot of its C entry (d_scslot, -1:
   // none) and the (a_pu slot, a_uq slot) pairs [d_spptr[k], d_spptr[k+1]) in row order of A;
   // empty (values by search instead) when the lists would exceed SCHUR_MAX_PAIRS
@@ -380,6 +381,7 @@ void error_norms(Space *s, int block, const double *x, const double *ref, int fn
 void divergence_norm(Space *s, int block0, const double *x, double *out);
 void error_norms_vec(Space *s, int nb, const int *blocks, const double *x, const double *ref, const int *fns,
                      const double *cs, double *out);
+void l2_pair_norms(Space *s, int nb, const int *blocks, const double *a, const double *b, double *out);
 void linf_norm(Space *s, int block, const double *x, const double *ref, double *out);
 void axpby(Ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
 // allocation-free entry points: the device address of the static scratch array on `device`

@@ -710,26 +710,38 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
   }
 }
 
+// dinv (and its FP32 copy) = 0 on the flagged rows
+__global__ void k_zero_flagged(int64_t n, const uint8_t *__restrict__ flag, double *__restrict__ dinv,
+                               float *__restrict__ dinv32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || !flag[r]) return;
+  dinv[r] = 0.0;
+  if (dinv32) dinv32[r] = 0.0f;
+}
+
+// f[k] = a_pu dpos[u] for the slots base + k of the Schur rows (each product's first factor once)
+__global__ void k_schur_bfac(int64_t nk, int64_t base, const int32_t *__restrict__ col,
+                             const double *__restrict__ val, const double *__restrict__ dpos, double *__restrict__ f) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nk) f[k] = val[base + k] * dpos[col[base + k]];
+}
+
 // The same values from the product lists built with S's pattern: one thread per entry of S,
-// -a_pq then the products in the order of row p's slots, the order of k_schur_values.
+// -a_pq then the products in the order of row p's slots, the order of k_schur_values (bf from
+// k_schur_bfac: the same products, so the same values bit for bit).
 __global__ void __launch_bounds__(NTHR) k_schur_values_pairs(int64_t snnz, const int32_t *__restrict__ cslot,
                                                              const int32_t *__restrict__ pptr,
                                                              const int32_t *__restrict__ pj,
-                                                             const int32_t *__restrict__ pt,
-                                                             const int32_t *__restrict__ col,
+                                                             const int32_t *__restrict__ pt, int64_t base,
+                                                             const double *__restrict__ bf,
                                                              const double *__restrict__ val,
-                                                             const double *__restrict__ dpos,
                                                              double *__restrict__ sval) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= snnz) return;
   const int32_t c = cslot[k];
   double acc = c >= 0 ? -val[c] : 0.0;
   const int32_t m1 = pptr[k + 1];
-  for (int32_t m = pptr[k]; m < m1; ++m) {
-    const int32_t j = pj[m];
-    const double f = val[j] * dpos[col[j]];
-    acc += f * val[pt[m]];
-  }
+  for (int32_t m = pptr[k]; m < m1; ++m) acc += bf[pj[m] - base] * val[pt[m]];
   sval[k] = acc;
 }
 
@@ -1688,7 +1700,11 @@ void precond_update(Precond *P) {
         P->schur_flag.upload(f, st);
         OCMP_HIP(hipStreamSynchronize(st));
       }
-      {
+      const bool schur_step = P->schur_iters > 0 || P->schur_amg > 0;
+      if (schur_step) {
+        // the flexible step solves the Schur rows itself: its Jacobi entries there are zero
+        k_zero_flagged<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+      } else {
         Work &w = work(A);
         const unsigned nb = nblk(A->n);
         ensure_part(w, (size_t)nb + 8);
@@ -1698,7 +1714,7 @@ void precond_update(Precond *P) {
         k_max_reduce<<<1, NTHR, 0, st>>>(w.part.p, (int)nb, smax);
         k_schur_finish<<<nb, NTHR, 0, st>>>(A->n, P->schur_flag.p, smax, P->dinv.p, mixed ? P->dinv32.p : nullptr);
       }
-      if (P->schur_iters > 0 || P->schur_amg > 0) {
+      if (schur_step) {
         if (!P->S) build_schur_matrix(P);
         Matrix *S = P->S;
         const int64_t ns = S->n;
@@ -1706,11 +1722,13 @@ void precond_update(Precond *P) {
         k_schur_dpos<<<nblk(A->n * DPOS_LANES), NTHR, 0, st>>>(A->n, P->s0, P->s1, A->d_row_ptr.p, A->d_col.p,
                                                                A->d_dslot.p, A->d_val.p, A->space->d_free.p,
                                                                P->sdpos.p);
-        if (P->d_spptr.p)
+        if (P->d_spptr.p) {
+          const int64_t base = A->row_ptr[P->s0], nk = A->row_ptr[P->s1] - base;
+          P->sbf.alloc(std::max<int64_t>(nk, 1));
+          k_schur_bfac<<<nblk(nk), NTHR, 0, st>>>(nk, base, A->d_col.p, A->d_val.p, P->sdpos.p, P->sbf.p);
           k_schur_values_pairs<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, P->d_scslot.p, P->d_spptr.p, P->d_spj.p,
-                                                              P->d_spt.p, A->d_col.p, A->d_val.p, P->sdpos.p,
-                                                              S->d_val.p);
-        else
+                                                              P->d_spt.p, base, P->sbf.p, A->d_val.p, S->d_val.p);
+        } else
           k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
                                                          A->d_col.p, A->d_val.p, P->d_sseg.p, P->sdpos.p, S->d_val.p);
         ensure_diag_slots(S);

@@ -310,6 +310,17 @@ class Model:
             m = max(m, out[0])
         return m
 
+    def picard_norms(self, W: C.GridFunction, gfu: C.GridFunction) -> Tuple[float, float]:
+        """(norm('l2_norm', W, u), mean(u)) of the Picard test (opencmp/models/ins.py:346-347) in one
+        pass (ocmp_l2_pair_norms)."""
+        import ctypes
+        blocks = self.fes.comp_blocks[self.components.index('u')]
+        bl = (ctypes.c_int32 * len(blocks))(*blocks)
+        out = (ctypes.c_double * 2)()
+        L.check(self.ctx.lib.ocmp_l2_pair_norms(self.fes.h, len(blocks), bl, ctypes.c_void_p(W.vec.data_ptr()),
+                                                ctypes.c_void_p(gfu.vec.data_ptr()), out), 'l2_pair_norms')
+        return out[0], out[1]
+
     def l2_diff(self, a: C.GridFunction, b: C.GridFunction, comp: str) -> float:
         """sqrt(sum over the component's blocks of int (a - b)^2) -- norm('l2_norm', W, u) of the Picard loop."""
         import ctypes
@@ -554,10 +565,9 @@ class INS(Stokes):
             res = self.linear_solve(a, Lf, pre, gfu)
             self.linear_iterations += res.iters
             self.precond_iterations += res.precond_iters
-            err = self.l2_diff(self.W, gfu, 'u')
+            err, unorm = self.picard_norms(self.W, gfu)
             self.picard_errors.append(err)
             self.picard_log.append((res.iters, res.outer, float('%.3g' % res.initial_rel_residual), float('%.3g' % err)))
-            unorm = self.l2_diff(gfu, C.GridFunction(self.fes, torch.zeros_like(gfu.vec)), 'u')
             it += 1
             self.W.vec.copy_(gfu.vec)
             self.picard_converged = err < self.abs_tol + self.rel_tol * unorm

@@ -153,6 +153,36 @@ def test_oseen_picard_converges(gpu_ctx):
     assert m.picard_converged and 2 <= m.picard_iterations < 20
 
 
+def test_picard_norms_match_the_separate_norms(gpu_ctx):
+    """ocmp_l2_pair_norms (the Picard test's ||W - u|| and ||u|| in one pass, rule of degree 2p)
+    equals the two separate L2 norms (rule of degree 2p + 6) on random P2 and P3 vector fields,
+    and both equal the closed form for a linear field: ||(x, y)|| on the unit square is sqrt(2/3)."""
+    from opencmp_amd import core as C
+    from opencmp_amd.models import INS
+    say('picard pair norms')
+    hm = unit_square(6)
+    g = np.random.default_rng(4)
+    for order in (2, 3):
+        m = INS(hm, kv=1.0, interp_ord=order, ctx=gpu_ctx)
+        dev = gpu_ctx.torch_device
+        W = C.GridFunction(m.fes, torch.tensor(g.standard_normal(m.fes.ndof), device=dev))
+        u = C.GridFunction(m.fes, torch.tensor(g.standard_normal(m.fes.ndof), device=dev))
+        err, un = m.picard_norms(W, u)
+        zero = C.GridFunction(m.fes, torch.zeros(m.fes.ndof, dtype=torch.float64, device=dev))
+        e2, u2 = m.l2_diff(W, u, 'u'), m.l2_diff(u, zero, 'u')
+        say(f'  P{order}: pair {err:.15g} {un:.15g}, separate {e2:.15g} {u2:.15g}')
+        assert abs(err - e2) <= 1e-13 * e2 and abs(un - u2) <= 1e-13 * u2
+    off = m.fes.block_offsets()
+    bx, by = m.fes.comp_blocks[0]
+    v = np.zeros(m.fes.ndof)
+    nv = hm.coords.shape[0]
+    v[off[bx]:off[bx] + nv] = hm.coords[:, 0]
+    v[off[by]:off[by] + nv] = hm.coords[:, 1]
+    lin = C.GridFunction(m.fes, torch.tensor(v, device=dev))
+    err, un = m.picard_norms(zero, lin)
+    assert abs(err - np.sqrt(2 / 3)) <= 1e-14 and abs(un - np.sqrt(2 / 3)) <= 1e-14
+
+
 def test_boundary_upwind_term_closed_forms(gpu_ctx):
     """The stress-BC no-backflow term coef * max(w.n, 0) u v (opencmp/models/ins.py:210-216) on the
     unit square, H1 P2: it couples boundary dofs only, and P2 traces interpolate x and y exactly, so
