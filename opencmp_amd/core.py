@@ -412,7 +412,10 @@ class BilinearForm:
                 gf, blocks = t.wind_gf
                 for b in blocks:
                     ft.index(gf, b)
-        chunks = [self.terms[i:i + 8] for i in range(0, max(1, len(self.terms)), 8)]
+        # H1: one call (the library splits what its element scatter cannot take at once, and the row
+        # gather forms all volume terms in one pass); DG: at most 8 terms per call
+        per = 8 if self.is_dg else 32
+        chunks = [self.terms[i:i + per] for i in range(0, max(1, len(self.terms)), per)]
         for ci, chunk in enumerate(chunks):
             arr, keep = self._ctypes_terms(chunk, ft)
             farr, nf = ft.array()

@@ -26,7 +26,8 @@
 
 namespace ocmp {
 
-constexpr int MAXT = 8;   // terms per launch
+constexpr int MAXT = 8;   // terms per launch of the element-scatter kernels
+constexpr int MAXTG = 32; // volume terms per launch of the row gather (they share factor sets)
 constexpr int MAXF = 8;   // fields per launch
 constexpr int MAXB = 8;   // scalar blocks per space
 constexpr int MAXL = 40;  // compound local dofs
@@ -572,10 +573,15 @@ struct KPairG {
 
 struct KGather {
   int NL, NF, nb, accumulate, emax, accmax;
+  int NFP, NLP;     // NF rounded up to 4 (the MFMA's K step), NL rounded up to 16 (its N)
+  int td;           // offset in tab of TD[L][f][j] (NL x NFP x NLP, zero where factor f is not
+                    // one of the (block(L), block(j)) pair's)
   int loc_off[MAXB + 1];
   KPairG pg[MAXB * MAXB];
   const double *tab;
 };
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // block id with consecutive ids on one XCD (gridDim.x is a multiple of 8; blocks b, b+8, ... of a
 // launch run on one XCD), so neighbouring row blocks share an L2
@@ -664,11 +670,11 @@ __global__ void __launch_bounds__(256) k_h1_factors(const KFactors *__restrict__
   }
 }
 
-template <typename SlotT, int MAXPF>
+template <typename SlotT>
 __global__ void __launch_bounds__(256) k_h1_gather(const KGather *__restrict__ Kp, int64_t nblk,
                                                    const int32_t *__restrict__ gblk, const int32_t *__restrict__ rp,
                                                    const int32_t *__restrict__ pptr, const int32_t *__restrict__ pe,
-                                                   const uint8_t *__restrict__ pli,
+                                                   const uint16_t *__restrict__ gloff,
                                                    const uint16_t *__restrict__ gpos,
                                                    const SlotT *__restrict__ pslot, const double *__restrict__ F,
                                                    double *__restrict__ val) {
@@ -686,45 +692,45 @@ __global__ void __launch_bounds__(256) k_h1_gather(const KGather *__restrict__ K
   uint16_t *Gp = reinterpret_cast<uint16_t *>(S + (size_t)K.emax * NL);  // ne sorted positions
   for (int t = threadIdx.x; t < nz; t += blockDim.x) acc[t] = 0.0;
   for (int t = threadIdx.x; t < ne; t += blockDim.x) Gp[t] = gpos[e0 + t];
-  // phase A: (entry, column) items, four independent ones in flight per thread
+  // phase A: the block's element-matrix rows as FP64 MFMA tiles. The entries with local row L
+  // (sorted: [go[L], go[L+1])) give V[q][j] = sum_f F[e_q][f] TD[L][f][j], a (entries x NF) by
+  // (NF x NL) product; a 16 x 16 tile takes NFP / 4 v_mfma_f64_16x16x4 steps. Lane l holds the
+  // A operand (entry l % 16, factor k0 + l / 16), the B operand (factor k0 + l / 16, column
+  // l % 16) and, after the steps, the results of entries 4 r + l / 16 (r = 0..3) at column l % 16.
+  // The tiles of all L are dealt round-robin to the block's waves.
   {
-    constexpr int U = 4;
-    const int nv = ne * NL;
-    for (int base = threadIdx.x; base < nv; base += U * blockDim.x) {
-      int64_t e[U];
-      int tq[U], L[U], lj[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        tq[u] = base + u * blockDim.x;
-        const int t = min(tq[u], nv - 1);
-        const int q = t / NL;
-        lj[u] = t - q * NL;
-        e[u] = pe[e0 + q];
-        L[u] = pli[e0 + q];
-      }
-      double v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        int bi = 0, bj = 0;
-        while (bi + 1 < K.nb && L[u] >= K.loc_off[bi + 1]) ++bi;
-        while (bj + 1 < K.nb && lj[u] >= K.loc_off[bj + 1]) ++bj;
-        const KPairG P = K.pg[bi * K.nb + bj];
-        const double *__restrict__ T =
-            K.tab + P.toff + (L[u] - K.loc_off[bi]) * P.nlj + (lj[u] - K.loc_off[bj]);
-        const double *__restrict__ Fe = F + e[u] * K.NF + P.fbase;
-        const int stride = P.nli * P.nlj;
-        double acc_v = 0.0;
-#pragma unroll
-        for (int k = 0; k < MAXPF; ++k)
-          if (k < P.nfp) acc_v += Fe[k] * T[k * stride];
-        v[u] = acc_v;
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (tq[u] < nv) {
-          V[tq[u]] = v[u];
-          S[tq[u]] = pslot[(int64_t)e0 * NL + tq[u]];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const uint16_t *go = gloff + b * (NL + 1);
+    const int ntn = K.NLP >> 4;
+    int tbase = 0;
+    for (int L = 0; L < NL; ++L) {
+      const int g0 = go[L], cnt = go[L + 1] - g0;
+      const int nt = ((cnt + 15) >> 4) * ntn;
+      const double *__restrict__ TL = K.tab + K.td + (size_t)L * K.NFP * K.NLP;
+      for (int t = ((wv - tbase) % nw + nw) % nw; t < nt; t += nw) {
+        const int mt = t / ntn, n0 = (t - mt * ntn) << 4;
+        const bool va = mt * 16 + li < cnt;
+        const double *__restrict__ Fe = F + (va ? (int64_t)pe[e0 + g0 + mt * 16 + li] : 0) * K.NF;
+        f64x4 d = {0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < K.NFP; k0 += 4) {
+          const int k = k0 + lk;
+          const double av = (va && k < K.NF) ? Fe[k] : 0.0;
+          const double bv = TL[k * K.NLP + n0 + li];
+          d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
         }
+        const int col = n0 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = mt * 16 + 4 * r + lk;
+          if (row < cnt && col < NL) {
+            const int tq = (g0 + row) * NL + col;
+            V[tq] = d[r];
+            S[tq] = pslot[(int64_t)e0 * NL + tq];
+          }
+        }
+      }
+      tbase += nt;
     }
   }
   __syncthreads();
@@ -1880,7 +1886,7 @@ static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const
   hipStream_t st = ctx->stream;
   const Mesh &m = *s.mesh;
   const int d = m.dim;
-  OCMP_REQUIRE((int)vol.size() <= MAXT && (int)fields.size() <= MAXF, "too many terms or fields in one call");
+  OCMP_REQUIRE((int)vol.size() <= MAXTG && (int)fields.size() <= MAXF, "too many terms or fields in one call");
   KFactors KF;
   KGather KG;
   std::memset(&KF, 0, sizeof KF);
@@ -1997,9 +2003,32 @@ static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const
   }
   if (maxpf > 64) return false;
   KF.NF = NF;
+  // TD[L][f][j]: for each compound local row L and column j, the (block(L), block(j)) pair's
+  // tables at the pair's factor positions, zero elsewhere and in the padding
+  const int NL = s.nloc, NFP = (NF + 3) / 4 * 4, NLP = (NL + 15) / 16 * 16;
+  {
+    std::vector<double> td((size_t)NL * NFP * NLP, 0.0);
+    auto blk = [&](int l) {
+      int bb = 0;
+      while (bb + 1 < nb && l >= s.blocks[bb + 1].loc_off) ++bb;
+      return bb;
+    };
+    for (int L = 0; L < NL; ++L)
+      for (int j = 0; j < NL; ++j) {
+        const int bi = blk(L), bj = blk(j), pi = bi * nb + bj;
+        const KPairG &P = KG.pg[pi];
+        if (plist[pi].empty()) continue;
+        const int li = L - s.blocks[bi].loc_off, lj = j - s.blocks[bj].loc_off;
+        for (int k = 0; k < P.nfp; ++k)
+          td[((size_t)L * NFP + P.fbase + k) * NLP + j] = ptab[pi][(size_t)k * P.nli * P.nlj + (size_t)li * P.nlj + lj];
+      }
+    KG.td = tb.add(td);
+  }
   DBuf<double> *tab = cached_table(s, "G" + term_key(vol, fields), tb, st);
   KG.NL = s.nloc;
   KG.NF = NF;
+  KG.NFP = NFP;
+  KG.NLP = NLP;
   KG.nb = nb;
   KG.emax = M->gemax;
   KG.accmax = M->gaccmax;
@@ -2034,21 +2063,14 @@ static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const
     const unsigned grid = (unsigned)((nb_rows + 7) / 8 * 8);
     const size_t shm = (size_t)M->gemax * s.nloc * (sizeof(double) + sb) + (size_t)M->gaccmax * sizeof(double) +
                        (size_t)M->gemax * sizeof(uint16_t);
-#define OCMP_GATHER(ST, SLOTS, MPF)                                                                               \
-  k_h1_gather<ST, MPF><<<grid, 256, shm, st>>>(dKG, nb_rows, M->d_gblk.p, M->d_row_ptr.p, M->d_pptr.p, M->d_pe.p, \
-                                               M->d_pli.p, M->d_gpos.p, SLOTS, M->d_factors.p, M->d_val.p)
-#define OCMP_GATHER_PF(ST, SLOTS)              \
-  if (maxpf <= 8) OCMP_GATHER(ST, SLOTS, 8);   \
-  else if (maxpf <= 16) OCMP_GATHER(ST, SLOTS, 16); \
-  else if (maxpf <= 32) OCMP_GATHER(ST, SLOTS, 32); \
-  else OCMP_GATHER(ST, SLOTS, 64);
-    if (sb == 1) {
-      OCMP_GATHER_PF(uint8_t, M->d_pslot8.p)
-    } else {
-      OCMP_GATHER_PF(uint16_t, M->d_pslot16.p)
-    }
-#undef OCMP_GATHER_PF
-#undef OCMP_GATHER
+    if (sb == 1)
+      k_h1_gather<uint8_t><<<grid, 256, shm, st>>>(dKG, nb_rows, M->d_gblk.p, M->d_row_ptr.p, M->d_pptr.p,
+                                                   M->d_pe.p, M->d_gloff.p, M->d_gpos.p, M->d_pslot8.p,
+                                                   M->d_factors.p, M->d_val.p);
+    else
+      k_h1_gather<uint16_t><<<grid, 256, shm, st>>>(dKG, nb_rows, M->d_gblk.p, M->d_row_ptr.p, M->d_pptr.p,
+                                                    M->d_pe.p, M->d_gloff.p, M->d_gpos.p, M->d_pslot16.p,
+                                                    M->d_factors.p, M->d_val.p);
   }
   // the descriptor copy reads host memory that goes out of scope on return
   OCMP_HIP(hipStreamSynchronize(st));
@@ -2106,16 +2128,25 @@ void launch_assemble_h1(Matrix *M, const std::vector<ocmp_term> &terms, const st
   for (auto &t : terms) OCMP_REQUIRE(t.kind != OCMP_BFI_DG_FACET, "facet terms need an L2 (DG) space");
   std::vector<ocmp_term> vol, bnd;
   for (auto &t : terms) (is_bnd_kind(t.kind) ? bnd : vol).push_back(t);
-  const bool gather = !vol.empty() && gather_supported(M, vol, fields);
-  if (!gather) {
-    launch_h1_colour(M, terms, fields, accumulate, !vol.empty());
+  // the element scatter takes MAXT terms per launch: later launches add to the first
+  auto colour = [&](const std::vector<ocmp_term> &ts, bool acc) {
+    if (ts.empty()) {
+      launch_h1_colour(M, ts, fields, acc, false);
+      return;
+    }
+    for (size_t i = 0; i < ts.size(); i += MAXT) {
+      std::vector<ocmp_term> c(ts.begin() + i, ts.begin() + std::min(ts.size(), i + MAXT));
+      bool has_vol = false;
+      for (auto &t : c) has_vol |= !is_bnd_kind(t.kind);
+      launch_h1_colour(M, c, fields, acc || i > 0, has_vol);
+    }
+  };
+  const bool gather = !vol.empty() && (int)vol.size() <= MAXTG && gather_supported(M, vol, fields);
+  if (!gather || !launch_h1_gather(M, vol, fields, accumulate)) {
+    colour(terms, accumulate);
     return;
   }
-  if (!launch_h1_gather(M, vol, fields, accumulate)) {
-    launch_h1_colour(M, terms, fields, accumulate, true);
-    return;
-  }
-  if (!bnd.empty()) launch_h1_colour(M, bnd, fields, true, false);
+  if (!bnd.empty()) colour(bnd, true);
 }
 
 void launch_assemble_dg(Matrix *M, const std::vector<ocmp_term> &terms, bool accumulate) {
