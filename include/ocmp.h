@@ -399,6 +399,13 @@ int ocmp_precond_schur_amg_levels(const ocmp_precond *P, int64_t *rows, int64_t 
 /* Replaces Preconditioner.Update() (opencmp/solvers/base_solver.py:711-719):
  * recompute from A's current values. */
 int ocmp_precond_update(ocmp_precond *P);
+/* A partial Update() for the Picard iterations after the first (opencmp/models/ins.py:
+ * 338-344 updates the preconditioner at each one): the point-Jacobi rows and the FP32
+ * copy of A from A's current values, while the flexible Schur step (ocmp_precond_set_
+ * schur_solve or _set_schur_amg) keeps the Schur matrix and hierarchy of the last full
+ * ocmp_precond_update. The solution still meets each solve's tolerance; only the
+ * preconditioner lags. Needs a full update first. */
+int ocmp_precond_update_jacobi(ocmp_precond *P);
 int ocmp_precond_destroy(ocmp_precond *P);
 
 /* OCMP_SOLVER_MINRES replaces ngs.solvers.MinRes (opencmp/models/base_model.py:929-932):

@@ -111,6 +111,7 @@ SIGNATURES = {
     'ocmp_precond_set_schur_amg': [P, C.c_int],
     'ocmp_precond_schur_amg_levels': [P, P, P, C.c_int, C.POINTER(C.c_int)],
     'ocmp_precond_update': [P],
+    'ocmp_precond_update_jacobi': [P],
     'ocmp_precond_destroy': [P],
     'ocmp_solve': [P, P, P, P, C.POINTER(SolveOpts), C.POINTER(SolveStats)],
     'ocmp_hessenberg_eigvals': [C.c_int32, C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_double),

@@ -572,6 +572,10 @@ class Preconditioner:
     def Update(self):
         L.check(self.a.ctx.lib.ocmp_precond_update(self.h), 'precond_update')
 
+    def UpdateJacobi(self):
+        """The point-Jacobi rows only; the Schur step keeps its last full Update (ocmp_precond_update_jacobi)."""
+        L.check(self.a.ctx.lib.ocmp_precond_update_jacobi(self.h), 'precond_update_jacobi')
+
     def amg_levels(self) -> list:
         """(rows, nonzeros) of each level of the Schur AMG hierarchy the last Update built (empty if none)."""
         rows, nnz, nl = (C.c_int64 * 16)(), (C.c_int64 * 16)(), C.c_int()

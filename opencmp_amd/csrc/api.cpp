@@ -1848,6 +1848,15 @@ int ocmp_precond_update(ocmp_precond *P) {
   });
 }
 
+int ocmp_precond_update_jacobi(ocmp_precond *P) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.A->assembled, "matrix not assembled");
+    OCMP_HIP(hipSetDevice(P->P.A->ctx->device));
+    precond_update_jacobi(&P->P);
+  });
+}
+
 int ocmp_precond_destroy(ocmp_precond *P) {
   return guard([&] { delete P; });
 }

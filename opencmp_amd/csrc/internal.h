@@ -373,6 +373,7 @@ void check_fn(const Ctx *ctx, int fn, const double *c);
 void spmv_f64(Matrix *A, const double *x, double *y);
 void solve(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o, ocmp_solve_stats &st);
 void precond_update(Precond *P);
+void precond_update_jacobi(Precond *P);
 void residual_norms(Matrix *A, const double *b, const double *x, double *out);
 bool hessenberg_eigvals(int n, const double *H, int ldh, double *wr, double *wi);
 void newton_shifts(int n, const double *wr, const double *wi, int s, double *d, double *c);

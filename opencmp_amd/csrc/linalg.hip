@@ -1810,6 +1810,25 @@ void precond_update(Precond *P) {
   P->ready = true;
 }
 
+// The point-Jacobi rows (and the FP32 copy of A) from A's current values; the flexible Schur step
+// (S, its AMG hierarchy or inner solve) stays as the last full update left it: a lagged step,
+// still a fixed preconditioner for each FGMRES solve.
+void precond_update_jacobi(Precond *P) {
+  Matrix *A = P->A;
+  Ctx *ctx = A->ctx;
+  hipStream_t st = ctx->stream;
+  const bool mixed = P->precision == OCMP_MIXED;
+  OCMP_REQUIRE(P->ready, "a full preconditioner update must come first");
+  OCMP_REQUIRE(P->kind == OCMP_PRE_JACOBI && P->schur_blocks && (P->schur_iters > 0 || P->schur_amg > 0) && !P->btri,
+               "the Jacobi-only update is for the Schur inner solve and the Schur AMG step");
+  KTimer tm(ctx, ctx->cls("precond_setup_jacobi"));
+  k_jacobi_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_dslot.p, A->d_val.p, A->space->d_free.p, P->dinv.p,
+                                             mixed ? P->dinv32.p : nullptr);
+  k_zero_flagged<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
+  if (mixed) k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+  OCMP_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ CG
 
 // Batches of `check` iterations are enqueued ahead; the host checks the

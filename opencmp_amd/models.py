@@ -69,7 +69,7 @@ class Model:
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
                  inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
                  t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0,
-                 bf16_precond: bool = False, schur_amg: int = 0):
+                 bf16_precond: bool = False, schur_amg: int = 0, schur_lag: bool = False):
         self.ctx = ctx or C.Context.default()
         # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
         # expressions of t (boundary data, sources) are read at the level a form is built for
@@ -98,6 +98,9 @@ class Model:
         self.schur_amg = schur_amg if self.preconditioner == 'schur' else 0
         if self.schur_amg:
             self.schur_iters = 0
+        # Picard iterations after a step's first refresh only the Jacobi rows of the Schur
+        # preconditioner (its Schur step lags by at most the step's Picard iterations)
+        self.schur_lag = bool(schur_lag) and bool(self.schur_iters or self.schur_amg)
         if (self.schur_iters or self.schur_amg) and linear_solver == 'GMRes':
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
         self.vel_iters = vel_iters if self.preconditioner == 'block_triangular' else 0
@@ -561,7 +564,7 @@ class INS(Stokes):
         while True:
             self.apply_dirichlet_bcs_to(gfu)
             if reassemble is not None and not (assembled and it == 1):
-                reassemble()
+                reassemble(full=it == 1 or not getattr(self, 'schur_lag', False))
             res = self.linear_solve(a, Lf, pre, gfu)
             self.linear_iterations += res.iters
             self.precond_iterations += res.precond_iters

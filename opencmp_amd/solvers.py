@@ -47,12 +47,13 @@ class StationarySolver:
     def _create_preconditioners(self):
         self.pre = self.model.construct_preconditioners(self.a)
 
-    def _assemble(self):
-        """Solver._assemble: a.Assemble(), L.Assemble(), preconditioner.Update()."""
+    def _assemble(self, full: bool = True):
+        """Solver._assemble: a.Assemble(), L.Assemble(), preconditioner.Update() (full=False: the
+        Jacobi rows only, keeping a lagged Schur step)."""
         self.a.Assemble()
         self.L.Assemble()
         if self.pre is not None:
-            self.pre.Update()
+            self.pre.Update() if full else self.pre.UpdateJacobi()
 
     def solve(self) -> C.GridFunction:
         t0 = time.perf_counter()
@@ -106,10 +107,10 @@ class ImplicitEulerSolver:
         if self.guess is not None:
             self.guess.push(self.gfu_prev.vec)
 
-    def _assemble_a(self):
+    def _assemble_a(self, full: bool = True):
         self.a.Assemble()
         if self.pre is not None:
-            self.pre.Update()
+            self.pre.Update() if full else self.pre.UpdateJacobi()
 
     def _rhs(self):
         """L_step = dt L + M u_n (time_integration_schemes.py:663-685)."""
@@ -242,10 +243,10 @@ class TransientMultiStepSolver:
             self.Aode.Assemble()
         self.dt = dt
 
-    def _assemble_a(self):
+    def _assemble_a(self, full: bool = True):
         self.a.Assemble()
         if self.pre is not None:
-            self.pre.Update()
+            self.pre.Update() if full else self.pre.UpdateJacobi()
 
     def _axpy(self, a: float, x, y):
         L.check(self.model.ctx.lib.ocmp_axpby(self.model.ctx.h, self.model.fes.ndof, a,
