@@ -462,6 +462,43 @@ void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb) {
   rb.nnzb = nnzb;
 }
 
+// The Krylov operator of A: the entries whose (row block, column block) pair is active, in A's
+// order, with their slots in A's values (the values are gathered by the solver when stale)
+void build_solver_operator(Matrix *A) {
+  const Space &s = *A->space;
+  const int nb = (int)s.blocks.size();
+  std::vector<int8_t> blk(A->n, 0);
+  for (int k = 0; k < nb; ++k)
+    for (int64_t i = 0; i < s.blocks[k].n1; ++i) blk[s.blocks[k].dof_off + i] = (int8_t)k;
+  auto *M = new Matrix();
+  M->space = A->space;
+  M->ctx = A->ctx;
+  M->n = A->n;
+  M->row_ptr.assign(A->n + 1, 0);
+  std::vector<int32_t> src;
+  for (int64_t r = 0; r < A->n; ++r) {
+    for (int32_t k = A->row_ptr[r]; k < A->row_ptr[r + 1]; ++k)
+      if (A->active_pairs >> (blk[r] * nb + blk[A->col[k]]) & 1) {
+        M->col.push_back(A->col[k]);
+        src.push_back(k);
+      }
+    M->row_ptr[r + 1] = (int32_t)M->col.size();
+  }
+  M->nnz = (int64_t)M->col.size();
+  hipStream_t st = A->ctx->stream;
+  M->d_row_ptr.upload(M->row_ptr, st);
+  M->d_col.upload(M->col.empty() ? std::vector<int32_t>(1, 0) : M->col, st);
+  M->d_src.upload(src.empty() ? std::vector<int32_t>(1, 0) : src, st);
+  M->d_val.alloc(std::max<int64_t>(M->nnz, 1));
+  build_row_blocks(M, M->rb64, SPMV_NNZB64);
+  M->assembled = true;
+  OCMP_HIP(hipStreamSynchronize(st));
+  delete A->op;
+  A->op = M;
+  A->op_pairs = A->active_pairs;
+  A->op_version = A->op_version32 = ~0ull;
+}
+
 void ensure_diag_slots(Matrix *A) {
   if (A->d_dslot.p) return;
   std::vector<int32_t> d(A->n, -1);
@@ -1562,6 +1599,11 @@ int ocmp_assemble_bilinear(ocmp_matrix *A, int nterms, const ocmp_term *terms, i
     else
       launch_assemble_h1(&A->A, t, f, accumulate != 0);
     A->A.assembled = true;
+    const int nb = (int)s.blocks.size();
+    uint64_t pairs = 0;
+    for (auto &x : t) pairs |= nb <= 8 ? 1ull << (x.test * nb + x.trial) : ~0ull;
+    A->A.active_pairs = (accumulate ? A->A.active_pairs : 0) | pairs;
+    ++A->A.version;
   });
 }
 

@@ -273,10 +273,21 @@ struct Matrix {
   DBuf<bf16s> d_valbf;   // BF16 copy (the inner solves of a preconditioner stored in BF16)
   // solver scratch owned by this matrix (one per matrix, so distinct matrices never share it)
   Work *work = nullptr;
+  // block pairs (test block * nblocks + trial block; nblocks <= 8) that the terms since the last
+  // non-accumulating assembly touched, and a counter bumped by every assembly
+  uint64_t active_pairs = 0, version = 0;
+  // the Krylov operator: this matrix without the entries of untouched block pairs (structural
+  // zeros, e.g. the two velocity components' cross blocks), values gathered from d_val when stale
+  Matrix *op = nullptr;
+  uint64_t op_pairs = 0, op_version = ~0ull, op_version32 = ~0ull;
+  DBuf<int32_t> d_src;               // in an operator: each entry's slot in the parent's values
   Matrix() = default;
   Matrix(const Matrix &) = delete;
   Matrix &operator=(const Matrix &) = delete;
-  ~Matrix() { work_release(work); }
+  ~Matrix() {
+    work_release(work);
+    delete op;
+  }
 };
 
 struct Precond {
@@ -442,6 +453,7 @@ constexpr int SPMV_NNZB32 = 4096;
 constexpr int SCG_NNZB = 4096;
 constexpr int SCG_NT = 512;
 void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb);
+void build_solver_operator(Matrix *A);
 void ensure_row_blocks32(Matrix *A);
 // the pattern of a Schur preconditioner's S = C + B diag(A)^{-1} B^T and the per-row Schur
 // column ranges of A (host, once per preconditioner)

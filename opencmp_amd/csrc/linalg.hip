@@ -3175,8 +3175,8 @@ static void btri_apply(Precond *P, const T *v, const double *sv, T *z, int K) {
 }
 
 template <typename T>
-static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, double beta, double *x, int m,
-                        double stop_abs, int maxit, int check, Work &w) {
+static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const double *r0, double beta, double *x,
+                        int m, double stop_abs, int maxit, int check, Work &w) {
   Ctx *ctx = A->ctx;
   hipStream_t st = ctx->stream;
   const int64_t n = A->n;
@@ -3206,10 +3206,10 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
   const T *vals;
   if constexpr (sizeof(T) == 8) {
     dinv = P->dinv.p;
-    vals = A->d_val.p;
+    vals = K->d_val.p;
   } else {
     dinv = P->dinv32.p;
-    vals = A->d_val32.p;
+    vals = K->d_val32.p;
   }
   const int c_op = ctx->cls(sizeof(T) == 8 ? "spmv_csr_f64_fgmres" : "spmv_csr_f32_fgmres");
   const int c_pre = ctx->cls(sizeof(T) == 8 ? "fgmres_jacobi_f64" : "fgmres_jacobi_f32");
@@ -3255,8 +3255,8 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
       }
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
-        KTimer tm(ctx, c_op, csr_bytes(A, sizeof(T), sizeof(T)));
-        spmv_dispatch<T, T>(A, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid<T, T>(A), st);
+        KTimer tm(ctx, c_op, csr_bytes(K, sizeof(T), sizeof(T)));
+        spmv_dispatch<T, T>(K, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid<T, T>(K), st);
       }
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
@@ -3314,6 +3314,36 @@ static int fgmres_cycle(Matrix *A, Precond *P, bool learned, const double *r0, d
 // learned preconditioner, the second with the plain one; the rest use whichever reduced
 // the FP64 residual faster per iteration. A learned cycle that does not reduce the
 // residual switches to the plain preconditioner for good.
+// The operator FGMRES multiplies by: A without the entries of block pairs no term of the last
+// assembly touched (structural zeros: the SpMV streams fewer bytes, same products otherwise), its
+// FP64 and FP32 values gathered from A's after every assembly; A itself when every pair is live.
+// The outer residual always uses A.
+static Matrix *krylov_operator(Matrix *A, bool mixed) {
+  if (A->dg) return A;
+  const int nb = (int)A->space->blocks.size();
+  if (nb < 2 || nb > 8) return A;
+  const uint64_t all = nb * nb == 64 ? ~0ull : (1ull << (nb * nb)) - 1;
+  if (A->active_pairs == 0 || (A->active_pairs & all) == all) return A;
+  if (!A->op || A->op_pairs != A->active_pairs) build_solver_operator(A);
+  Matrix *K = A->op;
+  if (K->nnz == 0) return A;
+  hipStream_t st = A->ctx->stream;
+  if (mixed) {
+    K->d_val32.alloc(std::max<int64_t>(K->nnz, 1));
+    ensure_row_blocks32(K);
+  }
+  if (A->op_version != A->version) {
+    k_gather_vals<<<nblk(K->nnz), NTHR, 0, st>>>(K->nnz, K->d_src.p, A->d_val.p, K->d_val.p,
+                                                  mixed ? K->d_val32.p : nullptr);
+    A->op_version = A->version;
+    if (mixed) A->op_version32 = A->version;
+  } else if (mixed && A->op_version32 != A->version) {
+    k_to_f32<<<nblk(K->nnz), NTHR, 0, st>>>(K->nnz, K->d_val.p, K->d_val32.p);
+    A->op_version32 = A->version;
+  }
+  return K;
+}
+
 static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, const ocmp_solve_opts &o,
                          ocmp_solve_stats &st_) {
   OCMP_REQUIRE(P && P->kind == OCMP_PRE_JACOBI, "FGMRES needs a point-Jacobi (or Schur-diagonal) preconditioner");
@@ -3344,6 +3374,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   w.r64.alloc(n);
   const bool mixed = o.precision == OCMP_MIXED;
   if (mixed) OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
+  Matrix *K = krylov_operator(A, mixed);
   const double bn = norm_masked(A, b, w);
   double rn = residual64(A, P, b, x, w.r64.p, w);
   const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
@@ -3371,9 +3402,9 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
     const double stop_abs = mixed ? std::max(o.tol * bref, inner * rn) : o.tol * bref;
     int used;
     if (mixed)
-      used = fgmres_cycle<float>(A, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
+      used = fgmres_cycle<float>(A, K, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
     else
-      used = fgmres_cycle<double>(A, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
+      used = fgmres_cycle<double>(A, K, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
     total += used;
     const double rn_old = rn;
     rn = residual64(A, P, b, x, w.r64.p, w);

@@ -139,3 +139,43 @@ def test_schur_amg_matches_cpu_restatement(gpu_ctx):
     xg = x.cpu().numpy()
     uu = slice(0, s0)
     assert np.abs(xg[uu] - xc[uu]).max() <= 1e-7 * np.abs(xc[uu]).max()
+
+
+def test_jacobi_only_update_keeps_a_lagged_schur_step(gpu_ctx):
+    """ocmp_precond_update_jacobi: after the operator changes (a new time step size), refreshing only
+    the Jacobi rows keeps the Schur AMG of the old operator; FGMRES still reaches the tolerance and the
+    same solution as with a full update. Refused before any full update and for a plain Jacobi."""
+    from opencmp_amd import core as C
+    say('Jacobi-only update with a lagged Schur AMG step')
+    m, hm = channel(gpu_ctx, 32, 'mixed')
+
+    def form(dt):
+        a = C.BilinearForm(m.fes, m.time_derivative_terms() + m.construct_bilinear_time_coefficient(dt) +
+                           m.construct_bilinear_time_ODE(dt))
+        return a
+
+    a = form(1e-3)
+    a.Assemble()
+    pre = C.Preconditioner(a, 'schur', precision='mixed', schur_blocks=m.schur_blocks(), schur_amg=1)
+    with pytest.raises(L.OcmpError, match='full preconditioner update'):
+        pre.UpdateJacobi()
+    pre.Update()
+    g = torch.Generator(device='cpu').manual_seed(5)
+    b = torch.randn(m.fes.ndof, dtype=torch.float64, generator=g).to(gpu_ctx.torch_device)
+    b *= torch.from_numpy(m.fes.FreeDofs().astype(np.float64)).to(b.device)
+    # the same matrix object re-assembled with dt 2e-3: lagged (Jacobi only) vs full update
+    a.terms = form(2e-3).terms
+    a.Assemble()
+    pre.UpdateJacobi()
+    xl = torch.zeros_like(b)
+    rl = C.solve(a, b, xl, pre, 'FGMRes', tol=1e-10, restart=50, maxit=20000, precision='mixed')
+    pre.Update()
+    xf = torch.zeros_like(b)
+    rf = C.solve(a, b, xf, pre, 'FGMRes', tol=1e-10, restart=50, maxit=20000, precision='mixed')
+    say(f'  lagged {rl}\n  full   {rf}')
+    assert rl.converged and rf.converged and rl.iters <= 1.5 * rf.iters + 10
+    assert float(torch.linalg.norm(xl - xf) / torch.linalg.norm(xf)) < 1e-6
+    plain = C.Preconditioner(a, 'local')
+    plain.Update()
+    with pytest.raises(L.OcmpError, match='Jacobi-only update'):
+        plain.UpdateJacobi()
