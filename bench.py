@@ -4,7 +4,7 @@ Headline workload (the default, config 4 of the benchmark set): transient incomp
 Navier-Stokes, Taylor-Hood P2/P1, implicit Euler dt 1e-3 with the Oseen-Picard linearisation
 (<= 3 iterations, tolerance 1e-6), nu 1e-3, on the channel [0,4]x[0,1] with 8 seeded circular
 holes (n=228: 396,953 triangles, 1,794,910 DOFs), parabolic inflow Umax 1.5, do-nothing (stress)
-outlet with the no-backflow term; FGMRES(50) with the Schur preconditioner (velocity Jacobi,
+outlet with the no-backflow term; FGMRES(20) with the Schur preconditioner (velocity Jacobi,
 one smoothed-aggregation AMG V-cycle on C + B D^-1 B^T for the pressure; the Schur step is
 rebuilt at each step's first Picard iteration, the Jacobi rows at every one) to
 ||b - A x|| / ||b|| <= 1e-8.
@@ -229,11 +229,11 @@ def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     return gbs
 
 
-def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, n_iter=30):
+def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, restart=50, n_iter=30):
     """Config 4 on the host cores, timed on the full-size problem: the oracle's C++/OpenMP
     restatement (oracle/cpu_solver.cpp, FP64) assembles the Oseen operator with the wind of the
     last timed step (textbook element loop: quadrature per element, atomic scatter) and runs the
-    same FGMRES(50) + Schur preconditioner (velocity Jacobi; for the pressure `amg`
+    same FGMRES(restart) + Schur preconditioner (velocity Jacobi; for the pressure `amg`
     smoothed-aggregation V-cycles on C + B D^-1 B^T, or Jacobi-PCG on it to 1e-1 when amg is 0)
     on the FP64 path's last system: its preconditioner setup (S built by rows, the AMG hierarchy
     built) alone, then `n_iter` outer iterations. Per step: (assembly + setup) x the Picard
@@ -267,13 +267,13 @@ def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, n_iter=30):
     P = lambda a: a.ctypes.data_as(C.c_void_p)
     x = np.zeros(n)
     t0 = time.perf_counter()
-    lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, 0, 1e-1, 200,
+    lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), restart, 1e-30, 0, 1e-1, 200,
                               C.byref(rel), C.byref(inner), amg, None)
     t_setup = time.perf_counter() - t0
     x[:] = 0.0
     t0 = time.perf_counter()
-    it = lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), 50, 1e-30, n_iter, 1e-1,
-                                   200, C.byref(rel), C.byref(inner), amg, None)
+    it = lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), restart, 1e-30, n_iter,
+                                   1e-1, 200, C.byref(rel), C.byref(inner), amg, None)
     t_it = (time.perf_counter() - t0 - t_setup) / max(it, 1)
     picard = float(np.mean([row[1] for row in r['rows64']]))
     outer = float(np.mean(gpu_outer_fp64))
@@ -283,7 +283,7 @@ def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, n_iter=30):
             'assembly_and_setup_s': round(s_asm, 3), 'solve_s': round(s_solve, 3),
             'sample': f'oracle/cpu_solver.cpp (C++ -O3 OpenMP, FP64, {cores} threads) on the full-size problem '
                       f'({n} dofs, {va.size} nnz): Oseen assembly {t_asm:.3f} s and Schur setup {t_setup:.3f} s, '
-                      f'times {picard:.1f} Picard iterations per step; FGMRES(50) + Schur ('
+                      f'times {picard:.1f} Picard iterations per step; FGMRES({restart}) + Schur ('
                       + (f'{amg} smoothed-aggregation AMG V-cycle(s) per step, the hierarchy rebuilt at each '
                          f'setup' if amg else 'inner Jacobi-PCG to 1e-1') +
                       f') {it} outer iterations ({inner.value} inner) at {t_it * 1e3:.1f} ms per outer '
@@ -581,8 +581,8 @@ def main():
                          'block-triangular preconditioner, the default)')
     ap.add_argument('--tol', type=float, default=1e-8)
     ap.add_argument('--restart', type=int, default=None,
-                    help='GMRES restart (defaults: 50; 200 for config 5, whose diffuse-interface saddle point '
-                         'stagnates under GMRES(50) and GMRES(100) from 32^3 on)')
+                    help='GMRES restart (defaults: 50; 20 for config 4; 200 for config 5 with the Schur diagonal, '
+                         'whose diffuse-interface saddle point stagnates under GMRES(50) and GMRES(100) from 32^3 on)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--inner-tol', type=float, default=None, help='mixed path: relative reduction per inner solve')
     ap.add_argument('--maxit', type=int, default=500000, help='Krylov iteration cap (short profiling runs)')
@@ -597,7 +597,9 @@ def main():
         sys.exit(2)
     n = args.n or {1: 32, 2: 512, 3: 64, 4: 228, 5: 64}[args.config]
     if args.restart is None:
-        args.restart = 200 if (args.config == 5 and not args.vel_iters) else 50
+        # config 4: FGMRES(20) (each Picard solve takes 20-45 iterations; a sweep over 10-50 at
+        # n=228 put 20 fastest: the orthogonalisation shrinks, the iteration count barely grows)
+        args.restart = 200 if (args.config == 5 and not args.vel_iters) else (20 if args.config == 4 else 50)
     import torch
     from opencmp_amd import core as C
     ctx = C.Context.default(0)
@@ -656,7 +658,7 @@ def main():
             'config': {'workload': f"{r['desc']} to rel residual {args.tol:g}", 'config_index': 4,
                        'ndof': r['ndof'], 'tol': args.tol},
             'plain_fp64': {'s_per_step': round(r['s64'], 5), 'median_s_per_step': round(r['med64'], 5),
-                           'algorithm': ('FP64 FGMRES(50) + Schur (' + (f'FP64 AMG, {args.schur_amg} V-cycle(s)'
+                           'algorithm': (f'FP64 FGMRES({args.restart}) + Schur (' + (f'FP64 AMG, {args.schur_amg} V-cycle(s)'
                                                                          if args.schur_amg else 'FP64 inner PCG') +
                                          '): the same algorithm as the mixed leg, the best FP64 path of this build'),
                            'outer_iters_per_step': it64, 'iters_picard_res': r['rows64']},
@@ -691,7 +693,7 @@ def main():
                 out['roofline']['traffic_source'] = 'profiles/ins_channel_n228_fgmres_schur_hbm_traffic.md'
         if not args.no_cpu:
             log('cpu baseline (config 4) ...')
-            cb = cpu_baseline_ins(r, it64, amg=args.schur_amg)
+            cb = cpu_baseline_ins(r, it64, amg=args.schur_amg, restart=args.restart)
             cb['speedup_gpu_mixed_learned'] = round(cb['value'] / r['sml'], 1)
             cb['speedup_gpu_fp64'] = round(cb['value'] / r['s64'], 1)
             out['cpu_baseline'] = cb
