@@ -1449,13 +1449,32 @@ static RbCsr<T> rb_csr(const Matrix *A, const T *val) {
 // the first ga logical blocks form z = w D^-1 (2 b - w A D^-1 b) (the iterate after damped-Jacobi
 // pre-smoothing from zero and post-smoothing without the coarse correction), the rest the
 // restricted residual b_c = W^T b; bout (optional) receives b.
+// the flexible step's Jacobi rows, carried by the top level's descending launch: z = dinv v sv[0]
+// over all n rows (the Schur rows, where dinv is 0, are overwritten by the cycle's last launch)
+template <typename T>
+struct JacRows {
+  int64_t n;
+  int nblk;  // blocks of the launch that do it (0: none)
+  const T *dinv, *v;
+  const double *sv;
+  T *z;
+};
+
 template <typename T, int NNZB>
 __global__ void __launch_bounds__(NTHR) k_amg_down(int ga, RbCsr<T> A, RbCsr<T> Wt, const T *__restrict__ u,
                                                    const T *__restrict__ dinv, const double *__restrict__ sig,
-                                                   T w, T *__restrict__ z, T *__restrict__ bc, T *__restrict__ bout) {
+                                                   T w, T *__restrict__ z, T *__restrict__ bc, T *__restrict__ bout,
+                                                   JacRows<T> J) {
   __shared__ double lds[NNZB * sizeof(T) / sizeof(double)];
   const int lb = xcd_block();
   const double g = sig ? -sig[0] : 1.0;
+  const int gw = (int)gridDim.x - ga - J.nblk;  // blocks of the W^T task
+  if (lb >= ga + gw) {
+    const double sv = J.sv[0];
+    for (int64_t i = (int64_t)(lb - ga - gw) * blockDim.x + threadIdx.x; i < J.n; i += (int64_t)J.nblk * blockDim.x)
+      J.z[i] = (T)((double)J.dinv[i] * (double)J.v[i] * sv);
+    return;
+  }
   if (lb < ga) {
     if (lb >= A.nrblk) return;
     csr_row_block<T, T, NNZB>(
@@ -1547,7 +1566,8 @@ static void amg_update(Precond *Pc, bool mixed) {
 // b = -sig[0] u (b = u without sig); bout (optional) receives b. Two launches per level (the
 // smoothing steps ride on the grid transfers through W) and the dense solve on the last.
 template <typename T>
-static void amg_vcycle(Amg &M, size_t l, const T *u, const double *sig, T *x, T *bout, hipStream_t st) {
+static void amg_vcycle(Amg &M, size_t l, const T *u, const double *sig, T *x, T *bout, hipStream_t st,
+                       JacRows<T> J = JacRows<T>{0, 0, nullptr, nullptr, nullptr, nullptr}) {
   AmgLevel &L = *M.lv[l];
   const int64_t n = L.A->n;
   auto vecs = [](AmgLevel &V) -> T * {
@@ -1589,7 +1609,8 @@ static void amg_vcycle(Amg &M, size_t l, const T *u, const double *sig, T *x, T 
   T *bc = vecs(C), *xc = vecs(C) + C.A->n;
   const int ga = spmv_grid<double, double>(L.A), gw = spmv_grid<double, double>(L.Wt);
   const RbCsr<T> a = rb_csr<T>(L.A, vals(L.A)), wt = rb_csr<T>(L.Wt, vals(L.Wt));
-  k_amg_down<T, SPMV_NNZB64><<<ga + gw, NTHR, 0, st>>>(ga, a, wt, u, dinv, sig, (T)AMG_SMOOTH, z, bc, bout);
+  k_amg_down<T, SPMV_NNZB64><<<ga + gw + J.nblk, NTHR, 0, st>>>(ga, a, wt, u, dinv, sig, (T)AMG_SMOOTH, z, bc, bout,
+                                                                 J);
   amg_vcycle<T>(M, l + 1, bc, nullptr, xc, nullptr, st);
   spmv_dispatch<T, T>(L.W, vals(L.W), xc, x, nullptr, nullptr, nullptr, nullptr, 6, nullptr, spmv_grid<T, T>(L.W), st,
                       (T)0, z, (T)0);
@@ -1604,9 +1625,13 @@ __global__ void k_add(int64_t n, const T *__restrict__ e, T *__restrict__ x) {
   if (i < n) x[i] += e[i];
 }
 
+// with jac, the top level's first launch also writes the Jacobi rows z = dinv v sv[0] (all rows,
+// the Schur ones overwritten after); returns false (nothing done) when the hierarchy is a single
+// dense level, which has no such launch
 template <typename T>
-static void schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z) {
+static bool schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z, const T *jac_dinv = nullptr) {
   Amg &M = *Pc->amg;
+  if (jac_dinv && M.lv.size() < 2) return false;
   Ctx *ctx = Pc->A->ctx;
   hipStream_t st = ctx->stream;
   AmgLevel &L0 = *M.lv[0];
@@ -1624,13 +1649,16 @@ static void schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z) {
   T *x = z + Pc->s0;
   KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "schur_amg_vcycle_f64" : "schur_amg_vcycle_f32"));
   const bool more = Pc->schur_amg > 1;
-  amg_vcycle<T>(M, 0, v + Pc->s0, sv, x, more ? b : nullptr, st);
+  JacRows<T> J{0, 0, nullptr, nullptr, nullptr, nullptr};
+  if (jac_dinv) J = JacRows<T>{Pc->A->n, STREAM_BLOCKS, jac_dinv, v, sv, z};
+  amg_vcycle<T>(M, 0, v + Pc->s0, sv, x, more ? b : nullptr, st, J);
   for (int c = 1; c < Pc->schur_amg; ++c) {
     spmv_dispatch<T, T>(L0.A, val, x, rr, nullptr, nullptr, nullptr, nullptr, 5, nullptr, spmv_grid<T, T>(L0.A), st,
                         (T)0, b, (T)0);
     amg_vcycle<T>(M, 0, rr, nullptr, ee, nullptr, st);
     k_add<T><<<nblk(n), NTHR, 0, st>>>(n, ee, x);
   }
+  return true;
 }
 
 // the block-triangular step's blocks of A, their Jacobi, and 1/d on the Schur rows
@@ -3241,6 +3269,8 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
       T *wv = V + (int64_t)(jj + 1) * ldv;
       if (btri) {
         btri_apply<T>(P, vj, vs + jj, zj, cap);
+      } else if (P->schur_amg > 0 && !learned && schur_amg_apply<T>(P, vj, vs + jj, zj, dinv)) {
+        P->amg_cycles += P->schur_amg;  // the Jacobi rows rode on the cycle's first launch
       } else {
         {
           KTimer tm(ctx, c_pre, 3.0 * n * sizeof(T));
