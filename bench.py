@@ -637,7 +637,8 @@ def main():
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
                          'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
                          'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2)},
-            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
+            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2),
+                                          'alg_bytes_per_launch': round(b_ / max(c, 1))}
                                       for k, (c, m_, b_) in rep.items()},
         }
         print(json.dumps(out), flush=True)
@@ -680,7 +681,8 @@ def main():
                          'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2),
                          'measured_copy_GBps': round(copy_gbs, 1),
                          'frac_of_measured_copy': round(achieved / copy_gbs, 4)},
-            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2)}
+            'kernels_mixed_learned': {k: {'launches': c, 'ms': round(m_, 3), 'avg_us': round(1e3 * m_ / max(c, 1), 2),
+                                          'alg_bytes_per_launch': round(b_ / max(c, 1))}
                                       for k, (c, m_, b_) in rep.items()},
         }
         # HBM bytes per launch: the calibrated FETCH_SIZE + WRITE_SIZE ratio to the algorithmic bytes,
