@@ -676,6 +676,7 @@ def main():
                              'first_picard_outer_iters_learned': [row[3][0][0] for row in r['rowsml']]},
             'dofs_per_s': round(r['ndof'] / r['sml'], 1),
             'learned': r['learned'],
+            'schur_amg_levels': (r['fp64_solver'].pre.amg_levels() if args.schur_amg else []),
             'roofline': {'bound': 'hbm', 'kernel': dom, 'achieved': round(achieved, 1), 'peak': PEAK_HBM_GBS,
                          'unit': 'GB/s', 'frac': round(achieved / PEAK_HBM_GBS, 4), 'traffic': None,
                          'alg_bytes_per_launch': round(byt / cnt), 'avg_launch_us': round(1e3 * ms / cnt, 2),
@@ -723,16 +724,18 @@ def bench_plain(args, ctx, config, n, steps, warmup, with_cpu):
     log(f'setup {time.perf_counter() - t0:.1f}s: ndof {gs.ndof} nnz {a.nnz}')
     kw = dict(solver=solver, inner_tol=args.inner_tol, maxit=args.maxit)
     ss = args.s_step if solver == 'GMRes' else 0
-    s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, steps, warmup, **kw)
-    smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, steps, warmup,
+    # configs 1-3 keep GMRES(50) when they ride as extra keys on another config's line
+    restart = args.restart if args.config == config else 50
+    s64, r64, it64, x64, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, restart, steps, warmup, **kw)
+    smx, rmx, itmx, xmx, _ = run_path(C, ctx, a, f, gs, 'mixed', args.tol, restart, steps, warmup,
                                       s_step=ss, **kw)
     # the same s-step algorithm in FP64, for reference (not the plain path)
     s64s = None
     if ss:
-        s64s, r64s, it64s, _, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, s_step=ss, **kw)
+        s64s, r64s, it64s, _, _ = run_path(C, ctx, a, f, gs, 'fp64', args.tol, restart, 1, 0, s_step=ss, **kw)
     # one more step of each path with HIP events around every launch: the per-kernel roofline
-    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, args.restart, 1, 0, timing=True, s_step=ss, **kw)
-    _, _, _, _, rep_fp = run_path(C, ctx, a, f, gs, 'fp64', args.tol, args.restart, 1, 0, timing=True, **kw)
+    _, _, _, _, rep = run_path(C, ctx, a, f, gs, 'mixed', args.tol, restart, 1, 0, timing=True, s_step=ss, **kw)
+    _, _, _, _, rep_fp = run_path(C, ctx, a, f, gs, 'fp64', args.tol, restart, 1, 0, timing=True, **kw)
     err = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
 
     def table(r):

@@ -540,12 +540,12 @@ __global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
 // k_h1_factors: one thread per element computes every factor set once and writes it into the
 // element's factor row, where each (test block, trial block) pair finds its factors contiguous.
 // k_h1_gather: one workgroup per row block; its element entries are stored sorted by local row
-// index L. Phase A, for each L: a thread (column j) keeps the pair's table column T[:, L, j] in
-// registers and forms sum_f F_e[f] T[f, L, j] for the entries with that L, staging value and
-// column position in LDS; the tables are read once per (block, L) instead of once per entry.
-// Phase B: a group of nloc lanes per row walks the row's entries in element order, adding each
-// lane's value at its column's position in an LDS copy of the row (the columns of one entry are
-// distinct). The block's rows, one contiguous CSR range, are then written once. No colouring, no
+// index L. Phase A, for each L: the entries' element-matrix rows V[q][j] = sum_f F_e[f] TD[L][f][j]
+// (TD: every block pair's tables at that pair's factor positions, zero elsewhere) as FP64 MFMA
+// tiles of 16 entries x 16 columns, staged in LDS with each value's column position. Phase B: a
+// group of nloc lanes per row walks the row's entries in element order, adding each lane's value
+// at its column's position in an LDS copy of the row (the columns of one entry are distinct). The
+// block's rows, one contiguous CSR range, are then written once. No colouring, no
 // read-modify-write of CSR values, a fixed summation order (bitwise reproducible).
 
 constexpr int MAXFS = 16;  // factor sets per launch
