@@ -6,12 +6,15 @@ with the defaults of opencmp/config_functions/expanded_config_parser.py:26-90.
 Values keep the reference syntax: comma-separated lists, ``key -> value`` lines
 for dictionaries, arithmetic strings (parsed by :mod:`opencmp_amd.expr`).
 
-Two keys select and tune the GPU path; both default so that an unmodified
+Three keys select and tune the GPU path; all default so that an unmodified
 reference config runs on it:
 
     [SOLVER]
-    backend   = mi355x          # the only backend of this package
-    precision = fp64            # fp64 | mixed (FP32 inner solve in FP64 refinement)
+    backend    = mi355x         # the only backend of this package
+    precision  = fp64           # fp64 | mixed (FP32 inner solve in FP64 refinement)
+    schur_step = pcg            # Stokes/INS: pcg (inner Jacobi-PCG on the Schur matrix) | amg (one
+                                # AMG V-cycle: faster where the velocity block is mass-dominated,
+                                # transient INS; steady Stokes P3/P2 stalls with it)
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ CONFIG_DEFAULTS: Dict[str, Dict[str, Any]] = {
     'SOLVER': {'linear_solver': 'default', 'preconditioner': 'default', 'linear_tolerance': '1e-15',
                'linear_max_iterations': '100', 'linearization_method': 'Oseen', 'nonlinear_solver': 'default',
                'nonlinear_tolerance': 'absolute -> 0.0', 'nonlinear_max_iterations': '10',
-               'backend': 'mi355x', 'precision': 'fp64'},
+               'backend': 'mi355x', 'precision': 'fp64', 'schur_step': 'pcg'},
     'TRANSIENT': {'transient': 'False', 'scheme': 'implicit euler', 'time_range': '0.0, 5.0', 'dt': '0.001',
                   'dt_tolerance': 'absolute -> 0.0', 'dt_range': '1e-6, 0.1', 'maximum_rejected_solves': '1000'},
     'ERROR ANALYSIS': {'check_error': 'False', 'check_error_every_timestep': 'False',

@@ -278,8 +278,14 @@ def build_model(cfg: ConfigParser, mesh, bc: SubConfig, mcfg: SubConfig, t_param
         common.setdefault('preconditioner', 'default')
         if common['preconditioner'] in ('default', 'direct', 'local'):
             common['preconditioner'] = 'schur'
-            common['schur_iters'] = 200
-            common['schur_tol'] = 1e-3
+            step = cfg.get_item(['SOLVER', 'schur_step'], str)
+            if step == 'amg':
+                common['schur_amg'] = 1
+            elif step == 'pcg':
+                common['schur_iters'] = 200
+                common['schur_tol'] = 1e-3
+            else:
+                raise ValueError(f'[SOLVER] schur_step must be amg or pcg, not {step!r}')
         if name == 'Stokes':
             return M.Stokes(mesh, kv=kv, f=f, dirichlet=dirichlet_u, stress=stress, **common)
         tol = {k: float(v) for k, v in cfg.get_dict(['SOLVER', 'nonlinear_tolerance']).items()}

@@ -28,6 +28,7 @@ def test_main_config_items_lists_dicts_and_defaults():
     assert c.get_item(['SOLVER', 'linear_max_iterations'], int) == 100
     assert c.get_item(['SOLVER', 'backend'], str) == 'mi355x'
     assert c.get_item(['SOLVER', 'precision'], str) == 'fp64'
+    assert c.get_item(['SOLVER', 'schur_step'], str) == 'pcg'
     t = K.ConfigParser(cfg_path('mcins/purely_convective'))
     assert t.get_list(['TRANSIENT', 'time_range'], float) == [0.0, 20.0]
     assert t.get_dict(['SOLVER', 'nonlinear_tolerance']) == {'relative': '1e-4', 'absolute': '1e-8'}
