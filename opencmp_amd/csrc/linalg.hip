@@ -3347,7 +3347,7 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
 // The operator FGMRES multiplies by: A without the entries of block pairs no term of the last
 // assembly touched (structural zeros: the SpMV streams fewer bytes, same products otherwise), its
 // FP64 and FP32 values gathered from A's after every assembly; A itself when every pair is live.
-// The outer residual always uses A.
+// The FP64 refinement residual uses it too: the dropped entries are exact zeros.
 static Matrix *krylov_operator(Matrix *A, bool mixed) {
   if (A->dg) return A;
   const int nb = (int)A->space->blocks.size();
@@ -3406,7 +3406,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   if (mixed) OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
   Matrix *K = krylov_operator(A, mixed);
   const double bn = norm_masked(A, b, w);
-  double rn = residual64(A, P, b, x, w.r64.p, w);
+  double rn = residual64(K, P, b, x, w.r64.p, w);
   const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
   st_.initial_rel_residual = rn / bref;
   st_.rel_residual = rn / bref;
@@ -3437,7 +3437,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
       used = fgmres_cycle<double>(A, K, P, use, w.r64.p, rn, x, m, stop_abs, lim, check, w);
     total += used;
     const double rn_old = rn;
-    rn = residual64(A, P, b, x, w.r64.p, w);
+    rn = residual64(K, P, b, x, w.r64.p, w);
     st_.rel_residual = rn / bref;
     if (used > 0 && rn > 0 && rn_old > 0) {
       const double rate = std::log(rn / rn_old) / used;
