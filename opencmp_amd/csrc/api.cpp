@@ -462,33 +462,77 @@ void build_row_blocks(Matrix *A, RowBlocks &rb, int nnzb) {
   rb.nnzb = nnzb;
 }
 
-// The Krylov operator of A: the entries whose (row block, column block) pair is active, in A's
-// order, with their slots in A's values (the values are gathered by the solver when stale)
+// The Krylov operator of A: the entries whose (row block, column block) pair is active, with their
+// slots in A's values (the values are gathered by the solver when stale). Its rows are A's rows in
+// the Z-order of their dofs' positions (the mean centroid of the elements holding the dof), so a
+// row block of consecutive operator rows is a compact patch of the mesh whose rows share most of
+// their columns; columns keep A's numbering and each row's result goes to its row of A.
 void build_solver_operator(Matrix *A) {
   const Space &s = *A->space;
+  const Mesh &m = *s.mesh;
   const int nb = (int)s.blocks.size();
   std::vector<int8_t> blk(A->n, 0);
   for (int k = 0; k < nb; ++k)
     for (int64_t i = 0; i < s.blocks[k].n1; ++i) blk[s.blocks[k].dof_off + i] = (int8_t)k;
+  // dof positions and their Morton keys
+  const int d = m.dim, nvl = d + 1, nl = s.nloc;
+  std::vector<double> pos((size_t)A->n * d, 0.0);
+  std::vector<int32_t> cnt(A->n, 0);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t e = 0; e < m.nel; ++e) {
+    double c[3] = {0, 0, 0};
+    for (int k = 0; k < nvl; ++k)
+      for (int a = 0; a < d; ++a) c[a] += m.coords[(size_t)m.elems[e * nvl + k] * d + a] / nvl;
+    for (int i = 0; i < nl; ++i) {
+      const int32_t r = s.edofs[e * nl + i];
+      for (int a = 0; a < d; ++a) pos[(size_t)r * d + a] += c[a];
+      ++cnt[r];
+    }
+  }
+  for (int64_t r = 0; r < A->n; ++r)
+    for (int a = 0; a < d; ++a) {
+      double &p = pos[(size_t)r * d + a];
+      p = cnt[r] ? p / cnt[r] : 0.0;
+      lo[a] = std::min(lo[a], p);
+      hi[a] = std::max(hi[a], p);
+    }
+  const int bits = d == 2 ? 31 : 21;
+  std::vector<uint64_t> key(A->n, 0);
+  for (int64_t r = 0; r < A->n; ++r) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int a = 0; a < d; ++a) {
+      const double t = hi[a] > lo[a] ? (pos[(size_t)r * d + a] - lo[a]) / (hi[a] - lo[a]) : 0.0;
+      q[a] = (uint64_t)std::min(t * (double)((1ull << bits) - 1), (double)((1ull << bits) - 1));
+    }
+    uint64_t z = 0;
+    for (int bit = bits - 1; bit >= 0; --bit)
+      for (int a = 0; a < d; ++a) z = (z << 1) | ((q[a] >> bit) & 1);
+    key[r] = z;
+  }
+  std::vector<int32_t> order(A->n);
+  for (int64_t r = 0; r < A->n; ++r) order[r] = (int32_t)r;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
   auto *M = new Matrix();
   M->space = A->space;
   M->ctx = A->ctx;
   M->n = A->n;
   M->row_ptr.assign(A->n + 1, 0);
   std::vector<int32_t> src;
-  for (int64_t r = 0; r < A->n; ++r) {
+  for (int64_t i = 0; i < A->n; ++i) {
+    const int32_t r = order[i];
     for (int32_t k = A->row_ptr[r]; k < A->row_ptr[r + 1]; ++k)
       if (A->active_pairs >> (blk[r] * nb + blk[A->col[k]]) & 1) {
         M->col.push_back(A->col[k]);
         src.push_back(k);
       }
-    M->row_ptr[r + 1] = (int32_t)M->col.size();
+    M->row_ptr[i + 1] = (int32_t)M->col.size();
   }
   M->nnz = (int64_t)M->col.size();
   hipStream_t st = A->ctx->stream;
   M->d_row_ptr.upload(M->row_ptr, st);
   M->d_col.upload(M->col.empty() ? std::vector<int32_t>(1, 0) : M->col, st);
   M->d_src.upload(src.empty() ? std::vector<int32_t>(1, 0) : src, st);
+  M->d_rowmap.upload(order, st);
   M->d_val.alloc(std::max<int64_t>(M->nnz, 1));
   build_row_blocks(M, M->rb64, SPMV_NNZB64);
   M->assembled = true;

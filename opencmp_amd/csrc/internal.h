@@ -281,6 +281,7 @@ struct Matrix {
   Matrix *op = nullptr;
   uint64_t op_pairs = 0, op_version = ~0ull, op_version32 = ~0ull;
   DBuf<int32_t> d_src;               // in an operator: each entry's slot in the parent's values
+  DBuf<int32_t> d_rowmap;            // in an operator: the parent's row of each (reordered) row
   Matrix() = default;
   Matrix(const Matrix &) = delete;
   Matrix &operator=(const Matrix &) = delete;

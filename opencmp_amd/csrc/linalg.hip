@@ -274,7 +274,8 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
                                                    const uint8_t *__restrict__ mask, const XT *__restrict__ cs,
                                                    const double *__restrict__ sx, int mode,
                                                    double *__restrict__ part, XT dsh, const XT *__restrict__ xp,
-                                                   XT csh, const int *__restrict__ skip) {
+                                                   XT csh, const int *__restrict__ skip,
+                                                   const int32_t *__restrict__ rowmap) {
   if (skip && *skip) return;  // a converged inner solve: the rest of its fixed launch sequence is empty
   __shared__ double lds[NNZB * sizeof(XT) / sizeof(double)];
   const int lb = xcd_block();
@@ -285,7 +286,10 @@ __global__ void __launch_bounds__(NTHR) k_spmv_csr(int64_t nrblk, const int32_t 
         lb, rblk, rp, col, ubase, ucol, lcol, val, lds,
         [&](int32_t c) { return x[c]; },
         [&](int32_t c, XT v) { return (mode >= 3 && cs) ? v * cs[c] : v; },
-        [&](int64_t r, double s) { spmv_epilogue<VT, XT>(mode, r, s, y, x, b, mask, scale, acc, dsh, xp, csh); });
+        [&](int64_t r, double s) {
+          // a reordered operator's row r is row rowmap[r] of the vectors
+          spmv_epilogue<VT, XT>(mode, rowmap ? (int64_t)rowmap[r] : r, s, y, x, b, mask, scale, acc, dsh, xp, csh);
+        });
   if (part) {
     __syncthreads();  // the block-sum scratch aliases the products
     acc = block_sum(acc, lds);
@@ -316,18 +320,21 @@ static void spmv_dispatch(Matrix *A, const VT *val, const XT *x, XT *y, const do
   if (rb.nnzb == SPMV_NNZB32)
     k_spmv_csr<VT, XT, SPMV_NNZB32><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip,
+                                                          A->d_rowmap.p);
   else
     k_spmv_csr<VT, XT, SPMV_NNZB64><<<grid, NTHR, 0, st>>>(rb.nrblk, rb.d_rblk.p, A->d_row_ptr.p, A->d_col.p,
                                                           rb.d_ubase.p, rb.d_ucol.p, rb.d_lcol.p, val, x, y, b,
-                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip);
+                                                          mask, cs, sx, mode, part, dsh, xp, csh, skip,
+                                                          A->d_rowmap.p);
 }
 
 // algorithmic bytes of one CSR SpMV: values + 16-bit local columns + the blocks' distinct
 // columns + row pointers + x once + y once
 static double csr_bytes(const Matrix *A, int wv, int wx) {
   const RowBlocks &rb = (wv == 4 && wx == 4 && A->rb32.nnzb) ? A->rb32 : A->rb64;
-  return (double)A->nnz * (wv + 2) + 4.0 * rb.nucol + 4.0 * (A->n + 1) + 2.0 * A->n * wx;
+  return (double)A->nnz * (wv + 2) + 4.0 * rb.nucol + 4.0 * (A->n + 1) + 2.0 * A->n * wx +
+         (A->d_rowmap.p ? 4.0 * A->n : 0.0);
 }
 
 void spmv_f64(Matrix *A, const double *x, double *y) {
