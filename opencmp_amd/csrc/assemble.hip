@@ -2078,6 +2078,50 @@ static bool launch_h1_gather(Matrix *M, const std::vector<ocmp_term> &vol, const
   return true;
 }
 
+// The elements with a facet on the markers of `mask`, greedily coloured among themselves (built
+// once per marker set): a boundary-only assembly scatters over these few elements in a few colours
+// instead of over every boundary element in the mesh's colours.
+static Space::BndSet &boundary_set(Space &s, uint64_t mask) {
+  auto it = s.bnd_sets.find(mask);
+  if (it != s.bnd_sets.end()) return *it->second;
+  const Mesh &m = *s.mesh;
+  const int nf = m.dim + 1;
+  std::vector<int32_t> el;
+  for (int64_t e = 0; e < m.nel; ++e) {
+    bool on = false;
+    for (int k = 0; k < nf; ++k) {
+      const int32_t nb = m.elem_nb[(size_t)e * nf + k];
+      if (nb < 0 && -(nb + 1) < 64 && (mask >> (-(nb + 1)) & 1)) on = true;
+    }
+    if (on) el.push_back((int32_t)e);
+  }
+  std::map<int32_t, std::vector<int>> vcol;  // vertex -> colours of the set's elements on it
+  std::vector<int> col(el.size());
+  int ncol = 0;
+  for (size_t i = 0; i < el.size(); ++i) {
+    uint64_t used = 0;
+    for (int k = 0; k < nf; ++k)
+      for (int c : vcol[m.elems[(size_t)el[i] * nf + k]]) used |= 1ull << c;
+    int c = 0;
+    while (c < 63 && (used >> c & 1)) ++c;
+    col[i] = c;
+    ncol = std::max(ncol, c + 1);
+    for (int k = 0; k < nf; ++k) vcol[m.elems[(size_t)el[i] * nf + k]].push_back(c);
+  }
+  auto B = std::make_unique<Space::BndSet>();
+  B->off.assign(ncol + 1, 0);
+  for (int c : col) B->off[c + 1]++;
+  for (int c = 0; c < ncol; ++c) B->off[c + 1] += B->off[c];
+  std::vector<int32_t> order(el.size());
+  std::vector<int64_t> fill(B->off.begin(), B->off.end() - 1);
+  for (size_t i = 0; i < el.size(); ++i) order[fill[col[i]]++] = el[i];
+  B->elems.upload(order.empty() ? std::vector<int32_t>(1, 0) : order, m.ctx->stream);
+  OCMP_HIP(hipStreamSynchronize(m.ctx->stream));
+  auto &ref = *B;
+  s.bnd_sets.emplace(mask, std::move(B));
+  return ref;
+}
+
 // the colour-scheduled element scatter: boundary-facet terms, and volume terms the gather does
 // not cover
 static void launch_h1_colour(Matrix *M, const std::vector<ocmp_term> &terms, const std::vector<ocmp_field> &fields,
@@ -2091,13 +2135,36 @@ static void launch_h1_colour(Matrix *M, const std::vector<ocmp_term> &terms, con
   build_kargs(s, terms, fields, A, tb, st);
   DBuf<double> *tab = cached_table(s, term_key(terms, fields), tb, st);
   A.tab = tab->p;
+  // algorithmic bytes for the volume scatter (every value, the elements, the coordinates); the
+  // boundary-only scatter touches a few rows and is timed without a byte count
   KTimer tm(ctx, ctx->cls(volume_terms ? "assemble_h1" : "assemble_h1_bnd"),
-            8.0 * M->nnz + 4.0 * s.mesh->nel * (s.mesh->dim + 1) + 8.0 * s.mesh->nv * s.mesh->dim);
+            volume_terms ? 8.0 * M->nnz + 4.0 * s.mesh->nel * (s.mesh->dim + 1) + 8.0 * s.mesh->nv * s.mesh->dim
+                         : 0.0);
   if (!accumulate) k_fill<<<nblk(M->nnz), 256, 0, st>>>(M->d_val.p, M->nnz, 0.0);
   const Mesh &m = *s.mesh;
   bool has_bnd = false;
-  for (auto &t : terms) has_bnd |= is_bnd_kind(t.kind);
+  uint64_t bmask = 0;
+  for (auto &t : terms)
+    if (is_bnd_kind(t.kind)) {
+      has_bnd = true;
+      bmask |= t.dirichlet_mask;
+    }
   OCMP_REQUIRE(!has_bnd || (int64_t)s.bnd_colour_off.size() == s.ncolours + 1, "boundary element lists missing");
+  if (has_bnd && !volume_terms) {
+    Space::BndSet &B = boundary_set(s, bmask);
+    for (size_t c = 0; c + 1 < B.off.size(); ++c) {
+      const int64_t bb = B.off[c], bn = B.off[c + 1] - bb;
+      if (!bn) continue;
+      if (m.dim == 2)
+        k_h1_bnd_assemble<2><<<nblk(bn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, B.elems.p, bb, bn,
+                                                                 M->d_slot.p, M->d_val.p);
+      else
+        k_h1_bnd_assemble<3><<<nblk(bn * s.nloc), 256, 0, st>>>(A, m.d_coords.p, m.d_elems.p, B.elems.p, bb, bn,
+                                                                 M->d_slot.p, M->d_val.p);
+    }
+    OCMP_HIP(hipGetLastError());
+    return;
+  }
   for (int c = 0; c < s.ncolours; ++c) {
     int64_t cb = s.colour_off[c], cn = s.colour_off[c + 1] - cb;
     if (!cn) continue;

@@ -175,6 +175,13 @@ struct Space {
   DBuf<int32_t> d_colour_elems;      // elements sorted by colour
   std::vector<int64_t> bnd_colour_off;  // ncolours+1: the elements with a boundary facet, per colour
   DBuf<int32_t> d_bnd_colour_elems;
+  // per boundary-marker set: the elements with a facet on those markers, coloured among themselves
+  // (elements sharing a vertex differ), for boundary-only assembly calls
+  struct BndSet {
+    std::vector<int64_t> off;
+    DBuf<int32_t> elems;
+  };
+  std::map<uint64_t, std::unique_ptr<BndSet>> bnd_sets;
   // cached kernel tables keyed by a signature
   std::map<std::string, DBuf<double> *> tables;
   ~Space() {

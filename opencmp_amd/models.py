@@ -562,7 +562,10 @@ class INS(Stokes):
         self.picard_log = []      # (iterations, refinement steps, initial rel residual, Picard error) per iteration
         self.picard_errors = []   # ||W - u||_L2 per iteration, full precision
         while True:
-            self.apply_dirichlet_bcs_to(gfu)
+            # the Dirichlet values are those of this step at every Picard iteration, and the solves
+            # leave the Dirichlet dofs as they are (their Krylov rows are zero), so one lift suffices
+            if it == 1:
+                self.apply_dirichlet_bcs_to(gfu)
             if reassemble is not None and not (assembled and it == 1):
                 reassemble(full=it == 1 or not getattr(self, 'schur_lag', False))
             res = self.linear_solve(a, Lf, pre, gfu)
