@@ -733,9 +733,12 @@ __global__ void k_schur_bfac(int64_t nk, int64_t base, const int32_t *__restrict
   if (k < nk) f[k] = val[base + k] * dpos[col[base + k]];
 }
 
-// The same values from the product lists built with S's pattern: one thread per entry of S,
-// -a_pq then the products in the order of row p's slots, the order of k_schur_values (bf from
-// k_schur_bfac: the same products, so the same values bit for bit).
+// lanes per output of the pair-list kernels: each lane sums every PAIR_LANES-th pair of its
+// output, then a fixed shuffle tree adds the lanes (a fixed order: bitwise repeatable)
+constexpr int PAIR_LANES = 4;
+
+// The same values from the product lists built with S's pattern: PAIR_LANES lanes per entry of S,
+// -a_pq plus the products of its list (bf from k_schur_bfac: a_pu dpos[u] per slot)
 __global__ void __launch_bounds__(NTHR) k_schur_values_pairs(int64_t snnz, const int32_t *__restrict__ cslot,
                                                              const int32_t *__restrict__ pptr,
                                                              const int32_t *__restrict__ pj,
@@ -743,13 +746,20 @@ __global__ void __launch_bounds__(NTHR) k_schur_values_pairs(int64_t snnz, const
                                                              const double *__restrict__ bf,
                                                              const double *__restrict__ val,
                                                              double *__restrict__ sval) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= snnz) return;
-  const int32_t c = cslot[k];
-  double acc = c >= 0 ? -val[c] : 0.0;
-  const int32_t m1 = pptr[k + 1];
-  for (int32_t m = pptr[k]; m < m1; ++m) acc += bf[pj[m] - base] * val[pt[m]];
-  sval[k] = acc;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = t / PAIR_LANES;
+  const int q = (int)(t % PAIR_LANES);
+  double acc = 0.0;
+  if (k < snnz) {
+    const int32_t m1 = pptr[k + 1];
+    for (int32_t m = pptr[k] + q; m < m1; m += PAIR_LANES) acc += bf[pj[m] - base] * val[pt[m]];
+  }
+#pragma unroll
+  for (int o = 1; o < PAIR_LANES; o <<= 1) acc += __shfl_xor(acc, o, PAIR_LANES);
+  if (k < snnz && q == 0) {
+    const int32_t c = cslot[k];
+    sval[k] = (c >= 0 ? -val[c] : 0.0) + acc;
+  }
 }
 
 // Jacobi of S, in two passes like the Schur diagonal: S_ii of the free rows and per-block maxima,
@@ -1327,16 +1337,23 @@ __global__ void k_amg_pvals(int64_t np, const int32_t *__restrict__ prow, const 
   if (pval32) pval32[e] = (float)v;
 }
 
-// out[o] = sum over the pairs [ptr[o], ptr[o+1]) of a[ia[t]] b[ib[t]] (fixed order)
+// out[o] = sum over the pairs [ptr[o], ptr[o+1]) of a[ia[t]] b[ib[t]] (PAIR_LANES lanes per output,
+// fixed order)
 __global__ void k_amg_pairs(int64_t no, const int32_t *__restrict__ ptr, const int32_t *__restrict__ ia,
                             const int32_t *__restrict__ ib, const double *__restrict__ a, const double *__restrict__ b,
                             double *__restrict__ out, float *__restrict__ out32) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= no) return;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t o = g / PAIR_LANES;
+  const int q = (int)(g % PAIR_LANES);
   double s = 0.0;
-  for (int32_t t = ptr[o]; t < ptr[o + 1]; ++t) s += a[ia[t]] * b[ib[t]];
-  out[o] = s;
-  if (out32) out32[o] = (float)s;
+  if (o < no)
+    for (int32_t t = ptr[o] + q; t < ptr[o + 1]; t += PAIR_LANES) s += a[ia[t]] * b[ib[t]];
+#pragma unroll
+  for (int w = 1; w < PAIR_LANES; w <<= 1) s += __shfl_xor(s, w, PAIR_LANES);
+  if (o < no && q == 0) {
+    out[o] = s;
+    if (out32) out32[o] = (float)s;
+  }
 }
 
 // the augmented dense matrix [A | I] (row-major, n x 2n) of the coarsest level. Rows left out of the
@@ -1558,14 +1575,14 @@ static void amg_update(Precond *Pc, bool mixed) {
     }
     k_amg_pvals<<<nblk(P->nnz), NTHR, 0, st>>>(P->nnz, L.d_prow.p, L.d_plptr.p, L.d_plist.p, L.d_pident.p, A->d_val.p,
                                                L.dinv.p, L.omega, P->d_val.p, nullptr);
-    k_amg_pairs<<<nblk(L.nq), NTHR, 0, st>>>(L.nq, L.d_qptr.p, L.d_qa.p, L.d_qp.p, A->d_val.p, P->d_val.p, L.q.p,
-                                             nullptr);
+    k_amg_pairs<<<nblk(L.nq * PAIR_LANES), NTHR, 0, st>>>(L.nq, L.d_qptr.p, L.d_qa.p, L.d_qp.p, A->d_val.p,
+                                                          P->d_val.p, L.q.p, nullptr);
     k_amg_wvals<<<nblk(W->nnz), NTHR, 0, st>>>(W->nnz, L.d_wrow.p, L.d_wps.p, L.d_wqs.p, P->d_val.p, L.q.p, L.dinv.p,
                                                AMG_SMOOTH, W->d_val.p, mixed ? W->d_val32.p : nullptr);
     k_gather_vals<<<nblk(Wt->nnz), NTHR, 0, st>>>(Wt->nnz, L.d_wtslot.p, W->d_val.p, Wt->d_val.p,
                                                   mixed ? Wt->d_val32.p : nullptr);
-    k_amg_pairs<<<nblk(An->nnz), NTHR, 0, st>>>(An->nnz, L.d_cptr.p, L.d_cp.p, L.d_cq.p, P->d_val.p, L.q.p,
-                                                An->d_val.p, nullptr);
+    k_amg_pairs<<<nblk(An->nnz * PAIR_LANES), NTHR, 0, st>>>(An->nnz, L.d_cptr.p, L.d_cp.p, L.d_cq.p, P->d_val.p,
+                                                             L.q.p, An->d_val.p, nullptr);
   }
 }
 
@@ -1761,8 +1778,8 @@ void precond_update(Precond *P) {
           const int64_t base = A->row_ptr[P->s0], nk = A->row_ptr[P->s1] - base;
           P->sbf.alloc(std::max<int64_t>(nk, 1));
           k_schur_bfac<<<nblk(nk), NTHR, 0, st>>>(nk, base, A->d_col.p, A->d_val.p, P->sdpos.p, P->sbf.p);
-          k_schur_values_pairs<<<nblk(S->nnz), NTHR, 0, st>>>(S->nnz, P->d_scslot.p, P->d_spptr.p, P->d_spj.p,
-                                                              P->d_spt.p, base, P->sbf.p, A->d_val.p, S->d_val.p);
+          k_schur_values_pairs<<<nblk(S->nnz * PAIR_LANES), NTHR, 0, st>>>(
+              S->nnz, P->d_scslot.p, P->d_spptr.p, P->d_spj.p, P->d_spt.p, base, P->sbf.p, A->d_val.p, S->d_val.p);
         } else
           k_schur_values<<<nblk(ns * 64), NTHR, 0, st>>>(ns, P->s0, S->d_row_ptr.p, S->d_col.p, A->d_row_ptr.p,
                                                          A->d_col.p, A->d_val.p, P->d_sseg.p, P->sdpos.p, S->d_val.p);
