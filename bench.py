@@ -5,8 +5,9 @@ Navier-Stokes, Taylor-Hood P2/P1, implicit Euler dt 1e-3 with the Oseen-Picard l
 (<= 3 iterations, tolerance 1e-6), nu 1e-3, on the channel [0,4]x[0,1] with 8 seeded circular
 holes (n=228: 396,953 triangles, 1,794,910 DOFs), parabolic inflow Umax 1.5, do-nothing (stress)
 outlet with the no-backflow term; FGMRES(20) with the Schur preconditioner (velocity Jacobi,
-one smoothed-aggregation AMG V-cycle on C + B D^-1 B^T for the pressure; the Schur step is
-rebuilt at each step's first Picard iteration, the Jacobi rows at every one) to
+one smoothed-aggregation AMG V-cycle on C + B D^-1 B^T for the pressure, then the velocity rows
+as the upper block-triangular step; the Schur step is rebuilt at each step's first Picard
+iteration, the Jacobi rows at every one) to
 ||b - A x|| / ||b|| <= 1e-8.
 A plain FP64 run records 40 steps, the per-DOF MLP is trained on them, then from the same
 state both legs time the same steps:
@@ -324,7 +325,7 @@ def bench_learned(args, ctx):
                    nonlinear_tolerance=(1e-6, 1e-6), nonlinear_max_iters=3, precision=precision,
                    inner_tol=(args.inner_tol or 1e-4) if precision == 'mixed' else 0.0, s_step=s_step,
                    schur_iters=args.schur_iters, schur_tol=args.schur_tol, schur_amg=args.schur_amg,
-                   schur_lag=args.schur_lag, ctx=ctx)
+                   schur_lag=args.schur_lag, schur_tri=args.schur_tri, ctx=ctx)
 
     m0 = model('fp64')
     ndof = m0.fes.ndof
@@ -407,6 +408,8 @@ def bench_learned(args, ctx):
                        f'C + B diag(A)^-1 B^T (to {args.schur_tol:g}, at most {args.schur_iters} iterations)'
                        if args.schur_iters else
                        f'GMRES({args.restart}) + Schur-diagonal preconditioner') +
+                      (', upper block-triangular velocity rows' if args.schur_tri and (args.schur_amg or args.schur_iters)
+                       else '') +
                       (' (Schur step rebuilt at the first Picard iteration of each step, Jacobi rows at every one)'
                        if args.schur_lag and (args.schur_amg or args.schur_iters) else '')),
                 ndof=ndof, s64=s64, sml=sml, rows64=rows64, rowsml=rowsml, err=err, rep=rep, med64=med64,
@@ -576,6 +579,9 @@ def main():
     ap.add_argument('--schur-lag', type=int, default=1,
                     help='config 4: 1 = Picard iterations after the first refresh only the Jacobi rows of the '
                          'Schur preconditioner (the Schur step lags within a step; 0: full update each time)')
+    ap.add_argument('--schur-tri', type=int, default=1,
+                    help='config 4: 1 = upper block-triangular velocity rows after the Schur step (0: the '
+                         'block-diagonal step)')
     ap.add_argument('--n', type=int, default=None,
                     help='mesh cells per direction (defaults: 32 / 512 / 64 / 228 / 64; config 5 at 64^3 needs the '
                          'block-triangular preconditioner, the default)')

@@ -390,6 +390,14 @@ int ocmp_precond_set_bf16(ocmp_precond *P, int on);
  * are recomputed on the device at every update. The step is a fixed linear operator.
  * Replaces the Schur inner solve (turn that off first); vcycles = 0 turns it off. */
 int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles);
+/* Upper block-triangular velocity rows for a flexible Schur step (ocmp_precond_set_schur_solve
+ * or _set_schur_amg; the same replacement of the reference's factorisation,
+ * opencmp/models/ins.py:178-270 with base_solver.py:711-719): after the Schur rows
+ * z_p, the velocity rows become z_u = diag(A)^{-1} (v_u - B^T z_p) instead of
+ * diag(A)^{-1} v_u, with B^T the velocity rows' Schur columns of A (the Schur blocks
+ * last). Costs one SpMV with B^T per step. on = 0 turns it off; the next update gathers
+ * B^T's values. */
+int ocmp_precond_set_schur_triangular(ocmp_precond *P, int on);
 /* The Schur AMG hierarchy built by the last update (no counterpart in the reference,
  * whose preconditioners are NGSolve objects: this reports what ocmp_precond_set_schur_amg
  * built). *levels gets the number of levels (0: none built); rows[l] and nnz[l] the size

@@ -110,6 +110,7 @@ SIGNATURES = {
     'ocmp_precond_set_bf16': [P, C.c_int],
     'ocmp_precond_set_schur_amg': [P, C.c_int],
     'ocmp_precond_schur_amg_levels': [P, P, P, C.c_int, C.POINTER(C.c_int)],
+    'ocmp_precond_set_schur_triangular': [P, C.c_int],
     'ocmp_precond_update': [P],
     'ocmp_precond_update_jacobi': [P],
     'ocmp_precond_destroy': [P],

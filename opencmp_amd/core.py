@@ -564,6 +564,11 @@ class Preconditioner:
         L.check(self.a.ctx.lib.ocmp_precond_set_block_triangular(self.h, C.c_void_p(_ptr(d)), int(vel_iters),
                                                                  float(vel_tol)), 'precond_set_block_triangular')
 
+    def set_schur_triangular(self, on: bool = True) -> None:
+        """Upper block-triangular velocity rows after the flexible Schur step,
+        z_u = diag(A)^-1 (v_u - A_up z_p) (ocmp_precond_set_schur_triangular); the next Update gathers A_up."""
+        L.check(self.a.ctx.lib.ocmp_precond_set_schur_triangular(self.h, 1 if on else 0), 'precond_set_schur_triangular')
+
     def set_bf16(self, on: bool = True) -> None:
         """Keep the inner solves' matrices (Schur S, the block-triangular velocity block) in BF16 for the
         mixed path (ocmp_precond_set_bf16); the next Update makes the copies."""

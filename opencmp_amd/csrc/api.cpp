@@ -1910,6 +1910,16 @@ int ocmp_precond_set_schur_amg(ocmp_precond *P, int vcycles) {
   });
 }
 
+int ocmp_precond_set_schur_triangular(ocmp_precond *P, int on) {
+  return guard([&] {
+    OCMP_REQUIRE(P, "NULL argument");
+    OCMP_REQUIRE(P->P.schur_blocks != 0, "the triangular velocity rows need a Schur preconditioner");
+    OCMP_REQUIRE(!on || !P->P.btri, "the block-triangular step already solves the velocity rows");
+    if (on && !P->P.schur_tri) P->P.ready = false;  // A_up is gathered by the next update
+    P->P.schur_tri = on != 0;
+  });
+}
+
 int ocmp_precond_schur_amg_levels(const ocmp_precond *P, int64_t *rows, int64_t *nnz, int max_levels,
                                   int *levels) {
   return guard([&] {

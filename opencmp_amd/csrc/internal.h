@@ -366,6 +366,9 @@ struct Precond {
   // hierarchy (aggregates, patterns, product lists) is built on the host at the first update with S's
   // values, its values on the device at every update
   int schur_amg = 0;       // V-cycles per preconditioner step (0: off)
+  // with a flexible Schur step: the velocity rows as the upper block-triangular step,
+  // z_u = D^-1 (s v_u - A_up z_p) after z_p, instead of D^-1 s v_u
+  bool schur_tri = false;
   struct Amg *amg = nullptr;
   int64_t amg_cycles = 0;  // V-cycles of the running solve
   Precond() = default;

@@ -717,6 +717,22 @@ __global__ void __launch_bounds__(NTHR) k_schur_values(int64_t ns, int64_t s0, c
   }
 }
 
+// the block-triangular velocity rows: z_u[r] -= dinv[r] sum_k A_up[r, k] z_p[k], one thread per row
+// (a velocity row couples to a handful of Schur columns, contiguous in A_up's row)
+template <typename T>
+__global__ void __launch_bounds__(NTHR) k_tri_rows(int64_t nu, const int32_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                   const T *__restrict__ dinv, const T *__restrict__ zp,
+                                                   T *__restrict__ zu) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nu) return;
+  const T d = dinv[r];
+  if (d == (T)0) return;  // Dirichlet rows (and rows left out) keep z_u = 0
+  double s = 0.0;
+  for (int32_t k = rp[r]; k < rp[r + 1]; ++k) s += (double)val[k] * (double)zp[col[k]];
+  zu[r] = (T)((double)zu[r] - (double)d * s);
+}
+
 // dinv (and its FP32 copy) = 0 on the flagged rows
 __global__ void k_zero_flagged(int64_t n, const uint8_t *__restrict__ flag, double *__restrict__ dinv,
                                float *__restrict__ dinv32) {
@@ -1728,6 +1744,18 @@ static void btri_update(Precond *P, bool mixed) {
   P->vflag.alloc(1);
 }
 
+// the block-triangular velocity rows of a flexible Schur step: A_up's values (FP64, and FP32 for
+// the mixed path) from A's
+static void tri_update(Precond *P, bool mixed) {
+  Matrix *A = P->A;
+  hipStream_t st = A->ctx->stream;
+  if (!P->Aup) build_velocity_blocks(P);
+  Matrix *Aup = P->Aup;
+  if (mixed) Aup->d_val32.alloc(Aup->nnz);
+  k_gather_vals<<<nblk(Aup->nnz), NTHR, 0, st>>>(Aup->nnz, P->d_upslot.p, A->d_val.p, Aup->d_val.p,
+                                                  mixed ? Aup->d_val32.p : nullptr);
+}
+
 void precond_update(Precond *P) {
   Matrix *A = P->A;
   Ctx *ctx = A->ctx;
@@ -1824,6 +1852,7 @@ void precond_update(Precond *P) {
         }
       }
       if (P->btri) btri_update(P, mixed);
+      if (P->schur_tri && schur_step) tri_update(P, mixed);
     }
     if (mixed) {
       A->d_val32.alloc(A->nnz);
@@ -1878,6 +1907,7 @@ void precond_update_jacobi(Precond *P) {
                                              mixed ? P->dinv32.p : nullptr);
   k_zero_flagged<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
   if (mixed) k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+  if (P->schur_tri) tri_update(P, mixed);
   OCMP_HIP(hipGetLastError());
 }
 
@@ -3307,6 +3337,19 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
           schur_inner<T>(P, vj, vs + jj, zj, cap);
         }
       }
+      if (P->schur_tri && !btri) {
+        // upper block-triangular velocity rows: z_u -= D^-1 A_up z_p (z_u holds D^-1 s v_u)
+        const T *upval;
+        if constexpr (sizeof(T) == 8)
+          upval = P->Aup->d_val.p;
+        else
+          upval = P->Aup->d_val32.p;
+        const int64_t nu = P->s0;
+        KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "schur_tri_f64" : "schur_tri_f32"),
+                  (double)P->Aup->nnz * (4 + sizeof(T)) + 4.0 * (nu + 1) + 3.0 * nu * sizeof(T) +
+                      (double)(P->A->n - nu) * sizeof(T));
+        k_tri_rows<T><<<nblk(nu), NTHR, 0, st>>>(nu, P->Aup->d_row_ptr.p, P->Aup->d_col.p, upval, dinv, zj + nu, zj);
+      }
       if (learned) unet_correct<T>(P->unet, mask, vj, vs + jj, zj, 1);
       {
         KTimer tm(ctx, c_op, csr_bytes(K, sizeof(T), sizeof(T)));
@@ -3408,6 +3451,8 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
                "update the preconditioner after turning the block-triangular step on");
   OCMP_REQUIRE(P->schur_amg == 0 || (P->amg && (o.precision != OCMP_MIXED || P->amg->lv[0]->dinv32.p)),
                "update the preconditioner after turning the Schur AMG step on");
+  OCMP_REQUIRE(!P->schur_tri || (P->Aup && (o.precision != OCMP_MIXED || P->Aup->d_val32.p)),
+               "update the preconditioner after turning the triangular velocity rows on");
   P->amg_cycles = 0;
   OCMP_REQUIRE(!P->bf16 || o.precision != OCMP_MIXED ||
                    ((P->schur_iters == 0 || P->btri || P->S->d_valbf.p) && (!P->btri || P->Auu->d_valbf.p)),

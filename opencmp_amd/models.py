@@ -69,7 +69,7 @@ class Model:
                  linear_max_iterations: int = 100000, precision: str = 'fp64', restart: int = 50,
                  inner_tol: float = 0.0, s_step: int = 5, schur_iters: int = 0, schur_tol: float = 0.0,
                  t_param: Optional[List[E.Parameter]] = None, vel_iters: int = 0, vel_tol: float = 0.0,
-                 bf16_precond: bool = False, schur_amg: int = 0, schur_lag: bool = False):
+                 bf16_precond: bool = False, schur_amg: int = 0, schur_lag: bool = False, schur_tri: bool = False):
         self.ctx = ctx or C.Context.default()
         # t_param[k]: the time of step n+1-k (opencmp/solvers/base_solver.py: t_param); config
         # expressions of t (boundary data, sources) are read at the level a form is built for
@@ -101,6 +101,8 @@ class Model:
         # Picard iterations after a step's first refresh only the Jacobi rows of the Schur
         # preconditioner (its Schur step lags by at most the step's Picard iterations)
         self.schur_lag = bool(schur_lag) and bool(self.schur_iters or self.schur_amg)
+        # the velocity rows as the upper block-triangular step after the Schur step
+        self.schur_tri = bool(schur_tri) and bool(self.schur_iters or self.schur_amg)
         if (self.schur_iters or self.schur_amg) and linear_solver == 'GMRes':
             linear_solver = 'FGMRes'   # the inner PCG makes the preconditioner step nonlinear
         self.vel_iters = vel_iters if self.preconditioner == 'block_triangular' else 0
@@ -190,6 +192,8 @@ class Model:
                                    schur_tol=self.schur_tol, schur_amg=self.schur_amg)
             if self.bf16_precond:
                 pre.set_bf16(True)
+            if self.schur_tri:
+                pre.set_schur_triangular(True)
             return pre
         if self.preconditioner == 'block_triangular':
             pre = C.Preconditioner(a_assembled, 'schur', precision=self.precision, schur_blocks=self.schur_blocks())

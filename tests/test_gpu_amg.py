@@ -41,21 +41,25 @@ def test_schur_amg_fgmres_matches_inner_pcg(gpu_ctx, precision):
     b *= torch.from_numpy(m.fes.FreeDofs().astype(np.float64)).to(b.device)
     out = {}
     for tag, kw in (('pcg', dict(schur_iters=200, schur_tol=1e-1)), ('amg1', dict(schur_amg=1)),
-                    ('amg2', dict(schur_amg=2))):
+                    ('amg2', dict(schur_amg=2)), ('amg1tri', dict(schur_amg=1))):
         pre = C.Preconditioner(a, 'schur', precision=precision, schur_blocks=m.schur_blocks(), **kw)
+        if tag.endswith('tri'):
+            pre.set_schur_triangular(True)
         pre.Update()
         x = torch.zeros_like(b)
         r = C.solve(a, b, x, pre, 'FGMRes', tol=1e-10, restart=50, maxit=20000, precision=precision)
         say(f'  {tag}: {r}')
         assert r.converged and r.rel_residual <= 1e-10 and r.precond_iters > 0
         out[tag] = (x, r)
-    for tag in ('amg1', 'amg2'):
+    for tag in ('amg1', 'amg2', 'amg1tri'):
         rel = float(torch.linalg.norm(out[tag][0] - out['pcg'][0]) / torch.linalg.norm(out['pcg'][0]))
         say(f'  {tag} vs pcg: {rel:.2e}')
         assert rel < 1e-6
     # one V-cycle is about as good a Schur solve as the inner PCG to 1e-1 (within 30 % more outer iterations)
     assert out['amg1'][1].iters <= 1.3 * out['pcg'][1].iters + 5
     assert out['amg2'][1].iters <= out['amg1'][1].iters + 5
+    # the upper block-triangular velocity rows take fewer outer iterations than the block-diagonal step
+    assert out['amg1tri'][1].iters < out['amg1'][1].iters
 
 
 def test_schur_amg_single_dense_level_matches_oracle(gpu_ctx):
