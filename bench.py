@@ -230,7 +230,7 @@ def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     return gbs
 
 
-def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, restart=50, n_iter=30):
+def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, restart=50, tri=0, n_iter=30):
     """Config 4 on the host cores, timed on the full-size problem: the oracle's C++/OpenMP
     restatement (oracle/cpu_solver.cpp, FP64) assembles the Oseen operator with the wind of the
     last timed step (textbook element loop: quadrature per element, atomic scatter) and runs the
@@ -269,12 +269,12 @@ def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, restart=50, n_iter=30):
     x = np.zeros(n)
     t0 = time.perf_counter()
     lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), restart, 1e-30, 0, 1e-1, 200,
-                              C.byref(rel), C.byref(inner), amg, None)
+                              C.byref(rel), C.byref(inner), amg, None, tri)
     t_setup = time.perf_counter() - t0
     x[:] = 0.0
     t0 = time.perf_counter()
     it = lib.ocmp_cpu_fgmres_schur(n, P(rp), P(ci), P(va), P(free), s0, s1, P(b), P(x), restart, 1e-30, n_iter,
-                                   1e-1, 200, C.byref(rel), C.byref(inner), amg, None)
+                                   1e-1, 200, C.byref(rel), C.byref(inner), amg, None, tri)
     t_it = (time.perf_counter() - t0 - t_setup) / max(it, 1)
     picard = float(np.mean([row[1] for row in r['rows64']]))
     outer = float(np.mean(gpu_outer_fp64))
@@ -287,6 +287,7 @@ def cpu_baseline_ins(r, gpu_outer_fp64, amg=0, restart=50, n_iter=30):
                       f'times {picard:.1f} Picard iterations per step; FGMRES({restart}) + Schur ('
                       + (f'{amg} smoothed-aggregation AMG V-cycle(s) per step, the hierarchy rebuilt at each '
                          f'setup' if amg else 'inner Jacobi-PCG to 1e-1') +
+                      (', upper block-triangular velocity rows' if tri else '') +
                       f') {it} outer iterations ({inner.value} inner) at {t_it * 1e3:.1f} ms per outer '
                       f'iteration, times the FP64 GPU leg\'s {outer:.0f} outer iterations per step. The outlet\'s '
                       f'boundary term and the right-hand side are not in the CPU assembly. Host-CPU '
@@ -702,7 +703,8 @@ def main():
                 out['roofline']['traffic_source'] = 'profiles/ins_channel_n228_fgmres_schur_hbm_traffic.md'
         if not args.no_cpu:
             log('cpu baseline (config 4) ...')
-            cb = cpu_baseline_ins(r, it64, amg=args.schur_amg, restart=args.restart)
+            cb = cpu_baseline_ins(r, it64, amg=args.schur_amg, restart=args.restart,
+                                  tri=args.schur_tri if (args.schur_amg or args.schur_iters) else 0)
             cb['speedup_gpu_mixed_learned'] = round(cb['value'] / r['sml'], 1)
             cb['speedup_gpu_fp64'] = round(cb['value'] / r['s64'], 1)
             out['cpu_baseline'] = cb

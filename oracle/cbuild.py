@@ -37,7 +37,7 @@ def load():
                                             P, P, P]
     lib.ocmp_cpu_assemble_oseen.restype = C.c_int
     lib.ocmp_cpu_fgmres_schur.argtypes = [I64, P, P, P, P, I64, I64, P, P, I32, D, I32, D, I32,
-                                          C.POINTER(C.c_double), C.POINTER(C.c_int64), I32, P]
+                                          C.POINTER(C.c_double), C.POINTER(C.c_int64), I32, P, I32]
     lib.ocmp_cpu_pcg_amg.argtypes = [I64, P, P, P, P, P, P, D, I32, I32, P, C.POINTER(C.c_double)]
     lib.ocmp_cpu_pcg_amg.restype = C.c_int
     lib.ocmp_cpu_fgmres_schur.restype = C.c_int

@@ -702,11 +702,12 @@ int ocmp_cpu_assemble_oseen(int dim, int64_t nel, const double *X, int nlu, cons
 // or smax iterations, S = C + B D^{-1} B^T with D_uu = max(|a_uu|, 1e-2 max_j |a_uj|) (columns
 // outside [s0, s1)) over the free velocity rows, and S_ii below 1e-10 max S_ii left out. With amg > 0
 // the pressure step is amg smoothed-aggregation V-cycles on S instead of the PCG (levels, optional,
-// gets the hierarchy's rows per level). Returns the outer iterations; *inner gets the inner PCG
-// iterations (or the V-cycle applications) summed.
+// gets the hierarchy's rows per level). With tri, the velocity rows follow the Schur rows as the
+// upper block-triangular step: z_u = (v_u - A_up z_p) / a_uu. Returns the outer iterations; *inner
+// gets the inner PCG iterations (or the V-cycle applications) summed.
 int ocmp_cpu_fgmres_schur(int64_t n, const int32_t *rp, const int32_t *col, const double *val, const uint8_t *freem,
                           int64_t s0, int64_t s1, const double *b, double *x, int m, double tol, int maxit,
-                          double stol, int smax, double *rel, int64_t *inner, int amg, int64_t *levels) {
+                          double stol, int smax, double *rel, int64_t *inner, int amg, int64_t *levels, int tri) {
   const int64_t ns = s1 - s0;
   std::vector<double> dinv(n, 0.0), dpos(n, 0.0);
 #pragma omp parallel for schedule(static)
@@ -862,6 +863,16 @@ int ocmp_cpu_fgmres_schur(int64_t n, const int32_t *rp, const int32_t *col, cons
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < n; ++i) zk[i] = dinv[i] * vk[i];
       schur_solve(vk, zk);
+      if (tri) {
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < n; ++r) {
+          if ((r >= s0 && r < s1) || dinv[r] == 0.0) continue;
+          double sc = 0.0;
+          for (int32_t j = rp[r]; j < rp[r + 1]; ++j)
+            if (col[j] >= s0 && col[j] < s1) sc += val[j] * zk[col[j]];
+          zk[r] -= dinv[r] * sc;
+        }
+      }
       masked_spmv(zk, w.data());
       for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = dot(n, &V[(size_t)j * n], w.data());
 #pragma omp parallel for schedule(static)

@@ -135,7 +135,7 @@ def test_schur_amg_matches_cpu_restatement(gpu_ctx):
     rel, inner = C.c_double(), C.c_int64()
     it = lib.ocmp_cpu_fgmres_schur(m.fes.ndof, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, fr.ctypes.data,
                                    s0, s1, bh.ctypes.data, xc.ctypes.data, 50, 1e-10, 20000, 1e-1, 200,
-                                   C.byref(rel), C.byref(inner), 1, lv.ctypes.data)
+                                   C.byref(rel), C.byref(inner), 1, lv.ctypes.data, 0)
     lv_cpu = [int(v) for v in lv if v]
     say(f'  cpu: {it} outer iterations, rel {rel.value:.2e}, levels {lv_cpu}')
     assert lv_cpu == lv_gpu
@@ -143,6 +143,18 @@ def test_schur_amg_matches_cpu_restatement(gpu_ctx):
     xg = x.cpu().numpy()
     uu = slice(0, s0)
     assert np.abs(xg[uu] - xc[uu]).max() <= 1e-7 * np.abs(xc[uu]).max()
+    # the upper block-triangular velocity rows (ocmp_precond_set_schur_triangular): the same count again
+    pre.set_schur_triangular(True)
+    pre.Update()
+    x.zero_()
+    rt = Co.solve(a, b, x, pre, 'FGMRes', tol=1e-10, restart=50, maxit=20000)
+    xc[:] = 0.0
+    itt = lib.ocmp_cpu_fgmres_schur(m.fes.ndof, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, fr.ctypes.data,
+                                    s0, s1, bh.ctypes.data, xc.ctypes.data, 50, 1e-10, 20000, 1e-1, 200,
+                                    C.byref(rel), C.byref(inner), 1, None, 1)
+    say(f'  triangular: gpu {rt.iters}, cpu {itt} outer iterations')
+    assert rt.converged and rel.value <= 1e-10 and abs(itt - rt.iters) <= max(2, 0.05 * itt) and itt < it
+    assert np.abs(x.cpu().numpy()[uu] - xc[uu]).max() <= 1e-7 * np.abs(xc[uu]).max()
 
 
 def test_jacobi_only_update_keeps_a_lagged_schur_step(gpu_ctx):

@@ -554,7 +554,7 @@ def test_cpu_oseen_assembly_matches_oracle(dim):
     ci = A.indices.astype(np.int32)
     it = lib.ocmp_cpu_fgmres_schur(T.ndof, rp.ctypes.data, ci.ctypes.data, va.ctypes.data, fr.ctypes.data,
                                    T.poff, T.ndof, b.ctypes.data, x.ctypes.data, 30, 1e-10, 5000, 1e-1, 200,
-                                   C.byref(rel), C.byref(inner), 0, None)
+                                   C.byref(rel), C.byref(inner), 0, None, 0)
     assert rel.value <= 1e-10 and 0 < it < 5000 and inner.value >= it
     xo = np.zeros(T.ndof)
     Af = A[free][:, free].tocsc()
