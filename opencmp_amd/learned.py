@@ -233,11 +233,18 @@ def choose_guess(a, b: torch.Tensor, plain: torch.Tensor, learned: torch.Tensor)
 def choose_best(a, b: torch.Tensor, plain: torch.Tensor, candidates) -> Tuple[str, float, float]:
     """Fallback rule over several candidates: keep the one with the lowest FP64 residual
     if it beats the plain guess. Copies the winner into `plain` and returns (choice,
-    plain residual, residual of the best candidate)."""
-    rp, _ = residual_norms(a, b, plain)
+    plain residual, residual of the best candidate). The residuals are enqueued back to back
+    into device memory (ocmp_residual_norms_noalloc) and read with one synchronisation."""
+    xs = [plain] + [v for _, v in candidates]
+    out = torch.empty(2 * len(xs), dtype=torch.float64, device=b.device)
+    for k, x in enumerate(xs):
+        L.check(a.ctx.lib.ocmp_residual_norms_noalloc(a.h, C.c_void_p(_ptr(b)), C.c_void_p(_ptr(x)),
+                                                      C.c_void_p(out.data_ptr() + 16 * k)), 'residual_norms_noalloc')
+    res = out.cpu().tolist()
+    rp = res[0]
     best, rb, vb = 'plain', math.inf, None
-    for name, v in candidates:
-        r, _ = residual_norms(a, b, v)
+    for k, (name, v) in enumerate(candidates):
+        r = res[2 * (k + 1)]
         if math.isfinite(r) and r < rb:
             best, rb, vb = name, r, v
     if vb is not None and rb < rp:
