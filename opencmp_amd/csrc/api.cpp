@@ -1128,6 +1128,14 @@ void build_velocity_blocks(Precond *P) {
 #pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < s0; ++r)
     cut[r] = (int32_t)(std::lower_bound(col + rp[r], col + rp[r + 1], (int32_t)s0) - col);
+  // entries of block pairs no term of the last assembly touched are structural zeros: left out
+  const Space &sp = *A->space;
+  const int nb = (int)sp.blocks.size();
+  const bool known = nb <= 8 && A->active_pairs != 0;
+  std::vector<int8_t> blk(n, 0);
+  for (int k = 0; k < nb; ++k)
+    for (int64_t i = 0; i < sp.blocks[k].n1; ++i) blk[sp.blocks[k].dof_off + i] = (int8_t)k;
+  auto live = [&](int64_t r, int32_t c) { return !known || (A->active_pairs >> (blk[r] * nb + blk[c]) & 1); };
   hipStream_t st = A->ctx->stream;
   auto make = [&](bool uu, std::vector<int32_t> &slot) {
     auto *M = new Matrix();
@@ -1135,17 +1143,25 @@ void build_velocity_blocks(Precond *P) {
     M->ctx = A->ctx;
     M->n = s0;
     M->row_ptr.assign(s0 + 1, 0);
-    for (int64_t r = 0; r < s0; ++r) M->row_ptr[r + 1] = M->row_ptr[r] + (uu ? cut[r] - rp[r] : rp[r + 1] - cut[r]);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < s0; ++r) {
+      int32_t c = 0;
+      for (int32_t j = uu ? rp[r] : cut[r]; j < (uu ? cut[r] : rp[r + 1]); ++j) c += live(r, col[j]) ? 1 : 0;
+      M->row_ptr[r + 1] = c;
+    }
+    for (int64_t r = 0; r < s0; ++r) M->row_ptr[r + 1] += M->row_ptr[r];
     M->nnz = M->row_ptr[s0];
     M->col.resize(M->nnz);
     slot.resize(M->nnz);
 #pragma omp parallel for schedule(static)
     for (int64_t r = 0; r < s0; ++r) {
       const int32_t a = uu ? rp[r] : cut[r], e = uu ? cut[r] : rp[r + 1];
+      int32_t k = M->row_ptr[r];
       for (int32_t j = a; j < e; ++j) {
-        const int32_t k = M->row_ptr[r] + (j - a);
+        if (!live(r, col[j])) continue;
         M->col[k] = uu ? col[j] : col[j] - (int32_t)s0;
         slot[k] = j;
+        ++k;
       }
     }
     M->d_row_ptr.upload(M->row_ptr, st);
@@ -1164,6 +1180,7 @@ void build_velocity_blocks(Precond *P) {
   OCMP_HIP(hipStreamSynchronize(st));
   P->Auu = Auu;
   P->Aup = Aup;
+  P->vel_pairs = A->active_pairs;
 }
 
 void build_colouring(Space &s) {

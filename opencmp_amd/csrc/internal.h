@@ -369,6 +369,7 @@ struct Precond {
   // with a flexible Schur step: the velocity rows as the upper block-triangular step,
   // z_u = D^-1 (s v_u - A_up z_p) after z_p, instead of D^-1 s v_u
   bool schur_tri = false;
+  uint64_t vel_pairs = 0;  // A's active block pairs when Auu / Aup were built (rebuilt when they change)
   struct Amg *amg = nullptr;
   int64_t amg_cycles = 0;  // V-cycles of the running solve
   Precond() = default;

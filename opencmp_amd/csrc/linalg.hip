@@ -1701,11 +1701,19 @@ static bool schur_amg_apply(Precond *Pc, const T *v, const double *sv, T *z, con
   return true;
 }
 
+// (re)build A_uu and A_up from A's pattern and its current active block pairs
+static void rebuild_velocity_blocks(Precond *P) {
+  delete P->Auu;
+  delete P->Aup;
+  P->Auu = P->Aup = nullptr;
+  build_velocity_blocks(P);
+}
+
 // the block-triangular step's blocks of A, their Jacobi, and 1/d on the Schur rows
 static void btri_update(Precond *P, bool mixed) {
   Matrix *A = P->A;
   hipStream_t st = A->ctx->stream;
-  if (!P->Auu) build_velocity_blocks(P);
+  if (!P->Auu || P->vel_pairs != A->active_pairs) rebuild_velocity_blocks(P);
   Matrix *Auu = P->Auu, *Aup = P->Aup;
   const int64_t nu = P->s0, ns = A->n - P->s0;
   if (mixed) {
@@ -1749,7 +1757,7 @@ static void btri_update(Precond *P, bool mixed) {
 static void tri_update(Precond *P, bool mixed) {
   Matrix *A = P->A;
   hipStream_t st = A->ctx->stream;
-  if (!P->Aup) build_velocity_blocks(P);
+  if (!P->Aup || P->vel_pairs != A->active_pairs) rebuild_velocity_blocks(P);
   Matrix *Aup = P->Aup;
   if (mixed) Aup->d_val32.alloc(Aup->nnz);
   k_gather_vals<<<nblk(Aup->nnz), NTHR, 0, st>>>(Aup->nnz, P->d_upslot.p, A->d_val.p, Aup->d_val.p,
