@@ -505,6 +505,13 @@ struct Amg {
   DBuf<double> cinv;                        // its dense inverse (row-major)
   DBuf<float> cinv32;
   DBuf<double> cden;                        // scratch for the inversion (ncoarse x 2 ncoarse)
+  // the levels from `tail` down, as one dense matrix of level tail's rows: the V-cycle from there
+  // on is linear in its right-hand side (fixed Jacobi weights, the dense inverse at the bottom), so
+  // it is composed once per update and applied by one dense product; tail = the last level when no
+  // level above it is small enough
+  size_t tail = 0;
+  DBuf<double> tinv, tscr;                  // tail's matrix (row-major); composition scratch
+  DBuf<float> tinv32;
   ~Amg();
 };
 void build_schur_amg(Precond *P, const std::vector<double> &sval, const std::vector<double> &sdinv);

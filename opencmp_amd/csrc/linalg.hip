@@ -32,6 +32,7 @@ constexpr int SCHUR_SCAL = 8;        // scalars at the head of the Schur inner s
 constexpr int AMG_DENSE_MAX = 1200;
 // a last level of at most this many rows is inverted in LDS (n x 2n doubles: 36 KB)
 constexpr int AMG_LDS_MAX = 48;
+constexpr int64_t AMG_TAIL_ROWS = 1200;  // levels (below the top) at or below this size end the V-cycle as one dense matrix
 constexpr double AMG_SMOOTH = 2.0 / 3.0;  // damping of the AMG levels' Jacobi smoothing
 
 static inline unsigned nblk(int64_t n, int b = NTHR) { return (unsigned)((n + b - 1) / b); }
@@ -927,57 +928,53 @@ __global__ void k_axpy64(int64_t n, double scale, const T *__restrict__ d, doubl
 
 constexpr int KC = 16;  // dot products per pass
 
-// Partial sums of V[k].w for k in [k0, k0+KC), k <= nk (index nk: w.w), one row of `part` per k.
-// (A single-kernel "last block reduces" variant needs device-scope fences in every block; on
-// gfx950 each of those writes back the XCD's whole L2 and made this kernel 3x slower, so the
-// partials are reduced by a separate small launch.)
-template <typename T>
+// Partial sums of V[k].w for k in [k0, k0+KE), k <= nk (index nk: w.w), one row of `part` per k.
+// KE is the number of accumulators, a compile-time count so that every row's load is issued
+// without a branch (all KE loads of a thread in flight at once); rows past nk read w again (L1
+// hits) and are not stored. (A single-kernel "last block reduces" variant needs device-scope
+// fences in every block; on gfx950 each of those writes back the XCD's whole L2 and made this
+// kernel 3x slower, so the partials are reduced by a separate small launch.)
+template <typename T, int KE>
 __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ V, int64_t ldv,
                                                const T *__restrict__ w, int nk, double *__restrict__ part) {
   constexpr int VW = Vec<T>::w;
-  __shared__ double sh[NTHR / 64][KC];
+  __shared__ double sh[NTHR / 64][KE];
   const int lb = xcd_block();
   const int64_t nvec = n / VW;
   const int64_t chunk = (nvec + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = lb * chunk, i1 = min(nvec, i0 + chunk);
-  const int k0 = blockIdx.y * KC;
-  const int kend = min(KC, nk + 1 - k0);  // number of live accumulators (uniform)
-  double acc[KC];
+  const int k0 = blockIdx.y * KE;
+  const int kend = min(KE, nk + 1 - k0);  // live accumulators of this block row
+  const T *rows[KE];
+  double acc[KE];
 #pragma unroll
-  for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+  for (int c = 0; c < KE; ++c) {
+    rows[c] = k0 + c < nk ? V + (int64_t)(k0 + c) * ldv : w;
+    acc[c] = 0.0;
+  }
   for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
     T wv[VW];
     vload(w + i * VW, wv);
+    T vv[KE][VW];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
-      if (c < kend) {
-        const int k = k0 + c;
-        T vv[VW];
-        if (k < nk) {
-          vload(V + (int64_t)k * ldv + i * VW, vv);
-        } else {
+    for (int c = 0; c < KE; ++c) vload(rows[c] + i * VW, vv[c]);
 #pragma unroll
-          for (int l = 0; l < VW; ++l) vv[l] = wv[l];
-        }
-        double s = 0.0;
+    for (int c = 0; c < KE; ++c) {
+      double s = 0.0;
 #pragma unroll
-        for (int l = 0; l < VW; ++l) s += (double)vv[l] * (double)wv[l];
-        acc[c] += s;
-      }
+      for (int l = 0; l < VW; ++l) s += (double)vv[c][l] * (double)wv[l];
+      acc[c] += s;
     }
   }
   if (lb == gridDim.x - 1) {  // tail rows (n % VW)
     for (int64_t i = nvec * VW + threadIdx.x; i < n; i += blockDim.x) {
       const double wi = (double)w[i];
 #pragma unroll
-      for (int c = 0; c < KC; ++c) {
-        const int k = k0 + c;
-        if (c < kend) acc[c] += (k < nk ? (double)V[(int64_t)k * ldv + i] : wi) * wi;
-      }
+      for (int c = 0; c < KE; ++c) acc[c] += (double)rows[c][i] * wi;
     }
   }
 #pragma unroll
-  for (int c = 0; c < KC; ++c) {
+  for (int c = 0; c < KE; ++c) {
     const double v = wave_sum(acc[c]);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][c] = v;
   }
@@ -987,6 +984,30 @@ __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ 
 #pragma unroll
     for (int wv = 0; wv < NTHR / 64; ++wv) s += sh[wv][threadIdx.x];
     part[(int64_t)(k0 + threadIdx.x) * gridDim.x + lb] = s;
+  }
+}
+
+// the partial sums of V[k].w for k < nk and of w.w (k = nk) over `grid` blocks: one block row when
+// the nk+1 dots fit KC accumulators, else block rows of KC
+template <typename T>
+static void mdot_launch(int grid, int64_t n, const T *V, int64_t ldv, const T *w, int nk, double *part,
+                        hipStream_t st) {
+  const int nd = nk + 1;
+  if (nd > KC) {
+    k_mdot<T, KC><<<dim3(grid, (nd + KC - 1) / KC), NTHR, 0, st>>>(n, V, ldv, w, nk, part);
+    return;
+  }
+  switch (nd) {
+#define OCMP_MDOT(K) \
+  case K:            \
+    k_mdot<T, K><<<grid, NTHR, 0, st>>>(n, V, ldv, w, nk, part); \
+    break;
+    OCMP_MDOT(1) OCMP_MDOT(2) OCMP_MDOT(3) OCMP_MDOT(4) OCMP_MDOT(5) OCMP_MDOT(6) OCMP_MDOT(7) OCMP_MDOT(8)
+    OCMP_MDOT(9) OCMP_MDOT(10) OCMP_MDOT(11) OCMP_MDOT(12) OCMP_MDOT(13) OCMP_MDOT(14) OCMP_MDOT(15)
+    OCMP_MDOT(16)
+#undef OCMP_MDOT
+    default:
+      break;
   }
 }
 
@@ -1541,6 +1562,40 @@ __global__ void k_neg_scale(int64_t n, const T *__restrict__ v, const double *__
   if (i < n) b[i] = (T)(-(double)v[i] * sv[0]);
 }
 
+// one level's V-cycle as a dense matrix, from the dense matrix Mc of the levels below it:
+// M = w D^-1 (2 I - w A D^-1) + W Mc W^T (the descending launch's z, plus the prolongated coarse
+// solve of the restricted right-hand side W^T b); rows with dinv 0 have z = 0, as in that launch.
+// One block per row i (n, nc <= AMG_TAIL_ROWS): A's row i scattered into LDS, r = W(i,:) Mc in
+// LDS, then M(i,j) = z(i,j) + W(j,:) . r for every j.
+__global__ void __launch_bounds__(256) k_amg_compose(int64_t n, const int32_t *__restrict__ arp,
+                                                     const int32_t *__restrict__ acol,
+                                                     const double *__restrict__ aval, const double *__restrict__ dinv,
+                                                     double w, const int32_t *__restrict__ wrp,
+                                                     const int32_t *__restrict__ wcol, const double *__restrict__ wval,
+                                                     int64_t nc, const double *__restrict__ Mc, double *__restrict__ M,
+                                                     float *__restrict__ M32) {
+  __shared__ double arow[AMG_TAIL_ROWS];
+  __shared__ double r[AMG_TAIL_ROWS];
+  const int64_t i = blockIdx.x;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) arow[j] = 0.0;
+  __syncthreads();
+  for (int32_t k = arp[i] + threadIdx.x; k < arp[i + 1]; k += blockDim.x) atomicAdd(&arow[acol[k]], aval[k]);
+  for (int64_t c = threadIdx.x; c < nc; c += blockDim.x) {
+    double t = 0.0;
+    for (int32_t p = wrp[i]; p < wrp[i + 1]; ++p) t += wval[p] * Mc[(int64_t)wcol[p] * nc + c];
+    r[c] = t;
+  }
+  __syncthreads();
+  const double di = dinv[i];
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double z = di != 0.0 ? w * di * ((i == j ? 2.0 : 0.0) - w * arow[j] * dinv[j]) : 0.0;
+    double s = 0.0;
+    for (int32_t q = wrp[j]; q < wrp[j + 1]; ++q) s += r[wcol[q]] * wval[q];
+    M[i * n + j] = z + s;
+    if (M32) M32[i * n + j] = (float)(z + s);
+  }
+}
+
 // the hierarchy's values from S's (and S's Jacobi) at every update
 static void amg_update(Precond *Pc, bool mixed) {
   Amg &M = *Pc->amg;
@@ -1582,6 +1637,27 @@ static void amg_update(Precond *Pc, bool mixed) {
         k_amg_invert_lds<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
       else
         k_amg_invert<<<1, 1024, 0, st>>>(nc, M.cden.p, M.cinv.p, mixed ? M.cinv32.p : nullptr);
+      // compose the levels from the tail's up to the last one (the inverse) into one matrix
+      M.tail = l;
+      if (nc <= AMG_TAIL_ROWS)
+        while (M.tail > 1 && M.lv[M.tail - 1]->A->n <= AMG_TAIL_ROWS) --M.tail;
+      if (M.tail < l) {
+        const int64_t nt = M.lv[M.tail]->A->n;
+        M.tinv.alloc((size_t)nt * nt);
+        if (mixed) M.tinv32.alloc((size_t)nt * nt);
+        M.tscr.alloc((size_t)2 * nt * nt);
+        const double *mc = M.cinv.p;
+        for (size_t k = l; k-- > M.tail;) {
+          AmgLevel &K = *M.lv[k];
+          const int64_t nk = K.A->n, nkc = M.lv[k + 1]->A->n;
+          double *out = k == M.tail ? M.tinv.p : M.tscr.p + (size_t)((l - k) & 1) * nt * nt;
+          k_amg_compose<<<(unsigned)nk, 256, 0, st>>>(nk, K.A->d_row_ptr.p, K.A->d_col.p, K.A->d_val.p, K.dinv.p,
+                                                         AMG_SMOOTH, K.W->d_row_ptr.p, K.W->d_col.p, K.W->d_val.p,
+                                                         nkc, mc, out,
+                                                         k == M.tail && mixed ? M.tinv32.p : nullptr);
+          mc = out;
+        }
+      }
       break;
     }
     Matrix *P = L.P, *W = L.W, *Wt = L.Wt, *An = M.lv[l + 1]->A;
@@ -1622,12 +1698,13 @@ static void amg_vcycle(Amg &M, size_t l, const T *u, const double *sig, T *x, T 
     else
       return A->d_val32.p;
   };
-  if (l + 1 == M.lv.size()) {
+  if (l == M.tail) {
+    const bool last = l + 1 == M.lv.size();
     const T *inv;
     if constexpr (sizeof(T) == 8)
-      inv = M.cinv.p;
+      inv = last ? M.cinv.p : M.tinv.p;
     else
-      inv = M.cinv32.p;
+      inv = last ? M.cinv32.p : M.tinv32.p;
     const T *b = u;
     if (sig) {
       T *bb = bout ? bout : vecs(L);
@@ -2162,8 +2239,7 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
       }
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
-        dim3 gd(grid, (jj + 2 + KC - 1) / KC);
-        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p);
+        mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
       }
       reduce_to(w.part.p, grid, jj + 1, dot, st);
       {
@@ -3365,8 +3441,7 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
       }
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
-        dim3 gd(grid, (jj + 2 + KC - 1) / KC);
-        k_mdot<T><<<gd, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, w.part.p);
+        mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
       }
       reduce_to(w.part.p, grid, jj + 1, dot, st);
       {
