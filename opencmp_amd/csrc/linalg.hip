@@ -987,27 +987,44 @@ __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ 
   }
 }
 
-// the partial sums of V[k].w for k < nk and of w.w (k = nk) over `grid` blocks: one block row when
-// the nk+1 dots fit KC accumulators, else block rows of KC
+// the grid of k_mdot<T, KE>: at most the blocks that are resident together (the kernel's occupancy,
+// times the CUs the streaming grid assumes), so that no second, partial round of blocks runs
+template <typename T, int KE>
+static int mdot_grid(int grid) {
+  static const int per_cu = [] {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_mdot<T, KE>, NTHR, 0) != hipSuccess || b < 1) b = 1;
+    return b;
+  }();
+  const int cap = per_cu * (STREAM_BLOCKS / 4) / NXCD * NXCD;
+  return std::max(NXCD, std::min(grid, cap));
+}
+
+// the partial sums of V[k].w for k < nk and of w.w (k = nk), one row of `part` per k; returns the
+// number of partials per row (the blocks of the launch, at most `grid`). One block row when the
+// nk+1 dots fit KC accumulators, else block rows of KC.
 template <typename T>
-static void mdot_launch(int grid, int64_t n, const T *V, int64_t ldv, const T *w, int nk, double *part,
-                        hipStream_t st) {
+static int mdot_launch(int grid, int64_t n, const T *V, int64_t ldv, const T *w, int nk, double *part,
+                       hipStream_t st) {
   const int nd = nk + 1;
   if (nd > KC) {
-    k_mdot<T, KC><<<dim3(grid, (nd + KC - 1) / KC), NTHR, 0, st>>>(n, V, ldv, w, nk, part);
-    return;
+    const int g = mdot_grid<T, KC>(grid);
+    k_mdot<T, KC><<<dim3(g, (nd + KC - 1) / KC), NTHR, 0, st>>>(n, V, ldv, w, nk, part);
+    return g;
   }
   switch (nd) {
-#define OCMP_MDOT(K) \
-  case K:            \
-    k_mdot<T, K><<<grid, NTHR, 0, st>>>(n, V, ldv, w, nk, part); \
-    break;
+#define OCMP_MDOT(K)                                                   \
+  case K: {                                                            \
+    const int g = mdot_grid<T, K>(grid);                               \
+    k_mdot<T, K><<<g, NTHR, 0, st>>>(n, V, ldv, w, nk, part);          \
+    return g;                                                          \
+  }
     OCMP_MDOT(1) OCMP_MDOT(2) OCMP_MDOT(3) OCMP_MDOT(4) OCMP_MDOT(5) OCMP_MDOT(6) OCMP_MDOT(7) OCMP_MDOT(8)
     OCMP_MDOT(9) OCMP_MDOT(10) OCMP_MDOT(11) OCMP_MDOT(12) OCMP_MDOT(13) OCMP_MDOT(14) OCMP_MDOT(15)
     OCMP_MDOT(16)
 #undef OCMP_MDOT
     default:
-      break;
+      return grid;
   }
 }
 
@@ -2237,11 +2254,12 @@ static int gmres_cycle(Matrix *A, Precond *P, const double *r0, double beta, dou
           spmv_dispatch<T, T>(A, vals, vj, wv, nullptr, mask, cs, vs + jj, 3, nullptr, spmv_grid<T, T>(A), st);
         }
       }
+      int dgrid;
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
-        mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
+        dgrid = mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
       }
-      reduce_to(w.part.p, grid, jj + 1, dot, st);
+      reduce_to(w.part.p, dgrid, jj + 1, dot, st);
       {
         KTimer tm(ctx, c_upd, (double)(jj + 3) * n * sizeof(T));
         k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p);
@@ -3439,11 +3457,12 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
         KTimer tm(ctx, c_op, csr_bytes(K, sizeof(T), sizeof(T)));
         spmv_dispatch<T, T>(K, vals, zj, wv, nullptr, mask, nullptr, nullptr, 3, nullptr, spmv_grid<T, T>(K), st);
       }
+      int dgrid;
       {
         KTimer tm(ctx, c_dot, (double)(jj + 2) * n * sizeof(T));
-        mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
+        dgrid = mdot_launch<T>(grid, n, V, ldv, wv, jj + 1, w.part.p, st);
       }
-      reduce_to(w.part.p, grid, jj + 1, dot, st);
+      reduce_to(w.part.p, dgrid, jj + 1, dot, st);
       {
         KTimer tm(ctx, c_upd, (double)(jj + 3) * n * sizeof(T));
         k_gs_update<T><<<grid, NTHR, 0, st>>>(n, V, ldv, wv, jj + 1, dot, vs, w.part.p);
