@@ -3035,8 +3035,12 @@ __global__ void __launch_bounds__(NTHR) k_scg_init(int64_t n, const T *__restric
                                                    double sign, const T *__restrict__ sub,
                                                    const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
                                                    T *__restrict__ u, double *__restrict__ gpart,
-                                                   int *__restrict__ flag) {
+                                                   int *__restrict__ flag, const int *__restrict__ outer) {
   __shared__ double sh[NTHR / 64];
+  if (outer && *outer) {  // the outer cycle converged: this solve starts converged, so its launches are empty
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 1;
+    return;
+  }
   const int lb = xcd_block();
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = lb * chunk, i1 = min(n, i0 + chunk);
@@ -3272,6 +3276,7 @@ struct PcgArgs {
   int *flag;
   double tol;
   int c_op, c_it;
+  const int *outer = nullptr;  // a converged outer FGMRES cycle (every launch of this solve returns at entry)
 };
 
 // at most K iterations: the start (x = 0, r_0, u_0, gamma_0 partials; then w_0 = M u_0 with the
@@ -3298,7 +3303,7 @@ static void inner_pcg(const PcgArgs<T, VT> &q, const T *v, const double *sv, dou
   int *flag = q.flag;
   const double tol2 = q.tol * q.tol;
   const double it_bytes = csr_bytes(S, sizeof(VT), sizeof(T)) + 10.0 * n * sizeof(T);
-  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v, sv, sign, sub, dinv, x, r[0], u, g0, flag);
+  k_scg_init<T><<<vg, NTHR, 0, st>>>(n, v, sv, sign, sub, dinv, x, r[0], u, g0, flag, q.outer);
   {
     KTimer tm(ctx, q.c_op, csr_bytes(S, sizeof(VT), sizeof(T)));
     spmv_dispatch<VT, T>(S, val, u, w[0], nullptr, nullptr, nullptr, nullptr, 2, dq[0], sg, st, (T)0, nullptr, (T)0,
@@ -3327,9 +3332,11 @@ static size_t pcg_part_size(Matrix *M, const RowBlocks &rb) {
 
 // z[s0:s1] = -S^{-1} (s v)[s0:s1] by at most K PCG iterations from zero
 template <typename T, typename VT>
-static void schur_inner_t(Precond *P, const VT *val, const T *v, const double *sv, T *z, int K) {
+static void schur_inner_t(Precond *P, const VT *val, const T *v, const double *sv, T *z, int K,
+                          const int *outer) {
   Ctx *ctx = P->S->ctx;
   PcgArgs<T, VT> q;
+  q.outer = outer;
   q.M = P->S;
   q.rb = &P->srb;
   q.val = val;
@@ -3350,28 +3357,30 @@ static void schur_inner_t(Precond *P, const VT *val, const T *v, const double *s
 }
 
 template <typename T>
-static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K) {
+static void schur_inner(Precond *P, const T *v, const double *sv, T *z, int K, const int *outer = nullptr) {
   if constexpr (sizeof(T) == 8) {
-    schur_inner_t<double, double>(P, P->S->d_val.p, v, sv, z, K);
+    schur_inner_t<double, double>(P, P->S->d_val.p, v, sv, z, K, outer);
   } else {
     if (P->bf16)
-      schur_inner_t<float, bf16s>(P, P->S->d_valbf.p, v, sv, z, K);
+      schur_inner_t<float, bf16s>(P, P->S->d_valbf.p, v, sv, z, K, outer);
     else
-      schur_inner_t<float, float>(P, P->S->d_val32.p, v, sv, z, K);
+      schur_inner_t<float, float>(P, P->S->d_val32.p, v, sv, z, K, outer);
   }
 }
 
 // z_p = -(s v_p) pdinv on the Schur rows
 template <typename T>
 __global__ void k_btri_p(int64_t ns, const T *__restrict__ pdinv, const T *__restrict__ v,
-                         const double *__restrict__ sv, T *__restrict__ z) {
+                         const double *__restrict__ sv, T *__restrict__ z, const int *__restrict__ skip) {
+  if (skip && *skip) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < ns) z[i] = (T)(-(double)pdinv[i] * (double)v[i] * sv[0]);
 }
 
 // the block-triangular step: z_p = -(s v_p) / d_p, then z_u = A_uu^{-1} (s v_u - A_up z_p)
 template <typename T, typename VT>
-static void btri_apply_t(Precond *P, const VT *uuval, const T *v, const double *sv, T *z, int K) {
+static void btri_apply_t(Precond *P, const VT *uuval, const T *v, const double *sv, T *z, int K,
+                         const int *outer) {
   Ctx *ctx = P->A->ctx;
   hipStream_t st = ctx->stream;
   const int64_t nu = P->s0, ns = P->A->n - P->s0;
@@ -3396,26 +3405,27 @@ static void btri_apply_t(Precond *P, const VT *uuval, const T *v, const double *
   q.tol = P->vel_tol;
   q.c_op = ctx->cls(sizeof(T) == 8 ? "spmv_vel_f64" : (sizeof(VT) == 2 ? "spmv_vel_bf16" : "spmv_vel_f32"));
   q.c_it = ctx->cls(sizeof(T) == 8 ? "vel_pcg_fused_f64" : (sizeof(VT) == 2 ? "vel_pcg_fused_bf16" : "vel_pcg_fused_f32"));
+  q.outer = outer;
   T *y = q.buf + 8 * nu;
-  k_btri_p<T><<<nblk(ns), NTHR, 0, st>>>(ns, pdinv, v + nu, sv, z + nu);
+  k_btri_p<T><<<nblk(ns), NTHR, 0, st>>>(ns, pdinv, v + nu, sv, z + nu, outer);
   {
     KTimer tm(ctx, ctx->cls(sizeof(T) == 8 ? "spmv_coupling_f64" : "spmv_coupling_f32"),
               csr_bytes(P->Aup, sizeof(T), sizeof(T)));
     spmv_dispatch<T, T>(P->Aup, upval, z + nu, y, nullptr, nullptr, nullptr, nullptr, 0, nullptr,
-                        spmv_grid<T, T>(P->Aup), st);
+                        spmv_grid<T, T>(P->Aup), st, (T)0, nullptr, (T)0, outer);
   }
   inner_pcg<T, VT>(q, v, sv, 1.0, y, z, K);
 }
 
 template <typename T>
-static void btri_apply(Precond *P, const T *v, const double *sv, T *z, int K) {
+static void btri_apply(Precond *P, const T *v, const double *sv, T *z, int K, const int *outer = nullptr) {
   if constexpr (sizeof(T) == 8) {
-    btri_apply_t<double, double>(P, P->Auu->d_val.p, v, sv, z, K);
+    btri_apply_t<double, double>(P, P->Auu->d_val.p, v, sv, z, K, outer);
   } else {
     if (P->bf16)
-      btri_apply_t<float, bf16s>(P, P->Auu->d_valbf.p, v, sv, z, K);
+      btri_apply_t<float, bf16s>(P, P->Auu->d_valbf.p, v, sv, z, K, outer);
     else
-      btri_apply_t<float, float>(P, P->Auu->d_val32.p, v, sv, z, K);
+      btri_apply_t<float, float>(P, P->Auu->d_val32.p, v, sv, z, K, outer);
   }
 }
 
@@ -3493,7 +3503,7 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
       T *zj = Z + (int64_t)jj * ldv;
       T *wv = V + (int64_t)(jj + 1) * ldv;
       if (btri) {
-        btri_apply<T>(P, vj, vs + jj, zj, cap);
+        btri_apply<T>(P, vj, vs + jj, zj, cap, skip);
       } else if (P->schur_amg > 0 && !learned && schur_amg_apply<T>(P, vj, vs + jj, zj, dinv, skip)) {
         P->amg_cycles += P->schur_amg;  // the Jacobi rows rode on the cycle's first launch
       } else {
@@ -3505,7 +3515,7 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
           schur_amg_apply<T>(P, vj, vs + jj, zj, nullptr, skip);
           P->amg_cycles += P->schur_amg;
         } else if (P->schur_iters > 0) {
-          schur_inner<T>(P, vj, vs + jj, zj, cap);
+          schur_inner<T>(P, vj, vs + jj, zj, cap, skip);
         }
       }
       if (P->schur_tri && !btri) {
@@ -3558,17 +3568,21 @@ static int fgmres_cycle(Matrix *A, Matrix *K, Precond *P, bool learned, const do
     Batch b = inflight.front();
     inflight.pop_front();
     OCMP_HIP(hipEventSynchronize(work_event(w, b.ev)));
+    int conv = -1;
+    for (int s = 0; s < b.nb; ++s)
+      if (!(hres[b.j0 + s] > stop_abs)) {
+        conv = s;
+        break;
+      }
     if (adapt) {
+      // with the flag on, the batch's columns past the converged one ran no inner iterations
+      const int nb_run = (skip && conv >= 0) ? conv + 1 : b.nb;
       const double cnt = hres[m + 2 + b.ev];
-      const double per = (cnt - in_seen) / b.nb;
+      const double per = (cnt - in_seen) / nb_run;
       in_seen = cnt;
       in_cap = per >= b.cap - 0.5 ? std::min(in_iters, 2 * b.cap) : std::min(in_iters, (int)(1.3 * per) + 10);
     }
-    for (int s = 0; s < b.nb; ++s)
-      if (!(hres[b.j0 + s] > stop_abs)) {
-        k_used = b.j0 + s + 1;
-        break;
-      }
+    if (conv >= 0) k_used = b.j0 + conv + 1;
   }
   // x += Z y  (the z_j already carry the scale of v_j)
   k_gmres_solve<<<1, 256, 0, st>>>(k_used, m, R, g, nullptr, coef);
