@@ -3981,10 +3981,29 @@ static void solve_minres(Matrix *A, Precond *P, const double *b, double *x, cons
 }
 
 // out[0] = ||b - A x||_2, out[1] = ||b||_2, both over the free dofs in FP64
+// the matrix a residual b - A x is formed with: the Krylov operator when it exists for A's current
+// block pairs (its FP64 values, and the FP32 copy if it has one, refreshed in place: no allocation),
+// else A. The operator leaves out only exact zeros, so the residual is A's up to summation order.
+static Matrix *residual_operator(Matrix *A) {
+  Matrix *K = A->op;
+  if (A->dg || !K || A->op_pairs != A->active_pairs || K->nnz == 0 || K->rb64.nnzb == 0) return A;
+  const int nb = (int)A->space->blocks.size();
+  const uint64_t all = nb * nb == 64 ? ~0ull : (1ull << (nb * nb)) - 1;
+  if (A->active_pairs == 0 || (A->active_pairs & all) == all) return A;
+  if (A->op_version != A->version) {
+    const bool f32 = K->d_val32.p && K->d_val32.n == (size_t)std::max<int64_t>(K->nnz, 1);
+    k_gather_vals<<<nblk(K->nnz), NTHR, 0, A->ctx->stream>>>(K->nnz, K->d_src.p, A->d_val.p, K->d_val.p,
+                                                              f32 ? K->d_val32.p : nullptr);
+    A->op_version = A->version;
+    if (f32) A->op_version32 = A->version;
+  }
+  return K;
+}
+
 void residual_norms(Matrix *A, const double *b, const double *x, double *out) {
   Work &w = work(A);
   w.r64.alloc(A->n);
-  out[0] = residual64(A, nullptr, b, x, w.r64.p, w);
+  out[0] = residual64(residual_operator(A), nullptr, b, x, w.r64.p, w);
   out[1] = norm_masked(A, b, w);
 }
 
@@ -4033,7 +4052,8 @@ double *noalloc_scratch_init(int device) {
 // out_dev[0] = ||b - A x||_2, out_dev[1] = ||b||_2 over the free dofs (FP64), written on the
 // context stream without allocating or synchronising. The same kernels as residual_norms,
 // except that the residual vector is not stored (mode 1 of the SpMV with y = nullptr).
-void residual_norms_noalloc(Matrix *A, const double *b, const double *x, double *out_dev) {
+void residual_norms_noalloc(Matrix *A0, const double *b, const double *x, double *out_dev) {
+  Matrix *A = residual_operator(A0);
   Ctx *ctx = A->ctx;
   double *sp = ctx->noalloc_scratch;
   OCMP_REQUIRE(sp != nullptr, "the context has no allocation-free scratch");

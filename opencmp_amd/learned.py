@@ -253,6 +253,32 @@ def choose_best(a, b: torch.Tensor, plain: torch.Tensor, candidates) -> Tuple[st
     return 'plain', rp, rb
 
 
+def choose_best_async(a, b: torch.Tensor, plain: torch.Tensor, candidates):
+    """choose_best without a synchronisation: the FP64 residuals, the choice and the copy of the
+    winner into `plain` are enqueued on the device, and the returned function reads the outcome
+    (choice, plain residual, residual of the best candidate) once that work has run. The rule is
+    choose_best's: the first candidate with the lowest finite residual, kept only if that residual
+    is below the plain guess's; a NaN plain residual keeps the plain guess."""
+    xs = [plain] + [v for _, v in candidates]
+    out = torch.empty(2 * len(xs), dtype=torch.float64, device=b.device)
+    for k, x in enumerate(xs):
+        L.check(a.ctx.lib.ocmp_residual_norms_noalloc(a.h, C.c_void_p(_ptr(b)), C.c_void_p(_ptr(x)),
+                                                      C.c_void_p(out.data_ptr() + 16 * k)), 'residual_norms_noalloc')
+    r = out[0::2]
+    key = torch.where(torch.isfinite(r), r, torch.full_like(r, math.inf))
+    key[0] = torch.where(torch.isnan(r[0]), torch.full_like(r[0], -math.inf), key[0])
+    idx = torch.argmin(key)  # the first minimum: plain on a tie, the earlier candidate among equals
+    for k, (_, v) in enumerate(candidates, 1):
+        torch.where(idx == k, v, plain, out=plain)
+    names = ['plain'] + [name for name, _ in candidates]
+
+    def outcome() -> Tuple[str, float, float]:
+        res = out.cpu().tolist()
+        rb = min((res[2 * k] for k in range(1, len(xs)) if math.isfinite(res[2 * k])), default=math.inf)
+        return names[int(idx.item())], res[0], rb
+    return outcome
+
+
 class GuessPolicy:
     """The last four accepted states of a transient run (device ring), optional
     snapshot recording for training, and the initial-guess candidates of the next

@@ -104,6 +104,7 @@ class ImplicitEulerSolver:
         self._assemble_a()
         self.step_results = []
         self.guess_log = []
+        self._fixed = None  # device mask of the space's fixed (non-free) dofs, for the guess candidates
         if self.guess is not None:
             self.guess.push(self.gfu_prev.vec)
 
@@ -128,33 +129,48 @@ class ImplicitEulerSolver:
             self._assemble_a()
         self.model.apply_dirichlet_bcs_to(self.gfu)
         assembled = False
+        pending = None
         if self.guess is not None and self.guess.ready():
             if self.model.nonlinear:
                 # the step's first Picard system, linearised about the previous solution (W holds it)
                 self._assemble_a()
                 assembled = True
-            if self._learned_guess() != 'plain' and self.model.nonlinear and self.linearize_at_guess:
-                self.model.W.vec.copy_(self.gfu.vec)
-                assembled = False
+            if self.model.nonlinear and self.linearize_at_guess:
+                # the choice decides the linearisation point: read it now
+                if self._learned_guess() != 'plain':
+                    self.model.W.vec.copy_(self.gfu.vec)
+                    assembled = False
+            else:
+                # the choice only sets the Krylov start vector: made on the device, read after the solve
+                pending = self._learned_guess(wait=False)
         reassemble = self._assemble_a if self.model.nonlinear else None
         res = self.model.solve_single_step(self.a, self.L, self.pre, self.gfu, reassemble=reassemble,
                                            assembled=assembled)
+        if pending is not None:
+            self.guess_log.append(pending())
         self.step_results.append(res)
         self.gfu_prev.vec.copy_(self.gfu.vec)
         if self.guess is not None:
             self.guess.push(self.gfu.vec)
         return res
 
-    def _learned_guess(self) -> str:
+    def _learned_guess(self, wait: bool = True):
         """Replace the plain initial guess (previous solution + Dirichlet lift) by the
         policy's candidate with the lowest FP64 residual against the current operator,
-        if that beats the plain one; returns the choice."""
-        from .learned import choose_best
+        if that beats the plain one; returns the choice (wait=True), or with wait=False a
+        function that returns (choice, plain residual, best candidate's residual) once the
+        device has made the choice (no synchronisation before the solve is enqueued)."""
+        from .learned import choose_best, choose_best_async
+        if self._fixed is None:
+            self._fixed = torch.from_numpy(~self.model.fes.FreeDofs()).to(self.gfu.vec.device)
         cands = []
         for name, v in self.guess.candidates():
-            g = C.GridFunction(self.model.fes, v)
-            self.model.apply_dirichlet_bcs_to(g)
-            cands.append((name, g.vec))
+            # the plain guess already holds this step's Dirichlet lift on the fixed dofs: the candidates
+            # take those values (what apply_dirichlet_bcs_to would set) and keep their own on the free ones
+            torch.where(self._fixed, self.gfu.vec, v, out=v)
+            cands.append((name, v))
+        if not wait:
+            return choose_best_async(self.a, self.L.vec, self.gfu.vec, cands)
         choice, rp, rb = choose_best(self.a, self.L.vec, self.gfu.vec, cands)
         self.guess_log.append((choice, rp, rb))
         return choice

@@ -41,6 +41,37 @@ def test_channel_ins_transient_schur(gpu_ctx):
     say('  max |u_x|', float(u[:nu].abs().max()), 'max |p|', float(u[2 * nu:].abs().max()))
 
 
+def test_guess_choice_on_the_device_matches_the_host_rule(gpu_ctx):
+    """choose_best_async (residuals, choice and copy of the winner all on the device, the outcome read
+    afterwards) against choose_best (residuals read back, choice on the host) on the same system:
+    a NaN candidate never wins, the first of two equal best candidates wins, a worse candidate loses
+    to the plain guess, and a NaN plain residual keeps the plain guess."""
+    from opencmp_amd.learned import choose_best, choose_best_async
+    from opencmp_amd.solvers import ImplicitEulerSolver
+    say('guess choice: device rule against the host rule')
+    m = channel_model(gpu_ctx, 16)
+    s = ImplicitEulerSolver(m, dt=1e-3)
+    for _ in range(2):
+        s.step()
+    a, b = s.a, s.L.vec
+    x0 = s.gfu_prev.vec.clone()  # solves the last Picard system, the one `a` holds
+    g = torch.Generator(device=x0.device).manual_seed(3)
+    worse = x0 + 1e-3 * torch.randn(x0.shape, generator=g, device=x0.device, dtype=x0.dtype)
+    nan = x0.clone()
+    nan[5] = float('nan')
+    cases = [(worse, [('nan', nan), ('good', x0), ('dup', x0.clone())], 'good'),
+             (x0, [('worse', worse)], 'plain'),
+             (nan, [('good', x0)], 'plain')]
+    for plain, cands, expect in cases:
+        p_host, p_dev = plain.clone(), plain.clone()
+        host = choose_best(a, b, p_host, cands)
+        dev = choose_best_async(a, b, p_dev, cands)()
+        say('  host', host, 'device', dev)
+        assert host[0] == dev[0] == expect
+        assert (host[1] == dev[1] or (np.isnan(host[1]) and np.isnan(dev[1]))) and host[2] == dev[2]
+        assert torch.equal(p_host.view(torch.int64), p_dev.view(torch.int64))
+
+
 def test_learned_guess_lowers_picard_error_at_equal_work(gpu_ctx):
     """Train the MLP on 40 plain steps, then run 8 more steps from the same state with the plain
     and with the learned guess. Measured gain: the learned guess wins the residual comparison on
@@ -154,6 +185,9 @@ def test_guess_candidates_keep_the_reference_iteration(gpu_ctx):
     say('  rel diff of the final states', rel)
     assert rel < 1e-5
     assert it_g <= 1.05 * it_p
+    # the kept candidates carry the step's Dirichlet lift on the fixed dofs, which no solve changes
+    fixed = torch.from_numpy(~m.fes.FreeDofs()).to(u_p.device)
+    assert int(fixed.sum()) > 0 and torch.equal(u_g[fixed], u_p[fixed])
 
 
 def test_learned_guess_saves_outer_iterations_with_schur_inner_solve(gpu_ctx):
