@@ -510,6 +510,7 @@ struct Amg {
   // it is composed once per update and applied by one dense product; tail = the last level when no
   // level above it is small enough
   size_t tail = 0;
+  int64_t tld = 0;                          // leading dimension of tail's matrix (rows padded with zeros)
   DBuf<double> tinv, tscr;                  // tail's matrix (row-major); composition scratch
   DBuf<float> tinv32;
   ~Amg();
