@@ -246,12 +246,13 @@ __device__ __forceinline__ void csr_row_block(int lb, const int32_t *__restrict_
         a = rp[r0 + ri] - j0;
         e = rp[r0 + ri + 1] - j0;
       }
-      double s = 0.0;
+      // partial row sums in the vector type (FP32 products summed in FP32, FP64 in FP64)
+      XT s = (XT)0;
       if (ri < nrows)
-        for (int t = a + q; t < e; t += 4) s += (double)prod[t];
+        for (int t = a + q; t < e; t += 4) s += prod[t];
       s += __shfl_xor(s, 1, 4);
       s += __shfl_xor(s, 2, 4);
-      if (ri < nrows && q == 0) ef((int64_t)(r0 + ri), s);
+      if (ri < nrows && q == 0) ef((int64_t)(r0 + ri), (double)s);
     }
   } else {
     // a single row longer than the staging buffer
