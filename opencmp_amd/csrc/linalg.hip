@@ -3229,12 +3229,12 @@ __global__ void __launch_bounds__(NT) k_scg_fused(
           ja = rp[r0 + ri] - j0;
           je = rp[r0 + ri + 1] - j0;
         }
-        double sv = 0.0;
+        T sv = (T)0;  // partial row sums in the vector type, as in csr_row_block
         if (ri < nrows)
-          for (int t = ja + q; t < je; t += 4) sv += (double)prod[t];
+          for (int t = ja + q; t < je; t += 4) sv += prod[t];
         sv += __shfl_xor(sv, 1, 4);
         sv += __shfl_xor(sv, 2, 4);
-        if (ri < nrows && q == 0) finish(r0 + ri, sv);
+        if (ri < nrows && q == 0) finish(r0 + ri, (double)sv);
       }
     } else {
       // a single row longer than the staging buffer
