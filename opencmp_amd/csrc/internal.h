@@ -283,6 +283,7 @@ struct Matrix {
   // block pairs (test block * nblocks + trial block; nblocks <= 8) that the terms since the last
   // non-accumulating assembly touched, and a counter bumped by every assembly
   uint64_t active_pairs = 0, version = 0;
+  uint64_t val32_version = ~0ull;  // the `version` whose values d_val32 holds (the FP32 copy of d_val)
   // the Krylov operator: this matrix without the entries of untouched block pairs (structural
   // zeros, e.g. the two velocity components' cross blocks), values gathered from d_val when stale
   Matrix *op = nullptr;

@@ -1667,6 +1667,19 @@ __global__ void __launch_bounds__(256) k_amg_tail_mv(int64_t n, int64_t ld, cons
   }
 }
 
+// the FP32 copy of an assembled matrix's values, made from its current values
+static void refresh_val32(Matrix *A, hipStream_t st) {
+  A->d_val32.alloc(A->nnz);
+  k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+  A->val32_version = A->version;
+}
+
+// true when FGMRES multiplies by A's Krylov operator (a separate matrix with its own FP32 values,
+// current for A's block pairs) rather than by A: then A's FP32 copy is not read
+static bool krylov_operator_separate(const Matrix *A) {
+  return A->op && A->op_pairs == A->active_pairs && A->op->nnz > 0;
+}
+
 // the hierarchy's values from S's (and S's Jacobi) at every update
 static void amg_update(Precond *Pc, bool mixed) {
   Amg &M = *Pc->amg;
@@ -2019,8 +2032,14 @@ void precond_update(Precond *P) {
       if (P->schur_tri && schur_step) tri_update(P, mixed);
     }
     if (mixed) {
-      A->d_val32.alloc(A->nnz);
-      k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+      // a Schur step runs only inside FGMRES, which reads A's FP32 copy only when it multiplies by A
+      // itself (and then refreshes a stale copy first): skip the conversion while a separate Krylov
+      // operator is current
+      const bool fgmres_only = P->schur_blocks && (P->schur_iters > 0 || P->schur_amg > 0);
+      if (!(fgmres_only && krylov_operator_separate(A)))
+        refresh_val32(A, st);
+      else
+        A->d_val32.alloc(A->nnz);
       ensure_row_blocks32(A);
     }
   } else {
@@ -2070,7 +2089,7 @@ void precond_update_jacobi(Precond *P) {
   k_jacobi_diag<<<nblk(A->n), NTHR, 0, st>>>(A->n, A->d_dslot.p, A->d_val.p, A->space->d_free.p, P->dinv.p,
                                              mixed ? P->dinv32.p : nullptr);
   k_zero_flagged<<<nblk(A->n), NTHR, 0, st>>>(A->n, P->schur_flag.p, P->dinv.p, mixed ? P->dinv32.p : nullptr);
-  if (mixed) k_to_f32<<<nblk(A->nnz), NTHR, 0, st>>>(A->nnz, A->d_val.p, A->d_val32.p);
+  if (mixed && !krylov_operator_separate(A)) refresh_val32(A, st);  // FGMRES only: see precond_update
   if (P->schur_tri) tri_update(P, mixed);
   OCMP_HIP(hipGetLastError());
 }
@@ -2192,6 +2211,7 @@ static void solve_cg(Matrix *A, Precond *P, const double *b, double *x, const oc
     st_.outer = outer;
   } else {
     OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
+    if (A->val32_version != A->version) refresh_val32(A, st);  // an update that left the copy to FGMRES
     w.vf.alloc((size_t)5 * n);
     float *d = w.vf.p, *r = d + n, *z = r + n, *p = z + n, *q = p + n;
     // each inner FP32 CG reduces its residual by this factor (H1 P3, 7.2M dofs: 1e-5 took 560
@@ -3661,6 +3681,7 @@ static void solve_fgmres(Matrix *A, Precond *P, const double *b, double *x, cons
   const bool mixed = o.precision == OCMP_MIXED;
   if (mixed) OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
   Matrix *K = krylov_operator(A, mixed);
+  if (mixed && K == A && A->val32_version != A->version) refresh_val32(A, A->ctx->stream);
   const double bn = norm_masked(A, b, w);
   double rn = residual64(K, P, b, x, w.r64.p, w);
   const double bref = bn > 0.0 ? bn : (rn > 0.0 ? rn : 1.0);
@@ -3729,16 +3750,17 @@ static void solve_gmres(Matrix *A, Precond *P, const double *b, double *x, const
   ensure_part(w, 0);
   w.r64.alloc(n);
   const bool mixed = o.precision == OCMP_MIXED;
-  if (mixed && P && P->kind == OCMP_PRE_JACOBI)
+  if (mixed && P && P->kind == OCMP_PRE_JACOBI) {
     OCMP_REQUIRE(P->dinv32.p && A->d_val32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
+    if (A->val32_version != A->version) refresh_val32(A, A->ctx->stream);  // an update that left it to FGMRES
+  }
   if (mixed && P && P->kind == OCMP_PRE_BLOCK_JACOBI)
     OCMP_REQUIRE(P->ahat32.p, "mixed precision needs a preconditioner built with OCMP_MIXED");
   if (!mixed && P && P->kind == OCMP_PRE_BLOCK_JACOBI)
     OCMP_REQUIRE(P->ahat.p, "FP64 GMRES needs a preconditioner built with OCMP_FP64");
   if (mixed && !P) {
     // plain GMRES in FP32 needs an FP32 copy of A
-    A->d_val32.alloc(A->nnz);
-    k_to_f32<<<nblk(A->nnz), NTHR, 0, A->ctx->stream>>>(A->nnz, A->d_val.p, A->d_val32.p);
+    refresh_val32(A, A->ctx->stream);
     ensure_row_blocks32(A);
   }
   const double bn = norm_masked(A, b, w);
