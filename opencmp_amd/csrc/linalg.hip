@@ -993,7 +993,7 @@ __global__ void __launch_bounds__(NTHR) k_mdot(int64_t n, const T *__restrict__ 
 }
 
 // the grid of k_mdot<T, KE>: at most the blocks that are resident together (the kernel's occupancy,
-// times the CUs the streaming grid assumes), so that no second, partial round of blocks runs
+// times the CUs the streaming grid assumes), so that no second, partial wave of blocks runs
 template <typename T, int KE>
 static int mdot_grid(int grid) {
   static const int per_cu = [] {
